@@ -1,0 +1,195 @@
+"""ctypes mirror of include/cortex_rt.h and the loader of the in-tree libcortex_rt.so.
+
+The library is the product: there is no Python or CPU fallback. If the shared object is missing,
+importing the renderer raises immediately.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcortex_rt.so")
+
+CRT_ABI_VERSION = 1
+
+CRT_OK = 0
+CRT_ERR_NO_DEVICE = 1
+CRT_ERR_OUT_OF_MEMORY = 2
+CRT_ERR_OVER_BUDGET = 3
+CRT_ERR_INVALID_ARGUMENT = 4
+CRT_ERR_INVALID_SCENE = 5
+CRT_ERR_IO = 6
+CRT_ERR_CANCELLED = 7
+CRT_ERR_NOT_READY = 8
+CRT_ERR_HIP = 9
+CRT_ERR_INTERNAL = 10
+
+CRT_MAT_LAMBERTIAN = 1
+CRT_MAT_METAL = 2
+CRT_MAT_DIELECTRIC = 3
+CRT_MAT_DIFFUSE_LIGHT = 8
+CRT_MATF_ONE_SIDED = 1
+
+CRT_TEX_SOLID = 0
+CRT_TEX_CHECKER = 1
+CRT_TEX_IMAGE = 2
+
+CRT_RNG_PCG = 0
+CRT_RNG_MINSTD = 1
+
+CRT_RF_TIME_STAGES = 1
+
+f3 = C.c_float * 3
+
+
+class crt_camera(C.Structure):
+    _fields_ = [("lookfrom", f3), ("lookat", f3), ("vup", f3), ("vfov_deg", C.c_float), ("aspect_ratio", C.c_float),
+                ("aperture", C.c_float), ("focus_dist", C.c_float), ("time0", C.c_float), ("time1", C.c_float)]
+
+
+class crt_material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("texture", C.c_int32), ("albedo", f3), ("fuzz", C.c_float), ("ior", C.c_float),
+                ("intensity", C.c_float), ("flags", C.c_uint32)]
+
+
+class crt_texture(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("color", f3), ("scale", C.c_float), ("even", C.c_int32), ("odd", C.c_int32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32),
+                ("pixels", C.POINTER(C.c_uint8))]
+
+
+class crt_uvmap(C.Structure):
+    _fields_ = [("scale_u", C.c_float), ("scale_v", C.c_float), ("offset_u", C.c_float), ("offset_v", C.c_float),
+                ("repeat_u", C.c_float), ("repeat_v", C.c_float)]
+
+
+class crt_sphere(C.Structure):
+    _fields_ = [("center", f3), ("radius", C.c_float), ("material", C.c_uint32), ("mapping", crt_uvmap)]
+
+
+class crt_quad(C.Structure):
+    _fields_ = [("corner", f3), ("u", f3), ("v", f3), ("material", C.c_uint32)]
+
+
+class crt_area_light(C.Structure):
+    _fields_ = [("center", f3), ("u", f3), ("v", f3), ("rgb", f3), ("intensity", C.c_float), ("invisible", C.c_uint32)]
+
+
+class crt_mesh(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("num_vertices", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("num_triangles", C.c_uint32), ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
+                ("material", C.c_uint32), ("tri_materials", C.POINTER(C.c_uint32))]
+
+
+class crt_instance(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("object_to_world", C.c_float * 12), ("material_override", C.c_int32)]
+
+
+class crt_scene_desc(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("camera", crt_camera), ("background", f3),
+                ("materials", C.POINTER(crt_material)), ("num_materials", C.c_uint32),
+                ("textures", C.POINTER(crt_texture)), ("num_textures", C.c_uint32),
+                ("lights", C.POINTER(crt_area_light)), ("num_lights", C.c_uint32),
+                ("spheres", C.POINTER(crt_sphere)), ("num_spheres", C.c_uint32),
+                ("quads", C.POINTER(crt_quad)), ("num_quads", C.c_uint32),
+                ("meshes", C.POINTER(crt_mesh)), ("num_meshes", C.c_uint32),
+                ("instances", C.POINTER(crt_instance)), ("num_instances", C.c_uint32)]
+
+
+class crt_render_desc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("seed", C.c_uint64), ("rng_mode", C.c_uint32), ("sample_begin", C.c_uint32),
+                ("sample_count", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class crt_image(C.Structure):
+    _fields_ = [("rgb8", C.POINTER(C.c_uint8)), ("accum", C.POINTER(C.c_float))]
+
+
+class crt_stats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("retraces", C.c_uint64), ("light_pdf_tests", C.c_uint64),
+                ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64), ("gpu_ms", C.c_double),
+                ("extend_ms", C.c_double), ("host_ms", C.c_double)]
+
+
+class crt_config(C.Structure):
+    _fields_ = [("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("max_paths", C.c_uint32),
+                ("block_size", C.c_uint32), ("count_work", C.c_uint32), ("leaf_size", C.c_uint32)]
+
+
+class crt_scene_info(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint64), ("num_leaves", C.c_uint64), ("num_prims", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("max_depth", C.c_uint32), ("num_instances", C.c_uint32),
+                ("build_ms", C.c_double), ("upload_ms", C.c_double), ("sah_cost", C.c_double)]
+
+
+class crt_batch_item(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("rd", crt_render_desc), ("out", crt_image)]
+
+
+class crt_request_timing(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("admit_s", C.c_double), ("first_pass_s", C.c_double), ("done_s", C.c_double),
+                ("rays", C.c_uint64), ("passes", C.c_uint32), ("status", C.c_int32)]
+
+
+STRUCTS_IN_ABI_ORDER = [crt_camera, crt_material, crt_texture, crt_uvmap, crt_sphere, crt_quad, crt_area_light,
+                        crt_mesh, crt_instance, crt_scene_desc, crt_render_desc, crt_image, crt_stats, crt_config,
+                        crt_scene_info, crt_batch_item, crt_request_timing]
+
+# Every symbol declared in include/cortex_rt.h.
+EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "crt_scene_upload", "crt_scene_release",
+            "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
+            "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_struct_sizes"]
+
+_P = C.POINTER
+_SIGS = {
+    "crt_create": ([_P(crt_config), _P(C.c_void_p)], C.c_int),
+    "crt_destroy": ([C.c_void_p], C.c_int),
+    "crt_last_error": ([], C.c_char_p),
+    "crt_device_count": ([_P(C.c_int32)], C.c_int),
+    "crt_scene_upload": ([C.c_void_p, _P(crt_scene_desc), _P(C.c_uint32)], C.c_int),
+    "crt_scene_release": ([C.c_void_p, C.c_uint32], C.c_int),
+    "crt_scene_get_info": ([C.c_void_p, C.c_uint32, _P(crt_scene_info)], C.c_int),
+    "crt_scene_build_check": ([_P(crt_scene_desc), C.c_uint32, _P(crt_scene_info)], C.c_int),
+    "crt_render": ([C.c_void_p, C.c_uint32, _P(crt_render_desc), _P(crt_image), _P(crt_stats)], C.c_int),
+    "crt_trace_rays": ([C.c_void_p, C.c_uint32, _P(C.c_float), C.c_uint64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
+                        _P(C.c_float), _P(C.c_float)], C.c_int),
+    "crt_trace_primary": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, _P(C.c_int32), _P(C.c_float)], C.c_int),
+    "crt_render_batch": ([C.c_void_p, _P(crt_batch_item), C.c_uint32, _P(crt_stats)], C.c_int),
+    "crt_submit": ([C.c_void_p, C.c_uint32, _P(crt_render_desc), _P(C.c_uint64)], C.c_int),
+    "crt_poll": ([C.c_void_p, C.c_uint64, _P(C.c_int32)], C.c_int),
+    "crt_wait": ([C.c_void_p, C.c_uint64, _P(crt_image), _P(crt_stats)], C.c_int),
+    "crt_cancel": ([C.c_void_p, C.c_uint64], C.c_int),
+    "crt_request_get_timing": ([C.c_void_p, C.c_uint64, _P(crt_request_timing)], C.c_int),
+    "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
+}
+
+_lib = None
+
+
+class CrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("crt error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Loads libcortex_rt.so (built by __graft_entry__.build()); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libcortex_rt.so is not built at %s; run __graft_entry__.build()" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != CRT_OK:
+        msg = load().crt_last_error()
+        raise CrtError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,660 @@
+// api.cpp - the C-ABI (include/cortex_rt.h): context, scene residency, the pass scheduler shared
+// by the plain, batched and serving paths, and first-hit queries.
+//
+// Replaces launchGPU/renderGPU (core/src/gpu.cu:1982-2149): instead of one megakernel launch per
+// render with the scene rebuilt on the device each time, scenes stay resident and every render is
+// a sequence of wavefront passes whose path slots may be shared by several requests.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "cortex_rt.h"
+#include "crt_kernels.h"
+#include "crt_layout.h"
+#include "crt_runtime.h"
+#include "scene_build.h"
+
+namespace crt {
+
+static thread_local std::string g_last_error;
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return set_error(e_ == hipErrorOutOfMemory ? CRT_ERR_OUT_OF_MEMORY : CRT_ERR_HIP,              \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));                           \
+    } while (0)
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------------ device allocations
+int DevBuf::alloc(size_t n, Context* ctx) {
+    release();
+    if (n == 0) return CRT_OK;
+    if (ctx && ctx->fail_alloc_after >= 0) {
+        if (ctx->fail_alloc_after == 0) return set_error(CRT_ERR_OUT_OF_MEMORY, "injected allocation failure");
+        ctx->fail_alloc_after--;
+    }
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) { p = nullptr; return set_error(CRT_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+    bytes = n;
+    return CRT_OK;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr; bytes = 0;
+}
+
+template <class T> static int upload_vec(DevBuf& b, const std::vector<T>& v, Context* ctx, hipStream_t s) {
+    if (v.empty()) { b.release(); return CRT_OK; }
+    int rc = b.alloc(v.size() * sizeof(T), ctx);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return CRT_OK;
+}
+
+// ------------------------------------------------------------------ camera (host, FP32)
+// perspective_camera::initialize (core/src/cameras/perspective_camera.cuh:41-104) with the same
+// float expression order; degrees_to_radians = deg * M_PI / 180 (math_utils.cuh:55-58).
+void camera_setup(const crt_camera& c, crt_dev_request& r) {
+    struct V { float x, y, z; };
+    auto sub = [](V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; };
+    auto add = [](V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; };
+    auto mul = [](float s, V a) { return V{s * a.x, s * a.y, s * a.z}; };
+    auto crs = [](V a, V b) { return V{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; };
+    auto unit = [](V v) { float l = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); return V{v.x / l, v.y / l, v.z / l}; };
+    V from{c.lookfrom[0], c.lookfrom[1], c.lookfrom[2]}, at{c.lookat[0], c.lookat[1], c.lookat[2]}, up{c.vup[0], c.vup[1], c.vup[2]};
+    float theta = c.vfov_deg * 3.14159265358979323846f / 180.0f;
+    float focus = c.focus_dist, ratio = c.aspect_ratio;
+    V w = unit(sub(from, at));
+    V u = unit(crs(up, w));
+    V v = crs(w, u);
+    float hh = std::tan(theta / 2.0f);
+    float hw = ratio * hh;
+    V llc = sub(sub(sub(from, mul(hw * focus, u)), mul(hh * focus, v)), mul(focus, w));
+    V hor = mul(2.0f * hw * focus, u);
+    V ver = mul(2.0f * hh * focus, v);
+    auto put = [](float* d, V a) { d[0] = a.x; d[1] = a.y; d[2] = a.z; };
+    put(r.origin, from); put(r.llc, llc); put(r.horizontal, hor); put(r.vertical, ver); put(r.cu, u); put(r.cv, v);
+    r.lens_radius = c.aperture / 2.0f;
+    r.time0 = c.time0; r.time1 = c.time1;
+    (void)add;
+}
+
+// ------------------------------------------------------------------ context
+int Context::init(const crt_config* cfg) {
+    if (cfg) config = *cfg;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(CRT_ERR_NO_DEVICE, "no HIP device found");
+    if (config.device < 0 || config.device >= n) return set_error(CRT_ERR_NO_DEVICE, "device ordinal out of range");
+    device = config.device;
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    budget = config.hbm_budget_bytes ? config.hbm_budget_bytes : (uint64_t)(0.9 * (double)freeb);
+    max_paths = config.max_paths ? config.max_paths : (1u << 22);
+    leaf_size = config.leaf_size ? config.leaf_size : 4u;
+    extend_grid = (uint32_t)num_cus * 8u;
+    shade_grid = (uint32_t)num_cus * 8u;
+    int rc;
+    if ((rc = scene_table.alloc(sizeof(crt_dev_scene) * kMaxScenes, this))) return rc;
+    if ((rc = counters.alloc(sizeof(crt_dev_counters), this))) return rc;
+    if ((rc = counts.alloc(sizeof(uint32_t) * 4, this))) return rc;
+    if ((rc = gstack.alloc((size_t)extend_grid * 256u * crt_k_gstack_entries() * sizeof(uint32_t), this))) return rc;
+    for (int k = 0; k < 2; ++k) {
+        if ((rc = req_table[k].alloc(sizeof(crt_dev_request) * kMaxPassRequests, this))) return rc;
+        HIP_TRY(hipHostMalloc(&req_host[k], sizeof(crt_dev_request) * kMaxPassRequests, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&req_event[k], hipEventDisableTiming));
+    }
+    HIP_TRY(hipHostMalloc((void**)&counters_host, sizeof(crt_dev_counters), hipHostMallocDefault));
+    HIP_TRY(hipEventCreate(&ev_start));
+    HIP_TRY(hipEventCreate(&ev_stop));
+    HIP_TRY(hipEventCreate(&ev_ext[0]));
+    HIP_TRY(hipEventCreate(&ev_ext[1]));
+    if ((rc = ensure_paths(max_paths))) return rc;
+    reserved = used_bytes();
+    return CRT_OK;
+}
+
+uint64_t Context::used_bytes() const {
+    uint64_t b = scene_table.bytes + counters.bytes + counts.bytes + gstack.bytes + req_table[0].bytes + req_table[1].bytes;
+    b += ray_o.bytes + ray_d.bytes + thr.bytes + rad.bytes + ids.bytes + hit.bytes + hit_inst.bytes + queue[0].bytes +
+         queue[1].bytes;
+    return b;
+}
+
+int Context::ensure_paths(uint32_t n) {
+    if (n <= path_capacity) return CRT_OK;
+    int rc;
+    if ((rc = ray_o.alloc((size_t)n * 16, this))) return rc;
+    if ((rc = ray_d.alloc((size_t)n * 16, this))) return rc;
+    if ((rc = thr.alloc((size_t)n * 16, this))) return rc;
+    if ((rc = rad.alloc((size_t)n * 16, this))) return rc;
+    if ((rc = ids.alloc((size_t)n * 8, this))) return rc;
+    if ((rc = hit.alloc((size_t)n * 16, this))) return rc;
+    if ((rc = hit_inst.alloc((size_t)n * 4, this))) return rc;
+    if ((rc = queue[0].alloc((size_t)n * 4, this))) return rc;
+    if ((rc = queue[1].alloc((size_t)n * 4, this))) return rc;
+    path_capacity = n;
+    return CRT_OK;
+}
+
+Context::~Context() {
+    stop_server();
+    if (stream) (void)hipStreamSynchronize(stream);
+    scenes.clear();
+    for (int k = 0; k < 2; ++k) {
+        if (req_host[k]) (void)hipHostFree(req_host[k]);
+        if (req_event[k]) (void)hipEventDestroy(req_event[k]);
+    }
+    if (counters_host) (void)hipHostFree(counters_host);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    for (int k = 0; k < 2; ++k) if (ev_ext[k]) (void)hipEventDestroy(ev_ext[k]);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ------------------------------------------------------------------ scenes
+int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
+    std::lock_guard<std::mutex> g(gpu_mutex);
+    auto sc = std::make_unique<DeviceScene>();
+    std::string err;
+    unsigned hw = std::thread::hardware_concurrency();
+    int rc = build_host_scene(desc, leaf_size, (int)std::min(16u, hw ? hw : 1u), sc->host, err);
+    if (rc) return set_error(rc, err);
+    uint64_t need = sc->host.device_bytes();
+    if (scene_bytes + request_bytes + reserved + need > budget)
+        return set_error(CRT_ERR_OVER_BUDGET, "scene of " + std::to_string(need) + " bytes exceeds the HBM budget");
+    uint32_t slot = kMaxScenes;
+    for (uint32_t k = 0; k < kMaxScenes; ++k) if (!slot_used[k]) { slot = k; break; }
+    if (slot == kMaxScenes) return set_error(CRT_ERR_OVER_BUDGET, "scene table full");
+    auto t0 = std::chrono::steady_clock::now();
+    HostScene& H = sc->host;
+    if ((rc = upload_vec(sc->nodes, H.nodes, this, stream))) return rc;
+    if ((rc = upload_vec(sc->leaves, H.leaves, this, stream))) return rc;
+    if ((rc = upload_vec(sc->tris, H.tris, this, stream))) return rc;
+    if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
+    if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
+    if ((rc = upload_vec(sc->spheres, H.spheres, this, stream))) return rc;
+    if ((rc = upload_vec(sc->sphere_aux, H.sphere_aux, this, stream))) return rc;
+    if ((rc = upload_vec(sc->sphere_map, H.sphere_map, this, stream))) return rc;
+    if ((rc = upload_vec(sc->quads, H.quads, this, stream))) return rc;
+    if ((rc = upload_vec(sc->insts, H.insts, this, stream))) return rc;
+    if ((rc = upload_vec(sc->mats, H.mats, this, stream))) return rc;
+    if ((rc = upload_vec(sc->texs, H.texs, this, stream))) return rc;
+    if ((rc = upload_vec(sc->texels, H.texels, this, stream))) return rc;
+    if ((rc = upload_vec(sc->light_quads, H.light_quads, this, stream))) return rc;
+    crt_dev_scene& ds = sc->dev;
+    std::memset(&ds, 0, sizeof(ds));
+    ds.nodes = sc->nodes.p; ds.leaves = sc->leaves.p; ds.tris = sc->tris.p; ds.tri_normals = sc->tri_normals.p;
+    ds.tri_uvs = sc->tri_uvs.p; ds.spheres = sc->spheres.p; ds.sphere_aux = sc->sphere_aux.p; ds.sphere_map = sc->sphere_map.p;
+    ds.quads = sc->quads.p; ds.insts = sc->insts.p; ds.mats = sc->mats.p; ds.texs = sc->texs.p;
+    ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;
+    ds.num_lights = H.num_lights; ds.root = H.root; ds.num_quads = H.num_quads;
+    for (int k = 0; k < 3; ++k) ds.background[k] = H.background[k];
+    HIP_TRY(hipMemcpyAsync((crt_dev_scene*)scene_table.p + slot, &ds, sizeof(ds), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    sc->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    sc->slot = slot;
+    sc->bytes = need;
+    // leaf order -> global primitive id for first-hit queries, then drop the big host copies
+    sc->tri_gid.resize(H.tris.size() / 3);
+    for (size_t i = 0; i < sc->tri_gid.size(); ++i) std::memcpy(&sc->tri_gid[i], &H.tris[3 * i].w, 4);
+    sc->quad_gid.resize(H.quads.size() / 5);
+    for (size_t i = 0; i < sc->quad_gid.size(); ++i) std::memcpy(&sc->quad_gid[i], &H.quads[5 * i + 3].w, 4);
+    H.nodes.clear(); H.nodes.shrink_to_fit(); H.tris.clear(); H.tris.shrink_to_fit();
+    H.texels.clear(); H.texels.shrink_to_fit();
+    slot_used[slot] = true;
+    scene_bytes += need;
+    crt_scene_id id = next_scene_id++;
+    scenes[id] = std::move(sc);
+    *out = id;
+    return CRT_OK;
+}
+
+int Context::release_scene(crt_scene_id id) {
+    std::lock_guard<std::mutex> g(gpu_mutex);
+    auto it = scenes.find(id);
+    if (it == scenes.end()) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+    if (it->second->refs > 0) return set_error(CRT_ERR_INVALID_ARGUMENT, "scene has requests in flight");
+    (void)hipStreamSynchronize(stream);
+    slot_used[it->second->slot] = false;
+    scene_bytes -= it->second->bytes;
+    scenes.erase(it);
+    return CRT_OK;
+}
+
+DeviceScene* Context::find_scene(crt_scene_id id) {
+    auto it = scenes.find(id);
+    return it == scenes.end() ? nullptr : it->second.get();
+}
+
+// ------------------------------------------------------------------ jobs and passes
+int Job::validate(const crt_render_desc& r) {
+    if (r.width == 0 || r.height == 0) return set_error(CRT_ERR_INVALID_ARGUMENT, "width and height must be positive");
+    if ((uint64_t)r.width * r.height > (1ull << 28)) return set_error(CRT_ERR_INVALID_ARGUMENT, "image too large");
+    if (r.spp == 0) return set_error(CRT_ERR_INVALID_ARGUMENT, "spp must be positive");
+    if (r.max_depth == 0 || r.max_depth > 255) return set_error(CRT_ERR_INVALID_ARGUMENT, "max_depth must be in [1, 255]");
+    if (r.rng_mode != CRT_RNG_PCG && r.rng_mode != CRT_RNG_MINSTD) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown rng_mode");
+    uint32_t sq = (uint32_t)std::sqrt((double)r.spp);
+    if (sq * sq > 65536u) return set_error(CRT_ERR_INVALID_ARGUMENT, "spp too large");
+    uint32_t ns = sq * sq;
+    if (r.sample_begin >= ns && !(r.sample_begin == 0)) return set_error(CRT_ERR_INVALID_ARGUMENT, "sample_begin beyond spp");
+    if (r.rng_mode == CRT_RNG_MINSTD && (r.sample_begin != 0 || (r.sample_count != 0 && r.sample_count != ns)))
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "MINSTD streams run every sample of a pixel in one call");
+    return CRT_OK;
+}
+
+uint64_t Job::bytes_needed(const crt_render_desc& r) {
+    uint64_t npix = (uint64_t)r.width * r.height;
+    return npix * 16 + npix * 3 + (r.rng_mode == CRT_RNG_MINSTD ? npix * 4 : 0);
+}
+
+int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r) {
+    rd = r;
+    scene = sc;
+    npix = r.width * r.height;
+    sq = (uint32_t)std::sqrt((double)r.spp);
+    nsamples = sq * sq;
+    s_begin = std::min(r.sample_begin, nsamples);
+    s_end = r.sample_count ? std::min(nsamples, s_begin + r.sample_count) : nsamples;
+    s_next = s_begin;
+    p_next = 0;
+    std::memset(&req, 0, sizeof(req));
+    camera_setup(sc->host.camera, req);
+    req.width = r.width; req.height = r.height; req.sqrt_spp = sq; req.spp = r.spp; req.max_depth = r.max_depth;
+    req.rng_mode = r.rng_mode; req.scene = sc->slot; req.seed = r.seed;
+    int rc;
+    if ((rc = accum.alloc((size_t)npix * 16, ctx))) return rc;
+    if ((rc = rgb.alloc((size_t)npix * 3, ctx))) return rc;
+    if (r.rng_mode == CRT_RNG_MINSTD && (rc = rng.alloc((size_t)npix * 4, ctx))) return rc;
+    req.accum = (float*)accum.p;
+    req.rng_state = (uint32_t*)rng.p;
+    if (crt_k_init_request(ctx->stream, (float4*)accum.p, (uint32_t*)rng.p, r.width, r.height, r.seed))
+        return set_error(CRT_ERR_HIP, "init kernel launch failed");
+    return CRT_OK;
+}
+
+// Takes up to `cap` path slots of this job's remaining work (sample-major). MINSTD jobs take at
+// most one sample per pass so each pixel's engine advances sample after sample.
+bool Job::take(uint32_t cap, WorkItem& w) {
+    if (s_next >= s_end || cap == 0) return false;
+    w.job = this;
+    if (p_next == 0 && cap >= npix) {
+        uint32_t k = cap / npix;
+        if (rd.rng_mode == CRT_RNG_MINSTD) k = 1;
+        k = std::min(k, s_end - s_next);
+        w.pix_begin = 0; w.pix_count = npix; w.sample_begin = s_next; w.sample_count = k;
+        s_next += k;
+    } else {
+        uint32_t c = std::min(cap, npix - p_next);
+        w.pix_begin = p_next; w.pix_count = c; w.sample_begin = s_next; w.sample_count = 1;
+        p_next += c;
+        if (p_next == npix) { p_next = 0; s_next += 1; }
+    }
+    return true;
+}
+
+// Runs one fused pass over the given work items (all with the same rng mode). Asynchronous on
+// ctx->stream; the caller synchronizes before reading results.
+int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms) {
+    if (items.empty()) return CRT_OK;
+    int buf = pass_parity;
+    pass_parity ^= 1;
+    HIP_TRY(hipEventSynchronize(req_event[buf]));
+    crt_dev_request* host = (crt_dev_request*)req_host[buf];
+    uint32_t base = 0, max_pix = 0, max_depth = 1;
+    int mode = (int)items[0].job->rd.rng_mode;
+    for (size_t k = 0; k < items.size(); ++k) {
+        WorkItem& w = items[k];
+        crt_dev_request r = w.job->req;
+        r.path_base = base;
+        r.pix_begin = w.pix_begin; r.pix_count = w.pix_count;
+        r.sample_begin = w.sample_begin; r.sample_count = w.sample_count;
+        host[k] = r;
+        base += w.pix_count * w.sample_count;
+        max_pix = std::max(max_pix, w.pix_count);
+        max_depth = std::max(max_depth, w.job->rd.max_depth);
+        w.job->rays_in_flight += (uint64_t)w.pix_count * w.sample_count;
+    }
+    uint32_t npaths = base;
+    if (npaths > path_capacity) return set_error(CRT_ERR_INTERNAL, "pass exceeds path capacity");
+    crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
+    HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request) * items.size(), hipMemcpyHostToDevice, stream));
+    const crt_dev_scene* dsc = (const crt_dev_scene*)scene_table.p;
+    int nreq = (int)items.size();
+    if (crt_k_generate(stream, mode, dreq, nreq, npaths, (float4*)ray_o.p, (float4*)ray_d.p, (float4*)thr.p, (float4*)rad.p,
+                       (uint2*)ids.p))
+        return set_error(CRT_ERR_HIP, "generate launch failed");
+    uint32_t* cnt = (uint32_t*)counts.p;
+    const uint32_t* qin = nullptr;
+    const uint32_t* cin = nullptr;
+    int qi = 0;
+    uint32_t egrid = std::min(extend_grid, std::max(1u, (npaths + 255) / 256));
+    uint32_t sgrid = std::min(shade_grid, std::max(1u, (npaths + 255) / 256));
+    for (uint32_t b = 0; b < max_depth; ++b) {
+        uint32_t* qout = (uint32_t*)queue[qi].p;
+        uint32_t* cout = cnt + qi;
+        if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
+        if (crt_k_extend(stream, config.count_work ? 1 : 0, egrid, dsc, dreq, qin, cin, npaths, (const float4*)ray_o.p,
+                         (const float4*)ray_d.p, (const uint2*)ids.p, (float4*)hit.p, (uint32_t*)hit_inst.p,
+                         (uint32_t*)gstack.p, cout, (crt_dev_counters*)counters.p))
+            return set_error(CRT_ERR_HIP, "extend launch failed");
+        if (time_stages) {
+            HIP_TRY(hipEventRecord(ev_ext[1], stream));
+            HIP_TRY(hipEventSynchronize(ev_ext[1]));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev_ext[0], ev_ext[1]));
+            if (extend_ms) *extend_ms += ms;
+            extend_launches++;
+        }
+        if (crt_k_shade(stream, mode, sgrid, dsc, dreq, qin, cin, npaths, qout, cout, (float4*)ray_o.p, (float4*)ray_d.p,
+                        (float4*)thr.p, (float4*)rad.p, (const uint2*)ids.p, (const float4*)hit.p,
+                        (const uint32_t*)hit_inst.p, (crt_dev_counters*)counters.p))
+            return set_error(CRT_ERR_HIP, "shade launch failed");
+        qin = qout; cin = cout;
+        qi ^= 1;
+    }
+    if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)rad.p)) return set_error(CRT_ERR_HIP, "resolve launch failed");
+    HIP_TRY(hipEventRecord(req_event[buf], stream));
+    passes++;
+    return CRT_OK;
+}
+
+int Context::finish_job(Job& j, crt_image* out) {
+    if (crt_k_tonemap(stream, (const float4*)j.accum.p, j.npix, j.rd.spp, (uint8_t*)j.rgb.p))
+        return set_error(CRT_ERR_HIP, "tonemap launch failed");
+    if (out && out->rgb8) HIP_TRY(hipMemcpyAsync(out->rgb8, j.rgb.p, (size_t)j.npix * 3, hipMemcpyDeviceToHost, stream));
+    if (out && out->accum) HIP_TRY(hipMemcpyAsync(out->accum, j.accum.p, (size_t)j.npix * 16, hipMemcpyDeviceToHost, stream));
+    return CRT_OK;
+}
+
+int Context::reset_counters() {
+    if (crt_k_fill_u32(stream, (uint32_t*)counters.p, sizeof(crt_dev_counters) / 4, 0u))
+        return set_error(CRT_ERR_HIP, "fill launch failed");
+    return CRT_OK;
+}
+
+int Context::read_counters(crt_dev_counters& c) {
+    HIP_TRY(hipMemcpyAsync(counters_host, counters.p, sizeof(crt_dev_counters), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    c = *(crt_dev_counters*)counters_host;
+    return CRT_OK;
+}
+
+void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms) {
+    if (!st) return;
+    st->paths = paths;
+    st->rays = c.rays;
+    st->retraces = c.retraces;
+    st->light_pdf_tests = c.light_tests;
+    st->node_visits = c.node_visits;
+    st->prim_tests = c.tri_tests + c.sphere_tests + c.quad_tests;
+    st->gpu_ms = gpu_ms;
+    st->extend_ms = ext_ms;
+    st->host_ms = host_ms;
+}
+
+// Plain and batched rendering: all items are admitted at once, passes are packed greedily.
+int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* stats) {
+    double h0 = now_s();
+    int rc;
+    if ((rc = reset_counters())) return rc;
+    HIP_TRY(hipEventRecord(ev_start, stream));
+    double ext_ms = 0.0;
+    uint64_t paths = 0;
+    for (int mode = 0; mode < 2; ++mode) {
+        for (;;) {
+            std::vector<WorkItem> items;
+            uint32_t cap = max_paths;
+            for (Job* j : jobs) {
+                if ((int)j->rd.rng_mode != mode) continue;
+                while (cap > 0 && items.size() < kMaxPassRequests) {
+                    WorkItem w;
+                    if (!j->take(cap, w)) break;
+                    cap -= w.pix_count * w.sample_count;
+                    paths += (uint64_t)w.pix_count * w.sample_count;
+                    items.push_back(w);
+                    if (mode == CRT_RNG_MINSTD) break;  // one sample per pixel per pass
+                }
+            }
+            if (items.empty()) break;
+            if ((rc = run_pass(items, time_stages, &ext_ms))) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(ev_stop, stream));
+    HIP_TRY(hipEventSynchronize(ev_stop));
+    float gms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&gms, ev_start, ev_stop));
+    crt_dev_counters c;
+    if ((rc = read_counters(c))) return rc;
+    if (c.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
+    fill_stats(stats, c, paths, gms, ext_ms, (now_s() - h0) * 1e3);
+    return CRT_OK;
+}
+
+}  // namespace crt
+
+using namespace crt;
+
+extern "C" {
+
+const char* crt_last_error(void) { return g_last_error.c_str(); }
+
+int crt_device_count(int32_t* count) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+    return CRT_OK;
+}
+
+int crt_create(const crt_config* cfg, crt_context** out) {
+    if (!out) return set_error(CRT_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    auto ctx = std::make_unique<Context>();
+    int rc = ctx->init(cfg);
+    if (rc) return rc;
+    *out = reinterpret_cast<crt_context*>(ctx.release());
+    return CRT_OK;
+}
+
+int crt_destroy(crt_context* c) {
+    delete reinterpret_cast<Context*>(c);
+    return CRT_OK;
+}
+
+int crt_scene_upload(crt_context* c, const crt_scene_desc* d, crt_scene_id* out) {
+    if (!c || !d || !out) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    return reinterpret_cast<Context*>(c)->upload_scene(d, out);
+}
+
+int crt_scene_release(crt_context* c, crt_scene_id s) {
+    if (!c) return set_error(CRT_ERR_INVALID_ARGUMENT, "null context");
+    return reinterpret_cast<Context*>(c)->release_scene(s);
+}
+
+int crt_scene_get_info(crt_context* c, crt_scene_id s, crt_scene_info* info) {
+    if (!c || !info) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    DeviceScene* sc = ctx->find_scene(s);
+    if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+    info->num_nodes = sc->host.stats.nodes;
+    info->num_leaves = sc->host.stats.leaves;
+    info->num_prims = sc->host.stats.prims;
+    info->device_bytes = sc->bytes;
+    info->max_depth = sc->host.stats.depth;
+    info->num_instances = sc->host.num_instances;
+    info->build_ms = sc->host.stats.build_ms;
+    info->upload_ms = sc->upload_ms;
+    info->sah_cost = sc->host.stats.sah_cost;
+    return CRT_OK;
+}
+
+int crt_render(crt_context* c, crt_scene_id s, const crt_render_desc* rd, crt_image* out, crt_stats* stats) {
+    if (!c || !rd) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    crt_batch_item item;
+    item.scene = s; item.rd = *rd;
+    if (out) item.out = *out; else { item.out.rgb8 = nullptr; item.out.accum = nullptr; }
+    return crt_render_batch(c, &item, 1, stats);
+}
+
+int crt_render_batch(crt_context* c, crt_batch_item* items, uint32_t n, crt_stats* stats) {
+    if (!c || (!items && n)) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    uint64_t need = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!ctx->find_scene(items[k].scene)) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+        int rc = Job::validate(items[k].rd);
+        if (rc) return rc;
+        need += Job::bytes_needed(items[k].rd);
+    }
+    if (ctx->scene_bytes + ctx->request_bytes + ctx->reserved + need > ctx->budget)
+        return set_error(CRT_ERR_OVER_BUDGET, "requests exceed the HBM budget");
+    std::vector<std::unique_ptr<Job>> jobs;
+    std::vector<Job*> jp;
+    for (uint32_t k = 0; k < n; ++k) {
+        auto j = std::make_unique<Job>();
+        int rc = j->setup(ctx, ctx->find_scene(items[k].scene), items[k].rd);
+        if (rc) return rc;
+        jp.push_back(j.get());
+        jobs.push_back(std::move(j));
+    }
+    bool time_stages = (n > 0) && (items[0].rd.flags & 1u);
+    int rc = ctx->render_jobs(jp, time_stages, stats);
+    if (rc) return rc;
+    for (uint32_t k = 0; k < n; ++k) {
+        if ((rc = ctx->finish_job(*jobs[k], &items[k].out))) return rc;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return set_error(CRT_ERR_HIP, "stream synchronize failed");
+    return CRT_OK;
+}
+
+int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n, int32_t camera_rays, int32_t* prim,
+                   int32_t* inst, float* t, float* uv) {
+    if (!c || (!rays && n)) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > (1ull << 30)) return set_error(CRT_ERR_INVALID_ARGUMENT, "too many rays");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    DeviceScene* sc = ctx->find_scene(s);
+    if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+    if (n == 0) return CRT_OK;
+    DevBuf dr, dh, di;
+    int rc;
+    if ((rc = dr.alloc(n * 32, ctx))) return rc;
+    if ((rc = dh.alloc(n * 16, ctx))) return rc;
+    if ((rc = di.alloc(n * 4, ctx))) return rc;
+    HIP_TRY(hipMemcpyAsync(dr.p, rays, n * 32, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = ctx->reset_counters())) return rc;
+    uint32_t grid = std::min(ctx->extend_grid, (uint32_t)std::max<uint64_t>(1, (n + 255) / 256));
+    if (crt_k_trace_rays(ctx->stream, grid, (const crt_dev_scene*)ctx->scene_table.p + sc->slot, (const float4*)dr.p,
+                         (uint32_t)n, camera_rays, (float4*)dh.p, (uint32_t*)di.p, (uint32_t*)ctx->gstack.p,
+                         (crt_dev_counters*)ctx->counters.p))
+        return set_error(CRT_ERR_HIP, "trace launch failed");
+    std::vector<float> hh(n * 4);
+    std::vector<uint32_t> hi(n);
+    HIP_TRY(hipMemcpyAsync(hh.data(), dh.p, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hi.data(), di.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    crt_dev_counters cnt;
+    if ((rc = ctx->read_counters(cnt))) return rc;
+    if (cnt.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
+    // map leaf-ordered hits back to global ids: copies of the id words are kept on the host
+    const HostScene& H = sc->host;
+    for (uint64_t k = 0; k < n; ++k) {
+        uint32_t p;
+        std::memcpy(&p, &hh[4 * k + 1], 4);
+        int32_t gid = -1;
+        if (p != CRT_MISS) {
+            uint32_t type = CRT_HIT_TYPE(p), idx = CRT_HIT_INDEX(p);
+            if (type == CRT_PRIM_TRI) gid = (int32_t)sc->tri_gid[idx];
+            else if (type == CRT_PRIM_SPHERE) gid = (int32_t)H.sphere_aux[idx].x;
+            else gid = (int32_t)sc->quad_gid[idx];
+        }
+        if (prim) prim[k] = gid;
+        if (inst) inst[k] = (p == CRT_MISS || hi[k] == CRT_NO_INST) ? -1 : (int32_t)hi[k];
+        if (t) t[k] = p == CRT_MISS ? 0.0f : hh[4 * k];
+        if (uv) { uv[2 * k] = p == CRT_MISS ? 0.0f : hh[4 * k + 2]; uv[2 * k + 1] = p == CRT_MISS ? 0.0f : hh[4 * k + 3]; }
+    }
+    return CRT_OK;
+}
+
+int crt_trace_primary(crt_context* c, crt_scene_id s, uint32_t W, uint32_t H, int32_t* prim, float* t) {
+    if (!c) return set_error(CRT_ERR_INVALID_ARGUMENT, "null context");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    crt_dev_request r;
+    {
+        std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+        DeviceScene* sc = ctx->find_scene(s);
+        if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+        camera_setup(sc->host.camera, r);
+    }
+    std::vector<float> rays((size_t)W * H * 8);
+    for (uint32_t jj = 0; jj < H; ++jj) {
+        uint32_t j = H - 1 - jj;
+        for (uint32_t i = 0; i < W; ++i) {
+            float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
+            float* q = &rays[8 * ((size_t)jj * W + i)];
+            for (int k = 0; k < 3; ++k) {
+                q[k] = r.origin[k];
+                q[4 + k] = r.llc[k] + u * r.horizontal[k] + v * r.vertical[k] - r.origin[k];
+            }
+            q[3] = 0.00001f; q[7] = 3.402823466e+38f;
+        }
+    }
+    std::vector<float> uv((size_t)W * H * 2);
+    return crt_trace_rays(c, s, rays.data(), (uint64_t)W * H, 1, prim, nullptr, t, uv.data());
+}
+
+int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, crt_scene_info* info) {
+    HostScene H;
+    std::string err;
+    unsigned hw = std::thread::hardware_concurrency();
+    int rc = build_host_scene(d, leaf_size ? leaf_size : 4u, (int)std::min(16u, hw ? hw : 1u), H, err);
+    if (rc) return set_error(rc, err);
+    rc = check_host_scene(H, err);
+    if (rc) return set_error(rc, err);
+    if (info) {
+        info->num_nodes = H.stats.nodes; info->num_leaves = H.stats.leaves; info->num_prims = H.stats.prims;
+        info->device_bytes = H.device_bytes(); info->max_depth = H.stats.depth; info->num_instances = H.num_instances;
+        info->build_ms = H.stats.build_ms; info->upload_ms = 0.0; info->sah_cost = H.stats.sah_cost;
+    }
+    return CRT_OK;
+}
+
+int crt_struct_sizes(uint64_t* sizes, uint32_t n) {
+    uint64_t v[] = {sizeof(crt_camera), sizeof(crt_material), sizeof(crt_texture), sizeof(crt_uvmap), sizeof(crt_sphere),
+                    sizeof(crt_quad), sizeof(crt_area_light), sizeof(crt_mesh), sizeof(crt_instance), sizeof(crt_scene_desc),
+                    sizeof(crt_render_desc), sizeof(crt_image), sizeof(crt_stats), sizeof(crt_config), sizeof(crt_scene_info),
+                    sizeof(crt_batch_item), sizeof(crt_request_timing)};
+    uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
+    for (uint32_t k = 0; k < n && k < m; ++k) sizes[k] = v[k];
+    return (int)m;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// crt_layout.h - HBM layout of scenes, requests and wavefront path state, shared by the host
+// builder (C++) and the gfx950 kernels (HIP). Everything here is POD and 16-byte aligned where it
+// is read as float4.
+//
+// Scene (one per crt_scene_id, resident until released):
+//   nodes   : BVH2, 4 x float4 (64 B) per interior node; both children's boxes live in the parent
+//             n0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//             n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//             n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//             n3 = (child0, child1, -, -) as int: >= 0 interior node, < 0 leaf ~index into leaves
+//   leaves  : uint2 (first, type << 28 | count) into the per-type leaf-ordered primitive arrays
+//   tris    : 3 x float4 per triangle in leaf order: (v0, gid) (v1, material) (v2, flags)
+//   spheres : float4 (centre, radius) per sphere in leaf order + uint4 aux (gid, material, flags, -)
+//   quads   : 5 x float4 per quad/area light: (Q, D) (u, flags) (v, area) (n, gid) (w, material)
+//   insts   : 4 x float4 per instance: three rows of world->object [R|t] and
+//             (blas_root, instance id, -, -) as int
+// Materials (2 x float4), textures (2 x float4) and one texel byte pool complete the scene.
+#pragma once
+#include <stdint.h>
+
+#define CRT_PRIM_TRI 0u
+#define CRT_PRIM_SPHERE 1u
+#define CRT_PRIM_QUAD 2u
+#define CRT_PRIM_INST 3u
+
+#define CRT_LEAF_TYPE_SHIFT 28u
+#define CRT_LEAF_COUNT_MASK 0x0fffffffu
+
+// quad flags (float4 #1 .w as uint)
+#define CRT_QF_LIGHT 1u        // centred parametrisation, interior in [-0.5, 0.5]^2, uv shifted by 0.5
+#define CRT_QF_INVISIBLE 2u    // hidden from camera rays (directional_light.cuh:106-109)
+#define CRT_QF_ONE_SIDED 4u    // back faces trigger the pass-through re-trace
+// triangle flags (float4 #2 .w as uint)
+#define CRT_TF_SMOOTH 1u
+#define CRT_TF_UV 2u
+
+// hit.prim encoding: type << 30 | leaf-ordered index; CRT_MISS = no hit
+#define CRT_MISS 0xffffffffu
+#define CRT_HIT_TYPE(p) ((p) >> 30)
+#define CRT_HIT_INDEX(p) ((p)&0x3fffffffu)
+// hit.inst: instance id or CRT_NO_INST; bit 31 marks a pass-through re-trace result
+#define CRT_NO_INST 0x7fffffffu
+#define CRT_PASSTHROUGH 0x80000000u
+
+// material record (2 x float4)
+//  m0 = (albedo.rgb, param)   param = fuzz (metal) | ior (dielectric) | intensity (light)
+//  m1 = (type, texture, flags, -) as int
+// texture record (2 x float4)
+//  t0 = (color.rgb | -, scale)
+//  t1 = (type | width<<? ...) -> as ints: (type, a, b, c) with
+//       SOLID: -, CHECKER: (type, even, odd, -), IMAGE: (type, width, height, pool offset / 4 dwords)
+struct crt_dev_scene {
+    const void* nodes;      // float4
+    const void* leaves;     // uint2
+    const void* tris;       // float4
+    const void* tri_normals;// float4 x3 per triangle (optional)
+    const void* tri_uvs;    // float2 x3 per triangle (optional)
+    const void* spheres;    // float4
+    const void* sphere_aux; // uint4
+    const void* sphere_map; // float2 x3 per sphere (scale, offset, repeat)
+    const void* quads;      // float4
+    const void* insts;      // float4
+    const void* mats;       // float4
+    const void* texs;       // float4
+    const uint8_t* texels;  // byte pool (rows top-down, RGB(A)8)
+    const uint32_t* light_quads;  // leaf-ordered quad index of each area light (world-list order)
+    uint32_t num_lights;
+    uint32_t root;          // root node index
+    uint32_t num_quads;
+    uint32_t pad0;
+    float background[3];
+    float pad1;
+};
+
+// Per-request parameters read by the generate / shade / resolve kernels.
+struct crt_dev_request {
+    // camera (core/src/cameras/perspective_camera.cuh:41-113), computed on the host in FP32
+    float origin[3], llc[3], horizontal[3], vertical[3], cu[3], cv[3];
+    float lens_radius, time0, time1;
+    uint32_t width, height, sqrt_spp, spp, max_depth;
+    uint32_t rng_mode;
+    uint32_t scene;          // index into the scene table
+    uint64_t seed;
+    // pass work: pixel range x sample range, laid out sample-major from path_base
+    uint32_t path_base;      // first path slot of this request in the pass
+    uint32_t pix_begin, pix_count;
+    uint32_t sample_begin, sample_count;
+    float* accum;            // float4 per pixel (rgb sums, sample count), rows top-down
+    uint32_t* rng_state;     // MINSTD: per-pixel engine state carried across samples
+    uint32_t pad[2];
+};
+
+// Traversal statistics (counting kernel only)
+struct crt_dev_counters {
+    unsigned long long node_visits, leaf_visits, tri_tests, sphere_tests, quad_tests, inst_visits;
+    unsigned long long rays, retraces, light_tests, stack_overflow;
+};

@@ -1,0 +1,142 @@
+// crt_runtime.h - host runtime objects behind the C-ABI: device buffers, resident scenes, render
+// jobs, the pass scheduler and the serving loop.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "cortex_rt.h"
+#include "crt_layout.h"
+#include "scene_build.h"
+
+namespace crt {
+
+int set_error(int code, const std::string& msg);
+double now_s();
+void camera_setup(const crt_camera& c, crt_dev_request& r);
+
+struct Context;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int alloc(size_t n, Context* ctx);
+    void release();
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+};
+
+struct DeviceScene {
+    HostScene host;  // geometry arrays are dropped after upload; stats and small tables stay
+    DevBuf nodes, leaves, tris, tri_normals, tri_uvs, spheres, sphere_aux, sphere_map, quads, insts, mats, texs, texels,
+        light_quads;
+    std::vector<uint32_t> tri_gid, quad_gid;   // leaf order -> global primitive id
+    crt_dev_scene dev{};
+    uint32_t slot = 0;
+    uint64_t bytes = 0;
+    double upload_ms = 0.0;
+    std::atomic<int> refs{0};
+};
+
+struct Job;
+struct WorkItem {
+    Job* job = nullptr;
+    uint32_t pix_begin = 0, pix_count = 0, sample_begin = 0, sample_count = 0;
+};
+
+struct Job {
+    crt_render_desc rd{};
+    DeviceScene* scene = nullptr;
+    crt_dev_request req{};
+    uint32_t npix = 0, sq = 0, nsamples = 0, s_begin = 0, s_end = 0, s_next = 0, p_next = 0;
+    DevBuf accum, rgb, rng;
+    uint64_t rays_in_flight = 0;
+    static int validate(const crt_render_desc& r);
+    static uint64_t bytes_needed(const crt_render_desc& r);
+    int setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r);
+    bool take(uint32_t cap, WorkItem& w);
+    bool scheduled_all() const { return s_next >= s_end; }
+};
+
+// Serving request (continuous batching)
+struct ServeReq {
+    crt_request_id id = 0;
+    crt_scene_id scene = 0;
+    DeviceScene* sc = nullptr;     // pinned by sc->refs until the request completes
+    crt_render_desc rd{};
+    std::unique_ptr<Job> job;
+    std::vector<uint8_t> rgb;      // result (host)
+    std::vector<float> accum;
+    crt_request_timing timing{};
+    int status = CRT_ERR_NOT_READY;
+    bool cancel = false;
+    uint32_t passes_pending = 0;   // passes submitted to the GPU that include this request
+    uint64_t last_pass = 0;        // sequence number of the last pass that includes it
+    crt_dev_counters counters{};
+};
+
+struct Context {
+    static constexpr uint32_t kMaxScenes = 1024;
+    static constexpr uint32_t kMaxPassRequests = 1024;
+    crt_config config{};
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, reserved = 0, scene_bytes = 0, request_bytes = 0;
+    uint32_t max_paths = 0, path_capacity = 0, leaf_size = 4, extend_grid = 2048, shade_grid = 2048;
+    std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
+    std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
+    bool slot_used[kMaxScenes] = {};
+    crt_scene_id next_scene_id = 1;
+    DevBuf scene_table, counters, counts, gstack;
+    DevBuf req_table[2];
+    void* req_host[2] = {nullptr, nullptr};
+    hipEvent_t req_event[2] = {nullptr, nullptr};
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_ext[2] = {nullptr, nullptr};
+    void* counters_host = nullptr;
+    int pass_parity = 0;
+    uint64_t passes = 0, extend_launches = 0;
+    DevBuf ray_o, ray_d, thr, rad, ids, hit, hit_inst, queue[2];
+    int fail_alloc_after = -1;       // test hook: fail the n-th next allocation
+
+    // serving
+    std::mutex srv_mutex;
+    std::condition_variable srv_cv, srv_done_cv;
+    std::thread srv_thread;
+    bool srv_running = false, srv_stop = false;
+    std::deque<ServeReq*> srv_waiting;                 // submitted, not admitted
+    std::vector<ServeReq*> srv_active;                 // admitted, on the device
+    std::map<crt_request_id, std::unique_ptr<ServeReq>> srv_all;
+    crt_request_id next_req_id = 1;
+
+    int init(const crt_config* cfg);
+    ~Context();
+    uint64_t used_bytes() const;
+    int ensure_paths(uint32_t n);
+    int upload_scene(const crt_scene_desc* desc, crt_scene_id* out);
+    int release_scene(crt_scene_id id);
+    DeviceScene* find_scene(crt_scene_id id);
+    int run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms);
+    int finish_job(Job& j, crt_image* out);
+    int reset_counters();
+    int read_counters(crt_dev_counters& c);
+    int render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* stats);
+    // serving
+    int submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_id* out);
+    void server_loop();
+    void stop_server();
+};
+
+void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms);
+
+}  // namespace crt

@@ -1,0 +1,988 @@
+// kernels.hip - gfx950 wavefront path-tracing kernels.
+//
+// One pass of the hot path (core/src/gpu.cu:761-802 per pixel, core/src/cameras/camera.cuh:149-264
+// per bounce) is split into stages that run over every live path of every request in the pass:
+//   k_generate  camera rays + keyed RNG            (render kernel sample loop + get_ray)
+//   k_extend    BVH2 traversal, closest hit         (hittable_list/bvh_node/aabb/primitive ::hit)
+//   k_shade     emission, scatter, light mixture    (ray_color body, materials, pdfs, textures)
+//               + queue compaction (wave ballot, block prefix scan, one atomic per block)
+//   k_resolve   per-pixel sums in sample order      (pixel_color += ray_color)
+//   k_tonemap   NaN->0, /spp, sqrt, clamp, *255.99  (prepare_pixel_color + quantize)
+// MFMA is not used: nothing here is a dense contraction. The traversal is bound by node/primitive
+// fetches (HBM / L2 / MALL) and VALU; layouts are SoA float4 so every fetch is a dwordx4.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cortex_rt.h"
+#include "crt_layout.h"
+#include "crt_kernels.h"
+
+namespace {
+
+// ------------------------------------------------------------------ small vector helpers
+struct V3 { float x, y, z; };
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 v3(float4 a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 v3p(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ float len2(V3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+__device__ __forceinline__ float vlen(V3 v) { return sqrtf(len2(v)); }
+__device__ __forceinline__ V3 unit(V3 v) { float l = vlen(v); return {v.x / l, v.y / l, v.z / l}; }
+__device__ __forceinline__ float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+
+template <class T> __device__ __forceinline__ T ldg(const T* p) { return *p; }
+
+// ------------------------------------------------------------------ random streams
+// Thrust minstd_rand step with Carta's reduction for m = 2^31 - 1 (exact).
+__device__ __forceinline__ uint32_t minstd_next(uint32_t& x) {
+    uint64_t p = (uint64_t)x * 48271u;
+    uint32_t r = (uint32_t)(p & 0x7fffffffu) + (uint32_t)(p >> 31);
+    if (r >= 0x7fffffffu) r -= 0x7fffffffu;
+    x = r;
+    return r;
+}
+// uniform_real_distribution<float>(0,1): float(x-1) / 2^31 (the float denominator rounds to 2^31)
+__device__ __forceinline__ float minstd_uniform(uint32_t& x) {
+    float r = (float)(minstd_next(x) - 1u);
+    return r / 2147483648.0f;
+}
+struct Pcg { uint64_t state, inc; };
+__device__ __forceinline__ uint32_t pcg_next(Pcg& g) {
+    uint64_t old = g.state;
+    g.state = old * 6364136223846793005ULL + g.inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ Pcg pcg_keyed(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce) {
+    uint64_t key = ((uint64_t)pixel << 32) | (uint64_t)sample;
+    uint64_t initstate = splitmix64(seed ^ splitmix64(key));
+    Pcg g;
+    g.state = 0u;
+    g.inc = ((uint64_t)bounce << 1u) | 1u;
+    pcg_next(g);
+    g.state += initstate;
+    pcg_next(g);
+    return g;
+}
+
+// One draw source; MODE is a template constant so each kernel instance is branch-free.
+template <int MODE> struct Rng {
+    uint32_t ms;
+    Pcg pcg;
+    __device__ __forceinline__ float uniform() {
+        if (MODE == CRT_RNG_MINSTD) return minstd_uniform(ms);
+        return (float)(pcg_next(pcg) >> 8) * (1.0f / 16777216.0f);
+    }
+    __device__ __forceinline__ void skip() {
+        if (MODE == CRT_RNG_MINSTD) minstd_next(ms); else pcg_next(pcg);
+    }
+};
+
+// ------------------------------------------------------------------ traversal stack
+// Short per-lane stack: RS entries in VGPRs (static shifts, no dynamic register indexing), spilling
+// to an LS-entry ring per lane in LDS, which in turn spills its oldest entries to a per-thread
+// overflow area in global memory. Lower levels are used only while the registers are full.
+template <int BLOCK, int RS, int LS, int GS> struct Stack {
+    uint32_t r[RS];
+    int rc, lbase, lcnt, gsp;
+    uint32_t* lds;     // column of this lane: lds[e * BLOCK]
+    uint32_t* gmem;    // GS entries of this thread
+    bool overflow;
+    __device__ __forceinline__ void init(uint32_t* lds_col, uint32_t* g) {
+        rc = 0; lbase = 0; lcnt = 0; gsp = 0; lds = lds_col; gmem = g; overflow = false;
+    }
+    __device__ __forceinline__ int depth() const { return rc + lcnt + gsp; }
+    __device__ __forceinline__ bool empty() const { return rc == 0; }
+    __device__ __forceinline__ void spill(uint32_t v) {
+        if (lcnt == LS) {
+            if (gsp < GS) gmem[gsp++] = lds[lbase * BLOCK]; else overflow = true;
+            lbase = (lbase + 1) & (LS - 1);
+            --lcnt;
+        }
+        lds[((lbase + lcnt) & (LS - 1)) * BLOCK] = v;
+        ++lcnt;
+    }
+    __device__ __forceinline__ uint32_t refill() {
+        if (lcnt > 0) { --lcnt; return lds[((lbase + lcnt) & (LS - 1)) * BLOCK]; }
+        return gmem[--gsp];
+    }
+    __device__ __forceinline__ void push(uint32_t v) {
+        if (rc == RS) spill(r[RS - 1]); else ++rc;
+#pragma unroll
+        for (int k = RS - 1; k > 0; --k) r[k] = r[k - 1];
+        r[0] = v;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        uint32_t v = r[0];
+#pragma unroll
+        for (int k = 0; k < RS - 1; ++k) r[k] = r[k + 1];
+        if (lcnt > 0 || gsp > 0) r[RS - 1] = refill(); else --rc;
+        return v;
+    }
+};
+
+// ------------------------------------------------------------------ primitive tests
+struct TriPre { int kx, ky, kz; float Sx, Sy, Sz; };
+__device__ __forceinline__ TriPre tri_pre(V3 d) {
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = comp(d, kz);
+    if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    TriPre p;
+    p.kx = kx; p.ky = ky; p.kz = kz;
+    p.Sx = comp(d, kx) / dz; p.Sy = comp(d, ky) / dz; p.Sz = 1.0f / dz;
+    return p;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013), two-sided. The edge
+// functions are computed without FMA contraction so a shared edge gives exactly negated values
+// in both triangles; exact zeros are re-evaluated in double as in the paper.
+__device__ __forceinline__ bool tri_hit(V3 o, const TriPre& p, float4 a, float4 b, float4 c, float tmin, float tmax,
+                                        float& t, float& bu, float& bv) {
+#pragma clang fp contract(off)
+    V3 A = v3(a) - o, B = v3(b) - o, C = v3(c) - o;
+    float Akz = comp(A, p.kz), Bkz = comp(B, p.kz), Ckz = comp(C, p.kz);
+    float Ax = comp(A, p.kx) - p.Sx * Akz, Ay = comp(A, p.ky) - p.Sy * Akz;
+    float Bx = comp(B, p.kx) - p.Sx * Bkz, By = comp(B, p.ky) - p.Sy * Bkz;
+    float Cx = comp(C, p.kx) - p.Sx * Ckz, Cy = comp(C, p.ky) - p.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (p.Sz * Akz) + V * (p.Sz * Bkz) + W * (p.Sz * Ckz);
+    float inv = 1.0f / det;
+    float tt = T * inv;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt; bu = V * inv; bv = W * inv;
+    return true;
+}
+
+// Sphere: strict interval like sphere::hit (core/src/primitives/sphere.cuh:111-132), computed with
+// the perpendicular-distance discriminant and the stable root pair.
+__device__ __forceinline__ bool sphere_hit(V3 o, V3 d, float4 s, float tmin, float tmax, float& t) {
+    V3 oc = o - v3(s);
+    float a = len2(d);
+    float hb = dot(oc, d);
+    float cc = len2(oc) - s.w * s.w;
+    V3 l = oc - (hb / a) * d;
+    float disc = a * (s.w * s.w - len2(l));
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    float q = hb >= 0.0f ? -(hb + sq) : -(hb - sq);
+    float r0, r1;
+    if (q == 0.0f) { r0 = r1 = -hb / a; }
+    else { r0 = cc / q; r1 = q / a; }
+    if (r0 > r1) { float x = r0; r0 = r1; r1 = x; }
+    if (tmin < r0 && r0 < tmax) { t = r0; return true; }
+    if (tmin < r1 && r1 < tmax) { t = r1; return true; }
+    return false;
+}
+
+// Quad / area light: quad::hit (core/src/primitives/quad.cuh:100-133) and directional_light::hit
+// (core/src/lights/directional_light.cuh:76-118), inclusive interval.
+__device__ __forceinline__ bool quad_hit(V3 o, V3 d, const float4* q, float tmin, float tmax, bool camera,
+                                         float& t, float& qa, float& qb) {
+    float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+    V3 n = v3(q3);
+    float denom = dot(n, d);
+    if (fabsf(denom) < 1e-8f) return false;
+    float tt = (q0.w - dot(n, o)) / denom;
+    if (!(tmin <= tt && tt <= tmax)) return false;
+    V3 p = o + tt * d;
+    V3 hp = p - v3(q0);
+    V3 w = v3(q4);
+    float alpha = dot(w, cross(hp, v3(q2)));
+    float beta = dot(w, cross(v3(q1), hp));
+    uint32_t flags = fbits(q1.w);
+    if (flags & CRT_QF_LIGHT) {
+        if (alpha < -0.5f || 0.5f < alpha || beta < -0.5f || 0.5f < beta) return false;
+        if ((flags & CRT_QF_INVISIBLE) && camera) return false;
+        qa = alpha + 0.5f; qb = beta + 0.5f;
+    } else {
+        if (alpha < 0.0f || 1.0f < alpha || beta < 0.0f || 1.0f < beta) return false;
+        qa = alpha; qb = beta;
+    }
+    t = tt;
+    return true;
+}
+
+struct HitRec { float t, u, v; uint32_t prim, inst; };
+
+struct SceneView {
+    const float4* nodes; const uint2* leaves; const float4* tris; const float4* spheres; const float4* quads;
+    const float4* insts; uint32_t root;
+};
+__device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
+    SceneView v;
+    v.nodes = (const float4*)sc->nodes; v.leaves = (const uint2*)sc->leaves; v.tris = (const float4*)sc->tris;
+    v.spheres = (const float4*)sc->spheres; v.quads = (const float4*)sc->quads; v.insts = (const float4*)sc->insts;
+    v.root = sc->root;
+    return v;
+}
+
+struct TravCount { uint32_t nodes, leaves, tris, spheres, quads, insts; };
+
+// Closest hit over the two-level BVH2. Interior nodes test both child boxes (slab test on
+// (lo - o) * inv_d, Ize's robust factor on the far distance) and descend near-first.
+template <int BLOCK, int RS, int LS, int GS, bool COUNT>
+__device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
+                                      uint32_t* lds_col, uint32_t* gstk, HitRec& best, TravCount& tc, bool& ovf) {
+    Stack<BLOCK, RS, LS, GS> st;
+    st.init(lds_col, gstk);
+    V3 o = ow, d = dw;
+    auto inv_of = [](float x) { return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x)); };
+    V3 id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
+    TriPre tp = tri_pre(d);
+    best.t = tmax; best.prim = CRT_MISS; best.inst = CRT_NO_INST; best.u = 0.0f; best.v = 0.0f;
+    uint32_t cur_inst = CRT_NO_INST;
+    bool in_blas = false;
+    int blas_level = 0;
+    int32_t node = (int32_t)S.root;
+    const float robust = 1.0000004f;
+    for (;;) {
+        bool next = true;
+        if (node >= 0) {
+            const float4* np = S.nodes + 4 * (uint32_t)node;
+            float4 n0 = np[0], n1 = np[1], n2 = np[2];
+            int4 n3 = *(const int4*)(np + 3);
+            if (COUNT) tc.nodes++;
+            float tf = best.t;
+            // child 0
+            float l0x = (n0.x - o.x) * id.x, h0x = (n0.y - o.x) * id.x;
+            float l0y = (n0.z - o.y) * id.y, h0y = (n0.w - o.y) * id.y;
+            float l0z = (n2.x - o.z) * id.z, h0z = (n2.y - o.z) * id.z;
+            float near0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), tmin));
+            float far0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tf));
+            // child 1
+            float l1x = (n1.x - o.x) * id.x, h1x = (n1.y - o.x) * id.x;
+            float l1y = (n1.z - o.y) * id.y, h1y = (n1.w - o.y) * id.y;
+            float l1z = (n2.z - o.z) * id.z, h1z = (n2.w - o.z) * id.z;
+            float near1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), tmin));
+            float far1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tf));
+            bool hit0 = near0 <= far0 * robust;
+            bool hit1 = near1 <= far1 * robust;
+            if (hit0 && hit1) {
+                bool swap = near1 < near0;
+                int32_t a = swap ? n3.y : n3.x, b = swap ? n3.x : n3.y;
+                st.push((uint32_t)b);
+                node = a; next = false;
+            } else if (hit0) { node = n3.x; next = false; }
+            else if (hit1) { node = n3.y; next = false; }
+        } else {
+            uint2 lf = S.leaves[(uint32_t)(~node)];
+            uint32_t type = lf.y >> CRT_LEAF_TYPE_SHIFT;
+            uint32_t cnt = lf.y & CRT_LEAF_COUNT_MASK;
+            uint32_t first = lf.x;
+            if (COUNT) tc.leaves++;
+            if (type == CRT_PRIM_TRI) {
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const float4* tp4 = S.tris + 3 * (first + k);
+                    float t, bu, bv;
+                    if (COUNT) tc.tris++;
+                    if (tri_hit(o, tp, tp4[0], tp4[1], tp4[2], tmin, best.t, t, bu, bv)) {
+                        best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
+                        best.inst = cur_inst;
+                    }
+                }
+            } else if (type == CRT_PRIM_SPHERE) {
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    float4 s = S.spheres[first + k];
+                    float t;
+                    if (COUNT) tc.spheres++;
+                    if (sphere_hit(o, d, s, tmin, best.t, t)) {
+                        best.t = t; best.u = 0.0f; best.v = 0.0f; best.prim = (CRT_PRIM_SPHERE << 30) | (first + k);
+                        best.inst = cur_inst;
+                    }
+                }
+            } else if (type == CRT_PRIM_QUAD) {
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    float t, qa, qb;
+                    if (COUNT) tc.quads++;
+                    if (quad_hit(o, d, S.quads + 5 * (first + k), tmin, best.t, camera, t, qa, qb)) {
+                        best.t = t; best.u = qa; best.v = qb; best.prim = (CRT_PRIM_QUAD << 30) | (first + k);
+                        best.inst = cur_inst;
+                    }
+                }
+            } else if (!in_blas) {
+                // instance leaf (count 1): move the ray to object space and descend into the BLAS
+                const float4* ip = S.insts + 4 * first;
+                float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
+                int4 meta = *(const int4*)(ip + 3);
+                if (COUNT) tc.insts++;
+                o = v3(r0.x * ow.x + r0.y * ow.y + r0.z * ow.z + r0.w, r1.x * ow.x + r1.y * ow.y + r1.z * ow.z + r1.w,
+                       r2.x * ow.x + r2.y * ow.y + r2.z * ow.z + r2.w);
+                d = v3(r0.x * dw.x + r0.y * dw.y + r0.z * dw.z, r1.x * dw.x + r1.y * dw.y + r1.z * dw.z,
+                       r2.x * dw.x + r2.y * dw.y + r2.z * dw.z);
+                id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
+                tp = tri_pre(d);
+                cur_inst = (uint32_t)meta.y;
+                in_blas = true;
+                blas_level = st.depth();
+                node = meta.x;
+                next = false;
+            }
+        }
+        if (next) {
+            if (in_blas && st.depth() == blas_level) {
+                o = ow; d = dw;
+                id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
+                tp = tri_pre(d);
+                in_blas = false; cur_inst = CRT_NO_INST;
+            }
+            if (st.empty()) break;
+            node = (int32_t)st.pop();
+        }
+    }
+    ovf = st.overflow;
+}
+
+// ------------------------------------------------------------------ block-wide append (compaction)
+// Each wave ballots its keepers, lane offsets come from mbcnt, the block's wave totals are scanned
+// in LDS and one atomicAdd per block reserves the output range.
+template <int BLOCK> __device__ __forceinline__ uint32_t block_append(bool keep, uint32_t* counter, uint32_t* s_wave,
+                                                                    uint32_t* s_base) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NW = BLOCK / 64;
+    uint64_t m = __ballot(keep);
+    uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+    uint32_t rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < NW; ++w) { uint32_t c = s_wave[w]; s_wave[w] = run; run += c; }
+        *s_base = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = *s_base + s_wave[wave] + rank;
+    __syncthreads();
+    return pos;
+}
+
+}  // namespace
+
+// ======================================================================================== kernels
+
+// Path state (SoA over path slots):
+//   ray_o  float4 (origin, time)           ray_d float4 (dir, state: depth | bounce << 8 | camera << 16)
+//   thr    float4 (throughput, minstd state) rad   float4 (radiance sum, -)
+//   ids    uint2  (storage pixel, request << 16 | sample-in-request)
+//   hit    float4 (t, prim bits, u, v)      hit_inst uint32
+
+#define ST_DEPTH(s) ((s)&0xffu)
+#define ST_BOUNCE(s) (((s) >> 8) & 0xffu)
+#define ST_CAMERA 0x10000u
+
+__device__ __forceinline__ int find_request(const crt_dev_request* reqs, int nreq, uint32_t p) {
+    int lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].path_base <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Camera rays: render kernel sample loop (core/src/gpu.cu:776-787) + perspective_camera::get_ray
+// (core/src/cameras/perspective_camera.cuh:106-113): jitter r, diagonal stratified (u, v),
+// lens disk by rejection, time draw.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_generate(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t npaths,
+                                                  float4* __restrict__ ray_o, float4* __restrict__ ray_d,
+                                                  float4* __restrict__ thr, float4* __restrict__ rad,
+                                                  uint2* __restrict__ ids) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npaths) return;
+    int r = find_request(reqs, nreq, p);
+    const crt_dev_request& R = reqs[r];
+    uint32_t local = p - R.path_base;
+    uint32_t k = local / R.pix_count;
+    uint32_t pix = R.pix_begin + (local - k * R.pix_count);
+    uint32_t sample = R.sample_begin + k;
+    uint32_t W = R.width, H = R.height;
+    uint32_t jj = pix / W, i = pix - jj * W;
+    uint32_t j = H - 1u - jj;                  // the reference's row index counts from the bottom
+    uint32_t ref_pixel = j * W + i;
+    int sq = (int)R.sqrt_spp;
+    int sj = (int)(sample / (uint32_t)sq), si = (int)(sample - (uint32_t)sj * (uint32_t)sq);
+    Rng<MODE> g;
+    if (MODE == CRT_RNG_MINSTD) g.ms = R.rng_state[pix];
+    else g.pcg = pcg_keyed(R.seed, ref_pixel, sample, 0u);
+    float rr = g.uniform();
+    float u = ((float)i + ((float)si + rr) / (float)sq) / (float)W;
+    float v = ((float)j + ((float)sj + rr) / (float)sq) / (float)H;
+    float dx, dy;
+    do {
+        float a = g.uniform();
+        float b = g.uniform();
+        dx = 2.0f * a - 1.0f; dy = 2.0f * b - 1.0f;
+    } while (dx * dx + dy * dy + 0.0f * 0.0f >= 1.0f);
+    V3 rd = R.lens_radius * v3(dx, dy, 0.0f);
+    V3 cu = v3p(R.cu), cv = v3p(R.cv), org = v3p(R.origin);
+    V3 off = cu * rd.x + cv * rd.y;
+    float tm = R.time0 + g.uniform() * (R.time1 - R.time0);
+    V3 o = org + off;
+    V3 dir = v3p(R.llc) + u * v3p(R.horizontal) + v * v3p(R.vertical) - org - off;
+    uint32_t state = (R.max_depth & 0xffu) | ST_CAMERA;
+    ray_o[p] = make_float4(o.x, o.y, o.z, tm);
+    ray_d[p] = make_float4(dir.x, dir.y, dir.z, bitsf(state));
+    thr[p] = make_float4(1.0f, 1.0f, 1.0f, bitsf(g.ms));
+    rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ids[p] = make_uint2(pix, ((uint32_t)r << 16) | (sample & 0xffffu));
+}
+
+// Closest-hit stage. One-sided emitters hit from behind are re-traced from t + 0.001 in the same
+// stage (core/src/cameras/camera.cuh:179-183); the result is flagged as a pass-through.
+template <int BLOCK, int RS, int LS, int GS, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restrict__ scenes,
+                                                  const crt_dev_request* __restrict__ reqs,
+                                                  const uint32_t* __restrict__ queue, const uint32_t* __restrict__ count_ptr,
+                                                  uint32_t count_direct, const float4* __restrict__ ray_o,
+                                                  const float4* __restrict__ ray_d, const uint2* __restrict__ ids,
+                                                  float4* __restrict__ hit, uint32_t* __restrict__ hit_inst,
+                                                  uint32_t* __restrict__ gstack, uint32_t* __restrict__ zero_counter,
+                                                  crt_dev_counters* __restrict__ counters) {
+    __shared__ uint32_t s_stack[LS * BLOCK];
+    if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
+    const uint32_t n = queue ? *count_ptr : count_direct;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t stride = gridDim.x * BLOCK;
+    uint32_t* lds_col = s_stack + threadIdx.x;
+    uint32_t* gst = gstack + (size_t)gtid * GS;
+    TravCount tc = {0, 0, 0, 0, 0, 0};
+    uint32_t nrays = 0, nretrace = 0, novf = 0;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        uint32_t slot = queue ? queue[i] : i;
+        float4 ro = ray_o[slot], rdv = ray_d[slot];
+        uint32_t state = fbits(rdv.w);
+        uint32_t req = ids[slot].y >> 16;
+        const crt_dev_scene* sc = scenes + reqs[req].scene;
+        SceneView S = scene_view(sc);
+        V3 o = v3(ro), d = v3(rdv);
+        bool camera = (state & ST_CAMERA) != 0;
+        HitRec h;
+        bool ovf = false;
+        trace<BLOCK, RS, LS, GS, COUNT>(S, o, d, 0.00001f, 3.402823466e+38f, camera, lds_col, gst, h, tc, ovf);
+        nrays++;
+        uint32_t inst_word = h.inst;
+        if (h.prim != CRT_MISS && CRT_HIT_TYPE(h.prim) == CRT_PRIM_QUAD) {
+            const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
+            uint32_t fl = fbits(q[1].w);
+            if ((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(q[3])) < 0.0f)) {
+                HitRec h2;
+                trace<BLOCK, RS, LS, GS, COUNT>(S, o, d, h.t + 0.001f, 3.402823466e+38f, camera, lds_col, gst, h2, tc, ovf);
+                nretrace++;
+                h = h2;
+                inst_word = h2.inst | CRT_PASSTHROUGH;
+            }
+        }
+        novf += ovf ? 1u : 0u;
+        hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
+        hit_inst[slot] = inst_word;
+    }
+    if (nrays) atomicAdd(&counters->rays, (unsigned long long)nrays);
+    if (nretrace) atomicAdd(&counters->retraces, (unsigned long long)nretrace);
+    if (COUNT) {
+        atomicAdd(&counters->node_visits, (unsigned long long)tc.nodes);
+        atomicAdd(&counters->leaf_visits, (unsigned long long)tc.leaves);
+        atomicAdd(&counters->tri_tests, (unsigned long long)tc.tris);
+        atomicAdd(&counters->sphere_tests, (unsigned long long)tc.spheres);
+        atomicAdd(&counters->quad_tests, (unsigned long long)tc.quads);
+        atomicAdd(&counters->inst_visits, (unsigned long long)tc.insts);
+    }
+    if (novf) atomicAdd(&counters->stack_overflow, (unsigned long long)novf);
+}
+
+// Closest hit of caller rays (first-hit parity queries).
+template <int BLOCK, int RS, int LS, int GS>
+__global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __restrict__ sc, const float4* __restrict__ rays,
+                                                      uint32_t n, int camera, float4* __restrict__ hit,
+                                                      uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
+                                                      crt_dev_counters* __restrict__ counters) {
+    __shared__ uint32_t s_stack[LS * BLOCK];
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t stride = gridDim.x * BLOCK;
+    SceneView S = scene_view(sc);
+    TravCount tc = {0, 0, 0, 0, 0, 0};
+    uint32_t novf = 0;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        float4 a = rays[2 * i], b = rays[2 * i + 1];
+        HitRec h;
+        bool ovf = false;
+        trace<BLOCK, RS, LS, GS, false>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
+                                        gstack + (size_t)gtid * GS, h, tc, ovf);
+        novf += ovf ? 1u : 0u;
+        hit[i] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
+        hit_inst[i] = h.inst;
+    }
+    if (novf) atomicAdd(&counters->stack_overflow, (unsigned long long)novf);
+}
+
+// ------------------------------------------------------------------ shading
+namespace {
+struct Mat { float4 m0; int4 m1; };
+
+__device__ __forceinline__ float4 tex_value(const float4* texs, const uint8_t* texels, int ti, float u, float v, V3 p) {
+    for (int hop = 0; hop < 2; ++hop) {
+        float4 t0 = texs[2 * ti];
+        int4 t1 = *(const int4*)(texs + 2 * ti + 1);
+        if (t1.x == CRT_TEX_SOLID) return t0;
+        if (t1.x == CRT_TEX_CHECKER) {
+            // checker_texture::value (core/src/textures/checker_texture.cuh:33-42); t0.w = 1/scale
+            float inv = t0.w;
+            int xi = (int)floorf(inv * p.x), yi = (int)floorf(inv * p.y), zi = (int)floorf(inv * p.z);
+            bool even = ((xi + yi + zi) % 2) == 0;
+            ti = even ? t1.y : t1.z;
+            if (hop == 1) break;
+            continue;
+        }
+        // image_texture::value (core/src/textures/image_texture.cuh:54-71)
+        uint32_t W = (uint32_t)t1.y, H = (uint32_t)t1.z;
+        uint32_t off = (uint32_t)fbits(t0.x), ch = (uint32_t)fbits(t0.y);
+        if (H == 0) return make_float4(0.0f, 1.0f, 1.0f, 0.0f);
+        float uu = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+        float vv = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+        vv = 1.0f - vv;
+        int i = (int)(uu * (float)W), j = (int)(vv * (float)H);
+        i = i < 0 ? 0 : (i < (int)W ? i : (int)W - 1);
+        j = j < 0 ? 0 : (j < (int)H ? j : (int)H - 1);
+        const uint8_t* px = texels + off + ((size_t)j * W + i) * ch;
+        const float sc = 1.0f / 255.0f;
+        if (ch == 4u) {
+            uint32_t w4 = *(const uint32_t*)px;
+            return make_float4(sc * (float)(w4 & 0xffu), sc * (float)((w4 >> 8) & 0xffu), sc * (float)((w4 >> 16) & 0xffu), 0.0f);
+        }
+        return make_float4(sc * (float)px[0], sc * (float)px[1], sc * (float)px[2], 0.0f);
+    }
+    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+}  // namespace
+
+template <int MODE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict__ scenes,
+                                                 const crt_dev_request* __restrict__ reqs,
+                                                 const uint32_t* __restrict__ queue_in, const uint32_t* __restrict__ count_in,
+                                                 uint32_t count_direct, uint32_t* __restrict__ queue_out,
+                                                 uint32_t* __restrict__ count_out, float4* __restrict__ ray_o,
+                                                 float4* __restrict__ ray_d, float4* __restrict__ thr,
+                                                 float4* __restrict__ rad, const uint2* __restrict__ ids,
+                                                 const float4* __restrict__ hit, const uint32_t* __restrict__ hit_inst,
+                                                 crt_dev_counters* __restrict__ counters) {
+    __shared__ uint32_t s_wave[BLOCK / 64];
+    __shared__ uint32_t s_base;
+    const uint32_t n = queue_in ? *count_in : count_direct;
+    const uint32_t stride = gridDim.x * BLOCK;
+    uint32_t light_tests = 0;
+    // grid-stride loop with a block-uniform trip count so every thread reaches the block scan
+    const uint32_t iters = (n + stride - 1) / stride;
+    for (uint32_t it = 0; it < iters; ++it) {
+        uint32_t i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
+        bool keep = false;
+        uint32_t slot = 0;
+        if (i < n) {
+            slot = queue_in ? queue_in[i] : i;
+            float4 hv = hit[slot];
+            uint32_t iw = hit_inst[slot];
+            float4 ro = ray_o[slot], rdv = ray_d[slot];
+            float4 tv = thr[slot], rv = rad[slot];
+            uint2 idv = ids[slot];
+            uint32_t state = fbits(rdv.w);
+            uint32_t req = idv.y >> 16;
+            const crt_dev_request& R = reqs[req];
+            const crt_dev_scene* sc = scenes + R.scene;
+            V3 o = v3(ro), d = v3(rdv);
+            V3 att = v3(tv), L = v3(rv);
+            uint32_t prim = fbits(hv.y);
+            bool passthrough = (iw & CRT_PASSTHROUGH) != 0;
+            uint32_t inst = iw & ~CRT_PASSTHROUGH;
+            uint32_t depth = ST_DEPTH(state), bounce = ST_BOUNCE(state);
+            Rng<MODE> g;
+            g.ms = fbits(tv.w);
+            bool done = true;
+            if (prim == CRT_MISS) {
+                // miss: background (camera.cuh:161-174); a missed pass-through re-trace keeps the
+                // light record, which does not scatter and adds colour(0,0,0,0)
+                if (!passthrough) L = L + att * v3(sc->background[0], sc->background[1], sc->background[2]);
+            } else {
+                bounce += 1u;
+                if (MODE == CRT_RNG_PCG) {
+                    uint32_t W = R.width;
+                    uint32_t jj = idv.x / W, ii = idv.x - jj * W;
+                    uint32_t ref_pixel = (R.height - 1u - jj) * W + ii;
+                    g.pcg = pcg_keyed(R.seed, ref_pixel, idv.y & 0xffffu, bounce);
+                }
+                uint32_t type = CRT_HIT_TYPE(prim), idx = CRT_HIT_INDEX(prim);
+                float t = hv.x;
+                V3 p = o + t * d;
+                V3 outward;
+                uint32_t mat_id;
+                float tu = hv.z, tvv = hv.w;  // texture coordinates (filled lazily below)
+                bool need_uv_sphere = false;
+                bool is_light_quad = false;
+                V3 sphere_c; float sphere_r = 1.0f;
+                uint32_t tri_flags = 0;
+                float4 tq0, tq1, tq2;
+                if (type == CRT_PRIM_TRI) {
+                    const float4* tp = (const float4*)sc->tris + 3 * idx;
+                    tq0 = tp[0]; tq1 = tp[1]; tq2 = tp[2];
+                    mat_id = fbits(tq1.w);
+                    tri_flags = fbits(tq2.w);
+                    // flat normal = unit(cross(v0 - v1, v0 - v2)) (core/src/primitives/triangle.cuh:118)
+                    V3 a = v3(tq0), b = v3(tq1), c = v3(tq2);
+                    V3 nrm = unit(cross(a - b, a - c));
+                    if (tri_flags & CRT_TF_SMOOTH) {
+                        const float4* tn = (const float4*)sc->tri_normals + 3 * idx;
+                        // u weights v1, v weights v2 (triangle.cuh:167-174), not renormalised
+                        nrm = hv.z * v3(tn[1]) + hv.w * v3(tn[2]) + (1.0f - hv.z - hv.w) * v3(tn[0]);
+                    }
+                    if (inst != CRT_NO_INST) {
+                        // object -> world normal: transpose of the world -> object 3x3
+                        const float4* ip = (const float4*)sc->insts + 4 * inst;
+                        float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
+                        int ovr = (int)fbits(ip[3].z);
+                        if (ovr >= 0) mat_id = (uint32_t)ovr;
+                        V3 nw = v3(r0.x * nrm.x + r1.x * nrm.y + r2.x * nrm.z, r0.y * nrm.x + r1.y * nrm.y + r2.y * nrm.z,
+                                   r0.z * nrm.x + r1.z * nrm.y + r2.z * nrm.z);
+                        nrm = (tri_flags & CRT_TF_SMOOTH) ? nw : unit(nw);
+                    }
+                    outward = nrm;
+                } else if (type == CRT_PRIM_SPHERE) {
+                    float4 s = ((const float4*)sc->spheres)[idx];
+                    uint4 aux = ((const uint4*)sc->sphere_aux)[idx];
+                    mat_id = aux.y;
+                    sphere_c = v3(s); sphere_r = s.w;
+                    outward = (p - sphere_c) / sphere_r;
+                    need_uv_sphere = true;
+                } else {
+                    const float4* q = (const float4*)sc->quads + 5 * idx;
+                    float4 q1 = q[1], q3 = q[3], q4 = q[4];
+                    mat_id = fbits(q4.w);
+                    outward = v3(q3);
+                    is_light_quad = (fbits(q1.w) & CRT_QF_LIGHT) != 0;
+                }
+                bool front = dot(d, outward) < 0.0f;
+                V3 nrm = front ? outward : -outward;
+                const float4* mats = (const float4*)sc->mats;
+                Mat m;
+                m.m0 = mats[2 * mat_id];
+                m.m1 = *(const int4*)(mats + 2 * mat_id + 1);
+                if (m.m1.x == CRT_MAT_DIFFUSE_LIGHT) {
+                    // emitter: light does not scatter (material::scatter returns false)
+                    if (!passthrough) {
+                        bool one_sided = (m.m1.z & (int)CRT_MATF_ONE_SIDED) != 0 || is_light_quad;
+                        if (!one_sided || front) L = L + att * (v3(m.m0) * m.m0.w);
+                    }
+                } else {
+                    // albedo from texture (lambertian/metal use the diffuse texture)
+                    V3 alb = v3(m.m0);
+                    if (m.m1.y >= 0) {
+                        if (need_uv_sphere) {
+                            // get_sphere_uv on the outward normal (core/src/utilities/uvmapping.cuh:204-228)
+                            const float2* mp = (const float2*)sc->sphere_map + 3 * idx;
+                            float2 scl = mp[0], ofs = mp[1], rep = mp[2];
+                            float theta = acosf(-outward.y);
+                            float phi = atan2f(-outward.z, outward.x) + 3.14159265358979323846f;
+                            float s_ = phi / (2.0f * 3.14159265358979323846f);
+                            float t_ = theta / 3.14159265358979323846f;
+                            s_ = fmodf(s_ * rep.x, 1.0f);
+                            t_ = fmodf(t_ * rep.y, 1.0f);
+                            tu = scl.x * s_ + ofs.x;
+                            tvv = scl.y * t_ + ofs.y;
+                        } else if (type == CRT_PRIM_TRI) {
+                            tu = 0.0f; tvv = 0.0f;
+                            if (tri_flags & CRT_TF_UV) {
+                                const float2* uvp = (const float2*)sc->tri_uvs + 3 * idx;
+                                float2 a = uvp[0], b = uvp[1], c = uvp[2];
+                                float w0 = 1.0f - hv.z - hv.w;
+                                tu = w0 * a.x + hv.z * b.x + hv.w * c.x;
+                                tvv = w0 * a.y + hv.z * b.y + hv.w * c.y;
+                                tu = fmodf(tu, 1.0f); tvv = fmodf(tvv, 1.0f);
+                                if (tu < 0.0f) tu += 1.0f;
+                                if (tvv < 0.0f) tvv += 1.0f;
+                            }
+                        }
+                        float4 tx = tex_value((const float4*)sc->texs, sc->texels, m.m1.y, tu, tvv, p);
+                        alb = v3(tx);
+                    }
+                    if (m.m1.x == CRT_MAT_METAL) {
+                        // metal::scatter (core/src/materials/metal.cuh:57-65): never absorbs
+                        V3 ud = unit(d);
+                        V3 refl = ud - (nrm * dot(nrm, ud)) * 2.0f;
+                        float x, y, z;
+                        do {
+                            float a = g.uniform(), b = g.uniform(), c = g.uniform();
+                            x = 2.0f * a - 1.0f; y = 2.0f * b - 1.0f; z = 2.0f * c - 1.0f;
+                        } while (x * x + y * y + z * z >= 1.0f);
+                        d = refl + m.m0.w * v3(x, y, z);
+                        att = att * alb;
+                        done = false;
+                    } else if (m.m1.x == CRT_MAT_DIELECTRIC) {
+                        // dielectric::scatter (core/src/materials/dielectric.cuh:32-51)
+                        float ir = m.m0.w;
+                        float ratio = front ? (1.0f / ir) : ir;
+                        V3 ud = unit(d);
+                        float dn = dot(-ud, nrm);
+                        float cos_t = dn < 1.0f ? dn : 1.0f;
+                        float sin_t = sqrtf(1.0f - cos_t * cos_t);
+                        bool reflect_ray = ratio * sin_t > 1.0f;
+                        if (!reflect_ray) {
+                            float r0 = (1.0f - ratio) / (1.0f + ratio);
+                            r0 = r0 * r0;
+                            float x = 1.0f - cos_t;
+                            float x5 = powf(x, 5.0f);
+                            reflect_ray = (r0 + (1.0f - r0) * x5) > g.uniform();
+                        }
+                        if (reflect_ray) {
+                            d = ud - (nrm * dot(nrm, ud)) * 2.0f;
+                        } else {
+                            float dd = dot(nrm, ud);
+                            float kk = 1.0f - ratio * ratio * (1.0f - dd * dd);
+                            d = kk >= 0.0f ? (ratio * ud - (ratio * dd + sqrtf(kk)) * nrm) : v3(0.0f, 0.0f, 0.0f);
+                        }
+                        done = false;
+                    } else if (m.m1.x == CRT_MAT_LAMBERTIAN) {
+                        // mixture of light sampling and cosine sampling (camera.cuh:203-216)
+                        // onb::build_from_w (core/src/misc/onb.cuh:122-128)
+                        V3 w = unit(nrm);
+                        V3 aa = fabsf(w.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+                        V3 bv = unit(cross(w, aa));
+                        V3 bu = cross(w, bv);
+                        const uint32_t nl = sc->num_lights;
+                        const float4* qd = (const float4*)sc->quads;
+                        V3 dir;
+                        if (g.uniform() < 0.5f) {
+                            // hittable_list::random -> light 0 (get_int quirk) -> directional_light::random
+                            if (nl == 0) {
+                                dir = v3(0.0f, 0.0f, 0.0f);
+                            } else {
+                                if (nl > 1) g.skip();
+                                const float4* q = qd + 5 * sc->light_quads[0];
+                                float a = g.uniform(), b = g.uniform();
+                                V3 pp = v3(q[0]) + (a - 0.5f) * v3(q[1]) + (b - 0.5f) * v3(q[2]);
+                                dir = pp - p;
+                            }
+                        } else {
+                            // random_cosine_direction (core/src/misc/gpu_randomizer.cuh:122-133)
+                            float r1 = g.uniform(), r2 = g.uniform();
+                            float phi = 6.28318530718f * r1;
+                            float z = sqrtf(1.0f - r2);
+                            float rs = sqrtf(r2);
+                            float x = cosf(phi) * rs, y = sinf(phi) * rs;
+                            dir = x * bu + y * bv + z * w;
+                        }
+                        // light pdf (hittable_list::pdf_value -> directional_light::pdf_value)
+                        float weight = 1.0f / (float)nl;
+                        float sum = 0.0f;
+                        for (uint32_t li = 0; li < nl; ++li) {
+                            const float4* q = qd + 5 * sc->light_quads[li];
+                            float tl, qa, qb;
+                            light_tests++;
+                            float pv = 0.0f;
+                            if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb)) {
+                                float dist2 = tl * tl * len2(dir);
+                                float cosine = fabsf(dot(dir, v3(q[3])) / vlen(dir));
+                                pv = dist2 / (cosine * q[2].w);
+                            }
+                            sum += weight * pv;
+                        }
+                        V3 ud = unit(dir);
+                        float cv = dot(ud, w) / 3.14159265358979323846f;
+                        float pc = 0.0f > cv ? 0.0f : cv;
+                        float pdf = 0.5f * sum + (1.0f - 0.5f) * pc;
+                        float ct = dot(nrm, ud);
+                        float spdf = ct < 0.0f ? 0.0f : ct / 3.14159265358979323846f;
+                        att = att * ((1.0f / pdf) * (alb * spdf));
+                        d = dir;
+                        done = false;
+                    }
+                }
+                if (!done) {
+                    o = p;
+                    depth -= 1u;
+                    if (depth == 0u) done = true;
+                }
+            }
+            if (!done) {
+                uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8);
+                ray_o[slot] = make_float4(o.x, o.y, o.z, ro.w);
+                ray_d[slot] = make_float4(d.x, d.y, d.z, bitsf(ns));
+                thr[slot] = make_float4(att.x, att.y, att.z, bitsf(g.ms));
+                rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+                keep = true;
+            } else {
+                rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+                if (MODE == CRT_RNG_MINSTD) R.rng_state[idv.x] = g.ms;
+            }
+        }
+        uint32_t pos = block_append<BLOCK>(keep, count_out, s_wave, &s_base);
+        if (keep) queue_out[pos] = slot;
+    }
+    if (light_tests) atomicAdd(&counters->light_tests, (unsigned long long)light_tests);
+}
+
+// Per-pixel sums in sample order: accum += rad[sample 0] + ... (same float additions as the
+// reference's pixel_color += ray_color(...) loop, core/src/gpu.cu:776-790).
+__global__ __launch_bounds__(256) void k_resolve(const crt_dev_request* __restrict__ reqs, int nreq,
+                                                 const float4* __restrict__ rad) {
+    int r = blockIdx.y;
+    if (r >= nreq) return;
+    const crt_dev_request& R = reqs[r];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < R.pix_count; q += gridDim.x * blockDim.x) {
+        uint32_t pix = R.pix_begin + q;
+        float4* a = (float4*)R.accum + pix;
+        float4 acc = *a;
+        for (uint32_t k = 0; k < R.sample_count; ++k) {
+            float4 v = rad[R.path_base + k * R.pix_count + q];
+            acc.x += v.x; acc.y += v.y; acc.z += v.z;
+        }
+        acc.w += (float)R.sample_count;
+        *a = acc;
+    }
+}
+
+// prepare_pixel_color + quantize (core/src/misc/color.cuh:289-309, core/src/gpu.cu:792-799).
+__global__ __launch_bounds__(256) void k_tonemap(const float4* __restrict__ accum, uint32_t npix, uint32_t spp,
+                                                 uint8_t* __restrict__ rgb) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float4 a = accum[p];
+    float c[3] = {a.x, a.y, a.z};
+    for (int k = 0; k < 3; ++k) {
+        float x = c[k];
+        x = (x != x) ? 0.0f : x;
+        if (spp > 0u) { float scale = 1.0f / (float)spp; x *= scale; }
+        x = sqrtf(x);
+        float cl = x < 0.0f ? 0.0f : (x > 0.999f ? 0.999f : x);
+        rgb[3 * p + k] = (uint8_t)(int)(255.99f * cl);
+    }
+}
+
+// Start of a request: zero its accumulation buffer and seed the per-pixel minstd engines with
+// seed + reference pixel index (core/src/gpu.cu:770), Thrust's seed() rule (s mod m, 0 -> 1).
+__global__ __launch_bounds__(256) void k_init_request(float4* __restrict__ accum, uint32_t* __restrict__ rng_state,
+                                                      uint32_t W, uint32_t H, uint64_t seed) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= W * H) return;
+    accum[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (rng_state) {
+        uint32_t jj = p / W, i = p - jj * W;
+        uint32_t ref_pixel = (H - 1u - jj) * W + i;
+        uint64_t s = (seed + ref_pixel) & 0xffffffffull;
+        uint32_t x = (uint32_t)(s % 2147483647ull);
+        rng_state[p] = x == 0u ? 1u : x;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ======================================================================================== launchers
+#define STK_RS 4
+#define STK_LS 16
+#define STK_GS 96
+
+extern "C" int crt_k_generate(hipStream_t s, int rng_mode, const crt_dev_request* reqs, int nreq, uint32_t npaths,
+                              float4* ray_o, float4* ray_d, float4* thr, float4* rad, uint2* ids) {
+    if (npaths == 0) return 0;
+    dim3 grid((npaths + 255) / 256);
+    if (rng_mode == CRT_RNG_MINSTD)
+        hipLaunchKernelGGL(k_generate<CRT_RNG_MINSTD>, grid, dim3(256), 0, s, reqs, nreq, npaths, ray_o, ray_d, thr, rad, ids);
+    else
+        hipLaunchKernelGGL(k_generate<CRT_RNG_PCG>, grid, dim3(256), 0, s, reqs, nreq, npaths, ray_o, ray_d, thr, rad, ids);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_extend(hipStream_t s, int count_mode, uint32_t grid, const crt_dev_scene* scenes,
+                            const crt_dev_request* reqs, const uint32_t* queue, const uint32_t* count_ptr,
+                            uint32_t count_direct, const float4* ray_o, const float4* ray_d, const uint2* ids,
+                            float4* hit, uint32_t* hit_inst, uint32_t* gstack, uint32_t* zero_counter,
+                            crt_dev_counters* counters) {
+    if (count_mode)
+        hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, true>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue,
+                           count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters);
+    else
+        hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, false>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue,
+                           count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes,
+                           const crt_dev_request* reqs, const uint32_t* queue_in, const uint32_t* count_in,
+                           uint32_t count_direct, uint32_t* queue_out, uint32_t* count_out, float4* ray_o, float4* ray_d,
+                           float4* thr, float4* rad, const uint2* ids, const float4* hit, const uint32_t* hit_inst,
+                           crt_dev_counters* counters) {
+    if (rng_mode == CRT_RNG_MINSTD)
+        hipLaunchKernelGGL((k_shade<CRT_RNG_MINSTD, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue_in, count_in,
+                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, counters);
+    else
+        hipLaunchKernelGGL((k_shade<CRT_RNG_PCG, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue_in, count_in,
+                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, counters);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_trace_rays(hipStream_t s, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n,
+                                int camera, float4* hit, uint32_t* hit_inst, uint32_t* gstack, crt_dev_counters* counters) {
+    hipLaunchKernelGGL((k_trace_rays<256, STK_RS, STK_LS, STK_GS>), dim3(grid), dim3(256), 0, s, sc, rays, n, camera, hit,
+                       hit_inst, gstack, counters);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_resolve(hipStream_t s, const crt_dev_request* reqs, int nreq, uint32_t max_pix, const float4* rad) {
+    if (nreq == 0) return 0;
+    uint32_t gx = (max_pix + 255) / 256;
+    if (gx > 4096) gx = 4096;
+    if (gx == 0) gx = 1;
+    hipLaunchKernelGGL(k_resolve, dim3(gx, nreq), dim3(256), 0, s, reqs, nreq, rad);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_tonemap(hipStream_t s, const float4* accum, uint32_t npix, uint32_t spp, uint8_t* rgb) {
+    if (npix == 0) return 0;
+    hipLaunchKernelGGL(k_tonemap, dim3((npix + 255) / 256), dim3(256), 0, s, accum, npix, spp, rgb);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_init_request(hipStream_t s, float4* accum, uint32_t* rng_state, uint32_t W, uint32_t H, uint64_t seed) {
+    uint32_t n = W * H;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_init_request, dim3((n + 255) / 256), dim3(256), 0, s, accum, rng_state, W, H, seed);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return (int)hipGetLastError();
+}
+
+extern "C" uint32_t crt_k_stack_capacity(void) { return STK_RS + STK_LS + STK_GS; }
+extern "C" uint32_t crt_k_gstack_entries(void) { return STK_GS; }

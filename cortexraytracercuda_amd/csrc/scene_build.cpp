@@ -1,0 +1,672 @@
+// scene_build.cpp - host flattening of a crt_scene_desc and the binned-SAH BVH2 builder.
+//
+// The reference builds its scene on the GPU with one thread (load_scene<<<1,1>>>,
+// core/src/gpu.cu:481-751) as a graph of heap objects behind virtual calls, and its BVH by a
+// random-axis O(n^2) sort with a count-median split (core/src/misc/bvh_node.cuh:63-108). Here the
+// host resolves every index once, builds a binned-SAH tree with worker threads and emits SoA
+// records in leaf order so a leaf is a contiguous run of float4 loads on the device.
+#include "scene_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <thread>
+
+#include "crt_layout.h"
+
+namespace crt {
+
+namespace {
+
+inline float fb(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+struct V { float x, y, z; };
+inline V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V vadd(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V vmul(float s, V a) { return {s * a.x, s * a.y, s * a.z}; }
+inline float vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V vcross(V a, V b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+inline V vload(const float* p) { return {p[0], p[1], p[2]}; }
+
+struct Box {
+    float lo[3], hi[3];
+    void empty() { for (int a = 0; a < 3; ++a) { lo[a] = 3.4e38f; hi[a] = -3.4e38f; } }
+    void grow(const float* p) { for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); } }
+    void grow(V p) { float q[3] = {p.x, p.y, p.z}; grow(q); }
+    void grow(const Box& b) { for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], b.lo[a]); hi[a] = std::max(hi[a], b.hi[a]); } }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0 || dy < 0 || dz < 0) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Ref {
+    Box box;
+    float c[3];
+    uint32_t type, idx;
+};
+
+struct BNode {
+    Box box;
+    int32_t l = -1, r = -1;
+    uint32_t first = 0, count = 0;  // leaf range in the ref array
+    int32_t type = -1;              // leaf primitive type, -1 interior
+};
+
+class Builder {
+public:
+    Builder(std::vector<Ref>& refs, uint32_t leaf_size, int threads)
+        : refs_(refs), leaf_size_(std::max(1u, std::min(leaf_size, 64u))), threads_(std::max(1, threads)) {
+        nodes_.resize(2 * refs.size() + 2);
+        next_.store(0);
+        active_.store(1);
+    }
+    int32_t build() {
+        if (refs_.empty()) return -1;
+        return build_rec(0, (uint32_t)refs_.size(), 0);
+    }
+    std::vector<BNode>& nodes() { return nodes_; }
+
+private:
+    static constexpr int NB = 32;
+    std::vector<Ref>& refs_;
+    std::vector<BNode> nodes_;
+    std::atomic<int32_t> next_;
+    std::atomic<int> active_;
+    uint32_t leaf_size_;
+    int threads_;
+
+    int32_t alloc() { return next_.fetch_add(1); }
+
+    int32_t make_leaf(uint32_t b, uint32_t e, const Box& box) {
+        int32_t id = alloc();
+        BNode& n = nodes_[id];
+        n.box = box; n.first = b; n.count = e - b; n.type = (int32_t)refs_[b].type;
+        return id;
+    }
+
+    int32_t build_rec(uint32_t b, uint32_t e, int depth) {
+        Box box, cbox;
+        box.empty(); cbox.empty();
+        uint32_t type0 = refs_[b].type;
+        bool homog = true, has_inst = false;
+        for (uint32_t i = b; i < e; ++i) {
+            box.grow(refs_[i].box);
+            cbox.grow(refs_[i].c);
+            homog = homog && refs_[i].type == type0;
+            has_inst = has_inst || refs_[i].type == CRT_PRIM_INST;
+        }
+        uint32_t n = e - b;
+        if (homog && (n == 1 || (!has_inst && n <= leaf_size_))) return make_leaf(b, e, box);
+        uint32_t mid = b;
+        if (n <= leaf_size_ && !homog) {
+            // small mixed node: split by primitive type so every leaf is homogeneous
+            std::stable_sort(refs_.begin() + b, refs_.begin() + e, [](const Ref& x, const Ref& y) { return x.type < y.type; });
+            mid = b + 1;
+            while (mid < e && refs_[mid].type == refs_[b].type) ++mid;
+        } else {
+            mid = sah_split(b, e, box, cbox, depth);
+        }
+        int32_t id = alloc();
+        int32_t L, R;
+        if (n > 65536 && active_.load() < threads_) {
+            active_.fetch_add(1);
+            std::thread t([&]() { L = build_rec(b, mid, depth + 1); });
+            R = build_rec(mid, e, depth + 1);
+            t.join();
+            active_.fetch_sub(1);
+        } else {
+            L = build_rec(b, mid, depth + 1);
+            R = build_rec(mid, e, depth + 1);
+        }
+        BNode& nd = nodes_[id];
+        nd.box = box; nd.l = L; nd.r = R; nd.type = -1;
+        return id;
+    }
+
+    // Binned SAH over centroid bounds, 32 bins on each axis; falls back to an object-median
+    // split when the centroids coincide, the best split is one-sided, or the tree gets deep.
+    uint32_t sah_split(uint32_t b, uint32_t e, const Box& box, const Box& cbox, int depth) {
+        uint32_t n = e - b;
+        int best_axis = -1, best_bin = -1;
+        float best_cost = 3.4e38f;
+        if (depth < 48) {
+            for (int a = 0; a < 3; ++a) {
+                float ext = cbox.hi[a] - cbox.lo[a];
+                if (!(ext > 0.0f)) continue;
+                float scale = (float)NB * (1.0f - 1e-6f) / ext;
+                uint32_t cnt[NB] = {0};
+                Box bb[NB];
+                for (int k = 0; k < NB; ++k) bb[k].empty();
+                for (uint32_t i = b; i < e; ++i) {
+                    int k = (int)((refs_[i].c[a] - cbox.lo[a]) * scale);
+                    k = std::min(std::max(k, 0), NB - 1);
+                    cnt[k]++;
+                    bb[k].grow(refs_[i].box);
+                }
+                float rarea[NB];
+                uint32_t rcnt[NB];
+                Box acc; acc.empty();
+                uint32_t c = 0;
+                for (int k = NB - 1; k > 0; --k) {
+                    acc.grow(bb[k]); c += cnt[k];
+                    rarea[k] = acc.area(); rcnt[k] = c;
+                }
+                acc.empty(); c = 0;
+                for (int k = 0; k < NB - 1; ++k) {
+                    acc.grow(bb[k]); c += cnt[k];
+                    if (c == 0 || rcnt[k + 1] == 0) continue;
+                    float cost = acc.area() * (float)c + rarea[k + 1] * (float)rcnt[k + 1];
+                    if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = k; }
+                }
+            }
+        }
+        if (best_axis >= 0) {
+            int a = best_axis;
+            float ext = cbox.hi[a] - cbox.lo[a];
+            float scale = (float)NB * (1.0f - 1e-6f) / ext;
+            float lo = cbox.lo[a];
+            auto it = std::partition(refs_.begin() + b, refs_.begin() + e, [&](const Ref& r) {
+                int k = (int)((r.c[a] - lo) * scale);
+                k = std::min(std::max(k, 0), NB - 1);
+                return k <= best_bin;
+            });
+            uint32_t mid = (uint32_t)(it - refs_.begin());
+            if (mid > b && mid < e) return mid;
+        }
+        // object median on the largest centroid extent
+        int a = 0;
+        float ex = -1.0f;
+        for (int k = 0; k < 3; ++k) { float x = cbox.hi[k] - cbox.lo[k]; if (x > ex) { ex = x; a = k; } }
+        uint32_t mid = b + n / 2;
+        std::nth_element(refs_.begin() + b, refs_.begin() + mid, refs_.begin() + e,
+                         [a](const Ref& x, const Ref& y) { return x.c[a] < y.c[a]; });
+        return mid;
+    }
+};
+
+// Emits one built tree into the device arrays. Returns the root node index (always interior).
+struct Emitter {
+    HostScene& S;
+    const std::vector<BNode>& nodes;
+    const std::vector<Ref>& refs;
+    // per-type emitters of one primitive record, given the source index
+    std::function<void(uint32_t)> emit_tri, emit_sphere, emit_quad, emit_inst;
+    uint32_t max_depth = 0;
+    double sah = 0.0;
+    float root_area = 1.0f;
+
+    int32_t leaf_ref(const BNode& n) {
+        uint32_t first = 0;
+        uint32_t type = (uint32_t)n.type;
+        if (type == CRT_PRIM_TRI) first = (uint32_t)(S.tris.size() / 3);
+        else if (type == CRT_PRIM_SPHERE) first = (uint32_t)S.spheres.size();
+        else if (type == CRT_PRIM_QUAD) first = (uint32_t)(S.quads.size() / 5);
+        else first = (uint32_t)(S.insts.size() / 4);
+        for (uint32_t i = 0; i < n.count; ++i) {
+            const Ref& r = refs[n.first + i];
+            if (type == CRT_PRIM_TRI) emit_tri(r.idx);
+            else if (type == CRT_PRIM_SPHERE) emit_sphere(r.idx);
+            else if (type == CRT_PRIM_QUAD) emit_quad(r.idx);
+            else emit_inst(r.idx);
+        }
+        uint32_t li = (uint32_t)S.leaves.size();
+        S.leaves.push_back(U2{first, (type << CRT_LEAF_TYPE_SHIFT) | n.count});
+        sah += (double)n.box.area() / root_area * (double)n.count;
+        return ~(int32_t)li;
+    }
+
+    void write_node(uint32_t slot, const Box& b0, const Box& b1, int32_t c0, int32_t c1) {
+        F4* np = &S.nodes[4 * (size_t)slot];
+        np[0] = F4{b0.lo[0], b0.hi[0], b0.lo[1], b0.hi[1]};
+        np[1] = F4{b1.lo[0], b1.hi[0], b1.lo[1], b1.hi[1]};
+        np[2] = F4{b0.lo[2], b0.hi[2], b1.lo[2], b1.hi[2]};
+        uint32_t u0 = (uint32_t)c0, u1 = (uint32_t)c1;
+        np[3] = F4{fb(u0), fb(u1), 0.0f, 0.0f};
+    }
+
+    int32_t emit(int32_t id, uint32_t depth) {
+        const BNode& n = nodes[id];
+        if (n.type >= 0) return leaf_ref(n);
+        max_depth = std::max(max_depth, depth + 1);
+        uint32_t slot = (uint32_t)(S.nodes.size() / 4);
+        S.nodes.resize(S.nodes.size() + 4);
+        sah += (double)n.box.area() / root_area;
+        int32_t L = emit(n.l, depth + 1);
+        int32_t R = emit(n.r, depth + 1);
+        write_node(slot, nodes[n.l].box, nodes[n.r].box, L, R);
+        return (int32_t)slot;
+    }
+
+    uint32_t emit_root(int32_t root) {
+        if (root < 0) {
+            // empty tree: one node whose two children are empty leaves with zero-size boxes
+            uint32_t slot = (uint32_t)(S.nodes.size() / 4);
+            S.nodes.resize(S.nodes.size() + 4);
+            uint32_t li = (uint32_t)S.leaves.size();
+            S.leaves.push_back(U2{0u, (CRT_PRIM_SPHERE << CRT_LEAF_TYPE_SHIFT) | 0u});
+            Box z; for (int a = 0; a < 3; ++a) { z.lo[a] = 0.0f; z.hi[a] = 0.0f; }
+            write_node(slot, z, z, ~(int32_t)li, ~(int32_t)li);
+            max_depth = std::max(max_depth, 1u);
+            return slot;
+        }
+        root_area = std::max(nodes[root].box.area(), 1e-30f);
+        if (nodes[root].type >= 0) {
+            // a leaf root is wrapped so traversal always starts at an interior node
+            uint32_t slot = (uint32_t)(S.nodes.size() / 4);
+            S.nodes.resize(S.nodes.size() + 4);
+            int32_t L = leaf_ref(nodes[root]);
+            write_node(slot, nodes[root].box, nodes[root].box, L, L);
+            max_depth = std::max(max_depth, 1u);
+            return slot;
+        }
+        return (uint32_t)emit(root, 0);
+    }
+};
+
+Ref make_ref(const Box& b, uint32_t type, uint32_t idx) {
+    Ref r;
+    r.box = b;
+    for (int a = 0; a < 3; ++a) r.c[a] = 0.5f * (b.lo[a] + b.hi[a]);
+    r.type = type; r.idx = idx;
+    return r;
+}
+
+bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+}  // namespace
+
+uint64_t HostScene::device_bytes() const {
+    return nodes.size() * 16 + leaves.size() * 8 + tris.size() * 16 + tri_normals.size() * 16 + tri_uvs.size() * 8 +
+           spheres.size() * 16 + sphere_aux.size() * 16 + sphere_map.size() * 8 + quads.size() * 16 + insts.size() * 16 +
+           mats.size() * 16 + texs.size() * 16 + texels.size() + light_quads.size() * 4;
+}
+
+int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, HostScene& S, std::string& err) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!d) { err = "null scene descriptor"; return CRT_ERR_INVALID_ARGUMENT; }
+    if (d->abi_version != CRT_ABI_VERSION) { err = "scene descriptor ABI version mismatch"; return CRT_ERR_INVALID_ARGUMENT; }
+    auto need = [&](bool ok, const char* what) { if (!ok) err = what; return ok; };
+    if (!need(d->num_materials == 0 || d->materials, "materials pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_textures == 0 || d->textures, "textures pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_lights == 0 || d->lights, "lights pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_spheres == 0 || d->spheres, "spheres pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_quads == 0 || d->quads, "quads pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_meshes == 0 || d->meshes, "meshes pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_instances == 0 || d->instances, "instances pointer is null")) return CRT_ERR_INVALID_SCENE;
+    S = HostScene();
+    S.camera = d->camera;
+    for (int k = 0; k < 3; ++k) S.background[k] = d->background[k];
+    const uint32_t NM = d->num_materials;
+
+    // ---------------------------------------------------------------- textures
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+        const crt_texture& t = d->textures[i];
+        F4 t0{0, 0, 0, 0};
+        uint32_t t1[4] = {t.type, 0, 0, 0};
+        if (t.type == CRT_TEX_SOLID) {
+            t0 = F4{t.color[0], t.color[1], t.color[2], 0.0f};
+        } else if (t.type == CRT_TEX_CHECKER) {
+            if (t.even < 0 || t.odd < 0 || (uint32_t)t.even >= d->num_textures || (uint32_t)t.odd >= d->num_textures ||
+                d->textures[t.even].type == CRT_TEX_CHECKER || d->textures[t.odd].type == CRT_TEX_CHECKER) {
+                err = "checker texture " + std::to_string(i) + " references an invalid or nested texture";
+                return CRT_ERR_INVALID_SCENE;
+            }
+            if (!(t.scale != 0.0f)) { err = "checker texture scale is zero"; return CRT_ERR_INVALID_SCENE; }
+            t0 = F4{0, 0, 0, 1.0f / t.scale};  // the reference stores 1/scale (checker_texture.cuh:23-26)
+            t1[1] = (uint32_t)t.even; t1[2] = (uint32_t)t.odd;
+        } else if (t.type == CRT_TEX_IMAGE) {
+            if (t.width && t.height && !t.pixels) { err = "image texture without pixels"; return CRT_ERR_INVALID_SCENE; }
+            if (t.channels != 3 && t.channels != 4) { err = "image texture channels must be 3 or 4"; return CRT_ERR_INVALID_SCENE; }
+            // 4-byte aligned start so RGBA texels are single dword loads
+            while (S.texels.size() % 4) S.texels.push_back(0);
+            uint32_t off = (uint32_t)S.texels.size();
+            size_t nb = (size_t)t.width * t.height * t.channels;
+            if (nb) S.texels.insert(S.texels.end(), t.pixels, t.pixels + nb);
+            t0 = F4{fb(off), fb(t.channels), 0.0f, 0.0f};
+            t1[1] = t.width; t1[2] = t.height;
+        } else {
+            err = "unknown texture type"; return CRT_ERR_INVALID_SCENE;
+        }
+        S.texs.push_back(t0);
+        S.texs.push_back(F4{fb(t1[0]), fb(t1[1]), fb(t1[2]), fb(t1[3])});
+    }
+
+    // ---------------------------------------------------------------- materials (+ one per area light)
+    for (uint32_t i = 0; i < NM; ++i) {
+        const crt_material& m = d->materials[i];
+        float param = 0.0f;
+        if (m.type == CRT_MAT_METAL) param = m.fuzz < 1.0f ? m.fuzz : 1.0f;  // metal ctor clamp (metal.cuh:52-55)
+        else if (m.type == CRT_MAT_DIELECTRIC) param = m.ior;
+        else if (m.type == CRT_MAT_DIFFUSE_LIGHT) param = m.intensity;
+        else if (m.type != CRT_MAT_LAMBERTIAN) { err = "unsupported material type " + std::to_string(m.type); return CRT_ERR_INVALID_SCENE; }
+        if (m.texture >= 0 && (uint32_t)m.texture >= d->num_textures) {
+            err = "material " + std::to_string(i) + " references missing texture"; return CRT_ERR_INVALID_SCENE;
+        }
+        S.mats.push_back(F4{m.albedo[0], m.albedo[1], m.albedo[2], param});
+        S.mats.push_back(F4{fb(m.type), fb((uint32_t)m.texture), fb(m.flags), 0.0f});
+    }
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
+        const crt_area_light& L = d->lights[i];
+        // directional_light owns diffuse_light(rgb, intensity, directional = true, invisible)
+        S.mats.push_back(F4{L.rgb[0], L.rgb[1], L.rgb[2], L.intensity});
+        S.mats.push_back(F4{fb((uint32_t)CRT_MAT_DIFFUSE_LIGHT), fb(0xffffffffu), fb(CRT_MATF_ONE_SIDED), 0.0f});
+    }
+
+    // ---------------------------------------------------------------- primitive refs (top level)
+    std::vector<Ref> refs;
+    // quads: lights first (world-list order), then quads
+    struct QuadSrc { V q, u, v; uint32_t flags, mat, gid; };
+    std::vector<QuadSrc> qsrc;
+    uint32_t gid = 0;
+    for (uint32_t i = 0; i < d->num_lights; ++i, ++gid) {
+        const crt_area_light& L = d->lights[i];
+        if (!finite3(L.center) || !finite3(L.u) || !finite3(L.v)) { err = "non-finite light geometry"; return CRT_ERR_INVALID_SCENE; }
+        QuadSrc s{vload(L.center), vload(L.u), vload(L.v), CRT_QF_LIGHT | CRT_QF_ONE_SIDED | (L.invisible ? CRT_QF_INVISIBLE : 0u),
+                  NM + i, gid};
+        qsrc.push_back(s);
+    }
+    std::vector<uint32_t> sphere_gid;
+    uint32_t sphere_gid0 = gid;
+    gid += d->num_spheres;
+    for (uint32_t i = 0; i < d->num_quads; ++i, ++gid) {
+        const crt_quad& Q = d->quads[i];
+        if (Q.material >= NM) { err = "quad " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
+        if (!finite3(Q.corner) || !finite3(Q.u) || !finite3(Q.v)) { err = "non-finite quad geometry"; return CRT_ERR_INVALID_SCENE; }
+        qsrc.push_back(QuadSrc{vload(Q.corner), vload(Q.u), vload(Q.v), 0u, Q.material, gid});
+    }
+    for (uint32_t i = 0; i < (uint32_t)qsrc.size(); ++i) {
+        const QuadSrc& q = qsrc[i];
+        Box b; b.empty();
+        V c0 = q.q;
+        if (q.flags & CRT_QF_LIGHT) c0 = vsub(q.q, vadd(vmul(0.5f, q.u), vmul(0.5f, q.v)));
+        b.grow(c0); b.grow(vadd(c0, q.u)); b.grow(vadd(c0, q.v)); b.grow(vadd(vadd(c0, q.u), q.v));
+        refs.push_back(make_ref(b, CRT_PRIM_QUAD, i));
+    }
+    for (uint32_t i = 0; i < d->num_spheres; ++i) {
+        const crt_sphere& s = d->spheres[i];
+        if (s.material >= NM) { err = "sphere " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
+        if (!finite3(s.center) || !std::isfinite(s.radius)) { err = "non-finite sphere"; return CRT_ERR_INVALID_SCENE; }
+        Box b;
+        float r = std::fabs(s.radius);
+        for (int a = 0; a < 3; ++a) { b.lo[a] = s.center[a] - r; b.hi[a] = s.center[a] + r; }
+        refs.push_back(make_ref(b, CRT_PRIM_SPHERE, i));
+    }
+    // meshes
+    for (uint32_t m = 0; m < d->num_meshes; ++m) {
+        const crt_mesh& M = d->meshes[m];
+        if (M.num_triangles && (!M.indices || !M.positions)) { err = "mesh without indices/positions"; return CRT_ERR_INVALID_SCENE; }
+        if (!M.tri_materials && M.material >= NM && M.num_triangles) { err = "mesh references missing material"; return CRT_ERR_INVALID_SCENE; }
+        for (uint64_t k = 0; k < (uint64_t)M.num_triangles * 3; ++k)
+            if (M.indices[k] >= M.num_vertices) { err = "mesh index out of range"; return CRT_ERR_INVALID_SCENE; }
+        if (M.tri_materials)
+            for (uint32_t t = 0; t < M.num_triangles; ++t)
+                if (M.tri_materials[t] >= NM) { err = "mesh triangle references missing material"; return CRT_ERR_INVALID_SCENE; }
+        for (uint64_t k = 0; k < (uint64_t)M.num_vertices; ++k)
+            if (!finite3(M.positions + 3 * k)) { err = "non-finite mesh vertex"; return CRT_ERR_INVALID_SCENE; }
+    }
+    // world triangles (non-instanced): a flat list of (mesh, triangle)
+    struct TriSrc { uint32_t mesh, tri, gid; };
+    std::vector<TriSrc> wtris;
+    if (d->num_instances == 0) {
+        for (uint32_t m = 0; m < d->num_meshes; ++m) {
+            const crt_mesh& M = d->meshes[m];
+            for (uint32_t t = 0; t < M.num_triangles; ++t) wtris.push_back(TriSrc{m, t, gid + t});
+            gid += M.num_triangles;
+        }
+        refs.reserve(refs.size() + wtris.size());
+        for (uint32_t i = 0; i < (uint32_t)wtris.size(); ++i) {
+            const crt_mesh& M = d->meshes[wtris[i].mesh];
+            const uint32_t* ix = M.indices + 3 * (size_t)wtris[i].tri;
+            Box b; b.empty();
+            for (int k = 0; k < 3; ++k) b.grow(M.positions + 3 * (size_t)ix[k]);
+            refs.push_back(make_ref(b, CRT_PRIM_TRI, i));
+        }
+    }
+    S.num_world_tris = wtris.size();
+
+    // record emitters ----------------------------------------------------------------
+    auto emit_triangle = [&](const crt_mesh& M, uint32_t t, uint32_t id_word, const float* xf) {
+        (void)xf;
+        const uint32_t* ix = M.indices + 3 * (size_t)t;
+        const float* p0 = M.positions + 3 * (size_t)ix[0];
+        const float* p1 = M.positions + 3 * (size_t)ix[1];
+        const float* p2 = M.positions + 3 * (size_t)ix[2];
+        uint32_t mat = M.tri_materials ? M.tri_materials[t] : M.material;
+        uint32_t flags = (M.normals ? CRT_TF_SMOOTH : 0u) | (M.uvs ? CRT_TF_UV : 0u);
+        size_t ti = S.tris.size() / 3;
+        S.tris.push_back(F4{p0[0], p0[1], p0[2], fb(id_word)});
+        S.tris.push_back(F4{p1[0], p1[1], p1[2], fb(mat)});
+        S.tris.push_back(F4{p2[0], p2[1], p2[2], fb(flags)});
+        if (M.normals || !S.tri_normals.empty()) {
+            if (S.tri_normals.size() < 3 * ti) S.tri_normals.resize(3 * ti, F4{0, 0, 0, 0});
+            for (int k = 0; k < 3; ++k) {
+                if (!M.normals) { S.tri_normals.push_back(F4{0, 0, 0, 0}); continue; }
+                V n = vload(M.normals + 3 * (size_t)ix[k]);
+                float l = std::sqrt(vdot(n, n));
+                // vert_normals = unit_vector(vn) (core/src/primitives/triangle.cuh:97-99)
+                S.tri_normals.push_back(F4{n.x / l, n.y / l, n.z / l, 0.0f});
+            }
+        }
+        if (M.uvs || !S.tri_uvs.empty()) {
+            if (S.tri_uvs.size() < 3 * ti) S.tri_uvs.resize(3 * ti, F2{0, 0});
+            for (int k = 0; k < 3; ++k)
+                S.tri_uvs.push_back(M.uvs ? F2{M.uvs[2 * (size_t)ix[k]], M.uvs[2 * (size_t)ix[k] + 1]} : F2{0, 0});
+        }
+    };
+
+    // ---------------------------------------------------------------- BLAS per instanced mesh
+    std::vector<uint32_t> blas_root(d->num_meshes, 0);
+    std::vector<Box> blas_box(d->num_meshes);
+    uint32_t blas_depth = 0;
+    S.num_instances = d->num_instances;
+    if (d->num_instances) {
+        std::vector<char> used(d->num_meshes, 0);
+        for (uint32_t k = 0; k < d->num_instances; ++k) {
+            const crt_instance& I = d->instances[k];
+            if (I.mesh >= d->num_meshes) { err = "instance references missing mesh"; return CRT_ERR_INVALID_SCENE; }
+            if (I.material_override >= 0 && (uint32_t)I.material_override >= NM) { err = "instance material out of range"; return CRT_ERR_INVALID_SCENE; }
+            used[I.mesh] = 1;
+        }
+        for (uint32_t m = 0; m < d->num_meshes; ++m) {
+            if (!used[m]) continue;
+            const crt_mesh& M = d->meshes[m];
+            std::vector<Ref> brefs;
+            brefs.reserve(M.num_triangles);
+            Box mb; mb.empty();
+            for (uint32_t t = 0; t < M.num_triangles; ++t) {
+                Box b; b.empty();
+                for (int k = 0; k < 3; ++k) b.grow(M.positions + 3 * (size_t)M.indices[3 * (size_t)t + k]);
+                mb.grow(b);
+                brefs.push_back(make_ref(b, CRT_PRIM_TRI, t));
+            }
+            blas_box[m] = mb;
+            Builder bb(brefs, leaf_size ? leaf_size : 4, threads);
+            int32_t r = bb.build();
+            Emitter em{S, bb.nodes(), brefs, nullptr, nullptr, nullptr, nullptr};
+            em.emit_tri = [&](uint32_t t) { emit_triangle(M, t, t, nullptr); };
+            blas_root[m] = em.emit_root(r);
+            blas_depth = std::max(blas_depth, em.max_depth);
+        }
+        // instance refs: world box of the transformed BLAS box corners
+        for (uint32_t k = 0; k < d->num_instances; ++k) {
+            const crt_instance& I = d->instances[k];
+            const float* x = I.object_to_world;
+            const Box& mb = blas_box[I.mesh];
+            Box wb; wb.empty();
+            for (int c = 0; c < 8; ++c) {
+                float p[3] = {(c & 1) ? mb.hi[0] : mb.lo[0], (c & 2) ? mb.hi[1] : mb.lo[1], (c & 4) ? mb.hi[2] : mb.lo[2]};
+                V w{x[0] * p[0] + x[1] * p[1] + x[2] * p[2] + x[3], x[4] * p[0] + x[5] * p[1] + x[6] * p[2] + x[7],
+                    x[8] * p[0] + x[9] * p[1] + x[10] * p[2] + x[11]};
+                wb.grow(w);
+            }
+            // conservative widening for the rounding of the object-space ray
+            for (int a = 0; a < 3; ++a) {
+                float e = 1e-6f * (std::fabs(wb.lo[a]) + std::fabs(wb.hi[a])) + 1e-7f;
+                wb.lo[a] -= e; wb.hi[a] += e;
+            }
+            refs.push_back(make_ref(wb, CRT_PRIM_INST, k));
+        }
+    }
+
+    // ---------------------------------------------------------------- top level
+    Builder tb(refs, leaf_size ? leaf_size : 4, threads);
+    int32_t troot = tb.build();
+    Emitter em{S, tb.nodes(), refs, nullptr, nullptr, nullptr, nullptr};
+    em.emit_tri = [&](uint32_t i) {
+        const TriSrc& s = wtris[i];
+        emit_triangle(d->meshes[s.mesh], s.tri, s.gid, nullptr);
+    };
+    em.emit_sphere = [&](uint32_t i) {
+        const crt_sphere& s = d->spheres[i];
+        S.spheres.push_back(F4{s.center[0], s.center[1], s.center[2], s.radius});
+        S.sphere_aux.push_back(U4{sphere_gid0 + i, s.material, 0u, 0u});
+        S.sphere_map.push_back(F2{s.mapping.scale_u, s.mapping.scale_v});
+        S.sphere_map.push_back(F2{s.mapping.offset_u, s.mapping.offset_v});
+        S.sphere_map.push_back(F2{s.mapping.repeat_u, s.mapping.repeat_v});
+    };
+    std::vector<uint32_t> quad_slot(qsrc.size(), 0);
+    em.emit_quad = [&](uint32_t i) {
+        const QuadSrc& q = qsrc[i];
+        // quad ctor (core/src/primitives/quad.cuh:66-79) / directional_light ctor (:45-58)
+        V n = vcross(q.u, q.v);
+        float nn = vdot(n, n);
+        float len = std::sqrt(nn);
+        V un{n.x / len, n.y / len, n.z / len};
+        float D = vdot(un, q.q);
+        V w{n.x / nn, n.y / nn, n.z / nn};
+        quad_slot[i] = (uint32_t)(S.quads.size() / 5);
+        S.quads.push_back(F4{q.q.x, q.q.y, q.q.z, D});
+        S.quads.push_back(F4{q.u.x, q.u.y, q.u.z, fb(q.flags)});
+        S.quads.push_back(F4{q.v.x, q.v.y, q.v.z, len});
+        S.quads.push_back(F4{un.x, un.y, un.z, fb(q.gid)});
+        S.quads.push_back(F4{w.x, w.y, w.z, fb(q.mat)});
+    };
+    em.emit_inst = [&](uint32_t k) {
+        const crt_instance& I = d->instances[k];
+        const float* x = I.object_to_world;
+        // world -> object = inverse of the affine [A|t]: [A^-1 | -A^-1 t]
+        double a[9] = {x[0], x[1], x[2], x[4], x[5], x[6], x[8], x[9], x[10]};
+        double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+        double inv[9] = {(a[4] * a[8] - a[5] * a[7]) / det, (a[2] * a[7] - a[1] * a[8]) / det, (a[1] * a[5] - a[2] * a[4]) / det,
+                         (a[5] * a[6] - a[3] * a[8]) / det, (a[0] * a[8] - a[2] * a[6]) / det, (a[2] * a[3] - a[0] * a[5]) / det,
+                         (a[3] * a[7] - a[4] * a[6]) / det, (a[1] * a[6] - a[0] * a[7]) / det, (a[0] * a[4] - a[1] * a[3]) / det};
+        double tx = x[3], ty = x[7], tz = x[11];
+        for (int r = 0; r < 3; ++r) {
+            double tr = -(inv[3 * r] * tx + inv[3 * r + 1] * ty + inv[3 * r + 2] * tz);
+            S.insts.push_back(F4{(float)inv[3 * r], (float)inv[3 * r + 1], (float)inv[3 * r + 2], (float)tr});
+        }
+        S.insts.push_back(F4{fb(blas_root[I.mesh]), fb(k), fb((uint32_t)I.material_override), 0.0f});
+    };
+    S.root = em.emit_root(troot);
+    S.stats.sah_cost = em.sah;
+    S.stats.depth = em.max_depth + blas_depth;
+    // per-instance material overrides are resolved in the shade stage (instance record word z)
+    S.light_quads.resize(d->num_lights);
+    for (uint32_t i = 0; i < d->num_lights; ++i) S.light_quads[i] = quad_slot[i];
+    S.num_lights = d->num_lights;
+    S.num_spheres = d->num_spheres;
+    S.num_quads = d->num_quads;
+    if (!S.tri_normals.empty() && S.tri_normals.size() < S.tris.size()) S.tri_normals.resize(S.tris.size(), F4{0, 0, 0, 0});
+    if (!S.tri_uvs.empty() && S.tri_uvs.size() < S.tris.size()) S.tri_uvs.resize(S.tris.size(), F2{0, 0});
+    S.stats.nodes = S.nodes.size() / 4;
+    S.stats.leaves = S.leaves.size();
+    S.stats.prims = refs.size();
+    S.stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CRT_OK;
+}
+
+}  // namespace crt
+
+namespace crt {
+
+// Structural invariants of a built scene: every child box contains the primitives below it, every
+// primitive record belongs to exactly one leaf, every instance's BLAS root is an interior node.
+int check_host_scene(const HostScene& S, std::string& err) {
+    const size_t nn = S.nodes.size() / 4;
+    std::vector<uint32_t> cover_tri(S.tris.size() / 3, 0), cover_sph(S.spheres.size(), 0), cover_quad(S.quads.size() / 5, 0),
+        cover_inst(S.insts.size() / 4, 0);
+    auto u32 = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
+    struct B { float lo[3], hi[3]; };
+    auto inside = [](const B& in, const float* lo, const float* hi) {
+        for (int a = 0; a < 3; ++a) {
+            float tol = 1e-5f * (std::fabs(lo[a]) + std::fabs(hi[a]) + 1.0f);
+            if (in.lo[a] < lo[a] - tol || in.hi[a] > hi[a] + tol) return false;
+        }
+        return true;
+    };
+    std::vector<char> visited(nn, 0);
+    std::function<bool(int32_t, const float*, const float*, int)> walk = [&](int32_t ref, const float* lo, const float* hi,
+                                                                             int depth) -> bool {
+        if (depth > 256) { err = "BVH deeper than 256"; return false; }
+        if (ref >= 0) {
+            if ((size_t)ref >= nn) { err = "child node index out of range"; return false; }
+            if (visited[ref]++) { err = "node reached twice"; return false; }
+            const F4* np = &S.nodes[4 * (size_t)ref];
+            float l0[3] = {np[0].x, np[0].z, np[2].x}, h0[3] = {np[0].y, np[0].w, np[2].y};
+            float l1[3] = {np[1].x, np[1].z, np[2].z}, h1[3] = {np[1].y, np[1].w, np[2].w};
+            B b0{{l0[0], l0[1], l0[2]}, {h0[0], h0[1], h0[2]}}, b1{{l1[0], l1[1], l1[2]}, {h1[0], h1[1], h1[2]}};
+            if (lo && (!inside(b0, lo, hi) || !inside(b1, lo, hi))) { err = "child box outside parent box"; return false; }
+            int32_t c0 = (int32_t)u32(np[3].x), c1 = (int32_t)u32(np[3].y);
+            if (!walk(c0, l0, h0, depth + 1)) return false;
+            if (c1 == c0 && c0 < 0) return true;  // a wrapped leaf root lists its leaf twice
+            return walk(c1, l1, h1, depth + 1);
+        }
+        uint32_t li = (uint32_t)(~ref);
+        if (li >= S.leaves.size()) { err = "leaf index out of range"; return false; }
+        U2 lf = S.leaves[li];
+        uint32_t type = lf.y >> CRT_LEAF_TYPE_SHIFT, cnt = lf.y & CRT_LEAF_COUNT_MASK, first = lf.x;
+        for (uint32_t k = first; k < first + cnt; ++k) {
+            B pb;
+            if (type == CRT_PRIM_TRI) {
+                if (k >= cover_tri.size()) { err = "triangle leaf out of range"; return false; }
+                cover_tri[k]++;
+                const F4* t = &S.tris[3 * (size_t)k];
+                for (int a = 0; a < 3; ++a) {
+                    float v0 = a == 0 ? t[0].x : (a == 1 ? t[0].y : t[0].z);
+                    float v1 = a == 0 ? t[1].x : (a == 1 ? t[1].y : t[1].z);
+                    float v2 = a == 0 ? t[2].x : (a == 1 ? t[2].y : t[2].z);
+                    pb.lo[a] = std::min(v0, std::min(v1, v2)); pb.hi[a] = std::max(v0, std::max(v1, v2));
+                }
+            } else if (type == CRT_PRIM_SPHERE) {
+                if (k >= cover_sph.size()) { err = "sphere leaf out of range"; return false; }
+                cover_sph[k]++;
+                const F4& s = S.spheres[k];
+                float r = std::fabs(s.w);
+                pb = B{{s.x - r, s.y - r, s.z - r}, {s.x + r, s.y + r, s.z + r}};
+            } else if (type == CRT_PRIM_QUAD) {
+                if (k >= cover_quad.size()) { err = "quad leaf out of range"; return false; }
+                cover_quad[k]++;
+                continue;  // quad corner form depends on the light flag; covered by the GPU parity tests
+            } else {
+                if (k >= cover_inst.size()) { err = "instance leaf out of range"; return false; }
+                cover_inst[k]++;
+                int32_t br = (int32_t)u32(S.insts[4 * (size_t)k + 3].x);
+                if (br < 0 || (size_t)br >= nn) { err = "instance BLAS root is not an interior node"; return false; }
+                continue;
+            }
+            if (lo && !inside(pb, lo, hi)) { err = "primitive outside its leaf box"; return false; }
+        }
+        return true;
+    };
+    // BLAS roots first (each visited once from its own root), then the top level
+    std::vector<int32_t> blas;
+    for (size_t k = 0; k < S.insts.size() / 4; ++k) blas.push_back((int32_t)u32(S.insts[4 * k + 3].x));
+    std::sort(blas.begin(), blas.end());
+    blas.erase(std::unique(blas.begin(), blas.end()), blas.end());
+    for (int32_t b : blas) if (!walk(b, nullptr, nullptr, 0)) return CRT_ERR_INTERNAL;
+    if (!walk((int32_t)S.root, nullptr, nullptr, 0)) return CRT_ERR_INTERNAL;
+    auto once = [](const std::vector<uint32_t>& v) { for (uint32_t c : v) if (c != 1) return false; return true; };
+    if (!once(cover_tri) || !once(cover_sph) || !once(cover_quad) || !once(cover_inst)) {
+        err = "a primitive record is not in exactly one leaf";
+        return CRT_ERR_INTERNAL;
+    }
+    for (size_t k = 0; k < nn; ++k) if (!visited[k]) { err = "unreachable node"; return CRT_ERR_INTERNAL; }
+    return CRT_OK;
+}
+
+}  // namespace crt

@@ -1,0 +1,59 @@
+// scene_build.h - host-side flattening of a crt_scene_desc into the HBM layout of crt_layout.h,
+// including the binned-SAH BVH2 build (two levels when the scene has instances).
+//
+// Replaces the reference's device-side scene construction: load_scene<<<1,1>>>
+// (core/src/gpu.cu:481-751, object graph + name lookups at :132-474) and the single-thread BVH
+// (core/src/misc/bvh_node.cuh:63-108: random axis, O(n^2) sort, count-median split).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "cortex_rt.h"
+
+namespace crt {
+
+struct F4 { float x, y, z, w; };
+struct F2 { float x, y; };
+struct U2 { uint32_t x, y; };
+struct U4 { uint32_t x, y, z, w; };
+
+struct BuildStats {
+    uint64_t nodes = 0, leaves = 0, prims = 0;
+    uint32_t depth = 0;        // max interior depth of the top level (+ deepest BLAS when instanced)
+    double build_ms = 0.0;
+    double sah_cost = 0.0;     // SAH cost of the top-level tree (C_trav = 1, C_isect = 1)
+};
+
+struct HostScene {
+    std::vector<F4> nodes;        // 4 per interior node
+    std::vector<U2> leaves;
+    std::vector<F4> tris;         // 3 per triangle
+    std::vector<F4> tri_normals;  // 3 per triangle, only when a mesh has normals
+    std::vector<F2> tri_uvs;      // 3 per triangle, only when a mesh has uvs
+    std::vector<F4> spheres;
+    std::vector<U4> sphere_aux;
+    std::vector<F2> sphere_map;   // 3 per sphere: scale, offset, repeat
+    std::vector<F4> quads;        // 5 per quad / light
+    std::vector<F4> insts;        // 4 per instance
+    std::vector<F4> mats;         // 2 per material (scene materials, then one per area light)
+    std::vector<F4> texs;         // 2 per texture
+    std::vector<uint8_t> texels;
+    std::vector<uint32_t> light_quads;
+    uint32_t root = 0;
+    float background[3] = {0, 0, 0};
+    crt_camera camera{};
+    uint32_t num_lights = 0, num_spheres = 0, num_quads = 0, num_instances = 0;
+    uint64_t num_world_tris = 0;
+    BuildStats stats;
+    uint64_t device_bytes() const;
+};
+
+// Flattens and builds. leaf_size = max primitives per leaf (instances always get their own leaf).
+int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, HostScene& out, std::string& err);
+
+// Structural BVH invariants (containment, exact leaf coverage, reachability).
+int check_host_scene(const HostScene& S, std::string& err);
+
+}  // namespace crt

@@ -1,0 +1,210 @@
+"""Python front end of the C-ABI (ctypes over the in-tree libcortex_rt.so).
+
+Mirrors the reference's render interface: `Renderer.render` is the blocking render of one scene
+(launchGPU, core/src/gpu.cu:2127-2149), `launch_gpu` keeps launchGPU's argument list and writes the
+PNG, and the batched / serving calls expose the fused-pass scheduler.
+"""
+import ctypes as C
+import struct
+import zlib
+
+import numpy as np
+
+from . import _abi as A
+
+_RNG = {"pcg": A.CRT_RNG_PCG, "minstd": A.CRT_RNG_MINSTD}
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def stats_dict(st):
+    return {k: getattr(st, k) for k, _ in A.crt_stats._fields_}
+
+
+def render_desc(width, height, spp, max_depth, seed=1, rng="pcg", sample_begin=0, sample_count=0, flags=0):
+    rd = A.crt_render_desc()
+    rd.width, rd.height, rd.spp, rd.max_depth = int(width), int(height), int(spp), int(max_depth)
+    rd.seed = int(seed)
+    rd.rng_mode = _RNG[rng] if isinstance(rng, str) else int(rng)
+    rd.sample_begin, rd.sample_count, rd.flags = int(sample_begin), int(sample_count), int(flags)
+    return rd
+
+
+class Renderer:
+    """One device context: resident scenes, path-state buffers and the serving loop."""
+
+    def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0):
+        self.lib = A.load()
+        cfg = A.crt_config()
+        cfg.device = device
+        cfg.hbm_budget_bytes = int(hbm_budget)
+        cfg.max_paths = int(max_paths)
+        cfg.count_work = 1 if count_work else 0
+        cfg.leaf_size = int(leaf_size)
+        h = C.c_void_p()
+        A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.crt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- scenes
+    def upload(self, scene):
+        desc, keep = scene.to_desc()
+        sid = C.c_uint32()
+        A.check(self.lib.crt_scene_upload(self.h, C.byref(desc), C.byref(sid)))
+        del keep
+        return sid.value
+
+    def release(self, sid):
+        A.check(self.lib.crt_scene_release(self.h, sid))
+
+    def info(self, sid):
+        inf = A.crt_scene_info()
+        A.check(self.lib.crt_scene_get_info(self.h, sid, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in A.crt_scene_info._fields_}
+
+    # ---------------------------------------------------------------- rendering
+    def render(self, sid, width, height, spp, max_depth, seed=1, rng="pcg", sample_begin=0, sample_count=0,
+               want_accum=True, time_stages=False):
+        rd = render_desc(width, height, spp, max_depth, seed, rng, sample_begin, sample_count,
+                         A.CRT_RF_TIME_STAGES if time_stages else 0)
+        rgb = np.zeros((height, width, 3), np.uint8)
+        acc = np.zeros((height, width, 4), np.float32) if want_accum else None
+        img = A.crt_image(rgb.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(acc) if acc is not None else None)
+        st = A.crt_stats()
+        A.check(self.lib.crt_render(self.h, sid, C.byref(rd), C.byref(img), C.byref(st)))
+        return rgb, acc, stats_dict(st)
+
+    def render_batch(self, jobs, want_accum=True, time_stages=False):
+        """jobs: list of dicts (sid, width, height, spp, max_depth, seed, rng). Returns
+        ([(rgb, accum)], stats) with every request rendered in fused passes."""
+        n = len(jobs)
+        items = (A.crt_batch_item * n)()
+        outs = []
+        for k, j in enumerate(jobs):
+            W, H = j["width"], j["height"]
+            rgb = np.zeros((H, W, 3), np.uint8)
+            acc = np.zeros((H, W, 4), np.float32) if want_accum else None
+            items[k].scene = j["sid"]
+            items[k].rd = render_desc(W, H, j["spp"], j["max_depth"], j.get("seed", 1), j.get("rng", "pcg"),
+                                      flags=A.CRT_RF_TIME_STAGES if time_stages else 0)
+            items[k].out = A.crt_image(rgb.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(acc) if acc is not None else None)
+            outs.append((rgb, acc))
+        st = A.crt_stats()
+        A.check(self.lib.crt_render_batch(self.h, items, n, C.byref(st)))
+        return outs, stats_dict(st)
+
+    def trace_rays(self, sid, rays, camera=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        prim = np.zeros(n, np.int32)
+        inst = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        uv = np.zeros((n, 2), np.float32)
+        A.check(self.lib.crt_trace_rays(self.h, sid, _fp(rays), n, 1 if camera else 0, _ip(prim), _ip(inst), _fp(t),
+                                        _fp(uv)))
+        return prim, inst, t, uv
+
+    def trace_primary(self, sid, width, height):
+        prim = np.zeros((height, width), np.int32)
+        t = np.zeros((height, width), np.float32)
+        A.check(self.lib.crt_trace_primary(self.h, sid, width, height, _ip(prim), _fp(t)))
+        return prim, t
+
+    # ---------------------------------------------------------------- serving
+    def submit(self, sid, width, height, spp, max_depth, seed=1, rng="pcg"):
+        rd = render_desc(width, height, spp, max_depth, seed, rng)
+        rid = C.c_uint64()
+        A.check(self.lib.crt_submit(self.h, sid, C.byref(rd), C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid):
+        d = C.c_int32()
+        A.check(self.lib.crt_poll(self.h, rid, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid, width, height, want_accum=True):
+        rgb = np.zeros((height, width, 3), np.uint8)
+        acc = np.zeros((height, width, 4), np.float32) if want_accum else None
+        img = A.crt_image(rgb.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(acc) if acc is not None else None)
+        st = A.crt_stats()
+        A.check(self.lib.crt_wait(self.h, rid, C.byref(img), C.byref(st)))
+        return rgb, acc, stats_dict(st)
+
+    def cancel(self, rid):
+        A.check(self.lib.crt_cancel(self.h, rid))
+
+    def timing(self, rid):
+        t = A.crt_request_timing()
+        A.check(self.lib.crt_request_get_timing(self.h, rid, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in A.crt_request_timing._fields_}
+
+
+def write_png(path, rgb):
+    """RGB8 PNG writer (stbi_write_png replacement, core/src/gpu.cu:2110); rows top-down."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    h, w, _ = rgb.shape
+    raw = b"".join(b"\x00" + rgb[y].tobytes() for y in range(h))
+
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xffffffff)
+
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0))
+    png += chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b"")
+    with open(path, "wb") as f:
+        f.write(png)
+
+
+def pixel_lines(rgb):
+    """The reference's stdout pixel protocol 'p x y r g b' (core/src/gpu.cu:801): x left to right,
+    y counted from the top row of the written image as H - j - 1 with j from the bottom."""
+    h, w, _ = rgb.shape
+    out = []
+    for jj in range(h):
+        for i in range(w):
+            r, g, b = rgb[jj, i]
+            out.append("p %d %d %d %d %d" % (i, jj, r, g, b))
+    return out
+
+
+def launch_gpu(scene, width, height, spp, max_depth, tx=16, ty=16, filepath=None, quiet=True, seed=2580,
+               rng="minstd", renderer=None):
+    """launchGPU(sceneConfig, width, height, spp, max_depth, tx, ty, filepath, quietMode)
+    (core/src/renderers/gpu_cuda_renderer.h:17). tx/ty are accepted and ignored: the tile shape
+    is the pass scheduler's. Defaults reproduce the reference's RNG stream (minstd, seed 2580)."""
+    own = renderer is None
+    r = renderer or Renderer()
+    try:
+        sid = r.upload(scene)
+        rgb, _, st = r.render(sid, width, height, spp, max_depth, seed=seed, rng=rng, want_accum=False)
+        r.release(sid)
+    finally:
+        if own:
+            r.close()
+    if filepath:
+        write_png(filepath, rgb)
+    if not quiet:
+        for line in pixel_lines(rgb):
+            print(line)
+    return rgb, st

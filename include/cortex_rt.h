@@ -1,0 +1,294 @@
+/*
+ * cortex_rt.h - C-ABI of the MI355X (gfx950) render hot path.
+ *
+ * This is the drop-in boundary for the reference renderer's one render entry point
+ *
+ *   void launchGPU(const sceneConfig&, int width, int height, int spp, int max_depth,
+ *                  int tx, int ty, const char* filepath, bool quietMode);
+ *
+ * declared at core/src/renderers/gpu_cuda_renderer.h:17 and defined at core/src/gpu.cu:2127-2149.
+ * The reference takes a C++ object graph (sceneConfig, core/src/scenes/scene_config.h:452-460) whose
+ * materials and textures are linked by name and rebuilt on the device by one thread
+ * (load_scene<<<1,1>>>, core/src/gpu.cu:481-751). Here the caller hands over a flat, POD
+ * description (indices instead of names, uint32 counts instead of uint8 ones); the library copies
+ * it, builds the BVH on the host and keeps everything resident in HBM until the scene is released.
+ *
+ * Conventions
+ *  - Every function returns a crt_status; 0 is success. crt_last_error() gives the message of the
+ *    last failure on the calling thread.
+ *  - All pointers in the descriptors are host pointers; the library copies what it needs before
+ *    returning, so the caller may free its buffers as soon as the call returns.
+ *  - Images are row-major with row 0 at the TOP (the reference flips rows when it writes the PNG,
+ *    core/src/gpu.cu:2098-2107; the float accumulation buffer uses the same top-down order).
+ */
+#ifndef CORTEX_RT_H
+#define CORTEX_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRT_ABI_VERSION 1u
+
+/* ------------------------------------------------------------------ status codes */
+typedef enum crt_status {
+    CRT_OK = 0,
+    CRT_ERR_NO_DEVICE = 1,        /* reference: "No GPU found" print + return (core/src/gpu.cu:2131-2134) */
+    CRT_ERR_OUT_OF_MEMORY = 2,    /* hipMalloc failed */
+    CRT_ERR_OVER_BUDGET = 3,      /* the request does not fit the context's HBM budget */
+    CRT_ERR_INVALID_ARGUMENT = 4,
+    CRT_ERR_INVALID_SCENE = 5,    /* dangling material/texture/mesh index, NaN geometry, ... */
+    CRT_ERR_IO = 6,
+    CRT_ERR_CANCELLED = 7,
+    CRT_ERR_NOT_READY = 8,        /* crt_poll / crt_try_result: request still running */
+    CRT_ERR_HIP = 9,              /* any other HIP runtime error (reference: exit(99), core/src/gpu.cu:119-129) */
+    CRT_ERR_INTERNAL = 10
+} crt_status;
+
+/* ------------------------------------------------------------------ scene description */
+
+/* Perspective camera, core/src/cameras/perspective_camera.cuh:41-119 (initialize + get_ray).
+ * The aspect ratio is the camera's own (reference: -ratio CLI flag), not width/height. */
+typedef struct crt_camera {
+    float lookfrom[3];
+    float lookat[3];
+    float vup[3];
+    float vfov_deg;
+    float aspect_ratio;
+    float aperture;      /* lens radius = aperture / 2 */
+    float focus_dist;    /* primary ray directions have length ~focus_dist (not normalized) */
+    float time0, time1;  /* ray time = time0 + U*(time1-time0); the reference passes 0 and 1 */
+} crt_camera;
+
+/* Material type codes follow MaterialTypeID (core/src/materials/material.cuh:8-19). */
+enum {
+    CRT_MAT_LAMBERTIAN = 1,      /* core/src/materials/lambertian.cuh:63-143 */
+    CRT_MAT_METAL = 2,           /* core/src/materials/metal.cuh:52-65 */
+    CRT_MAT_DIELECTRIC = 3,      /* core/src/materials/dielectric.cuh:32-63 */
+    CRT_MAT_DIFFUSE_LIGHT = 8    /* core/src/materials/diffuse_light.cuh:45-67 */
+};
+/* crt_material.flags */
+#define CRT_MATF_ONE_SIDED 1u    /* diffuse_light "directional": back faces emit colour(0,0,0,0) */
+
+typedef struct crt_material {
+    uint32_t type;
+    int32_t texture;     /* index into textures[], or -1 for the constant albedo below */
+    float albedo[3];
+    float fuzz;          /* metal; clamped to <= 1 at upload like the reference constructor */
+    float ior;           /* dielectric index of refraction */
+    float intensity;     /* diffuse light: emitted = albedo * intensity */
+    uint32_t flags;
+} crt_material;
+
+enum {
+    CRT_TEX_SOLID = 0,   /* core/src/textures/solid_color_texture.cuh:34-37 */
+    CRT_TEX_CHECKER = 1, /* 3-D spatial checker, core/src/textures/checker_texture.cuh:33-42 */
+    CRT_TEX_IMAGE = 2    /* nearest texel, V flip, clamp, core/src/textures/image_texture.cuh:54-71 */
+};
+
+typedef struct crt_texture {
+    uint32_t type;
+    float color[3];          /* SOLID */
+    float scale;             /* CHECKER: cell size (the reference stores 1/scale) */
+    int32_t even, odd;       /* CHECKER: texture indices (must not be checkers themselves) */
+    uint32_t width, height;  /* IMAGE */
+    uint32_t channels;       /* IMAGE: 3 (RGB8) or 4 (RGBA8); only RGB is sampled */
+    const uint8_t* pixels;   /* IMAGE: width*height*channels bytes, row 0 at the top */
+} crt_texture;
+
+/* Texture-coordinate mapping of spheres (uvmapping, core/src/utilities/uvmapping.cuh:8-60). */
+typedef struct crt_uvmap {
+    float scale_u, scale_v, offset_u, offset_v, repeat_u, repeat_v;
+} crt_uvmap;
+
+typedef struct crt_sphere {      /* core/src/primitives/sphere.cuh:111-166 */
+    float center[3];
+    float radius;
+    uint32_t material;
+    crt_uvmap mapping;
+} crt_sphere;
+
+typedef struct crt_quad {        /* core/src/primitives/quad.cuh:56-170; corner + two edges */
+    float corner[3];
+    float u[3];
+    float v[3];
+    uint32_t material;
+} crt_quad;
+
+/* Quad area light centred at `center` (directional_light, core/src/lights/directional_light.cuh:45-172).
+ * Emission = rgb * intensity from the face that u x v points to; invisible lights are skipped by
+ * camera rays only (depth == max_depth, directional_light.cuh:106-109). */
+typedef struct crt_area_light {
+    float center[3];
+    float u[3];
+    float v[3];
+    float rgb[3];
+    float intensity;
+    uint32_t invisible;
+} crt_area_light;
+
+/* Indexed triangle mesh (Moller-Trumbore in the reference, core/src/primitives/triangle.cuh:130-185;
+ * watertight here). normals/uvs are optional per-vertex arrays (NULL = flat normal / zero uv). */
+typedef struct crt_mesh {
+    const float* positions;        /* 3 floats per vertex */
+    uint32_t num_vertices;
+    const uint32_t* indices;       /* 3 per triangle */
+    uint32_t num_triangles;
+    const float* normals;          /* optional, 3 per vertex: smooth shading */
+    const float* uvs;              /* optional, 2 per vertex */
+    uint32_t material;             /* material of every triangle unless tri_materials is given */
+    const uint32_t* tri_materials; /* optional, 1 per triangle */
+} crt_mesh;
+
+/* Rigid or affine placement of a mesh (the reference's translate/rotate/scale wrappers,
+ * core/src/primitives/{translate,rotate,scale}.cuh, applied per primitive). Row-major 3x4
+ * object-to-world matrix [R|t]. When a scene has instances, meshes[] are only reachable through
+ * instances (two-level BVH); otherwise every mesh is placed in world space as given. */
+typedef struct crt_instance {
+    uint32_t mesh;
+    float object_to_world[12];
+    int32_t material_override;     /* -1: keep the mesh's materials */
+} crt_instance;
+
+typedef struct crt_scene_desc {
+    uint32_t abi_version;          /* CRT_ABI_VERSION */
+    crt_camera camera;
+    float background[3];           /* miss colour (imageCfg.background.rgb, core/src/gpu.cu:741) */
+    const crt_material* materials; uint32_t num_materials;
+    const crt_texture* textures;   uint32_t num_textures;
+    const crt_area_light* lights;  uint32_t num_lights;
+    const crt_sphere* spheres;     uint32_t num_spheres;
+    const crt_quad* quads;         uint32_t num_quads;
+    const crt_mesh* meshes;        uint32_t num_meshes;
+    const crt_instance* instances; uint32_t num_instances;
+} crt_scene_desc;
+
+/* Global primitive numbering used by first-hit queries (mirrors the reference's world-list order,
+ * lights first, then spheres, quads, mesh triangles: core/src/gpu.cu:504-640):
+ *   lights [0, L), spheres [L, L+S), quads [L+S, L+S+Q), triangles [L+S+Q, ...) in mesh order.
+ * For instanced scenes the triangle id is the mesh-local id and the instance is reported apart. */
+
+/* ------------------------------------------------------------------ rendering */
+enum {
+    CRT_RNG_PCG = 0,     /* PCG32 stream keyed by (seed, pixel, sample, bounce): order independent */
+    CRT_RNG_MINSTD = 1   /* reference stream: one minstd_rand per pixel seeded seed+pixel
+                            (core/src/gpu.cu:770, seed 2580 at :2080); samples run in order */
+};
+
+typedef struct crt_render_desc {
+    uint32_t width, height;
+    uint32_t spp;          /* the loop runs floor(sqrt(spp))^2 samples and divides by spp,
+                              as the reference does (core/src/gpu.cu:776-792) */
+    uint32_t max_depth;
+    uint64_t seed;         /* PCG: stream seed; MINSTD: the per-pixel seed base (2580 in the reference) */
+    uint32_t rng_mode;     /* CRT_RNG_* */
+    uint32_t sample_begin; /* first sample index of this call (progressive rendering) */
+    uint32_t sample_count; /* 0 = all remaining samples */
+    uint32_t flags;        /* reserved, 0 */
+} crt_render_desc;
+
+typedef struct crt_image {
+    uint8_t* rgb8;         /* optional: width*height*3, tonemapped like core/src/gpu.cu:792-799 */
+    float* accum;          /* optional: width*height*4 float radiance sums (rgb, sample count) */
+} crt_image;
+
+typedef struct crt_stats {
+    uint64_t paths;            /* camera paths traced */
+    uint64_t rays;             /* closest-hit queries (= path segments; the estimator has no shadow rays) */
+    uint64_t retraces;         /* pass-through re-traces behind one-sided lights (core/src/cameras/camera.cuh:180-183) */
+    uint64_t light_pdf_tests;  /* analytic light-PDF intersection tests */
+    uint64_t node_visits;      /* interior nodes fetched (counting kernels only, else 0) */
+    uint64_t prim_tests;       /* primitive intersection tests (counting kernels only, else 0) */
+    double gpu_ms;             /* device time of the render passes (HIP events) */
+    double extend_ms;          /* device time spent in the traversal kernel */
+    double host_ms;            /* wall time of the call */
+} crt_stats;
+
+/* ------------------------------------------------------------------ context */
+typedef struct crt_context crt_context;
+typedef uint32_t crt_scene_id;
+typedef uint64_t crt_request_id;
+
+typedef struct crt_config {
+    int32_t device;             /* HIP device ordinal */
+    uint64_t hbm_budget_bytes;  /* admission budget for scenes + requests; 0 = 90% of free memory */
+    uint32_t max_paths;         /* path slots per wavefront pass; 0 = autotuned default */
+    uint32_t block_size;        /* threads per block of the stage kernels; 0 = default (256) */
+    uint32_t count_work;        /* 1: use the counting traversal kernel (fills node_visits/prim_tests) */
+    uint32_t leaf_size;         /* BVH max primitives per leaf; 0 = default (4) */
+} crt_config;
+
+int crt_create(const crt_config* cfg, crt_context** out);
+int crt_destroy(crt_context* ctx);
+const char* crt_last_error(void);
+int crt_device_count(int32_t* count);
+
+/* Copies and uploads a scene (host BVH build + SoA upload). */
+int crt_scene_upload(crt_context* ctx, const crt_scene_desc* desc, crt_scene_id* out);
+int crt_scene_release(crt_context* ctx, crt_scene_id scene);
+/* BVH statistics of an uploaded scene: nodes, leaves, depth, device bytes, build ms. */
+typedef struct crt_scene_info {
+    uint64_t num_nodes, num_leaves, num_prims, device_bytes;
+    uint32_t max_depth, num_instances;
+    double build_ms, upload_ms, sah_cost;
+} crt_scene_info;
+int crt_scene_get_info(crt_context* ctx, crt_scene_id scene, crt_scene_info* info);
+/* Host-only: flattens and builds a scene without a device, checks the BVH invariants
+ * (containment, every primitive in exactly one leaf, reachability) and reports its statistics.
+ * Same validation errors as crt_scene_upload (CRT_ERR_INVALID_SCENE for dangling indices). */
+int crt_scene_build_check(const crt_scene_desc* desc, uint32_t leaf_size, crt_scene_info* info);
+
+/* Blocking render of one request (the launchGPU replacement). */
+int crt_render(crt_context* ctx, crt_scene_id scene, const crt_render_desc* rd, crt_image* out,
+               crt_stats* stats);
+
+/* Closest-hit queries for arbitrary rays (first-hit parity). rays: n x 8 floats
+ * (ox,oy,oz,tmin, dx,dy,dz,tmax). Outputs per ray: global primitive id (-1 = miss), instance
+ * (-1 = none), t, and the barycentric / planar coordinates u, v. camera_rays != 0 hides invisible
+ * lights as the reference does for depth == max_depth. */
+int crt_trace_rays(crt_context* ctx, crt_scene_id scene, const float* rays, uint64_t n,
+                   int32_t camera_rays, int32_t* prim, int32_t* inst, float* t, float* uv);
+
+/* Camera rays of sample 0 at jitter r = 0.5 with no lens/time draws (pixel-centre first hits). */
+int crt_trace_primary(crt_context* ctx, crt_scene_id scene, uint32_t width, uint32_t height,
+                      int32_t* prim, float* t);
+
+/* ------------------------------------------------------------------ batched launch */
+/* Renders several requests with fused wavefront passes over one scene table: every stage kernel
+ * processes the rays of all requests together. Each request's image is identical to what
+ * crt_render gives for it alone (keyed RNG + fixed per-pixel summation order). */
+typedef struct crt_batch_item {
+    crt_scene_id scene;
+    crt_render_desc rd;
+    crt_image out;
+} crt_batch_item;
+int crt_render_batch(crt_context* ctx, crt_batch_item* items, uint32_t n, crt_stats* stats);
+
+/* ------------------------------------------------------------------ serving */
+/* Continuous batching: requests are admitted against the HBM budget as they arrive, their
+ * progressive passes are interleaved with those of requests already on the device, and each
+ * completes independently. Thread-safe for concurrent submit/poll/wait/cancel on one context. */
+int crt_submit(crt_context* ctx, crt_scene_id scene, const crt_render_desc* rd, crt_request_id* out);
+int crt_poll(crt_context* ctx, crt_request_id req, int32_t* done);
+int crt_wait(crt_context* ctx, crt_request_id req, crt_image* out, crt_stats* stats);
+int crt_cancel(crt_context* ctx, crt_request_id req);
+
+typedef struct crt_request_timing {
+    double submit_s, admit_s, first_pass_s, done_s;  /* seconds on the library's steady clock */
+    uint64_t rays;
+    uint32_t passes;
+    int32_t status;
+} crt_request_timing;
+int crt_request_get_timing(crt_context* ctx, crt_request_id req, crt_request_timing* out);
+
+/* ------------------------------------------------------------------ introspection */
+/* Sizes of the public structs as compiled into the library (binding self-check). */
+int crt_struct_sizes(uint64_t* sizes, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CORTEX_RT_H */
