@@ -131,14 +131,20 @@ class crt_request_timing(C.Structure):
                 ("rays", C.c_uint64), ("passes", C.c_uint32), ("status", C.c_int32)]
 
 
+class crt_work_counters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("rays", "retraces", "light_pdf_tests", "node_visits", "leaf_visits", "tri_tests",
+                                          "sphere_tests", "quad_tests", "inst_visits", "extend_launches", "passes")]
+
+
 STRUCTS_IN_ABI_ORDER = [crt_camera, crt_material, crt_texture, crt_uvmap, crt_sphere, crt_quad, crt_area_light,
                         crt_mesh, crt_instance, crt_scene_desc, crt_render_desc, crt_image, crt_stats, crt_config,
-                        crt_scene_info, crt_batch_item, crt_request_timing]
+                        crt_scene_info, crt_batch_item, crt_request_timing, crt_work_counters]
 
 # Every symbol declared in include/cortex_rt.h.
 EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "crt_scene_upload", "crt_scene_release",
             "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
-            "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_struct_sizes"]
+            "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
+            "crt_struct_sizes"]
 
 _P = C.POINTER
 _SIGS = {
@@ -160,6 +166,7 @@ _SIGS = {
     "crt_wait": ([C.c_void_p, C.c_uint64, _P(crt_image), _P(crt_stats)], C.c_int),
     "crt_cancel": ([C.c_void_p, C.c_uint64], C.c_int),
     "crt_request_get_timing": ([C.c_void_p, C.c_uint64, _P(crt_request_timing)], C.c_int),
+    "crt_last_counters": ([C.c_void_p, _P(crt_work_counters)], C.c_int),
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
 }
 

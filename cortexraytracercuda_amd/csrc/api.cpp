@@ -427,6 +427,7 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     HIP_TRY(hipEventRecord(ev_start, stream));
     double ext_ms = 0.0;
     uint64_t paths = 0;
+    uint64_t launches0 = extend_launches, passes0 = passes;
     for (int mode = 0; mode < 2; ++mode) {
         for (;;) {
             std::vector<WorkItem> items;
@@ -454,6 +455,11 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     if ((rc = read_counters(c))) return rc;
     if (c.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
     fill_stats(stats, c, paths, gms, ext_ms, (now_s() - h0) * 1e3);
+    last.rays = c.rays; last.retraces = c.retraces; last.light_pdf_tests = c.light_tests;
+    last.node_visits = c.node_visits; last.leaf_visits = c.leaf_visits; last.tri_tests = c.tri_tests;
+    last.sphere_tests = c.sphere_tests; last.quad_tests = c.quad_tests; last.inst_visits = c.inst_visits;
+    last.extend_launches = (time_stages ? extend_launches - launches0 : 0);
+    last.passes = passes - passes0;
     return CRT_OK;
 }
 
@@ -647,11 +653,19 @@ int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, crt_scene
     return CRT_OK;
 }
 
+int crt_last_counters(crt_context* c, crt_work_counters* out) {
+    if (!c || !out) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    *out = ctx->last;
+    return CRT_OK;
+}
+
 int crt_struct_sizes(uint64_t* sizes, uint32_t n) {
     uint64_t v[] = {sizeof(crt_camera), sizeof(crt_material), sizeof(crt_texture), sizeof(crt_uvmap), sizeof(crt_sphere),
                     sizeof(crt_quad), sizeof(crt_area_light), sizeof(crt_mesh), sizeof(crt_instance), sizeof(crt_scene_desc),
                     sizeof(crt_render_desc), sizeof(crt_image), sizeof(crt_stats), sizeof(crt_config), sizeof(crt_scene_info),
-                    sizeof(crt_batch_item), sizeof(crt_request_timing)};
+                    sizeof(crt_batch_item), sizeof(crt_request_timing), sizeof(crt_work_counters)};
     uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
     for (uint32_t k = 0; k < n && k < m; ++k) sizes[k] = v[k];
     return (int)m;

@@ -108,6 +108,7 @@ struct Context {
     uint64_t passes = 0, extend_launches = 0;
     DevBuf ray_o, ray_d, thr, rad, ids, hit, hit_inst, queue[2];
     int fail_alloc_after = -1;       // test hook: fail the n-th next allocation
+    crt_work_counters last{};        // counters of the last blocking render
 
     // serving
     std::mutex srv_mutex;

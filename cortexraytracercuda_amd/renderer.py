@@ -96,6 +96,15 @@ class Renderer:
         A.check(self.lib.crt_render(self.h, sid, C.byref(rd), C.byref(img), C.byref(st)))
         return rgb, acc, stats_dict(st)
 
+    def counters(self):
+        """Work counters of the last blocking render (traversal counts need count_work=True)."""
+        c = A.crt_work_counters()
+        A.check(self.lib.crt_last_counters(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in A.crt_work_counters._fields_}
+
+    def extend_launches(self):
+        return self.counters()["extend_launches"]
+
     def render_batch(self, jobs, want_accum=True, time_stages=False):
         """jobs: list of dicts (sid, width, height, spp, max_depth, seed, rng). Returns
         ([(rgb, accum)], stats) with every request rendered in fused passes."""

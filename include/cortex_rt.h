@@ -285,6 +285,15 @@ typedef struct crt_request_timing {
 int crt_request_get_timing(crt_context* ctx, crt_request_id req, crt_request_timing* out);
 
 /* ------------------------------------------------------------------ introspection */
+/* Work counters of the last crt_render / crt_render_batch on this context. The traversal counts
+ * (node_visits .. inst_visits) are filled only by a context created with count_work = 1. */
+typedef struct crt_work_counters {
+    uint64_t rays, retraces, light_pdf_tests;
+    uint64_t node_visits, leaf_visits, tri_tests, sphere_tests, quad_tests, inst_visits;
+    uint64_t extend_launches, passes;
+} crt_work_counters;
+int crt_last_counters(crt_context* ctx, crt_work_counters* out);
+
 /* Sizes of the public structs as compiled into the library (binding self-check). */
 int crt_struct_sizes(uint64_t* sizes, uint32_t n);
 

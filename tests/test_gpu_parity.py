@@ -53,19 +53,45 @@ def test_box_render_vs_fp64(renderer, box):
 
 
 def test_box_render_minstd_vs_refcpu(renderer, box):
+    """The reference's own stream (minstd per pixel, seed 2580 + pixel). One sample per pixel
+    follows the reference-faithful CPU renderer path for path; over 16 samples a pixel whose path
+    once flips a float decision (the parametric 1e-5 self-intersection epsilon of the reference on
+    grazing ceiling-to-light rays) continues on other draws, so the 16-spp image is compared
+    statistically: its distance to the reference render must stay below the distance between two
+    reference renders with different seeds."""
     W = H = 64
-    spp, depth = 16, 8
+    depth = 8
     sid = renderer.upload(box)
+    rgb1, acc1, _ = renderer.render(sid, W, H, 1, depth, seed=2580, rng="minstd")
+    o1, _ = O.render(box, W, H, 1, depth, seed=2580, rng="minstd", precision=32, threads=NT)
+    close1 = np.all(np.abs(acc1[..., :3] - o1[..., :3]) <= 1e-3 * np.maximum(o1[..., :3], 1.0), axis=-1)
+    spp = 16
     rgb, acc, st = renderer.render(sid, W, H, spp, depth, seed=2580, rng="minstd")
-    oacc, ost = O.render(box, W, H, spp, depth, seed=2580, rng="minstd", precision=32, threads=NT)
-    g = acc[..., :3] / spp
-    o = oacc[..., :3] / spp
-    close = np.all(np.abs(g - o) <= 1e-3 * np.maximum(o, 1.0), axis=-1)
-    orgb = O.tonemap(oacc, spp)
-    same8 = np.all(rgb == orgb, axis=-1).mean()
+    oacc, _ = O.render(box, W, H, spp, depth, seed=2580, rng="minstd", precision=32, threads=NT)
+    oacc2, _ = O.render(box, W, H, spp, depth, seed=99991, rng="minstd", precision=32, threads=NT)
+    g, o, o2 = acc[..., :3] / spp, oacc[..., :3] / spp, oacc2[..., :3] / spp
     rmse = float(np.sqrt(np.mean((g - o) ** 2)))
-    print("minstd vs refcpu: pixels equal %.4f rgb8 equal %.4f rmse %.5f" % (close.mean(), same8, rmse))
-    assert close.mean() >= 0.95
+    noise = float(np.sqrt(np.mean((o2 - o) ** 2)))
+    print("minstd vs refcpu: 1-spp pixels equal %.4f; 16-spp rmse %.5f (seed-to-seed %.5f)" % (close1.mean(), rmse, noise))
+    assert close1.mean() >= 0.99
+    assert rmse <= 0.9 * noise
+    renderer.release(sid)
+
+
+def test_box_pcg_vs_reference_render_statistically(renderer, box):
+    """Keyed PCG streams vs the reference's minstd stream: same estimator, independent noise."""
+    W = H = 64
+    spp, depth = 64, 8
+    sid = renderer.upload(box)
+    _, acc, _ = renderer.render(sid, W, H, spp, depth, seed=1, rng="pcg")
+    o, _ = O.render(box, W, H, spp, depth, seed=2580, rng="minstd", precision=32, threads=NT)
+    o2, _ = O.render(box, W, H, spp, depth, seed=99991, rng="minstd", precision=32, threads=NT)
+    g, a, b = acc[..., :3] / spp, o[..., :3] / spp, o2[..., :3] / spp
+    rmse = float(np.sqrt(np.mean((g - a) ** 2)))
+    noise = float(np.sqrt(np.mean((b - a) ** 2)))
+    print("pcg vs reference render: rmse %.5f, reference seed-to-seed %.5f, means %.4f %.4f" % (rmse, noise, g.mean(), a.mean()))
+    assert rmse <= 1.2 * noise
+    assert abs(g.mean() - a.mean()) <= 0.02 * a.mean()
     renderer.release(sid)
 
 
