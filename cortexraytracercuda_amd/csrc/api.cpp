@@ -342,6 +342,8 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
         w.job->rays_in_flight += (uint64_t)w.pix_count * w.sample_count;
     }
     uint32_t npaths = base;
+    int32_t uniform_scene = (int32_t)items[0].job->req.scene;
+    for (auto& w : items) if ((int32_t)w.job->req.scene != uniform_scene) uniform_scene = -1;
     if (npaths > path_capacity) return set_error(CRT_ERR_INTERNAL, "pass exceeds path capacity");
     crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
     HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request) * items.size(), hipMemcpyHostToDevice, stream));
@@ -360,7 +362,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
         uint32_t* qout = (uint32_t*)queue[qi].p;
         uint32_t* cout = cnt + qi;
         if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-        if (crt_k_extend(stream, config.count_work ? 1 : 0, egrid, dsc, dreq, qin, cin, npaths, (const float4*)ray_o.p,
+        if (crt_k_extend(stream, config.count_work ? 1 : 0, egrid, dsc, dreq, uniform_scene, qin, cin, npaths, (const float4*)ray_o.p,
                          (const float4*)ray_d.p, (const uint2*)ids.p, (float4*)hit.p, (uint32_t*)hit_inst.p,
                          (uint32_t*)gstack.p, cout, (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "extend launch failed");

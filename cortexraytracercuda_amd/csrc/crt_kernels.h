@@ -9,7 +9,7 @@ extern "C" {
 int crt_k_generate(hipStream_t s, int rng_mode, const crt_dev_request* reqs, int nreq, uint32_t npaths, float4* ray_o,
                    float4* ray_d, float4* thr, float4* rad, uint2* ids);
 int crt_k_extend(hipStream_t s, int count_mode, uint32_t grid, const crt_dev_scene* scenes, const crt_dev_request* reqs,
-                 const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
+                 int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                  uint32_t* zero_counter, crt_dev_counters* counters);
 int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes, const crt_dev_request* reqs,

@@ -460,9 +460,10 @@ __global__ __launch_bounds__(256) void k_generate(const crt_dev_request* __restr
 
 // Closest-hit stage. One-sided emitters hit from behind are re-traced from t + 0.001 in the same
 // stage (core/src/cameras/camera.cuh:179-183); the result is flagged as a pass-through.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT>
+// UNIFORM: every ray of the launch belongs to scene `uniform_scene` (scene pointers stay scalar).
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM>
 __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restrict__ scenes,
-                                                  const crt_dev_request* __restrict__ reqs,
+                                                  const crt_dev_request* __restrict__ reqs, uint32_t uniform_scene,
                                                   const uint32_t* __restrict__ queue, const uint32_t* __restrict__ count_ptr,
                                                   uint32_t count_direct, const float4* __restrict__ ray_o,
                                                   const float4* __restrict__ ray_d, const uint2* __restrict__ ids,
@@ -478,34 +479,36 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
     uint32_t* gst = gstack + (size_t)gtid * GS;
     TravCount tc = {0, 0, 0, 0, 0, 0};
     uint32_t nrays = 0, nretrace = 0, novf = 0;
+    SceneView SU;
+    if (UNIFORM) SU = scene_view(scenes + uniform_scene);
     for (uint32_t i = gtid; i < n; i += stride) {
         uint32_t slot = queue ? queue[i] : i;
         float4 ro = ray_o[slot], rdv = ray_d[slot];
         uint32_t state = fbits(rdv.w);
-        uint32_t req = ids[slot].y >> 16;
-        const crt_dev_scene* sc = scenes + reqs[req].scene;
-        SceneView S = scene_view(sc);
+        SceneView S;
+        if (UNIFORM) S = SU;
+        else S = scene_view(scenes + reqs[ids[slot].y >> 16].scene);
         V3 o = v3(ro), d = v3(rdv);
         bool camera = (state & ST_CAMERA) != 0;
         HitRec h;
         bool ovf = false;
-        trace<BLOCK, RS, LS, GS, COUNT>(S, o, d, 0.00001f, 3.402823466e+38f, camera, lds_col, gst, h, tc, ovf);
-        nrays++;
-        uint32_t inst_word = h.inst;
-        if (h.prim != CRT_MISS && CRT_HIT_TYPE(h.prim) == CRT_PRIM_QUAD) {
+        float tmin = 0.00001f;
+        uint32_t pass_flag = 0u;
+        for (;;) {
+            trace<BLOCK, RS, LS, GS, COUNT>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc, ovf);
+            nrays++;
+            if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
             const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
             uint32_t fl = fbits(q[1].w);
-            if ((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(q[3])) < 0.0f)) {
-                HitRec h2;
-                trace<BLOCK, RS, LS, GS, COUNT>(S, o, d, h.t + 0.001f, 3.402823466e+38f, camera, lds_col, gst, h2, tc, ovf);
-                nretrace++;
-                h = h2;
-                inst_word = h2.inst | CRT_PASSTHROUGH;
-            }
+            if (!((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(q[3])) < 0.0f))) break;
+            tmin = h.t + 0.001f;
+            pass_flag = CRT_PASSTHROUGH;
+            nretrace++;
+            nrays--;
         }
         novf += ovf ? 1u : 0u;
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
-        hit_inst[slot] = inst_word;
+        hit_inst[slot] = h.inst | pass_flag;
     }
     if (nrays) atomicAdd(&counters->rays, (unsigned long long)nrays);
     if (nretrace) atomicAdd(&counters->retraces, (unsigned long long)nretrace);
@@ -922,16 +925,20 @@ extern "C" int crt_k_generate(hipStream_t s, int rng_mode, const crt_dev_request
 }
 
 extern "C" int crt_k_extend(hipStream_t s, int count_mode, uint32_t grid, const crt_dev_scene* scenes,
-                            const crt_dev_request* reqs, const uint32_t* queue, const uint32_t* count_ptr,
-                            uint32_t count_direct, const float4* ray_o, const float4* ray_d, const uint2* ids,
-                            float4* hit, uint32_t* hit_inst, uint32_t* gstack, uint32_t* zero_counter,
+                            const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue,
+                            const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o, const float4* ray_d,
+                            const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack, uint32_t* zero_counter,
                             crt_dev_counters* counters) {
-    if (count_mode)
-        hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, true>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue,
-                           count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters);
-    else
-        hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, false>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue,
-                           count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters);
+    uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
+#define CRT_EXT(C, U)                                                                                          \
+    hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, C, U>), dim3(grid), dim3(256), 0, s, scenes, reqs, us, \
+                       queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters)
+    if (count_mode) {
+        if (uniform_scene >= 0) CRT_EXT(true, true); else CRT_EXT(true, false);
+    } else {
+        if (uniform_scene >= 0) CRT_EXT(false, true); else CRT_EXT(false, false);
+    }
+#undef CRT_EXT
     return (int)hipGetLastError();
 }
 
