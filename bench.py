@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller image for a fast check")
+    ap.add_argument("--bvh-width", type=int, default=0, help="2 or 4 children per node (0: library default)")
+    ap.add_argument("--max-paths", type=int, default=0, help="path slots per pass (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and this path is single-GPU "
@@ -143,7 +145,7 @@ def main():
     if args.quick:
         cfg = dict(width=480, height=270, spp=16, max_depth=10)
     log("scene generated in %.1f s" % (time.perf_counter() - t_gen))
-    r = Renderer()
+    r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths)
     t0 = time.perf_counter()
     sid = r.upload(scene)
     info = r.info(sid)
@@ -163,7 +165,7 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = rays / wall / 1e6
     # roofline of the traversal kernel, measured outside the timed region
-    rc = Renderer(count_work=True)
+    rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths)
     sidc = rc.upload(scene)
     _, _, stc = rc.render(sidc, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, want_accum=False)
     cnt = rc.counters()

@@ -113,7 +113,8 @@ class crt_stats(C.Structure):
 
 class crt_config(C.Structure):
     _fields_ = [("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("max_paths", C.c_uint32),
-                ("block_size", C.c_uint32), ("count_work", C.c_uint32), ("leaf_size", C.c_uint32)]
+                ("block_size", C.c_uint32), ("count_work", C.c_uint32), ("leaf_size", C.c_uint32),
+                ("bvh_width", C.c_uint32)]
 
 
 class crt_scene_info(C.Structure):
@@ -136,15 +137,20 @@ class crt_work_counters(C.Structure):
                                           "sphere_tests", "quad_tests", "inst_visits", "extend_launches", "passes")]
 
 
+class crt_memory_info(C.Structure):
+    _fields_ = [("budget", C.c_uint64), ("reserved", C.c_uint64), ("scenes", C.c_uint64), ("requests", C.c_uint64)]
+
+
 STRUCTS_IN_ABI_ORDER = [crt_camera, crt_material, crt_texture, crt_uvmap, crt_sphere, crt_quad, crt_area_light,
                         crt_mesh, crt_instance, crt_scene_desc, crt_render_desc, crt_image, crt_stats, crt_config,
-                        crt_scene_info, crt_batch_item, crt_request_timing, crt_work_counters]
+                        crt_scene_info, crt_batch_item, crt_request_timing, crt_work_counters,
+                        crt_memory_info]
 
 # Every symbol declared in include/cortex_rt.h.
 EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "crt_scene_upload", "crt_scene_release",
             "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
             "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
-            "crt_struct_sizes"]
+            "crt_memory_get_info", "crt_struct_sizes"]
 
 _P = C.POINTER
 _SIGS = {
@@ -155,7 +161,7 @@ _SIGS = {
     "crt_scene_upload": ([C.c_void_p, _P(crt_scene_desc), _P(C.c_uint32)], C.c_int),
     "crt_scene_release": ([C.c_void_p, C.c_uint32], C.c_int),
     "crt_scene_get_info": ([C.c_void_p, C.c_uint32, _P(crt_scene_info)], C.c_int),
-    "crt_scene_build_check": ([_P(crt_scene_desc), C.c_uint32, _P(crt_scene_info)], C.c_int),
+    "crt_scene_build_check": ([_P(crt_scene_desc), C.c_uint32, C.c_uint32, _P(crt_scene_info)], C.c_int),
     "crt_render": ([C.c_void_p, C.c_uint32, _P(crt_render_desc), _P(crt_image), _P(crt_stats)], C.c_int),
     "crt_trace_rays": ([C.c_void_p, C.c_uint32, _P(C.c_float), C.c_uint64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
                         _P(C.c_float), _P(C.c_float)], C.c_int),
@@ -167,6 +173,7 @@ _SIGS = {
     "crt_cancel": ([C.c_void_p, C.c_uint64], C.c_int),
     "crt_request_get_timing": ([C.c_void_p, C.c_uint64, _P(crt_request_timing)], C.c_int),
     "crt_last_counters": ([C.c_void_p, _P(crt_work_counters)], C.c_int),
+    "crt_memory_get_info": ([C.c_void_p, _P(crt_memory_info)], C.c_int),
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
 }
 

@@ -118,6 +118,8 @@ int Context::init(const crt_config* cfg) {
     budget = config.hbm_budget_bytes ? config.hbm_budget_bytes : (uint64_t)(0.9 * (double)freeb);
     max_paths = config.max_paths ? config.max_paths : (1u << 22);
     leaf_size = config.leaf_size ? config.leaf_size : 4u;
+    bvh_width = config.bvh_width ? config.bvh_width : 4u;
+    if (bvh_width != 2 && bvh_width != 4) return set_error(CRT_ERR_INVALID_ARGUMENT, "bvh_width must be 2 or 4");
     extend_grid = (uint32_t)num_cus * 8u;
     shade_grid = (uint32_t)num_cus * 8u;
     int rc;
@@ -184,7 +186,7 @@ int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
     auto sc = std::make_unique<DeviceScene>();
     std::string err;
     unsigned hw = std::thread::hardware_concurrency();
-    int rc = build_host_scene(desc, leaf_size, (int)std::min(16u, hw ? hw : 1u), sc->host, err);
+    int rc = build_host_scene(desc, leaf_size, bvh_width, (int)std::min(16u, hw ? hw : 1u), sc->host, err);
     if (rc) return set_error(rc, err);
     uint64_t need = sc->host.device_bytes();
     if (scene_bytes + request_bytes + reserved + need > budget)
@@ -214,7 +216,7 @@ int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
     ds.tri_uvs = sc->tri_uvs.p; ds.spheres = sc->spheres.p; ds.sphere_aux = sc->sphere_aux.p; ds.sphere_map = sc->sphere_map.p;
     ds.quads = sc->quads.p; ds.insts = sc->insts.p; ds.mats = sc->mats.p; ds.texs = sc->texs.p;
     ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;
-    ds.num_lights = H.num_lights; ds.root = H.root; ds.num_quads = H.num_quads;
+    ds.num_lights = H.num_lights; ds.root = H.root; ds.num_quads = H.num_quads; ds.bvh_width = H.bvh_width;
     for (int k = 0; k < 3; ++k) ds.background[k] = H.background[k];
     HIP_TRY(hipMemcpyAsync((crt_dev_scene*)scene_table.p + slot, &ds, sizeof(ds), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));
@@ -343,7 +345,11 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
     }
     uint32_t npaths = base;
     int32_t uniform_scene = (int32_t)items[0].job->req.scene;
-    for (auto& w : items) if ((int32_t)w.job->req.scene != uniform_scene) uniform_scene = -1;
+    int two_level = 0;
+    for (auto& w : items) {
+        if ((int32_t)w.job->req.scene != uniform_scene) uniform_scene = -1;
+        if (w.job->scene->host.num_instances) two_level = 1;
+    }
     if (npaths > path_capacity) return set_error(CRT_ERR_INTERNAL, "pass exceeds path capacity");
     crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
     HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request) * items.size(), hipMemcpyHostToDevice, stream));
@@ -362,7 +368,8 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
         uint32_t* qout = (uint32_t*)queue[qi].p;
         uint32_t* cout = cnt + qi;
         if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-        if (crt_k_extend(stream, config.count_work ? 1 : 0, egrid, dsc, dreq, uniform_scene, qin, cin, npaths, (const float4*)ray_o.p,
+        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, egrid, dsc, dreq, uniform_scene, qin,
+                         cin, npaths, (const float4*)ray_o.p,
                          (const float4*)ray_d.p, (const uint2*)ids.p, (float4*)hit.p, (uint32_t*)hit_inst.p,
                          (uint32_t*)gstack.p, cout, (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "extend launch failed");
@@ -580,7 +587,8 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
     HIP_TRY(hipMemcpyAsync(dr.p, rays, n * 32, hipMemcpyHostToDevice, ctx->stream));
     if ((rc = ctx->reset_counters())) return rc;
     uint32_t grid = std::min(ctx->extend_grid, (uint32_t)std::max<uint64_t>(1, (n + 255) / 256));
-    if (crt_k_trace_rays(ctx->stream, grid, (const crt_dev_scene*)ctx->scene_table.p + sc->slot, (const float4*)dr.p,
+    if (crt_k_trace_rays(ctx->stream, (int)sc->host.bvh_width, sc->host.num_instances ? 1 : 0, grid,
+                         (const crt_dev_scene*)ctx->scene_table.p + sc->slot, (const float4*)dr.p,
                          (uint32_t)n, camera_rays, (float4*)dh.p, (uint32_t*)di.p, (uint32_t*)ctx->gstack.p,
                          (crt_dev_counters*)ctx->counters.p))
         return set_error(CRT_ERR_HIP, "trace launch failed");
@@ -639,11 +647,12 @@ int crt_trace_primary(crt_context* c, crt_scene_id s, uint32_t W, uint32_t H, in
     return crt_trace_rays(c, s, rays.data(), (uint64_t)W * H, 1, prim, nullptr, t, uv.data());
 }
 
-int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, crt_scene_info* info) {
+int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, crt_scene_info* info) {
     HostScene H;
     std::string err;
     unsigned hw = std::thread::hardware_concurrency();
-    int rc = build_host_scene(d, leaf_size ? leaf_size : 4u, (int)std::min(16u, hw ? hw : 1u), H, err);
+    int rc = build_host_scene(d, leaf_size ? leaf_size : 4u, bvh_width ? bvh_width : 4u, (int)std::min(16u, hw ? hw : 1u),
+                              H, err);
     if (rc) return set_error(rc, err);
     rc = check_host_scene(H, err);
     if (rc) return set_error(rc, err);
@@ -663,11 +672,21 @@ int crt_last_counters(crt_context* c, crt_work_counters* out) {
     return CRT_OK;
 }
 
+int crt_memory_get_info(crt_context* c, crt_memory_info* out) {
+    if (!c || !out) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    std::lock_guard<std::mutex> lk(ctx->srv_mutex);
+    out->budget = ctx->budget; out->reserved = ctx->reserved; out->scenes = ctx->scene_bytes;
+    out->requests = ctx->request_bytes;
+    return CRT_OK;
+}
+
 int crt_struct_sizes(uint64_t* sizes, uint32_t n) {
     uint64_t v[] = {sizeof(crt_camera), sizeof(crt_material), sizeof(crt_texture), sizeof(crt_uvmap), sizeof(crt_sphere),
                     sizeof(crt_quad), sizeof(crt_area_light), sizeof(crt_mesh), sizeof(crt_instance), sizeof(crt_scene_desc),
                     sizeof(crt_render_desc), sizeof(crt_image), sizeof(crt_stats), sizeof(crt_config), sizeof(crt_scene_info),
-                    sizeof(crt_batch_item), sizeof(crt_request_timing), sizeof(crt_work_counters)};
+                    sizeof(crt_batch_item), sizeof(crt_request_timing), sizeof(crt_work_counters), sizeof(crt_memory_info)};
     uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
     for (uint32_t k = 0; k < n && k < m; ++k) sizes[k] = v[k];
     return (int)m;

@@ -8,15 +8,15 @@
 extern "C" {
 int crt_k_generate(hipStream_t s, int rng_mode, const crt_dev_request* reqs, int nreq, uint32_t npaths, float4* ray_o,
                    float4* ray_d, float4* thr, float4* rad, uint2* ids);
-int crt_k_extend(hipStream_t s, int count_mode, uint32_t grid, const crt_dev_scene* scenes, const crt_dev_request* reqs,
-                 int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
+int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
+                 const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                  uint32_t* zero_counter, crt_dev_counters* counters);
 int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes, const crt_dev_request* reqs,
                 const uint32_t* queue_in, const uint32_t* count_in, uint32_t count_direct, uint32_t* queue_out,
                 uint32_t* count_out, float4* ray_o, float4* ray_d, float4* thr, float4* rad, const uint2* ids,
                 const float4* hit, const uint32_t* hit_inst, crt_dev_counters* counters);
-int crt_k_trace_rays(hipStream_t s, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n, int camera,
+int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n, int camera,
                      float4* hit, uint32_t* hit_inst, uint32_t* gstack, crt_dev_counters* counters);
 int crt_k_resolve(hipStream_t s, const crt_dev_request* reqs, int nreq, uint32_t max_pix, const float4* rad);
 int crt_k_tonemap(hipStream_t s, const float4* accum, uint32_t npix, uint32_t spp, uint8_t* rgb);

@@ -67,7 +67,7 @@ struct crt_dev_scene {
     uint32_t num_lights;
     uint32_t root;          // root node index
     uint32_t num_quads;
-    uint32_t pad0;
+    uint32_t bvh_width;
     float background[3];
     float pad1;
 };

@@ -93,7 +93,7 @@ struct Context {
     int num_cus = 256;
     hipStream_t stream = nullptr;
     uint64_t budget = 0, reserved = 0, scene_bytes = 0, request_bytes = 0;
-    uint32_t max_paths = 0, path_capacity = 0, leaf_size = 4, extend_grid = 2048, shade_grid = 2048;
+    uint32_t max_paths = 0, path_capacity = 0, leaf_size = 4, bvh_width = 4, extend_grid = 2048, shade_grid = 2048;
     std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
     bool slot_used[kMaxScenes] = {};

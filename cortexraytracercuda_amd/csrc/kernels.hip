@@ -250,9 +250,30 @@ __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
 
 struct TravCount { uint32_t nodes, leaves, tris, spheres, quads, insts; };
 
-// Closest hit over the two-level BVH2. Interior nodes test both child boxes (slab test on
-// (lo - o) * inv_d, Ize's robust factor on the far distance) and descend near-first.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT>
+// Slab test of one child box on (lo - o) * inv_d; Ize's robust factor widens the far distance.
+__device__ __forceinline__ float box_near(float lx, float hx, float ly, float hy, float lz, float hz, V3 o, V3 id,
+                                          float tmin, float tmax, bool& hit) {
+    float ax = (lx - o.x) * id.x, bx = (hx - o.x) * id.x;
+    float ay = (ly - o.y) * id.y, by = (hy - o.y) * id.y;
+    float az = (lz - o.z) * id.z, bz = (hz - o.z) * id.z;
+    float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+    float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+    hit = nr <= fr * 1.0000004f;
+    return nr;
+}
+
+__device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t& vb) {
+    bool s = kb < ka;
+    float k = s ? kb : ka; kb = s ? ka : kb; ka = k;
+    int32_t v = s ? vb : va; vb = s ? va : vb; va = v;
+}
+
+#define CRT_EMPTY_CHILD 0x7fffffff
+
+// Closest hit over the BVH (WIDTH 2: both child boxes in a 64-B parent node; WIDTH 4: four child
+// boxes in a 128-B SoA node, children visited near-first). TWO_LEVEL enables instance leaves that
+// move the ray to object space and descend into the instance's BLAS.
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
 __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
                                       uint32_t* lds_col, uint32_t* gstk, HitRec& best, TravCount& tc, bool& ovf) {
     Stack<BLOCK, RS, LS, GS> st;
@@ -266,36 +287,51 @@ __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tm
     bool in_blas = false;
     int blas_level = 0;
     int32_t node = (int32_t)S.root;
-    const float robust = 1.0000004f;
     for (;;) {
         bool next = true;
         if (node >= 0) {
-            const float4* np = S.nodes + 4 * (uint32_t)node;
-            float4 n0 = np[0], n1 = np[1], n2 = np[2];
-            int4 n3 = *(const int4*)(np + 3);
             if (COUNT) tc.nodes++;
             float tf = best.t;
-            // child 0
-            float l0x = (n0.x - o.x) * id.x, h0x = (n0.y - o.x) * id.x;
-            float l0y = (n0.z - o.y) * id.y, h0y = (n0.w - o.y) * id.y;
-            float l0z = (n2.x - o.z) * id.z, h0z = (n2.y - o.z) * id.z;
-            float near0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), tmin));
-            float far0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tf));
-            // child 1
-            float l1x = (n1.x - o.x) * id.x, h1x = (n1.y - o.x) * id.x;
-            float l1y = (n1.z - o.y) * id.y, h1y = (n1.w - o.y) * id.y;
-            float l1z = (n2.z - o.z) * id.z, h1z = (n2.w - o.z) * id.z;
-            float near1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), tmin));
-            float far1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tf));
-            bool hit0 = near0 <= far0 * robust;
-            bool hit1 = near1 <= far1 * robust;
-            if (hit0 && hit1) {
-                bool swap = near1 < near0;
-                int32_t a = swap ? n3.y : n3.x, b = swap ? n3.x : n3.y;
-                st.push((uint32_t)b);
-                node = a; next = false;
-            } else if (hit0) { node = n3.x; next = false; }
-            else if (hit1) { node = n3.y; next = false; }
+            if (WIDTH == 2) {
+                const float4* np = S.nodes + 4 * (uint32_t)node;
+                float4 n0 = np[0], n1 = np[1], n2 = np[2];
+                int4 n3 = *(const int4*)(np + 3);
+                bool hit0, hit1;
+                float near0 = box_near(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, o, id, tmin, tf, hit0);
+                float near1 = box_near(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, o, id, tmin, tf, hit1);
+                if (hit0 && hit1) {
+                    bool swap = near1 < near0;
+                    int32_t a = swap ? n3.y : n3.x, b = swap ? n3.x : n3.y;
+                    st.push((uint32_t)b);
+                    node = a; next = false;
+                } else if (hit0) { node = n3.x; next = false; }
+                else if (hit1) { node = n3.y; next = false; }
+            } else {
+                const float4* np = S.nodes + 8 * (uint32_t)node;
+                float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+                int4 rf = *(const int4*)(np + 6);
+                bool h0, h1, h2, h3;
+                float k0 = box_near(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, id, tmin, tf, h0);
+                float k1 = box_near(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, id, tmin, tf, h1);
+                float k2 = box_near(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, id, tmin, tf, h2);
+                float k3 = box_near(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, id, tmin, tf, h3);
+                const float INF = __builtin_inff();
+                k0 = (h0 && rf.x != CRT_EMPTY_CHILD) ? k0 : INF;
+                k1 = (h1 && rf.y != CRT_EMPTY_CHILD) ? k1 : INF;
+                k2 = (h2 && rf.z != CRT_EMPTY_CHILD) ? k2 : INF;
+                k3 = (h3 && rf.w != CRT_EMPTY_CHILD) ? k3 : INF;
+                int32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
+                int nh = (k0 < INF) + (k1 < INF) + (k2 < INF) + (k3 < INF);
+                if (nh > 0) {
+                    cswap(k0, c0, k1, c1); cswap(k2, c2, k3, c3);
+                    cswap(k0, c0, k2, c2); cswap(k1, c1, k3, c3);
+                    cswap(k1, c1, k2, c2);
+                    if (nh > 3) st.push((uint32_t)c3);
+                    if (nh > 2) st.push((uint32_t)c2);
+                    if (nh > 1) st.push((uint32_t)c1);
+                    node = c0; next = false;
+                }
+            }
         } else {
             uint2 lf = S.leaves[(uint32_t)(~node)];
             uint32_t type = lf.y >> CRT_LEAF_TYPE_SHIFT;
@@ -331,7 +367,7 @@ __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tm
                         best.inst = cur_inst;
                     }
                 }
-            } else if (!in_blas) {
+            } else if (TWO_LEVEL && !in_blas) {
                 // instance leaf (count 1): move the ray to object space and descend into the BLAS
                 const float4* ip = S.insts + 4 * first;
                 float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
@@ -351,7 +387,7 @@ __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tm
             }
         }
         if (next) {
-            if (in_blas && st.depth() == blas_level) {
+            if (TWO_LEVEL && in_blas && st.depth() == blas_level) {
                 o = ow; d = dw;
                 id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
                 tp = tri_pre(d);
@@ -461,7 +497,7 @@ __global__ __launch_bounds__(256) void k_generate(const crt_dev_request* __restr
 // Closest-hit stage. One-sided emitters hit from behind are re-traced from t + 0.001 in the same
 // stage (core/src/cameras/camera.cuh:179-183); the result is flagged as a pass-through.
 // UNIFORM: every ray of the launch belongs to scene `uniform_scene` (scene pointers stay scalar).
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
 __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restrict__ scenes,
                                                   const crt_dev_request* __restrict__ reqs, uint32_t uniform_scene,
                                                   const uint32_t* __restrict__ queue, const uint32_t* __restrict__ count_ptr,
@@ -495,7 +531,8 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
         float tmin = 0.00001f;
         uint32_t pass_flag = 0u;
         for (;;) {
-            trace<BLOCK, RS, LS, GS, COUNT>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc, ovf);
+            trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc,
+                                                            ovf);
             nrays++;
             if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
             const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
@@ -524,7 +561,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
 }
 
 // Closest hit of caller rays (first-hit parity queries).
-template <int BLOCK, int RS, int LS, int GS>
+template <int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
 __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __restrict__ sc, const float4* __restrict__ rays,
                                                       uint32_t n, int camera, float4* __restrict__ hit,
                                                       uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
@@ -539,7 +576,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __res
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         HitRec h;
         bool ovf = false;
-        trace<BLOCK, RS, LS, GS, false>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
+        trace<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
                                         gstack + (size_t)gtid * GS, h, tc, ovf);
         novf += ovf ? 1u : 0u;
         hit[i] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
@@ -924,20 +961,24 @@ extern "C" int crt_k_generate(hipStream_t s, int rng_mode, const crt_dev_request
     return (int)hipGetLastError();
 }
 
-extern "C" int crt_k_extend(hipStream_t s, int count_mode, uint32_t grid, const crt_dev_scene* scenes,
-                            const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue,
-                            const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o, const float4* ray_d,
-                            const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack, uint32_t* zero_counter,
-                            crt_dev_counters* counters) {
+extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32_t grid,
+                            const crt_dev_scene* scenes, const crt_dev_request* reqs, int32_t uniform_scene,
+                            const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
+                            const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
+                            uint32_t* zero_counter, crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
-#define CRT_EXT(C, U)                                                                                          \
-    hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, C, U>), dim3(grid), dim3(256), 0, s, scenes, reqs, us, \
-                       queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters)
+#define CRT_EXT(C, U, W, T)                                                                                       \
+    hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, C, U, W, T>), dim3(grid), dim3(256), 0, s, scenes, reqs, \
+                       us, queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters)
+#define CRT_EXT_WT(C, U)                                                   \
+    if (width == 2) { if (two_level) CRT_EXT(C, U, 2, true); else CRT_EXT(C, U, 2, false); } \
+    else { if (two_level) CRT_EXT(C, U, 4, true); else CRT_EXT(C, U, 4, false); }
     if (count_mode) {
-        if (uniform_scene >= 0) CRT_EXT(true, true); else CRT_EXT(true, false);
+        if (uniform_scene >= 0) { CRT_EXT_WT(true, true) } else { CRT_EXT_WT(true, false) }
     } else {
-        if (uniform_scene >= 0) CRT_EXT(false, true); else CRT_EXT(false, false);
+        if (uniform_scene >= 0) { CRT_EXT_WT(false, true) } else { CRT_EXT_WT(false, false) }
     }
+#undef CRT_EXT_WT
 #undef CRT_EXT
     return (int)hipGetLastError();
 }
@@ -956,10 +997,15 @@ extern "C" int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt
     return (int)hipGetLastError();
 }
 
-extern "C" int crt_k_trace_rays(hipStream_t s, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n,
-                                int camera, float4* hit, uint32_t* hit_inst, uint32_t* gstack, crt_dev_counters* counters) {
-    hipLaunchKernelGGL((k_trace_rays<256, STK_RS, STK_LS, STK_GS>), dim3(grid), dim3(256), 0, s, sc, rays, n, camera, hit,
-                       hit_inst, gstack, counters);
+extern "C" int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_t grid, const crt_dev_scene* sc,
+                                const float4* rays, uint32_t n, int camera, float4* hit, uint32_t* hit_inst,
+                                uint32_t* gstack, crt_dev_counters* counters) {
+#define CRT_TR(W, T)                                                                                              \
+    hipLaunchKernelGGL((k_trace_rays<256, STK_RS, STK_LS, STK_GS, W, T>), dim3(grid), dim3(256), 0, s, sc, rays, n, \
+                       camera, hit, hit_inst, gstack, counters)
+    if (width == 2) { if (two_level) CRT_TR(2, true); else CRT_TR(2, false); }
+    else { if (two_level) CRT_TR(4, true); else CRT_TR(4, false); }
+#undef CRT_TR
     return (int)hipGetLastError();
 }
 

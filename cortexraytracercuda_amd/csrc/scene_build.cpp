@@ -190,6 +190,8 @@ private:
 };
 
 // Emits one built tree into the device arrays. Returns the root node index (always interior).
+#define CRT_EMPTY_CHILD 0x7fffffff
+
 struct Emitter {
     HostScene& S;
     const std::vector<BNode>& nodes;
@@ -229,7 +231,54 @@ struct Emitter {
         np[3] = F4{fb(u0), fb(u1), 0.0f, 0.0f};
     }
 
+    // Four-wide node: the binary tree is collapsed by repeatedly opening the child with the
+    // largest surface area until four children are gathered (or only leaves remain).
+    int32_t emit4(int32_t id, uint32_t depth) {
+        const BNode& n = nodes[id];
+        if (n.type >= 0) return leaf_ref(n);
+        max_depth = std::max(max_depth, depth + 1);
+        int32_t ch[4] = {n.l, n.r, -1, -1};
+        int nc = 2;
+        while (nc < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int k = 0; k < nc; ++k) {
+                const BNode& c = nodes[ch[k]];
+                if (c.type < 0 && c.box.area() > ba) { ba = c.box.area(); best = k; }
+            }
+            if (best < 0) break;
+            int32_t c = ch[best];
+            ch[best] = nodes[c].l;
+            ch[nc++] = nodes[c].r;
+        }
+        uint32_t slot = (uint32_t)(S.nodes.size() / 8);
+        S.nodes.resize(S.nodes.size() + 8);
+        sah += (double)n.box.area() / root_area;
+        int32_t refs[4] = {CRT_EMPTY_CHILD, CRT_EMPTY_CHILD, CRT_EMPTY_CHILD, CRT_EMPTY_CHILD};
+        Box boxes[4];
+        for (int k = 0; k < 4; ++k) for (int a = 0; a < 3; ++a) { boxes[k].lo[a] = 0.0f; boxes[k].hi[a] = 0.0f; }
+        for (int k = 0; k < nc; ++k) {
+            refs[k] = emit4(ch[k], depth + 1);
+            boxes[k] = nodes[ch[k]].box;
+        }
+        write_node4(slot, boxes, refs);
+        return (int32_t)slot;
+    }
+
+    void write_node4(uint32_t slot, const Box* b, const int32_t* refs) {
+        F4* np = &S.nodes[8 * (size_t)slot];
+        for (int a = 0; a < 3; ++a) {
+            np[2 * a] = F4{b[0].lo[a], b[1].lo[a], b[2].lo[a], b[3].lo[a]};
+            np[2 * a + 1] = F4{b[0].hi[a], b[1].hi[a], b[2].hi[a], b[3].hi[a]};
+        }
+        np[6] = F4{fb((uint32_t)refs[0]), fb((uint32_t)refs[1]), fb((uint32_t)refs[2]), fb((uint32_t)refs[3])};
+        np[7] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+
+    uint32_t width = 2;
+
     int32_t emit(int32_t id, uint32_t depth) {
+        if (width == 4) return emit4(id, depth);
         const BNode& n = nodes[id];
         if (n.type >= 0) return leaf_ref(n);
         max_depth = std::max(max_depth, depth + 1);
@@ -243,27 +292,35 @@ struct Emitter {
     }
 
     uint32_t emit_root(int32_t root) {
-        if (root < 0) {
-            // empty tree: one node whose two children are empty leaves with zero-size boxes
-            uint32_t slot = (uint32_t)(S.nodes.size() / 4);
-            S.nodes.resize(S.nodes.size() + 4);
-            uint32_t li = (uint32_t)S.leaves.size();
-            S.leaves.push_back(U2{0u, (CRT_PRIM_SPHERE << CRT_LEAF_TYPE_SHIFT) | 0u});
-            Box z; for (int a = 0; a < 3; ++a) { z.lo[a] = 0.0f; z.hi[a] = 0.0f; }
-            write_node(slot, z, z, ~(int32_t)li, ~(int32_t)li);
+        const uint32_t per = width == 4 ? 8u : 4u;
+        if (root < 0 || nodes[root].type >= 0) {
+            // empty tree, or a leaf root wrapped so traversal always starts at an interior node
+            uint32_t slot = (uint32_t)(S.nodes.size() / per);
+            S.nodes.resize(S.nodes.size() + per);
+            Box z;
+            for (int a = 0; a < 3; ++a) { z.lo[a] = 0.0f; z.hi[a] = 0.0f; }
+            int32_t L;
+            Box lb = z;
+            if (root < 0) {
+                uint32_t li = (uint32_t)S.leaves.size();
+                S.leaves.push_back(U2{0u, (CRT_PRIM_SPHERE << CRT_LEAF_TYPE_SHIFT) | 0u});
+                L = ~(int32_t)li;
+            } else {
+                root_area = std::max(nodes[root].box.area(), 1e-30f);
+                L = leaf_ref(nodes[root]);
+                lb = nodes[root].box;
+            }
+            if (width == 4) {
+                Box bs[4] = {lb, z, z, z};
+                int32_t refs[4] = {L, CRT_EMPTY_CHILD, CRT_EMPTY_CHILD, CRT_EMPTY_CHILD};
+                write_node4(slot, bs, refs);
+            } else {
+                write_node(slot, lb, lb, L, L);
+            }
             max_depth = std::max(max_depth, 1u);
             return slot;
         }
         root_area = std::max(nodes[root].box.area(), 1e-30f);
-        if (nodes[root].type >= 0) {
-            // a leaf root is wrapped so traversal always starts at an interior node
-            uint32_t slot = (uint32_t)(S.nodes.size() / 4);
-            S.nodes.resize(S.nodes.size() + 4);
-            int32_t L = leaf_ref(nodes[root]);
-            write_node(slot, nodes[root].box, nodes[root].box, L, L);
-            max_depth = std::max(max_depth, 1u);
-            return slot;
-        }
         return (uint32_t)emit(root, 0);
     }
 };
@@ -286,7 +343,8 @@ uint64_t HostScene::device_bytes() const {
            mats.size() * 16 + texs.size() * 16 + texels.size() + light_quads.size() * 4;
 }
 
-int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, HostScene& S, std::string& err) {
+int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, int threads, HostScene& S,
+                     std::string& err) {
     auto t0 = std::chrono::steady_clock::now();
     if (!d) { err = "null scene descriptor"; return CRT_ERR_INVALID_ARGUMENT; }
     if (d->abi_version != CRT_ABI_VERSION) { err = "scene descriptor ABI version mismatch"; return CRT_ERR_INVALID_ARGUMENT; }
@@ -299,6 +357,8 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, H
     if (!need(d->num_meshes == 0 || d->meshes, "meshes pointer is null")) return CRT_ERR_INVALID_SCENE;
     if (!need(d->num_instances == 0 || d->instances, "instances pointer is null")) return CRT_ERR_INVALID_SCENE;
     S = HostScene();
+    if (bvh_width != 2 && bvh_width != 4) { err = "bvh width must be 2 or 4"; return CRT_ERR_INVALID_ARGUMENT; }
+    S.bvh_width = bvh_width;
     S.camera = d->camera;
     for (int k = 0; k < 3; ++k) S.background[k] = d->background[k];
     const uint32_t NM = d->num_materials;
@@ -488,6 +548,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, H
             Builder bb(brefs, leaf_size ? leaf_size : 4, threads);
             int32_t r = bb.build();
             Emitter em{S, bb.nodes(), brefs, nullptr, nullptr, nullptr, nullptr};
+            em.width = bvh_width;
             em.emit_tri = [&](uint32_t t) { emit_triangle(M, t, t, nullptr); };
             blas_root[m] = em.emit_root(r);
             blas_depth = std::max(blas_depth, em.max_depth);
@@ -517,6 +578,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, H
     Builder tb(refs, leaf_size ? leaf_size : 4, threads);
     int32_t troot = tb.build();
     Emitter em{S, tb.nodes(), refs, nullptr, nullptr, nullptr, nullptr};
+    em.width = bvh_width;
     em.emit_tri = [&](uint32_t i) {
         const TriSrc& s = wtris[i];
         emit_triangle(d->meshes[s.mesh], s.tri, s.gid, nullptr);
@@ -573,7 +635,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, H
     S.num_quads = d->num_quads;
     if (!S.tri_normals.empty() && S.tri_normals.size() < S.tris.size()) S.tri_normals.resize(S.tris.size(), F4{0, 0, 0, 0});
     if (!S.tri_uvs.empty() && S.tri_uvs.size() < S.tris.size()) S.tri_uvs.resize(S.tris.size(), F2{0, 0});
-    S.stats.nodes = S.nodes.size() / 4;
+    S.stats.nodes = S.nodes.size() / (bvh_width == 4 ? 8 : 4);
     S.stats.leaves = S.leaves.size();
     S.stats.prims = refs.size();
     S.stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -587,7 +649,8 @@ namespace crt {
 // Structural invariants of a built scene: every child box contains the primitives below it, every
 // primitive record belongs to exactly one leaf, every instance's BLAS root is an interior node.
 int check_host_scene(const HostScene& S, std::string& err) {
-    const size_t nn = S.nodes.size() / 4;
+    const size_t per = S.bvh_width == 4 ? 8 : 4;
+    const size_t nn = S.nodes.size() / per;
     std::vector<uint32_t> cover_tri(S.tris.size() / 3, 0), cover_sph(S.spheres.size(), 0), cover_quad(S.quads.size() / 5, 0),
         cover_inst(S.insts.size() / 4, 0);
     auto u32 = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
@@ -606,6 +669,23 @@ int check_host_scene(const HostScene& S, std::string& err) {
         if (ref >= 0) {
             if ((size_t)ref >= nn) { err = "child node index out of range"; return false; }
             if (visited[ref]++) { err = "node reached twice"; return false; }
+            if (per == 8) {
+                const F4* np = &S.nodes[8 * (size_t)ref];
+                const float* L = &np[0].x;
+                int32_t rf[4] = {(int32_t)u32(np[6].x), (int32_t)u32(np[6].y), (int32_t)u32(np[6].z), (int32_t)u32(np[6].w)};
+                int32_t prev = 0x7ffffffe;
+                for (int k = 0; k < 4; ++k) {
+                    if (rf[k] == CRT_EMPTY_CHILD) continue;
+                    float cl[3], ch[3];
+                    for (int a = 0; a < 3; ++a) { cl[a] = L[8 * a + k]; ch[a] = L[8 * a + 4 + k]; }
+                    B bk{{cl[0], cl[1], cl[2]}, {ch[0], ch[1], ch[2]}};
+                    if (lo && !inside(bk, lo, hi)) { err = "child box outside parent box"; return false; }
+                    if (rf[k] == prev && rf[k] < 0) continue;
+                    prev = rf[k];
+                    if (!walk(rf[k], cl, ch, depth + 1)) return false;
+                }
+                return true;
+            }
             const F4* np = &S.nodes[4 * (size_t)ref];
             float l0[3] = {np[0].x, np[0].z, np[2].x}, h0[3] = {np[0].y, np[0].w, np[2].y};
             float l1[3] = {np[1].x, np[1].z, np[2].z}, h1[3] = {np[1].y, np[1].w, np[2].w};

@@ -42,6 +42,7 @@ struct HostScene {
     std::vector<uint8_t> texels;
     std::vector<uint32_t> light_quads;
     uint32_t root = 0;
+    uint32_t bvh_width = 4;       // 2: 64-B binary nodes, 4: 128-B four-wide SoA nodes
     float background[3] = {0, 0, 0};
     crt_camera camera{};
     uint32_t num_lights = 0, num_spheres = 0, num_quads = 0, num_instances = 0;
@@ -50,8 +51,10 @@ struct HostScene {
     uint64_t device_bytes() const;
 };
 
-// Flattens and builds. leaf_size = max primitives per leaf (instances always get their own leaf).
-int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, int threads, HostScene& out, std::string& err);
+// Flattens and builds. leaf_size = max primitives per leaf (instances always get their own leaf);
+// bvh_width = 2 or 4 children per node (the binary SAH tree is collapsed for 4).
+int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, int threads, HostScene& out,
+                     std::string& err);
 
 // Structural BVH invariants (containment, exact leaf coverage, reachability).
 int check_host_scene(const HostScene& S, std::string& err);

@@ -39,7 +39,7 @@ def render_desc(width, height, spp, max_depth, seed=1, rng="pcg", sample_begin=0
 class Renderer:
     """One device context: resident scenes, path-state buffers and the serving loop."""
 
-    def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0):
+    def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0):
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -47,6 +47,7 @@ class Renderer:
         cfg.max_paths = int(max_paths)
         cfg.count_work = 1 if count_work else 0
         cfg.leaf_size = int(leaf_size)
+        cfg.bvh_width = int(bvh_width)
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -101,6 +102,11 @@ class Renderer:
         c = A.crt_work_counters()
         A.check(self.lib.crt_last_counters(self.h, C.byref(c)))
         return {k: getattr(c, k) for k, _ in A.crt_work_counters._fields_}
+
+    def memory(self):
+        m = A.crt_memory_info()
+        A.check(self.lib.crt_memory_get_info(self.h, C.byref(m)))
+        return {k: getattr(m, k) for k, _ in A.crt_memory_info._fields_}
 
     def extend_launches(self):
         return self.counters()["extend_launches"]

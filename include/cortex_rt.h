@@ -219,6 +219,7 @@ typedef struct crt_config {
     uint32_t block_size;        /* threads per block of the stage kernels; 0 = default (256) */
     uint32_t count_work;        /* 1: use the counting traversal kernel (fills node_visits/prim_tests) */
     uint32_t leaf_size;         /* BVH max primitives per leaf; 0 = default (4) */
+    uint32_t bvh_width;         /* children per BVH node, 2 or 4; 0 = default (4) */
 } crt_config;
 
 int crt_create(const crt_config* cfg, crt_context** out);
@@ -239,7 +240,7 @@ int crt_scene_get_info(crt_context* ctx, crt_scene_id scene, crt_scene_info* inf
 /* Host-only: flattens and builds a scene without a device, checks the BVH invariants
  * (containment, every primitive in exactly one leaf, reachability) and reports its statistics.
  * Same validation errors as crt_scene_upload (CRT_ERR_INVALID_SCENE for dangling indices). */
-int crt_scene_build_check(const crt_scene_desc* desc, uint32_t leaf_size, crt_scene_info* info);
+int crt_scene_build_check(const crt_scene_desc* desc, uint32_t leaf_size, uint32_t bvh_width, crt_scene_info* info);
 
 /* Blocking render of one request (the launchGPU replacement). */
 int crt_render(crt_context* ctx, crt_scene_id scene, const crt_render_desc* rd, crt_image* out,
@@ -293,6 +294,13 @@ typedef struct crt_work_counters {
     uint64_t extend_launches, passes;
 } crt_work_counters;
 int crt_last_counters(crt_context* ctx, crt_work_counters* out);
+
+/* HBM accounting of a context: the admission budget, the bytes reserved by the context itself
+ * (path-state pools, scene table, traversal overflow stacks), resident scenes and admitted requests. */
+typedef struct crt_memory_info {
+    uint64_t budget, reserved, scenes, requests;
+} crt_memory_info;
+int crt_memory_get_info(crt_context* ctx, crt_memory_info* out);
 
 /* Sizes of the public structs as compiled into the library (binding self-check). */
 int crt_struct_sizes(uint64_t* sizes, uint32_t n);

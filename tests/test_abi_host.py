@@ -37,10 +37,10 @@ def test_struct_layout_matches_ctypes():
         assert sizes[k] == C.sizeof(s), s.__name__
 
 
-def _check(scene, leaf=4):
+def _check(scene, leaf=4, width=4):
     desc, keep = scene.to_desc()
     info = A.crt_scene_info()
-    rc = A.load().crt_scene_build_check(C.byref(desc), leaf, C.byref(info))
+    rc = A.load().crt_scene_build_check(C.byref(desc), leaf, width, C.byref(info))
     return rc, {k: getattr(info, k) for k, _ in A.crt_scene_info._fields_}
 
 
@@ -55,9 +55,17 @@ def test_bvh_invariants_box_and_large():
 
 
 @pytest.mark.parametrize("leaf", [1, 2, 4, 8])
-def test_bvh_invariants_leaf_sizes(leaf):
-    rc, _ = _check(scenes.large_scene(levels=3, n_spheres=100), leaf)
+@pytest.mark.parametrize("width", [2, 4])
+def test_bvh_invariants_leaf_sizes(leaf, width):
+    rc, _ = _check(scenes.large_scene(levels=3, n_spheres=100), leaf, width)
     assert rc == 0, A.load().crt_last_error()
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_bvh_invariants_small_and_instanced(width):
+    for s in (scenes.box_scene(), Scene(), scenes.instanced_scene(n_instances=20, n_textures=1, tex_size=8, nu=6, nv=4)):
+        rc, _ = _check(s, 4, width)
+        assert rc == 0, A.load().crt_last_error()
 
 
 def test_bvh_invariants_instanced():
@@ -100,7 +108,9 @@ def test_invalid_scenes_are_rejected():
     assert rc == A.CRT_ERR_INVALID_SCENE
     desc, keep = Scene().to_desc()
     desc.abi_version = 99
-    assert lib.crt_scene_build_check(C.byref(desc), 4, None) == A.CRT_ERR_INVALID_ARGUMENT
+    assert lib.crt_scene_build_check(C.byref(desc), 4, 4, None) == A.CRT_ERR_INVALID_ARGUMENT
+    desc, keep = Scene().to_desc()
+    assert lib.crt_scene_build_check(C.byref(desc), 4, 3, None) == A.CRT_ERR_INVALID_ARGUMENT
 
 
 def test_empty_scene_builds():
