@@ -141,16 +141,21 @@ class crt_memory_info(C.Structure):
     _fields_ = [("budget", C.c_uint64), ("reserved", C.c_uint64), ("scenes", C.c_uint64), ("requests", C.c_uint64)]
 
 
+class crt_obj_mesh(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("num_vertices", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("num_triangles", C.c_uint32), ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float))]
+
+
 STRUCTS_IN_ABI_ORDER = [crt_camera, crt_material, crt_texture, crt_uvmap, crt_sphere, crt_quad, crt_area_light,
                         crt_mesh, crt_instance, crt_scene_desc, crt_render_desc, crt_image, crt_stats, crt_config,
                         crt_scene_info, crt_batch_item, crt_request_timing, crt_work_counters,
-                        crt_memory_info]
+                        crt_memory_info, crt_obj_mesh]
 
 # Every symbol declared in include/cortex_rt.h.
 EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "crt_scene_upload", "crt_scene_release",
             "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
             "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
-            "crt_memory_get_info", "crt_struct_sizes"]
+            "crt_memory_get_info", "crt_obj_load", "crt_obj_free", "crt_struct_sizes"]
 
 _P = C.POINTER
 _SIGS = {
@@ -174,6 +179,8 @@ _SIGS = {
     "crt_request_get_timing": ([C.c_void_p, C.c_uint64, _P(crt_request_timing)], C.c_int),
     "crt_last_counters": ([C.c_void_p, _P(crt_work_counters)], C.c_int),
     "crt_memory_get_info": ([C.c_void_p, _P(crt_memory_info)], C.c_int),
+    "crt_obj_load": ([C.c_char_p, _P(crt_obj_mesh)], C.c_int),
+    "crt_obj_free": ([_P(crt_obj_mesh)], C.c_int),
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
 }
 

@@ -128,6 +128,27 @@ class Scene:
         self.meshes.append(m)
         return len(self.meshes) - 1
 
+    def load_obj(self, path, material, use_smoothing=True, transform=None):
+        """Adds an OBJ mesh read by the library's loader (crt_obj_load). use_smoothing=False keeps
+        flat normals like the reference's objMesh use_smoothing flag; `transform` (3x4) places it."""
+        lib = A.load()
+        m = A.crt_obj_mesh()
+        A.check(lib.crt_obj_load(str(path).encode(), C.byref(m)))
+        try:
+            nv, nt = m.num_vertices, m.num_triangles
+            pos = np.ctypeslib.as_array(m.positions, (nv * 3,)).reshape(-1, 3).copy() if nv else np.zeros((0, 3), np.float32)
+            idx = np.ctypeslib.as_array(m.indices, (nt * 3,)).reshape(-1, 3).copy() if nt else np.zeros((0, 3), np.uint32)
+            nrm = np.ctypeslib.as_array(m.normals, (nv * 3,)).reshape(-1, 3).copy() if m.normals else None
+            uv = np.ctypeslib.as_array(m.uvs, (nv * 2,)).reshape(-1, 2).copy() if m.uvs else None
+        finally:
+            lib.crt_obj_free(C.byref(m))
+        if transform is not None:
+            x = np.asarray(transform, np.float64).reshape(3, 4)
+            pos = (pos.astype(np.float64) @ x[:, :3].T + x[:, 3]).astype(np.float32)
+            if nrm is not None:
+                nrm = (nrm.astype(np.float64) @ np.linalg.inv(x[:, :3])).astype(np.float32)
+        return self.mesh(pos, idx, material, normals=nrm if use_smoothing else None, uvs=uv)
+
     def instance(self, mesh, object_to_world, material_override=-1):
         x = np.asarray(object_to_world, dtype=np.float32).reshape(-1)
         if x.size == 16:

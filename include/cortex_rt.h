@@ -285,6 +285,20 @@ typedef struct crt_request_timing {
 } crt_request_timing;
 int crt_request_get_timing(crt_context* ctx, crt_request_id req, crt_request_timing* out);
 
+/* ------------------------------------------------------------------ mesh ingestion */
+/* Wavefront OBJ reader (replaces the reference's tinyobjloader path, core/src/utilities/
+ * mesh_loader.cuh:94-146 + core/src/gpu.cu:1776-1878): polygons are fanned into triangles, each
+ * distinct (position, texcoord, normal) corner becomes one vertex, negative indices are resolved;
+ * normals / uvs are NULL unless every corner has one. Host only; free with crt_obj_free. */
+typedef struct crt_obj_mesh {
+    float* positions; uint32_t num_vertices;
+    uint32_t* indices; uint32_t num_triangles;
+    float* normals;
+    float* uvs;
+} crt_obj_mesh;
+int crt_obj_load(const char* path, crt_obj_mesh* out);
+int crt_obj_free(crt_obj_mesh* mesh);
+
 /* ------------------------------------------------------------------ introspection */
 /* Work counters of the last crt_render / crt_render_batch on this context. The traversal counts
  * (node_visits .. inst_visits) are filled only by a context created with count_work = 1. */
