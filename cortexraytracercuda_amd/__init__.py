@@ -5,7 +5,7 @@ package holds the ctypes front end, the scene description and the synthetic work
 """
 from . import _abi
 from .scene import Scene
-from .renderer import Renderer, launch_gpu, render_desc, write_png, pixel_lines
+from .renderer import Renderer, SceneFile, launch_gpu, read_scene_file, render_desc, write_png, pixel_lines
 from . import scenes
 
-__all__ = ["Scene", "Renderer", "launch_gpu", "render_desc", "write_png", "pixel_lines", "scenes", "_abi"]
+__all__ = ["Scene", "SceneFile", "read_scene_file", "Renderer", "launch_gpu", "render_desc", "write_png", "pixel_lines", "scenes", "_abi"]

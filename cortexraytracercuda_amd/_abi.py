@@ -146,16 +146,23 @@ class crt_obj_mesh(C.Structure):
                 ("num_triangles", C.c_uint32), ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float))]
 
 
+class crt_scenefile_info(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("aspect_ratio", C.c_float), ("unsupported", C.c_uint32), ("skipped", C.c_uint32),
+                ("output_path", C.c_char * 512)]
+
+
 STRUCTS_IN_ABI_ORDER = [crt_camera, crt_material, crt_texture, crt_uvmap, crt_sphere, crt_quad, crt_area_light,
                         crt_mesh, crt_instance, crt_scene_desc, crt_render_desc, crt_image, crt_stats, crt_config,
                         crt_scene_info, crt_batch_item, crt_request_timing, crt_work_counters,
-                        crt_memory_info, crt_obj_mesh]
+                        crt_memory_info, crt_obj_mesh, crt_scenefile_info]
 
 # Every symbol declared in include/cortex_rt.h.
 EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "crt_scene_upload", "crt_scene_release",
             "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
             "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
-            "crt_memory_get_info", "crt_obj_load", "crt_obj_free", "crt_struct_sizes"]
+            "crt_memory_get_info", "crt_obj_load", "crt_obj_free", "crt_scenefile_load",
+            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes"]
 
 _P = C.POINTER
 _SIGS = {
@@ -181,6 +188,9 @@ _SIGS = {
     "crt_memory_get_info": ([C.c_void_p, _P(crt_memory_info)], C.c_int),
     "crt_obj_load": ([C.c_char_p, _P(crt_obj_mesh)], C.c_int),
     "crt_obj_free": ([_P(crt_obj_mesh)], C.c_int),
+    "crt_scenefile_load": ([C.c_char_p, _P(C.c_void_p)], C.c_int),
+    "crt_scenefile_desc": ([C.c_void_p, _P(crt_scene_desc), _P(crt_scenefile_info)], C.c_int),
+    "crt_scenefile_close": ([C.c_void_p], C.c_int),
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
 }
 

@@ -7,7 +7,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcortex_rt.so")
-SOURCES = ["kernels.hip", "scene_build.cpp", "api.cpp", "serve.cpp", "obj_loader.cpp"]
+SOURCES = ["kernels.hip", "scene_build.cpp", "api.cpp", "serve.cpp", "obj_loader.cpp", "scenefile.cpp"]
+CLI = os.path.join(HERE, "crt_render")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 (v_pk_*_f32) is disabled for every device compile
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -49,8 +50,13 @@ def build(verbose=False, jobs=4):
         sys.stderr.write("\n".join(failed) + "\n")
         raise RuntimeError("compilation failed")
     tmp = LIB + ".tmp"
-    _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread", "-lz"])
     os.replace(tmp, LIB)
+    # command-line renderer with the reference executable's flags, linked against the library
+    tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "crt_cli.cpp"), "-o", tmp,
+          "-L" + HERE, "-lcortex_rt", "-Wl,-rpath,$ORIGIN", "-lz"])
+    os.replace(tmp, CLI)
     return LIB
 
 

@@ -686,7 +686,8 @@ int crt_struct_sizes(uint64_t* sizes, uint32_t n) {
     uint64_t v[] = {sizeof(crt_camera), sizeof(crt_material), sizeof(crt_texture), sizeof(crt_uvmap), sizeof(crt_sphere),
                     sizeof(crt_quad), sizeof(crt_area_light), sizeof(crt_mesh), sizeof(crt_instance), sizeof(crt_scene_desc),
                     sizeof(crt_render_desc), sizeof(crt_image), sizeof(crt_stats), sizeof(crt_config), sizeof(crt_scene_info),
-                    sizeof(crt_batch_item), sizeof(crt_request_timing), sizeof(crt_work_counters), sizeof(crt_memory_info), sizeof(crt_obj_mesh)};
+                    sizeof(crt_batch_item), sizeof(crt_request_timing), sizeof(crt_work_counters), sizeof(crt_memory_info), sizeof(crt_obj_mesh),
+                    sizeof(crt_scenefile_info)};
     uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
     for (uint32_t k = 0; k < n && k < m; ++k) sizes[k] = v[k];
     return (int)m;

@@ -77,6 +77,15 @@ class Renderer:
         del keep
         return sid.value
 
+    def upload_scene_file(self, path):
+        """Loads a reference `.scene` file with the library's reader and uploads it.
+        Returns (scene id, file info dict)."""
+        desc, info = read_scene_file(path)
+        sid = C.c_uint32()
+        A.check(self.lib.crt_scene_upload(self.h, C.byref(desc._desc), C.byref(sid)))
+        desc.close()
+        return sid.value, info
+
     def release(self, sid):
         A.check(self.lib.crt_scene_release(self.h, sid))
 
@@ -174,6 +183,42 @@ class Renderer:
         t = A.crt_request_timing()
         A.check(self.lib.crt_request_get_timing(self.h, rid, C.byref(t)))
         return {k: getattr(t, k) for k, _ in A.crt_request_timing._fields_}
+
+
+class SceneFile:
+    """A `.scene` file parsed by the library (crt_scenefile_load); owns the descriptor's memory."""
+
+    def __init__(self, path):
+        self.lib = A.load()
+        self.h = C.c_void_p()
+        A.check(self.lib.crt_scenefile_load(str(path).encode(), C.byref(self.h)))
+        self._desc = A.crt_scene_desc()
+        self._info = A.crt_scenefile_info()
+        A.check(self.lib.crt_scenefile_desc(self.h, C.byref(self._desc), C.byref(self._info)))
+
+    def to_desc(self):
+        return self._desc, [self]
+
+    def info(self):
+        d = {k: getattr(self._info, k) for k, _ in A.crt_scenefile_info._fields_}
+        d["output_path"] = d["output_path"].decode()
+        return d
+
+    def close(self):
+        if self.h:
+            self.lib.crt_scenefile_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_scene_file(path):
+    f = SceneFile(path)
+    return f, f.info()
 
 
 def write_png(path, rgb):

@@ -299,6 +299,24 @@ typedef struct crt_obj_mesh {
 int crt_obj_load(const char* path, crt_obj_mesh* out);
 int crt_obj_free(crt_obj_mesh* mesh);
 
+/* ------------------------------------------------------------------ scene files */
+/* Reader of the reference's libconfig `.scene` files (core/src/scenes/scene_loader.cpp:21-120,
+ * scene_builder, scene_manager::load_scene) for the render-path features: image, camera,
+ * directional lights, solid/checker/PNG image textures, Lambertian/metal/glass materials, spheres,
+ * quads, boxes and OBJ meshes. Everything else is counted in `unsupported` and skipped. The
+ * descriptor's pointers stay valid until crt_scenefile_close. */
+typedef struct crt_scenefile crt_scenefile;
+typedef struct crt_scenefile_info {
+    uint32_t width, height, spp, max_depth;   /* image {} values (the reference CLI overrides them) */
+    float aspect_ratio;
+    uint32_t unsupported;                     /* entries outside the render path, skipped */
+    uint32_t skipped;                         /* primitives whose material name was not found */
+    char output_path[512];
+} crt_scenefile_info;
+int crt_scenefile_load(const char* path, crt_scenefile** out);
+int crt_scenefile_desc(crt_scenefile* file, crt_scene_desc* desc, crt_scenefile_info* info);
+int crt_scenefile_close(crt_scenefile* file);
+
 /* ------------------------------------------------------------------ introspection */
 /* Work counters of the last crt_render / crt_render_batch on this context. The traversal counts
  * (node_visits .. inst_visits) are filled only by a context created with count_work = 1. */
