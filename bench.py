@@ -26,10 +26,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one closest-hit query in k_extend (declared layout, crt_layout.h):
 #   ray 32 (ray_o + ray_d) + ids 8 + queue index 4 + hit record 16 + hit instance 4 = 64 B of state
-#   + 64 B per interior node visit (two child boxes + refs), 8 B per leaf record,
+#   + 64 B per BVH2 / 128 B per BVH4 interior node visit (child boxes + refs), 8 B per leaf record,
 #   + 48 B per triangle, 16 B per sphere, 80 B per quad/light, 64 B per instance record.
 B_STATE = 64
-B_NODE, B_LEAF, B_TRI, B_SPHERE, B_QUAD, B_INST = 64, 8, 48, 16, 80, 64
+B_LEAF, B_TRI, B_SPHERE, B_QUAD, B_INST = 8, 48, 16, 80, 64   # node bytes come from the scene (64 or 128)
 
 
 def log(*a):
@@ -124,11 +124,162 @@ def serving(seed=11, n=48, rate_hz=60.0):
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "aggregate_mrays_s": rays / wall / 1e6}
 
 
+def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False):
+    """k_extend roofline for one render of the workload: algorithmic bytes from the counting
+    variant of the kernel (same deterministic workload) over the HIP-event time of the launches."""
+    from cortexraytracercuda_amd import Renderer
+    rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths)
+    sidc = rc.upload(scene)
+    node_b = rc.info(sidc)["node_bytes"]
+    cnt = dict.fromkeys(["rays", "node_visits", "leaf_visits", "tri_tests", "sphere_tests", "quad_tests", "inst_visits"], 0)
+    ext_ms = 0.0
+    launches = 0
+    ranges = [(k, 1) for k in range(cfg["spp"])] if progressive else [(0, 0)]
+    for sb, sc in ranges:
+        rc.render(sidc, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb,
+                  sample_count=sc, want_accum=False)
+        c = rc.counters()
+        for k in cnt:
+            cnt[k] += c[k]
+        _, _, stt = timed_renderer.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1,
+                                          sample_begin=sb, sample_count=sc, want_accum=False, time_stages=True)
+        ext_ms += stt["extend_ms"]
+        launches += timed_renderer.extend_launches()
+    rc.close()
+    alg = (B_STATE * cnt["rays"] + node_b * cnt["node_visits"] + B_LEAF * cnt["leaf_visits"] + B_TRI * cnt["tri_tests"] +
+           B_SPHERE * cnt["sphere_tests"] + B_QUAD * cnt["quad_tests"] + B_INST * cnt["inst_visits"])
+    achieved = alg / (ext_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "k_extend", "launches": launches, "avg_launch_ms": ext_ms / max(launches, 1),
+            "bytes_per_launch": alg / max(launches, 1), "bytes_per_ray": alg / max(cnt["rays"], 1),
+            "node_bytes": node_b, "node_visits_per_ray": cnt["node_visits"] / max(cnt["rays"], 1),
+            "prim_tests_per_ray": (cnt["tri_tests"] + cnt["sphere_tests"] + cnt["quad_tests"]) / max(cnt["rays"], 1)}
+
+
+def single_render_bench(scene, cfg, args, workload, progressive=False):
+    from cortexraytracercuda_amd import Renderer
+    r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths)
+    t0 = time.perf_counter()
+    sid = r.upload(scene)
+    info = r.info(sid)
+    log("upload+build %.2f s %s" % (time.perf_counter() - t0, info))
+    ranges = [(k, 1) for k in range(cfg["spp"])] if progressive else [(0, 0)]
+
+    def one_step():
+        rays = 0
+        gpu = 0.0
+        for sb, sc in ranges:
+            _, _, st = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb,
+                                sample_count=sc, want_accum=False)
+            rays += st["rays"]
+            gpu += st["gpu_ms"]
+        return rays, gpu, st
+
+    for _ in range(args.warmup):
+        one_step()
+        log("warmup done")
+    rays = 0
+    gpu_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rr, gg, st = one_step()
+        rays += rr
+        gpu_ms += gg
+        log("step %d: %.1f ms GPU, %d rays" % (k, gg, rr))
+    wall = time.perf_counter() - t0
+    value = rays / wall / 1e6
+    roof = traversal_roofline(scene, cfg, args, r, sid, progressive)
+    r.close()
+    return {"metric": "Mrays/s on 1 MI355X + p50/p99 request latency, batched vs plain, equal RMSE",
+            "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": dict(workload=workload, **cfg),
+            "rays_per_step": rays // args.steps, "segments_per_s": value * 1e6, "gpu_ms_per_step": gpu_ms / args.steps,
+            "bvh": {"nodes": info["num_nodes"], "width": info["bvh_width"], "depth": info["max_depth"],
+                    "build_ms": info["build_ms"], "upload_ms": info["upload_ms"], "device_bytes": info["device_bytes"]},
+            "roofline": roof}
+
+
+def batch_workload(seed=4, n=64, tri_range=(1e4, 2e6), res_choices=(256, 384, 512, 768, 1024), spp_range=(1, 64)):
+    """Config 4: n scenes of uneven size (log-uniform triangle counts), resolutions 256^2-1024^2,
+    1-64 spp: one crt_render per request vs one crt_render_batch over all of them."""
+    from cortexraytracercuda_amd import Renderer, scenes
+    rng = np.random.default_rng(seed)
+    r = Renderer()
+    jobs = []
+    tg = time.perf_counter()
+    tris_total = 0
+    for k in range(n):
+        tris = int(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]))))
+        sid = r.upload(scenes.sphere_mesh_scene(tris, seed=k))
+        tris_total += tris
+        res = int(rng.choice(res_choices))
+        spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
+        jobs.append(dict(sid=sid, width=res, height=res, spp=spp, max_depth=8, seed=1000 + k))
+    log("batch workload: %d scenes, %d triangles, built in %.1f s" % (n, tris_total, time.perf_counter() - tg))
+    r.render(jobs[0]["sid"], 64, 64, 1, 8)
+    r.render_batch([dict(jobs[0], width=64, height=64)])
+    t0 = time.perf_counter()
+    plain_rays = 0
+    plain_imgs = []
+    for j in jobs:
+        _, acc, st = r.render(j["sid"], j["width"], j["height"], j["spp"], j["max_depth"], seed=j["seed"])
+        plain_rays += st["rays"]
+        plain_imgs.append(acc)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    outs, st = r.render_batch(jobs)
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(a, b[1]) for a, b in zip(plain_imgs, outs))
+    r.close()
+    return {"requests": n, "triangles_total": tris_total, "plain_mrays_s": plain_rays / t_plain / 1e6,
+            "batched_mrays_s": st["rays"] / t_batch / 1e6, "speedup": (st["rays"] / t_batch) / (plain_rays / t_plain),
+            "identical_accumulation": bool(same), "plain_s": t_plain, "batched_s": t_batch}
+
+
+def serving_workload(seed=5, n=96, rate_hz=20.0, pool_tris=(1e4, 5e4, 2e5, 5e5, 1e6, 2e6), res_choices=(256, 512, 1024, 2048),
+                     spp_choices=(1, 4, 16, 64, 256), budget_gb=200.0):
+    """Config 5: Poisson arrivals; each request picks a resident scene (10k-2M triangles), a resolution
+    256^2-2048^2 and 1-256 spp; admission against an HBM budget; latency = completion - arrival."""
+    from cortexraytracercuda_amd import Renderer, scenes
+    rng = np.random.default_rng(seed)
+    r = Renderer(hbm_budget=int(budget_gb * 1e9))
+    pool = [r.upload(scenes.sphere_mesh_scene(int(t), seed=i)) for i, t in enumerate(pool_tris)]
+    reqs = []
+    for k in range(n):
+        res = int(rng.choice(res_choices))
+        reqs.append(dict(sid=pool[int(rng.integers(0, len(pool)))], width=res, height=res,
+                         spp=int(rng.choice(spp_choices)), max_depth=8, seed=5000 + k))
+    gaps = rng.exponential(1.0 / rate_hz, n)
+    r.render(pool[0], 64, 64, 1, 8)
+    t0 = time.perf_counter()
+    ids = []
+    for k, q in enumerate(reqs):
+        target = t0 + gaps[:k + 1].sum()
+        while time.perf_counter() < target:
+            time.sleep(0.0005)
+        ids.append(r.submit(q["sid"], q["width"], q["height"], q["spp"], q["max_depth"], seed=q["seed"]))
+    lat = []
+    rays = 0
+    for q, i in zip(reqs, ids):
+        r.wait(i, q["width"], q["height"], want_accum=False)
+        tm = r.timing(i)
+        lat.append(tm["done_s"] - tm["submit_s"])
+        rays += tm["rays"]
+    wall = time.perf_counter() - t0
+    r.close()
+    lat = np.array(lat)
+    return {"requests": n, "rate_hz": rate_hz, "budget_gb": budget_gb, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "aggregate_mrays_s": rays / wall / 1e6, "wall_s": wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="large", choices=["large", "box", "instanced", "batch", "serving"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller image for a fast check")
@@ -139,63 +290,34 @@ def main():
         print("bench.py measures one GPU: every box here has one MI355X, and this path is single-GPU "
               "(multi-GPU sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    from cortexraytracercuda_amd import Renderer
+    from cortexraytracercuda_amd import scenes
+    if args.config == "batch":
+        print(json.dumps({"config": "batch", "batched": batch_workload()}), flush=True)
+        return
+    if args.config == "serving":
+        print(json.dumps({"config": "serving", "serving": serving_workload()}), flush=True)
+        return
     t_gen = time.perf_counter()
-    scene, cfg = large_workload()
+    progressive = False
+    if args.config == "box":
+        scene, cfg = scenes.box_scene(), dict(width=256, height=256, spp=16, max_depth=8)
+        workload = "small box scene: 12 wall triangles, area light, 3 spheres, 256x256, 16 spp, depth 8"
+    elif args.config == "instanced":
+        scene = scenes.instanced_scene(seed=1, n_instances=4096, n_textures=4, tex_size=2048, nu=50, nv=50)
+        cfg = dict(width=3840, height=2160, spp=16, max_depth=12)
+        progressive = True
+        workload = ("instanced scene: 4,096 instances of a 5,000-triangle displaced torus (20,480,000 triangles), "
+                    "four 2048^2 RGBA8 textures, 3840x2160, 16 progressive 1-spp passes, depth 12")
+    else:
+        scene, cfg = large_workload()
+        workload = ("large scene: icosphere subdivided to 1,310,720 triangles displaced by 3-octave value noise "
+                    "(amplitude 0.05) + 500 random spheres (60/30/10 Lambertian/metal/dielectric), 1920x1080, "
+                    "64 spp, depth 10, keyed PCG seed 1")
     if args.quick:
-        cfg = dict(width=480, height=270, spp=16, max_depth=10)
+        cfg = dict(cfg, width=cfg["width"] // 4, height=cfg["height"] // 4, spp=min(cfg["spp"], 16))
     log("scene generated in %.1f s" % (time.perf_counter() - t_gen))
-    r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths)
-    t0 = time.perf_counter()
-    sid = r.upload(scene)
-    info = r.info(sid)
-    log("upload+build %.2f s %s" % (time.perf_counter() - t0, info))
-    for _ in range(args.warmup):
-        r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, want_accum=False)
-        log("warmup done")
-    rays = 0
-    gpu_ms = 0.0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        rgb, _, st = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, want_accum=False)
-        rays += st["rays"]
-        gpu_ms += st["gpu_ms"]
-        log("step %d: %.1f ms, %d rays" % (k, st["host_ms"], st["rays"]))
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
-    value = rays / wall / 1e6
-    # roofline of the traversal kernel, measured outside the timed region
-    rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths)
-    sidc = rc.upload(scene)
-    _, _, stc = rc.render(sidc, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, want_accum=False)
-    cnt = rc.counters()
-    rc.close()
-    _, _, stt = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, want_accum=False,
-                         time_stages=True)
-    launches = r.extend_launches()
-    r.close()
-    alg_bytes = (B_STATE * cnt["rays"] + B_NODE * cnt["node_visits"] + B_LEAF * cnt["leaf_visits"] +
-                 B_TRI * cnt["tri_tests"] + B_SPHERE * cnt["sphere_tests"] + B_QUAD * cnt["quad_tests"] +
-                 B_INST * cnt["inst_visits"])
-    achieved = alg_bytes / (stt["extend_ms"] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "k_extend", "launches": launches,
-            "avg_launch_ms": stt["extend_ms"] / max(launches, 1), "bytes_per_launch": alg_bytes / max(launches, 1),
-            "bytes_per_ray": alg_bytes / max(cnt["rays"], 1), "node_visits_per_ray": cnt["node_visits"] / max(cnt["rays"], 1)}
-    out = {"metric": "Mrays/s on 1 MI355X + p50/p99 request latency, batched vs plain, equal RMSE",
-           "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "large scene: displaced icosphere 1,310,720 tris + 500 spheres, %dx%d, %d spp, "
-                                  "depth %d, keyed PCG seed 1" % (cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"]),
-                      "triangles": 1310720, "spheres": 500, "width": cfg["width"], "height": cfg["height"],
-                      "spp": cfg["spp"], "max_depth": cfg["max_depth"]},
-           "rays_per_step": rays // args.steps, "paths_per_step": st["paths"],
-           "segments_per_s": value * 1e6, "gpu_ms_per_step": gpu_ms / args.steps,
-           "bvh": {"nodes": info["num_nodes"], "depth": info["max_depth"], "build_ms": info["build_ms"],
-                   "upload_ms": info["upload_ms"]},
-           "roofline": roof}
-    if not args.no_extras:
+    out = single_render_bench(scene, cfg, args, workload, progressive)
+    if args.config == "large" and not args.no_extras:
         try:
             out["batched"] = batched_vs_plain()
             log("batched: %s" % out["batched"])
@@ -206,7 +328,7 @@ def main():
             log("serving: %s" % out["serving"])
         except Exception as e:
             out["serving"] = {"error": repr(e)}
-    if not args.no_cpu:
+    if not args.no_cpu and args.config in ("large", "box"):
         out["cpu_baseline"] = cpu_baseline(scene, cfg)
         log("cpu baseline: %s" % out["cpu_baseline"])
     print(json.dumps(out), flush=True)

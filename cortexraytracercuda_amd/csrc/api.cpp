@@ -524,6 +524,8 @@ int crt_scene_get_info(crt_context* c, crt_scene_id s, crt_scene_info* info) {
     info->device_bytes = sc->bytes;
     info->max_depth = sc->host.stats.depth;
     info->num_instances = sc->host.num_instances;
+    info->bvh_width = sc->host.bvh_width;
+    info->node_bytes = sc->host.bvh_width == 4 ? 128u : 64u;
     info->build_ms = sc->host.stats.build_ms;
     info->upload_ms = sc->upload_ms;
     info->sah_cost = sc->host.stats.sah_cost;
@@ -659,6 +661,7 @@ int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, uint32_t 
     if (info) {
         info->num_nodes = H.stats.nodes; info->num_leaves = H.stats.leaves; info->num_prims = H.stats.prims;
         info->device_bytes = H.device_bytes(); info->max_depth = H.stats.depth; info->num_instances = H.num_instances;
+        info->bvh_width = H.bvh_width; info->node_bytes = H.bvh_width == 4 ? 128u : 64u;
         info->build_ms = H.stats.build_ms; info->upload_ms = 0.0; info->sah_cost = H.stats.sah_cost;
     }
     return CRT_OK;

@@ -234,6 +234,7 @@ int crt_scene_release(crt_context* ctx, crt_scene_id scene);
 typedef struct crt_scene_info {
     uint64_t num_nodes, num_leaves, num_prims, device_bytes;
     uint32_t max_depth, num_instances;
+    uint32_t bvh_width, node_bytes;          /* children per node and bytes per node record */
     double build_ms, upload_ms, sah_cost;
 } crt_scene_info;
 int crt_scene_get_info(crt_context* ctx, crt_scene_id scene, crt_scene_info* info);

@@ -189,3 +189,48 @@ def test_invalid_arguments_raise():
         r.release(sid)
         with pytest.raises(A.CrtError):
             r.release(sid)
+
+
+def _read_png_rgb(path):
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    o, idat, W, H = 8, b"", 0, 0
+    while o < len(data):
+        n = struct.unpack(">I", data[o:o + 4])[0]
+        tag = data[o + 4:o + 8]
+        if tag == b"IHDR":
+            W, H = struct.unpack(">II", data[o + 8:o + 16])
+        elif tag == b"IDAT":
+            idat += data[o + 8:o + 8 + n]
+        o += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(H, 1 + 3 * W)
+    assert np.all(raw[:, 0] == 0)
+    return raw[:, 1:].reshape(H, W, 3)
+
+
+def test_cli_renders_scene_file_like_the_library(tmp_path):
+    """The command-line renderer (reference flags, .scene input, PNG + pixel protocol) gives the
+    same image as the library called directly on the same file."""
+    import subprocess
+    from cortexraytracercuda_amd import SceneFile
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    scene = os.path.join(here, "golden", "scenes", "box_c1.scene")
+    out = tmp_path / "cli.png"
+    cmd = [os.path.join(root, "cortexraytracercuda_amd", "crt_render"), "-scene", scene, "-width", "64", "-height", "64",
+           "-ratio", "1:1", "-spp", "4", "-maxdepth", "6", "-save", str(out)]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    lines = [l for l in p.stdout.splitlines() if l.startswith("p ")]
+    assert len(lines) == 64 * 64 + 4 * 64
+    rgb_cli = _read_png_rgb(out)
+    f = SceneFile(scene)
+    with Renderer() as r:
+        sid = r.upload(f)
+        rgb, _, _ = r.render(sid, 64, 64, 4, 6, seed=2580, rng="minstd")
+    assert np.array_equal(rgb_cli, rgb)
+    # protocol: p x y r g b with y counted from the top row of the PNG
+    x, y, cr, cg, cb = map(int, lines[64 * 10 + 5].split()[1:])
+    assert (x, y) == (5, 10) and [cr, cg, cb] == list(rgb[10, 5])
