@@ -226,6 +226,8 @@ int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
     // leaf order -> global primitive id for first-hit queries, then drop the big host copies
     sc->tri_gid.resize(H.tris.size() / 3);
     for (size_t i = 0; i < sc->tri_gid.size(); ++i) std::memcpy(&sc->tri_gid[i], &H.tris[3 * i].w, 4);
+    sc->inst_gid.resize(H.insts.size() / 4);
+    for (size_t i = 0; i < sc->inst_gid.size(); ++i) std::memcpy(&sc->inst_gid[i], &H.insts[4 * i + 3].y, 4);
     sc->quad_gid.resize(H.quads.size() / 5);
     for (size_t i = 0; i < sc->quad_gid.size(); ++i) std::memcpy(&sc->quad_gid[i], &H.quads[5 * i + 3].w, 4);
     H.nodes.clear(); H.nodes.shrink_to_fit(); H.tris.clear(); H.tris.shrink_to_fit();
@@ -443,13 +445,15 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
             uint32_t cap = max_paths;
             for (Job* j : jobs) {
                 if ((int)j->rd.rng_mode != mode) continue;
-                while (cap > 0 && items.size() < kMaxPassRequests) {
+                // at most one work item per request and pass: a pixel's samples are resolved by one
+                // thread in sample order, so two items of one request never race on a pixel
+                if (cap > 0 && items.size() < kMaxPassRequests) {
                     WorkItem w;
-                    if (!j->take(cap, w)) break;
-                    cap -= w.pix_count * w.sample_count;
-                    paths += (uint64_t)w.pix_count * w.sample_count;
-                    items.push_back(w);
-                    if (mode == CRT_RNG_MINSTD) break;  // one sample per pixel per pass
+                    if (j->take(cap, w)) {
+                        cap -= w.pix_count * w.sample_count;
+                        paths += (uint64_t)w.pix_count * w.sample_count;
+                        items.push_back(w);
+                    }
                 }
             }
             if (items.empty()) break;
@@ -615,7 +619,7 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
             else gid = (int32_t)sc->quad_gid[idx];
         }
         if (prim) prim[k] = gid;
-        if (inst) inst[k] = (p == CRT_MISS || hi[k] == CRT_NO_INST) ? -1 : (int32_t)hi[k];
+        if (inst) inst[k] = (p == CRT_MISS || hi[k] == CRT_NO_INST) ? -1 : (int32_t)sc->inst_gid[hi[k]];
         if (t) t[k] = p == CRT_MISS ? 0.0f : hh[4 * k];
         if (uv) { uv[2 * k] = p == CRT_MISS ? 0.0f : hh[4 * k + 2]; uv[2 * k + 1] = p == CRT_MISS ? 0.0f : hh[4 * k + 3]; }
     }

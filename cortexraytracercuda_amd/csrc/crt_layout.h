@@ -38,7 +38,7 @@
 #define CRT_MISS 0xffffffffu
 #define CRT_HIT_TYPE(p) ((p) >> 30)
 #define CRT_HIT_INDEX(p) ((p)&0x3fffffffu)
-// hit.inst: instance id or CRT_NO_INST; bit 31 marks a pass-through re-trace result
+// hit.inst: instance record index (leaf order) or CRT_NO_INST; bit 31 marks a pass-through re-trace result
 #define CRT_NO_INST 0x7fffffffu
 #define CRT_PASSTHROUGH 0x80000000u
 

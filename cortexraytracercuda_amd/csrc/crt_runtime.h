@@ -41,6 +41,7 @@ struct DeviceScene {
     DevBuf nodes, leaves, tris, tri_normals, tri_uvs, spheres, sphere_aux, sphere_map, quads, insts, mats, texs, texels,
         light_quads;
     std::vector<uint32_t> tri_gid, quad_gid;   // leaf order -> global primitive id
+    std::vector<uint32_t> inst_gid;            // instance record (leaf order) -> caller's instance index
     crt_dev_scene dev{};
     uint32_t slot = 0;
     uint64_t bytes = 0;

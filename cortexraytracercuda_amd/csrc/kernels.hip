@@ -272,22 +272,42 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t
 
 // Closest hit over the BVH (WIDTH 2: both child boxes in a 64-B parent node; WIDTH 4: four child
 // boxes in a 128-B SoA node, children visited near-first). TWO_LEVEL enables instance leaves that
-// move the ray to object space and descend into the instance's BLAS.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
-__device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
-                                      uint32_t* lds_col, uint32_t* gstk, HitRec& best, TravCount& tc, bool& ovf) {
+// move the ray to object space and descend into the instance's BLAS. The traversal is a resumable
+// state machine: step() performs one node or leaf visit, so a persistent kernel can interleave
+// traversal steps with fetching new rays for finished lanes.
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL> struct Traverser {
     Stack<BLOCK, RS, LS, GS> st;
-    st.init(lds_col, gstk);
-    V3 o = ow, d = dw;
-    auto inv_of = [](float x) { return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x)); };
-    V3 id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
-    TriPre tp = tri_pre(d);
-    best.t = tmax; best.prim = CRT_MISS; best.inst = CRT_NO_INST; best.u = 0.0f; best.v = 0.0f;
-    uint32_t cur_inst = CRT_NO_INST;
-    bool in_blas = false;
-    int blas_level = 0;
-    int32_t node = (int32_t)S.root;
-    for (;;) {
+    V3 o, d, id, ow, dw;
+    TriPre tp;
+    HitRec best;
+    uint32_t cur_inst;
+    bool in_blas, camera;
+    int blas_level;
+    int32_t node;
+    float tmin;
+
+    static __device__ __forceinline__ float inv_of(float x) {
+        return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x));
+    }
+    __device__ __forceinline__ void set_ray(V3 ro, V3 rd) {
+        o = ro; d = rd;
+        id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
+        tp = tri_pre(d);
+    }
+    __device__ __forceinline__ void init(const SceneView& S, V3 ro, V3 rd, float t0, float t1, bool cam,
+                                         uint32_t* lds_col, uint32_t* gstk) {
+        st.init(lds_col, gstk);
+        ow = ro; dw = rd;
+        set_ray(ro, rd);
+        best.t = t1; best.prim = CRT_MISS; best.inst = CRT_NO_INST; best.u = 0.0f; best.v = 0.0f;
+        cur_inst = CRT_NO_INST;
+        in_blas = false; camera = cam;
+        blas_level = 0;
+        node = (int32_t)S.root;
+        tmin = t0;
+    }
+    // One visit; returns true when the traversal is complete (result in `best`).
+    __device__ __forceinline__ bool step(const SceneView& S, TravCount& tc) {
         bool next = true;
         if (node >= 0) {
             if (COUNT) tc.nodes++;
@@ -373,13 +393,12 @@ __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tm
                 float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
                 int4 meta = *(const int4*)(ip + 3);
                 if (COUNT) tc.insts++;
-                o = v3(r0.x * ow.x + r0.y * ow.y + r0.z * ow.z + r0.w, r1.x * ow.x + r1.y * ow.y + r1.z * ow.z + r1.w,
-                       r2.x * ow.x + r2.y * ow.y + r2.z * ow.z + r2.w);
-                d = v3(r0.x * dw.x + r0.y * dw.y + r0.z * dw.z, r1.x * dw.x + r1.y * dw.y + r1.z * dw.z,
-                       r2.x * dw.x + r2.y * dw.y + r2.z * dw.z);
-                id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
-                tp = tri_pre(d);
-                cur_inst = (uint32_t)meta.y;
+                (void)meta.y;
+                set_ray(v3(r0.x * ow.x + r0.y * ow.y + r0.z * ow.z + r0.w, r1.x * ow.x + r1.y * ow.y + r1.z * ow.z + r1.w,
+                           r2.x * ow.x + r2.y * ow.y + r2.z * ow.z + r2.w),
+                        v3(r0.x * dw.x + r0.y * dw.y + r0.z * dw.z, r1.x * dw.x + r1.y * dw.y + r1.z * dw.z,
+                           r2.x * dw.x + r2.y * dw.y + r2.z * dw.z));
+                cur_inst = first;   // instance record index (leaf order); the host maps it to the caller's id
                 in_blas = true;
                 blas_level = st.depth();
                 node = meta.x;
@@ -388,16 +407,25 @@ __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tm
         }
         if (next) {
             if (TWO_LEVEL && in_blas && st.depth() == blas_level) {
-                o = ow; d = dw;
-                id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
-                tp = tri_pre(d);
+                set_ray(ow, dw);
                 in_blas = false; cur_inst = CRT_NO_INST;
             }
-            if (st.empty()) break;
+            if (st.empty()) return true;
             node = (int32_t)st.pop();
         }
+        return false;
     }
-    ovf = st.overflow;
+};
+
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+__device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
+                                      uint32_t* lds_col, uint32_t* gstk, HitRec& best, TravCount& tc, bool& ovf) {
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
+    tr.init(S, ow, dw, tmin, tmax, camera, lds_col, gstk);
+    while (!tr.step(S, tc)) {
+    }
+    best = tr.best;
+    ovf = tr.st.overflow;
 }
 
 // ------------------------------------------------------------------ block-wide append (compaction)

@@ -138,22 +138,17 @@ void Context::server_loop() {
                     if (!r->cancel && (int)r->rd.rng_mode == mode && !r->job->scheduled_all()) order.push_back(r);
                 }
                 rr++;
+                // one work item per request and pass (a pixel's samples resolve in order in one
+                // thread, so two items of one request would race); each request gets a fair share
                 uint32_t share = order.empty() ? cap : std::max<uint32_t>(cap / (uint32_t)order.size(), 1u);
-                for (int round = 0; round < 2 && cap > 0; ++round) {
-                    for (ServeReq* r : order) {
-                        if (cap == 0 || items.size() >= kMaxPassRequests) break;
-                        uint32_t lim = round == 0 ? std::min(cap, std::max(share, r->job->npix)) : cap;
-                        uint32_t took = 0;
-                        while (took < lim && items.size() < kMaxPassRequests) {
-                            WorkItem w;
-                            if (!r->job->take(lim - took, w)) break;
-                            took += w.pix_count * w.sample_count;
-                            items.push_back(w);
-                            if (mode == CRT_RNG_MINSTD) break;
-                        }
-                        cap -= took;
-                        if (took && std::find(in_pass.begin(), in_pass.end(), r) == in_pass.end()) in_pass.push_back(r);
-                    }
+                for (ServeReq* r : order) {
+                    if (cap == 0 || items.size() >= kMaxPassRequests) break;
+                    uint32_t lim = std::min(cap, std::max(share, r->job->npix));
+                    WorkItem w;
+                    if (!r->job->take(lim, w)) continue;
+                    cap -= w.pix_count * w.sample_count;
+                    items.push_back(w);
+                    in_pass.push_back(r);
                 }
                 if (!items.empty()) {
                     double t = now_s();

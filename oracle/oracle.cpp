@@ -351,7 +351,7 @@ template <class R> static int build_scene(Scene<R>& s, const crt_scene_desc* d, 
         for (uint32_t k = 0; k < d->num_instances; ++k) {
             const crt_instance& I = d->instances[k];
             if (I.mesh >= d->num_meshes) return CRT_ERR_INVALID_SCENE;
-            int rc = add_mesh(d->meshes[I.mesh], I.object_to_world, (int32_t)k, I.material_override, gid);
+            int rc = add_mesh(d->meshes[I.mesh], I.object_to_world, (int32_t)k, I.material_override, 0);
             if (rc) return rc;
         }
     }

@@ -15,12 +15,11 @@ NT = os.cpu_count() or 4
 
 def _first_hit_check(r, s, W, H, accel):
     sid = r.upload(s)
-    prim, t = r.trace_primary(sid, W, H)
     rays = O.primary_rays(s, W, H)
-    oprim, _, ot, _ = O.trace(s, rays, camera=True, precision=64, accel=accel)
-    prim = prim.reshape(-1)
-    t = t.reshape(-1).astype(np.float64)
-    agree = prim == oprim
+    prim, inst, t, _ = r.trace_rays(sid, rays.astype(np.float32), camera=True)
+    oprim, oinst, ot, _ = O.trace(s, rays, camera=True, precision=64, accel=accel)
+    t = t.astype(np.float64)
+    agree = (prim == oprim) & (inst == oinst)
     tie = np.abs(t - ot) <= 1e-5 * np.maximum(np.abs(ot), 1e-30)
     bad = ~agree & ~tie
     hit = agree & (oprim >= 0)
