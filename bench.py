@@ -8,8 +8,10 @@ Printed JSON line (rank 0):
   value        Mrays/s over the timed steps (closest-hit queries / wall time of the renders)
   roofline     traversal kernel (k_extend): algorithmic bytes per launch / HIP-event time
   cpu_baseline reference-faithful CPU renderer (oracle/, FP32 minstd) on the host cores, 1 spp
-  batched      fused passes over many small requests vs one render per launch (same images)
-  serving      Poisson request loop: p50/p99 latency and aggregate Mrays/s
+  megakernel   the plain path on the same workload: one kernel launch renders the image
+  batched      fused passes over many small requests vs one megakernel launch per request
+  serving      Poisson requests: continuous batching vs the plain one-launch-per-request server,
+               p50/p99 latency and aggregate Mrays/s of each
 """
 import argparse
 import json
@@ -54,8 +56,43 @@ def cpu_baseline(scene, cfg):
                                                                          st["rays"], st["seconds"])}
 
 
+def _plain_and_batched(r, jobs):
+    """Times the same requests three ways: the plain path (one megakernel launch per request), the
+    wavefront pipeline one request at a time, and one fused crt_render_batch over all of them."""
+    j0 = jobs[0]
+    r.render(j0["sid"], 64, 64, 1, 8, megakernel=True)
+    r.render(j0["sid"], 64, 64, 1, 8)
+    r.render_batch([dict(j0, width=64, height=64)])
+    out = {}
+    accs = {}
+    for name, kw in (("plain", dict(megakernel=True)), ("wavefront_per_request", {})):
+        t0 = time.perf_counter()
+        rays = 0
+        accs[name] = []
+        for j in jobs:
+            _, acc, st = r.render(j["sid"], j["width"], j["height"], j["spp"], j["max_depth"], seed=j["seed"], **kw)
+            rays += st["rays"]
+            accs[name].append(acc)
+        dt = time.perf_counter() - t0
+        out[name + "_mrays_s"] = rays / dt / 1e6
+        out[name + "_s"] = dt
+    t0 = time.perf_counter()
+    outs, st = r.render_batch(jobs)
+    dt = time.perf_counter() - t0
+    out["batched_mrays_s"] = st["rays"] / dt / 1e6
+    out["batched_s"] = dt
+    out["speedup"] = out["batched_mrays_s"] / out["plain_mrays_s"]
+    out["speedup_vs_wavefront_per_request"] = out["batched_mrays_s"] / out["wavefront_per_request_mrays_s"]
+    # batched accumulations equal the per-request wavefront ones bit for bit; the megakernel runs the
+    # same estimator on the same streams (pixels agreeing to 1e-4 relative)
+    out["batched_identical_to_alone"] = bool(all(np.array_equal(a, b[1]) for a, b in zip(accs["wavefront_per_request"], outs)))
+    agree = [np.all(np.abs(a - b[1]) <= 1e-4 * np.maximum(np.abs(b[1]), 1.0), axis=-1).mean() for a, b in zip(accs["plain"], outs)]
+    out["plain_vs_batched_pixels_agree_min"] = float(min(agree))
+    return out
+
+
 def batched_vs_plain(seed=7, n=24):
-    """Many small requests: one crt_render per request vs crt_render_batch (fused passes)."""
+    """Many small requests on resident scenes (10k-200k triangles, 256^2-512^2, 1-16 spp)."""
     from cortexraytracercuda_amd import Renderer, scenes
     rng = np.random.default_rng(seed)
     r = Renderer()
@@ -69,66 +106,90 @@ def batched_vs_plain(seed=7, n=24):
         res = int(rng.choice([256, 384, 512]))
         spp = int(rng.choice([1, 4, 9, 16]))
         jobs.append(dict(sid=pool[key], width=res, height=res, spp=spp, max_depth=8, seed=1000 + k))
-    # warm up both paths
-    r.render(jobs[0]["sid"], 64, 64, 1, 8)
-    r.render_batch([dict(jobs[0], width=64, height=64)])
-    t0 = time.perf_counter()
-    plain_rays = 0
-    plain_imgs = []
-    for j in jobs:
-        rgb, _, st = r.render(j["sid"], j["width"], j["height"], j["spp"], j["max_depth"], seed=j["seed"], want_accum=False)
-        plain_rays += st["rays"]
-        plain_imgs.append(rgb)
-    t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    outs, st = r.render_batch(jobs, want_accum=False)
-    t_batch = time.perf_counter() - t0
-    same = all(np.array_equal(a, b[0]) for a, b in zip(plain_imgs, outs))
+    out = dict(requests=n, **_plain_and_batched(r, jobs))
     r.close()
-    return {"requests": n, "plain_mrays_s": plain_rays / t_plain / 1e6, "batched_mrays_s": st["rays"] / t_batch / 1e6,
-            "speedup": (st["rays"] / t_batch) / (plain_rays / t_plain), "identical_images": bool(same),
-            "plain_s": t_plain, "batched_s": t_batch}
+    return out
+
+
+def _poisson_requests(rng, pool, n, rate_hz, res_choices, spp_choices, max_depth=8, seed0=5000):
+    reqs = []
+    for k in range(n):
+        res = int(rng.choice(res_choices))
+        reqs.append(dict(sid=pool[int(rng.integers(0, len(pool)))], width=res, height=res,
+                         spp=int(rng.choice(spp_choices)), max_depth=max_depth, seed=seed0 + k))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n))
+    return reqs, arrivals
+
+
+def _serve_batched(r, reqs, arrivals):
+    """Continuous batching (crt_submit): requests join the fused passes as they arrive."""
+    t0 = time.perf_counter()
+    ids = []
+    for q, a in zip(reqs, arrivals):
+        while time.perf_counter() < t0 + a:
+            time.sleep(0.0005)
+        ids.append(r.submit(q["sid"], q["width"], q["height"], q["spp"], q["max_depth"], seed=q["seed"]))
+    lat, rays, imgs = [], 0, []
+    for q, i in zip(reqs, ids):
+        rgb, _, _ = r.wait(i, q["width"], q["height"], want_accum=False)
+        tm = r.timing(i)
+        lat.append(tm["done_s"] - tm["submit_s"])
+        rays += tm["rays"]
+        imgs.append(rgb)
+    wall = time.perf_counter() - t0
+    return np.array(lat), rays, wall, imgs
+
+
+def _serve_plain(r, reqs, arrivals):
+    """The plain server: requests in arrival order, each rendered by one megakernel launch; a
+    request waits until the ones before it are done. Latency = completion - arrival."""
+    t0 = time.perf_counter()
+    lat, rays, imgs = [], 0, []
+    for q, a in zip(reqs, arrivals):
+        while time.perf_counter() < t0 + a:
+            time.sleep(0.0005)
+        rgb, _, st = r.render(q["sid"], q["width"], q["height"], q["spp"], q["max_depth"], seed=q["seed"],
+                              want_accum=False, megakernel=True)
+        lat.append(time.perf_counter() - (t0 + a))
+        rays += st["rays"]
+        imgs.append(rgb)
+    wall = time.perf_counter() - t0
+    return np.array(lat), rays, wall, imgs
+
+
+def _serving_compare(r, reqs, arrivals):
+    out = {}
+    lat_b, rays_b, wall_b, img_b = _serve_batched(r, reqs, arrivals)
+    lat_p, rays_p, wall_p, img_p = _serve_plain(r, reqs, arrivals)
+    for name, lat, rays, wall in (("batched", lat_b, rays_b, wall_b), ("plain", lat_p, rays_p, wall_p)):
+        out[name] = {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "aggregate_mrays_s": rays / wall / 1e6, "wall_s": wall}
+    out["speedup"] = out["batched"]["aggregate_mrays_s"] / out["plain"]["aggregate_mrays_s"]
+    # equal image quality: 8-bit pixels of the two servers agree (same estimator and streams)
+    out["rgb_equal_fraction"] = float(np.mean([np.mean(np.all(a == b, axis=-1)) for a, b in zip(img_b, img_p)]))
+    return out
 
 
 def serving(seed=11, n=48, rate_hz=60.0):
-    """Poisson arrivals at rate_hz into the continuous-batching loop; latency = done - submit."""
+    """Poisson arrivals into the continuous-batching loop and into the plain server."""
     from cortexraytracercuda_amd import Renderer, scenes
     rng = np.random.default_rng(seed)
     r = Renderer()
     pool = [r.upload(scenes.sphere_mesh_scene(int(t), seed=i)) for i, t in enumerate([1e4, 5e4, 2e5, 1e6])]
-    reqs = []
-    for k in range(n):
-        res = int(rng.choice([256, 384, 512, 768]))
-        reqs.append(dict(sid=pool[int(rng.integers(0, len(pool)))], width=res, height=res,
-                         spp=int(rng.choice([1, 4, 16, 64])), max_depth=8, seed=5000 + k))
-    gaps = rng.exponential(1.0 / rate_hz, n)
+    reqs, arrivals = _poisson_requests(rng, pool, n, rate_hz, (256, 384, 512, 768), (1, 4, 16, 64))
     r.render(pool[0], 64, 64, 1, 8)
-    t0 = time.perf_counter()
-    ids = []
-    for k, q in enumerate(reqs):
-        target = t0 + gaps[:k + 1].sum()
-        while time.perf_counter() < target:
-            time.sleep(0.0005)
-        ids.append(r.submit(q["sid"], q["width"], q["height"], q["spp"], q["max_depth"], seed=q["seed"]))
-    lat = []
-    rays = 0
-    for q, i in zip(reqs, ids):
-        r.wait(i, q["width"], q["height"], want_accum=False)
-        tm = r.timing(i)
-        lat.append(tm["done_s"] - tm["submit_s"])
-        rays += tm["rays"]
-    wall = time.perf_counter() - t0
+    r.render(pool[0], 64, 64, 1, 8, megakernel=True)
+    out = dict(requests=n, rate_hz=rate_hz, **_serving_compare(r, reqs, arrivals))
     r.close()
-    lat = np.array(lat)
-    return {"requests": n, "rate_hz": rate_hz, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3), "aggregate_mrays_s": rays / wall / 1e6}
+    return out
 
 
 def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False):
     """k_extend roofline for one render of the workload: algorithmic bytes from the counting
     variant of the kernel (same deterministic workload) over the HIP-event time of the launches."""
     from cortexraytracercuda_amd import Renderer
-    rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths)
+    rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
+                  refill_min=args.refill_min)
     sidc = rc.upload(scene)
     node_b = rc.info(sidc)["node_bytes"]
     cnt = dict.fromkeys(["rays", "node_visits", "leaf_visits", "tri_tests", "sphere_tests", "quad_tests", "inst_visits"], 0)
@@ -158,7 +219,8 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False)
 
 def single_render_bench(scene, cfg, args, workload, progressive=False):
     from cortexraytracercuda_amd import Renderer
-    r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths)
+    r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
+                 refill_min=args.refill_min)
     t0 = time.perf_counter()
     sid = r.upload(scene)
     info = r.info(sid)
@@ -188,6 +250,16 @@ def single_render_bench(scene, cfg, args, workload, progressive=False):
         log("step %d: %.1f ms GPU, %d rays" % (k, gg, rr))
     wall = time.perf_counter() - t0
     value = rays / wall / 1e6
+    # the plain path on the same workload: one megakernel launch per render (per pass when progressive)
+    r.render(sid, 64, 64, 1, cfg["max_depth"], megakernel=True)
+    t1 = time.perf_counter()
+    mk_rays = 0
+    for sb, sc in ranges:
+        _, _, stm = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb,
+                             sample_count=sc, want_accum=False, megakernel=True)
+        mk_rays += stm["rays"]
+    mk_wall = time.perf_counter() - t1
+    log("megakernel: %.1f ms, %d rays" % (mk_wall * 1e3, mk_rays))
     roof = traversal_roofline(scene, cfg, args, r, sid, progressive)
     r.close()
     return {"metric": "Mrays/s on 1 MI355X + p50/p99 request latency, batched vs plain, equal RMSE",
@@ -198,7 +270,9 @@ def single_render_bench(scene, cfg, args, workload, progressive=False):
             "rays_per_step": rays // args.steps, "segments_per_s": value * 1e6, "gpu_ms_per_step": gpu_ms / args.steps,
             "bvh": {"nodes": info["num_nodes"], "width": info["bvh_width"], "depth": info["max_depth"],
                     "build_ms": info["build_ms"], "upload_ms": info["upload_ms"], "device_bytes": info["device_bytes"]},
-            "roofline": roof}
+            "roofline": roof,
+            "megakernel": {"mrays_s": mk_rays / mk_wall / 1e6, "ms_per_step": mk_wall * 1e3,
+                           "wavefront_speedup": value / (mk_rays / mk_wall / 1e6)}}
 
 
 def batch_workload(seed=4, n=64, tri_range=(1e4, 2e6), res_choices=(256, 384, 512, 768, 1024), spp_range=(1, 64)):
@@ -218,24 +292,9 @@ def batch_workload(seed=4, n=64, tri_range=(1e4, 2e6), res_choices=(256, 384, 51
         spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
         jobs.append(dict(sid=sid, width=res, height=res, spp=spp, max_depth=8, seed=1000 + k))
     log("batch workload: %d scenes, %d triangles, built in %.1f s" % (n, tris_total, time.perf_counter() - tg))
-    r.render(jobs[0]["sid"], 64, 64, 1, 8)
-    r.render_batch([dict(jobs[0], width=64, height=64)])
-    t0 = time.perf_counter()
-    plain_rays = 0
-    plain_imgs = []
-    for j in jobs:
-        _, acc, st = r.render(j["sid"], j["width"], j["height"], j["spp"], j["max_depth"], seed=j["seed"])
-        plain_rays += st["rays"]
-        plain_imgs.append(acc)
-    t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    outs, st = r.render_batch(jobs)
-    t_batch = time.perf_counter() - t0
-    same = all(np.array_equal(a, b[1]) for a, b in zip(plain_imgs, outs))
+    out = dict(requests=n, triangles_total=tris_total, **_plain_and_batched(r, jobs))
     r.close()
-    return {"requests": n, "triangles_total": tris_total, "plain_mrays_s": plain_rays / t_plain / 1e6,
-            "batched_mrays_s": st["rays"] / t_batch / 1e6, "speedup": (st["rays"] / t_batch) / (plain_rays / t_plain),
-            "identical_accumulation": bool(same), "plain_s": t_plain, "batched_s": t_batch}
+    return out
 
 
 def serving_workload(seed=5, n=96, rate_hz=20.0, pool_tris=(1e4, 5e4, 2e5, 5e5, 1e6, 2e6), res_choices=(256, 512, 1024, 2048),
@@ -246,32 +305,12 @@ def serving_workload(seed=5, n=96, rate_hz=20.0, pool_tris=(1e4, 5e4, 2e5, 5e5, 
     rng = np.random.default_rng(seed)
     r = Renderer(hbm_budget=int(budget_gb * 1e9))
     pool = [r.upload(scenes.sphere_mesh_scene(int(t), seed=i)) for i, t in enumerate(pool_tris)]
-    reqs = []
-    for k in range(n):
-        res = int(rng.choice(res_choices))
-        reqs.append(dict(sid=pool[int(rng.integers(0, len(pool)))], width=res, height=res,
-                         spp=int(rng.choice(spp_choices)), max_depth=8, seed=5000 + k))
-    gaps = rng.exponential(1.0 / rate_hz, n)
+    reqs, arrivals = _poisson_requests(rng, pool, n, rate_hz, res_choices, spp_choices)
     r.render(pool[0], 64, 64, 1, 8)
-    t0 = time.perf_counter()
-    ids = []
-    for k, q in enumerate(reqs):
-        target = t0 + gaps[:k + 1].sum()
-        while time.perf_counter() < target:
-            time.sleep(0.0005)
-        ids.append(r.submit(q["sid"], q["width"], q["height"], q["spp"], q["max_depth"], seed=q["seed"]))
-    lat = []
-    rays = 0
-    for q, i in zip(reqs, ids):
-        r.wait(i, q["width"], q["height"], want_accum=False)
-        tm = r.timing(i)
-        lat.append(tm["done_s"] - tm["submit_s"])
-        rays += tm["rays"]
-    wall = time.perf_counter() - t0
+    r.render(pool[0], 64, 64, 1, 8, megakernel=True)
+    out = dict(requests=n, rate_hz=rate_hz, budget_gb=budget_gb, **_serving_compare(r, reqs, arrivals))
     r.close()
-    lat = np.array(lat)
-    return {"requests": n, "rate_hz": rate_hz, "budget_gb": budget_gb, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3), "aggregate_mrays_s": rays / wall / 1e6, "wall_s": wall}
+    return out
 
 
 def main():
@@ -285,6 +324,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="smaller image for a fast check")
     ap.add_argument("--bvh-width", type=int, default=0, help="2 or 4 children per node (0: library default)")
     ap.add_argument("--max-paths", type=int, default=0, help="path slots per pass (0: library default)")
+    ap.add_argument("--extend-kernel", type=int, default=0, help="1 grid-stride, 2 persistent refill (0: default)")
+    ap.add_argument("--refill-min", type=int, default=0, help="persistent stage: idle lanes before a refill")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and this path is single-GPU "

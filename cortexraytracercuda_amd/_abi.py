@@ -37,6 +37,11 @@ CRT_RNG_PCG = 0
 CRT_RNG_MINSTD = 1
 
 CRT_RF_TIME_STAGES = 1
+CRT_RF_MEGAKERNEL = 2
+
+CRT_EXTEND_DEFAULT = 0
+CRT_EXTEND_GRID_STRIDE = 1
+CRT_EXTEND_PERSISTENT = 2
 
 f3 = C.c_float * 3
 
@@ -114,7 +119,8 @@ class crt_stats(C.Structure):
 class crt_config(C.Structure):
     _fields_ = [("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("max_paths", C.c_uint32),
                 ("block_size", C.c_uint32), ("count_work", C.c_uint32), ("leaf_size", C.c_uint32),
-                ("bvh_width", C.c_uint32)]
+                ("bvh_width", C.c_uint32), ("extend_kernel", C.c_uint32), ("refill_min", C.c_uint32),
+                ("pass_paths", C.c_uint32)]
 
 
 class crt_scene_info(C.Structure):
@@ -135,7 +141,8 @@ class crt_request_timing(C.Structure):
 
 class crt_work_counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("rays", "retraces", "light_pdf_tests", "node_visits", "leaf_visits", "tri_tests",
-                                          "sphere_tests", "quad_tests", "inst_visits", "extend_launches", "passes")]
+                                          "sphere_tests", "quad_tests", "inst_visits", "extend_launches", "passes",
+                                          "iterations")]
 
 
 class crt_memory_info(C.Structure):

@@ -119,13 +119,18 @@ int Context::init(const crt_config* cfg) {
     max_paths = config.max_paths ? config.max_paths : (1u << 22);
     leaf_size = config.leaf_size ? config.leaf_size : 4u;
     bvh_width = config.bvh_width ? config.bvh_width : 4u;
+    if (config.extend_kernel > CRT_EXTEND_PERSISTENT) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown extend_kernel");
+    extend_persistent = config.extend_kernel == CRT_EXTEND_PERSISTENT;
+    refill_min = config.refill_min ? std::min(config.refill_min, 64u) : 16u;
     if (bvh_width != 2 && bvh_width != 4) return set_error(CRT_ERR_INVALID_ARGUMENT, "bvh_width must be 2 or 4");
     extend_grid = (uint32_t)num_cus * 8u;
     shade_grid = (uint32_t)num_cus * 8u;
     int rc;
     if ((rc = scene_table.alloc(sizeof(crt_dev_scene) * kMaxScenes, this))) return rc;
     if ((rc = counters.alloc(sizeof(crt_dev_counters), this))) return rc;
-    if ((rc = counts.alloc(sizeof(uint32_t) * 4, this))) return rc;
+    if ((rc = ctl.alloc(sizeof(uint32_t) * 2 * CRT_CTL_WORDS, this))) return rc;
+    HIP_TRY(hipHostMalloc((void**)&ctl_host, sizeof(uint32_t) * CRT_CTL_WORDS * kCtlRing, hipHostMallocDefault));
+    for (int k = 0; k < kCtlRing; ++k) HIP_TRY(hipEventCreateWithFlags(&ctl_ev[k], hipEventDisableTiming));
     if ((rc = gstack.alloc((size_t)extend_grid * 256u * crt_k_gstack_entries() * sizeof(uint32_t), this))) return rc;
     for (int k = 0; k < 2; ++k) {
         if ((rc = req_table[k].alloc(sizeof(crt_dev_request) * kMaxPassRequests, this))) return rc;
@@ -137,31 +142,40 @@ int Context::init(const crt_config* cfg) {
     HIP_TRY(hipEventCreate(&ev_stop));
     HIP_TRY(hipEventCreate(&ev_ext[0]));
     HIP_TRY(hipEventCreate(&ev_ext[1]));
-    if ((rc = ensure_paths(max_paths))) return rc;
+    // path ids per pass: every path of a 1080p 64-spp frame fits one pass by default (2 GiB of
+    // per-path radiance); at least one id per slot
+    uint64_t pp = config.pass_paths ? config.pass_paths : std::min<uint64_t>(1ull << 27, 32ull * max_paths);
+    if ((rc = ensure_paths(max_paths, (uint32_t)std::max<uint64_t>(pp, max_paths)))) return rc;
     reserved = used_bytes();
     return CRT_OK;
 }
 
 uint64_t Context::used_bytes() const {
-    uint64_t b = scene_table.bytes + counters.bytes + counts.bytes + gstack.bytes + req_table[0].bytes + req_table[1].bytes;
+    uint64_t b = scene_table.bytes + counters.bytes + ctl.bytes + gstack.bytes + req_table[0].bytes + req_table[1].bytes;
     b += ray_o.bytes + ray_d.bytes + thr.bytes + rad.bytes + ids.bytes + hit.bytes + hit_inst.bytes + queue[0].bytes +
-         queue[1].bytes;
+         queue[1].bytes + freelist.bytes + out_rad.bytes;
     return b;
 }
 
-int Context::ensure_paths(uint32_t n) {
-    if (n <= path_capacity) return CRT_OK;
+int Context::ensure_paths(uint32_t n, uint32_t pass_ids) {
     int rc;
-    if ((rc = ray_o.alloc((size_t)n * 16, this))) return rc;
-    if ((rc = ray_d.alloc((size_t)n * 16, this))) return rc;
-    if ((rc = thr.alloc((size_t)n * 16, this))) return rc;
-    if ((rc = rad.alloc((size_t)n * 16, this))) return rc;
-    if ((rc = ids.alloc((size_t)n * 8, this))) return rc;
-    if ((rc = hit.alloc((size_t)n * 16, this))) return rc;
-    if ((rc = hit_inst.alloc((size_t)n * 4, this))) return rc;
-    if ((rc = queue[0].alloc((size_t)n * 4, this))) return rc;
-    if ((rc = queue[1].alloc((size_t)n * 4, this))) return rc;
-    path_capacity = n;
+    if (n > path_capacity) {
+        if ((rc = ray_o.alloc((size_t)n * 16, this))) return rc;
+        if ((rc = ray_d.alloc((size_t)n * 16, this))) return rc;
+        if ((rc = thr.alloc((size_t)n * 16, this))) return rc;
+        if ((rc = rad.alloc((size_t)n * 16, this))) return rc;
+        if ((rc = ids.alloc((size_t)n * 8, this))) return rc;
+        if ((rc = hit.alloc((size_t)n * 16, this))) return rc;
+        if ((rc = hit_inst.alloc((size_t)n * 4, this))) return rc;
+        if ((rc = queue[0].alloc((size_t)n * 4, this))) return rc;
+        if ((rc = queue[1].alloc((size_t)n * 4, this))) return rc;
+        if ((rc = freelist.alloc((size_t)n * 4, this))) return rc;
+        path_capacity = n;
+    }
+    if (pass_ids > pass_capacity) {
+        if ((rc = out_rad.alloc((size_t)pass_ids * 16, this))) return rc;
+        pass_capacity = pass_ids;
+    }
     return CRT_OK;
 }
 
@@ -174,6 +188,8 @@ Context::~Context() {
         if (req_event[k]) (void)hipEventDestroy(req_event[k]);
     }
     if (counters_host) (void)hipHostFree(counters_host);
+    if (ctl_host) (void)hipHostFree(ctl_host);
+    for (int k = 0; k < kCtlRing; ++k) if (ctl_ev[k]) (void)hipEventDestroy(ctl_ev[k]);
     if (ev_start) (void)hipEventDestroy(ev_start);
     for (int k = 0; k < 2; ++k) if (ev_ext[k]) (void)hipEventDestroy(ev_ext[k]);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
@@ -352,28 +368,41 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
         if ((int32_t)w.job->req.scene != uniform_scene) uniform_scene = -1;
         if (w.job->scene->host.num_instances) two_level = 1;
     }
-    if (npaths > path_capacity) return set_error(CRT_ERR_INTERNAL, "pass exceeds path capacity");
+    if (npaths > pass_capacity) return set_error(CRT_ERR_INTERNAL, "pass exceeds the pass capacity");
     crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
     HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request) * items.size(), hipMemcpyHostToDevice, stream));
     const crt_dev_scene* dsc = (const crt_dev_scene*)scene_table.p;
     int nreq = (int)items.size();
-    if (crt_k_generate(stream, mode, dreq, nreq, npaths, (float4*)ray_o.p, (float4*)ray_d.p, (float4*)thr.p, (float4*)rad.p,
-                       (uint2*)ids.p))
-        return set_error(CRT_ERR_HIP, "generate launch failed");
-    uint32_t* cnt = (uint32_t*)counts.p;
-    const uint32_t* qin = nullptr;
-    const uint32_t* cin = nullptr;
+    const uint32_t nslots = std::min(path_capacity, npaths);
+    uint32_t* cb = (uint32_t*)ctl.p;
+    uint32_t* fl = (uint32_t*)freelist.p;
+    if (crt_k_pass_init(stream, cb, fl, nslots)) return set_error(CRT_ERR_HIP, "pass init launch failed");
+    const uint32_t grid = std::min(extend_grid, std::max(1u, (nslots + 255) / 256));
+    // every path finishes within max_depth iterations of entering; a slot hosts at most
+    // ceil(npaths / nslots) paths in turn
+    const uint64_t max_iter = (uint64_t)max_depth * ((npaths + nslots - 1) / nslots + 1) + 2;
     int qi = 0;
-    uint32_t egrid = std::min(extend_grid, std::max(1u, (npaths + 255) / 256));
-    uint32_t sgrid = std::min(shade_grid, std::max(1u, (npaths + 255) / 256));
-    for (uint32_t b = 0; b < max_depth; ++b) {
-        uint32_t* qout = (uint32_t*)queue[qi].p;
-        uint32_t* cout = cnt + qi;
+    for (uint64_t k = 0;; ++k) {
+        if (k >= (uint64_t)kCtlLag) {
+            // the pass has drained once an iteration ends with every path generated and none alive
+            uint64_t j = k - kCtlLag;
+            HIP_TRY(hipEventSynchronize(ctl_ev[j % kCtlRing]));
+            const uint32_t* h = ctl_host + (j % kCtlRing) * CRT_CTL_WORDS;
+            if (h[CRT_CTL_GEN] >= npaths && h[CRT_CTL_SURV] == 0) break;
+        }
+        if (k > max_iter) return set_error(CRT_ERR_INTERNAL, "pass did not drain");
+        uint32_t* cin = cb + (k & 1) * CRT_CTL_WORDS;
+        uint32_t* cout = cb + ((k + 1) & 1) * CRT_CTL_WORDS;
+        uint32_t* qcur = (uint32_t*)queue[qi].p;
+        uint32_t* qnext = (uint32_t*)queue[qi ^ 1].p;
+        if (crt_k_regen(stream, mode, grid, dreq, nreq, npaths, cin, cout, fl, qcur, (float4*)ray_o.p, (float4*)ray_d.p,
+                        (float4*)thr.p, (float4*)rad.p, (uint2*)ids.p))
+            return set_error(CRT_ERR_HIP, "regen launch failed");
         if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, egrid, dsc, dreq, uniform_scene, qin,
-                         cin, npaths, (const float4*)ray_o.p,
-                         (const float4*)ray_d.p, (const uint2*)ids.p, (float4*)hit.p, (uint32_t*)hit_inst.p,
-                         (uint32_t*)gstack.p, cout, (crt_dev_counters*)counters.p))
+        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, grid, dsc, dreq, uniform_scene, qcur,
+                         cin + CRT_CTL_LIVE, 0, (const float4*)ray_o.p, (const float4*)ray_d.p, (const uint2*)ids.p,
+                         (float4*)hit.p, (uint32_t*)hit_inst.p, (uint32_t*)gstack.p, nullptr,
+                         extend_persistent ? cin + CRT_CTL_WORK : nullptr, refill_min, (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "extend launch failed");
         if (time_stages) {
             HIP_TRY(hipEventRecord(ev_ext[1], stream));
@@ -383,14 +412,51 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             if (extend_ms) *extend_ms += ms;
             extend_launches++;
         }
-        if (crt_k_shade(stream, mode, sgrid, dsc, dreq, qin, cin, npaths, qout, cout, (float4*)ray_o.p, (float4*)ray_d.p,
-                        (float4*)thr.p, (float4*)rad.p, (const uint2*)ids.p, (const float4*)hit.p,
-                        (const uint32_t*)hit_inst.p, (crt_dev_counters*)counters.p))
+        if (crt_k_shade(stream, mode, grid, dsc, dreq, qcur, cin + CRT_CTL_LIVE, 0, qnext, cout + CRT_CTL_SURV,
+                        (float4*)ray_o.p, (float4*)ray_d.p, (float4*)thr.p, (float4*)rad.p, (const uint2*)ids.p,
+                        (const float4*)hit.p, (const uint32_t*)hit_inst.p, fl, cout + CRT_CTL_NFREE, (float4*)out_rad.p,
+                        (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "shade launch failed");
-        qin = qout; cin = cout;
+        HIP_TRY(hipMemcpyAsync(ctl_host + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
+                               hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipEventRecord(ctl_ev[k % kCtlRing], stream));
         qi ^= 1;
+        iterations++;
     }
-    if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)rad.p)) return set_error(CRT_ERR_HIP, "resolve launch failed");
+    if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)out_rad.p)) return set_error(CRT_ERR_HIP, "resolve launch failed");
+    HIP_TRY(hipEventRecord(req_event[buf], stream));
+    passes++;
+    return CRT_OK;
+}
+
+// One megakernel launch over the whole request (every pixel, every remaining sample in order).
+int Context::run_plain(Job& j, bool time_stages, double* extend_ms) {
+    if (j.scheduled_all()) return CRT_OK;
+    int buf = pass_parity;
+    pass_parity ^= 1;
+    HIP_TRY(hipEventSynchronize(req_event[buf]));
+    crt_dev_request* host = (crt_dev_request*)req_host[buf];
+    crt_dev_request r = j.req;
+    r.path_base = 0;
+    r.pix_begin = 0; r.pix_count = j.npix;
+    r.sample_begin = j.s_next; r.sample_count = j.s_end - j.s_next;
+    host[0] = r;
+    j.s_next = j.s_end;
+    crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
+    HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request), hipMemcpyHostToDevice, stream));
+    uint32_t grid = std::min(extend_grid, std::max(1u, (j.npix + 255) / 256));
+    if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
+    if (crt_k_plain(stream, (int)j.rd.rng_mode, (int)bvh_width, j.scene->host.num_instances ? 1 : 0, grid,
+                    (const crt_dev_scene*)scene_table.p, dreq, (uint32_t*)gstack.p, (crt_dev_counters*)counters.p))
+        return set_error(CRT_ERR_HIP, "megakernel launch failed");
+    if (time_stages) {
+        HIP_TRY(hipEventRecord(ev_ext[1], stream));
+        HIP_TRY(hipEventSynchronize(ev_ext[1]));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev_ext[0], ev_ext[1]));
+        if (extend_ms) *extend_ms += ms;
+        extend_launches++;
+    }
     HIP_TRY(hipEventRecord(req_event[buf], stream));
     passes++;
     return CRT_OK;
@@ -438,13 +504,18 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     HIP_TRY(hipEventRecord(ev_start, stream));
     double ext_ms = 0.0;
     uint64_t paths = 0;
-    uint64_t launches0 = extend_launches, passes0 = passes;
+    uint64_t launches0 = extend_launches, passes0 = passes, iter0 = iterations;
+    for (Job* j : jobs) {
+        if (!(j->rd.flags & CRT_RENDER_MEGAKERNEL)) continue;
+        paths += (uint64_t)j->npix * (j->s_end - j->s_next);
+        if ((rc = run_plain(*j, time_stages, &ext_ms))) return rc;
+    }
     for (int mode = 0; mode < 2; ++mode) {
         for (;;) {
             std::vector<WorkItem> items;
-            uint32_t cap = max_paths;
+            uint32_t cap = pass_capacity;
             for (Job* j : jobs) {
-                if ((int)j->rd.rng_mode != mode) continue;
+                if ((int)j->rd.rng_mode != mode || (j->rd.flags & CRT_RENDER_MEGAKERNEL)) continue;
                 // at most one work item per request and pass: a pixel's samples are resolved by one
                 // thread in sample order, so two items of one request never race on a pixel
                 if (cap > 0 && items.size() < kMaxPassRequests) {
@@ -473,6 +544,7 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     last.sphere_tests = c.sphere_tests; last.quad_tests = c.quad_tests; last.inst_visits = c.inst_visits;
     last.extend_launches = (time_stages ? extend_launches - launches0 : 0);
     last.passes = passes - passes0;
+    last.iterations = iterations - iter0;
     return CRT_OK;
 }
 
@@ -566,7 +638,7 @@ int crt_render_batch(crt_context* c, crt_batch_item* items, uint32_t n, crt_stat
         jp.push_back(j.get());
         jobs.push_back(std::move(j));
     }
-    bool time_stages = (n > 0) && (items[0].rd.flags & 1u);
+    bool time_stages = (n > 0) && (items[0].rd.flags & CRT_RENDER_TIME_STAGES);
     int rc = ctx->render_jobs(jp, time_stages, stats);
     if (rc) return rc;
     for (uint32_t k = 0; k < n; ++k) {

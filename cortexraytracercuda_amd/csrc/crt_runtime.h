@@ -95,19 +95,26 @@ struct Context {
     hipStream_t stream = nullptr;
     uint64_t budget = 0, reserved = 0, scene_bytes = 0, request_bytes = 0;
     uint32_t max_paths = 0, path_capacity = 0, leaf_size = 4, bvh_width = 4, extend_grid = 2048, shade_grid = 2048;
+    bool extend_persistent = false;  // closest-hit stage with per-lane refill (else grid-stride)
+    uint32_t refill_min = 16;        // idle lanes before a wave fetches more rays
     std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
     bool slot_used[kMaxScenes] = {};
     crt_scene_id next_scene_id = 1;
-    DevBuf scene_table, counters, counts, gstack;
+    DevBuf scene_table, counters, ctl, gstack;
     DevBuf req_table[2];
     void* req_host[2] = {nullptr, nullptr};
     hipEvent_t req_event[2] = {nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_ext[2] = {nullptr, nullptr};
     void* counters_host = nullptr;
     int pass_parity = 0;
-    uint64_t passes = 0, extend_launches = 0;
-    DevBuf ray_o, ray_d, thr, rad, ids, hit, hit_inst, queue[2];
+    uint64_t passes = 0, extend_launches = 0, iterations = 0;
+    // streaming pass: path slot pools (max_paths slots), free list, queues, per-path-id radiance
+    DevBuf ray_o, ray_d, thr, rad, ids, hit, hit_inst, queue[2], freelist, out_rad;
+    uint32_t pass_capacity = 0;      // path ids per pass (out_rad entries)
+    static constexpr int kCtlRing = 8, kCtlLag = 4;   // control snapshots in flight / host look-behind
+    uint32_t* ctl_host = nullptr;    // pinned: kCtlRing snapshots of the control block's "out" half
+    hipEvent_t ctl_ev[kCtlRing] = {};
     int fail_alloc_after = -1;       // test hook: fail the n-th next allocation
     crt_work_counters last{};        // counters of the last blocking render
 
@@ -124,11 +131,12 @@ struct Context {
     int init(const crt_config* cfg);
     ~Context();
     uint64_t used_bytes() const;
-    int ensure_paths(uint32_t n);
+    int ensure_paths(uint32_t slots, uint32_t pass_ids);
     int upload_scene(const crt_scene_desc* desc, crt_scene_id* out);
     int release_scene(crt_scene_id id);
     DeviceScene* find_scene(crt_scene_id id);
     int run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms);
+    int run_plain(Job& j, bool time_stages, double* extend_ms);
     int finish_job(Job& j, crt_image* out);
     int reset_counters();
     int read_counters(crt_dev_counters& c);

@@ -2,10 +2,14 @@
 //
 // One pass of the hot path (core/src/gpu.cu:761-802 per pixel, core/src/cameras/camera.cuh:149-264
 // per bounce) is split into stages that run over every live path of every request in the pass:
-//   k_generate  camera rays + keyed RNG            (render kernel sample loop + get_ray)
-//   k_extend    BVH2 traversal, closest hit         (hittable_list/bvh_node/aabb/primitive ::hit)
+//   k_regen     camera rays into free path slots   (render kernel sample loop + get_ray)
+//   k_extend    BVH traversal, closest hit          (hittable_list/bvh_node/aabb/primitive ::hit)
 //   k_shade     emission, scatter, light mixture    (ray_color body, materials, pdfs, textures)
-//               + queue compaction (wave ballot, block prefix scan, one atomic per block)
+//               + queue compaction (wave ballot, block prefix scan, one atomic per block); finished
+//               paths write their radiance by path id and return their slot to the free list
+// The stages repeat over a fixed pool of path slots until every path of the pass has finished:
+// slots freed by one iteration are refilled with new camera paths by the next, so each launch
+// works on a full queue instead of the thinning tail of one generation of paths.
 //   k_resolve   per-pixel sums in sample order      (pixel_color += ray_color)
 //   k_tonemap   NaN->0, /spp, sqrt, clamp, *255.99  (prepare_pixel_color + quantize)
 // MFMA is not used: nothing here is a dense contraction. The traversal is bound by node/primitive
@@ -451,6 +455,15 @@ template <int BLOCK> __device__ __forceinline__ uint32_t block_append(bool keep,
     return pos;
 }
 
+// Adds every lane's v to *p with one atomic per wave (a per-lane atomic on one address serialises
+// in L2). All lanes of the wave must call it.
+__device__ __forceinline__ void wave_add(unsigned long long* p, uint32_t v) {
+    uint32_t s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += (uint32_t)__shfl_xor((int)s, off);
+    if ((threadIdx.x & 63u) == 0u && s) atomicAdd(p, (unsigned long long)s);
+}
+
 }  // namespace
 
 // ======================================================================================== kernels
@@ -474,31 +487,18 @@ __device__ __forceinline__ int find_request(const crt_dev_request* reqs, int nre
     return lo;
 }
 
-// Camera rays: render kernel sample loop (core/src/gpu.cu:776-787) + perspective_camera::get_ray
-// (core/src/cameras/perspective_camera.cuh:106-113): jitter r, diagonal stratified (u, v),
-// lens disk by rejection, time draw.
+// Camera ray of (pixel, sample): render kernel sample loop (core/src/gpu.cu:776-787) +
+// perspective_camera::get_ray (core/src/cameras/perspective_camera.cuh:106-113): jitter r,
+// diagonal stratified (u, v), lens disk by rejection, time draw. `g` is positioned at the start
+// of the sample's draws.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_generate(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t npaths,
-                                                  float4* __restrict__ ray_o, float4* __restrict__ ray_d,
-                                                  float4* __restrict__ thr, float4* __restrict__ rad,
-                                                  uint2* __restrict__ ids) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npaths) return;
-    int r = find_request(reqs, nreq, p);
-    const crt_dev_request& R = reqs[r];
-    uint32_t local = p - R.path_base;
-    uint32_t k = local / R.pix_count;
-    uint32_t pix = R.pix_begin + (local - k * R.pix_count);
-    uint32_t sample = R.sample_begin + k;
+__device__ __forceinline__ void camera_ray(const crt_dev_request& R, uint32_t pix, uint32_t sample, Rng<MODE>& g,
+                                           V3& o, V3& dir, float& tm) {
     uint32_t W = R.width, H = R.height;
     uint32_t jj = pix / W, i = pix - jj * W;
     uint32_t j = H - 1u - jj;                  // the reference's row index counts from the bottom
-    uint32_t ref_pixel = j * W + i;
     int sq = (int)R.sqrt_spp;
     int sj = (int)(sample / (uint32_t)sq), si = (int)(sample - (uint32_t)sj * (uint32_t)sq);
-    Rng<MODE> g;
-    if (MODE == CRT_RNG_MINSTD) g.ms = R.rng_state[pix];
-    else g.pcg = pcg_keyed(R.seed, ref_pixel, sample, 0u);
     float rr = g.uniform();
     float u = ((float)i + ((float)si + rr) / (float)sq) / (float)W;
     float v = ((float)j + ((float)sj + rr) / (float)sq) / (float)H;
@@ -511,19 +511,98 @@ __global__ __launch_bounds__(256) void k_generate(const crt_dev_request* __restr
     V3 rd = R.lens_radius * v3(dx, dy, 0.0f);
     V3 cu = v3p(R.cu), cv = v3p(R.cv), org = v3p(R.origin);
     V3 off = cu * rd.x + cv * rd.y;
-    float tm = R.time0 + g.uniform() * (R.time1 - R.time0);
-    V3 o = org + off;
-    V3 dir = v3p(R.llc) + u * v3p(R.horizontal) + v * v3p(R.vertical) - org - off;
-    uint32_t state = (R.max_depth & 0xffu) | ST_CAMERA;
-    ray_o[p] = make_float4(o.x, o.y, o.z, tm);
-    ray_d[p] = make_float4(dir.x, dir.y, dir.z, bitsf(state));
-    thr[p] = make_float4(1.0f, 1.0f, 1.0f, bitsf(g.ms));
-    rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ids[p] = make_uint2(pix, ((uint32_t)r << 16) | (sample & 0xffffu));
+    tm = R.time0 + g.uniform() * (R.time1 - R.time0);
+    o = org + off;
+    dir = v3p(R.llc) + u * v3p(R.horizontal) + v * v3p(R.vertical) - org - off;
 }
 
-// Closest-hit stage. One-sided emitters hit from behind are re-traced from t + 0.001 in the same
-// stage (core/src/cameras/camera.cuh:179-183); the result is flagged as a pass-through.
+// Pixel index in the reference's numbering (rows counted from the bottom), which keys the streams.
+__device__ __forceinline__ uint32_t ref_pixel_of(const crt_dev_request& R, uint32_t pix) {
+    uint32_t jj = pix / R.width, i = pix - jj * R.width;
+    return (R.height - 1u - jj) * R.width + i;
+}
+
+// Streaming pass control block (CRT_CTL_* words, crt_kernels.h): two halves alternate between
+// iterations. Iteration k reads half k%2 ("in": survivors and free slots left by the previous shade
+// stage, next path id to generate) and prepares half (k+1)%2 ("out") for its own shade stage.
+// Start of a pass: every slot free, nothing generated.
+__global__ __launch_bounds__(256) void k_pass_init(uint32_t* __restrict__ ctl, uint32_t* __restrict__ freelist,
+                                                   uint32_t nslots) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nslots) freelist[i] = i;
+    if (i < 2 * CRT_CTL_WORDS) ctl[i] = (i == CRT_CTL_NFREE) ? nslots : 0u;
+}
+
+// Regeneration: the free slots left by the last shade stage take the next path ids of the pass
+// (camera rays, in path-id order: sample-major within each request) and join the survivors in the
+// queue of this iteration.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t total,
+                                               uint32_t* __restrict__ ctl_in, uint32_t* __restrict__ ctl_out,
+                                               const uint32_t* __restrict__ freelist, uint32_t* __restrict__ queue,
+                                               float4* __restrict__ ray_o, float4* __restrict__ ray_d,
+                                               float4* __restrict__ thr, float4* __restrict__ rad,
+                                               uint2* __restrict__ ids) {
+    const uint32_t surv = ctl_in[CRT_CTL_SURV], nfree = ctl_in[CRT_CTL_NFREE], g0 = ctl_in[CRT_CTL_GEN];
+    const uint32_t n_new = min(nfree, total - g0);
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        ctl_in[CRT_CTL_LIVE] = surv + n_new;
+        ctl_in[CRT_CTL_WORK] = 0u;
+        ctl_out[CRT_CTL_GEN] = g0 + n_new;
+        ctl_out[CRT_CTL_SURV] = 0u;
+        ctl_out[CRT_CTL_NFREE] = 0u;
+    }
+    for (uint32_t i = tid; i < n_new; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = freelist[i], pid = g0 + i;
+        int r = find_request(reqs, nreq, pid);
+        const crt_dev_request& R = reqs[r];
+        uint32_t local = pid - R.path_base;
+        uint32_t k = local / R.pix_count;
+        uint32_t pix = R.pix_begin + (local - k * R.pix_count);
+        uint32_t sample = R.sample_begin + k;
+        Rng<MODE> g;
+        if (MODE == CRT_RNG_MINSTD) g.ms = R.rng_state[pix];
+        else g.pcg = pcg_keyed(R.seed, ref_pixel_of(R, pix), sample, 0u);
+        V3 o, dir;
+        float tm;
+        camera_ray<MODE>(R, pix, sample, g, o, dir, tm);
+        uint32_t state = (R.max_depth & 0xffu) | ST_CAMERA;
+        ray_o[slot] = make_float4(o.x, o.y, o.z, tm);
+        ray_d[slot] = make_float4(dir.x, dir.y, dir.z, bitsf(state));
+        thr[slot] = make_float4(1.0f, 1.0f, 1.0f, bitsf(g.ms));
+        rad[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ids[slot] = make_uint2(pix, ((uint32_t)r << 16) | (sample & 0xffffu));
+        queue[surv + i] = slot;
+    }
+}
+
+// Closest hit of one path segment. One-sided emitters hit from behind are re-traced from
+// t + 0.001 (core/src/cameras/camera.cuh:179-183); the result is then flagged as a pass-through
+// (returned flag CRT_PASSTHROUGH). `nrays` counts closest-hit queries, `nretrace` the re-traces.
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+__device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
+                                                uint32_t* gst, HitRec& h, TravCount& tc, bool& ovf, uint32_t& nrays,
+                                                uint32_t& nretrace) {
+    float tmin = 0.00001f;
+    uint32_t pass_flag = 0u;
+    for (;;) {
+        bool o1 = false;
+        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc, o1);
+        ovf = ovf || o1;
+        if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
+        const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
+        uint32_t fl = fbits(q[1].w);
+        if (!((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(q[3])) < 0.0f))) break;
+        tmin = h.t + 0.001f;
+        pass_flag = CRT_PASSTHROUGH;
+        nretrace++;
+    }
+    nrays++;
+    return pass_flag;
+}
+
+// Closest-hit stage over the live path queue.
 // UNIFORM: every ray of the launch belongs to scene `uniform_scene` (scene pointers stay scalar).
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
 __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restrict__ scenes,
@@ -556,36 +635,115 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
         bool camera = (state & ST_CAMERA) != 0;
         HitRec h;
         bool ovf = false;
-        float tmin = 0.00001f;
-        uint32_t pass_flag = 0u;
-        for (;;) {
-            trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc,
-                                                            ovf);
-            nrays++;
-            if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
-            const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
-            uint32_t fl = fbits(q[1].w);
-            if (!((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(q[3])) < 0.0f))) break;
-            tmin = h.t + 0.001f;
-            pass_flag = CRT_PASSTHROUGH;
-            nretrace++;
-            nrays--;
-        }
+        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst, h, tc,
+                                                                                   ovf, nrays, nretrace);
         novf += ovf ? 1u : 0u;
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
     }
-    if (nrays) atomicAdd(&counters->rays, (unsigned long long)nrays);
-    if (nretrace) atomicAdd(&counters->retraces, (unsigned long long)nretrace);
+    wave_add(&counters->rays, nrays);
+    wave_add(&counters->retraces, nretrace);
     if (COUNT) {
-        atomicAdd(&counters->node_visits, (unsigned long long)tc.nodes);
-        atomicAdd(&counters->leaf_visits, (unsigned long long)tc.leaves);
-        atomicAdd(&counters->tri_tests, (unsigned long long)tc.tris);
-        atomicAdd(&counters->sphere_tests, (unsigned long long)tc.spheres);
-        atomicAdd(&counters->quad_tests, (unsigned long long)tc.quads);
-        atomicAdd(&counters->inst_visits, (unsigned long long)tc.insts);
+        wave_add(&counters->node_visits, tc.nodes);
+        wave_add(&counters->leaf_visits, tc.leaves);
+        wave_add(&counters->tri_tests, tc.tris);
+        wave_add(&counters->sphere_tests, tc.spheres);
+        wave_add(&counters->quad_tests, tc.quads);
+        wave_add(&counters->inst_visits, tc.insts);
     }
-    if (novf) atomicAdd(&counters->stack_overflow, (unsigned long long)novf);
+    wave_add(&counters->stack_overflow, novf);
+}
+
+// Persistent closest-hit stage with per-lane refill: a lane whose ray is finished takes the next
+// ray of the queue at once, so a wave keeps its lanes busy until the queue drains instead of
+// waiting for its slowest ray. Rays are handed out through `work_counter` (zero at launch), one
+// atomicAdd per wave and refill; a refill is made when at least `refill_min` lanes are idle (or
+// all are). Every wave leaves once the counter has passed the queue length and its lanes are idle;
+// each traversal ends (every node is visited at most once per query, at most one re-trace).
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
+__global__ __launch_bounds__(BLOCK) void k_extend_pt(const crt_dev_scene* __restrict__ scenes,
+                                                     const crt_dev_request* __restrict__ reqs, uint32_t uniform_scene,
+                                                     const uint32_t* __restrict__ queue,
+                                                     const uint32_t* __restrict__ count_ptr, uint32_t count_direct,
+                                                     const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
+                                                     const uint2* __restrict__ ids, float4* __restrict__ hit,
+                                                     uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
+                                                     uint32_t* __restrict__ zero_counter, uint32_t* __restrict__ work_counter,
+                                                     uint32_t refill_min, crt_dev_counters* __restrict__ counters) {
+    __shared__ uint32_t s_stack[LS * BLOCK];
+    if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
+    const uint32_t n = queue ? *count_ptr : count_direct;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* lds_col = s_stack + threadIdx.x;
+    uint32_t* gst = gstack + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * GS;
+    TravCount tc = {0, 0, 0, 0, 0, 0};
+    uint32_t nrays = 0, nretrace = 0, novf = 0;
+    SceneView S;
+    if (UNIFORM) S = scene_view(scenes + uniform_scene);
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
+    bool active = false, exhausted = false;
+    uint32_t slot = 0, pass_flag = 0;
+    for (;;) {
+        if (!exhausted) {
+            uint64_t idle = __ballot(!active);
+            uint32_t cnt = (uint32_t)__popcll(idle);
+            if (cnt >= refill_min || cnt == 64u) {
+                uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(work_counter, cnt);
+                base = __shfl(base, (int)leader);
+                if (base + cnt >= n) exhausted = true;
+                if (!active) {
+                    uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                    if (i < n) {
+                        slot = queue ? queue[i] : i;
+                        float4 ro = ray_o[slot], rdv = ray_d[slot];
+                        if (!UNIFORM) S = scene_view(scenes + reqs[ids[slot].y >> 16].scene);
+                        tr.init(S, v3(ro), v3(rdv), 0.00001f, 3.402823466e+38f, (fbits(rdv.w) & ST_CAMERA) != 0,
+                                lds_col, gst);
+                        pass_flag = 0u;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && tr.step(S, tc)) {
+            const HitRec h = tr.best;
+            bool again = false;
+            if (!pass_flag && h.prim != CRT_MISS && CRT_HIT_TYPE(h.prim) == CRT_PRIM_QUAD) {
+                // one-sided emitter seen from behind: re-trace from t + 0.001 (camera.cuh:179-183)
+                const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
+                again = (fbits(q[1].w) & CRT_QF_ONE_SIDED) && !(dot(tr.dw, v3(q[3])) < 0.0f);
+            }
+            novf += tr.st.overflow ? 1u : 0u;
+            if (again) {
+                nretrace++;
+                pass_flag = CRT_PASSTHROUGH;
+                tr.init(S, tr.ow, tr.dw, h.t + 0.001f, 3.402823466e+38f, tr.camera, lds_col, gst);
+            } else {
+                nrays++;
+                hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
+                hit_inst[slot] = h.inst | pass_flag;
+                active = false;
+            }
+        }
+    }
+    wave_add(&counters->rays, nrays);
+    wave_add(&counters->retraces, nretrace);
+    if (COUNT) {
+        wave_add(&counters->node_visits, tc.nodes);
+        wave_add(&counters->leaf_visits, tc.leaves);
+        wave_add(&counters->tri_tests, tc.tris);
+        wave_add(&counters->sphere_tests, tc.spheres);
+        wave_add(&counters->quad_tests, tc.quads);
+        wave_add(&counters->inst_visits, tc.insts);
+    }
+    wave_add(&counters->stack_overflow, novf);
 }
 
 // Closest hit of caller rays (first-hit parity queries).
@@ -610,7 +768,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __res
         hit[i] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[i] = h.inst;
     }
-    if (novf) atomicAdd(&counters->stack_overflow, (unsigned long long)novf);
+    wave_add(&counters->stack_overflow, novf);
 }
 
 // ------------------------------------------------------------------ shading
@@ -653,6 +811,218 @@ __device__ __forceinline__ float4 tex_value(const float4* texs, const uint8_t* t
 }
 }  // namespace
 
+// Shading of one hit (core/src/cameras/camera.cuh:150-223): emission, background on a miss,
+// material scatter with the light/cosine mixture pdf for Lambertian surfaces. Updates the path in
+// place; returns true while the path continues (o, d then hold the next segment).
+template <int MODE>
+__device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, const crt_dev_request& R, uint32_t pix,
+                                          uint32_t sample, float4 hv, uint32_t iw, V3& o, V3& d, V3& att, V3& L,
+                                          uint32_t& depth, uint32_t& bounce, Rng<MODE>& g, uint32_t& light_tests) {
+    uint32_t prim = fbits(hv.y);
+    bool passthrough = (iw & CRT_PASSTHROUGH) != 0;
+    uint32_t inst = iw & ~CRT_PASSTHROUGH;
+    bool done = true;
+    if (prim == CRT_MISS) {
+        // miss: background (camera.cuh:161-174); a missed pass-through re-trace keeps the
+        // light record, which does not scatter and adds colour(0,0,0,0)
+        if (!passthrough) L = L + att * v3(sc->background[0], sc->background[1], sc->background[2]);
+    } else {
+        bounce += 1u;
+        if (MODE == CRT_RNG_PCG) g.pcg = pcg_keyed(R.seed, ref_pixel_of(R, pix), sample, bounce);
+        uint32_t type = CRT_HIT_TYPE(prim), idx = CRT_HIT_INDEX(prim);
+        float t = hv.x;
+        V3 p = o + t * d;
+        V3 outward;
+        uint32_t mat_id;
+        float tu = hv.z, tvv = hv.w;  // texture coordinates (filled lazily below)
+        bool need_uv_sphere = false;
+        bool is_light_quad = false;
+        V3 sphere_c; float sphere_r = 1.0f;
+        uint32_t tri_flags = 0;
+        float4 tq0, tq1, tq2;
+        if (type == CRT_PRIM_TRI) {
+            const float4* tp = (const float4*)sc->tris + 3 * idx;
+            tq0 = tp[0]; tq1 = tp[1]; tq2 = tp[2];
+            mat_id = fbits(tq1.w);
+            tri_flags = fbits(tq2.w);
+            // flat normal = unit(cross(v0 - v1, v0 - v2)) (core/src/primitives/triangle.cuh:118)
+            V3 a = v3(tq0), b = v3(tq1), c = v3(tq2);
+            V3 nrm = unit(cross(a - b, a - c));
+            if (tri_flags & CRT_TF_SMOOTH) {
+                const float4* tn = (const float4*)sc->tri_normals + 3 * idx;
+                // u weights v1, v weights v2 (triangle.cuh:167-174), not renormalised
+                nrm = hv.z * v3(tn[1]) + hv.w * v3(tn[2]) + (1.0f - hv.z - hv.w) * v3(tn[0]);
+            }
+            if (inst != CRT_NO_INST) {
+                // object -> world normal: transpose of the world -> object 3x3
+                const float4* ip = (const float4*)sc->insts + 4 * inst;
+                float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
+                int ovr = (int)fbits(ip[3].z);
+                if (ovr >= 0) mat_id = (uint32_t)ovr;
+                V3 nw = v3(r0.x * nrm.x + r1.x * nrm.y + r2.x * nrm.z, r0.y * nrm.x + r1.y * nrm.y + r2.y * nrm.z,
+                           r0.z * nrm.x + r1.z * nrm.y + r2.z * nrm.z);
+                nrm = (tri_flags & CRT_TF_SMOOTH) ? nw : unit(nw);
+            }
+            outward = nrm;
+        } else if (type == CRT_PRIM_SPHERE) {
+            float4 s = ((const float4*)sc->spheres)[idx];
+            uint4 aux = ((const uint4*)sc->sphere_aux)[idx];
+            mat_id = aux.y;
+            sphere_c = v3(s); sphere_r = s.w;
+            outward = (p - sphere_c) / sphere_r;
+            need_uv_sphere = true;
+        } else {
+            const float4* q = (const float4*)sc->quads + 5 * idx;
+            float4 q1 = q[1], q3 = q[3], q4 = q[4];
+            mat_id = fbits(q4.w);
+            outward = v3(q3);
+            is_light_quad = (fbits(q1.w) & CRT_QF_LIGHT) != 0;
+        }
+        bool front = dot(d, outward) < 0.0f;
+        V3 nrm = front ? outward : -outward;
+        const float4* mats = (const float4*)sc->mats;
+        Mat m;
+        m.m0 = mats[2 * mat_id];
+        m.m1 = *(const int4*)(mats + 2 * mat_id + 1);
+        if (m.m1.x == CRT_MAT_DIFFUSE_LIGHT) {
+            // emitter: light does not scatter (material::scatter returns false)
+            if (!passthrough) {
+                bool one_sided = (m.m1.z & (int)CRT_MATF_ONE_SIDED) != 0 || is_light_quad;
+                if (!one_sided || front) L = L + att * (v3(m.m0) * m.m0.w);
+            }
+        } else {
+            // albedo from texture (lambertian/metal use the diffuse texture)
+            V3 alb = v3(m.m0);
+            if (m.m1.y >= 0) {
+                if (need_uv_sphere) {
+                    // get_sphere_uv on the outward normal (core/src/utilities/uvmapping.cuh:204-228)
+                    const float2* mp = (const float2*)sc->sphere_map + 3 * idx;
+                    float2 scl = mp[0], ofs = mp[1], rep = mp[2];
+                    float theta = acosf(-outward.y);
+                    float phi = atan2f(-outward.z, outward.x) + 3.14159265358979323846f;
+                    float s_ = phi / (2.0f * 3.14159265358979323846f);
+                    float t_ = theta / 3.14159265358979323846f;
+                    s_ = fmodf(s_ * rep.x, 1.0f);
+                    t_ = fmodf(t_ * rep.y, 1.0f);
+                    tu = scl.x * s_ + ofs.x;
+                    tvv = scl.y * t_ + ofs.y;
+                } else if (type == CRT_PRIM_TRI) {
+                    tu = 0.0f; tvv = 0.0f;
+                    if (tri_flags & CRT_TF_UV) {
+                        const float2* uvp = (const float2*)sc->tri_uvs + 3 * idx;
+                        float2 a = uvp[0], b = uvp[1], c = uvp[2];
+                        float w0 = 1.0f - hv.z - hv.w;
+                        tu = w0 * a.x + hv.z * b.x + hv.w * c.x;
+                        tvv = w0 * a.y + hv.z * b.y + hv.w * c.y;
+                        tu = fmodf(tu, 1.0f); tvv = fmodf(tvv, 1.0f);
+                        if (tu < 0.0f) tu += 1.0f;
+                        if (tvv < 0.0f) tvv += 1.0f;
+                    }
+                }
+                float4 tx = tex_value((const float4*)sc->texs, sc->texels, m.m1.y, tu, tvv, p);
+                alb = v3(tx);
+            }
+            if (m.m1.x == CRT_MAT_METAL) {
+                // metal::scatter (core/src/materials/metal.cuh:57-65): never absorbs
+                V3 ud = unit(d);
+                V3 refl = ud - (nrm * dot(nrm, ud)) * 2.0f;
+                float x, y, z;
+                do {
+                    float a = g.uniform(), b = g.uniform(), c = g.uniform();
+                    x = 2.0f * a - 1.0f; y = 2.0f * b - 1.0f; z = 2.0f * c - 1.0f;
+                } while (x * x + y * y + z * z >= 1.0f);
+                d = refl + m.m0.w * v3(x, y, z);
+                att = att * alb;
+                done = false;
+            } else if (m.m1.x == CRT_MAT_DIELECTRIC) {
+                // dielectric::scatter (core/src/materials/dielectric.cuh:32-51)
+                float ir = m.m0.w;
+                float ratio = front ? (1.0f / ir) : ir;
+                V3 ud = unit(d);
+                float dn = dot(-ud, nrm);
+                float cos_t = dn < 1.0f ? dn : 1.0f;
+                float sin_t = sqrtf(1.0f - cos_t * cos_t);
+                bool reflect_ray = ratio * sin_t > 1.0f;
+                if (!reflect_ray) {
+                    float r0 = (1.0f - ratio) / (1.0f + ratio);
+                    r0 = r0 * r0;
+                    float x = 1.0f - cos_t;
+                    float x5 = powf(x, 5.0f);
+                    reflect_ray = (r0 + (1.0f - r0) * x5) > g.uniform();
+                }
+                if (reflect_ray) {
+                    d = ud - (nrm * dot(nrm, ud)) * 2.0f;
+                } else {
+                    float dd = dot(nrm, ud);
+                    float kk = 1.0f - ratio * ratio * (1.0f - dd * dd);
+                    d = kk >= 0.0f ? (ratio * ud - (ratio * dd + sqrtf(kk)) * nrm) : v3(0.0f, 0.0f, 0.0f);
+                }
+                done = false;
+            } else if (m.m1.x == CRT_MAT_LAMBERTIAN) {
+                // mixture of light sampling and cosine sampling (camera.cuh:203-216)
+                // onb::build_from_w (core/src/misc/onb.cuh:122-128)
+                V3 w = unit(nrm);
+                V3 aa = fabsf(w.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+                V3 bv = unit(cross(w, aa));
+                V3 bu = cross(w, bv);
+                const uint32_t nl = sc->num_lights;
+                const float4* qd = (const float4*)sc->quads;
+                V3 dir;
+                if (g.uniform() < 0.5f) {
+                    // hittable_list::random -> light 0 (get_int quirk) -> directional_light::random
+                    if (nl == 0) {
+                        dir = v3(0.0f, 0.0f, 0.0f);
+                    } else {
+                        if (nl > 1) g.skip();
+                        const float4* q = qd + 5 * sc->light_quads[0];
+                        float a = g.uniform(), b = g.uniform();
+                        V3 pp = v3(q[0]) + (a - 0.5f) * v3(q[1]) + (b - 0.5f) * v3(q[2]);
+                        dir = pp - p;
+                    }
+                } else {
+                    // random_cosine_direction (core/src/misc/gpu_randomizer.cuh:122-133)
+                    float r1 = g.uniform(), r2 = g.uniform();
+                    float phi = 6.28318530718f * r1;
+                    float z = sqrtf(1.0f - r2);
+                    float rs = sqrtf(r2);
+                    float x = cosf(phi) * rs, y = sinf(phi) * rs;
+                    dir = x * bu + y * bv + z * w;
+                }
+                // light pdf (hittable_list::pdf_value -> directional_light::pdf_value)
+                float weight = 1.0f / (float)nl;
+                float sum = 0.0f;
+                for (uint32_t li = 0; li < nl; ++li) {
+                    const float4* q = qd + 5 * sc->light_quads[li];
+                    float tl, qa, qb;
+                    light_tests++;
+                    float pv = 0.0f;
+                    if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb)) {
+                        float dist2 = tl * tl * len2(dir);
+                        float cosine = fabsf(dot(dir, v3(q[3])) / vlen(dir));
+                        pv = dist2 / (cosine * q[2].w);
+                    }
+                    sum += weight * pv;
+                }
+                V3 ud = unit(dir);
+                float cv = dot(ud, w) / 3.14159265358979323846f;
+                float pc = 0.0f > cv ? 0.0f : cv;
+                float pdf = 0.5f * sum + (1.0f - 0.5f) * pc;
+                float ct = dot(nrm, ud);
+                float spdf = ct < 0.0f ? 0.0f : ct / 3.14159265358979323846f;
+                att = att * ((1.0f / pdf) * (alb * spdf));
+                d = dir;
+                done = false;
+            }
+        }
+        if (!done) {
+            o = p;
+            depth -= 1u;
+            if (depth == 0u) done = true;
+        }
+    }
+    return !done;
+}
+
 template <int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict__ scenes,
                                                  const crt_dev_request* __restrict__ reqs,
@@ -662,7 +1032,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                                                  float4* __restrict__ ray_d, float4* __restrict__ thr,
                                                  float4* __restrict__ rad, const uint2* __restrict__ ids,
                                                  const float4* __restrict__ hit, const uint32_t* __restrict__ hit_inst,
-                                                 crt_dev_counters* __restrict__ counters) {
+                                                 uint32_t* __restrict__ freelist, uint32_t* __restrict__ nfree_out,
+                                                 float4* __restrict__ out_rad, crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_wave[BLOCK / 64];
     __shared__ uint32_t s_base;
     const uint32_t n = queue_in ? *count_in : count_direct;
@@ -672,7 +1043,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
     const uint32_t iters = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < iters; ++it) {
         uint32_t i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
-        bool keep = false;
+        bool keep = false, freed = false;
         uint32_t slot = 0;
         if (i < n) {
             slot = queue_in ? queue_in[i] : i;
@@ -687,216 +1058,11 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
             const crt_dev_scene* sc = scenes + R.scene;
             V3 o = v3(ro), d = v3(rdv);
             V3 att = v3(tv), L = v3(rv);
-            uint32_t prim = fbits(hv.y);
-            bool passthrough = (iw & CRT_PASSTHROUGH) != 0;
-            uint32_t inst = iw & ~CRT_PASSTHROUGH;
+            uint32_t pix = idv.x, sample = idv.y & 0xffffu;
             uint32_t depth = ST_DEPTH(state), bounce = ST_BOUNCE(state);
             Rng<MODE> g;
             g.ms = fbits(tv.w);
-            bool done = true;
-            if (prim == CRT_MISS) {
-                // miss: background (camera.cuh:161-174); a missed pass-through re-trace keeps the
-                // light record, which does not scatter and adds colour(0,0,0,0)
-                if (!passthrough) L = L + att * v3(sc->background[0], sc->background[1], sc->background[2]);
-            } else {
-                bounce += 1u;
-                if (MODE == CRT_RNG_PCG) {
-                    uint32_t W = R.width;
-                    uint32_t jj = idv.x / W, ii = idv.x - jj * W;
-                    uint32_t ref_pixel = (R.height - 1u - jj) * W + ii;
-                    g.pcg = pcg_keyed(R.seed, ref_pixel, idv.y & 0xffffu, bounce);
-                }
-                uint32_t type = CRT_HIT_TYPE(prim), idx = CRT_HIT_INDEX(prim);
-                float t = hv.x;
-                V3 p = o + t * d;
-                V3 outward;
-                uint32_t mat_id;
-                float tu = hv.z, tvv = hv.w;  // texture coordinates (filled lazily below)
-                bool need_uv_sphere = false;
-                bool is_light_quad = false;
-                V3 sphere_c; float sphere_r = 1.0f;
-                uint32_t tri_flags = 0;
-                float4 tq0, tq1, tq2;
-                if (type == CRT_PRIM_TRI) {
-                    const float4* tp = (const float4*)sc->tris + 3 * idx;
-                    tq0 = tp[0]; tq1 = tp[1]; tq2 = tp[2];
-                    mat_id = fbits(tq1.w);
-                    tri_flags = fbits(tq2.w);
-                    // flat normal = unit(cross(v0 - v1, v0 - v2)) (core/src/primitives/triangle.cuh:118)
-                    V3 a = v3(tq0), b = v3(tq1), c = v3(tq2);
-                    V3 nrm = unit(cross(a - b, a - c));
-                    if (tri_flags & CRT_TF_SMOOTH) {
-                        const float4* tn = (const float4*)sc->tri_normals + 3 * idx;
-                        // u weights v1, v weights v2 (triangle.cuh:167-174), not renormalised
-                        nrm = hv.z * v3(tn[1]) + hv.w * v3(tn[2]) + (1.0f - hv.z - hv.w) * v3(tn[0]);
-                    }
-                    if (inst != CRT_NO_INST) {
-                        // object -> world normal: transpose of the world -> object 3x3
-                        const float4* ip = (const float4*)sc->insts + 4 * inst;
-                        float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
-                        int ovr = (int)fbits(ip[3].z);
-                        if (ovr >= 0) mat_id = (uint32_t)ovr;
-                        V3 nw = v3(r0.x * nrm.x + r1.x * nrm.y + r2.x * nrm.z, r0.y * nrm.x + r1.y * nrm.y + r2.y * nrm.z,
-                                   r0.z * nrm.x + r1.z * nrm.y + r2.z * nrm.z);
-                        nrm = (tri_flags & CRT_TF_SMOOTH) ? nw : unit(nw);
-                    }
-                    outward = nrm;
-                } else if (type == CRT_PRIM_SPHERE) {
-                    float4 s = ((const float4*)sc->spheres)[idx];
-                    uint4 aux = ((const uint4*)sc->sphere_aux)[idx];
-                    mat_id = aux.y;
-                    sphere_c = v3(s); sphere_r = s.w;
-                    outward = (p - sphere_c) / sphere_r;
-                    need_uv_sphere = true;
-                } else {
-                    const float4* q = (const float4*)sc->quads + 5 * idx;
-                    float4 q1 = q[1], q3 = q[3], q4 = q[4];
-                    mat_id = fbits(q4.w);
-                    outward = v3(q3);
-                    is_light_quad = (fbits(q1.w) & CRT_QF_LIGHT) != 0;
-                }
-                bool front = dot(d, outward) < 0.0f;
-                V3 nrm = front ? outward : -outward;
-                const float4* mats = (const float4*)sc->mats;
-                Mat m;
-                m.m0 = mats[2 * mat_id];
-                m.m1 = *(const int4*)(mats + 2 * mat_id + 1);
-                if (m.m1.x == CRT_MAT_DIFFUSE_LIGHT) {
-                    // emitter: light does not scatter (material::scatter returns false)
-                    if (!passthrough) {
-                        bool one_sided = (m.m1.z & (int)CRT_MATF_ONE_SIDED) != 0 || is_light_quad;
-                        if (!one_sided || front) L = L + att * (v3(m.m0) * m.m0.w);
-                    }
-                } else {
-                    // albedo from texture (lambertian/metal use the diffuse texture)
-                    V3 alb = v3(m.m0);
-                    if (m.m1.y >= 0) {
-                        if (need_uv_sphere) {
-                            // get_sphere_uv on the outward normal (core/src/utilities/uvmapping.cuh:204-228)
-                            const float2* mp = (const float2*)sc->sphere_map + 3 * idx;
-                            float2 scl = mp[0], ofs = mp[1], rep = mp[2];
-                            float theta = acosf(-outward.y);
-                            float phi = atan2f(-outward.z, outward.x) + 3.14159265358979323846f;
-                            float s_ = phi / (2.0f * 3.14159265358979323846f);
-                            float t_ = theta / 3.14159265358979323846f;
-                            s_ = fmodf(s_ * rep.x, 1.0f);
-                            t_ = fmodf(t_ * rep.y, 1.0f);
-                            tu = scl.x * s_ + ofs.x;
-                            tvv = scl.y * t_ + ofs.y;
-                        } else if (type == CRT_PRIM_TRI) {
-                            tu = 0.0f; tvv = 0.0f;
-                            if (tri_flags & CRT_TF_UV) {
-                                const float2* uvp = (const float2*)sc->tri_uvs + 3 * idx;
-                                float2 a = uvp[0], b = uvp[1], c = uvp[2];
-                                float w0 = 1.0f - hv.z - hv.w;
-                                tu = w0 * a.x + hv.z * b.x + hv.w * c.x;
-                                tvv = w0 * a.y + hv.z * b.y + hv.w * c.y;
-                                tu = fmodf(tu, 1.0f); tvv = fmodf(tvv, 1.0f);
-                                if (tu < 0.0f) tu += 1.0f;
-                                if (tvv < 0.0f) tvv += 1.0f;
-                            }
-                        }
-                        float4 tx = tex_value((const float4*)sc->texs, sc->texels, m.m1.y, tu, tvv, p);
-                        alb = v3(tx);
-                    }
-                    if (m.m1.x == CRT_MAT_METAL) {
-                        // metal::scatter (core/src/materials/metal.cuh:57-65): never absorbs
-                        V3 ud = unit(d);
-                        V3 refl = ud - (nrm * dot(nrm, ud)) * 2.0f;
-                        float x, y, z;
-                        do {
-                            float a = g.uniform(), b = g.uniform(), c = g.uniform();
-                            x = 2.0f * a - 1.0f; y = 2.0f * b - 1.0f; z = 2.0f * c - 1.0f;
-                        } while (x * x + y * y + z * z >= 1.0f);
-                        d = refl + m.m0.w * v3(x, y, z);
-                        att = att * alb;
-                        done = false;
-                    } else if (m.m1.x == CRT_MAT_DIELECTRIC) {
-                        // dielectric::scatter (core/src/materials/dielectric.cuh:32-51)
-                        float ir = m.m0.w;
-                        float ratio = front ? (1.0f / ir) : ir;
-                        V3 ud = unit(d);
-                        float dn = dot(-ud, nrm);
-                        float cos_t = dn < 1.0f ? dn : 1.0f;
-                        float sin_t = sqrtf(1.0f - cos_t * cos_t);
-                        bool reflect_ray = ratio * sin_t > 1.0f;
-                        if (!reflect_ray) {
-                            float r0 = (1.0f - ratio) / (1.0f + ratio);
-                            r0 = r0 * r0;
-                            float x = 1.0f - cos_t;
-                            float x5 = powf(x, 5.0f);
-                            reflect_ray = (r0 + (1.0f - r0) * x5) > g.uniform();
-                        }
-                        if (reflect_ray) {
-                            d = ud - (nrm * dot(nrm, ud)) * 2.0f;
-                        } else {
-                            float dd = dot(nrm, ud);
-                            float kk = 1.0f - ratio * ratio * (1.0f - dd * dd);
-                            d = kk >= 0.0f ? (ratio * ud - (ratio * dd + sqrtf(kk)) * nrm) : v3(0.0f, 0.0f, 0.0f);
-                        }
-                        done = false;
-                    } else if (m.m1.x == CRT_MAT_LAMBERTIAN) {
-                        // mixture of light sampling and cosine sampling (camera.cuh:203-216)
-                        // onb::build_from_w (core/src/misc/onb.cuh:122-128)
-                        V3 w = unit(nrm);
-                        V3 aa = fabsf(w.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
-                        V3 bv = unit(cross(w, aa));
-                        V3 bu = cross(w, bv);
-                        const uint32_t nl = sc->num_lights;
-                        const float4* qd = (const float4*)sc->quads;
-                        V3 dir;
-                        if (g.uniform() < 0.5f) {
-                            // hittable_list::random -> light 0 (get_int quirk) -> directional_light::random
-                            if (nl == 0) {
-                                dir = v3(0.0f, 0.0f, 0.0f);
-                            } else {
-                                if (nl > 1) g.skip();
-                                const float4* q = qd + 5 * sc->light_quads[0];
-                                float a = g.uniform(), b = g.uniform();
-                                V3 pp = v3(q[0]) + (a - 0.5f) * v3(q[1]) + (b - 0.5f) * v3(q[2]);
-                                dir = pp - p;
-                            }
-                        } else {
-                            // random_cosine_direction (core/src/misc/gpu_randomizer.cuh:122-133)
-                            float r1 = g.uniform(), r2 = g.uniform();
-                            float phi = 6.28318530718f * r1;
-                            float z = sqrtf(1.0f - r2);
-                            float rs = sqrtf(r2);
-                            float x = cosf(phi) * rs, y = sinf(phi) * rs;
-                            dir = x * bu + y * bv + z * w;
-                        }
-                        // light pdf (hittable_list::pdf_value -> directional_light::pdf_value)
-                        float weight = 1.0f / (float)nl;
-                        float sum = 0.0f;
-                        for (uint32_t li = 0; li < nl; ++li) {
-                            const float4* q = qd + 5 * sc->light_quads[li];
-                            float tl, qa, qb;
-                            light_tests++;
-                            float pv = 0.0f;
-                            if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb)) {
-                                float dist2 = tl * tl * len2(dir);
-                                float cosine = fabsf(dot(dir, v3(q[3])) / vlen(dir));
-                                pv = dist2 / (cosine * q[2].w);
-                            }
-                            sum += weight * pv;
-                        }
-                        V3 ud = unit(dir);
-                        float cv = dot(ud, w) / 3.14159265358979323846f;
-                        float pc = 0.0f > cv ? 0.0f : cv;
-                        float pdf = 0.5f * sum + (1.0f - 0.5f) * pc;
-                        float ct = dot(nrm, ud);
-                        float spdf = ct < 0.0f ? 0.0f : ct / 3.14159265358979323846f;
-                        att = att * ((1.0f / pdf) * (alb * spdf));
-                        d = dir;
-                        done = false;
-                    }
-                }
-                if (!done) {
-                    o = p;
-                    depth -= 1u;
-                    if (depth == 0u) done = true;
-                }
-            }
+            bool done = !shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
             if (!done) {
                 uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8);
                 ray_o[slot] = make_float4(o.x, o.y, o.z, ro.w);
@@ -905,14 +1071,77 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                 rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
                 keep = true;
             } else {
-                rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-                if (MODE == CRT_RNG_MINSTD) R.rng_state[idv.x] = g.ms;
+                // finished: the radiance goes to the path's id, the slot returns to the free list
+                uint32_t pid = R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin);
+                out_rad[pid] = make_float4(L.x, L.y, L.z, 0.0f);
+                if (MODE == CRT_RNG_MINSTD) R.rng_state[pix] = g.ms;
+                freed = true;
             }
         }
         uint32_t pos = block_append<BLOCK>(keep, count_out, s_wave, &s_base);
         if (keep) queue_out[pos] = slot;
+        uint32_t fpos = block_append<BLOCK>(freed, nfree_out, s_wave, &s_base);
+        if (freed) freelist[fpos] = slot;
     }
-    if (light_tests) atomicAdd(&counters->light_tests, (unsigned long long)light_tests);
+    wave_add(&counters->light_tests, light_tests);
+}
+
+// One-render-per-launch path tracer, the reference's render kernel design (core/src/gpu.cu:756-801):
+// each thread owns pixels, runs their samples in order and follows every path to its end, tracing
+// and shading in one kernel. Same estimator, streams and per-pixel float sums as the wavefront
+// pipeline, so the images agree; it serves as the plain baseline the batched passes are measured
+// against. Grid-stride over the request's pixel range.
+template <int MODE, int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
+__global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict__ scenes,
+                                                 const crt_dev_request* __restrict__ req, uint32_t* __restrict__ gstack,
+                                                 crt_dev_counters* __restrict__ counters) {
+    __shared__ uint32_t s_stack[LS * BLOCK];
+    const crt_dev_request& R = *req;
+    const crt_dev_scene* sc = scenes + R.scene;
+    const SceneView S = scene_view(sc);
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t stride = gridDim.x * BLOCK;
+    uint32_t* lds_col = s_stack + threadIdx.x;
+    uint32_t* gst = gstack + (size_t)gtid * GS;
+    TravCount tc = {0, 0, 0, 0, 0, 0};
+    uint32_t nrays = 0, nretrace = 0, novf = 0, light_tests = 0;
+    float4* accum = (float4*)R.accum;
+    for (uint32_t q = gtid; q < R.pix_count; q += stride) {
+        uint32_t pix = R.pix_begin + q;
+        float4 acc = accum[pix];
+        Rng<MODE> g;
+        if (MODE == CRT_RNG_MINSTD) g.ms = R.rng_state[pix];
+        const uint32_t refp = ref_pixel_of(R, pix);
+        for (uint32_t k = 0; k < R.sample_count; ++k) {
+            uint32_t sample = R.sample_begin + k;
+            if (MODE == CRT_RNG_PCG) g.pcg = pcg_keyed(R.seed, refp, sample, 0u);
+            V3 o, d;
+            float tm;
+            camera_ray<MODE>(R, pix, sample, g, o, d, tm);
+            V3 att = v3(1.0f, 1.0f, 1.0f), L = v3(0.0f, 0.0f, 0.0f);
+            uint32_t depth = R.max_depth & 0xffu, bounce = 0u;
+            bool camera = true;
+            for (;;) {
+                HitRec h;
+                bool ovf = false;
+                uint32_t pf = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst, h, tc,
+                                                                                    ovf, nrays, nretrace);
+                novf += ovf ? 1u : 0u;
+                camera = false;
+                float4 hv = make_float4(h.t, bitsf(h.prim), h.u, h.v);
+                if (!shade_hit<MODE>(sc, R, pix, sample, hv, h.inst | pf, o, d, att, L, depth, bounce, g, light_tests))
+                    break;
+            }
+            acc.x += L.x; acc.y += L.y; acc.z += L.z;
+        }
+        acc.w += (float)R.sample_count;
+        accum[pix] = acc;
+        if (MODE == CRT_RNG_MINSTD) R.rng_state[pix] = g.ms;
+    }
+    wave_add(&counters->rays, nrays);
+    wave_add(&counters->retraces, nretrace);
+    wave_add(&counters->light_tests, light_tests);
+    wave_add(&counters->stack_overflow, novf);
 }
 
 // Per-pixel sums in sample order: accum += rad[sample 0] + ... (same float additions as the
@@ -978,14 +1207,21 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 #define STK_LS 16
 #define STK_GS 96
 
-extern "C" int crt_k_generate(hipStream_t s, int rng_mode, const crt_dev_request* reqs, int nreq, uint32_t npaths,
-                              float4* ray_o, float4* ray_d, float4* thr, float4* rad, uint2* ids) {
-    if (npaths == 0) return 0;
-    dim3 grid((npaths + 255) / 256);
+extern "C" int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t* freelist, uint32_t nslots) {
+    uint32_t n = nslots > 2 * CRT_CTL_WORDS ? nslots : 2 * CRT_CTL_WORDS;
+    hipLaunchKernelGGL(k_pass_init, dim3((n + 255) / 256), dim3(256), 0, s, ctl, freelist, nslots);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq,
+                           uint32_t total, uint32_t* ctl_in, uint32_t* ctl_out, const uint32_t* freelist,
+                           uint32_t* queue, float4* ray_o, float4* ray_d, float4* thr, float4* rad, uint2* ids) {
     if (rng_mode == CRT_RNG_MINSTD)
-        hipLaunchKernelGGL(k_generate<CRT_RNG_MINSTD>, grid, dim3(256), 0, s, reqs, nreq, npaths, ray_o, ray_d, thr, rad, ids);
+        hipLaunchKernelGGL(k_regen<CRT_RNG_MINSTD>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, ctl_in, ctl_out,
+                           freelist, queue, ray_o, ray_d, thr, rad, ids);
     else
-        hipLaunchKernelGGL(k_generate<CRT_RNG_PCG>, grid, dim3(256), 0, s, reqs, nreq, npaths, ray_o, ray_d, thr, rad, ids);
+        hipLaunchKernelGGL(k_regen<CRT_RNG_PCG>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, ctl_in, ctl_out,
+                           freelist, queue, ray_o, ray_d, thr, rad, ids);
     return (int)hipGetLastError();
 }
 
@@ -993,14 +1229,22 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                             const crt_dev_scene* scenes, const crt_dev_request* reqs, int32_t uniform_scene,
                             const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                             const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
-                            uint32_t* zero_counter, crt_dev_counters* counters) {
+                            uint32_t* zero_counter, uint32_t* work_counter, uint32_t refill_min,
+                            crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
-#define CRT_EXT(C, U, W, T)                                                                                       \
-    hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, C, U, W, T>), dim3(grid), dim3(256), 0, s, scenes, reqs, \
-                       us, queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, counters)
+    // work_counter != null selects the persistent lane-refill kernel, else the grid-stride one
+#define CRT_EXT(C, U, W, T)                                                                                            \
+    if (work_counter)                                                                                                   \
+        hipLaunchKernelGGL((k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>), dim3(grid), dim3(256), 0, s, scenes,  \
+                           reqs, us, queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack,          \
+                           zero_counter, work_counter, refill_min, counters);                                           \
+    else                                                                                                                \
+        hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, C, U, W, T>), dim3(grid), dim3(256), 0, s, scenes,     \
+                           reqs, us, queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack,          \
+                           zero_counter, counters)
 #define CRT_EXT_WT(C, U)                                                   \
-    if (width == 2) { if (two_level) CRT_EXT(C, U, 2, true); else CRT_EXT(C, U, 2, false); } \
-    else { if (two_level) CRT_EXT(C, U, 4, true); else CRT_EXT(C, U, 4, false); }
+    if (width == 2) { if (two_level) { CRT_EXT(C, U, 2, true); } else { CRT_EXT(C, U, 2, false); } } \
+    else { if (two_level) { CRT_EXT(C, U, 4, true); } else { CRT_EXT(C, U, 4, false); } }
     if (count_mode) {
         if (uniform_scene >= 0) { CRT_EXT_WT(true, true) } else { CRT_EXT_WT(true, false) }
     } else {
@@ -1015,13 +1259,15 @@ extern "C" int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt
                            const crt_dev_request* reqs, const uint32_t* queue_in, const uint32_t* count_in,
                            uint32_t count_direct, uint32_t* queue_out, uint32_t* count_out, float4* ray_o, float4* ray_d,
                            float4* thr, float4* rad, const uint2* ids, const float4* hit, const uint32_t* hit_inst,
-                           crt_dev_counters* counters) {
+                           uint32_t* freelist, uint32_t* nfree_out, float4* out_rad, crt_dev_counters* counters) {
     if (rng_mode == CRT_RNG_MINSTD)
         hipLaunchKernelGGL((k_shade<CRT_RNG_MINSTD, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue_in, count_in,
-                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, counters);
+                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, freelist,
+                           nfree_out, out_rad, counters);
     else
         hipLaunchKernelGGL((k_shade<CRT_RNG_PCG, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue_in, count_in,
-                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, counters);
+                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, freelist,
+                           nfree_out, out_rad, counters);
     return (int)hipGetLastError();
 }
 
@@ -1034,6 +1280,21 @@ extern "C" int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_
     if (width == 2) { if (two_level) CRT_TR(2, true); else CRT_TR(2, false); }
     else { if (two_level) CRT_TR(4, true); else CRT_TR(4, false); }
 #undef CRT_TR
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level, uint32_t grid,
+                           const crt_dev_scene* scenes, const crt_dev_request* req, uint32_t* gstack,
+                           crt_dev_counters* counters) {
+#define CRT_PL(M, W, T)                                                                                          \
+    hipLaunchKernelGGL((k_plain<M, 256, STK_RS, STK_LS, STK_GS, W, T>), dim3(grid), dim3(256), 0, s, scenes, req, \
+                       gstack, counters)
+#define CRT_PL_WT(M)                                                              \
+    if (width == 2) { if (two_level) CRT_PL(M, 2, true); else CRT_PL(M, 2, false); } \
+    else { if (two_level) CRT_PL(M, 4, true); else CRT_PL(M, 4, false); }
+    if (rng_mode == CRT_RNG_MINSTD) { CRT_PL_WT(CRT_RNG_MINSTD) } else { CRT_PL_WT(CRT_RNG_PCG) }
+#undef CRT_PL_WT
+#undef CRT_PL
     return (int)hipGetLastError();
 }
 

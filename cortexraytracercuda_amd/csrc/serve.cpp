@@ -18,6 +18,8 @@
 namespace crt {
 
 int Context::submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_id* out) {
+    if (rd->flags & CRT_RENDER_MEGAKERNEL)
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "the megakernel render is for blocking calls (crt_render)");
     int rc = Job::validate(*rd);
     if (rc) return rc;
     uint64_t need = Job::bytes_needed(*rd);

@@ -39,7 +39,8 @@ def render_desc(width, height, spp, max_depth, seed=1, rng="pcg", sample_begin=0
 class Renderer:
     """One device context: resident scenes, path-state buffers and the serving loop."""
 
-    def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0):
+    def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0,
+                 extend_kernel=0, refill_min=0, pass_paths=0):
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -48,6 +49,9 @@ class Renderer:
         cfg.count_work = 1 if count_work else 0
         cfg.leaf_size = int(leaf_size)
         cfg.bvh_width = int(bvh_width)
+        cfg.extend_kernel = int(extend_kernel)
+        cfg.refill_min = int(refill_min)
+        cfg.pass_paths = int(pass_paths)
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -96,9 +100,11 @@ class Renderer:
 
     # ---------------------------------------------------------------- rendering
     def render(self, sid, width, height, spp, max_depth, seed=1, rng="pcg", sample_begin=0, sample_count=0,
-               want_accum=True, time_stages=False):
-        rd = render_desc(width, height, spp, max_depth, seed, rng, sample_begin, sample_count,
-                         A.CRT_RF_TIME_STAGES if time_stages else 0)
+               want_accum=True, time_stages=False, megakernel=False):
+        """Blocking render. megakernel=True renders the request with one kernel launch (one thread
+        per pixel running its samples and bounces in turn) instead of the wavefront passes."""
+        flags = (A.CRT_RF_TIME_STAGES if time_stages else 0) | (A.CRT_RF_MEGAKERNEL if megakernel else 0)
+        rd = render_desc(width, height, spp, max_depth, seed, rng, sample_begin, sample_count, flags)
         rgb = np.zeros((height, width, 3), np.uint8)
         acc = np.zeros((height, width, 4), np.float32) if want_accum else None
         img = A.crt_image(rgb.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(acc) if acc is not None else None)

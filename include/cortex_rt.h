@@ -187,8 +187,17 @@ typedef struct crt_render_desc {
     uint32_t rng_mode;     /* CRT_RNG_* */
     uint32_t sample_begin; /* first sample index of this call (progressive rendering) */
     uint32_t sample_count; /* 0 = all remaining samples */
-    uint32_t flags;        /* reserved, 0 */
+    uint32_t flags;        /* CRT_RENDER_* bits */
 } crt_render_desc;
+
+/* crt_render_desc.flags */
+enum {
+    CRT_RENDER_TIME_STAGES = 1u,  /* time the closest-hit launches with HIP events (crt_stats.extend_ms) */
+    CRT_RENDER_MEGAKERNEL = 2u    /* blocking renders only: one kernel launch renders the request, one thread per
+                                     pixel running its samples and bounces in turn (the reference's render kernel
+                                     design, core/src/gpu.cu:756-801) instead of the wavefront passes. Same
+                                     estimator and streams: the image agrees with the wavefront render. */
+};
 
 typedef struct crt_image {
     uint8_t* rgb8;         /* optional: width*height*3, tonemapped like core/src/gpu.cu:792-799 */
@@ -215,12 +224,19 @@ typedef uint64_t crt_request_id;
 typedef struct crt_config {
     int32_t device;             /* HIP device ordinal */
     uint64_t hbm_budget_bytes;  /* admission budget for scenes + requests; 0 = 90% of free memory */
-    uint32_t max_paths;         /* path slots per wavefront pass; 0 = autotuned default */
+    uint32_t max_paths;         /* live path slots of the streaming passes (path state pools); 0 = 4M */
     uint32_t block_size;        /* threads per block of the stage kernels; 0 = default (256) */
     uint32_t count_work;        /* 1: use the counting traversal kernel (fills node_visits/prim_tests) */
     uint32_t leaf_size;         /* BVH max primitives per leaf; 0 = default (4) */
     uint32_t bvh_width;         /* children per BVH node, 2 or 4; 0 = default (4) */
+    uint32_t extend_kernel;     /* closest-hit stage: 0 = default (grid-stride), 1 = grid-stride (each lane
+                                   traces its rays to the end, the wave waits for its slowest lane),
+                                   2 = persistent with per-lane refill from a work counter */
+    uint32_t refill_min;        /* persistent stage: idle lanes of a wave before it fetches rays; 0 = default */
+    uint32_t pass_paths;        /* path ids per pass (16 B of radiance each); 0 = min(2^27, 32 * max_paths) */
 } crt_config;
+
+enum { CRT_EXTEND_DEFAULT = 0, CRT_EXTEND_GRID_STRIDE = 1, CRT_EXTEND_PERSISTENT = 2 };
 
 int crt_create(const crt_config* cfg, crt_context** out);
 int crt_destroy(crt_context* ctx);
@@ -325,6 +341,7 @@ typedef struct crt_work_counters {
     uint64_t rays, retraces, light_pdf_tests;
     uint64_t node_visits, leaf_visits, tri_tests, sphere_tests, quad_tests, inst_visits;
     uint64_t extend_launches, passes;
+    uint64_t iterations;        /* regenerate/extend/shade rounds of the streaming passes */
 } crt_work_counters;
 int crt_last_counters(crt_context* ctx, crt_work_counters* out);
 

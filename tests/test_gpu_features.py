@@ -233,3 +233,52 @@ def test_cli_renders_scene_file_like_the_library(tmp_path):
     # protocol: p x y r g b with y counted from the top row of the PNG
     x, y, cr, cg, cb = map(int, lines[64 * 10 + 5].split()[1:])
     assert (x, y) == (5, 10) and [cr, cg, cb] == list(rgb[10, 5])
+
+
+def _same_fraction(a, b, tol=1e-4):
+    return np.all(np.abs(a - b) <= tol * np.maximum(np.abs(b), 1.0), axis=-1).mean()
+
+
+@pytest.mark.parametrize("rng", ["pcg", "minstd"])
+def test_megakernel_matches_wavefront(rng, box):
+    # the one-launch render runs the same estimator on the same streams: the images agree up to
+    # rare last-bit differences of fused arithmetic that change a path
+    with Renderer() as r:
+        sid = r.upload(box)
+        _, acc_w, st_w = r.render(sid, 96, 80, 16, 8, seed=4, rng=rng)
+        _, acc_m, st_m = r.render(sid, 96, 80, 16, 8, seed=4, rng=rng, megakernel=True)
+        big = r.upload(scenes.large_scene(levels=5, n_spheres=200))
+        _, acc_w2, _ = r.render(big, 120, 68, 4, 10, seed=9)
+        _, acc_m2, st_m2 = r.render(big, 120, 68, 4, 10, seed=9, megakernel=True)
+    f1, f2 = _same_fraction(acc_m, acc_w), _same_fraction(acc_m2, acc_w2)
+    print("megakernel vs wavefront (%s): box %.4f large %.4f, rays %d vs %d" % (rng, f1, f2, st_m["rays"], st_w["rays"]))
+    assert np.array_equal(acc_m[..., 3], acc_w[..., 3])
+    assert f1 >= 0.99 and f2 >= 0.99
+    assert abs(st_m["rays"] - st_w["rays"]) <= 0.01 * st_w["rays"]
+
+
+def test_megakernel_instanced_and_progressive():
+    s = scenes.instanced_scene(seed=2, n_instances=32, n_textures=2, tex_size=64, nu=12, nv=8, extent=8.0)
+    s.set_camera((0.0, 3.0, 30.0), (0.0, 0.0, 0.0), vfov=50.0, aspect_ratio=1.0, focus_dist=10.0)
+    with Renderer() as r:
+        sid = r.upload(s)
+        _, acc_w, _ = r.render(sid, 64, 64, 9, 6, seed=3)
+        # progressive: two megakernel calls over sample ranges sum like one full render
+        _, a1, _ = r.render(sid, 64, 64, 9, 6, seed=3, sample_begin=0, sample_count=4, megakernel=True)
+        _, a2, _ = r.render(sid, 64, 64, 9, 6, seed=3, sample_begin=4, sample_count=5, megakernel=True)
+    acc_m = a1 + a2
+    assert np.array_equal(acc_m[..., 3], acc_w[..., 3])
+    f = np.all(np.abs(acc_m - acc_w) <= 1e-3 * np.maximum(np.abs(acc_w), 1.0), axis=-1).mean()
+    print("megakernel instanced progressive vs wavefront: %.4f" % f)
+    assert f >= 0.99
+
+
+def test_megakernel_not_served():
+    import ctypes
+    with Renderer() as r:
+        sid = r.upload(scenes.box_scene())
+        rd = A.crt_render_desc()
+        rd.width = rd.height = 8
+        rd.spp, rd.max_depth, rd.flags = 1, 2, A.CRT_RF_MEGAKERNEL
+        rid = ctypes.c_uint64()
+        assert r.lib.crt_submit(r.h, sid, ctypes.byref(rd), ctypes.byref(rid)) == A.CRT_ERR_INVALID_ARGUMENT
