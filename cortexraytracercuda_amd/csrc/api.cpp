@@ -152,24 +152,23 @@ int Context::init(const crt_config* cfg) {
 
 uint64_t Context::used_bytes() const {
     uint64_t b = scene_table.bytes + counters.bytes + ctl.bytes + gstack.bytes + req_table[0].bytes + req_table[1].bytes;
-    b += ray_o.bytes + ray_d.bytes + thr.bytes + rad.bytes + ids.bytes + hit.bytes + hit_inst.bytes + queue[0].bytes +
-         queue[1].bytes + freelist.bytes + out_rad.bytes;
+    for (int k = 0; k < 2; ++k) b += ray_o[k].bytes + ray_d[k].bytes + thr[k].bytes + rad[k].bytes + ids[k].bytes;
+    b += hit.bytes + hit_inst.bytes + out_rad.bytes;
     return b;
 }
 
 int Context::ensure_paths(uint32_t n, uint32_t pass_ids) {
     int rc;
     if (n > path_capacity) {
-        if ((rc = ray_o.alloc((size_t)n * 16, this))) return rc;
-        if ((rc = ray_d.alloc((size_t)n * 16, this))) return rc;
-        if ((rc = thr.alloc((size_t)n * 16, this))) return rc;
-        if ((rc = rad.alloc((size_t)n * 16, this))) return rc;
-        if ((rc = ids.alloc((size_t)n * 8, this))) return rc;
+        for (int k = 0; k < 2; ++k) {
+            if ((rc = ray_o[k].alloc((size_t)n * 16, this))) return rc;
+            if ((rc = ray_d[k].alloc((size_t)n * 16, this))) return rc;
+            if ((rc = thr[k].alloc((size_t)n * 16, this))) return rc;
+            if ((rc = rad[k].alloc((size_t)n * 16, this))) return rc;
+            if ((rc = ids[k].alloc((size_t)n * 8, this))) return rc;
+        }
         if ((rc = hit.alloc((size_t)n * 16, this))) return rc;
         if ((rc = hit_inst.alloc((size_t)n * 4, this))) return rc;
-        if ((rc = queue[0].alloc((size_t)n * 4, this))) return rc;
-        if ((rc = queue[1].alloc((size_t)n * 4, this))) return rc;
-        if ((rc = freelist.alloc((size_t)n * 4, this))) return rc;
         path_capacity = n;
     }
     if (pass_ids > pass_capacity) {
@@ -177,6 +176,13 @@ int Context::ensure_paths(uint32_t n, uint32_t pass_ids) {
         pass_capacity = pass_ids;
     }
     return CRT_OK;
+}
+
+crt_paths Context::paths(int k) {
+    crt_paths p;
+    p.ray_o = (float4*)ray_o[k].p; p.ray_d = (float4*)ray_d[k].p; p.thr = (float4*)thr[k].p;
+    p.rad = (float4*)rad[k].p; p.ids = (uint2*)ids[k].p;
+    return p;
 }
 
 Context::~Context() {
@@ -375,13 +381,11 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
     int nreq = (int)items.size();
     const uint32_t nslots = std::min(path_capacity, npaths);
     uint32_t* cb = (uint32_t*)ctl.p;
-    uint32_t* fl = (uint32_t*)freelist.p;
-    if (crt_k_pass_init(stream, cb, fl, nslots)) return set_error(CRT_ERR_HIP, "pass init launch failed");
+    if (crt_k_pass_init(stream, cb)) return set_error(CRT_ERR_HIP, "pass init launch failed");
     const uint32_t grid = std::min(extend_grid, std::max(1u, (nslots + 255) / 256));
     // every path finishes within max_depth iterations of entering; a slot hosts at most
     // ceil(npaths / nslots) paths in turn
     const uint64_t max_iter = (uint64_t)max_depth * ((npaths + nslots - 1) / nslots + 1) + 2;
-    int qi = 0;
     for (uint64_t k = 0;; ++k) {
         if (k >= (uint64_t)kCtlLag) {
             // the pass has drained once an iteration ends with every path generated and none alive
@@ -393,16 +397,14 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
         if (k > max_iter) return set_error(CRT_ERR_INTERNAL, "pass did not drain");
         uint32_t* cin = cb + (k & 1) * CRT_CTL_WORDS;
         uint32_t* cout = cb + ((k + 1) & 1) * CRT_CTL_WORDS;
-        uint32_t* qcur = (uint32_t*)queue[qi].p;
-        uint32_t* qnext = (uint32_t*)queue[qi ^ 1].p;
-        if (crt_k_regen(stream, mode, grid, dreq, nreq, npaths, cin, cout, fl, qcur, (float4*)ray_o.p, (float4*)ray_d.p,
-                        (float4*)thr.p, (float4*)rad.p, (uint2*)ids.p))
+        const crt_paths A = paths((int)(k & 1)), B = paths((int)((k + 1) & 1));
+        if (crt_k_regen(stream, mode, grid, dreq, nreq, npaths, nslots, cin, cout, A))
             return set_error(CRT_ERR_HIP, "regen launch failed");
         if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, grid, dsc, dreq, uniform_scene, qcur,
-                         cin + CRT_CTL_LIVE, 0, (const float4*)ray_o.p, (const float4*)ray_d.p, (const uint2*)ids.p,
-                         (float4*)hit.p, (uint32_t*)hit_inst.p, (uint32_t*)gstack.p, nullptr,
-                         extend_persistent ? cin + CRT_CTL_WORK : nullptr, refill_min, (crt_dev_counters*)counters.p))
+        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, grid, dsc, dreq, uniform_scene, nullptr,
+                         cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, (float4*)hit.p, (uint32_t*)hit_inst.p,
+                         (uint32_t*)gstack.p, nullptr, extend_persistent ? cin + CRT_CTL_WORK : nullptr, refill_min,
+                         (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "extend launch failed");
         if (time_stages) {
             HIP_TRY(hipEventRecord(ev_ext[1], stream));
@@ -412,15 +414,12 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             if (extend_ms) *extend_ms += ms;
             extend_launches++;
         }
-        if (crt_k_shade(stream, mode, grid, dsc, dreq, qcur, cin + CRT_CTL_LIVE, 0, qnext, cout + CRT_CTL_SURV,
-                        (float4*)ray_o.p, (float4*)ray_d.p, (float4*)thr.p, (float4*)rad.p, (const uint2*)ids.p,
-                        (const float4*)hit.p, (const uint32_t*)hit_inst.p, fl, cout + CRT_CTL_NFREE, (float4*)out_rad.p,
-                        (crt_dev_counters*)counters.p))
+        if (crt_k_shade(stream, mode, grid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, (const float4*)hit.p,
+                        (const uint32_t*)hit_inst.p, (float4*)out_rad.p, (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "shade launch failed");
         HIP_TRY(hipMemcpyAsync(ctl_host + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
                                hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipEventRecord(ctl_ev[k % kCtlRing], stream));
-        qi ^= 1;
         iterations++;
     }
     if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)out_rad.p)) return set_error(CRT_ERR_HIP, "resolve launch failed");

@@ -1,32 +1,38 @@
 // crt_kernels.h - host-side launchers of the gfx950 kernels (kernels.hip). Each returns the
 // hipError_t of the launch as an int.
 
-// words of one half of the streaming pass control block (two halves alternate)
-#define CRT_CTL_WORDS 8
-#define CRT_CTL_SURV 0
-#define CRT_CTL_NFREE 1
-#define CRT_CTL_GEN 2
-#define CRT_CTL_LIVE 3
-#define CRT_CTL_WORK 4
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "crt_layout.h"
 
+// words of one half of the streaming pass control block (two halves alternate)
+#define CRT_CTL_WORDS 8
+#define CRT_CTL_SURV 0   // survivors written to the front of the other state buffer by the shade stage
+#define CRT_CTL_GEN 2    // next path id of the pass to generate
+#define CRT_CTL_LIVE 3   // paths in this iteration's state buffer (survivors + new)
+#define CRT_CTL_WORK 4   // ray hand-out counter of the persistent closest-hit stage
+
+// SoA path state of one buffer (two buffers alternate between iterations)
+struct crt_paths {
+    float4* ray_o;   // origin, time
+    float4* ray_d;   // direction, state bits (depth | bounce << 8 | camera << 16)
+    float4* thr;     // throughput, minstd engine state
+    float4* rad;     // radiance gathered so far
+    uint2* ids;      // storage pixel, request << 16 | sample
+};
+
 extern "C" {
-int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t* freelist, uint32_t nslots);
+int crt_k_pass_init(hipStream_t s, uint32_t* ctl);
 int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq, uint32_t total,
-                uint32_t* ctl_in, uint32_t* ctl_out, const uint32_t* freelist, uint32_t* queue, float4* ray_o,
-                float4* ray_d, float4* thr, float4* rad, uint2* ids);
+                uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out, crt_paths P);
 int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                  const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                  uint32_t* zero_counter, uint32_t* work_counter, uint32_t refill_min, crt_dev_counters* counters);
 int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes, const crt_dev_request* reqs,
-                const uint32_t* queue_in, const uint32_t* count_in, uint32_t count_direct, uint32_t* queue_out,
-                uint32_t* count_out, float4* ray_o, float4* ray_d, float4* thr, float4* rad, const uint2* ids,
-                const float4* hit, const uint32_t* hit_inst, uint32_t* freelist, uint32_t* nfree_out, float4* out_rad,
-                crt_dev_counters* counters);
+                const uint32_t* count_in, crt_paths Pin, crt_paths Pout, uint32_t* count_out, const float4* hit,
+                const uint32_t* hit_inst, float4* out_rad, crt_dev_counters* counters);
 int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* req, uint32_t* gstack, crt_dev_counters* counters);
 int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n, int camera,

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "cortex_rt.h"
+#include "crt_kernels.h"
 #include "crt_layout.h"
 #include "scene_build.h"
 
@@ -109,8 +110,9 @@ struct Context {
     void* counters_host = nullptr;
     int pass_parity = 0;
     uint64_t passes = 0, extend_launches = 0, iterations = 0;
-    // streaming pass: path slot pools (max_paths slots), free list, queues, per-path-id radiance
-    DevBuf ray_o, ray_d, thr, rad, ids, hit, hit_inst, queue[2], freelist, out_rad;
+    // streaming pass: two compact path-state buffers of max_paths slots (alternating between
+    // iterations), hit records, per-path-id radiance
+    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad;
     uint32_t pass_capacity = 0;      // path ids per pass (out_rad entries)
     static constexpr int kCtlRing = 8, kCtlLag = 4;   // control snapshots in flight / host look-behind
     uint32_t* ctl_host = nullptr;    // pinned: kCtlRing snapshots of the control block's "out" half
@@ -132,6 +134,7 @@ struct Context {
     ~Context();
     uint64_t used_bytes() const;
     int ensure_paths(uint32_t slots, uint32_t pass_ids);
+    crt_paths paths(int k);
     int upload_scene(const crt_scene_desc* desc, crt_scene_id* out);
     int release_scene(crt_scene_id id);
     DeviceScene* find_scene(crt_scene_id id);

@@ -5,8 +5,8 @@
 //   k_regen     camera rays into free path slots   (render kernel sample loop + get_ray)
 //   k_extend    BVH traversal, closest hit          (hittable_list/bvh_node/aabb/primitive ::hit)
 //   k_shade     emission, scatter, light mixture    (ray_color body, materials, pdfs, textures)
-//               + queue compaction (wave ballot, block prefix scan, one atomic per block); finished
-//               paths write their radiance by path id and return their slot to the free list
+//               + compaction of the survivors (wave ballot, block prefix scan, one atomic per
+//               block) into the other state buffer; finished paths write their radiance by path id
 // The stages repeat over a fixed pool of path slots until every path of the pass has finished:
 // slots freed by one iteration are refilled with new camera paths by the next, so each launch
 // works on a full queue instead of the thinning tail of one generation of paths.
@@ -523,38 +523,32 @@ __device__ __forceinline__ uint32_t ref_pixel_of(const crt_dev_request& R, uint3
 }
 
 // Streaming pass control block (CRT_CTL_* words, crt_kernels.h): two halves alternate between
-// iterations. Iteration k reads half k%2 ("in": survivors and free slots left by the previous shade
-// stage, next path id to generate) and prepares half (k+1)%2 ("out") for its own shade stage.
-// Start of a pass: every slot free, nothing generated.
-__global__ __launch_bounds__(256) void k_pass_init(uint32_t* __restrict__ ctl, uint32_t* __restrict__ freelist,
-                                                   uint32_t nslots) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nslots) freelist[i] = i;
-    if (i < 2 * CRT_CTL_WORDS) ctl[i] = (i == CRT_CTL_NFREE) ? nslots : 0u;
+// iterations. Iteration k reads half k%2 ("in": survivors left by the previous shade stage, next
+// path id to generate) and prepares half (k+1)%2 ("out") for its own shade stage. Live paths are
+// kept compact: the shade stage writes survivors to the front of the other state buffer and the
+// next regeneration appends new camera paths behind them, so every state access is coalesced.
+__global__ void k_pass_init(uint32_t* __restrict__ ctl) {
+    uint32_t i = threadIdx.x;
+    if (i < 2 * CRT_CTL_WORDS) ctl[i] = 0u;
 }
 
-// Regeneration: the free slots left by the last shade stage take the next path ids of the pass
-// (camera rays, in path-id order: sample-major within each request) and join the survivors in the
-// queue of this iteration.
+// Regeneration: the slots behind the survivors take the next path ids of the pass (camera rays,
+// in path-id order: sample-major within each request).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t total,
-                                               uint32_t* __restrict__ ctl_in, uint32_t* __restrict__ ctl_out,
-                                               const uint32_t* __restrict__ freelist, uint32_t* __restrict__ queue,
-                                               float4* __restrict__ ray_o, float4* __restrict__ ray_d,
-                                               float4* __restrict__ thr, float4* __restrict__ rad,
-                                               uint2* __restrict__ ids) {
-    const uint32_t surv = ctl_in[CRT_CTL_SURV], nfree = ctl_in[CRT_CTL_NFREE], g0 = ctl_in[CRT_CTL_GEN];
-    const uint32_t n_new = min(nfree, total - g0);
+                                               uint32_t nslots, uint32_t* __restrict__ ctl_in,
+                                               uint32_t* __restrict__ ctl_out, crt_paths P) {
+    const uint32_t surv = ctl_in[CRT_CTL_SURV], g0 = ctl_in[CRT_CTL_GEN];
+    const uint32_t n_new = min(nslots - surv, total - g0);
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         ctl_in[CRT_CTL_LIVE] = surv + n_new;
         ctl_in[CRT_CTL_WORK] = 0u;
         ctl_out[CRT_CTL_GEN] = g0 + n_new;
         ctl_out[CRT_CTL_SURV] = 0u;
-        ctl_out[CRT_CTL_NFREE] = 0u;
     }
     for (uint32_t i = tid; i < n_new; i += gridDim.x * blockDim.x) {
-        const uint32_t slot = freelist[i], pid = g0 + i;
+        const uint32_t slot = surv + i, pid = g0 + i;
         int r = find_request(reqs, nreq, pid);
         const crt_dev_request& R = reqs[r];
         uint32_t local = pid - R.path_base;
@@ -568,12 +562,11 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
         float tm;
         camera_ray<MODE>(R, pix, sample, g, o, dir, tm);
         uint32_t state = (R.max_depth & 0xffu) | ST_CAMERA;
-        ray_o[slot] = make_float4(o.x, o.y, o.z, tm);
-        ray_d[slot] = make_float4(dir.x, dir.y, dir.z, bitsf(state));
-        thr[slot] = make_float4(1.0f, 1.0f, 1.0f, bitsf(g.ms));
-        rad[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ids[slot] = make_uint2(pix, ((uint32_t)r << 16) | (sample & 0xffffu));
-        queue[surv + i] = slot;
+        P.ray_o[slot] = make_float4(o.x, o.y, o.z, tm);
+        P.ray_d[slot] = make_float4(dir.x, dir.y, dir.z, bitsf(state));
+        P.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, bitsf(g.ms));
+        P.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.ids[slot] = make_uint2(pix, ((uint32_t)r << 16) | (sample & 0xffffu));
     }
 }
 
@@ -615,7 +608,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
                                                   crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
-    const uint32_t n = queue ? *count_ptr : count_direct;
+    const uint32_t n = count_ptr ? *count_ptr : count_direct;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t* lds_col = s_stack + threadIdx.x;
@@ -672,7 +665,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend_pt(const crt_dev_scene* __rest
                                                      uint32_t refill_min, crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
-    const uint32_t n = queue ? *count_ptr : count_direct;
+    const uint32_t n = count_ptr ? *count_ptr : count_direct;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t* lds_col = s_stack + threadIdx.x;
     uint32_t* gst = gstack + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * GS;
@@ -1026,32 +1019,28 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
 template <int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict__ scenes,
                                                  const crt_dev_request* __restrict__ reqs,
-                                                 const uint32_t* __restrict__ queue_in, const uint32_t* __restrict__ count_in,
-                                                 uint32_t count_direct, uint32_t* __restrict__ queue_out,
-                                                 uint32_t* __restrict__ count_out, float4* __restrict__ ray_o,
-                                                 float4* __restrict__ ray_d, float4* __restrict__ thr,
-                                                 float4* __restrict__ rad, const uint2* __restrict__ ids,
-                                                 const float4* __restrict__ hit, const uint32_t* __restrict__ hit_inst,
-                                                 uint32_t* __restrict__ freelist, uint32_t* __restrict__ nfree_out,
-                                                 float4* __restrict__ out_rad, crt_dev_counters* __restrict__ counters) {
+                                                 const uint32_t* __restrict__ count_in, crt_paths Pin, crt_paths Pout,
+                                                 uint32_t* __restrict__ count_out, const float4* __restrict__ hit,
+                                                 const uint32_t* __restrict__ hit_inst, float4* __restrict__ out_rad,
+                                                 crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_wave[BLOCK / 64];
     __shared__ uint32_t s_base;
-    const uint32_t n = queue_in ? *count_in : count_direct;
+    const uint32_t n = *count_in;
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t light_tests = 0;
     // grid-stride loop with a block-uniform trip count so every thread reaches the block scan
     const uint32_t iters = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < iters; ++it) {
         uint32_t i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
-        bool keep = false, freed = false;
-        uint32_t slot = 0;
+        bool keep = false;
+        float4 no, nd, nt, nr;
+        uint2 idv = make_uint2(0u, 0u);
         if (i < n) {
-            slot = queue_in ? queue_in[i] : i;
-            float4 hv = hit[slot];
-            uint32_t iw = hit_inst[slot];
-            float4 ro = ray_o[slot], rdv = ray_d[slot];
-            float4 tv = thr[slot], rv = rad[slot];
-            uint2 idv = ids[slot];
+            float4 hv = hit[i];
+            uint32_t iw = hit_inst[i];
+            float4 ro = Pin.ray_o[i], rdv = Pin.ray_d[i];
+            float4 tv = Pin.thr[i], rv = Pin.rad[i];
+            idv = Pin.ids[i];
             uint32_t state = fbits(rdv.w);
             uint32_t req = idv.y >> 16;
             const crt_dev_request& R = reqs[req];
@@ -1062,26 +1051,29 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
             uint32_t depth = ST_DEPTH(state), bounce = ST_BOUNCE(state);
             Rng<MODE> g;
             g.ms = fbits(tv.w);
-            bool done = !shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
-            if (!done) {
+            keep = shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
+            if (keep) {
                 uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8);
-                ray_o[slot] = make_float4(o.x, o.y, o.z, ro.w);
-                ray_d[slot] = make_float4(d.x, d.y, d.z, bitsf(ns));
-                thr[slot] = make_float4(att.x, att.y, att.z, bitsf(g.ms));
-                rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-                keep = true;
+                no = make_float4(o.x, o.y, o.z, ro.w);
+                nd = make_float4(d.x, d.y, d.z, bitsf(ns));
+                nt = make_float4(att.x, att.y, att.z, bitsf(g.ms));
+                nr = make_float4(L.x, L.y, L.z, 0.0f);
             } else {
-                // finished: the radiance goes to the path's id, the slot returns to the free list
+                // finished: the radiance goes to the path's id
                 uint32_t pid = R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin);
                 out_rad[pid] = make_float4(L.x, L.y, L.z, 0.0f);
                 if (MODE == CRT_RNG_MINSTD) R.rng_state[pix] = g.ms;
-                freed = true;
             }
         }
+        // survivors go to the front of the other state buffer, in wave/block rank order
         uint32_t pos = block_append<BLOCK>(keep, count_out, s_wave, &s_base);
-        if (keep) queue_out[pos] = slot;
-        uint32_t fpos = block_append<BLOCK>(freed, nfree_out, s_wave, &s_base);
-        if (freed) freelist[fpos] = slot;
+        if (keep) {
+            Pout.ray_o[pos] = no;
+            Pout.ray_d[pos] = nd;
+            Pout.thr[pos] = nt;
+            Pout.rad[pos] = nr;
+            Pout.ids[pos] = idv;
+        }
     }
     wave_add(&counters->light_tests, light_tests);
 }
@@ -1207,21 +1199,19 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 #define STK_LS 16
 #define STK_GS 96
 
-extern "C" int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t* freelist, uint32_t nslots) {
-    uint32_t n = nslots > 2 * CRT_CTL_WORDS ? nslots : 2 * CRT_CTL_WORDS;
-    hipLaunchKernelGGL(k_pass_init, dim3((n + 255) / 256), dim3(256), 0, s, ctl, freelist, nslots);
+extern "C" int crt_k_pass_init(hipStream_t s, uint32_t* ctl) {
+    hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(64), 0, s, ctl);
     return (int)hipGetLastError();
 }
 
 extern "C" int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq,
-                           uint32_t total, uint32_t* ctl_in, uint32_t* ctl_out, const uint32_t* freelist,
-                           uint32_t* queue, float4* ray_o, float4* ray_d, float4* thr, float4* rad, uint2* ids) {
+                           uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out, crt_paths P) {
     if (rng_mode == CRT_RNG_MINSTD)
-        hipLaunchKernelGGL(k_regen<CRT_RNG_MINSTD>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, ctl_in, ctl_out,
-                           freelist, queue, ray_o, ray_d, thr, rad, ids);
+        hipLaunchKernelGGL(k_regen<CRT_RNG_MINSTD>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, nslots, ctl_in,
+                           ctl_out, P);
     else
-        hipLaunchKernelGGL(k_regen<CRT_RNG_PCG>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, ctl_in, ctl_out,
-                           freelist, queue, ray_o, ray_d, thr, rad, ids);
+        hipLaunchKernelGGL(k_regen<CRT_RNG_PCG>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, nslots, ctl_in, ctl_out,
+                           P);
     return (int)hipGetLastError();
 }
 
@@ -1256,18 +1246,15 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
 }
 
 extern "C" int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes,
-                           const crt_dev_request* reqs, const uint32_t* queue_in, const uint32_t* count_in,
-                           uint32_t count_direct, uint32_t* queue_out, uint32_t* count_out, float4* ray_o, float4* ray_d,
-                           float4* thr, float4* rad, const uint2* ids, const float4* hit, const uint32_t* hit_inst,
-                           uint32_t* freelist, uint32_t* nfree_out, float4* out_rad, crt_dev_counters* counters) {
+                           const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout,
+                           uint32_t* count_out, const float4* hit, const uint32_t* hit_inst, float4* out_rad,
+                           crt_dev_counters* counters) {
     if (rng_mode == CRT_RNG_MINSTD)
-        hipLaunchKernelGGL((k_shade<CRT_RNG_MINSTD, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue_in, count_in,
-                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, freelist,
-                           nfree_out, out_rad, counters);
+        hipLaunchKernelGGL((k_shade<CRT_RNG_MINSTD, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout,
+                           count_out, hit, hit_inst, out_rad, counters);
     else
-        hipLaunchKernelGGL((k_shade<CRT_RNG_PCG, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, queue_in, count_in,
-                           count_direct, queue_out, count_out, ray_o, ray_d, thr, rad, ids, hit, hit_inst, freelist,
-                           nfree_out, out_rad, counters);
+        hipLaunchKernelGGL((k_shade<CRT_RNG_PCG, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout,
+                           count_out, hit, hit_inst, out_rad, counters);
     return (int)hipGetLastError();
 }
 
