@@ -189,7 +189,7 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False)
     variant of the kernel (same deterministic workload) over the HIP-event time of the launches."""
     from cortexraytracercuda_amd import Renderer
     rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
-                  refill_min=args.refill_min)
+                  refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu)
     sidc = rc.upload(scene)
     node_b = rc.info(sidc)["node_bytes"]
     cnt = dict.fromkeys(["rays", "node_visits", "leaf_visits", "tri_tests", "sphere_tests", "quad_tests", "inst_visits"], 0)
@@ -217,10 +217,24 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False)
             "prim_tests_per_ray": (cnt["tri_tests"] + cnt["sphere_tests"] + cnt["quad_tests"]) / max(cnt["rays"], 1)}
 
 
+def _megakernel_bench(r, sid, cfg, ranges):
+    """The plain path on the same workload: one megakernel launch per render (per pass when progressive)."""
+    r.render(sid, 64, 64, 1, cfg["max_depth"], megakernel=True)
+    t1 = time.perf_counter()
+    mk_rays = 0
+    for sb, sc in ranges:
+        _, _, stm = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb,
+                             sample_count=sc, want_accum=False, megakernel=True)
+        mk_rays += stm["rays"]
+    mk_wall = time.perf_counter() - t1
+    log("megakernel: %.1f ms, %d rays" % (mk_wall * 1e3, mk_rays))
+    return {"mrays_s": mk_rays / mk_wall / 1e6, "ms_per_step": mk_wall * 1e3}
+
+
 def single_render_bench(scene, cfg, args, workload, progressive=False):
     from cortexraytracercuda_amd import Renderer
     r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
-                 refill_min=args.refill_min)
+                 refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu)
     t0 = time.perf_counter()
     sid = r.upload(scene)
     info = r.info(sid)
@@ -250,17 +264,11 @@ def single_render_bench(scene, cfg, args, workload, progressive=False):
         log("step %d: %.1f ms GPU, %d rays" % (k, gg, rr))
     wall = time.perf_counter() - t0
     value = rays / wall / 1e6
-    # the plain path on the same workload: one megakernel launch per render (per pass when progressive)
-    r.render(sid, 64, 64, 1, cfg["max_depth"], megakernel=True)
-    t1 = time.perf_counter()
-    mk_rays = 0
-    for sb, sc in ranges:
-        _, _, stm = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb,
-                             sample_count=sc, want_accum=False, megakernel=True)
-        mk_rays += stm["rays"]
-    mk_wall = time.perf_counter() - t1
-    log("megakernel: %.1f ms, %d rays" % (mk_wall * 1e3, mk_rays))
+    mk = None
+    if not args.no_megakernel:
+        mk = _megakernel_bench(r, sid, cfg, ranges)
     roof = traversal_roofline(scene, cfg, args, r, sid, progressive)
+
     r.close()
     return {"metric": "Mrays/s on 1 MI355X + p50/p99 request latency, batched vs plain, equal RMSE",
             "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -271,8 +279,7 @@ def single_render_bench(scene, cfg, args, workload, progressive=False):
             "bvh": {"nodes": info["num_nodes"], "width": info["bvh_width"], "depth": info["max_depth"],
                     "build_ms": info["build_ms"], "upload_ms": info["upload_ms"], "device_bytes": info["device_bytes"]},
             "roofline": roof,
-            "megakernel": {"mrays_s": mk_rays / mk_wall / 1e6, "ms_per_step": mk_wall * 1e3,
-                           "wavefront_speedup": value / (mk_rays / mk_wall / 1e6)}}
+            "megakernel": mk and dict(mk, wavefront_speedup=value / mk["mrays_s"])}
 
 
 def batch_workload(seed=4, n=64, tri_range=(1e4, 2e6), res_choices=(256, 384, 512, 768, 1024), spp_range=(1, 64)):
@@ -326,7 +333,22 @@ def main():
     ap.add_argument("--max-paths", type=int, default=0, help="path slots per pass (0: library default)")
     ap.add_argument("--extend-kernel", type=int, default=0, help="1 grid-stride, 2 persistent refill (0: default)")
     ap.add_argument("--refill-min", type=int, default=0, help="persistent stage: idle lanes before a refill")
+    ap.add_argument("--grid-per-cu", type=int, default=0, help="workgroups per CU of the closest-hit launches (0: default)")
+    ap.add_argument("--shade-grid-per-cu", type=int, default=0, help="workgroups per CU of regenerate/shade (0: default)")
+    ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel comparison render")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 1 << 28                                  # 2^28 float32 = 1 GiB
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and this path is single-GPU "
               "(multi-GPU sharding is a design note only). Run with --gpus 1.", file=sys.stderr)

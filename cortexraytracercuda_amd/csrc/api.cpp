@@ -116,15 +116,15 @@ int Context::init(const crt_config* cfg) {
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     budget = config.hbm_budget_bytes ? config.hbm_budget_bytes : (uint64_t)(0.9 * (double)freeb);
-    max_paths = config.max_paths ? config.max_paths : (1u << 22);
+    max_paths = config.max_paths ? config.max_paths : (1u << 24);
     leaf_size = config.leaf_size ? config.leaf_size : 4u;
     bvh_width = config.bvh_width ? config.bvh_width : 4u;
-    if (config.extend_kernel > CRT_EXTEND_PERSISTENT) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown extend_kernel");
-    extend_persistent = config.extend_kernel == CRT_EXTEND_PERSISTENT;
-    refill_min = config.refill_min ? std::min(config.refill_min, 64u) : 16u;
+    if (config.extend_kernel > CRT_EXTEND_WAVE_CHUNK) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown extend_kernel");
+    extend_mode = config.extend_kernel ? (int)config.extend_kernel : CRT_EXTEND_GRID_STRIDE;
+    extend_param = config.refill_min ? std::min(config.refill_min, 64u) : (extend_mode == CRT_EXTEND_PERSISTENT ? 16u : 2u);
     if (bvh_width != 2 && bvh_width != 4) return set_error(CRT_ERR_INVALID_ARGUMENT, "bvh_width must be 2 or 4");
-    extend_grid = (uint32_t)num_cus * 8u;
-    shade_grid = (uint32_t)num_cus * 8u;
+    extend_grid = (uint32_t)num_cus * (config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 32u);
+    shade_grid = (uint32_t)num_cus * (config.shade_grid_per_cu ? std::min(config.shade_grid_per_cu, 256u) : 8u);
     int rc;
     if ((rc = scene_table.alloc(sizeof(crt_dev_scene) * kMaxScenes, this))) return rc;
     if ((rc = counters.alloc(sizeof(crt_dev_counters), this))) return rc;
@@ -144,7 +144,7 @@ int Context::init(const crt_config* cfg) {
     HIP_TRY(hipEventCreate(&ev_ext[1]));
     // path ids per pass: every path of a 1080p 64-spp frame fits one pass by default (2 GiB of
     // per-path radiance); at least one id per slot
-    uint64_t pp = config.pass_paths ? config.pass_paths : std::min<uint64_t>(1ull << 27, 32ull * max_paths);
+    uint64_t pp = config.pass_paths ? config.pass_paths : std::min<uint64_t>(1ull << 27, 16ull * max_paths);
     if ((rc = ensure_paths(max_paths, (uint32_t)std::max<uint64_t>(pp, max_paths)))) return rc;
     reserved = used_bytes();
     return CRT_OK;
@@ -382,7 +382,8 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
     const uint32_t nslots = std::min(path_capacity, npaths);
     uint32_t* cb = (uint32_t*)ctl.p;
     if (crt_k_pass_init(stream, cb)) return set_error(CRT_ERR_HIP, "pass init launch failed");
-    const uint32_t grid = std::min(extend_grid, std::max(1u, (nslots + 255) / 256));
+    const uint32_t egrid = std::min(extend_grid, std::max(1u, (nslots + 255) / 256));
+    const uint32_t sgrid = std::min(shade_grid, std::max(1u, (nslots + 255) / 256));
     // every path finishes within max_depth iterations of entering; a slot hosts at most
     // ceil(npaths / nslots) paths in turn
     const uint64_t max_iter = (uint64_t)max_depth * ((npaths + nslots - 1) / nslots + 1) + 2;
@@ -398,12 +399,12 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
         uint32_t* cin = cb + (k & 1) * CRT_CTL_WORDS;
         uint32_t* cout = cb + ((k + 1) & 1) * CRT_CTL_WORDS;
         const crt_paths A = paths((int)(k & 1)), B = paths((int)((k + 1) & 1));
-        if (crt_k_regen(stream, mode, grid, dreq, nreq, npaths, nslots, cin, cout, A))
+        if (crt_k_regen(stream, mode, sgrid, dreq, nreq, npaths, nslots, cin, cout, A))
             return set_error(CRT_ERR_HIP, "regen launch failed");
         if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, grid, dsc, dreq, uniform_scene, nullptr,
+        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, egrid, dsc, dreq, uniform_scene, nullptr,
                          cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, (float4*)hit.p, (uint32_t*)hit_inst.p,
-                         (uint32_t*)gstack.p, nullptr, extend_persistent ? cin + CRT_CTL_WORK : nullptr, refill_min,
+                         (uint32_t*)gstack.p, nullptr, extend_mode, cin + CRT_CTL_WORK, extend_param,
                          (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "extend launch failed");
         if (time_stages) {
@@ -414,7 +415,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             if (extend_ms) *extend_ms += ms;
             extend_launches++;
         }
-        if (crt_k_shade(stream, mode, grid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, (const float4*)hit.p,
+        if (crt_k_shade(stream, mode, sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, (const float4*)hit.p,
                         (const uint32_t*)hit_inst.p, (float4*)out_rad.p, (crt_dev_counters*)counters.p))
             return set_error(CRT_ERR_HIP, "shade launch failed");
         HIP_TRY(hipMemcpyAsync(ctl_host + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,

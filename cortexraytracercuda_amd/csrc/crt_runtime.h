@@ -96,8 +96,8 @@ struct Context {
     hipStream_t stream = nullptr;
     uint64_t budget = 0, reserved = 0, scene_bytes = 0, request_bytes = 0;
     uint32_t max_paths = 0, path_capacity = 0, leaf_size = 4, bvh_width = 4, extend_grid = 2048, shade_grid = 2048;
-    bool extend_persistent = false;  // closest-hit stage with per-lane refill (else grid-stride)
-    uint32_t refill_min = 16;        // idle lanes before a wave fetches more rays
+    int extend_mode = CRT_EXTEND_GRID_STRIDE;   // closest-hit work distribution (CRT_EXTEND_*)
+    uint32_t extend_param = 2;       // lane-refill threshold or 64-ray groups per wave fetch
     std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
     bool slot_used[kMaxScenes] = {};

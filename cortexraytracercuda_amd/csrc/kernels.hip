@@ -605,6 +605,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
                                                   const float4* __restrict__ ray_d, const uint2* __restrict__ ids,
                                                   float4* __restrict__ hit, uint32_t* __restrict__ hit_inst,
                                                   uint32_t* __restrict__ gstack, uint32_t* __restrict__ zero_counter,
+                                                  uint32_t* __restrict__ work_counter, uint32_t chunk,
                                                   crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
     uint32_t nrays = 0, nretrace = 0, novf = 0;
     SceneView SU;
     if (UNIFORM) SU = scene_view(scenes + uniform_scene);
-    for (uint32_t i = gtid; i < n; i += stride) {
+    auto one_ray = [&](uint32_t i) {
         uint32_t slot = queue ? queue[i] : i;
         float4 ro = ray_o[slot], rdv = ray_d[slot];
         uint32_t state = fbits(rdv.w);
@@ -633,6 +634,22 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
         novf += ovf ? 1u : 0u;
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
+    };
+    if (work_counter) {
+        // persistent waves (grid sized to the resident waves): each wave takes the next chunk x 64
+        // consecutive rays when all its lanes are done, so no wave idles while another still holds
+        // a backlog; the fetch and the exit test are wave-uniform
+        const uint32_t lane = threadIdx.x & 63u;
+        for (;;) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(work_counter, 64u * chunk);
+            base = (uint32_t)__shfl((int)base, 0);
+            if (base >= n) break;
+            const uint32_t wend = min(base + 64u * chunk, n);
+            for (uint32_t i = base + lane; i < wend; i += 64u) one_ray(i);
+        }
+    } else {
+        for (uint32_t i = gtid; i < n; i += stride) one_ray(i);
     }
     wave_add(&counters->rays, nrays);
     wave_add(&counters->retraces, nretrace);
@@ -1215,26 +1232,46 @@ extern "C" int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt
     return (int)hipGetLastError();
 }
 
+// Resident blocks of one kernel on this device (per-CU occupancy x CUs).
+template <typename K> static uint32_t resident_blocks(K kernel, int block) {
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb <= 0) nb = 1;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return (uint32_t)(nb * cus);
+}
+
 extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32_t grid,
                             const crt_dev_scene* scenes, const crt_dev_request* reqs, int32_t uniform_scene,
                             const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                             const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
-                            uint32_t* zero_counter, uint32_t* work_counter, uint32_t refill_min,
+                            uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param,
                             crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
-    // work_counter != null selects the persistent lane-refill kernel, else the grid-stride one
+    if (mode != CRT_EXTEND_PERSISTENT && mode != CRT_EXTEND_WAVE_CHUNK) work_counter = nullptr;
+    if (!work_counter) mode = CRT_EXTEND_GRID_STRIDE;
+    // persistent modes launch at most the resident blocks (they fetch work until the queue drains)
 #define CRT_EXT(C, U, W, T)                                                                                            \
-    if (work_counter)                                                                                                   \
-        hipLaunchKernelGGL((k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>), dim3(grid), dim3(256), 0, s, scenes,  \
-                           reqs, us, queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack,          \
-                           zero_counter, work_counter, refill_min, counters);                                           \
-    else                                                                                                                \
-        hipLaunchKernelGGL((k_extend<256, STK_RS, STK_LS, STK_GS, C, U, W, T>), dim3(grid), dim3(256), 0, s, scenes,     \
-                           reqs, us, queue, count_ptr, count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack,          \
-                           zero_counter, counters)
+    if (mode == CRT_EXTEND_PERSISTENT) {                                                                               \
+        auto kfn = k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                               \
+        static uint32_t rb = 0;                                                                                         \
+        if (!rb) rb = resident_blocks(kfn, 256);                                                                        \
+        hipLaunchKernelGGL(kfn, dim3(grid < rb ? grid : rb), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr,      \
+                           count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param,  \
+                           counters);                                                                                   \
+    } else {                                                                                                            \
+        auto kfn = k_extend<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                                  \
+        static uint32_t rb = 0;                                                                                         \
+        if (!rb) rb = resident_blocks(kfn, 256);                                                                        \
+        uint32_t g = (mode == CRT_EXTEND_WAVE_CHUNK && grid > rb) ? rb : grid;                                          \
+        hipLaunchKernelGGL(kfn, dim3(g), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr, count_direct, ray_o,     \
+                           ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param ? param : 1u,         \
+                           counters);                                                                                   \
+    }
 #define CRT_EXT_WT(C, U)                                                   \
-    if (width == 2) { if (two_level) { CRT_EXT(C, U, 2, true); } else { CRT_EXT(C, U, 2, false); } } \
-    else { if (two_level) { CRT_EXT(C, U, 4, true); } else { CRT_EXT(C, U, 4, false); } }
+    if (width == 2) { if (two_level) { CRT_EXT(C, U, 2, true) } else { CRT_EXT(C, U, 2, false) } } \
+    else { if (two_level) { CRT_EXT(C, U, 4, true) } else { CRT_EXT(C, U, 4, false) } }
     if (count_mode) {
         if (uniform_scene >= 0) { CRT_EXT_WT(true, true) } else { CRT_EXT_WT(true, false) }
     } else {

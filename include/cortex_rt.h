@@ -224,19 +224,24 @@ typedef uint64_t crt_request_id;
 typedef struct crt_config {
     int32_t device;             /* HIP device ordinal */
     uint64_t hbm_budget_bytes;  /* admission budget for scenes + requests; 0 = 90% of free memory */
-    uint32_t max_paths;         /* live path slots of the streaming passes (path state pools); 0 = 4M */
+    uint32_t max_paths;         /* live path slots of the streaming passes (path state pools, 164 B each);
+                                   0 = 16M */
     uint32_t block_size;        /* threads per block of the stage kernels; 0 = default (256) */
     uint32_t count_work;        /* 1: use the counting traversal kernel (fills node_visits/prim_tests) */
     uint32_t leaf_size;         /* BVH max primitives per leaf; 0 = default (4) */
     uint32_t bvh_width;         /* children per BVH node, 2 or 4; 0 = default (4) */
-    uint32_t extend_kernel;     /* closest-hit stage: 0 = default (grid-stride), 1 = grid-stride (each lane
+    uint32_t extend_kernel;     /* closest-hit stage (CRT_EXTEND_*): 0 = default, 1 = grid-stride (each lane
                                    traces its rays to the end, the wave waits for its slowest lane),
-                                   2 = persistent with per-lane refill from a work counter */
-    uint32_t refill_min;        /* persistent stage: idle lanes of a wave before it fetches rays; 0 = default */
-    uint32_t pass_paths;        /* path ids per pass (16 B of radiance each); 0 = min(2^27, 32 * max_paths) */
+                                   2 = persistent with per-lane refill from a work counter,
+                                   3 = persistent waves taking chunks of 64-ray groups from a work counter */
+    uint32_t refill_min;        /* mode 2: idle lanes of a wave before it fetches rays; mode 3: 64-ray groups
+                                   per fetch; 0 = default */
+    uint32_t pass_paths;        /* path ids per pass (16 B of radiance each); 0 = min(2^27, 16 * max_paths) */
+    uint32_t grid_per_cu;       /* workgroups per CU of the closest-hit launches (256 threads each); 0 = 32 */
+    uint32_t shade_grid_per_cu; /* workgroups per CU of the regenerate / shade launches; 0 = 8 */
 } crt_config;
 
-enum { CRT_EXTEND_DEFAULT = 0, CRT_EXTEND_GRID_STRIDE = 1, CRT_EXTEND_PERSISTENT = 2 };
+enum { CRT_EXTEND_DEFAULT = 0, CRT_EXTEND_GRID_STRIDE = 1, CRT_EXTEND_PERSISTENT = 2, CRT_EXTEND_WAVE_CHUNK = 3 };
 
 int crt_create(const crt_config* cfg, crt_context** out);
 int crt_destroy(crt_context* ctx);
