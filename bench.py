@@ -189,7 +189,8 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False)
     variant of the kernel (same deterministic workload) over the HIP-event time of the launches."""
     from cortexraytracercuda_amd import Renderer
     rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
-                  refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu)
+                  refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
+                  sub_passes=args.sub_passes)
     sidc = rc.upload(scene)
     node_b = rc.info(sidc)["node_bytes"]
     cnt = dict.fromkeys(["rays", "node_visits", "leaf_visits", "tri_tests", "sphere_tests", "quad_tests", "inst_visits"], 0)
@@ -234,7 +235,8 @@ def _megakernel_bench(r, sid, cfg, ranges):
 def single_render_bench(scene, cfg, args, workload, progressive=False):
     from cortexraytracercuda_amd import Renderer
     r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
-                 refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu)
+                 refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
+                  sub_passes=args.sub_passes)
     t0 = time.perf_counter()
     sid = r.upload(scene)
     info = r.info(sid)
@@ -335,6 +337,7 @@ def main():
     ap.add_argument("--refill-min", type=int, default=0, help="persistent stage: idle lanes before a refill")
     ap.add_argument("--grid-per-cu", type=int, default=0, help="workgroups per CU of the closest-hit launches (0: default)")
     ap.add_argument("--shade-grid-per-cu", type=int, default=0, help="workgroups per CU of regenerate/shade (0: default)")
+    ap.add_argument("--sub-passes", type=int, default=0, help="concurrent sub-passes per pass (0: default)")
     ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel comparison render")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")

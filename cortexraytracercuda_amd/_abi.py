@@ -121,7 +121,8 @@ class crt_config(C.Structure):
     _fields_ = [("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("max_paths", C.c_uint32),
                 ("block_size", C.c_uint32), ("count_work", C.c_uint32), ("leaf_size", C.c_uint32),
                 ("bvh_width", C.c_uint32), ("extend_kernel", C.c_uint32), ("refill_min", C.c_uint32),
-                ("pass_paths", C.c_uint32), ("grid_per_cu", C.c_uint32), ("shade_grid_per_cu", C.c_uint32)]
+                ("pass_paths", C.c_uint32), ("grid_per_cu", C.c_uint32), ("shade_grid_per_cu", C.c_uint32),
+                ("sub_passes", C.c_uint32)]
 
 
 class crt_scene_info(C.Structure):

@@ -128,9 +128,16 @@ int Context::init(const crt_config* cfg) {
     int rc;
     if ((rc = scene_table.alloc(sizeof(crt_dev_scene) * kMaxScenes, this))) return rc;
     if ((rc = counters.alloc(sizeof(crt_dev_counters), this))) return rc;
-    if ((rc = ctl.alloc(sizeof(uint32_t) * 2 * CRT_CTL_WORDS, this))) return rc;
-    HIP_TRY(hipHostMalloc((void**)&ctl_host, sizeof(uint32_t) * CRT_CTL_WORDS * kCtlRing, hipHostMallocDefault));
-    for (int k = 0; k < kCtlRing; ++k) HIP_TRY(hipEventCreateWithFlags(&ctl_ev[k], hipEventDisableTiming));
+    if ((rc = ctl.alloc(sizeof(uint32_t) * 2 * CRT_CTL_WORDS * kMaxSubPasses, this))) return rc;
+    HIP_TRY(hipHostMalloc((void**)&ctl_host, sizeof(uint32_t) * CRT_CTL_WORDS * kCtlRing * kMaxSubPasses,
+                          hipHostMallocDefault));
+    for (int k = 0; k < kCtlRing * kMaxSubPasses; ++k) HIP_TRY(hipEventCreateWithFlags(&ctl_ev[k], hipEventDisableTiming));
+    sub_passes = config.sub_passes ? std::min<uint32_t>(config.sub_passes, kMaxSubPasses) : 2u;
+    for (int k = 0; k < kMaxSubPasses - 1; ++k) {
+        HIP_TRY(hipStreamCreateWithFlags(&sub_stream[k], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&sub_done[k], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&ev_pass_ready, hipEventDisableTiming));
     if ((rc = gstack.alloc((size_t)extend_grid * 256u * crt_k_gstack_entries() * sizeof(uint32_t), this))) return rc;
     for (int k = 0; k < 2; ++k) {
         if ((rc = req_table[k].alloc(sizeof(crt_dev_request) * kMaxPassRequests, this))) return rc;
@@ -178,10 +185,10 @@ int Context::ensure_paths(uint32_t n, uint32_t pass_ids) {
     return CRT_OK;
 }
 
-crt_paths Context::paths(int k) {
+crt_paths Context::paths(int k, uint32_t slot0) {
     crt_paths p;
-    p.ray_o = (float4*)ray_o[k].p; p.ray_d = (float4*)ray_d[k].p; p.thr = (float4*)thr[k].p;
-    p.rad = (float4*)rad[k].p; p.ids = (uint2*)ids[k].p;
+    p.ray_o = (float4*)ray_o[k].p + slot0; p.ray_d = (float4*)ray_d[k].p + slot0; p.thr = (float4*)thr[k].p + slot0;
+    p.rad = (float4*)rad[k].p + slot0; p.ids = (uint2*)ids[k].p + slot0;
     return p;
 }
 
@@ -194,8 +201,14 @@ Context::~Context() {
         if (req_event[k]) (void)hipEventDestroy(req_event[k]);
     }
     if (counters_host) (void)hipHostFree(counters_host);
+    for (int k = 0; k < kMaxSubPasses - 1; ++k) if (sub_stream[k]) (void)hipStreamSynchronize(sub_stream[k]);
     if (ctl_host) (void)hipHostFree(ctl_host);
-    for (int k = 0; k < kCtlRing; ++k) if (ctl_ev[k]) (void)hipEventDestroy(ctl_ev[k]);
+    for (int k = 0; k < kCtlRing * kMaxSubPasses; ++k) if (ctl_ev[k]) (void)hipEventDestroy(ctl_ev[k]);
+    for (int k = 0; k < kMaxSubPasses - 1; ++k) {
+        if (sub_done[k]) (void)hipEventDestroy(sub_done[k]);
+        if (sub_stream[k]) (void)hipStreamDestroy(sub_stream[k]);
+    }
+    if (ev_pass_ready) (void)hipEventDestroy(ev_pass_ready);
     if (ev_start) (void)hipEventDestroy(ev_start);
     for (int k = 0; k < 2; ++k) if (ev_ext[k]) (void)hipEventDestroy(ev_ext[k]);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
@@ -379,49 +392,89 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
     HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request) * items.size(), hipMemcpyHostToDevice, stream));
     const crt_dev_scene* dsc = (const crt_dev_scene*)scene_table.p;
     int nreq = (int)items.size();
-    const uint32_t nslots = std::min(path_capacity, npaths);
+    // The pass runs as K independent sub-passes on K streams (contiguous path-id ranges, disjoint
+    // slot ranges, own control blocks and overflow stacks), so the tail of one sub-pass's launch
+    // overlaps full launches of the others. Timed-stage runs use one stream.
+    const uint32_t nslots_all = std::min(path_capacity, npaths);
+    uint32_t K = time_stages ? 1u : std::min<uint32_t>(sub_passes, std::max<uint32_t>(1u, nslots_all / 65536u));
+    struct Sub { uint32_t lo, hi, slot0, nslots, egrid, sgrid; hipStream_t s; uint64_t k; bool done; };
+    Sub subs[kMaxSubPasses];
     uint32_t* cb = (uint32_t*)ctl.p;
-    if (crt_k_pass_init(stream, cb)) return set_error(CRT_ERR_HIP, "pass init launch failed");
-    const uint32_t egrid = std::min(extend_grid, std::max(1u, (nslots + 255) / 256));
-    const uint32_t sgrid = std::min(shade_grid, std::max(1u, (nslots + 255) / 256));
-    // every path finishes within max_depth iterations of entering; a slot hosts at most
-    // ceil(npaths / nslots) paths in turn
-    const uint64_t max_iter = (uint64_t)max_depth * ((npaths + nslots - 1) / nslots + 1) + 2;
-    for (uint64_t k = 0;; ++k) {
-        if (k >= (uint64_t)kCtlLag) {
-            // the pass has drained once an iteration ends with every path generated and none alive
-            uint64_t j = k - kCtlLag;
-            HIP_TRY(hipEventSynchronize(ctl_ev[j % kCtlRing]));
-            const uint32_t* h = ctl_host + (j % kCtlRing) * CRT_CTL_WORDS;
-            if (h[CRT_CTL_GEN] >= npaths && h[CRT_CTL_SURV] == 0) break;
+    if (crt_k_pass_init(stream, cb, 2u * CRT_CTL_WORDS * K)) return set_error(CRT_ERR_HIP, "pass init launch failed");
+    HIP_TRY(hipEventRecord(ev_pass_ready, stream));
+    const uint32_t egrid_all = extend_grid / K;
+    for (uint32_t q = 0; q < K; ++q) {
+        Sub& u = subs[q];
+        u.lo = (uint32_t)((uint64_t)npaths * q / K);
+        u.hi = (uint32_t)((uint64_t)npaths * (q + 1) / K);
+        u.slot0 = (uint32_t)((uint64_t)nslots_all * q / K);
+        u.nslots = std::min<uint32_t>((uint32_t)((uint64_t)nslots_all * (q + 1) / K) - u.slot0, u.hi - u.lo);
+        u.egrid = std::min(egrid_all, std::max(1u, (u.nslots + 255) / 256));
+        u.sgrid = std::min(shade_grid, std::max(1u, (u.nslots + 255) / 256));
+        u.s = q == 0 ? stream : sub_stream[q - 1];
+        u.k = 0;
+        u.done = u.lo == u.hi;
+        if (q > 0) HIP_TRY(hipStreamWaitEvent(u.s, ev_pass_ready, 0));
+    }
+    uint32_t remaining = 0;
+    for (uint32_t q = 0; q < K; ++q) remaining += subs[q].done ? 0u : 1u;
+    while (remaining) {
+        for (uint32_t q = 0; q < K; ++q) {
+            Sub& u = subs[q];
+            if (u.done) continue;
+            const uint32_t total = u.hi - u.lo;
+            uint32_t* ring = ctl_host + (size_t)q * kCtlRing * CRT_CTL_WORDS;
+            hipEvent_t* evs = ctl_ev + (size_t)q * kCtlRing;
+            if (u.k >= (uint64_t)kCtlLag) {
+                // a sub-pass has drained once an iteration ends with every path generated and none alive
+                uint64_t j = u.k - kCtlLag;
+                HIP_TRY(hipEventSynchronize(evs[j % kCtlRing]));
+                const uint32_t* h = ring + (j % kCtlRing) * CRT_CTL_WORDS;
+                if (h[CRT_CTL_GEN] >= total && h[CRT_CTL_SURV] == 0) {
+                    u.done = true;
+                    remaining--;
+                    if (q > 0) {
+                        HIP_TRY(hipEventRecord(sub_done[q - 1], u.s));
+                        HIP_TRY(hipStreamWaitEvent(stream, sub_done[q - 1], 0));
+                    }
+                    continue;
+                }
+            }
+            // every path finishes within max_depth iterations of entering; a slot hosts at most
+            // ceil(total / nslots) paths in turn
+            const uint64_t max_iter = (uint64_t)max_depth * ((total + u.nslots - 1) / u.nslots + 1) + 2;
+            if (u.k > max_iter) return set_error(CRT_ERR_INTERNAL, "pass did not drain");
+            const uint64_t k = u.k;
+            uint32_t* cin = cb + (q * 2 + (k & 1)) * CRT_CTL_WORDS;
+            uint32_t* cout = cb + (q * 2 + ((k + 1) & 1)) * CRT_CTL_WORDS;
+            crt_paths A = paths((int)(k & 1), u.slot0), B = paths((int)((k + 1) & 1), u.slot0);
+            float4* hq = (float4*)hit.p + u.slot0;
+            uint32_t* hiq = (uint32_t*)hit_inst.p + u.slot0;
+            uint32_t* gq = (uint32_t*)gstack.p + (size_t)q * egrid_all * 256u * crt_k_gstack_entries();
+            if (crt_k_regen(u.s, mode, u.sgrid, dreq, nreq, u.lo, total, u.nslots, cin, cout, A))
+                return set_error(CRT_ERR_HIP, "regen launch failed");
+            if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], u.s));
+            if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)bvh_width, two_level, u.egrid, dsc, dreq, uniform_scene,
+                             nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, hq, hiq, gq, nullptr, extend_mode,
+                             cin + CRT_CTL_WORK, extend_param, (crt_dev_counters*)counters.p))
+                return set_error(CRT_ERR_HIP, "extend launch failed");
+            if (time_stages) {
+                HIP_TRY(hipEventRecord(ev_ext[1], u.s));
+                HIP_TRY(hipEventSynchronize(ev_ext[1]));
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, ev_ext[0], ev_ext[1]));
+                if (extend_ms) *extend_ms += ms;
+                extend_launches++;
+            }
+            if (crt_k_shade(u.s, mode, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, hq, hiq,
+                            (float4*)out_rad.p, (crt_dev_counters*)counters.p))
+                return set_error(CRT_ERR_HIP, "shade launch failed");
+            HIP_TRY(hipMemcpyAsync(ring + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
+                                   hipMemcpyDeviceToHost, u.s));
+            HIP_TRY(hipEventRecord(evs[k % kCtlRing], u.s));
+            u.k++;
+            iterations++;
         }
-        if (k > max_iter) return set_error(CRT_ERR_INTERNAL, "pass did not drain");
-        uint32_t* cin = cb + (k & 1) * CRT_CTL_WORDS;
-        uint32_t* cout = cb + ((k + 1) & 1) * CRT_CTL_WORDS;
-        const crt_paths A = paths((int)(k & 1)), B = paths((int)((k + 1) & 1));
-        if (crt_k_regen(stream, mode, sgrid, dreq, nreq, npaths, nslots, cin, cout, A))
-            return set_error(CRT_ERR_HIP, "regen launch failed");
-        if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-        if (crt_k_extend(stream, config.count_work ? 1 : 0, (int)bvh_width, two_level, egrid, dsc, dreq, uniform_scene, nullptr,
-                         cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, (float4*)hit.p, (uint32_t*)hit_inst.p,
-                         (uint32_t*)gstack.p, nullptr, extend_mode, cin + CRT_CTL_WORK, extend_param,
-                         (crt_dev_counters*)counters.p))
-            return set_error(CRT_ERR_HIP, "extend launch failed");
-        if (time_stages) {
-            HIP_TRY(hipEventRecord(ev_ext[1], stream));
-            HIP_TRY(hipEventSynchronize(ev_ext[1]));
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, ev_ext[0], ev_ext[1]));
-            if (extend_ms) *extend_ms += ms;
-            extend_launches++;
-        }
-        if (crt_k_shade(stream, mode, sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, (const float4*)hit.p,
-                        (const uint32_t*)hit_inst.p, (float4*)out_rad.p, (crt_dev_counters*)counters.p))
-            return set_error(CRT_ERR_HIP, "shade launch failed");
-        HIP_TRY(hipMemcpyAsync(ctl_host + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
-                               hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipEventRecord(ctl_ev[k % kCtlRing], stream));
-        iterations++;
     }
     if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)out_rad.p)) return set_error(CRT_ERR_HIP, "resolve launch failed");
     HIP_TRY(hipEventRecord(req_event[buf], stream));

@@ -23,9 +23,9 @@ struct crt_paths {
 };
 
 extern "C" {
-int crt_k_pass_init(hipStream_t s, uint32_t* ctl);
-int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq, uint32_t total,
-                uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out, crt_paths P);
+int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t nwords);
+int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq, uint32_t pid0,
+                uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out, crt_paths P);
 int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                  const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,

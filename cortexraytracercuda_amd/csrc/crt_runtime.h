@@ -115,8 +115,13 @@ struct Context {
     DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad;
     uint32_t pass_capacity = 0;      // path ids per pass (out_rad entries)
     static constexpr int kCtlRing = 8, kCtlLag = 4;   // control snapshots in flight / host look-behind
-    uint32_t* ctl_host = nullptr;    // pinned: kCtlRing snapshots of the control block's "out" half
-    hipEvent_t ctl_ev[kCtlRing] = {};
+    static constexpr int kMaxSubPasses = 4;           // concurrent sub-passes (streams) of one pass
+    uint32_t sub_passes = 2;
+    uint32_t* ctl_host = nullptr;    // pinned: kCtlRing snapshots per sub-pass of the control block's "out" half
+    hipEvent_t ctl_ev[kCtlRing * kMaxSubPasses] = {};
+    hipStream_t sub_stream[kMaxSubPasses - 1] = {};   // default priority, like `stream`
+    hipEvent_t sub_done[kMaxSubPasses - 1] = {};
+    hipEvent_t ev_pass_ready = nullptr;
     int fail_alloc_after = -1;       // test hook: fail the n-th next allocation
     crt_work_counters last{};        // counters of the last blocking render
 
@@ -134,7 +139,7 @@ struct Context {
     ~Context();
     uint64_t used_bytes() const;
     int ensure_paths(uint32_t slots, uint32_t pass_ids);
-    crt_paths paths(int k);
+    crt_paths paths(int k, uint32_t slot0);
     int upload_scene(const crt_scene_desc* desc, crt_scene_id* out);
     int release_scene(crt_scene_id id);
     DeviceScene* find_scene(crt_scene_id id);

@@ -527,16 +527,15 @@ __device__ __forceinline__ uint32_t ref_pixel_of(const crt_dev_request& R, uint3
 // path id to generate) and prepares half (k+1)%2 ("out") for its own shade stage. Live paths are
 // kept compact: the shade stage writes survivors to the front of the other state buffer and the
 // next regeneration appends new camera paths behind them, so every state access is coalesced.
-__global__ void k_pass_init(uint32_t* __restrict__ ctl) {
-    uint32_t i = threadIdx.x;
-    if (i < 2 * CRT_CTL_WORDS) ctl[i] = 0u;
+__global__ void k_pass_init(uint32_t* __restrict__ ctl, uint32_t nwords) {
+    for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) ctl[i] = 0u;
 }
 
-// Regeneration: the slots behind the survivors take the next path ids of the pass (camera rays,
-// in path-id order: sample-major within each request).
+// Regeneration: the slots behind the survivors take the next path ids of the (sub-)pass, whose
+// ids are [pid0, pid0 + total) (camera rays, in path-id order: sample-major within each request).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t total,
-                                               uint32_t nslots, uint32_t* __restrict__ ctl_in,
+__global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t pid0,
+                                               uint32_t total, uint32_t nslots, uint32_t* __restrict__ ctl_in,
                                                uint32_t* __restrict__ ctl_out, crt_paths P) {
     const uint32_t surv = ctl_in[CRT_CTL_SURV], g0 = ctl_in[CRT_CTL_GEN];
     const uint32_t n_new = min(nslots - surv, total - g0);
@@ -548,7 +547,7 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
         ctl_out[CRT_CTL_SURV] = 0u;
     }
     for (uint32_t i = tid; i < n_new; i += gridDim.x * blockDim.x) {
-        const uint32_t slot = surv + i, pid = g0 + i;
+        const uint32_t slot = surv + i, pid = pid0 + g0 + i;
         int r = find_request(reqs, nreq, pid);
         const crt_dev_request& R = reqs[r];
         uint32_t local = pid - R.path_base;
@@ -1216,19 +1215,20 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 #define STK_LS 16
 #define STK_GS 96
 
-extern "C" int crt_k_pass_init(hipStream_t s, uint32_t* ctl) {
-    hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(64), 0, s, ctl);
+extern "C" int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t nwords) {
+    hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(64), 0, s, ctl, nwords);
     return (int)hipGetLastError();
 }
 
 extern "C" int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq,
-                           uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out, crt_paths P) {
+                           uint32_t pid0, uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out,
+                           crt_paths P) {
     if (rng_mode == CRT_RNG_MINSTD)
-        hipLaunchKernelGGL(k_regen<CRT_RNG_MINSTD>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, nslots, ctl_in,
+        hipLaunchKernelGGL(k_regen<CRT_RNG_MINSTD>, dim3(grid), dim3(256), 0, s, reqs, nreq, pid0, total, nslots, ctl_in,
                            ctl_out, P);
     else
-        hipLaunchKernelGGL(k_regen<CRT_RNG_PCG>, dim3(grid), dim3(256), 0, s, reqs, nreq, total, nslots, ctl_in, ctl_out,
-                           P);
+        hipLaunchKernelGGL(k_regen<CRT_RNG_PCG>, dim3(grid), dim3(256), 0, s, reqs, nreq, pid0, total, nslots, ctl_in,
+                           ctl_out, P);
     return (int)hipGetLastError();
 }
 
