@@ -35,14 +35,6 @@ int set_error(int code, const std::string& msg) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                                      \
-    do {                                                                                                   \
-        hipError_t e_ = (expr);                                                                            \
-        if (e_ != hipSuccess)                                                                              \
-            return set_error(e_ == hipErrorOutOfMemory ? CRT_ERR_OUT_OF_MEMORY : CRT_ERR_HIP,              \
-                             std::string(#expr) + ": " + hipGetErrorString(e_));                           \
-    } while (0)
-
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -194,6 +186,7 @@ crt_paths Context::paths(int k, uint32_t slot0) {
 
 Context::~Context() {
     stop_server();
+    server_release();
     if (stream) (void)hipStreamSynchronize(stream);
     scenes.clear();
     for (int k = 0; k < 2; ++k) {
@@ -313,7 +306,7 @@ uint64_t Job::bytes_needed(const crt_render_desc& r) {
     return npix * 16 + npix * 3 + (r.rng_mode == CRT_RNG_MINSTD ? npix * 4 : 0);
 }
 
-int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r) {
+int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStream_t st) {
     rd = r;
     scene = sc;
     npix = r.width * r.height;
@@ -333,7 +326,7 @@ int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r) {
     if (r.rng_mode == CRT_RNG_MINSTD && (rc = rng.alloc((size_t)npix * 4, ctx))) return rc;
     req.accum = (float*)accum.p;
     req.rng_state = (uint32_t*)rng.p;
-    if (crt_k_init_request(ctx->stream, (float4*)accum.p, (uint32_t*)rng.p, r.width, r.height, r.seed))
+    if (crt_k_init_request(st ? st : ctx->stream, (float4*)accum.p, (uint32_t*)rng.p, r.width, r.height, r.seed))
         return set_error(CRT_ERR_HIP, "init kernel launch failed");
     return CRT_OK;
 }
@@ -451,7 +444,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             float4* hq = (float4*)hit.p + u.slot0;
             uint32_t* hiq = (uint32_t*)hit_inst.p + u.slot0;
             uint32_t* gq = (uint32_t*)gstack.p + (size_t)q * egrid_all * 256u * crt_k_gstack_entries();
-            if (crt_k_regen(u.s, mode, u.sgrid, dreq, nreq, u.lo, total, u.nslots, cin, cout, A))
+            if (crt_k_regen(u.s, mode, u.sgrid, dreq, nreq, u.lo, total, u.nslots, cin, cout, nullptr, A))
                 return set_error(CRT_ERR_HIP, "regen launch failed");
             if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], u.s));
             if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)bvh_width, two_level, u.egrid, dsc, dreq, uniform_scene,
@@ -467,7 +460,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
                 extend_launches++;
             }
             if (crt_k_shade(u.s, mode, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, hq, hiq,
-                            (float4*)out_rad.p, (crt_dev_counters*)counters.p))
+                            (float4*)out_rad.p, nullptr, (crt_dev_counters*)counters.p))
                 return set_error(CRT_ERR_HIP, "shade launch failed");
             HIP_TRY(hipMemcpyAsync(ring + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
                                    hipMemcpyDeviceToHost, u.s));

@@ -24,15 +24,19 @@ struct crt_paths {
 
 extern "C" {
 int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t nwords);
+// rng_mode CRT_RNG_PCG / CRT_RNG_MINSTD, or CRT_RNG_DYNAMIC (each path's mode read from its
+// request) for launches that mix requests of both kinds
+#define CRT_RNG_DYNAMIC 2
 int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq, uint32_t pid0,
-                uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out, crt_paths P);
+                uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out, const uint32_t* open_pass,
+                crt_paths P);
 int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                  const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                  uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, crt_dev_counters* counters);
 int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes, const crt_dev_request* reqs,
                 const uint32_t* count_in, crt_paths Pin, crt_paths Pout, uint32_t* count_out, const float4* hit,
-                const uint32_t* hit_inst, float4* out_rad, crt_dev_counters* counters);
+                const uint32_t* hit_inst, float4* out_rad, uint32_t* done_count, crt_dev_counters* counters);
 int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* req, uint32_t* gstack, crt_dev_counters* counters);
 int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n, int camera,

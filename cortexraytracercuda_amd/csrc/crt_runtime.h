@@ -21,6 +21,15 @@
 namespace crt {
 
 int set_error(int code, const std::string& msg);
+
+// Returns from the enclosing function with a crt status when a HIP call fails.
+#define HIP_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return set_error(e_ == hipErrorOutOfMemory ? CRT_ERR_OUT_OF_MEMORY : CRT_ERR_HIP,              \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));                           \
+    } while (0)
 double now_s();
 void camera_setup(const crt_camera& c, crt_dev_request& r);
 
@@ -65,7 +74,7 @@ struct Job {
     uint64_t rays_in_flight = 0;
     static int validate(const crt_render_desc& r);
     static uint64_t bytes_needed(const crt_render_desc& r);
-    int setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r);
+    int setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStream_t st = nullptr);
     bool take(uint32_t cap, WorkItem& w);
     bool scheduled_all() const { return s_next >= s_end; }
 };
@@ -82,9 +91,48 @@ struct ServeReq {
     crt_request_timing timing{};
     int status = CRT_ERR_NOT_READY;
     bool cancel = false;
-    uint32_t passes_pending = 0;   // passes submitted to the GPU that include this request
-    uint64_t last_pass = 0;        // sequence number of the last pass that includes it
-    crt_dev_counters counters{};
+    uint32_t segs = 0;             // segments appended to the open pass
+    uint32_t segs_resolved = 0;    // segments whose resolve has been enqueued (in segment order)
+    bool finishing = false;        // tonemap + readback enqueued, waiting for ev_done
+    hipEvent_t ev_done = nullptr;
+};
+
+// One segment (work item) of a request inside the server's open pass: a request-table entry.
+struct SrvEntry {
+    ServeReq* r = nullptr;
+    uint32_t seg = 0, paths = 0, done = 0;
+    uint64_t rays = 0;
+    bool resolved = false;
+};
+
+// The serving loop's device state: its own stream, path pool and overflow stacks, and one open
+// streaming pass that grows while it runs (the host appends request-table entries and path ids
+// between iterations; regeneration reads the totals from device memory).
+struct Server {
+    static constexpr uint32_t kEntries = 4096;   // request-table entries per pass (segments)
+    static constexpr int kRing = 8, kLag = 4;    // snapshots in flight / host look-behind
+    static constexpr int kOpenRing = 64;         // pinned (total, nreq) words for the appends
+    static constexpr int kSnapWords = CRT_CTL_WORDS + 20 + 2 * kEntries;   // ctl half, counters, per-entry stats
+    bool ready = false;
+    hipStream_t s = nullptr, copy_s = nullptr;
+    uint32_t slots = 0, cap = 0, egrid = 0, sgrid = 0;
+    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad, req, ctl, stats, gstack, counters;
+    crt_dev_request* req_host = nullptr;   // pinned mirror of the open pass's request table
+    uint32_t* open_host = nullptr;         // pinned ring of (total, nreq)
+    uint32_t* snap = nullptr;              // pinned ring of iteration snapshots
+    hipEvent_t ev[kRing] = {};
+    // open pass (host view)
+    std::vector<SrvEntry> entries;
+    uint32_t total = 0;                    // path ids appended
+    uint64_t iter = 0, processed = 0;      // iterations enqueued / snapshots read
+    uint64_t done_paths = 0;               // finished paths observed
+    uint32_t gen_seen = 0;                 // generated path ids observed
+    uint32_t appends = 0;
+    bool full = false;                     // no room for another segment: drain, then reset
+    bool two_level = false;                // an instanced scene has paths in the open pass
+    uint32_t rr = 0;                       // round-robin start among active requests
+    uint64_t progress_iter = 0;            // last iteration whose snapshot showed finished paths
+    uint32_t snap_ne[kRing] = {};          // entries covered by each snapshot
 };
 
 struct Context {
@@ -134,6 +182,7 @@ struct Context {
     std::vector<ServeReq*> srv_active;                 // admitted, on the device
     std::map<crt_request_id, std::unique_ptr<ServeReq>> srv_all;
     crt_request_id next_req_id = 1;
+    Server srv;
 
     int init(const crt_config* cfg);
     ~Context();
@@ -153,6 +202,12 @@ struct Context {
     int submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_id* out);
     void server_loop();
     void stop_server();
+    int server_init();
+    void server_release();
+    int server_iteration();
+    int server_append(std::vector<ServeReq*>& act);
+    int server_process_snapshot(std::vector<ServeReq*>& act);
+    void server_reset_pass();
 };
 
 void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms);

@@ -88,16 +88,22 @@ __device__ __forceinline__ Pcg pcg_keyed(uint64_t seed, uint32_t pixel, uint32_t
     return g;
 }
 
-// One draw source; MODE is a template constant so each kernel instance is branch-free.
+// One draw source. MODE is CRT_RNG_PCG or CRT_RNG_MINSTD (a template constant: each kernel
+// instance is branch-free) or CRT_RNG_DYNAMIC (the mode is read per path from its request, for
+// launches that mix requests of both kinds).
 template <int MODE> struct Rng {
     uint32_t ms;
     Pcg pcg;
+    uint32_t dyn;   // CRT_RNG_DYNAMIC: this path's mode
+    __device__ __forceinline__ bool minstd() const {
+        return MODE == CRT_RNG_MINSTD || (MODE == CRT_RNG_DYNAMIC && dyn == CRT_RNG_MINSTD);
+    }
     __device__ __forceinline__ float uniform() {
-        if (MODE == CRT_RNG_MINSTD) return minstd_uniform(ms);
+        if (minstd()) return minstd_uniform(ms);
         return (float)(pcg_next(pcg) >> 8) * (1.0f / 16777216.0f);
     }
     __device__ __forceinline__ void skip() {
-        if (MODE == CRT_RNG_MINSTD) minstd_next(ms); else pcg_next(pcg);
+        if (minstd()) minstd_next(ms); else pcg_next(pcg);
     }
 };
 
@@ -536,7 +542,11 @@ __global__ void k_pass_init(uint32_t* __restrict__ ctl, uint32_t nwords) {
 template <int MODE>
 __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t pid0,
                                                uint32_t total, uint32_t nslots, uint32_t* __restrict__ ctl_in,
-                                               uint32_t* __restrict__ ctl_out, crt_paths P) {
+                                               uint32_t* __restrict__ ctl_out, const uint32_t* __restrict__ open_pass,
+                                               crt_paths P) {
+    // an open pass (serving) grows while it runs: its path count and request count live in device
+    // memory, written by the host between iterations
+    if (open_pass) { total = open_pass[0]; nreq = (int)open_pass[1]; }
     const uint32_t surv = ctl_in[CRT_CTL_SURV], g0 = ctl_in[CRT_CTL_GEN];
     const uint32_t n_new = min(nslots - surv, total - g0);
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -555,7 +565,9 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
         uint32_t pix = R.pix_begin + (local - k * R.pix_count);
         uint32_t sample = R.sample_begin + k;
         Rng<MODE> g;
-        if (MODE == CRT_RNG_MINSTD) g.ms = R.rng_state[pix];
+        g.dyn = R.rng_mode;
+        g.ms = 0u;
+        if (g.minstd()) g.ms = R.rng_state[pix];
         else g.pcg = pcg_keyed(R.seed, ref_pixel_of(R, pix), sample, 0u);
         V3 o, dir;
         float tm;
@@ -837,7 +849,7 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
         if (!passthrough) L = L + att * v3(sc->background[0], sc->background[1], sc->background[2]);
     } else {
         bounce += 1u;
-        if (MODE == CRT_RNG_PCG) g.pcg = pcg_keyed(R.seed, ref_pixel_of(R, pix), sample, bounce);
+        if (!g.minstd()) g.pcg = pcg_keyed(R.seed, ref_pixel_of(R, pix), sample, bounce);
         uint32_t type = CRT_HIT_TYPE(prim), idx = CRT_HIT_INDEX(prim);
         float t = hv.x;
         V3 p = o + t * d;
@@ -1038,9 +1050,16 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                                                  const uint32_t* __restrict__ count_in, crt_paths Pin, crt_paths Pout,
                                                  uint32_t* __restrict__ count_out, const float4* __restrict__ hit,
                                                  const uint32_t* __restrict__ hit_inst, float4* __restrict__ out_rad,
-                                                 crt_dev_counters* __restrict__ counters) {
+                                                 uint32_t* __restrict__ done_count, crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_wave[BLOCK / 64];
     __shared__ uint32_t s_base;
+    // per-block cache of the per-entry (finished, rays) counts, keyed by entry & 7: one global
+    // atomic pair per cached entry and block at the end instead of one per wave and iteration
+    // (same-address atomics serialise across the XCDs)
+    __shared__ uint32_t s_key[8], s_cnt[16];
+    if (threadIdx.x < 8) s_key[threadIdx.x] = 0xffffffffu;
+    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
     const uint32_t n = *count_in;
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t light_tests = 0;
@@ -1048,7 +1067,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
     const uint32_t iters = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < iters; ++it) {
         uint32_t i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
-        bool keep = false;
+        bool keep = false, finished = false;
         float4 no, nd, nt, nr;
         uint2 idv = make_uint2(0u, 0u);
         if (i < n) {
@@ -1067,7 +1086,9 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
             uint32_t depth = ST_DEPTH(state), bounce = ST_BOUNCE(state);
             Rng<MODE> g;
             g.ms = fbits(tv.w);
+            g.dyn = R.rng_mode;
             keep = shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
+            finished = !keep;
             if (keep) {
                 uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8);
                 no = make_float4(o.x, o.y, o.z, ro.w);
@@ -1078,7 +1099,33 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                 // finished: the radiance goes to the path's id
                 uint32_t pid = R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin);
                 out_rad[pid] = make_float4(L.x, L.y, L.z, 0.0f);
-                if (MODE == CRT_RNG_MINSTD) R.rng_state[pix] = g.ms;
+                if (g.minstd()) R.rng_state[pix] = g.ms;
+            }
+        }
+        if (done_count) {
+            // per request-table entry: (finished paths, shaded segments = rays) - the serving loop
+            // resolves an entry once all its paths are done; one atomic pair per distinct entry
+            // among the wave's lanes
+            const bool valid = i < n;
+            const uint32_t key = valid ? (idv.y >> 16) : 0xffffffffu;
+            uint64_t pending = __ballot(valid);
+            while (pending) {
+                const int leader = __ffsll((unsigned long long)pending) - 1;
+                const uint32_t lk = (uint32_t)__shfl((int)key, leader);
+                const uint64_t same = __ballot(key == lk);
+                const uint64_t fin = __ballot(key == lk && finished);
+                if ((int)(threadIdx.x & 63u) == leader) {
+                    const uint32_t c = lk & 7u;
+                    const uint32_t prev = atomicCAS(&s_key[c], 0xffffffffu, lk);
+                    if (prev == 0xffffffffu || prev == lk) {
+                        atomicAdd(&s_cnt[2 * c + 1], (uint32_t)__popcll(same));
+                        if (fin) atomicAdd(&s_cnt[2 * c], (uint32_t)__popcll(fin));
+                    } else {
+                        atomicAdd(&done_count[2 * lk + 1], (uint32_t)__popcll(same));
+                        if (fin) atomicAdd(&done_count[2 * lk], (uint32_t)__popcll(fin));
+                    }
+                }
+                pending &= ~same;
             }
         }
         // survivors go to the front of the other state buffer, in wave/block rank order
@@ -1089,6 +1136,16 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
             Pout.thr[pos] = nt;
             Pout.rad[pos] = nr;
             Pout.ids[pos] = idv;
+        }
+    }
+    if (done_count) {
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const uint32_t k = s_key[threadIdx.x];
+            if (k != 0xffffffffu) {
+                if (s_cnt[2 * threadIdx.x + 1]) atomicAdd(&done_count[2 * k + 1], s_cnt[2 * threadIdx.x + 1]);
+                if (s_cnt[2 * threadIdx.x]) atomicAdd(&done_count[2 * k], s_cnt[2 * threadIdx.x]);
+            }
         }
     }
     wave_add(&counters->light_tests, light_tests);
@@ -1118,11 +1175,13 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
         uint32_t pix = R.pix_begin + q;
         float4 acc = accum[pix];
         Rng<MODE> g;
-        if (MODE == CRT_RNG_MINSTD) g.ms = R.rng_state[pix];
+        g.dyn = R.rng_mode;
+        g.ms = 0u;
+        if (g.minstd()) g.ms = R.rng_state[pix];
         const uint32_t refp = ref_pixel_of(R, pix);
         for (uint32_t k = 0; k < R.sample_count; ++k) {
             uint32_t sample = R.sample_begin + k;
-            if (MODE == CRT_RNG_PCG) g.pcg = pcg_keyed(R.seed, refp, sample, 0u);
+            if (!g.minstd()) g.pcg = pcg_keyed(R.seed, refp, sample, 0u);
             V3 o, d;
             float tm;
             camera_ray<MODE>(R, pix, sample, g, o, d, tm);
@@ -1144,7 +1203,7 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
         }
         acc.w += (float)R.sample_count;
         accum[pix] = acc;
-        if (MODE == CRT_RNG_MINSTD) R.rng_state[pix] = g.ms;
+        if (g.minstd()) R.rng_state[pix] = g.ms;
     }
     wave_add(&counters->rays, nrays);
     wave_add(&counters->retraces, nretrace);
@@ -1222,13 +1281,14 @@ extern "C" int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t nwords) {
 
 extern "C" int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq,
                            uint32_t pid0, uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out,
-                           crt_paths P) {
-    if (rng_mode == CRT_RNG_MINSTD)
-        hipLaunchKernelGGL(k_regen<CRT_RNG_MINSTD>, dim3(grid), dim3(256), 0, s, reqs, nreq, pid0, total, nslots, ctl_in,
-                           ctl_out, P);
-    else
-        hipLaunchKernelGGL(k_regen<CRT_RNG_PCG>, dim3(grid), dim3(256), 0, s, reqs, nreq, pid0, total, nslots, ctl_in,
-                           ctl_out, P);
+                           const uint32_t* open_pass, crt_paths P) {
+#define CRT_RG(M)                                                                                                 \
+    hipLaunchKernelGGL(k_regen<M>, dim3(grid), dim3(256), 0, s, reqs, nreq, pid0, total, nslots, ctl_in, ctl_out, \
+                       open_pass, P)
+    if (rng_mode == CRT_RNG_MINSTD) CRT_RG(CRT_RNG_MINSTD);
+    else if (rng_mode == CRT_RNG_PCG) CRT_RG(CRT_RNG_PCG);
+    else CRT_RG(CRT_RNG_DYNAMIC);
+#undef CRT_RG
     return (int)hipGetLastError();
 }
 
@@ -1285,13 +1345,14 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
 extern "C" int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes,
                            const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout,
                            uint32_t* count_out, const float4* hit, const uint32_t* hit_inst, float4* out_rad,
-                           crt_dev_counters* counters) {
-    if (rng_mode == CRT_RNG_MINSTD)
-        hipLaunchKernelGGL((k_shade<CRT_RNG_MINSTD, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout,
-                           count_out, hit, hit_inst, out_rad, counters);
-    else
-        hipLaunchKernelGGL((k_shade<CRT_RNG_PCG, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout,
-                           count_out, hit, hit_inst, out_rad, counters);
+                           uint32_t* done_count, crt_dev_counters* counters) {
+#define CRT_SH(M)                                                                                                     \
+    hipLaunchKernelGGL((k_shade<M, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout, count_out, \
+                       hit, hit_inst, out_rad, done_count, counters)
+    if (rng_mode == CRT_RNG_MINSTD) CRT_SH(CRT_RNG_MINSTD);
+    else if (rng_mode == CRT_RNG_PCG) CRT_SH(CRT_RNG_PCG);
+    else CRT_SH(CRT_RNG_DYNAMIC);
+#undef CRT_SH
     return (int)hipGetLastError();
 }
 

@@ -1,14 +1,20 @@
 // serve.cpp - continuous-batching request loop (crt_submit / crt_poll / crt_wait / crt_cancel).
 //
 // The reference serves one render per process (UI/Examples/UI/main.cpp:371-395 spawns the exe)
-// with no queue, batching or residency. Here one scheduler thread owns the HIP stream:
+// with no queue, batching or residency. Here one scheduler thread owns a stream and a path pool
+// of its own and keeps ONE streaming pass open:
 //   - requests are admitted first-fit against the HBM budget (accumulation, tonemap and RNG
 //     buffers; scenes are already resident and counted at upload);
-//   - every pass packs path slots from all admitted requests (progressive passes of each request
-//     interleave with the others), so requests arriving while others run share the device;
-//   - a request finishes as soon as its last sample is resolved, independently of the others.
+//   - an admitted request is cut into segments (pixel range x sample range); a segment becomes a
+//     request-table entry of the open pass and its path ids are appended behind the pass's last
+//     id between two iterations, so the next regeneration stage already starts its camera rays
+//     among the paths of every other request in flight;
+//   - the shade stage counts finished paths per entry; once all paths of a segment are done the
+//     host enqueues its resolve (segments of one request in order), and after the last one the
+//     tonemap and the readback: a request completes independently of the others;
+//   - when the pass has no room left (path ids or table entries) it drains and starts over.
 // Results do not depend on the mix: streams are keyed per (seed, pixel, sample, bounce) and each
-// pixel's samples are summed in sample order whatever the pass boundaries are.
+// pixel's samples are summed in sample order whatever the segment boundaries are.
 #include <algorithm>
 #include <cstring>
 
@@ -26,6 +32,7 @@ int Context::submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_i
     DeviceScene* sc = nullptr;
     {
         std::lock_guard<std::mutex> g(gpu_mutex);
+        if ((rc = server_init())) return rc;
         sc = find_scene(scene);
         if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
         if (reserved + scene_bytes + need > budget)
@@ -58,11 +65,83 @@ void Context::stop_server() {
         if (!srv_running) return;
         srv_stop = true;
         for (ServeReq* r : srv_waiting) r->cancel = true;
+        for (ServeReq* r : srv_active) r->cancel = true;
     }
     srv_cv.notify_all();
     if (srv_thread.joinable()) srv_thread.join();
     std::lock_guard<std::mutex> lk(srv_mutex);
     srv_running = false;
+}
+
+// Server buffers, allocated at the first submit (caller holds gpu_mutex). Counted as reserved.
+int Context::server_init() {
+    Server& S = srv;
+    if (S.ready) return CRT_OK;
+    int rc;
+    S.slots = max_paths;
+    S.cap = std::max<uint32_t>(pass_capacity, S.slots);
+    HIP_TRY(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&S.copy_s, hipStreamNonBlocking));
+    const size_t n = S.slots;
+    for (int k = 0; k < 2; ++k) {
+        if ((rc = S.ray_o[k].alloc(n * 16, this))) return rc;
+        if ((rc = S.ray_d[k].alloc(n * 16, this))) return rc;
+        if ((rc = S.thr[k].alloc(n * 16, this))) return rc;
+        if ((rc = S.rad[k].alloc(n * 16, this))) return rc;
+        if ((rc = S.ids[k].alloc(n * 8, this))) return rc;
+    }
+    if ((rc = S.hit.alloc(n * 16, this))) return rc;
+    if ((rc = S.hit_inst.alloc(n * 4, this))) return rc;
+    if ((rc = S.out_rad.alloc((size_t)S.cap * 16, this))) return rc;
+    if ((rc = S.req.alloc(sizeof(crt_dev_request) * Server::kEntries, this))) return rc;
+    if ((rc = S.ctl.alloc(sizeof(uint32_t) * (2 * CRT_CTL_WORDS + 2), this))) return rc;
+    if ((rc = S.stats.alloc(sizeof(uint32_t) * 2 * Server::kEntries, this))) return rc;
+    S.egrid = std::min(extend_grid, std::max(1u, (S.slots + 255) / 256));
+    S.sgrid = std::min(shade_grid, std::max(1u, (S.slots + 255) / 256));
+    if ((rc = S.gstack.alloc((size_t)S.egrid * 256u * crt_k_gstack_entries() * sizeof(uint32_t), this))) return rc;
+    if ((rc = S.counters.alloc(sizeof(crt_dev_counters), this))) return rc;
+    HIP_TRY(hipHostMalloc((void**)&S.req_host, sizeof(crt_dev_request) * Server::kEntries, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&S.open_host, sizeof(uint32_t) * 2 * Server::kOpenRing, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&S.snap, sizeof(uint32_t) * Server::kSnapWords * Server::kRing, hipHostMallocDefault));
+    for (int k = 0; k < Server::kRing; ++k) HIP_TRY(hipEventCreateWithFlags(&S.ev[k], hipEventDisableTiming));
+    if (crt_k_fill_u32(S.s, (uint32_t*)S.counters.p, sizeof(crt_dev_counters) / 4, 0u))
+        return set_error(CRT_ERR_HIP, "fill launch failed");
+    S.ready = true;
+    server_reset_pass();
+    reserved = used_bytes() + S.ray_o[0].bytes * 2 + S.ray_d[0].bytes * 2 + S.thr[0].bytes * 2 + S.rad[0].bytes * 2 +
+               S.ids[0].bytes * 2 + S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes +
+               S.stats.bytes + S.gstack.bytes + S.counters.bytes;
+    return CRT_OK;
+}
+
+void Context::server_release() {
+    Server& S = srv;
+    if (S.s) (void)hipStreamSynchronize(S.s);
+    if (S.copy_s) (void)hipStreamSynchronize(S.copy_s);
+    if (S.req_host) (void)hipHostFree(S.req_host);
+    if (S.open_host) (void)hipHostFree(S.open_host);
+    if (S.snap) (void)hipHostFree(S.snap);
+    for (int k = 0; k < Server::kRing; ++k) if (S.ev[k]) (void)hipEventDestroy(S.ev[k]);
+    if (S.s) (void)hipStreamDestroy(S.s);
+    if (S.copy_s) (void)hipStreamDestroy(S.copy_s);
+    S.req_host = nullptr; S.open_host = nullptr; S.snap = nullptr; S.s = nullptr; S.copy_s = nullptr;
+    S.ready = false;
+}
+
+// A new open pass: no entries, no path ids; control block and per-entry stats zeroed on the stream.
+void Context::server_reset_pass() {
+    Server& S = srv;
+    S.entries.clear();
+    S.total = 0;
+    S.iter = 0;
+    S.processed = 0;
+    S.done_paths = 0;
+    S.gen_seen = 0;
+    S.progress_iter = 0;
+    S.full = false;
+    S.two_level = false;
+    (void)crt_k_pass_init(S.s, (uint32_t*)S.ctl.p, 2 * CRT_CTL_WORDS + 2);
+    (void)crt_k_fill_u32(S.s, (uint32_t*)S.stats.p, 2 * Server::kEntries, 0u);
 }
 
 static void complete(ServeReq* r, int status, std::condition_variable& cv) {
@@ -72,13 +151,170 @@ static void complete(ServeReq* r, int status, std::condition_variable& cv) {
     cv.notify_all();
 }
 
+// Appends segments of active requests to the open pass while the backlog of ungenerated path ids
+// is below two slot pools (a new request starts within a few iterations). Round-robin, one segment
+// per request and call; MINSTD requests keep at most one segment in flight (each pixel's engine
+// advances sample after sample).
+int Context::server_append(std::vector<ServeReq*>& act) {
+    Server& S = srv;
+    if (act.empty() || S.full) return CRT_OK;
+    const uint32_t seg_cap = std::max<uint32_t>(S.slots / 2, 1u);
+    const uint32_t n0 = (uint32_t)S.entries.size();
+    const size_t na = act.size();
+    for (size_t t = 0; t < na; ++t) {
+        ServeReq* r = act[(S.rr + t) % na];
+        if (r->cancel || r->finishing || r->job->scheduled_all()) continue;
+        if (r->rd.rng_mode == CRT_RNG_MINSTD && r->segs_resolved < r->segs) continue;
+        if ((uint64_t)S.total - S.gen_seen >= 2ull * S.slots) break;
+        if (S.entries.size() >= Server::kEntries || S.total >= S.cap) { S.full = true; break; }
+        uint32_t room = S.cap - S.total;
+        WorkItem w;
+        if (!r->job->take(std::min(seg_cap, room), w)) continue;
+        crt_dev_request q = r->job->req;
+        q.path_base = S.total;
+        q.pix_begin = w.pix_begin; q.pix_count = w.pix_count;
+        q.sample_begin = w.sample_begin; q.sample_count = w.sample_count;
+        const uint32_t e = (uint32_t)S.entries.size();
+        S.req_host[e] = q;
+        SrvEntry en;
+        en.r = r;
+        en.seg = r->segs++;
+        en.paths = w.pix_count * w.sample_count;
+        S.entries.push_back(en);
+        S.total += en.paths;
+        if (r->sc->host.num_instances) S.two_level = true;
+        r->timing.passes++;
+        if (r->timing.first_pass_s == 0.0) r->timing.first_pass_s = now_s();
+    }
+    S.rr++;
+    const uint32_t n1 = (uint32_t)S.entries.size();
+    if (n1 == n0) return CRT_OK;
+    HIP_TRY(hipMemcpyAsync((crt_dev_request*)S.req.p + n0, S.req_host + n0, sizeof(crt_dev_request) * (n1 - n0),
+                           hipMemcpyHostToDevice, S.s));
+    uint32_t* w2 = S.open_host + 2 * (S.appends++ % Server::kOpenRing);
+    w2[0] = S.total;
+    w2[1] = n1;
+    HIP_TRY(hipMemcpyAsync((uint32_t*)S.ctl.p + 2 * CRT_CTL_WORDS, w2, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, S.s));
+    return CRT_OK;
+}
+
+// One regenerate / closest-hit / shade iteration of the open pass, then a snapshot of the control
+// block, the work counters and the per-entry (finished, rays) counts into the pinned ring.
+int Context::server_iteration() {
+    Server& S = srv;
+    const uint64_t k = S.iter;
+    uint32_t* cb = (uint32_t*)S.ctl.p;
+    uint32_t* cin = cb + (k & 1) * CRT_CTL_WORDS;
+    uint32_t* cout = cb + ((k + 1) & 1) * CRT_CTL_WORDS;
+    const uint32_t* open = cb + 2 * CRT_CTL_WORDS;
+    crt_paths P[2];
+    for (int b = 0; b < 2; ++b) {
+        P[b].ray_o = (float4*)S.ray_o[b].p; P[b].ray_d = (float4*)S.ray_d[b].p; P[b].thr = (float4*)S.thr[b].p;
+        P[b].rad = (float4*)S.rad[b].p; P[b].ids = (uint2*)S.ids[b].p;
+    }
+    const crt_paths& A = P[k & 1];
+    const crt_paths& B = P[(k + 1) & 1];
+    const crt_dev_request* dreq = (const crt_dev_request*)S.req.p;
+    const crt_dev_scene* dsc = (const crt_dev_scene*)scene_table.p;
+    const int two_level = S.two_level ? 1 : 0;
+    if (crt_k_regen(S.s, CRT_RNG_DYNAMIC, S.sgrid, dreq, 0, 0, 0, S.slots, cin, cout, open, A))
+        return set_error(CRT_ERR_HIP, "regen launch failed");
+    if (crt_k_extend(S.s, 0, (int)bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
+                     A.ray_d, A.ids, (float4*)S.hit.p, (uint32_t*)S.hit_inst.p, (uint32_t*)S.gstack.p, nullptr,
+                     extend_mode, cin + CRT_CTL_WORK, extend_param, (crt_dev_counters*)S.counters.p))
+        return set_error(CRT_ERR_HIP, "extend launch failed");
+    if (crt_k_shade(S.s, CRT_RNG_DYNAMIC, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
+                    (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.stats.p,
+                    (crt_dev_counters*)S.counters.p))
+        return set_error(CRT_ERR_HIP, "shade launch failed");
+    uint32_t* snap = S.snap + (size_t)(k % Server::kRing) * Server::kSnapWords;
+    HIP_TRY(hipMemcpyAsync(snap, cout, sizeof(uint32_t) * CRT_CTL_WORDS, hipMemcpyDeviceToHost, S.s));
+    HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS, S.counters.p, sizeof(crt_dev_counters), hipMemcpyDeviceToHost, S.s));
+    const size_t ne = S.entries.size();
+    if (ne) HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS + 20, S.stats.p, sizeof(uint32_t) * 2 * ne, hipMemcpyDeviceToHost, S.s));
+    S.snap_ne[k % Server::kRing] = (uint32_t)ne;
+    HIP_TRY(hipEventRecord(S.ev[k % Server::kRing], S.s));
+    S.iter++;
+    iterations++;
+    return CRT_OK;
+}
+
+// Reads the oldest outstanding snapshot (waits for its iteration) and enqueues the resolves of
+// segments whose paths have all finished, in segment order per request; a request whose last
+// segment is resolved gets its tonemap and a completion event.
+int Context::server_process_snapshot(std::vector<ServeReq*>& act) {
+    Server& S = srv;
+    const uint64_t j = S.processed;
+    HIP_TRY(hipEventSynchronize(S.ev[j % Server::kRing]));
+    const uint32_t* snap = S.snap + (size_t)(j % Server::kRing) * Server::kSnapWords;
+    S.processed++;
+    S.gen_seen = std::max(S.gen_seen, snap[CRT_CTL_GEN]);
+    crt_dev_counters c;
+    std::memcpy(&c, snap + CRT_CTL_WORDS, sizeof(c));
+    if (c.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
+    const uint32_t* st = snap + CRT_CTL_WORDS + 20;
+    // entries appended after this iteration was enqueued are not in its snapshot
+    const size_t ns = S.snap_ne[j % Server::kRing];
+    const size_t ne = S.entries.size();
+    uint64_t done = 0;
+    for (size_t e = 0; e < ne; ++e) {
+        SrvEntry& en = S.entries[e];
+        if (e < ns) {
+            uint32_t d = std::min(st[2 * e], en.paths), ry = st[2 * e + 1];
+            if (d > en.done) en.done = d;
+            if (ry > en.rays) { en.r->timing.rays += ry - en.rays; en.rays = ry; }
+        }
+        done += en.done;
+    }
+    if (done > S.done_paths) S.progress_iter = S.iter;
+    S.done_paths = done;
+    // every appended path enters within a few iterations (the backlog is bounded) and finishes
+    // within max_depth iterations of entering; a pass that stops finishing paths is an error
+    if ((uint64_t)S.total > S.done_paths && S.iter - S.progress_iter > 4096)
+        return set_error(CRT_ERR_INTERNAL, "serving pass did not drain");
+    // resolve finished segments in order per request
+    bool progress = true;
+    while (progress) {
+        progress = false;
+        for (size_t e = 0; e < ne; ++e) {
+            SrvEntry& en = S.entries[e];
+            if (en.resolved || en.done < en.paths || en.r->segs_resolved != en.seg) continue;
+            ServeReq* r = en.r;
+            if (!r->cancel) {
+                if (crt_k_resolve(S.s, (const crt_dev_request*)S.req.p + e, 1, S.req_host[e].pix_count,
+                                  (const float4*)S.out_rad.p))
+                    return set_error(CRT_ERR_HIP, "resolve launch failed");
+            }
+            en.resolved = true;
+            r->segs_resolved++;
+            progress = true;
+        }
+    }
+    for (ServeReq* r : act) {
+        if (r->finishing || (!r->job->scheduled_all() && !r->cancel)) continue;
+        if (r->segs_resolved != r->segs) continue;
+        // every segment resolved (or cancelled with nothing in flight): finish on the stream
+        if (!r->cancel) {
+            if (crt_k_tonemap(S.s, (const float4*)r->job->accum.p, r->job->npix, r->job->rd.spp, (uint8_t*)r->job->rgb.p))
+                return set_error(CRT_ERR_HIP, "tonemap launch failed");
+        }
+        HIP_TRY(hipEventCreateWithFlags(&r->ev_done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(r->ev_done, S.s));
+        r->finishing = true;
+    }
+    return CRT_OK;
+}
+
 void Context::server_loop() {
-    uint32_t rr = 0;  // round-robin start among active requests
+    Server& S = srv;
+    int err = CRT_OK;
     for (;;) {
-        std::vector<ServeReq*> admitted;
+        std::vector<ServeReq*> admitted, act;
+        const bool inflight = S.processed < S.iter;
         {
             std::unique_lock<std::mutex> lk(srv_mutex);
-            srv_cv.wait(lk, [&]() { return srv_stop || !srv_waiting.empty() || !srv_active.empty(); });
+            const bool work = (uint64_t)S.total > S.done_paths;
+            srv_cv.wait(lk, [&]() { return srv_stop || !srv_waiting.empty() || !srv_active.empty() || inflight || work; });
             // drop cancelled waiting requests
             for (auto it = srv_waiting.begin(); it != srv_waiting.end();) {
                 if ((*it)->cancel) {
@@ -90,7 +326,7 @@ void Context::server_loop() {
                     ++it;
                 }
             }
-            if (srv_stop && srv_active.empty()) break;
+            if (srv_stop && srv_active.empty() && !inflight) break;
             // first-fit admission against the budget
             for (auto it = srv_waiting.begin(); it != srv_waiting.end();) {
                 ServeReq* r = *it;
@@ -104,99 +340,88 @@ void Context::server_loop() {
                 }
             }
         }
-        std::vector<WorkItem> items;
-        std::vector<ServeReq*> in_pass;
-        int err = CRT_OK;
-        {
-            std::lock_guard<std::mutex> g(gpu_mutex);
-            for (ServeReq* r : admitted) {
-                r->job = std::make_unique<Job>();
-                int rc = r->job->setup(this, r->sc, r->rd);
-                r->timing.admit_s = now_s();
-                std::lock_guard<std::mutex> lk(srv_mutex);
-                if (rc) {
-                    request_bytes -= Job::bytes_needed(r->rd);
-                    r->job.reset();
-                    r->sc->refs--;
-                    complete(r, rc, srv_done_cv);
-                } else {
-                    srv_active.push_back(r);
-                }
-            }
-            std::vector<ServeReq*> act;
+        for (ServeReq* r : admitted) {
+            int rc;
             {
-                std::lock_guard<std::mutex> lk(srv_mutex);
-                act = srv_active;
+                std::lock_guard<std::mutex> g(gpu_mutex);
+                r->job = std::make_unique<Job>();
+                rc = r->job->setup(this, r->sc, r->rd, S.s);
             }
-            // pass packing: requests of the oldest pending rng mode, fair share first, then leftovers
-            int mode = -1;
-            for (ServeReq* r : act) if (!r->cancel && !r->job->scheduled_all()) { mode = (int)r->rd.rng_mode; break; }
-            if (mode >= 0) {
-                uint32_t cap = max_paths;
-                size_t n = act.size();
-                std::vector<ServeReq*> order;
-                for (size_t k = 0; k < n; ++k) {
-                    ServeReq* r = act[(rr + k) % n];
-                    if (!r->cancel && (int)r->rd.rng_mode == mode && !r->job->scheduled_all()) order.push_back(r);
-                }
-                rr++;
-                // one work item per request and pass (a pixel's samples resolve in order in one
-                // thread, so two items of one request would race); each request gets a fair share
-                uint32_t share = order.empty() ? cap : std::max<uint32_t>(cap / (uint32_t)order.size(), 1u);
-                for (ServeReq* r : order) {
-                    if (cap == 0 || items.size() >= kMaxPassRequests) break;
-                    uint32_t lim = std::min(cap, std::max(share, r->job->npix));
-                    WorkItem w;
-                    if (!r->job->take(lim, w)) continue;
-                    cap -= w.pix_count * w.sample_count;
-                    items.push_back(w);
-                    in_pass.push_back(r);
-                }
-                if (!items.empty()) {
-                    double t = now_s();
-                    for (ServeReq* r : in_pass) if (r->timing.first_pass_s == 0.0) r->timing.first_pass_s = t;
-                    err = reset_counters();
-                    if (!err) err = run_pass(items, false, nullptr);
-                    if (!err && hipStreamSynchronize(stream) != hipSuccess) err = set_error(CRT_ERR_HIP, "pass failed");
-                    crt_dev_counters c{};
-                    if (!err) err = read_counters(c);
-                    if (!err && c.stack_overflow) err = set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
-                    // attribute rays to requests in proportion to their paths in this pass
-                    uint64_t total = 0;
-                    for (auto& w : items) total += (uint64_t)w.pix_count * w.sample_count;
-                    for (auto& w : items) {
-                        for (ServeReq* r : in_pass) {
-                            if (r->job.get() != w.job) continue;
-                            double f = total ? (double)((uint64_t)w.pix_count * w.sample_count) / (double)total : 0.0;
-                            r->timing.rays += (uint64_t)(f * (double)c.rays + 0.5);
-                        }
-                    }
-                    for (ServeReq* r : in_pass) r->timing.passes++;
-                }
-            }
-            // completion: everything scheduled and resolved (the pass above was synchronised)
-            std::vector<ServeReq*> finished;
-            for (ServeReq* r : act) {
-                if (err || r->cancel || r->job->scheduled_all()) finished.push_back(r);
-            }
-            for (ServeReq* r : finished) {
-                int st = err ? err : (r->cancel ? CRT_ERR_CANCELLED : CRT_OK);
-                if (st == CRT_OK) {
-                    r->rgb.resize((size_t)r->job->npix * 3);
-                    r->accum.resize((size_t)r->job->npix * 4);
-                    crt_image im{r->rgb.data(), r->accum.data()};
-                    st = finish_job(*r->job, &im);
-                    if (!st && hipStreamSynchronize(stream) != hipSuccess) st = set_error(CRT_ERR_HIP, "readback failed");
-                }
-                std::lock_guard<std::mutex> lk(srv_mutex);
+            r->timing.admit_s = now_s();
+            std::lock_guard<std::mutex> lk(srv_mutex);
+            if (rc) {
                 request_bytes -= Job::bytes_needed(r->rd);
                 r->job.reset();
-                srv_active.erase(std::find(srv_active.begin(), srv_active.end(), r));
                 r->sc->refs--;
-                complete(r, st, srv_done_cv);
+                complete(r, rc, srv_done_cv);
+            } else {
+                srv_active.push_back(r);
             }
         }
+        {
+            std::lock_guard<std::mutex> lk(srv_mutex);
+            act = srv_active;
+        }
+        if (!err) err = server_append(act);
+        // keep up to kLag iterations ahead while paths are outstanding
+        const bool outstanding = (uint64_t)S.total > S.done_paths;
+        bool enqueued = false;
+        if (!err && outstanding && S.iter - S.processed < (uint64_t)Server::kLag) {
+            err = server_iteration();
+            enqueued = true;
+        }
+        if (!err && S.processed < S.iter && (!enqueued || S.iter - S.processed >= (uint64_t)Server::kLag))
+            err = server_process_snapshot(act);
+        // a full pass that has drained and resolved everything starts over
+        if (!err && S.processed == S.iter && (uint64_t)S.total == S.done_paths) {
+            bool all_resolved = true;
+            for (auto& en : S.entries) all_resolved = all_resolved && en.resolved;
+            if (all_resolved && (S.full || S.total > S.cap / 2 || S.entries.size() > Server::kEntries / 2)) server_reset_pass();
+        }
+        // completions: readback of finished requests whose stream work is done; with nothing else
+        // to do this turn, wait for the oldest finishing request instead of polling
+        const bool busy = enqueued || S.processed < S.iter || !admitted.empty();
+        std::vector<ServeReq*> finished;
+        for (ServeReq* r : act) {
+            if (err) { finished.push_back(r); continue; }
+            if (!r->finishing) continue;
+            hipError_t q = busy || !finished.empty() ? hipEventQuery(r->ev_done) : hipEventSynchronize(r->ev_done);
+            if (q == hipErrorNotReady) continue;
+            finished.push_back(r);
+        }
+        for (ServeReq* r : finished) {
+            int st = err ? err : (r->cancel ? CRT_ERR_CANCELLED : CRT_OK);
+            if (st == CRT_OK) {
+                r->rgb.resize((size_t)r->job->npix * 3);
+                r->accum.resize((size_t)r->job->npix * 4);
+                if (hipMemcpyAsync(r->rgb.data(), r->job->rgb.p, r->rgb.size(), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
+                    hipMemcpyAsync(r->accum.data(), r->job->accum.p, r->accum.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                   S.copy_s) != hipSuccess ||
+                    hipStreamSynchronize(S.copy_s) != hipSuccess)
+                    st = set_error(CRT_ERR_HIP, "readback failed");
+            }
+            if (r->ev_done) { (void)hipEventDestroy(r->ev_done); r->ev_done = nullptr; }
+            if (err) (void)hipStreamSynchronize(S.s);
+            {
+                // lock order: gpu_mutex is never taken while srv_mutex is held
+                std::lock_guard<std::mutex> g(gpu_mutex);
+                r->job.reset();
+            }
+            std::lock_guard<std::mutex> lk(srv_mutex);
+            request_bytes -= Job::bytes_needed(r->rd);
+            srv_active.erase(std::find(srv_active.begin(), srv_active.end(), r));
+            r->sc->refs--;
+            complete(r, st, srv_done_cv);
+        }
+        if (err) {
+            // a failed iteration leaves the open pass unusable: wait for the stream, start over
+            (void)hipStreamSynchronize(S.s);
+            S.iter = S.processed = 0;
+            server_reset_pass();
+            err = CRT_OK;
+        }
     }
+    (void)hipStreamSynchronize(S.s);
 }
 
 }  // namespace crt

@@ -153,12 +153,16 @@ def test_serving_admission_waits_for_budget():
     W = H = 256
     need = W * H * (16 + 3)
     with Renderer(max_paths=100000) as r0:
-        reserved = r0.memory()["reserved"]
+        reserved0 = r0.memory()["reserved"]
         sid0 = r0.upload(s)
+        r0.wait(r0.submit(sid0, 8, 8, 1, 1), 8, 8)   # the serving loop's own path pool is reserved at first use
+        reserved = r0.memory()["reserved"]
+        assert reserved > reserved0
         scene_bytes = r0.memory()["scenes"]
     budget = reserved + scene_bytes + int(1.5 * need)
     with Renderer(max_paths=100000, hbm_budget=budget) as r:
         sid = r.upload(s)
+        r.wait(r.submit(sid, 8, 8, 1, 1), 8, 8)
         ids = [r.submit(sid, W, H, 4, 4, seed=k) for k in range(3)]
         for rid in ids:
             r.wait(rid, W, H)
@@ -168,7 +172,7 @@ def test_serving_admission_waits_for_budget():
         with pytest.raises(A.CrtError) as e:
             r.submit(sid, 4096, 4096, 1, 1)
         assert e.value.code == A.CRT_ERR_OVER_BUDGET
-    with Renderer(max_paths=100000, hbm_budget=reserved + 16) as r:
+    with Renderer(max_paths=100000, hbm_budget=reserved0 + 16) as r:
         with pytest.raises(A.CrtError) as e:
             r.upload(s)
         assert e.value.code == A.CRT_ERR_OVER_BUDGET
