@@ -193,13 +193,13 @@ def serving(seed=11, n=48, rate_hz=60.0):
     return out
 
 
-def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False):
+def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False, workload_key=""):
     """k_extend roofline for one render of the workload: algorithmic bytes from the counting
     variant of the kernel (same deterministic workload) over the HIP-event time of the launches."""
     from cortexraytracercuda_amd import Renderer
     rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
                   refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
-                  sub_passes=args.sub_passes)
+                  sub_passes=args.sub_passes, leaf_size=args.leaf_size)
     sidc = rc.upload(scene)
     node_b = rc.info(sidc)["node_bytes"]
     cnt = dict.fromkeys(["rays", "node_visits", "leaf_visits", "tri_tests", "sphere_tests", "quad_tests", "inst_visits"], 0)
@@ -220,8 +220,17 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False)
     alg = (B_STATE * cnt["rays"] + node_b * cnt["node_visits"] + B_LEAF * cnt["leaf_visits"] + B_TRI * cnt["tri_tests"] +
            B_SPHERE * cnt["sphere_tests"] + B_QUAD * cnt["quad_tests"] + B_INST * cnt["inst_visits"])
     achieved = alg / (ext_ms * 1e-3) / 1e9
+    # measured DRAM bytes (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy) per ray of this
+    # workload's closest-hit kernel, times the rays of one launch; null for workloads not profiled
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_extend_traffic_%s.json" % workload_key)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        traffic = t["hbm_bytes_per_ray"] * cnt["rays"] / max(launches, 1)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "k_extend", "launches": launches, "avg_launch_ms": ext_ms / max(launches, 1),
+            "traffic": traffic, "traffic_unit": "bytes per launch", "kernel": "k_extend", "launches": launches,
+            "avg_launch_ms": ext_ms / max(launches, 1),
             "bytes_per_launch": alg / max(launches, 1), "bytes_per_ray": alg / max(cnt["rays"], 1),
             "node_bytes": node_b, "node_visits_per_ray": cnt["node_visits"] / max(cnt["rays"], 1),
             "prim_tests_per_ray": (cnt["tri_tests"] + cnt["sphere_tests"] + cnt["quad_tests"]) / max(cnt["rays"], 1)}
@@ -241,11 +250,11 @@ def _megakernel_bench(r, sid, cfg, ranges):
     return {"mrays_s": mk_rays / mk_wall / 1e6, "ms_per_step": mk_wall * 1e3}
 
 
-def single_render_bench(scene, cfg, args, workload, progressive=False):
+def single_render_bench(scene, cfg, args, workload, progressive=False, workload_key=""):
     from cortexraytracercuda_amd import Renderer
     r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
                  refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
-                  sub_passes=args.sub_passes)
+                  sub_passes=args.sub_passes, leaf_size=args.leaf_size)
     t0 = time.perf_counter()
     sid = r.upload(scene)
     info = r.info(sid)
@@ -278,7 +287,7 @@ def single_render_bench(scene, cfg, args, workload, progressive=False):
     mk = None
     if not args.no_megakernel:
         mk = _megakernel_bench(r, sid, cfg, ranges)
-    roof = traversal_roofline(scene, cfg, args, r, sid, progressive)
+    roof = traversal_roofline(scene, cfg, args, r, sid, progressive, workload_key)
 
     r.close()
     return {"metric": "Mrays/s on 1 MI355X + p50/p99 request latency, batched vs plain, equal RMSE",
@@ -347,6 +356,7 @@ def main():
     ap.add_argument("--grid-per-cu", type=int, default=0, help="workgroups per CU of the closest-hit launches (0: default)")
     ap.add_argument("--shade-grid-per-cu", type=int, default=0, help="workgroups per CU of regenerate/shade (0: default)")
     ap.add_argument("--sub-passes", type=int, default=0, help="concurrent sub-passes per pass (0: default)")
+    ap.add_argument("--leaf-size", type=int, default=0, help="BVH max primitives per leaf (0: default)")
     ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel comparison render")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
@@ -391,7 +401,9 @@ def main():
     if args.quick:
         cfg = dict(cfg, width=cfg["width"] // 4, height=cfg["height"] // 4, spp=min(cfg["spp"], 16))
     log("scene generated in %.1f s" % (time.perf_counter() - t_gen))
-    out = single_render_bench(scene, cfg, args, workload, progressive)
+    key = "" if (args.quick or args.bvh_width or args.leaf_size) else {
+        "large": "large_1080p_64spp", "box": "box_256_16spp", "instanced": "instanced_4k_16spp"}[args.config]
+    out = single_render_bench(scene, cfg, args, workload, progressive, key)
     if args.config == "large" and not args.no_extras:
         try:
             out["batched"] = batched_vs_plain()

@@ -109,7 +109,7 @@ int Context::init(const crt_config* cfg) {
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     budget = config.hbm_budget_bytes ? config.hbm_budget_bytes : (uint64_t)(0.9 * (double)freeb);
     max_paths = config.max_paths ? config.max_paths : (1u << 24);
-    leaf_size = config.leaf_size ? config.leaf_size : 4u;
+    leaf_size = config.leaf_size ? config.leaf_size : 2u;
     bvh_width = config.bvh_width ? config.bvh_width : 4u;
     if (config.extend_kernel > CRT_EXTEND_WAVE_CHUNK) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown extend_kernel");
     extend_mode = config.extend_kernel ? (int)config.extend_kernel : CRT_EXTEND_GRID_STRIDE;

@@ -228,7 +228,7 @@ typedef struct crt_config {
                                    0 = 16M */
     uint32_t block_size;        /* threads per block of the stage kernels; 0 = default (256) */
     uint32_t count_work;        /* 1: use the counting traversal kernel (fills node_visits/prim_tests) */
-    uint32_t leaf_size;         /* BVH max primitives per leaf; 0 = default (4) */
+    uint32_t leaf_size;         /* BVH max primitives per leaf; 0 = default (2) */
     uint32_t bvh_width;         /* children per BVH node, 2 or 4; 0 = default (4) */
     uint32_t extend_kernel;     /* closest-hit stage (CRT_EXTEND_*): 0 = default, 1 = grid-stride (each lane
                                    traces its rays to the end, the wave waits for its slowest lane),
