@@ -12,6 +12,10 @@ CLI = os.path.join(HERE, "crt_render")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 (v_pk_*_f32) is disabled for every device compile
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# no FMA contraction anywhere: every kernel variant (uniform / per-lane scene, one / two levels,
+# wavefront / megakernel) then rounds the same expressions the same way, so a request gives the same
+# bits whichever variant traced it, and the arithmetic matches the oracle (g++ -ffp-contract=off)
+FP_FLAGS = ["-ffp-contract=off"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
@@ -34,7 +38,7 @@ def build(verbose=False, jobs=4):
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmd = [HIPCC] + DEVICE_FLAGS + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + DEVICE_FLAGS + FP_FLAGS + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

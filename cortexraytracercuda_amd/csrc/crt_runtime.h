@@ -133,6 +133,9 @@ struct Server {
     uint32_t rr = 0;                       // round-robin start among active requests
     uint64_t progress_iter = 0;            // last iteration whose snapshot showed finished paths
     uint32_t snap_ne[kRing] = {};          // entries covered by each snapshot
+    bool use_graphs = true;                // replay captured iterations
+    uint32_t warm[2] = {};                 // eager iterations per variant (graphs are captured after kRing)
+    hipGraphExec_t graph[2][kRing] = {};   // [two-level][snapshot slot]
 };
 
 struct Context {

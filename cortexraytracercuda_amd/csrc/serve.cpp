@@ -96,6 +96,7 @@ int Context::server_init() {
     if ((rc = S.req.alloc(sizeof(crt_dev_request) * Server::kEntries, this))) return rc;
     if ((rc = S.ctl.alloc(sizeof(uint32_t) * (2 * CRT_CTL_WORDS + 2), this))) return rc;
     if ((rc = S.stats.alloc(sizeof(uint32_t) * 2 * Server::kEntries, this))) return rc;
+    S.use_graphs = config.graphs != 2u;
     S.egrid = std::min(extend_grid, std::max(1u, (S.slots + 255) / 256));
     S.sgrid = std::min(shade_grid, std::max(1u, (S.slots + 255) / 256));
     if ((rc = S.gstack.alloc((size_t)S.egrid * 256u * crt_k_gstack_entries() * sizeof(uint32_t), this))) return rc;
@@ -122,6 +123,9 @@ void Context::server_release() {
     if (S.open_host) (void)hipHostFree(S.open_host);
     if (S.snap) (void)hipHostFree(S.snap);
     for (int k = 0; k < Server::kRing; ++k) if (S.ev[k]) (void)hipEventDestroy(S.ev[k]);
+    for (int t = 0; t < 2; ++t)
+        for (int k = 0; k < Server::kRing; ++k)
+            if (S.graph[t][k]) { (void)hipGraphExecDestroy(S.graph[t][k]); S.graph[t][k] = nullptr; }
     if (S.s) (void)hipStreamDestroy(S.s);
     if (S.copy_s) (void)hipStreamDestroy(S.copy_s);
     S.req_host = nullptr; S.open_host = nullptr; S.snap = nullptr; S.s = nullptr; S.copy_s = nullptr;
@@ -200,9 +204,12 @@ int Context::server_append(std::vector<ServeReq*>& act) {
 
 // One regenerate / closest-hit / shade iteration of the open pass, then a snapshot of the control
 // block, the work counters and the per-entry (finished, rays) counts into the pinned ring.
-int Context::server_iteration() {
-    Server& S = srv;
-    const uint64_t k = S.iter;
+// One regenerate / closest-hit / shade iteration of the open pass, then a snapshot of the control
+// block, the work counters and the per-entry (finished, rays) counts into the pinned ring. The
+// arguments depend only on the snapshot slot (which fixes the buffer parity) and on the traversal
+// variant, so after a few eager iterations each (slot, variant) is captured once into a hipGraph and
+// replayed: one graph launch per iteration instead of three kernel launches and three copies.
+static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
     uint32_t* cb = (uint32_t*)S.ctl.p;
     uint32_t* cin = cb + (k & 1) * CRT_CTL_WORDS;
     uint32_t* cout = cb + ((k + 1) & 1) * CRT_CTL_WORDS;
@@ -215,13 +222,13 @@ int Context::server_iteration() {
     const crt_paths& A = P[k & 1];
     const crt_paths& B = P[(k + 1) & 1];
     const crt_dev_request* dreq = (const crt_dev_request*)S.req.p;
-    const crt_dev_scene* dsc = (const crt_dev_scene*)scene_table.p;
+    const crt_dev_scene* dsc = (const crt_dev_scene*)C.scene_table.p;
     const int two_level = S.two_level ? 1 : 0;
     if (crt_k_regen(S.s, CRT_RNG_DYNAMIC, S.sgrid, dreq, 0, 0, 0, S.slots, cin, cout, open, A))
         return set_error(CRT_ERR_HIP, "regen launch failed");
-    if (crt_k_extend(S.s, 0, (int)bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
+    if (crt_k_extend(S.s, 0, (int)C.bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
                      A.ray_d, A.ids, (float4*)S.hit.p, (uint32_t*)S.hit_inst.p, (uint32_t*)S.gstack.p, nullptr,
-                     extend_mode, cin + CRT_CTL_WORK, extend_param, (crt_dev_counters*)S.counters.p))
+                     C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
     if (crt_k_shade(S.s, CRT_RNG_DYNAMIC, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.stats.p,
@@ -230,10 +237,37 @@ int Context::server_iteration() {
     uint32_t* snap = S.snap + (size_t)(k % Server::kRing) * Server::kSnapWords;
     HIP_TRY(hipMemcpyAsync(snap, cout, sizeof(uint32_t) * CRT_CTL_WORDS, hipMemcpyDeviceToHost, S.s));
     HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS, S.counters.p, sizeof(crt_dev_counters), hipMemcpyDeviceToHost, S.s));
-    const size_t ne = S.entries.size();
-    if (ne) HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS + 20, S.stats.p, sizeof(uint32_t) * 2 * ne, hipMemcpyDeviceToHost, S.s));
-    S.snap_ne[k % Server::kRing] = (uint32_t)ne;
-    HIP_TRY(hipEventRecord(S.ev[k % Server::kRing], S.s));
+    HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS + 20, S.stats.p, sizeof(uint32_t) * 2 * Server::kEntries,
+                           hipMemcpyDeviceToHost, S.s));
+    return CRT_OK;
+}
+
+int Context::server_iteration() {
+    Server& S = srv;
+    const uint64_t k = S.iter;
+    const int slot = (int)(k % Server::kRing);
+    const int tl = S.two_level ? 1 : 0;
+    int rc;
+    if (S.use_graphs && S.warm[tl] >= (uint32_t)Server::kRing) {
+        hipGraphExec_t& g = S.graph[tl][slot];
+        if (!g) {
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(S.s, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_iteration(*this, S, k);
+            hipError_t e = hipStreamEndCapture(S.s, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            HIP_TRY(e);
+            e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) { g = nullptr; HIP_TRY(e); }
+        }
+        HIP_TRY(hipGraphLaunch(g, S.s));
+    } else {
+        if ((rc = enqueue_iteration(*this, S, k))) return rc;
+        S.warm[tl]++;
+    }
+    S.snap_ne[slot] = (uint32_t)S.entries.size();
+    HIP_TRY(hipEventRecord(S.ev[slot], S.s));
     S.iter++;
     iterations++;
     return CRT_OK;

@@ -40,7 +40,8 @@ class Renderer:
     """One device context: resident scenes, path-state buffers and the serving loop."""
 
     def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0,
-                 extend_kernel=0, refill_min=0, pass_paths=0, grid_per_cu=0, shade_grid_per_cu=0, sub_passes=0):
+                 extend_kernel=0, refill_min=0, pass_paths=0, grid_per_cu=0, shade_grid_per_cu=0, sub_passes=0,
+                 graphs=0):
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -55,6 +56,7 @@ class Renderer:
         cfg.grid_per_cu = int(grid_per_cu)
         cfg.shade_grid_per_cu = int(shade_grid_per_cu)
         cfg.sub_passes = int(sub_passes)
+        cfg.graphs = int(graphs)
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h

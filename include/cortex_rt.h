@@ -240,6 +240,8 @@ typedef struct crt_config {
     uint32_t grid_per_cu;       /* workgroups per CU of the closest-hit launches (256 threads each); 0 = 32 */
     uint32_t shade_grid_per_cu; /* workgroups per CU of the regenerate / shade launches; 0 = 8 */
     uint32_t sub_passes;        /* concurrent sub-passes (one stream each) of a pass, 1-4; 0 = 2 */
+    uint32_t graphs;            /* serving loop iterations replayed from hipGraphs: 0 = default (on), 1 = on,
+                                   2 = off (every iteration launched eagerly) */
 } crt_config;
 
 enum { CRT_EXTEND_DEFAULT = 0, CRT_EXTEND_GRID_STRIDE = 1, CRT_EXTEND_PERSISTENT = 2, CRT_EXTEND_WAVE_CHUNK = 3 };

@@ -145,6 +145,28 @@ def test_serving_alone_vs_mix_and_cancel():
         assert e.value.code == A.CRT_ERR_CANCELLED
 
 
+@pytest.mark.parametrize("graphs", [1, 2])
+def test_serving_graph_replays_match_eager(graphs):
+    """Serving iterations replayed from captured hipGraphs (graphs=1) or launched eagerly (2): many
+    replays with the program's own eager renders and synchronisations in between, every result
+    bit-identical to the blocking (eager) render of the same request."""
+    with Renderer(max_paths=150000, graphs=graphs) as r:
+        a = r.upload(scenes.sphere_mesh_scene(3000, seed=2))
+        b = r.upload(scenes.instanced_scene(seed=4, n_instances=16, n_textures=1, tex_size=32, nu=8, nv=6, extent=6.0))
+        jobs = [(a, 96, 64, 9, 6, 31), (b, 64, 64, 4, 6, 32), (a, 64, 96, 1, 8, 33)]
+        refs = [r.render(s, W, H, spp, d, seed=sd)[1] for s, W, H, spp, d, sd in jobs]
+        for rep in range(4):
+            for (s, W, H, spp, d, sd), ref in zip(jobs, refs):
+                rid = r.submit(s, W, H, spp, d, seed=sd)
+                _, acc, _ = r.wait(rid, W, H)
+                assert np.array_equal(acc, ref), (rep, W, H)
+            # eager work between replays
+            assert np.array_equal(r.render(a, 96, 64, 9, 6, seed=31)[1], refs[0])
+        ids = [(r.submit(s, W, H, spp, d, seed=sd), W, H, ref) for (s, W, H, spp, d, sd), ref in zip(jobs * 3, refs * 3)]
+        for rid, W, H, ref in ids:
+            assert np.array_equal(r.wait(rid, W, H)[1], ref)
+
+
 def test_serving_admission_waits_for_budget():
     """A budget with room for the resident scene and one request: requests are admitted one at a
     time (each admission after the previous completion) and all complete; a scene that can never
