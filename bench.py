@@ -199,7 +199,7 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False,
     from cortexraytracercuda_amd import Renderer
     rc = Renderer(count_work=True, bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
                   refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
-                  sub_passes=args.sub_passes, leaf_size=args.leaf_size)
+                  sub_passes=args.sub_passes, leaf_size=args.leaf_size, stack_lds=getattr(args, "stack_lds", 0))
     sidc = rc.upload(scene)
     node_b = rc.info(sidc)["node_bytes"]
     cnt = dict.fromkeys(["rays", "node_visits", "leaf_visits", "tri_tests", "sphere_tests", "quad_tests", "inst_visits"], 0)
@@ -224,7 +224,7 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False,
     # workload's closest-hit kernel, times the rays of one launch; null for workloads not profiled
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_extend_traffic_%s.json" % workload_key)
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.bvh_width in (0, 4) and args.leaf_size in (0, 2):   # the profiled BVH shape
         with open(tpath) as f:
             t = json.load(f)
         traffic = t["hbm_bytes_per_ray"] * cnt["rays"] / max(launches, 1)
@@ -259,6 +259,13 @@ def single_render_bench(scene, cfg, args, workload, progressive=False, workload_
     sid = r.upload(scene)
     info = r.info(sid)
     log("upload+build %.2f s %s" % (time.perf_counter() - t0, info))
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = r.autotune(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, scene=scene)
+        tune["seconds"] = time.perf_counter() - t0
+        info = r.info(sid)
+        log("autotune %s" % tune)
     ranges = [(k, 1) for k in range(cfg["spp"])] if progressive else [(0, 0)]
 
     def one_step():
@@ -287,6 +294,12 @@ def single_render_bench(scene, cfg, args, workload, progressive=False, workload_
     mk = None
     if not args.no_megakernel:
         mk = _megakernel_bench(r, sid, cfg, ranges)
+    if tune is not None:
+        # the counting and event-timed renders use the tuned shape too
+        args = argparse.Namespace(**vars(args))
+        args.max_paths, args.grid_per_cu, args.shade_grid_per_cu = tune["max_paths"], tune["grid_per_cu"], tune["shade_grid_per_cu"]
+        args.sub_passes, args.extend_kernel, args.refill_min = tune["sub_passes"], tune["extend_kernel"], tune["refill_min"]
+        args.stack_lds, args.bvh_width, args.leaf_size = tune["stack_lds"], tune["bvh_width"], tune["leaf_size"]
     roof = traversal_roofline(scene, cfg, args, r, sid, progressive, workload_key)
 
     r.close()
@@ -298,7 +311,7 @@ def single_render_bench(scene, cfg, args, workload, progressive=False, workload_
             "rays_per_step": rays // args.steps, "segments_per_s": value * 1e6, "gpu_ms_per_step": gpu_ms / args.steps,
             "bvh": {"nodes": info["num_nodes"], "width": info["bvh_width"], "depth": info["max_depth"],
                     "build_ms": info["build_ms"], "upload_ms": info["upload_ms"], "device_bytes": info["device_bytes"]},
-            "roofline": roof,
+            "roofline": roof, "autotune": tune,
             "megakernel": mk and dict(mk, wavefront_speedup=value / mk["mrays_s"])}
 
 
@@ -358,6 +371,7 @@ def main():
     ap.add_argument("--sub-passes", type=int, default=0, help="concurrent sub-passes per pass (0: default)")
     ap.add_argument("--leaf-size", type=int, default=0, help="BVH max primitives per leaf (0: default)")
     ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel comparison render")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library defaults (no crt_autotune before timing)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()

@@ -122,7 +122,7 @@ class crt_config(C.Structure):
                 ("block_size", C.c_uint32), ("count_work", C.c_uint32), ("leaf_size", C.c_uint32),
                 ("bvh_width", C.c_uint32), ("extend_kernel", C.c_uint32), ("refill_min", C.c_uint32),
                 ("pass_paths", C.c_uint32), ("grid_per_cu", C.c_uint32), ("shade_grid_per_cu", C.c_uint32),
-                ("sub_passes", C.c_uint32), ("graphs", C.c_uint32)]
+                ("sub_passes", C.c_uint32), ("graphs", C.c_uint32), ("stack_lds", C.c_uint32)]
 
 
 class crt_scene_info(C.Structure):
@@ -162,17 +162,23 @@ class crt_scenefile_info(C.Structure):
                 ("output_path", C.c_char * 512)]
 
 
+class crt_tune_result(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("max_paths", "grid_per_cu", "shade_grid_per_cu", "sub_passes", "extend_kernel",
+                                          "refill_min", "stack_lds", "bvh_width", "leaf_size", "trials")] + \
+               [("best_mrays_s", C.c_double), ("initial_mrays_s", C.c_double)]
+
+
 STRUCTS_IN_ABI_ORDER = [crt_camera, crt_material, crt_texture, crt_uvmap, crt_sphere, crt_quad, crt_area_light,
                         crt_mesh, crt_instance, crt_scene_desc, crt_render_desc, crt_image, crt_stats, crt_config,
                         crt_scene_info, crt_batch_item, crt_request_timing, crt_work_counters,
-                        crt_memory_info, crt_obj_mesh, crt_scenefile_info]
+                        crt_memory_info, crt_obj_mesh, crt_scenefile_info, crt_tune_result]
 
 # Every symbol declared in include/cortex_rt.h.
 EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "crt_scene_upload", "crt_scene_release",
             "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
             "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
             "crt_memory_get_info", "crt_obj_load", "crt_obj_free", "crt_scenefile_load",
-            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes"]
+            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes", "crt_autotune"]
 
 _P = C.POINTER
 _SIGS = {
@@ -202,6 +208,7 @@ _SIGS = {
     "crt_scenefile_desc": ([C.c_void_p, _P(crt_scene_desc), _P(crt_scenefile_info)], C.c_int),
     "crt_scenefile_close": ([C.c_void_p], C.c_int),
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
+    "crt_autotune": ([C.c_void_p, C.c_uint32, _P(crt_scene_desc), _P(crt_render_desc), _P(crt_tune_result)], C.c_int),
 }
 
 _lib = None

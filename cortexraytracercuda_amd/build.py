@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcortex_rt.so")
-SOURCES = ["kernels.hip", "scene_build.cpp", "api.cpp", "serve.cpp", "obj_loader.cpp", "scenefile.cpp"]
+SOURCES = ["kernels.hip", "scene_build.cpp", "api.cpp", "serve.cpp", "tune.cpp", "obj_loader.cpp", "scenefile.cpp"]
 CLI = os.path.join(HERE, "crt_render")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 (v_pk_*_f32) is disabled for every device compile

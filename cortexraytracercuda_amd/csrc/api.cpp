@@ -115,8 +115,13 @@ int Context::init(const crt_config* cfg) {
     extend_mode = config.extend_kernel ? (int)config.extend_kernel : CRT_EXTEND_GRID_STRIDE;
     extend_param = config.refill_min ? std::min(config.refill_min, 64u) : (extend_mode == CRT_EXTEND_PERSISTENT ? 16u : 2u);
     if (bvh_width != 2 && bvh_width != 4) return set_error(CRT_ERR_INVALID_ARGUMENT, "bvh_width must be 2 or 4");
-    extend_grid = (uint32_t)num_cus * (config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 32u);
-    shade_grid = (uint32_t)num_cus * (config.shade_grid_per_cu ? std::min(config.shade_grid_per_cu, 256u) : 8u);
+    stack_lds = config.stack_lds ? config.stack_lds : 16u;
+    if (stack_lds != 8 && stack_lds != 16 && stack_lds != 32)
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "stack_lds must be 8, 16 or 32");
+    grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 32u;
+    shade_grid_per_cu = config.shade_grid_per_cu ? std::min(config.shade_grid_per_cu, 256u) : 8u;
+    extend_grid = (uint32_t)num_cus * grid_per_cu;
+    shade_grid = (uint32_t)num_cus * shade_grid_per_cu;
     int rc;
     if ((rc = scene_table.alloc(sizeof(crt_dev_scene) * kMaxScenes, this))) return rc;
     if ((rc = counters.alloc(sizeof(crt_dev_counters), this))) return rc;
@@ -130,7 +135,7 @@ int Context::init(const crt_config* cfg) {
         HIP_TRY(hipEventCreateWithFlags(&sub_done[k], hipEventDisableTiming));
     }
     HIP_TRY(hipEventCreateWithFlags(&ev_pass_ready, hipEventDisableTiming));
-    if ((rc = gstack.alloc((size_t)extend_grid * 256u * crt_k_gstack_entries() * sizeof(uint32_t), this))) return rc;
+    if ((rc = ensure_gstack(extend_grid))) return rc;
     for (int k = 0; k < 2; ++k) {
         if ((rc = req_table[k].alloc(sizeof(crt_dev_request) * kMaxPassRequests, this))) return rc;
         HIP_TRY(hipHostMalloc(&req_host[k], sizeof(crt_dev_request) * kMaxPassRequests, hipHostMallocDefault));
@@ -177,6 +182,14 @@ int Context::ensure_paths(uint32_t n, uint32_t pass_ids) {
     return CRT_OK;
 }
 
+// Traversal overflow stacks for `blocks` workgroups of the closest-hit launches.
+int Context::ensure_gstack(uint32_t blocks) {
+    const size_t need = (size_t)blocks * 256u * crt_k_gstack_entries() * sizeof(uint32_t);
+    if (need <= gstack.bytes) return CRT_OK;
+    if (stream) HIP_TRY(hipStreamSynchronize(stream));
+    return gstack.alloc(need, this);
+}
+
 crt_paths Context::paths(int k, uint32_t slot0) {
     crt_paths p;
     p.ray_o = (float4*)ray_o[k].p + slot0; p.ray_d = (float4*)ray_d[k].p + slot0; p.thr = (float4*)thr[k].p + slot0;
@@ -211,10 +224,15 @@ Context::~Context() {
 // ------------------------------------------------------------------ scenes
 int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
     std::lock_guard<std::mutex> g(gpu_mutex);
+    return upload_scene_locked(desc, leaf_size, bvh_width, out);
+}
+
+// Builds and uploads a scene with the given BVH shape (caller holds gpu_mutex).
+int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint32_t width, crt_scene_id* out) {
     auto sc = std::make_unique<DeviceScene>();
     std::string err;
     unsigned hw = std::thread::hardware_concurrency();
-    int rc = build_host_scene(desc, leaf_size, bvh_width, (int)std::min(16u, hw ? hw : 1u), sc->host, err);
+    int rc = build_host_scene(desc, leaf, width, (int)std::min(16u, hw ? hw : 1u), sc->host, err);
     if (rc) return set_error(rc, err);
     uint64_t need = sc->host.device_bytes();
     if (scene_bytes + request_bytes + reserved + need > budget)
@@ -270,6 +288,10 @@ int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
 
 int Context::release_scene(crt_scene_id id) {
     std::lock_guard<std::mutex> g(gpu_mutex);
+    return release_scene_locked(id);
+}
+
+int Context::release_scene_locked(crt_scene_id id) {
     auto it = scenes.find(id);
     if (it == scenes.end()) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
     if (it->second->refs > 0) return set_error(CRT_ERR_INVALID_ARGUMENT, "scene has requests in flight");
@@ -376,9 +398,12 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
     uint32_t npaths = base;
     int32_t uniform_scene = (int32_t)items[0].job->req.scene;
     int two_level = 0;
+    const uint32_t width = items[0].job->scene->host.bvh_width;   // the traversal kernel follows the scenes' BVH
     for (auto& w : items) {
         if ((int32_t)w.job->req.scene != uniform_scene) uniform_scene = -1;
         if (w.job->scene->host.num_instances) two_level = 1;
+        if (w.job->scene->host.bvh_width != width)
+            return set_error(CRT_ERR_INVALID_ARGUMENT, "scenes of different BVH widths in one pass");
     }
     if (npaths > pass_capacity) return set_error(CRT_ERR_INTERNAL, "pass exceeds the pass capacity");
     crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
@@ -388,7 +413,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
     // The pass runs as K independent sub-passes on K streams (contiguous path-id ranges, disjoint
     // slot ranges, own control blocks and overflow stacks), so the tail of one sub-pass's launch
     // overlaps full launches of the others. Timed-stage runs use one stream.
-    const uint32_t nslots_all = std::min(path_capacity, npaths);
+    const uint32_t nslots_all = std::min(std::min(max_paths, path_capacity), npaths);
     uint32_t K = time_stages ? 1u : std::min<uint32_t>(sub_passes, std::max<uint32_t>(1u, nslots_all / 65536u));
     struct Sub { uint32_t lo, hi, slot0, nslots, egrid, sgrid; hipStream_t s; uint64_t k; bool done; };
     Sub subs[kMaxSubPasses];
@@ -447,9 +472,9 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             if (crt_k_regen(u.s, mode, u.sgrid, dreq, nreq, u.lo, total, u.nslots, cin, cout, nullptr, A))
                 return set_error(CRT_ERR_HIP, "regen launch failed");
             if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], u.s));
-            if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)bvh_width, two_level, u.egrid, dsc, dreq, uniform_scene,
+            if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)width, two_level, u.egrid, dsc, dreq, uniform_scene,
                              nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, hq, hiq, gq, nullptr, extend_mode,
-                             cin + CRT_CTL_WORK, extend_param, (crt_dev_counters*)counters.p))
+                             cin + CRT_CTL_WORK, extend_param, (int)stack_lds, (crt_dev_counters*)counters.p))
                 return set_error(CRT_ERR_HIP, "extend launch failed");
             if (time_stages) {
                 HIP_TRY(hipEventRecord(ev_ext[1], u.s));
@@ -490,9 +515,13 @@ int Context::run_plain(Job& j, bool time_stages, double* extend_ms) {
     j.s_next = j.s_end;
     crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
     HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request), hipMemcpyHostToDevice, stream));
-    uint32_t grid = std::min(extend_grid, std::max(1u, (j.npix + 255) / 256));
+    // the baseline keeps its own launch shape (one 256-thread workgroup per 256 pixels, at most 32
+    // per CU), independent of the tuned closest-hit grid
+    const uint32_t grid = std::min((uint32_t)num_cus * 32u, std::max(1u, (j.npix + 255) / 256));
+    int rc = ensure_gstack(grid);
+    if (rc) return rc;
     if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-    if (crt_k_plain(stream, (int)j.rd.rng_mode, (int)bvh_width, j.scene->host.num_instances ? 1 : 0, grid,
+    if (crt_k_plain(stream, (int)j.rd.rng_mode, (int)j.scene->host.bvh_width, j.scene->host.num_instances ? 1 : 0, grid,
                     (const crt_dev_scene*)scene_table.p, dreq, (uint32_t*)gstack.p, (crt_dev_counters*)counters.p))
         return set_error(CRT_ERR_HIP, "megakernel launch failed");
     if (time_stages) {
@@ -812,7 +841,7 @@ int crt_struct_sizes(uint64_t* sizes, uint32_t n) {
                     sizeof(crt_quad), sizeof(crt_area_light), sizeof(crt_mesh), sizeof(crt_instance), sizeof(crt_scene_desc),
                     sizeof(crt_render_desc), sizeof(crt_image), sizeof(crt_stats), sizeof(crt_config), sizeof(crt_scene_info),
                     sizeof(crt_batch_item), sizeof(crt_request_timing), sizeof(crt_work_counters), sizeof(crt_memory_info), sizeof(crt_obj_mesh),
-                    sizeof(crt_scenefile_info)};
+                    sizeof(crt_scenefile_info), sizeof(crt_tune_result)};
     uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
     for (uint32_t k = 0; k < n && k < m; ++k) sizes[k] = v[k];
     return (int)m;

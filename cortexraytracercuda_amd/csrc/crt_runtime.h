@@ -149,6 +149,8 @@ struct Context {
     uint32_t max_paths = 0, path_capacity = 0, leaf_size = 4, bvh_width = 4, extend_grid = 2048, shade_grid = 2048;
     int extend_mode = CRT_EXTEND_GRID_STRIDE;   // closest-hit work distribution (CRT_EXTEND_*)
     uint32_t extend_param = 2;       // lane-refill threshold or 64-ray groups per wave fetch
+    uint32_t stack_lds = 16;         // LDS ring entries per lane of the traversal stack (8, 16, 32)
+    uint32_t grid_per_cu = 32, shade_grid_per_cu = 8;
     std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
     bool slot_used[kMaxScenes] = {};
@@ -191,9 +193,16 @@ struct Context {
     ~Context();
     uint64_t used_bytes() const;
     int ensure_paths(uint32_t slots, uint32_t pass_ids);
+    int ensure_gstack(uint32_t blocks);
+    uint64_t server_bytes() const;
+    int autotune(crt_scene_id scene, const crt_scene_desc* desc, const crt_render_desc* rd, crt_tune_result* out);
+    int tune_trial(crt_scene_id scene, const crt_render_desc& rd, double& mrays_s);
+    int trial_scene(crt_scene_id scene, const crt_render_desc& rd, double& mrays_s, uint32_t& trials);
     crt_paths paths(int k, uint32_t slot0);
     int upload_scene(const crt_scene_desc* desc, crt_scene_id* out);
+    int upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint32_t width, crt_scene_id* out);
     int release_scene(crt_scene_id id);
+    int release_scene_locked(crt_scene_id id);
     DeviceScene* find_scene(crt_scene_id id);
     int run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms);
     int run_plain(Job& j, bool time_stages, double* extend_ms);

@@ -1306,22 +1306,15 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                             const crt_dev_scene* scenes, const crt_dev_request* reqs, int32_t uniform_scene,
                             const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                             const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
-                            uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param,
+                            uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
                             crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
     if (mode != CRT_EXTEND_PERSISTENT && mode != CRT_EXTEND_WAVE_CHUNK) work_counter = nullptr;
     if (!work_counter) mode = CRT_EXTEND_GRID_STRIDE;
-    // persistent modes launch at most the resident blocks (they fetch work until the queue drains)
-#define CRT_EXT(C, U, W, T)                                                                                            \
-    if (mode == CRT_EXTEND_PERSISTENT) {                                                                               \
-        auto kfn = k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                               \
-        static uint32_t rb = 0;                                                                                         \
-        if (!rb) rb = resident_blocks(kfn, 256);                                                                        \
-        hipLaunchKernelGGL(kfn, dim3(grid < rb ? grid : rb), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr,      \
-                           count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param,  \
-                           counters);                                                                                   \
-    } else {                                                                                                            \
-        auto kfn = k_extend<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                                  \
+    // grid-stride / wave-chunk kernel with an LDS stack ring of LSV entries per lane
+#define CRT_EXT_GS(LSV, C, U, W, T)                                                                                    \
+    {                                                                                                                   \
+        auto kfn = k_extend<256, STK_RS, LSV, STK_GS, C, U, W, T>;                                                     \
         static uint32_t rb = 0;                                                                                         \
         if (!rb) rb = resident_blocks(kfn, 256);                                                                        \
         uint32_t g = (mode == CRT_EXTEND_WAVE_CHUNK && grid > rb) ? rb : grid;                                          \
@@ -1329,16 +1322,37 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                            ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param ? param : 1u,         \
                            counters);                                                                                   \
     }
-#define CRT_EXT_WT(C, U)                                                   \
-    if (width == 2) { if (two_level) { CRT_EXT(C, U, 2, true) } else { CRT_EXT(C, U, 2, false) } } \
-    else { if (two_level) { CRT_EXT(C, U, 4, true) } else { CRT_EXT(C, U, 4, false) } }
+    // persistent lane-refill kernel (launches at most the resident blocks: it fetches work until the queue drains)
+#define CRT_EXT_PT(C, U, W, T)                                                                                         \
+    {                                                                                                                   \
+        auto kfn = k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                               \
+        static uint32_t rb = 0;                                                                                         \
+        if (!rb) rb = resident_blocks(kfn, 256);                                                                        \
+        hipLaunchKernelGGL(kfn, dim3(grid < rb ? grid : rb), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr,      \
+                           count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param,  \
+                           counters);                                                                                   \
+    }
+    // counting variants: one stack depth; timed variants: LDS ring of 8, 16 or 32 entries per lane
+#define CRT_EXT_C(U, W, T) \
+    if (mode == CRT_EXTEND_PERSISTENT) CRT_EXT_PT(true, U, W, T) else CRT_EXT_GS(STK_LS, true, U, W, T)
+#define CRT_EXT_N(U, W, T)                                                     \
+    if (mode == CRT_EXTEND_PERSISTENT) CRT_EXT_PT(false, U, W, T)              \
+    else if (stack_lds == 8) CRT_EXT_GS(8, false, U, W, T)                     \
+    else if (stack_lds == 32) CRT_EXT_GS(32, false, U, W, T)                   \
+    else CRT_EXT_GS(STK_LS, false, U, W, T)
+#define CRT_EXT_WT(X, U)                                                                     \
+    if (width == 2) { if (two_level) { X(U, 2, true) } else { X(U, 2, false) } }             \
+    else { if (two_level) { X(U, 4, true) } else { X(U, 4, false) } }
     if (count_mode) {
-        if (uniform_scene >= 0) { CRT_EXT_WT(true, true) } else { CRT_EXT_WT(true, false) }
+        if (uniform_scene >= 0) { CRT_EXT_WT(CRT_EXT_C, true) } else { CRT_EXT_WT(CRT_EXT_C, false) }
     } else {
-        if (uniform_scene >= 0) { CRT_EXT_WT(false, true) } else { CRT_EXT_WT(false, false) }
+        if (uniform_scene >= 0) { CRT_EXT_WT(CRT_EXT_N, true) } else { CRT_EXT_WT(CRT_EXT_N, false) }
     }
 #undef CRT_EXT_WT
-#undef CRT_EXT
+#undef CRT_EXT_N
+#undef CRT_EXT_C
+#undef CRT_EXT_PT
+#undef CRT_EXT_GS
     return (int)hipGetLastError();
 }
 

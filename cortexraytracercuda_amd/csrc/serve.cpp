@@ -109,10 +109,16 @@ int Context::server_init() {
         return set_error(CRT_ERR_HIP, "fill launch failed");
     S.ready = true;
     server_reset_pass();
-    reserved = used_bytes() + S.ray_o[0].bytes * 2 + S.ray_d[0].bytes * 2 + S.thr[0].bytes * 2 + S.rad[0].bytes * 2 +
-               S.ids[0].bytes * 2 + S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes +
-               S.stats.bytes + S.gstack.bytes + S.counters.bytes;
+    reserved = used_bytes() + server_bytes();
     return CRT_OK;
+}
+
+uint64_t Context::server_bytes() const {
+    const Server& S = srv;
+    uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.stats.bytes +
+                 S.gstack.bytes + S.counters.bytes;
+    for (int k = 0; k < 2; ++k) b += S.ray_o[k].bytes + S.ray_d[k].bytes + S.thr[k].bytes + S.rad[k].bytes + S.ids[k].bytes;
+    return b;
 }
 
 void Context::server_release() {
@@ -228,7 +234,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
         return set_error(CRT_ERR_HIP, "regen launch failed");
     if (crt_k_extend(S.s, 0, (int)C.bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
                      A.ray_d, A.ids, (float4*)S.hit.p, (uint32_t*)S.hit_inst.p, (uint32_t*)S.gstack.p, nullptr,
-                     C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (crt_dev_counters*)S.counters.p))
+                     C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
     if (crt_k_shade(S.s, CRT_RNG_DYNAMIC, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.stats.p,

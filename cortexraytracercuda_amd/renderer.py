@@ -41,7 +41,7 @@ class Renderer:
 
     def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0,
                  extend_kernel=0, refill_min=0, pass_paths=0, grid_per_cu=0, shade_grid_per_cu=0, sub_passes=0,
-                 graphs=0):
+                 graphs=0, stack_lds=0):
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -57,6 +57,7 @@ class Renderer:
         cfg.shade_grid_per_cu = int(shade_grid_per_cu)
         cfg.sub_passes = int(sub_passes)
         cfg.graphs = int(graphs)
+        cfg.stack_lds = int(stack_lds)
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -116,6 +117,20 @@ class Renderer:
         st = A.crt_stats()
         A.check(self.lib.crt_render(self.h, sid, C.byref(rd), C.byref(img), C.byref(st)))
         return rgb, acc, stats_dict(st)
+
+    def autotune(self, sid, width, height, spp, max_depth, seed=1, rng="pcg", scene=None):
+        """Times trial renders of this request over the launch/memory shape (and, with `scene`, the
+        BVH shape, rebuilt in place) and keeps the fastest as the context's setting."""
+        rd = render_desc(width, height, spp, max_depth, seed, rng)
+        res = A.crt_tune_result()
+        keep = None
+        dp = None
+        if scene is not None:
+            desc, keep = scene.to_desc()
+            dp = C.byref(desc)
+        A.check(self.lib.crt_autotune(self.h, sid, dp, C.byref(rd), C.byref(res)))
+        del keep
+        return {k: getattr(res, k) for k, _ in A.crt_tune_result._fields_}
 
     def counters(self):
         """Work counters of the last blocking render (traversal counts need count_work=True)."""

@@ -242,6 +242,8 @@ typedef struct crt_config {
     uint32_t sub_passes;        /* concurrent sub-passes (one stream each) of a pass, 1-4; 0 = 2 */
     uint32_t graphs;            /* serving loop iterations replayed from hipGraphs: 0 = default (on), 1 = on,
                                    2 = off (every iteration launched eagerly) */
+    uint32_t stack_lds;         /* traversal stack: LDS ring entries per lane behind the 4 register entries,
+                                   8, 16 or 32; 0 = 16 */
 } crt_config;
 
 enum { CRT_EXTEND_DEFAULT = 0, CRT_EXTEND_GRID_STRIDE = 1, CRT_EXTEND_PERSISTENT = 2, CRT_EXTEND_WAVE_CHUNK = 3 };
@@ -309,6 +311,31 @@ typedef struct crt_request_timing {
     int32_t status;
 } crt_request_timing;
 int crt_request_get_timing(crt_context* ctx, crt_request_id req, crt_request_timing* out);
+
+/* ------------------------------------------------------------------ autotuning */
+typedef struct crt_tune_result {
+    uint32_t max_paths;          /* live path slots */
+    uint32_t grid_per_cu;        /* closest-hit workgroups per CU */
+    uint32_t shade_grid_per_cu;  /* regenerate / shade workgroups per CU */
+    uint32_t sub_passes;         /* concurrent sub-passes per pass */
+    uint32_t extend_kernel;      /* CRT_EXTEND_* */
+    uint32_t refill_min;         /* its parameter */
+    uint32_t stack_lds;          /* LDS stack entries per lane */
+    uint32_t bvh_width, leaf_size;   /* the scene's BVH after tuning */
+    uint32_t trials;             /* trial renders timed */
+    double best_mrays_s;         /* trial throughput of the choice */
+    double initial_mrays_s;      /* trial throughput of the settings on entry */
+} crt_tune_result;
+/* No counterpart in the reference, which fixes its launch at 16x16-thread blocks
+ * (core/src/gpu.cu:2072-2080). Picks the launch and memory shape of the streaming passes for
+ * renders like `rd` of `scene` by timing trial renders (a sample range of `rd`, full image) of each
+ * candidate, one tunable at a time: live path slots, closest-hit and shade workgroups per CU,
+ * sub-passes, closest-hit work distribution, traversal stack depth in LDS. With `desc` (the
+ * scene's description, as uploaded) it first rebuilds the scene's BVH at widths 2/4 and leaf sizes
+ * 1/2/4 and keeps the fastest in place (same scene id). The choice becomes the context's setting
+ * for later renders and uploads. Blocking; `desc` may be NULL. */
+int crt_autotune(crt_context* ctx, crt_scene_id scene, const crt_scene_desc* desc, const crt_render_desc* rd,
+                 crt_tune_result* out);
 
 /* ------------------------------------------------------------------ mesh ingestion */
 /* Wavefront OBJ reader (replaces the reference's tinyobjloader path, core/src/utilities/

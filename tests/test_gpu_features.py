@@ -308,3 +308,23 @@ def test_megakernel_not_served():
         rd.spp, rd.max_depth, rd.flags = 1, 2, A.CRT_RF_MEGAKERNEL
         rid = ctypes.c_uint64()
         assert r.lib.crt_submit(r.h, sid, ctypes.byref(rd), ctypes.byref(rid)) == A.CRT_ERR_INVALID_ARGUMENT
+
+
+def test_autotune_keeps_images():
+    """The autotuner times trial renders over the launch/memory and BVH shapes; the image of a
+    request does not depend on them (same estimator, same streams, ties aside)."""
+    s = scenes.sphere_mesh_scene(20000, seed=3)
+    with Renderer(max_paths=1 << 20) as r:
+        sid = r.upload(s)
+        _, acc0, _ = r.render(sid, 96, 96, 16, 8, seed=5)
+        res = r.autotune(sid, 96, 96, 16, 8, seed=5, scene=s)
+        print("autotune:", res)
+        assert res["trials"] >= 10 and res["best_mrays_s"] > 0
+        assert res["bvh_width"] in (2, 4) and res["leaf_size"] in (1, 2, 4) and res["stack_lds"] in (8, 16, 32)
+        assert res["best_mrays_s"] >= res["initial_mrays_s"]
+        _, acc1, _ = r.render(sid, 96, 96, 16, 8, seed=5)
+        assert r.info(sid)["bvh_width"] == res["bvh_width"]
+    assert np.array_equal(acc0[..., 3], acc1[..., 3])
+    same = np.all(np.abs(acc0 - acc1) <= 1e-5 * np.maximum(np.abs(acc0), 1.0), axis=-1).mean()
+    print("pixels equal after tuning: %.5f" % same)
+    assert same >= 0.999
