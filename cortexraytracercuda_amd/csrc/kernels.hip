@@ -107,6 +107,26 @@ template <int MODE> struct Rng {
     }
 };
 
+// ------------------------------------------------------------------ global loads
+// Scene arrays are reached through pointers read from the scene table, so the compiler cannot
+// tell they are global memory and emits flat loads (which also wait on the LDS counter the
+// traversal stack uses). These loads go through address-space-1 vector pointers: global_load.
+typedef float crt_v4f __attribute__((ext_vector_type(4)));
+typedef int crt_v4i __attribute__((ext_vector_type(4)));
+typedef unsigned crt_v2u __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ldg(const float4* p) {
+    crt_v4f v = *(__attribute__((address_space(1))) const crt_v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 ldg_i4(const float4* p) {
+    crt_v4i v = *(__attribute__((address_space(1))) const crt_v4i*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ldg(const uint2* p) {
+    crt_v2u v = *(__attribute__((address_space(1))) const crt_v2u*)p;
+    return make_uint2(v.x, v.y);
+}
+
 // ------------------------------------------------------------------ traversal stack
 // Short per-lane stack: RS entries in VGPRs (static shifts, no dynamic register indexing), spilling
 // to an LS-entry ring per lane in LDS, which in turn spills its oldest entries to a per-thread
@@ -220,7 +240,7 @@ __device__ __forceinline__ bool sphere_hit(V3 o, V3 d, float4 s, float tmin, flo
 // (core/src/lights/directional_light.cuh:76-118), inclusive interval.
 __device__ __forceinline__ bool quad_hit(V3 o, V3 d, const float4* q, float tmin, float tmax, bool camera,
                                          float& t, float& qa, float& qb) {
-    float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+    float4 q0 = ldg(q), q1 = ldg(q + 1), q2 = ldg(q + 2), q3 = ldg(q + 3), q4 = ldg(q + 4);
     V3 n = v3(q3);
     float denom = dot(n, d);
     if (fabsf(denom) < 1e-8f) return false;
@@ -324,8 +344,8 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
             float tf = best.t;
             if (WIDTH == 2) {
                 const float4* np = S.nodes + 4 * (uint32_t)node;
-                float4 n0 = np[0], n1 = np[1], n2 = np[2];
-                int4 n3 = *(const int4*)(np + 3);
+                float4 n0 = ldg(np), n1 = ldg(np + 1), n2 = ldg(np + 2);
+                int4 n3 = ldg_i4(np + 3);
                 bool hit0, hit1;
                 float near0 = box_near(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, o, id, tmin, tf, hit0);
                 float near1 = box_near(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, o, id, tmin, tf, hit1);
@@ -338,8 +358,9 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 else if (hit1) { node = n3.y; next = false; }
             } else {
                 const float4* np = S.nodes + 8 * (uint32_t)node;
-                float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-                int4 rf = *(const int4*)(np + 6);
+                float4 lx = ldg(np), hx = ldg(np + 1), ly = ldg(np + 2), hy = ldg(np + 3), lz = ldg(np + 4),
+                       hz = ldg(np + 5);
+                int4 rf = ldg_i4(np + 6);
                 bool h0, h1, h2, h3;
                 float k0 = box_near(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, id, tmin, tf, h0);
                 float k1 = box_near(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, id, tmin, tf, h1);
@@ -363,7 +384,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 }
             }
         } else {
-            uint2 lf = S.leaves[(uint32_t)(~node)];
+            uint2 lf = ldg(S.leaves + (uint32_t)(~node));
             uint32_t type = lf.y >> CRT_LEAF_TYPE_SHIFT;
             uint32_t cnt = lf.y & CRT_LEAF_COUNT_MASK;
             uint32_t first = lf.x;
@@ -373,14 +394,14 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                     const float4* tp4 = S.tris + 3 * (first + k);
                     float t, bu, bv;
                     if (COUNT) tc.tris++;
-                    if (tri_hit(o, tp, tp4[0], tp4[1], tp4[2], tmin, best.t, t, bu, bv)) {
+                    if (tri_hit(o, tp, ldg(tp4), ldg(tp4 + 1), ldg(tp4 + 2), tmin, best.t, t, bu, bv)) {
                         best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
                         best.inst = cur_inst;
                     }
                 }
             } else if (type == CRT_PRIM_SPHERE) {
                 for (uint32_t k = 0; k < cnt; ++k) {
-                    float4 s = S.spheres[first + k];
+                    float4 s = ldg(S.spheres + first + k);
                     float t;
                     if (COUNT) tc.spheres++;
                     if (sphere_hit(o, d, s, tmin, best.t, t)) {
@@ -400,8 +421,8 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
             } else if (TWO_LEVEL && !in_blas) {
                 // instance leaf (count 1): move the ray to object space and descend into the BLAS
                 const float4* ip = S.insts + 4 * first;
-                float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
-                int4 meta = *(const int4*)(ip + 3);
+                float4 r0 = ldg(ip), r1 = ldg(ip + 1), r2 = ldg(ip + 2);
+                int4 meta = ldg_i4(ip + 3);
                 if (COUNT) tc.insts++;
                 (void)meta.y;
                 set_ray(v3(r0.x * ow.x + r0.y * ow.y + r0.z * ow.z + r0.w, r1.x * ow.x + r1.y * ow.y + r1.z * ow.z + r1.w,
@@ -596,8 +617,8 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
         ovf = ovf || o1;
         if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
         const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
-        uint32_t fl = fbits(q[1].w);
-        if (!((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(q[3])) < 0.0f))) break;
+        uint32_t fl = fbits(ldg(q + 1).w);
+        if (!((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(ldg(q + 3))) < 0.0f))) break;
         tmin = h.t + 0.001f;
         pass_flag = CRT_PASSTHROUGH;
         nretrace++;
@@ -739,7 +760,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend_pt(const crt_dev_scene* __rest
             if (!pass_flag && h.prim != CRT_MISS && CRT_HIT_TYPE(h.prim) == CRT_PRIM_QUAD) {
                 // one-sided emitter seen from behind: re-trace from t + 0.001 (camera.cuh:179-183)
                 const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
-                again = (fbits(q[1].w) & CRT_QF_ONE_SIDED) && !(dot(tr.dw, v3(q[3])) < 0.0f);
+                again = (fbits(ldg(q + 1).w) & CRT_QF_ONE_SIDED) && !(dot(tr.dw, v3(ldg(q + 3))) < 0.0f);
             }
             novf += tr.st.overflow ? 1u : 0u;
             if (again) {
