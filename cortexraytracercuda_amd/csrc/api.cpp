@@ -116,6 +116,8 @@ int Context::init(const crt_config* cfg) {
     extend_param = config.refill_min ? std::min(config.refill_min, 64u) : (extend_mode == CRT_EXTEND_PERSISTENT ? 16u : 2u);
     if (bvh_width != 2 && bvh_width != 4) return set_error(CRT_ERR_INVALID_ARGUMENT, "bvh_width must be 2 or 4");
     stack_lds = config.stack_lds ? config.stack_lds : 16u;
+    if (config.ray_sort > 2) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown ray_sort");
+    ray_sort = config.ray_sort == 1u ? 1u : 0u;
     if (stack_lds != 8 && stack_lds != 16 && stack_lds != 32)
         return set_error(CRT_ERR_INVALID_ARGUMENT, "stack_lds must be 8, 16 or 32");
     grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 32u;
@@ -485,7 +487,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
                 extend_launches++;
             }
             if (crt_k_shade(u.s, mode, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, hq, hiq,
-                            (float4*)out_rad.p, nullptr, (crt_dev_counters*)counters.p))
+                            (float4*)out_rad.p, nullptr, (int)ray_sort, (crt_dev_counters*)counters.p))
                 return set_error(CRT_ERR_HIP, "shade launch failed");
             HIP_TRY(hipMemcpyAsync(ring + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
                                    hipMemcpyDeviceToHost, u.s));

@@ -37,7 +37,8 @@ int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32
                  crt_dev_counters* counters);
 int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes, const crt_dev_request* reqs,
                 const uint32_t* count_in, crt_paths Pin, crt_paths Pout, uint32_t* count_out, const float4* hit,
-                const uint32_t* hit_inst, float4* out_rad, uint32_t* done_count, crt_dev_counters* counters);
+                const uint32_t* hit_inst, float4* out_rad, uint32_t* done_count, int sort_octant,
+                crt_dev_counters* counters);
 int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* req, uint32_t* gstack, crt_dev_counters* counters);
 int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n, int camera,

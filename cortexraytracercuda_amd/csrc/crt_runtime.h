@@ -150,6 +150,7 @@ struct Context {
     int extend_mode = CRT_EXTEND_GRID_STRIDE;   // closest-hit work distribution (CRT_EXTEND_*)
     uint32_t extend_param = 2;       // lane-refill threshold or 64-ray groups per wave fetch
     uint32_t stack_lds = 16;         // LDS ring entries per lane of the traversal stack (8, 16, 32)
+    uint32_t ray_sort = 0;           // shade groups each block's survivors by direction octant
     uint32_t grid_per_cu = 32, shade_grid_per_cu = 8;
     std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;

@@ -482,6 +482,34 @@ template <int BLOCK> __device__ __forceinline__ uint32_t block_append(bool keep,
     return pos;
 }
 
+// Block-wide append with the keepers grouped by a 3-bit key (the direction octant of the next
+// segment): within the block's output range, keepers are ordered key-major, then wave, then lane.
+// Waves of the next closest-hit launch then hold rays of neighbouring paths that share an octant,
+// so their near-first traversal orders agree. Deterministic (ballot + mbcnt ranks, one atomic per
+// block). s_cnt holds 8 * BLOCK/64 words.
+template <int BLOCK> __device__ __forceinline__ uint32_t block_append_sorted(bool keep, uint32_t key, uint32_t* counter,
+                                                                           uint32_t* s_cnt, uint32_t* s_base) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NW = BLOCK / 64;
+    uint32_t rank = 0;
+#pragma unroll
+    for (uint32_t o = 0; o < 8; ++o) {
+        uint64_t m = __ballot(keep && key == o);
+        if (key == o) rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (lane == 0) s_cnt[o * NW + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int e = 0; e < 8 * NW; ++e) { uint32_t c = s_cnt[e]; s_cnt[e] = run; run += c; }
+        *s_base = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = *s_base + s_cnt[(key & 7u) * NW + wave] + rank;
+    __syncthreads();
+    return pos;
+}
+
 // Adds every lane's v to *p with one atomic per wave (a per-lane atomic on one address serialises
 // in L2). All lanes of the wave must call it.
 __device__ __forceinline__ void wave_add(unsigned long long* p, uint32_t v) {
@@ -637,7 +665,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
                                                   const float4* __restrict__ ray_d, const uint2* __restrict__ ids,
                                                   float4* __restrict__ hit, uint32_t* __restrict__ hit_inst,
                                                   uint32_t* __restrict__ gstack, uint32_t* __restrict__ zero_counter,
-                                                  uint32_t* __restrict__ work_counter, uint32_t chunk,
+                                                  uint32_t* __restrict__ work_counter, uint32_t chunk_groups,
                                                   crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
@@ -674,10 +702,10 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restric
         const uint32_t lane = threadIdx.x & 63u;
         for (;;) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(work_counter, 64u * chunk);
+            if (lane == 0) base = atomicAdd(work_counter, 64u * chunk_groups);
             base = (uint32_t)__shfl((int)base, 0);
             if (base >= n) break;
-            const uint32_t wend = min(base + 64u * chunk, n);
+            const uint32_t wend = min(base + 64u * chunk_groups, n);
             for (uint32_t i = base + lane; i < wend; i += 64u) one_ray(i);
         }
     } else {
@@ -1071,8 +1099,9 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                                                  const uint32_t* __restrict__ count_in, crt_paths Pin, crt_paths Pout,
                                                  uint32_t* __restrict__ count_out, const float4* __restrict__ hit,
                                                  const uint32_t* __restrict__ hit_inst, float4* __restrict__ out_rad,
-                                                 uint32_t* __restrict__ done_count, crt_dev_counters* __restrict__ counters) {
-    __shared__ uint32_t s_wave[BLOCK / 64];
+                                                 uint32_t* __restrict__ done_count, int sort_octant,
+                                                 crt_dev_counters* __restrict__ counters) {
+    __shared__ uint32_t s_oct[8 * (BLOCK / 64)];
     __shared__ uint32_t s_base;
     // per-block cache of the per-entry (finished, rays) counts, keyed by entry & 7: one global
     // atomic pair per cached entry and block at the end instead of one per wave and iteration
@@ -1090,6 +1119,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
         uint32_t i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
         bool keep = false, finished = false;
         float4 no, nd, nt, nr;
+        uint32_t oct = 0;   // direction octant of the next segment
         uint2 idv = make_uint2(0u, 0u);
         if (i < n) {
             float4 hv = hit[i];
@@ -1116,6 +1146,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                 nd = make_float4(d.x, d.y, d.z, bitsf(ns));
                 nt = make_float4(att.x, att.y, att.z, bitsf(g.ms));
                 nr = make_float4(L.x, L.y, L.z, 0.0f);
+                oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
             } else {
                 // finished: the radiance goes to the path's id
                 uint32_t pid = R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin);
@@ -1149,8 +1180,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                 pending &= ~same;
             }
         }
-        // survivors go to the front of the other state buffer, in wave/block rank order
-        uint32_t pos = block_append<BLOCK>(keep, count_out, s_wave, &s_base);
+        // survivors go to the front of the other state buffer (grouped by direction octant when
+        // sort_octant is set; a launch-uniform choice)
+        uint32_t pos = sort_octant ? block_append_sorted<BLOCK>(keep, oct, count_out, s_oct, &s_base)
+                                   : block_append<BLOCK>(keep, count_out, s_oct, &s_base);
         if (keep) {
             Pout.ray_o[pos] = no;
             Pout.ray_d[pos] = nd;
@@ -1380,10 +1413,10 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
 extern "C" int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes,
                            const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout,
                            uint32_t* count_out, const float4* hit, const uint32_t* hit_inst, float4* out_rad,
-                           uint32_t* done_count, crt_dev_counters* counters) {
+                           uint32_t* done_count, int sort_octant, crt_dev_counters* counters) {
 #define CRT_SH(M)                                                                                                     \
     hipLaunchKernelGGL((k_shade<M, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout, count_out, \
-                       hit, hit_inst, out_rad, done_count, counters)
+                       hit, hit_inst, out_rad, done_count, sort_octant, counters)
     if (rng_mode == CRT_RNG_MINSTD) CRT_SH(CRT_RNG_MINSTD);
     else if (rng_mode == CRT_RNG_PCG) CRT_SH(CRT_RNG_PCG);
     else CRT_SH(CRT_RNG_DYNAMIC);

@@ -41,7 +41,7 @@ class Renderer:
 
     def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0,
                  extend_kernel=0, refill_min=0, pass_paths=0, grid_per_cu=0, shade_grid_per_cu=0, sub_passes=0,
-                 graphs=0, stack_lds=0):
+                 graphs=0, stack_lds=0, ray_sort=0):
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -58,6 +58,7 @@ class Renderer:
         cfg.sub_passes = int(sub_passes)
         cfg.graphs = int(graphs)
         cfg.stack_lds = int(stack_lds)
+        cfg.ray_sort = int(ray_sort)
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h

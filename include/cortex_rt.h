@@ -244,6 +244,8 @@ typedef struct crt_config {
                                    2 = off (every iteration launched eagerly) */
     uint32_t stack_lds;         /* traversal stack: LDS ring entries per lane behind the 4 register entries,
                                    8, 16 or 32; 0 = 16 */
+    uint32_t ray_sort;          /* 1: the shade stage groups each workgroup's surviving paths by direction
+                                   octant (coherent waves in the next closest-hit launch); 0 / 2: off */
 } crt_config;
 
 enum { CRT_EXTEND_DEFAULT = 0, CRT_EXTEND_GRID_STRIDE = 1, CRT_EXTEND_PERSISTENT = 2, CRT_EXTEND_WAVE_CHUNK = 3 };
