@@ -7,7 +7,7 @@ streams, seed 1. One step = one complete render of that image (every sample of e
 Printed JSON line (rank 0):
   value        Mrays/s over the timed steps (closest-hit queries / wall time of the renders)
   roofline     traversal kernel (k_extend): algorithmic bytes per launch / HIP-event time
-  cpu_baseline reference-faithful CPU renderer (oracle/, FP32 minstd) on the host cores, 1 spp
+  cpu_baseline reference-faithful CPU renderer (oracle/, FP32 minstd) on the host cores, 4 spp
   megakernel   the plain path on the same workload: one kernel launch renders the image
   batched      fused passes over many small requests vs one megakernel launch per request
   serving      Poisson requests: continuous batching vs the plain one-launch-per-request server,
@@ -28,10 +28,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one closest-hit query in k_extend (declared layout, crt_layout.h):
 #   ray 32 (ray_o + ray_d) + ids 8 + queue index 4 + hit record 16 + hit instance 4 = 64 B of state
-#   + 64 B per BVH2 / 128 B per BVH4 interior node visit (child boxes + refs), 8 B per leaf record,
+#   + 64 B per BVH2 / 128 B per BVH4 interior node visit (child boxes + refs; a leaf's type, count
+#   and first primitive are packed into its reference in the parent, so a leaf visit reads nothing),
 #   + 48 B per triangle, 16 B per sphere, 80 B per quad/light, 64 B per instance record.
 B_STATE = 64
-B_LEAF, B_TRI, B_SPHERE, B_QUAD, B_INST = 8, 48, 16, 80, 64   # node bytes come from the scene (64 or 128)
+B_LEAF, B_TRI, B_SPHERE, B_QUAD, B_INST = 0, 48, 16, 80, 64   # node bytes come from the scene (64 or 128)
 
 
 def log(*a):
@@ -43,17 +44,19 @@ def large_workload():
     return scenes.large_scene(seed=1, levels=8, n_spheres=500), dict(width=1920, height=1080, spp=64, max_depth=10)
 
 
-def cpu_baseline(scene, cfg):
+def cpu_baseline(scene, cfg, spp=4):
+    """The oracle's reference-faithful renderer on the host cores, on a bounded sample of the
+    workload: every pixel at `spp` samples (about 25 core-seconds on the bench host at spp=4)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(threads, 64))
-    acc, st = O.render(scene, cfg["width"], cfg["height"], 1, cfg["max_depth"], seed=2580, rng="minstd", precision=32,
+    acc, st = O.render(scene, cfg["width"], cfg["height"], spp, cfg["max_depth"], seed=2580, rng="minstd", precision=32,
                        threads=threads, ref_style=True)
     return {"value": st["rays"] / st["seconds"] / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "%dx%d at 1 spp (all pixels), depth %d, reference-faithful FP32 minstd renderer with the "
-                      "reference's median-split BVH; %d rays in %.1f s" % (cfg["width"], cfg["height"], cfg["max_depth"],
-                                                                         st["rays"], st["seconds"])}
+            "sample": "%dx%d at %d spp (all pixels), depth %d, reference-faithful FP32 minstd renderer with the "
+                      "reference's median-split BVH; %d rays in %.1f s on %d threads" %
+                      (cfg["width"], cfg["height"], spp, cfg["max_depth"], st["rays"], st["seconds"], threads)}
 
 
 def _plain_and_batched(r, jobs):
@@ -201,7 +204,8 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False,
                   refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
                   sub_passes=args.sub_passes, leaf_size=args.leaf_size, stack_lds=getattr(args, "stack_lds", 0))
     sidc = rc.upload(scene)
-    node_b = rc.info(sidc)["node_bytes"]
+    inf = rc.info(sidc)
+    node_b = inf["node_bytes"]
     cnt = dict.fromkeys(["rays", "node_visits", "leaf_visits", "tri_tests", "sphere_tests", "quad_tests", "inst_visits"], 0)
     ext_ms = 0.0
     launches = 0
@@ -224,12 +228,18 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False,
     # workload's closest-hit kernel, times the rays of one launch; null for workloads not profiled
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_extend_traffic_%s.json" % workload_key)
-    if os.path.exists(tpath) and args.bvh_width in (0, 4) and args.leaf_size in (0, 2):   # the profiled BVH shape
+    if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
-        traffic = t["hbm_bytes_per_ray"] * cnt["rays"] / max(launches, 1)
+        # only for the BVH shape that was profiled (the bytes per ray follow the tree)
+        if (t["bvh_width"], t["leaf_size"]) == (inf["bvh_width"], inf["leaf_size"]):
+            traffic = t["hbm_bytes_per_ray"] * cnt["rays"] / max(launches, 1)
+    dram = None if traffic is None else traffic / (ext_ms / max(launches, 1) * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic, "traffic_unit": "bytes per launch", "kernel": "k_extend", "launches": launches,
+            "traffic": traffic, "traffic_unit": "bytes per launch",
+            # measured DRAM rate of the kernel (traffic / launch time) against the same peak
+            "dram_gbs": dram, "dram_frac": None if dram is None else dram / HBM_PEAK_GBS,
+            "kernel": "k_extend", "launches": launches,
             "avg_launch_ms": ext_ms / max(launches, 1),
             "bytes_per_launch": alg / max(launches, 1), "bytes_per_ray": alg / max(cnt["rays"], 1),
             "node_bytes": node_b, "node_visits_per_ray": cnt["node_visits"] / max(cnt["rays"], 1),
@@ -260,7 +270,7 @@ def single_render_bench(scene, cfg, args, workload, progressive=False, workload_
     info = r.info(sid)
     log("upload+build %.2f s %s" % (time.perf_counter() - t0, info))
     tune = None
-    if not args.no_autotune:
+    if args.autotune:
         t0 = time.perf_counter()
         tune = r.autotune(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, scene=scene)
         tune["seconds"] = time.perf_counter() - t0
@@ -371,7 +381,8 @@ def main():
     ap.add_argument("--sub-passes", type=int, default=0, help="concurrent sub-passes per pass (0: default)")
     ap.add_argument("--leaf-size", type=int, default=0, help="BVH max primitives per leaf (0: default)")
     ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel comparison render")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the library defaults (no crt_autotune before timing)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run crt_autotune on the workload before timing (default: the library's default shape)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()

@@ -129,8 +129,8 @@ class crt_config(C.Structure):
 class crt_scene_info(C.Structure):
     _fields_ = [("num_nodes", C.c_uint64), ("num_leaves", C.c_uint64), ("num_prims", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("max_depth", C.c_uint32), ("num_instances", C.c_uint32),
-                ("bvh_width", C.c_uint32), ("node_bytes", C.c_uint32),
-                ("build_ms", C.c_double), ("upload_ms", C.c_double), ("sah_cost", C.c_double)]
+                ("bvh_width", C.c_uint32), ("node_bytes", C.c_uint32), ("leaf_size", C.c_uint32),
+                ("reserved", C.c_uint32), ("build_ms", C.c_double), ("upload_ms", C.c_double), ("sah_cost", C.c_double)]
 
 
 class crt_batch_item(C.Structure):

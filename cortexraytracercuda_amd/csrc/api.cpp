@@ -108,9 +108,9 @@ int Context::init(const crt_config* cfg) {
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     budget = config.hbm_budget_bytes ? config.hbm_budget_bytes : (uint64_t)(0.9 * (double)freeb);
-    max_paths = config.max_paths ? config.max_paths : (1u << 24);
-    leaf_size = config.leaf_size ? config.leaf_size : 2u;
-    bvh_width = config.bvh_width ? config.bvh_width : 4u;
+    max_paths = config.max_paths ? config.max_paths : (1u << 25);
+    leaf_size = config.leaf_size ? config.leaf_size : kDefaultLeafSize;
+    bvh_width = config.bvh_width ? config.bvh_width : kDefaultBvhWidth;
     if (config.extend_kernel > CRT_EXTEND_WAVE_CHUNK) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown extend_kernel");
     extend_mode = config.extend_kernel ? (int)config.extend_kernel : CRT_EXTEND_GRID_STRIDE;
     extend_param = config.refill_min ? std::min(config.refill_min, 64u) : (extend_mode == CRT_EXTEND_PERSISTENT ? 16u : 2u);
@@ -120,8 +120,8 @@ int Context::init(const crt_config* cfg) {
     ray_sort = config.ray_sort == 1u ? 1u : 0u;
     if (stack_lds != 8 && stack_lds != 16 && stack_lds != 32)
         return set_error(CRT_ERR_INVALID_ARGUMENT, "stack_lds must be 8, 16 or 32");
-    grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 32u;
-    shade_grid_per_cu = config.shade_grid_per_cu ? std::min(config.shade_grid_per_cu, 256u) : 8u;
+    grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 16u;
+    shade_grid_per_cu = config.shade_grid_per_cu ? std::min(config.shade_grid_per_cu, 256u) : 16u;
     extend_grid = (uint32_t)num_cus * grid_per_cu;
     shade_grid = (uint32_t)num_cus * shade_grid_per_cu;
     int rc;
@@ -245,7 +245,6 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     auto t0 = std::chrono::steady_clock::now();
     HostScene& H = sc->host;
     if ((rc = upload_vec(sc->nodes, H.nodes, this, stream))) return rc;
-    if ((rc = upload_vec(sc->leaves, H.leaves, this, stream))) return rc;
     if ((rc = upload_vec(sc->tris, H.tris, this, stream))) return rc;
     if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
     if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
@@ -260,7 +259,7 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     if ((rc = upload_vec(sc->light_quads, H.light_quads, this, stream))) return rc;
     crt_dev_scene& ds = sc->dev;
     std::memset(&ds, 0, sizeof(ds));
-    ds.nodes = sc->nodes.p; ds.leaves = sc->leaves.p; ds.tris = sc->tris.p; ds.tri_normals = sc->tri_normals.p;
+    ds.nodes = sc->nodes.p; ds.tris = sc->tris.p; ds.tri_normals = sc->tri_normals.p;
     ds.tri_uvs = sc->tri_uvs.p; ds.spheres = sc->spheres.p; ds.sphere_aux = sc->sphere_aux.p; ds.sphere_map = sc->sphere_map.p;
     ds.quads = sc->quads.p; ds.insts = sc->insts.p; ds.mats = sc->mats.p; ds.texs = sc->texs.p;
     ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;
@@ -679,6 +678,7 @@ int crt_scene_get_info(crt_context* c, crt_scene_id s, crt_scene_info* info) {
     info->num_instances = sc->host.num_instances;
     info->bvh_width = sc->host.bvh_width;
     info->node_bytes = sc->host.bvh_width == 4 ? 128u : 64u;
+    info->leaf_size = sc->host.leaf_size;
     info->build_ms = sc->host.stats.build_ms;
     info->upload_ms = sc->upload_ms;
     info->sah_cost = sc->host.stats.sah_cost;
@@ -806,7 +806,7 @@ int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, uint32_t 
     HostScene H;
     std::string err;
     unsigned hw = std::thread::hardware_concurrency();
-    int rc = build_host_scene(d, leaf_size ? leaf_size : 4u, bvh_width ? bvh_width : 4u, (int)std::min(16u, hw ? hw : 1u),
+    int rc = build_host_scene(d, leaf_size ? leaf_size : kDefaultLeafSize, bvh_width ? bvh_width : kDefaultBvhWidth, (int)std::min(16u, hw ? hw : 1u),
                               H, err);
     if (rc) return set_error(rc, err);
     rc = check_host_scene(H, err);
@@ -814,7 +814,7 @@ int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, uint32_t 
     if (info) {
         info->num_nodes = H.stats.nodes; info->num_leaves = H.stats.leaves; info->num_prims = H.stats.prims;
         info->device_bytes = H.device_bytes(); info->max_depth = H.stats.depth; info->num_instances = H.num_instances;
-        info->bvh_width = H.bvh_width; info->node_bytes = H.bvh_width == 4 ? 128u : 64u;
+        info->bvh_width = H.bvh_width; info->node_bytes = H.bvh_width == 4 ? 128u : 64u; info->leaf_size = H.leaf_size;
         info->build_ms = H.stats.build_ms; info->upload_ms = 0.0; info->sah_cost = H.stats.sah_cost;
     }
     return CRT_OK;

@@ -7,8 +7,10 @@
 //             n0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
 //             n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
 //             n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
-//             n3 = (child0, child1, -, -) as int: >= 0 interior node, < 0 leaf ~index into leaves
-//   leaves  : uint2 (first, type << 28 | count) into the per-type leaf-ordered primitive arrays
+//             n3 = (child0, child1, -, -) as int: >= 0 interior node, < 0 inline leaf reference
+//             (CRT_LEAF_REF below: type, count and first record of the leaf's primitives)
+//           or BVH4, 8 x float4 (128 B) per interior node, one float4 per axis bound over the four
+//             children: lo.x hi.x lo.y hi.y lo.z hi.z, then (child0..child3) as int, then padding
 //   tris    : 3 x float4 per triangle in leaf order: (v0, gid) (v1, material) (v2, flags)
 //   spheres : float4 (centre, radius) per sphere in leaf order + uint4 aux (gid, material, flags, -)
 //   quads   : 5 x float4 per quad/area light: (Q, D) (u, flags) (v, area) (n, gid) (w, material)
@@ -23,8 +25,15 @@
 #define CRT_PRIM_QUAD 2u
 #define CRT_PRIM_INST 3u
 
-#define CRT_LEAF_TYPE_SHIFT 28u
-#define CRT_LEAF_COUNT_MASK 0x0fffffffu
+// Child references in interior nodes: >= 0 an interior node; < 0 a leaf, written inline so the
+// traversal reads the leaf's primitive records without an indirection:
+//   bit 31 | type << 29 | (count - 1) << 26 | first   (first + count <= 2^26, 1 <= count <= 8)
+#define CRT_LEAF_MAX_COUNT 8u
+#define CRT_LEAF_MAX_FIRST (1u << 26)
+#define CRT_LEAF_REF(type, first, count) ((int32_t)(0x80000000u | ((type) << 29) | (((count)-1u) << 26) | (first)))
+#define CRT_LEAF_REF_TYPE(r) (((uint32_t)(r) >> 29) & 3u)
+#define CRT_LEAF_REF_COUNT(r) ((((uint32_t)(r) >> 26) & 7u) + 1u)
+#define CRT_LEAF_REF_FIRST(r) ((uint32_t)(r)&0x03ffffffu)
 
 // quad flags (float4 #1 .w as uint)
 #define CRT_QF_LIGHT 1u        // centred parametrisation, interior in [-0.5, 0.5]^2, uv shifted by 0.5
@@ -51,7 +60,6 @@
 //       SOLID: -, CHECKER: (type, even, odd, -), IMAGE: (type, width, height, pool offset / 4 dwords)
 struct crt_dev_scene {
     const void* nodes;      // float4
-    const void* leaves;     // uint2
     const void* tris;       // float4
     const void* tri_normals;// float4 x3 per triangle (optional)
     const void* tri_uvs;    // float2 x3 per triangle (optional)

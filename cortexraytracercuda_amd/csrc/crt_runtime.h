@@ -35,6 +35,10 @@ void camera_setup(const crt_camera& c, crt_dev_request& r);
 
 struct Context;
 
+// BVH shape used when the caller passes 0 (measured fastest for the traversal kernels with leaf
+// references inline in the parent node: one primitive per leaf, four children per node)
+constexpr uint32_t kDefaultLeafSize = 1, kDefaultBvhWidth = 4;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -48,7 +52,7 @@ struct DevBuf {
 
 struct DeviceScene {
     HostScene host;  // geometry arrays are dropped after upload; stats and small tables stay
-    DevBuf nodes, leaves, tris, tri_normals, tri_uvs, spheres, sphere_aux, sphere_map, quads, insts, mats, texs, texels,
+    DevBuf nodes, tris, tri_normals, tri_uvs, spheres, sphere_aux, sphere_map, quads, insts, mats, texs, texels,
         light_quads;
     std::vector<uint32_t> tri_gid, quad_gid;   // leaf order -> global primitive id
     std::vector<uint32_t> inst_gid;            // instance record (leaf order) -> caller's instance index
@@ -146,12 +150,12 @@ struct Context {
     int num_cus = 256;
     hipStream_t stream = nullptr;
     uint64_t budget = 0, reserved = 0, scene_bytes = 0, request_bytes = 0;
-    uint32_t max_paths = 0, path_capacity = 0, leaf_size = 4, bvh_width = 4, extend_grid = 2048, shade_grid = 2048;
+    uint32_t max_paths = 0, path_capacity = 0, leaf_size = kDefaultLeafSize, bvh_width = kDefaultBvhWidth, extend_grid = 2048, shade_grid = 2048;
     int extend_mode = CRT_EXTEND_GRID_STRIDE;   // closest-hit work distribution (CRT_EXTEND_*)
     uint32_t extend_param = 2;       // lane-refill threshold or 64-ray groups per wave fetch
     uint32_t stack_lds = 16;         // LDS ring entries per lane of the traversal stack (8, 16, 32)
     uint32_t ray_sort = 0;           // shade groups each block's survivors by direction octant
-    uint32_t grid_per_cu = 32, shade_grid_per_cu = 8;
+    uint32_t grid_per_cu = 16, shade_grid_per_cu = 16;
     std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
     bool slot_used[kMaxScenes] = {};

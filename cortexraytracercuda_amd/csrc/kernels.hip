@@ -267,12 +267,12 @@ __device__ __forceinline__ bool quad_hit(V3 o, V3 d, const float4* q, float tmin
 struct HitRec { float t, u, v; uint32_t prim, inst; };
 
 struct SceneView {
-    const float4* nodes; const uint2* leaves; const float4* tris; const float4* spheres; const float4* quads;
+    const float4* nodes; const float4* tris; const float4* spheres; const float4* quads;
     const float4* insts; uint32_t root;
 };
 __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
     SceneView v;
-    v.nodes = (const float4*)sc->nodes; v.leaves = (const uint2*)sc->leaves; v.tris = (const float4*)sc->tris;
+    v.nodes = (const float4*)sc->nodes; v.tris = (const float4*)sc->tris;
     v.spheres = (const float4*)sc->spheres; v.quads = (const float4*)sc->quads; v.insts = (const float4*)sc->insts;
     v.root = sc->root;
     return v;
@@ -384,10 +384,10 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 }
             }
         } else {
-            uint2 lf = ldg(S.leaves + (uint32_t)(~node));
-            uint32_t type = lf.y >> CRT_LEAF_TYPE_SHIFT;
-            uint32_t cnt = lf.y & CRT_LEAF_COUNT_MASK;
-            uint32_t first = lf.x;
+            // leaf reference written inline in the parent: no leaf-record fetch
+            const uint32_t type = CRT_LEAF_REF_TYPE(node);
+            const uint32_t cnt = CRT_LEAF_REF_COUNT(node);
+            const uint32_t first = CRT_LEAF_REF_FIRST(node);
             if (COUNT) tc.leaves++;
             if (type == CRT_PRIM_TRI) {
                 for (uint32_t k = 0; k < cnt; ++k) {
@@ -657,8 +657,16 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
 
 // Closest-hit stage over the live path queue.
 // UNIFORM: every ray of the launch belongs to scene `uniform_scene` (scene pointers stay scalar).
+// Minimum resident blocks per CU the register allocation must allow. Blocks are 256 threads, one
+// wave on each of a CU's 4 SIMDs, so this is also waves per SIMD. The traversal is latency-bound and
+// occupancy pays: the single-scene one-level kernel (the common render) is held to 6 (<= 80 VGPRs,
+// no spills); the others take the registers they need.
+constexpr int extend_min_waves(bool count, bool uniform, bool two_level) {
+    return (!count && uniform && !two_level) ? 6 : 1;
+}
+
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
-__global__ __launch_bounds__(BLOCK) void k_extend(const crt_dev_scene* __restrict__ scenes,
+__global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL)) void k_extend(const crt_dev_scene* __restrict__ scenes,
                                                   const crt_dev_request* __restrict__ reqs, uint32_t uniform_scene,
                                                   const uint32_t* __restrict__ queue, const uint32_t* __restrict__ count_ptr,
                                                   uint32_t count_direct, const float4* __restrict__ ray_o,

@@ -60,7 +60,7 @@ struct BNode {
 class Builder {
 public:
     Builder(std::vector<Ref>& refs, uint32_t leaf_size, int threads)
-        : refs_(refs), leaf_size_(std::max(1u, std::min(leaf_size, 64u))), threads_(std::max(1, threads)) {
+        : refs_(refs), leaf_size_(std::max(1u, std::min(leaf_size, CRT_LEAF_MAX_COUNT))), threads_(std::max(1, threads)) {
         nodes_.resize(2 * refs.size() + 2);
         next_.store(0);
         active_.store(1);
@@ -201,6 +201,7 @@ struct Emitter {
     uint32_t max_depth = 0;
     double sah = 0.0;
     float root_area = 1.0f;
+    bool too_large = false;   // a leaf does not fit the inline reference (more than 2^26 records of a type)
 
     int32_t leaf_ref(const BNode& n) {
         uint32_t first = 0;
@@ -216,10 +217,13 @@ struct Emitter {
             else if (type == CRT_PRIM_QUAD) emit_quad(r.idx);
             else emit_inst(r.idx);
         }
-        uint32_t li = (uint32_t)S.leaves.size();
-        S.leaves.push_back(U2{first, (type << CRT_LEAF_TYPE_SHIFT) | n.count});
+        S.stats.leaves++;
         sah += (double)n.box.area() / root_area * (double)n.count;
-        return ~(int32_t)li;
+        if (first + n.count > CRT_LEAF_MAX_FIRST || n.count == 0 || n.count > CRT_LEAF_MAX_COUNT) {
+            too_large = true;
+            return CRT_LEAF_REF(type, 0u, 1u);
+        }
+        return CRT_LEAF_REF(type, first, n.count);
     }
 
     void write_node(uint32_t slot, const Box& b0, const Box& b1, int32_t c0, int32_t c1) {
@@ -302,9 +306,10 @@ struct Emitter {
             int32_t L;
             Box lb = z;
             if (root < 0) {
-                uint32_t li = (uint32_t)S.leaves.size();
-                S.leaves.push_back(U2{0u, (CRT_PRIM_SPHERE << CRT_LEAF_TYPE_SHIFT) | 0u});
-                L = ~(int32_t)li;
+                // empty tree: children no ray can hit (a box at +inf: on every axis the slab is
+                // either entered at +inf or left at -inf); BVH4 also marks them empty
+                for (int a = 0; a < 3; ++a) { lb.lo[a] = __builtin_inff(); lb.hi[a] = __builtin_inff(); }
+                L = CRT_EMPTY_CHILD;
             } else {
                 root_area = std::max(nodes[root].box.area(), 1e-30f);
                 L = leaf_ref(nodes[root]);
@@ -338,7 +343,7 @@ bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1])
 }  // namespace
 
 uint64_t HostScene::device_bytes() const {
-    return nodes.size() * 16 + leaves.size() * 8 + tris.size() * 16 + tri_normals.size() * 16 + tri_uvs.size() * 8 +
+    return nodes.size() * 16 + tris.size() * 16 + tri_normals.size() * 16 + tri_uvs.size() * 8 +
            spheres.size() * 16 + sphere_aux.size() * 16 + sphere_map.size() * 8 + quads.size() * 16 + insts.size() * 16 +
            mats.size() * 16 + texs.size() * 16 + texels.size() + light_quads.size() * 4;
 }
@@ -359,6 +364,8 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     S = HostScene();
     if (bvh_width != 2 && bvh_width != 4) { err = "bvh width must be 2 or 4"; return CRT_ERR_INVALID_ARGUMENT; }
     S.bvh_width = bvh_width;
+    leaf_size = std::max(1u, std::min(leaf_size, CRT_LEAF_MAX_COUNT));   // leaf references hold 1-8 primitives
+    S.leaf_size = leaf_size;
     S.camera = d->camera;
     for (int k = 0; k < 3; ++k) S.background[k] = d->background[k];
     const uint32_t NM = d->num_materials;
@@ -545,7 +552,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
                 brefs.push_back(make_ref(b, CRT_PRIM_TRI, t));
             }
             blas_box[m] = mb;
-            Builder bb(brefs, leaf_size ? leaf_size : 4, threads);
+            Builder bb(brefs, leaf_size, threads);
             int32_t r = bb.build();
             Emitter em{S, bb.nodes(), brefs, nullptr, nullptr, nullptr, nullptr};
             em.width = bvh_width;
@@ -575,7 +582,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     }
 
     // ---------------------------------------------------------------- top level
-    Builder tb(refs, leaf_size ? leaf_size : 4, threads);
+    Builder tb(refs, leaf_size, threads);
     int32_t troot = tb.build();
     Emitter em{S, tb.nodes(), refs, nullptr, nullptr, nullptr, nullptr};
     em.width = bvh_width;
@@ -625,6 +632,10 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
         S.insts.push_back(F4{fb(blas_root[I.mesh]), fb(k), fb((uint32_t)I.material_override), 0.0f});
     };
     S.root = em.emit_root(troot);
+    if (em.too_large) {
+        err = "more than 2^26 primitive records of one type";
+        return CRT_ERR_INVALID_SCENE;
+    }
     S.stats.sah_cost = em.sah;
     S.stats.depth = em.max_depth + blas_depth;
     // per-instance material overrides are resolved in the shade stage (instance record word z)
@@ -636,7 +647,6 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     if (!S.tri_normals.empty() && S.tri_normals.size() < S.tris.size()) S.tri_normals.resize(S.tris.size(), F4{0, 0, 0, 0});
     if (!S.tri_uvs.empty() && S.tri_uvs.size() < S.tris.size()) S.tri_uvs.resize(S.tris.size(), F2{0, 0});
     S.stats.nodes = S.nodes.size() / (bvh_width == 4 ? 8 : 4);
-    S.stats.leaves = S.leaves.size();
     S.stats.prims = refs.size();
     S.stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return CRT_OK;
@@ -692,14 +702,12 @@ int check_host_scene(const HostScene& S, std::string& err) {
             B b0{{l0[0], l0[1], l0[2]}, {h0[0], h0[1], h0[2]}}, b1{{l1[0], l1[1], l1[2]}, {h1[0], h1[1], h1[2]}};
             if (lo && (!inside(b0, lo, hi) || !inside(b1, lo, hi))) { err = "child box outside parent box"; return false; }
             int32_t c0 = (int32_t)u32(np[3].x), c1 = (int32_t)u32(np[3].y);
+            if (c0 == CRT_EMPTY_CHILD && c1 == CRT_EMPTY_CHILD) return true;   // empty tree (boxes no ray hits)
             if (!walk(c0, l0, h0, depth + 1)) return false;
             if (c1 == c0 && c0 < 0) return true;  // a wrapped leaf root lists its leaf twice
             return walk(c1, l1, h1, depth + 1);
         }
-        uint32_t li = (uint32_t)(~ref);
-        if (li >= S.leaves.size()) { err = "leaf index out of range"; return false; }
-        U2 lf = S.leaves[li];
-        uint32_t type = lf.y >> CRT_LEAF_TYPE_SHIFT, cnt = lf.y & CRT_LEAF_COUNT_MASK, first = lf.x;
+        const uint32_t type = CRT_LEAF_REF_TYPE(ref), cnt = CRT_LEAF_REF_COUNT(ref), first = CRT_LEAF_REF_FIRST(ref);
         for (uint32_t k = first; k < first + cnt; ++k) {
             B pb;
             if (type == CRT_PRIM_TRI) {

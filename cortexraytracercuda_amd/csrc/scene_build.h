@@ -28,7 +28,6 @@ struct BuildStats {
 
 struct HostScene {
     std::vector<F4> nodes;        // 4 per interior node
-    std::vector<U2> leaves;
     std::vector<F4> tris;         // 3 per triangle
     std::vector<F4> tri_normals;  // 3 per triangle, only when a mesh has normals
     std::vector<F2> tri_uvs;      // 3 per triangle, only when a mesh has uvs
@@ -43,6 +42,7 @@ struct HostScene {
     std::vector<uint32_t> light_quads;
     uint32_t root = 0;
     uint32_t bvh_width = 4;       // 2: 64-B binary nodes, 4: 128-B four-wide SoA nodes
+    uint32_t leaf_size = 1;       // max primitives per leaf the tree was built with
     float background[3] = {0, 0, 0};
     crt_camera camera{};
     uint32_t num_lights = 0, num_spheres = 0, num_quads = 0, num_instances = 0;

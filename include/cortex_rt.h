@@ -225,10 +225,10 @@ typedef struct crt_config {
     int32_t device;             /* HIP device ordinal */
     uint64_t hbm_budget_bytes;  /* admission budget for scenes + requests; 0 = 90% of free memory */
     uint32_t max_paths;         /* live path slots of the streaming passes (path state pools, 164 B each);
-                                   0 = 16M */
+                                   0 = 32M */
     uint32_t block_size;        /* threads per block of the stage kernels; 0 = default (256) */
     uint32_t count_work;        /* 1: use the counting traversal kernel (fills node_visits/prim_tests) */
-    uint32_t leaf_size;         /* BVH max primitives per leaf; 0 = default (2) */
+    uint32_t leaf_size;         /* BVH max primitives per leaf (1-8); 0 = default (1) */
     uint32_t bvh_width;         /* children per BVH node, 2 or 4; 0 = default (4) */
     uint32_t extend_kernel;     /* closest-hit stage (CRT_EXTEND_*): 0 = default, 1 = grid-stride (each lane
                                    traces its rays to the end, the wave waits for its slowest lane),
@@ -237,8 +237,8 @@ typedef struct crt_config {
     uint32_t refill_min;        /* mode 2: idle lanes of a wave before it fetches rays; mode 3: 64-ray groups
                                    per fetch; 0 = default */
     uint32_t pass_paths;        /* path ids per pass (16 B of radiance each); 0 = min(2^27, 16 * max_paths) */
-    uint32_t grid_per_cu;       /* workgroups per CU of the closest-hit launches (256 threads each); 0 = 32 */
-    uint32_t shade_grid_per_cu; /* workgroups per CU of the regenerate / shade launches; 0 = 8 */
+    uint32_t grid_per_cu;       /* workgroups per CU of the closest-hit launches (256 threads each); 0 = 16 */
+    uint32_t shade_grid_per_cu; /* workgroups per CU of the regenerate / shade launches; 0 = 16 */
     uint32_t sub_passes;        /* concurrent sub-passes (one stream each) of a pass, 1-4; 0 = 2 */
     uint32_t graphs;            /* serving loop iterations replayed from hipGraphs: 0 = default (on), 1 = on,
                                    2 = off (every iteration launched eagerly) */
@@ -263,6 +263,7 @@ typedef struct crt_scene_info {
     uint64_t num_nodes, num_leaves, num_prims, device_bytes;
     uint32_t max_depth, num_instances;
     uint32_t bvh_width, node_bytes;          /* children per node and bytes per node record */
+    uint32_t leaf_size, reserved;            /* max primitives per leaf the BVH was built with; - */
     double build_ms, upload_ms, sah_cost;
 } crt_scene_info;
 int crt_scene_get_info(crt_context* ctx, crt_scene_id scene, crt_scene_info* info);

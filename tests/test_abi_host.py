@@ -61,6 +61,19 @@ def test_bvh_invariants_leaf_sizes(leaf, width):
     assert rc == 0, A.load().crt_last_error()
 
 
+def test_leaf_size_reported_and_bounded():
+    """Leaf references carry 1-8 primitives: 0 selects the default (1), larger sizes are capped at 8.
+    With one primitive per leaf every primitive has its own leaf."""
+    s = scenes.large_scene(levels=3, n_spheres=100)
+    for asked, built in ((0, 1), (1, 1), (3, 3), (8, 8), (20, 8)):
+        rc, info = _check(s, asked, 4)
+        assert rc == 0, A.load().crt_last_error()
+        assert info["leaf_size"] == built
+        assert info["num_prims"] / built <= info["num_leaves"] <= info["num_prims"]
+    rc, info = _check(s, 1, 4)
+    assert info["num_leaves"] == info["num_prims"]
+
+
 @pytest.mark.parametrize("width", [2, 4])
 def test_bvh_invariants_small_and_instanced(width):
     for s in (scenes.box_scene(), Scene(), scenes.instanced_scene(n_instances=20, n_textures=1, tex_size=8, nu=6, nv=4)):
