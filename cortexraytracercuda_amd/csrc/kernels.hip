@@ -732,22 +732,28 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
     wave_add(&counters->stack_overflow, novf);
 }
 
-// Persistent closest-hit stage with per-lane refill: a lane whose ray is finished takes the next
-// ray of the queue at once, so a wave keeps its lanes busy until the queue drains instead of
-// waiting for its slowest ray. Rays are handed out through `work_counter` (zero at launch), one
-// atomicAdd per wave and refill; a refill is made when at least `refill_min` lanes are idle (or
-// all are). Every wave leaves once the counter has passed the queue length and its lanes are idle;
-// each traversal ends (every node is visited at most once per query, at most one re-trace).
+// Closest-hit stage with per-lane refill: every wave owns a contiguous share of the queue, and a
+// lane whose ray is finished takes the wave's next ray instead of waiting for the slowest lane of
+// the wave. A refill is made when at least `refill_min` lanes are idle (or all are): the idle lanes
+// take the next rays of the share in lane order (mbcnt rank), so refilled rays are neighbours of
+// each other and of the rays in flight. The share pointer is wave-uniform and private: no atomics.
+// A wave leaves when its share is handed out and its lanes are idle; each traversal ends (every
+// node is visited at most once per query, at most one re-trace). Pays when rays of one wave differ
+// much in length (many primitive tests per ray, little coherence); costs coherence on primary rays.
+constexpr int extend_pt_min_waves(bool count, bool uniform, bool two_level, int width) {
+    return (!count && uniform && !two_level && width == 4) ? 5 : 1;   // 5: <= 96 VGPRs, no spills
+}
+
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
-__global__ __launch_bounds__(BLOCK) void k_extend_pt(const crt_dev_scene* __restrict__ scenes,
+__global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVEL, WIDTH)) void k_extend_pt(const crt_dev_scene* __restrict__ scenes,
                                                      const crt_dev_request* __restrict__ reqs, uint32_t uniform_scene,
                                                      const uint32_t* __restrict__ queue,
                                                      const uint32_t* __restrict__ count_ptr, uint32_t count_direct,
                                                      const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
                                                      const uint2* __restrict__ ids, float4* __restrict__ hit,
                                                      uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
-                                                     uint32_t* __restrict__ zero_counter, uint32_t* __restrict__ work_counter,
-                                                     uint32_t refill_min, crt_dev_counters* __restrict__ counters) {
+                                                     uint32_t* __restrict__ zero_counter, uint32_t refill_min,
+                                                     crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
     const uint32_t n = count_ptr ? *count_ptr : count_direct;
@@ -761,20 +767,25 @@ __global__ __launch_bounds__(BLOCK) void k_extend_pt(const crt_dev_scene* __rest
     Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
     bool active = false, exhausted = false;
     uint32_t slot = 0, pass_flag = 0;
+    // this wave's share [nxt, wend) of the queue
+    const uint32_t nwaves = gridDim.x * (BLOCK / 64);
+    const uint32_t wid = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    const uint32_t share = (n + nwaves - 1) / nwaves;
+    uint32_t nxt = min(wid * share, n);
+    const uint32_t wend = min(nxt + share, n);
+    if (nxt >= wend) exhausted = true;
     for (;;) {
         if (!exhausted) {
             uint64_t idle = __ballot(!active);
             uint32_t cnt = (uint32_t)__popcll(idle);
             if (cnt >= refill_min || cnt == 64u) {
-                uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(work_counter, cnt);
-                base = __shfl(base, (int)leader);
-                if (base + cnt >= n) exhausted = true;
+                const uint32_t base = nxt;
+                nxt += cnt;
+                if (nxt >= wend) exhausted = true;
                 if (!active) {
                     uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                    if (i < n) {
+                    if (i < wend) {
                         slot = queue ? queue[i] : i;
                         float4 ro = ray_o[slot], rdv = ray_d[slot];
                         if (!UNIFORM) S = scene_view(scenes + reqs[ids[slot].y >> 16].scene);
@@ -1371,8 +1382,8 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                             uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
                             crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
-    if (mode != CRT_EXTEND_PERSISTENT && mode != CRT_EXTEND_WAVE_CHUNK) work_counter = nullptr;
-    if (!work_counter) mode = CRT_EXTEND_GRID_STRIDE;
+    if (mode != CRT_EXTEND_WAVE_CHUNK) work_counter = nullptr;
+    if (mode == CRT_EXTEND_WAVE_CHUNK && !work_counter) mode = CRT_EXTEND_GRID_STRIDE;
     // grid-stride / wave-chunk kernel with an LDS stack ring of LSV entries per lane
 #define CRT_EXT_GS(LSV, C, U, W, T)                                                                                    \
     {                                                                                                                   \
@@ -1384,15 +1395,12 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                            ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param ? param : 1u,         \
                            counters);                                                                                   \
     }
-    // persistent lane-refill kernel (launches at most the resident blocks: it fetches work until the queue drains)
+    // lane-refill kernel over per-wave shares of the queue (same grid as the grid-stride kernel)
 #define CRT_EXT_PT(C, U, W, T)                                                                                         \
     {                                                                                                                   \
         auto kfn = k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                               \
-        static uint32_t rb = 0;                                                                                         \
-        if (!rb) rb = resident_blocks(kfn, 256);                                                                        \
-        hipLaunchKernelGGL(kfn, dim3(grid < rb ? grid : rb), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr,      \
-                           count_direct, ray_o, ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param,  \
-                           counters);                                                                                   \
+        hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr, count_direct, ray_o,  \
+                           ray_d, ids, hit, hit_inst, gstack, zero_counter, param ? param : 1u, counters);             \
     }
     // counting variants: one stack depth; timed variants: LDS ring of 8, 16 or 32 entries per lane
 #define CRT_EXT_C(U, W, T) \

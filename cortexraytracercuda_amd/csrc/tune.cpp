@@ -157,12 +157,14 @@ int Context::autotune(crt_scene_id scene, const crt_scene_desc* desc, const crt_
     if ((rc = sweep([](Shape& s, uint32_t v) { s.grid = v; }, {16u, 32u, 64u}))) return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.sgrid = v; }, {4u, 8u, 16u}))) return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.sub = v; }, {1u, 2u, 3u, 4u}))) return rc;
+    // closest-hit work distribution: grid-stride (coherent waves) or per-lane refill from the
+    // wave's share of the queue after 16 / 32 idle lanes (fewer idle lanes on long, uneven rays)
     if ((rc = sweep(
              [](Shape& s, uint32_t v) {
-                 s.mode = v == 0 ? (uint32_t)CRT_EXTEND_GRID_STRIDE : (uint32_t)CRT_EXTEND_WAVE_CHUNK;
+                 s.mode = v == 0 ? (uint32_t)CRT_EXTEND_GRID_STRIDE : (uint32_t)CRT_EXTEND_PERSISTENT;
                  s.param = v == 0 ? 2u : v;
              },
-             {0u, 1u, 2u})))
+             {0u, 16u, 32u})))
         return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.lds = v; }, {8u, 16u, 32u}))) return rc;
 

@@ -232,10 +232,11 @@ typedef struct crt_config {
     uint32_t bvh_width;         /* children per BVH node, 2 or 4; 0 = default (4) */
     uint32_t extend_kernel;     /* closest-hit stage (CRT_EXTEND_*): 0 = default, 1 = grid-stride (each lane
                                    traces its rays to the end, the wave waits for its slowest lane),
-                                   2 = persistent with per-lane refill from a work counter,
+                                   2 = per-lane refill: a lane whose ray is done takes the next ray of its
+                                       wave's contiguous share of the queue (no atomics),
                                    3 = persistent waves taking chunks of 64-ray groups from a work counter */
-    uint32_t refill_min;        /* mode 2: idle lanes of a wave before it fetches rays; mode 3: 64-ray groups
-                                   per fetch; 0 = default */
+    uint32_t refill_min;        /* mode 2: idle lanes of a wave before they take new rays (default 16);
+                                   mode 3: 64-ray groups per fetch; 0 = default */
     uint32_t pass_paths;        /* path ids per pass (16 B of radiance each); 0 = min(2^27, 16 * max_paths) */
     uint32_t grid_per_cu;       /* workgroups per CU of the closest-hit launches (256 threads each); 0 = 16 */
     uint32_t shade_grid_per_cu; /* workgroups per CU of the regenerate / shade launches; 0 = 16 */
