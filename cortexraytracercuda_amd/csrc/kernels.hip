@@ -118,6 +118,13 @@ __device__ __forceinline__ float4 ldg(const float4* p) {
     crt_v4f v = *(__attribute__((address_space(1))) const crt_v4f*)p;
     return make_float4(v.x, v.y, v.z, v.w);
 }
+// Load at base + a 32-bit byte offset: with a wave-uniform base the address is one scalar pair
+// plus one VGPR per load (the saddr form), not a 64-bit VGPR address per load.
+__device__ __forceinline__ float4 ldg_at(const float4* base, uint32_t byte_off) {
+    typedef __attribute__((address_space(1))) const char gchar;
+    crt_v4f v = *(__attribute__((address_space(1))) const crt_v4f*)((gchar*)base + byte_off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ int4 ldg_i4(const float4* p) {
     crt_v4i v = *(__attribute__((address_space(1))) const crt_v4i*)p;
     return make_int4(v.x, v.y, v.z, v.w);
@@ -131,20 +138,24 @@ __device__ __forceinline__ uint2 ldg(const uint2* p) {
 // Short per-lane stack: RS entries in VGPRs (static shifts, no dynamic register indexing), spilling
 // to an LS-entry ring per lane in LDS, which in turn spills its oldest entries to a per-thread
 // overflow area in global memory. Lower levels are used only while the registers are full.
+// Overflow area of one thread: the launch-wide base (kernel argument, scalar) and the thread's
+// offset in entries (one VGPR instead of a 64-bit per-lane pointer).
+struct GSlot { uint32_t* base; uint32_t off; };
+
 template <int BLOCK, int RS, int LS, int GS> struct Stack {
     uint32_t r[RS];
     int rc, lbase, lcnt, gsp;
     uint32_t* lds;     // column of this lane: lds[e * BLOCK]
-    uint32_t* gmem;    // GS entries of this thread
+    GSlot gm;          // GS entries of this thread: gm.base[gm.off + e]
     bool overflow;
-    __device__ __forceinline__ void init(uint32_t* lds_col, uint32_t* g) {
-        rc = 0; lbase = 0; lcnt = 0; gsp = 0; lds = lds_col; gmem = g; overflow = false;
+    __device__ __forceinline__ void init(uint32_t* lds_col, GSlot g) {
+        rc = 0; lbase = 0; lcnt = 0; gsp = 0; lds = lds_col; gm = g; overflow = false;
     }
     __device__ __forceinline__ int depth() const { return rc + lcnt + gsp; }
     __device__ __forceinline__ bool empty() const { return rc == 0; }
     __device__ __forceinline__ void spill(uint32_t v) {
         if (lcnt == LS) {
-            if (gsp < GS) gmem[gsp++] = lds[lbase * BLOCK]; else overflow = true;
+            if (gsp < GS) gm.base[gm.off + gsp++] = lds[lbase * BLOCK]; else overflow = true;
             lbase = (lbase + 1) & (LS - 1);
             --lcnt;
         }
@@ -153,7 +164,7 @@ template <int BLOCK, int RS, int LS, int GS> struct Stack {
     }
     __device__ __forceinline__ uint32_t refill() {
         if (lcnt > 0) { --lcnt; return lds[((lbase + lcnt) & (LS - 1)) * BLOCK]; }
-        return gmem[--gsp];
+        return gm.base[gm.off + --gsp];
     }
     __device__ __forceinline__ void push(uint32_t v) {
         if (rc == RS) spill(r[RS - 1]); else ++rc;
@@ -171,7 +182,7 @@ template <int BLOCK, int RS, int LS, int GS> struct Stack {
 };
 
 // ------------------------------------------------------------------ primitive tests
-struct TriPre { int kx, ky, kz; float Sx, Sy, Sz; };
+struct TriPre { uint32_t k; float Sx, Sy, Sz; };   // k = kx | ky << 2 | kz << 4
 __device__ __forceinline__ TriPre tri_pre(V3 d) {
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
@@ -180,7 +191,7 @@ __device__ __forceinline__ TriPre tri_pre(V3 d) {
     float dz = comp(d, kz);
     if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
     TriPre p;
-    p.kx = kx; p.ky = ky; p.kz = kz;
+    p.k = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
     p.Sx = comp(d, kx) / dz; p.Sy = comp(d, ky) / dz; p.Sz = 1.0f / dz;
     return p;
 }
@@ -191,11 +202,12 @@ __device__ __forceinline__ TriPre tri_pre(V3 d) {
 __device__ __forceinline__ bool tri_hit(V3 o, const TriPre& p, float4 a, float4 b, float4 c, float tmin, float tmax,
                                         float& t, float& bu, float& bv) {
 #pragma clang fp contract(off)
+    const int kx = (int)(p.k & 3u), ky = (int)((p.k >> 2) & 3u), kz = (int)(p.k >> 4);
     V3 A = v3(a) - o, B = v3(b) - o, C = v3(c) - o;
-    float Akz = comp(A, p.kz), Bkz = comp(B, p.kz), Ckz = comp(C, p.kz);
-    float Ax = comp(A, p.kx) - p.Sx * Akz, Ay = comp(A, p.ky) - p.Sy * Akz;
-    float Bx = comp(B, p.kx) - p.Sx * Bkz, By = comp(B, p.ky) - p.Sy * Bkz;
-    float Cx = comp(C, p.kx) - p.Sx * Ckz, Cy = comp(C, p.ky) - p.Sy * Ckz;
+    float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
+    float Ax = comp(A, kx) - p.Sx * Akz, Ay = comp(A, ky) - p.Sy * Akz;
+    float Bx = comp(B, kx) - p.Sx * Bkz, By = comp(B, ky) - p.Sy * Bkz;
+    float Cx = comp(C, kx) - p.Sx * Ckz, Cy = comp(C, ky) - p.Sy * Ckz;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
@@ -292,6 +304,17 @@ __device__ __forceinline__ float box_near(float lx, float hx, float ly, float hy
     return nr;
 }
 
+// The same slab test with the near and far plane of each axis chosen by the sign of the inverse
+// direction (per ray): (near - o) * inv_d <= (far - o) * inv_d holds after rounding, so the
+// per-axis min / max of the two-plane form are not needed and the result is bit-identical.
+__device__ __forceinline__ float box_near_oct(float nx, float fx, float ny, float fy, float nz, float fz, V3 o, V3 id,
+                                              float tmin, float tmax, bool& hit) {
+    float nr = fmaxf(fmaxf((nx - o.x) * id.x, (ny - o.y) * id.y), fmaxf((nz - o.z) * id.z, tmin));
+    float fr = fminf(fminf((fx - o.x) * id.x, (fy - o.y) * id.y), fminf((fz - o.z) * id.z, tmax));
+    hit = nr <= fr * 1.0000004f;
+    return nr;
+}
+
 __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t& vb) {
     bool s = kb < ka;
     float k = s ? kb : ka; kb = s ? ka : kb; ka = k;
@@ -305,9 +328,10 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t
 // move the ray to object space and descend into the instance's BLAS. The traversal is a resumable
 // state machine: step() performs one node or leaf visit, so a persistent kernel can interleave
 // traversal steps with fetching new rays for finished lanes.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL> struct Traverser {
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool OCT = true> struct Traverser {
     Stack<BLOCK, RS, LS, GS> st;
     V3 o, d, id, ow, dw;
+    uint32_t oct;          // bit a set where inv_d[a] < 0: the hi plane of axis a is the near one
     TriPre tp;
     HitRec best;
     uint32_t cur_inst;
@@ -322,10 +346,11 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     __device__ __forceinline__ void set_ray(V3 ro, V3 rd) {
         o = ro; d = rd;
         id = v3(inv_of(d.x), inv_of(d.y), inv_of(d.z));
+        oct = (id.x < 0.0f ? 1u : 0u) | (id.y < 0.0f ? 2u : 0u) | (id.z < 0.0f ? 4u : 0u);
         tp = tri_pre(d);
     }
     __device__ __forceinline__ void init(const SceneView& S, V3 ro, V3 rd, float t0, float t1, bool cam,
-                                         uint32_t* lds_col, uint32_t* gstk) {
+                                         uint32_t* lds_col, GSlot gstk) {
         st.init(lds_col, gstk);
         ow = ro; dw = rd;
         set_ray(ro, rd);
@@ -357,26 +382,49 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 } else if (hit0) { node = n3.x; next = false; }
                 else if (hit1) { node = n3.y; next = false; }
             } else {
-                const float4* np = S.nodes + 8 * (uint32_t)node;
-                float4 lx = ldg(np), hx = ldg(np + 1), ly = ldg(np + 2), hy = ldg(np + 3), lz = ldg(np + 4),
-                       hz = ldg(np + 5);
-                int4 rf = ldg_i4(np + 6);
+                // planes lo.x hi.x lo.y hi.y lo.z hi.z (four children each) and the child refs; empty
+                // child slots have boxes at +inf, which no ray hits
+                const uint32_t nb = (uint32_t)node * 128u;   // node byte offset (scenes stay below 4 GiB of nodes)
                 bool h0, h1, h2, h3;
-                float k0 = box_near(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, id, tmin, tf, h0);
-                float k1 = box_near(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, id, tmin, tf, h1);
-                float k2 = box_near(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, id, tmin, tf, h2);
-                float k3 = box_near(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, id, tmin, tf, h3);
+                float k0, k1, k2, k3;
+                int4 rf;
+                if (OCT) {
+                    // near / far plane of each axis picked by the ray's octant: fewer VALU, six
+                    // load addresses instead of one
+                    const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
+                    float4 nx = ldg_at(S.nodes, nb + sx), fx = ldg_at(S.nodes, nb + (16u - sx)),
+                           ny = ldg_at(S.nodes, nb + 32u + sy), fy = ldg_at(S.nodes, nb + (48u - sy)),
+                           nz = ldg_at(S.nodes, nb + 64u + sz), fz = ldg_at(S.nodes, nb + (80u - sz));
+                    float4 rff = ldg_at(S.nodes, nb + 96u);
+                    rf = make_int4((int)fbits(rff.x), (int)fbits(rff.y), (int)fbits(rff.z), (int)fbits(rff.w));
+                    k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, id, tmin, tf, h0);
+                    k1 = box_near_oct(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, id, tmin, tf, h1);
+                    k2 = box_near_oct(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, id, tmin, tf, h2);
+                    k3 = box_near_oct(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, id, tmin, tf, h3);
+                } else {
+                    const float4* np = S.nodes + 8 * (uint32_t)node;
+                    float4 lx = ldg(np), hx = ldg(np + 1), ly = ldg(np + 2), hy = ldg(np + 3), lz = ldg(np + 4),
+                           hz = ldg(np + 5);
+                    rf = ldg_i4(np + 6);
+                    k0 = box_near(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, id, tmin, tf, h0);
+                    k1 = box_near(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, id, tmin, tf, h1);
+                    k2 = box_near(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, id, tmin, tf, h2);
+                    k3 = box_near(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, id, tmin, tf, h3);
+                }
                 const float INF = __builtin_inff();
-                k0 = (h0 && rf.x != CRT_EMPTY_CHILD) ? k0 : INF;
-                k1 = (h1 && rf.y != CRT_EMPTY_CHILD) ? k1 : INF;
-                k2 = (h2 && rf.z != CRT_EMPTY_CHILD) ? k2 : INF;
-                k3 = (h3 && rf.w != CRT_EMPTY_CHILD) ? k3 : INF;
+                k0 = h0 ? k0 : INF;
+                k1 = h1 ? k1 : INF;
+                k2 = h2 ? k2 : INF;
+                k3 = h3 ? k3 : INF;
                 int32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
                 int nh = (k0 < INF) + (k1 < INF) + (k2 < INF) + (k3 < INF);
+                // the sort runs whatever nh is (a wave pays for it when any lane has a hit): the child
+                // refs are then used on the common path and their load is issued with the planes,
+                // not sunk behind the box tests into the branch
+                cswap(k0, c0, k1, c1); cswap(k2, c2, k3, c3);
+                cswap(k0, c0, k2, c2); cswap(k1, c1, k3, c3);
+                cswap(k1, c1, k2, c2);
                 if (nh > 0) {
-                    cswap(k0, c0, k1, c1); cswap(k2, c2, k3, c3);
-                    cswap(k0, c0, k2, c2); cswap(k1, c1, k3, c3);
-                    cswap(k1, c1, k2, c2);
                     if (nh > 3) st.push((uint32_t)c3);
                     if (nh > 2) st.push((uint32_t)c2);
                     if (nh > 1) st.push((uint32_t)c1);
@@ -448,10 +496,10 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     }
 };
 
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool OCT = true>
 __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
-                                      uint32_t* lds_col, uint32_t* gstk, HitRec& best, TravCount& tc, bool& ovf) {
-    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
+                                      uint32_t* lds_col, GSlot gstk, HitRec& best, TravCount& tc, bool& ovf) {
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, OCT> tr;
     tr.init(S, ow, dw, tmin, tmax, camera, lds_col, gstk);
     while (!tr.step(S, tc)) {
     }
@@ -633,15 +681,16 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
 // Closest hit of one path segment. One-sided emitters hit from behind are re-traced from
 // t + 0.001 (core/src/cameras/camera.cuh:179-183); the result is then flagged as a pass-through
 // (returned flag CRT_PASSTHROUGH). `nrays` counts closest-hit queries, `nretrace` the re-traces.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool OCT = true>
 __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
-                                                uint32_t* gst, HitRec& h, TravCount& tc, bool& ovf, uint32_t& nrays,
+                                                GSlot gst, HitRec& h, TravCount& tc, bool& ovf, uint32_t& nrays,
                                                 uint32_t& nretrace) {
     float tmin = 0.00001f;
     uint32_t pass_flag = 0u;
     for (;;) {
         bool o1 = false;
-        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc, o1);
+        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, OCT>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc,
+                                                               o1);
         ovf = ovf || o1;
         if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
         const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
@@ -664,6 +713,10 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
 constexpr int extend_min_waves(bool count, bool uniform, bool two_level) {
     return (!count && uniform && !two_level) ? 6 : 1;
 }
+// Octant plane selection (box_near_oct) in the node visit: taken where registers are free. In the
+// single-scene one-level kernel, held to 80 VGPRs, the six live load addresses cost more than the
+// VALU it saves (measured 1.5 % slower there; 4 % faster in the two-level and megakernel traversals).
+constexpr bool extend_octant(bool uniform, bool two_level) { return two_level || !uniform; }
 
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
 __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL)) void k_extend(const crt_dev_scene* __restrict__ scenes,
@@ -681,7 +734,7 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t* lds_col = s_stack + threadIdx.x;
-    uint32_t* gst = gstack + (size_t)gtid * GS;
+    const GSlot gst{gstack, gtid * GS};
     TravCount tc = {0, 0, 0, 0, 0, 0};
     uint32_t nrays = 0, nretrace = 0, novf = 0;
     SceneView SU;
@@ -697,8 +750,8 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
         bool camera = (state & ST_CAMERA) != 0;
         HitRec h;
         bool ovf = false;
-        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst, h, tc,
-                                                                                   ovf, nrays, nretrace);
+        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, extend_octant(UNIFORM, TWO_LEVEL)>(
+            S, o, d, camera, lds_col, gst, h, tc, ovf, nrays, nretrace);
         novf += ovf ? 1u : 0u;
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
@@ -759,7 +812,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
     const uint32_t n = count_ptr ? *count_ptr : count_direct;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t* lds_col = s_stack + threadIdx.x;
-    uint32_t* gst = gstack + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * GS;
+    const GSlot gst{gstack, (blockIdx.x * BLOCK + threadIdx.x) * GS};
     TravCount tc = {0, 0, 0, 0, 0, 0};
     uint32_t nrays = 0, nretrace = 0, novf = 0;
     SceneView S;
@@ -852,7 +905,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __res
         HitRec h;
         bool ovf = false;
         trace<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
-                                        gstack + (size_t)gtid * GS, h, tc, ovf);
+                                        GSlot{gstack, gtid * GS}, h, tc, ovf);
         novf += ovf ? 1u : 0u;
         hit[i] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[i] = h.inst;
@@ -1240,7 +1293,7 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t* lds_col = s_stack + threadIdx.x;
-    uint32_t* gst = gstack + (size_t)gtid * GS;
+    const GSlot gst{gstack, gtid * GS};
     TravCount tc = {0, 0, 0, 0, 0, 0};
     uint32_t nrays = 0, nretrace = 0, novf = 0, light_tests = 0;
     float4* accum = (float4*)R.accum;
@@ -1343,7 +1396,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 }
 
 // ======================================================================================== launchers
-#define STK_RS 4
+#define STK_RS 1
 #define STK_LS 16
 #define STK_GS 96
 

@@ -260,13 +260,21 @@ struct Emitter {
         sah += (double)n.box.area() / root_area;
         int32_t refs[4] = {CRT_EMPTY_CHILD, CRT_EMPTY_CHILD, CRT_EMPTY_CHILD, CRT_EMPTY_CHILD};
         Box boxes[4];
-        for (int k = 0; k < 4; ++k) for (int a = 0; a < 3; ++a) { boxes[k].lo[a] = 0.0f; boxes[k].hi[a] = 0.0f; }
+        for (int k = 0; k < 4; ++k) boxes[k] = empty_box();
         for (int k = 0; k < nc; ++k) {
             refs[k] = emit4(ch[k], depth + 1);
             boxes[k] = nodes[ch[k]].box;
         }
         write_node4(slot, boxes, refs);
         return (int32_t)slot;
+    }
+
+    // Box of an empty child slot: at +inf on every axis, so on every axis the slab is either entered
+    // at +inf or left at -inf and no ray (finite tmax) hits it; the traversal needs no empty test.
+    static Box empty_box() {
+        Box b;
+        for (int a = 0; a < 3; ++a) { b.lo[a] = __builtin_inff(); b.hi[a] = __builtin_inff(); }
+        return b;
     }
 
     void write_node4(uint32_t slot, const Box* b, const int32_t* refs) {
@@ -301,14 +309,10 @@ struct Emitter {
             // empty tree, or a leaf root wrapped so traversal always starts at an interior node
             uint32_t slot = (uint32_t)(S.nodes.size() / per);
             S.nodes.resize(S.nodes.size() + per);
-            Box z;
-            for (int a = 0; a < 3; ++a) { z.lo[a] = 0.0f; z.hi[a] = 0.0f; }
+            const Box z = empty_box();
             int32_t L;
             Box lb = z;
             if (root < 0) {
-                // empty tree: children no ray can hit (a box at +inf: on every axis the slab is
-                // either entered at +inf or left at -inf); BVH4 also marks them empty
-                for (int a = 0; a < 3; ++a) { lb.lo[a] = __builtin_inff(); lb.hi[a] = __builtin_inff(); }
                 L = CRT_EMPTY_CHILD;
             } else {
                 root_area = std::max(nodes[root].box.area(), 1e-30f);
