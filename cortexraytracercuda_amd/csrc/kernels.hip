@@ -328,7 +328,7 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t
 // move the ray to object space and descend into the instance's BLAS. The traversal is a resumable
 // state machine: step() performs one node or leaf visit, so a persistent kernel can interleave
 // traversal steps with fetching new rays for finished lanes.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool OCT = true> struct Traverser {
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL> struct Traverser {
     Stack<BLOCK, RS, LS, GS> st;
     V3 o, d, id, ow, dw;
     uint32_t oct;          // bit a set where inv_d[a] < 0: the hi plane of axis a is the near one
@@ -385,32 +385,22 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 // planes lo.x hi.x lo.y hi.y lo.z hi.z (four children each) and the child refs; empty
                 // child slots have boxes at +inf, which no ray hits
                 const uint32_t nb = (uint32_t)node * 128u;   // node byte offset (scenes stay below 4 GiB of nodes)
+                // the near and far plane of each axis are picked by the ray's octant (no per-axis
+                // min / max); the seven loads are kept together ahead of the box tests (one memory
+                // round trip per visit: the scheduler would otherwise place the ref load late to
+                // save registers)
+                const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
+                float4 rff = ldg_at(S.nodes, nb + 96u);
+                float4 nx = ldg_at(S.nodes, nb + sx), fx = ldg_at(S.nodes, nb + (16u - sx)),
+                       ny = ldg_at(S.nodes, nb + 32u + sy), fy = ldg_at(S.nodes, nb + (48u - sy)),
+                       nz = ldg_at(S.nodes, nb + 64u + sz), fz = ldg_at(S.nodes, nb + (80u - sz));
+                __builtin_amdgcn_sched_group_barrier(0x020, 7, 0);   // 0x020: VMEM reads
+                const int4 rf = make_int4((int)fbits(rff.x), (int)fbits(rff.y), (int)fbits(rff.z), (int)fbits(rff.w));
                 bool h0, h1, h2, h3;
-                float k0, k1, k2, k3;
-                int4 rf;
-                if (OCT) {
-                    // near / far plane of each axis picked by the ray's octant: fewer VALU, six
-                    // load addresses instead of one
-                    const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
-                    float4 nx = ldg_at(S.nodes, nb + sx), fx = ldg_at(S.nodes, nb + (16u - sx)),
-                           ny = ldg_at(S.nodes, nb + 32u + sy), fy = ldg_at(S.nodes, nb + (48u - sy)),
-                           nz = ldg_at(S.nodes, nb + 64u + sz), fz = ldg_at(S.nodes, nb + (80u - sz));
-                    float4 rff = ldg_at(S.nodes, nb + 96u);
-                    rf = make_int4((int)fbits(rff.x), (int)fbits(rff.y), (int)fbits(rff.z), (int)fbits(rff.w));
-                    k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, id, tmin, tf, h0);
-                    k1 = box_near_oct(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, id, tmin, tf, h1);
-                    k2 = box_near_oct(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, id, tmin, tf, h2);
-                    k3 = box_near_oct(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, id, tmin, tf, h3);
-                } else {
-                    const float4* np = S.nodes + 8 * (uint32_t)node;
-                    float4 lx = ldg(np), hx = ldg(np + 1), ly = ldg(np + 2), hy = ldg(np + 3), lz = ldg(np + 4),
-                           hz = ldg(np + 5);
-                    rf = ldg_i4(np + 6);
-                    k0 = box_near(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, id, tmin, tf, h0);
-                    k1 = box_near(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, id, tmin, tf, h1);
-                    k2 = box_near(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, id, tmin, tf, h2);
-                    k3 = box_near(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, id, tmin, tf, h3);
-                }
+                float k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, id, tmin, tf, h0);
+                float k1 = box_near_oct(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, id, tmin, tf, h1);
+                float k2 = box_near_oct(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, id, tmin, tf, h2);
+                float k3 = box_near_oct(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, id, tmin, tf, h3);
                 const float INF = __builtin_inff();
                 k0 = h0 ? k0 : INF;
                 k1 = h1 ? k1 : INF;
@@ -496,10 +486,10 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     }
 };
 
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool OCT = true>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
 __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
                                       uint32_t* lds_col, GSlot gstk, HitRec& best, TravCount& tc, bool& ovf) {
-    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, OCT> tr;
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
     tr.init(S, ow, dw, tmin, tmax, camera, lds_col, gstk);
     while (!tr.step(S, tc)) {
     }
@@ -681,7 +671,7 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
 // Closest hit of one path segment. One-sided emitters hit from behind are re-traced from
 // t + 0.001 (core/src/cameras/camera.cuh:179-183); the result is then flagged as a pass-through
 // (returned flag CRT_PASSTHROUGH). `nrays` counts closest-hit queries, `nretrace` the re-traces.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool OCT = true>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
 __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
                                                 GSlot gst, HitRec& h, TravCount& tc, bool& ovf, uint32_t& nrays,
                                                 uint32_t& nretrace) {
@@ -689,8 +679,7 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
     uint32_t pass_flag = 0u;
     for (;;) {
         bool o1 = false;
-        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, OCT>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc,
-                                                               o1);
+        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc, o1);
         ovf = ovf || o1;
         if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
         const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
@@ -713,10 +702,6 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
 constexpr int extend_min_waves(bool count, bool uniform, bool two_level) {
     return (!count && uniform && !two_level) ? 6 : 1;
 }
-// Octant plane selection (box_near_oct) in the node visit: taken where registers are free. In the
-// single-scene one-level kernel, held to 80 VGPRs, the six live load addresses cost more than the
-// VALU it saves (measured 1.5 % slower there; 4 % faster in the two-level and megakernel traversals).
-constexpr bool extend_octant(bool uniform, bool two_level) { return two_level || !uniform; }
 
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
 __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL)) void k_extend(const crt_dev_scene* __restrict__ scenes,
@@ -750,8 +735,8 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
         bool camera = (state & ST_CAMERA) != 0;
         HitRec h;
         bool ovf = false;
-        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, extend_octant(UNIFORM, TWO_LEVEL)>(
-            S, o, d, camera, lds_col, gst, h, tc, ovf, nrays, nretrace);
+        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst, h, tc,
+                                                                                   ovf, nrays, nretrace);
         novf += ovf ? 1u : 0u;
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
