@@ -133,6 +133,24 @@ __device__ __forceinline__ uint2 ldg(const uint2* p) {
     crt_v2u v = *(__attribute__((address_space(1))) const crt_v2u*)p;
     return make_uint2(v.x, v.y);
 }
+typedef unsigned crt_v4u __attribute__((ext_vector_type(4)));
+typedef float crt_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint4 ldg(const uint4* p) {
+    crt_v4u v = *(__attribute__((address_space(1))) const crt_v4u*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ldg(const float2* p) {
+    crt_v2f v = *(__attribute__((address_space(1))) const crt_v2f*)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ uint32_t ldg(const uint32_t* p) { return *(__attribute__((address_space(1))) const uint32_t*)p; }
+__device__ __forceinline__ uint8_t ldg(const uint8_t* p) { return *(__attribute__((address_space(1))) const uint8_t*)p; }
+// stores through pointers read from request records (same reason: global, not flat)
+__device__ __forceinline__ void stg(uint32_t* p, uint32_t v) { *(__attribute__((address_space(1))) uint32_t*)p = v; }
+__device__ __forceinline__ void stg(float4* p, float4 v) {
+    crt_v4f w = {v.x, v.y, v.z, v.w};
+    *(__attribute__((address_space(1))) crt_v4f*)p = w;
+}
 
 // ------------------------------------------------------------------ traversal stack
 // Short per-lane stack: RS entries in VGPRs (static shifts, no dynamic register indexing), spilling
@@ -142,20 +160,30 @@ __device__ __forceinline__ uint2 ldg(const uint2* p) {
 // offset in entries (one VGPR instead of a 64-bit per-lane pointer).
 struct GSlot { uint32_t* base; uint32_t off; };
 
+// The LDS ring and the overflow area are held as address-space-qualified pointers, so the two
+// refill sources stay a ds_read and a global_load: merged through one generic pointer they become
+// a flat load that waits on both memory counters at every pop.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(1))) uint32_t glb_u32;
+
 template <int BLOCK, int RS, int LS, int GS> struct Stack {
     uint32_t r[RS];
     int rc, lbase, lcnt, gsp;
-    uint32_t* lds;     // column of this lane: lds[e * BLOCK]
-    GSlot gm;          // GS entries of this thread: gm.base[gm.off + e]
+    lds_u32* lds;      // column of this lane: lds[e * BLOCK]
+    glb_u32* gbase;    // overflow area of the launch (scalar); this thread's entries start at goff
+    uint32_t goff;
     bool overflow;
     __device__ __forceinline__ void init(uint32_t* lds_col, GSlot g) {
-        rc = 0; lbase = 0; lcnt = 0; gsp = 0; lds = lds_col; gm = g; overflow = false;
+        rc = 0; lbase = 0; lcnt = 0; gsp = 0; overflow = false;
+        lds = (lds_u32*)lds_col;
+        gbase = (glb_u32*)g.base;
+        goff = g.off;
     }
     __device__ __forceinline__ int depth() const { return rc + lcnt + gsp; }
     __device__ __forceinline__ bool empty() const { return rc == 0; }
     __device__ __forceinline__ void spill(uint32_t v) {
         if (lcnt == LS) {
-            if (gsp < GS) gm.base[gm.off + gsp++] = lds[lbase * BLOCK]; else overflow = true;
+            if (gsp < GS) gbase[goff + gsp++] = lds[lbase * BLOCK]; else overflow = true;
             lbase = (lbase + 1) & (LS - 1);
             --lcnt;
         }
@@ -164,7 +192,7 @@ template <int BLOCK, int RS, int LS, int GS> struct Stack {
     }
     __device__ __forceinline__ uint32_t refill() {
         if (lcnt > 0) { --lcnt; return lds[((lbase + lcnt) & (LS - 1)) * BLOCK]; }
-        return gm.base[gm.off + --gsp];
+        return gbase[goff + --gsp];
     }
     __device__ __forceinline__ void push(uint32_t v) {
         if (rc == RS) spill(r[RS - 1]); else ++rc;
@@ -654,7 +682,7 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
         Rng<MODE> g;
         g.dyn = R.rng_mode;
         g.ms = 0u;
-        if (g.minstd()) g.ms = R.rng_state[pix];
+        if (g.minstd()) g.ms = ldg(R.rng_state + pix);
         else g.pcg = pcg_keyed(R.seed, ref_pixel_of(R, pix), sample, 0u);
         V3 o, dir;
         float tm;
@@ -904,8 +932,8 @@ struct Mat { float4 m0; int4 m1; };
 
 __device__ __forceinline__ float4 tex_value(const float4* texs, const uint8_t* texels, int ti, float u, float v, V3 p) {
     for (int hop = 0; hop < 2; ++hop) {
-        float4 t0 = texs[2 * ti];
-        int4 t1 = *(const int4*)(texs + 2 * ti + 1);
+        float4 t0 = ldg(texs + 2 * ti);
+        int4 t1 = ldg_i4(texs + 2 * ti + 1);
         if (t1.x == CRT_TEX_SOLID) return t0;
         if (t1.x == CRT_TEX_CHECKER) {
             // checker_texture::value (core/src/textures/checker_texture.cuh:33-42); t0.w = 1/scale
@@ -929,10 +957,10 @@ __device__ __forceinline__ float4 tex_value(const float4* texs, const uint8_t* t
         const uint8_t* px = texels + off + ((size_t)j * W + i) * ch;
         const float sc = 1.0f / 255.0f;
         if (ch == 4u) {
-            uint32_t w4 = *(const uint32_t*)px;
+            uint32_t w4 = ldg((const uint32_t*)px);
             return make_float4(sc * (float)(w4 & 0xffu), sc * (float)((w4 >> 8) & 0xffu), sc * (float)((w4 >> 16) & 0xffu), 0.0f);
         }
-        return make_float4(sc * (float)px[0], sc * (float)px[1], sc * (float)px[2], 0.0f);
+        return make_float4(sc * (float)ldg(px), sc * (float)ldg(px + 1), sc * (float)ldg(px + 2), 0.0f);
     }
     return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -969,7 +997,7 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
         float4 tq0, tq1, tq2;
         if (type == CRT_PRIM_TRI) {
             const float4* tp = (const float4*)sc->tris + 3 * idx;
-            tq0 = tp[0]; tq1 = tp[1]; tq2 = tp[2];
+            tq0 = ldg(tp); tq1 = ldg(tp + 1); tq2 = ldg(tp + 2);
             mat_id = fbits(tq1.w);
             tri_flags = fbits(tq2.w);
             // flat normal = unit(cross(v0 - v1, v0 - v2)) (core/src/primitives/triangle.cuh:118)
@@ -978,13 +1006,13 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
             if (tri_flags & CRT_TF_SMOOTH) {
                 const float4* tn = (const float4*)sc->tri_normals + 3 * idx;
                 // u weights v1, v weights v2 (triangle.cuh:167-174), not renormalised
-                nrm = hv.z * v3(tn[1]) + hv.w * v3(tn[2]) + (1.0f - hv.z - hv.w) * v3(tn[0]);
+                nrm = hv.z * v3(ldg(tn + 1)) + hv.w * v3(ldg(tn + 2)) + (1.0f - hv.z - hv.w) * v3(ldg(tn));
             }
             if (inst != CRT_NO_INST) {
                 // object -> world normal: transpose of the world -> object 3x3
                 const float4* ip = (const float4*)sc->insts + 4 * inst;
-                float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
-                int ovr = (int)fbits(ip[3].z);
+                float4 r0 = ldg(ip), r1 = ldg(ip + 1), r2 = ldg(ip + 2);
+                int ovr = (int)fbits(ldg(ip + 3).z);
                 if (ovr >= 0) mat_id = (uint32_t)ovr;
                 V3 nw = v3(r0.x * nrm.x + r1.x * nrm.y + r2.x * nrm.z, r0.y * nrm.x + r1.y * nrm.y + r2.y * nrm.z,
                            r0.z * nrm.x + r1.z * nrm.y + r2.z * nrm.z);
@@ -992,15 +1020,15 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
             }
             outward = nrm;
         } else if (type == CRT_PRIM_SPHERE) {
-            float4 s = ((const float4*)sc->spheres)[idx];
-            uint4 aux = ((const uint4*)sc->sphere_aux)[idx];
+            float4 s = ldg((const float4*)sc->spheres + idx);
+            uint4 aux = ldg((const uint4*)sc->sphere_aux + idx);
             mat_id = aux.y;
             sphere_c = v3(s); sphere_r = s.w;
             outward = (p - sphere_c) / sphere_r;
             need_uv_sphere = true;
         } else {
             const float4* q = (const float4*)sc->quads + 5 * idx;
-            float4 q1 = q[1], q3 = q[3], q4 = q[4];
+            float4 q1 = ldg(q + 1), q3 = ldg(q + 3), q4 = ldg(q + 4);
             mat_id = fbits(q4.w);
             outward = v3(q3);
             is_light_quad = (fbits(q1.w) & CRT_QF_LIGHT) != 0;
@@ -1009,8 +1037,8 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
         V3 nrm = front ? outward : -outward;
         const float4* mats = (const float4*)sc->mats;
         Mat m;
-        m.m0 = mats[2 * mat_id];
-        m.m1 = *(const int4*)(mats + 2 * mat_id + 1);
+        m.m0 = ldg(mats + 2 * mat_id);
+        m.m1 = ldg_i4(mats + 2 * mat_id + 1);
         if (m.m1.x == CRT_MAT_DIFFUSE_LIGHT) {
             // emitter: light does not scatter (material::scatter returns false)
             if (!passthrough) {
@@ -1024,7 +1052,7 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                 if (need_uv_sphere) {
                     // get_sphere_uv on the outward normal (core/src/utilities/uvmapping.cuh:204-228)
                     const float2* mp = (const float2*)sc->sphere_map + 3 * idx;
-                    float2 scl = mp[0], ofs = mp[1], rep = mp[2];
+                    float2 scl = ldg(mp), ofs = ldg(mp + 1), rep = ldg(mp + 2);
                     float theta = acosf(-outward.y);
                     float phi = atan2f(-outward.z, outward.x) + 3.14159265358979323846f;
                     float s_ = phi / (2.0f * 3.14159265358979323846f);
@@ -1037,7 +1065,7 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                     tu = 0.0f; tvv = 0.0f;
                     if (tri_flags & CRT_TF_UV) {
                         const float2* uvp = (const float2*)sc->tri_uvs + 3 * idx;
-                        float2 a = uvp[0], b = uvp[1], c = uvp[2];
+                        float2 a = ldg(uvp), b = ldg(uvp + 1), c = ldg(uvp + 2);
                         float w0 = 1.0f - hv.z - hv.w;
                         tu = w0 * a.x + hv.z * b.x + hv.w * c.x;
                         tvv = w0 * a.y + hv.z * b.y + hv.w * c.y;
@@ -1101,9 +1129,9 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                         dir = v3(0.0f, 0.0f, 0.0f);
                     } else {
                         if (nl > 1) g.skip();
-                        const float4* q = qd + 5 * sc->light_quads[0];
+                        const float4* q = qd + 5 * ldg(sc->light_quads);
                         float a = g.uniform(), b = g.uniform();
-                        V3 pp = v3(q[0]) + (a - 0.5f) * v3(q[1]) + (b - 0.5f) * v3(q[2]);
+                        V3 pp = v3(ldg(q)) + (a - 0.5f) * v3(ldg(q + 1)) + (b - 0.5f) * v3(ldg(q + 2));
                         dir = pp - p;
                     }
                 } else {
@@ -1119,14 +1147,14 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                 float weight = 1.0f / (float)nl;
                 float sum = 0.0f;
                 for (uint32_t li = 0; li < nl; ++li) {
-                    const float4* q = qd + 5 * sc->light_quads[li];
+                    const float4* q = qd + 5 * ldg(sc->light_quads + li);
                     float tl, qa, qb;
                     light_tests++;
                     float pv = 0.0f;
                     if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb)) {
                         float dist2 = tl * tl * len2(dir);
-                        float cosine = fabsf(dot(dir, v3(q[3])) / vlen(dir));
-                        pv = dist2 / (cosine * q[2].w);
+                        float cosine = fabsf(dot(dir, v3(ldg(q + 3))) / vlen(dir));
+                        pv = dist2 / (cosine * ldg(q + 2).w);
                     }
                     sum += weight * pv;
                 }
@@ -1150,8 +1178,11 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
     return !done;
 }
 
-template <int MODE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict__ scenes,
+// Shade stage. FLAT (no instanced scene among the paths) holds the kernel to 6 resident
+// workgroups per CU (80 VGPRs, a few spills outside the hot paths), measured 2 % faster per frame
+// on single-level scenes; with instances (transforms, textures) the unbounded kernel is faster.
+template <int MODE, int BLOCK, bool FLAT>
+__global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_scene* __restrict__ scenes,
                                                  const crt_dev_request* __restrict__ reqs,
                                                  const uint32_t* __restrict__ count_in, crt_paths Pin, crt_paths Pout,
                                                  uint32_t* __restrict__ count_out, const float4* __restrict__ hit,
@@ -1208,7 +1239,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const crt_dev_scene* __restrict
                 // finished: the radiance goes to the path's id
                 uint32_t pid = R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin);
                 out_rad[pid] = make_float4(L.x, L.y, L.z, 0.0f);
-                if (g.minstd()) R.rng_state[pix] = g.ms;
+                if (g.minstd()) stg(R.rng_state + pix, g.ms);
             }
         }
         if (done_count) {
@@ -1284,11 +1315,11 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
     float4* accum = (float4*)R.accum;
     for (uint32_t q = gtid; q < R.pix_count; q += stride) {
         uint32_t pix = R.pix_begin + q;
-        float4 acc = accum[pix];
+        float4 acc = ldg(accum + pix);
         Rng<MODE> g;
         g.dyn = R.rng_mode;
         g.ms = 0u;
-        if (g.minstd()) g.ms = R.rng_state[pix];
+        if (g.minstd()) g.ms = ldg(R.rng_state + pix);
         const uint32_t refp = ref_pixel_of(R, pix);
         for (uint32_t k = 0; k < R.sample_count; ++k) {
             uint32_t sample = R.sample_begin + k;
@@ -1313,8 +1344,8 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
             acc.x += L.x; acc.y += L.y; acc.z += L.z;
         }
         acc.w += (float)R.sample_count;
-        accum[pix] = acc;
-        if (g.minstd()) R.rng_state[pix] = g.ms;
+        stg(accum + pix, acc);
+        if (g.minstd()) stg(R.rng_state + pix, g.ms);
     }
     wave_add(&counters->rays, nrays);
     wave_add(&counters->retraces, nretrace);
@@ -1332,13 +1363,13 @@ __global__ __launch_bounds__(256) void k_resolve(const crt_dev_request* __restri
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < R.pix_count; q += gridDim.x * blockDim.x) {
         uint32_t pix = R.pix_begin + q;
         float4* a = (float4*)R.accum + pix;
-        float4 acc = *a;
+        float4 acc = ldg(a);
         for (uint32_t k = 0; k < R.sample_count; ++k) {
             float4 v = rad[R.path_base + k * R.pix_count + q];
             acc.x += v.x; acc.y += v.y; acc.z += v.z;
         }
         acc.w += (float)R.sample_count;
-        *a = acc;
+        stg(a, acc);
     }
 }
 
@@ -1464,16 +1495,18 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
     return (int)hipGetLastError();
 }
 
-extern "C" int crt_k_shade(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_scene* scenes,
+extern "C" int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                            const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout,
                            uint32_t* count_out, const float4* hit, const uint32_t* hit_inst, float4* out_rad,
                            uint32_t* done_count, int sort_octant, crt_dev_counters* counters) {
-#define CRT_SH(M)                                                                                                     \
-    hipLaunchKernelGGL((k_shade<M, 256>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout, count_out, \
+#define CRT_SH(M, F)                                                                                                      \
+    hipLaunchKernelGGL((k_shade<M, 256, F>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout, count_out, \
                        hit, hit_inst, out_rad, done_count, sort_octant, counters)
-    if (rng_mode == CRT_RNG_MINSTD) CRT_SH(CRT_RNG_MINSTD);
-    else if (rng_mode == CRT_RNG_PCG) CRT_SH(CRT_RNG_PCG);
-    else CRT_SH(CRT_RNG_DYNAMIC);
+#define CRT_SH_F(M) { if (two_level) CRT_SH(M, false); else CRT_SH(M, true); }
+    if (rng_mode == CRT_RNG_MINSTD) CRT_SH_F(CRT_RNG_MINSTD)
+    else if (rng_mode == CRT_RNG_PCG) CRT_SH_F(CRT_RNG_PCG)
+    else CRT_SH_F(CRT_RNG_DYNAMIC)
+#undef CRT_SH_F
 #undef CRT_SH
     return (int)hipGetLastError();
 }
