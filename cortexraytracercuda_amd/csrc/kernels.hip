@@ -725,10 +725,12 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
 // UNIFORM: every ray of the launch belongs to scene `uniform_scene` (scene pointers stay scalar).
 // Minimum resident blocks per CU the register allocation must allow. Blocks are 256 threads, one
 // wave on each of a CU's 4 SIMDs, so this is also waves per SIMD. The traversal is latency-bound and
-// occupancy pays: the single-scene one-level kernel (the common render) is held to 6 (<= 80 VGPRs,
-// no spills); the others take the registers they need.
+// occupancy pays more than the spills it costs (measured): one-level kernels are held to 6 (80
+// VGPRs; the single-scene one needs no spill, the multi-scene one spills its per-ray values: 8.6 %
+// faster on a 16-scene batch than at its natural 5), two-level kernels to 5 (96 VGPRs: 6 % faster
+// than at 4; at 6 they spill inside the loop and lose 17 %). Counting variants are not bounded.
 constexpr int extend_min_waves(bool count, bool uniform, bool two_level) {
-    return (!count && uniform && !two_level) ? 6 : 1;
+    return count ? 1 : (two_level ? 5 : 6);
 }
 
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
@@ -806,8 +808,10 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
 // A wave leaves when its share is handed out and its lanes are idle; each traversal ends (every
 // node is visited at most once per query, at most one re-trace). Pays when rays of one wave differ
 // much in length (many primitive tests per ray, little coherence); costs coherence on primary rays.
+// one-level: 6 (80 VGPRs, a few per-ray spills; 2.5 % faster than at 5); two-level: unbounded
+// (at 6 the loop spills and the instanced scene takes 1.6x as long)
 constexpr int extend_pt_min_waves(bool count, bool uniform, bool two_level, int width) {
-    return (!count && uniform && !two_level && width == 4) ? 5 : 1;   // 5: <= 96 VGPRs, no spills
+    return (count || two_level) ? 1 : 6;
 }
 
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
