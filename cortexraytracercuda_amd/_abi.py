@@ -38,6 +38,7 @@ CRT_RNG_MINSTD = 1
 
 CRT_RF_TIME_STAGES = 1
 CRT_RF_MEGAKERNEL = 2
+CRT_RF_IMAGE_ONLY = 4
 
 CRT_EXTEND_DEFAULT = 0
 CRT_EXTEND_GRID_STRIDE = 1

@@ -134,12 +134,14 @@ struct Server {
     uint32_t appends = 0;
     bool full = false;                     // no room for another segment: drain, then reset
     bool two_level = false;                // an instanced scene has paths in the open pass
+    bool mixed_rng = false;                // a MINSTD request has paths in the open pass (per-path RNG mode)
     uint32_t rr = 0;                       // round-robin start among active requests
     uint64_t progress_iter = 0;            // last iteration whose snapshot showed finished paths
     uint32_t snap_ne[kRing] = {};          // entries covered by each snapshot
     bool use_graphs = true;                // replay captured iterations
-    uint32_t warm[2] = {};                 // eager iterations per variant (graphs are captured after kRing)
-    hipGraphExec_t graph[2][kRing] = {};   // [two-level][snapshot slot]
+    // kernel variant of an iteration: two_level * 2 + mixed_rng
+    uint32_t warm[4] = {};                 // eager iterations per variant (graphs are captured after kRing)
+    hipGraphExec_t graph[4][kRing] = {};   // [variant][snapshot slot]
 };
 
 struct Context {

@@ -129,7 +129,7 @@ void Context::server_release() {
     if (S.open_host) (void)hipHostFree(S.open_host);
     if (S.snap) (void)hipHostFree(S.snap);
     for (int k = 0; k < Server::kRing; ++k) if (S.ev[k]) (void)hipEventDestroy(S.ev[k]);
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < 4; ++t)
         for (int k = 0; k < Server::kRing; ++k)
             if (S.graph[t][k]) { (void)hipGraphExecDestroy(S.graph[t][k]); S.graph[t][k] = nullptr; }
     if (S.s) (void)hipStreamDestroy(S.s);
@@ -150,6 +150,7 @@ void Context::server_reset_pass() {
     S.progress_iter = 0;
     S.full = false;
     S.two_level = false;
+    S.mixed_rng = false;
     (void)crt_k_pass_init(S.s, (uint32_t*)S.ctl.p, 2 * CRT_CTL_WORDS + 2);
     (void)crt_k_fill_u32(S.s, (uint32_t*)S.stats.p, 2 * Server::kEntries, 0u);
 }
@@ -193,6 +194,7 @@ int Context::server_append(std::vector<ServeReq*>& act) {
         S.entries.push_back(en);
         S.total += en.paths;
         if (r->sc->host.num_instances) S.two_level = true;
+        if (r->rd.rng_mode == CRT_RNG_MINSTD) S.mixed_rng = true;
         r->timing.passes++;
         if (r->timing.first_pass_s == 0.0) r->timing.first_pass_s = now_s();
     }
@@ -230,13 +232,15 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
     const crt_dev_request* dreq = (const crt_dev_request*)S.req.p;
     const crt_dev_scene* dsc = (const crt_dev_scene*)C.scene_table.p;
     const int two_level = S.two_level ? 1 : 0;
-    if (crt_k_regen(S.s, CRT_RNG_DYNAMIC, S.sgrid, dreq, 0, 0, 0, S.slots, cin, cout, open, A))
+    // the per-path RNG mode only once a MINSTD request is in the pass (keyed PCG otherwise)
+    const int mode = S.mixed_rng ? CRT_RNG_DYNAMIC : CRT_RNG_PCG;
+    if (crt_k_regen(S.s, mode, S.sgrid, dreq, 0, 0, 0, S.slots, cin, cout, open, A))
         return set_error(CRT_ERR_HIP, "regen launch failed");
     if (crt_k_extend(S.s, 0, (int)C.bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
                      A.ray_d, A.ids, (float4*)S.hit.p, (uint32_t*)S.hit_inst.p, (uint32_t*)S.gstack.p, nullptr,
                      C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
-    if (crt_k_shade(S.s, CRT_RNG_DYNAMIC, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
+    if (crt_k_shade(S.s, mode, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.stats.p,
                     (int)C.ray_sort, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "shade launch failed");
@@ -252,7 +256,7 @@ int Context::server_iteration() {
     Server& S = srv;
     const uint64_t k = S.iter;
     const int slot = (int)(k % Server::kRing);
-    const int tl = S.two_level ? 1 : 0;
+    const int tl = (S.two_level ? 2 : 0) + (S.mixed_rng ? 1 : 0);
     int rc;
     if (S.use_graphs && S.warm[tl] >= (uint32_t)Server::kRing) {
         hipGraphExec_t& g = S.graph[tl][slot];
@@ -432,11 +436,13 @@ void Context::server_loop() {
         for (ServeReq* r : finished) {
             int st = err ? err : (r->cancel ? CRT_ERR_CANCELLED : CRT_OK);
             if (st == CRT_OK) {
+                // the accumulation buffer (16 B per pixel against 3 for the image) only when asked for
+                const bool want_accum = !(r->rd.flags & CRT_RENDER_IMAGE_ONLY);
                 r->rgb.resize((size_t)r->job->npix * 3);
-                r->accum.resize((size_t)r->job->npix * 4);
+                r->accum.resize(want_accum ? (size_t)r->job->npix * 4 : 0);
                 if (hipMemcpyAsync(r->rgb.data(), r->job->rgb.p, r->rgb.size(), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
-                    hipMemcpyAsync(r->accum.data(), r->job->accum.p, r->accum.size() * sizeof(float), hipMemcpyDeviceToHost,
-                                   S.copy_s) != hipSuccess ||
+                    (want_accum && hipMemcpyAsync(r->accum.data(), r->job->accum.p, r->accum.size() * sizeof(float),
+                                                  hipMemcpyDeviceToHost, S.copy_s) != hipSuccess) ||
                     hipStreamSynchronize(S.copy_s) != hipSuccess)
                     st = set_error(CRT_ERR_HIP, "readback failed");
             }
@@ -497,6 +503,8 @@ int crt_wait(crt_context* c, crt_request_id req, crt_image* out, crt_stats* stat
         int st = r->status;
         return set_error(st, st == CRT_ERR_CANCELLED ? "request cancelled" : "request failed");
     }
+    if (out && out->accum && (r->rd.flags & CRT_RENDER_IMAGE_ONLY))
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "request was submitted with CRT_RENDER_IMAGE_ONLY: no accumulation buffer");
     if (out && out->rgb8) std::memcpy(out->rgb8, r->rgb.data(), r->rgb.size());
     if (out && out->accum) std::memcpy(out->accum, r->accum.data(), r->accum.size() * sizeof(float));
     if (stats) {

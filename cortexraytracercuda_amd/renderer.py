@@ -184,8 +184,10 @@ class Renderer:
         return prim, t
 
     # ---------------------------------------------------------------- serving
-    def submit(self, sid, width, height, spp, max_depth, seed=1, rng="pcg"):
-        rd = render_desc(width, height, spp, max_depth, seed, rng)
+    def submit(self, sid, width, height, spp, max_depth, seed=1, rng="pcg", image_only=False):
+        """Queues a request in the serving loop. image_only: only the RGB8 image is copied back
+        (wait(..., want_accum=False) then)."""
+        rd = render_desc(width, height, spp, max_depth, seed, rng, flags=A.CRT_RF_IMAGE_ONLY if image_only else 0)
         rid = C.c_uint64()
         A.check(self.lib.crt_submit(self.h, sid, C.byref(rd), C.byref(rid)))
         return rid.value

@@ -193,10 +193,14 @@ typedef struct crt_render_desc {
 /* crt_render_desc.flags */
 enum {
     CRT_RENDER_TIME_STAGES = 1u,  /* time the closest-hit launches with HIP events (crt_stats.extend_ms) */
-    CRT_RENDER_MEGAKERNEL = 2u    /* blocking renders only: one kernel launch renders the request, one thread per
+    CRT_RENDER_MEGAKERNEL = 2u,   /* blocking renders only: one kernel launch renders the request, one thread per
                                      pixel running its samples and bounces in turn (the reference's render kernel
                                      design, core/src/gpu.cu:756-801) instead of the wavefront passes. Same
                                      estimator and streams: the image agrees with the wavefront render. */
+    CRT_RENDER_IMAGE_ONLY = 4u    /* served requests: only the RGB8 image is copied back at completion;
+                                     crt_wait then returns no accumulation buffer (CRT_ERR_INVALID_ARGUMENT
+                                     if one is asked for). Blocking renders skip it when crt_image.accum
+                                     is NULL. */
 };
 
 typedef struct crt_image {

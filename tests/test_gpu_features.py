@@ -145,6 +145,35 @@ def test_serving_alone_vs_mix_and_cancel():
         assert e.value.code == A.CRT_ERR_CANCELLED
 
 
+def test_serving_rng_modes_mixed_into_one_pass():
+    """Keyed-PCG requests run with the static-mode kernels until a MINSTD request joins the open
+    pass, then with the per-path mode: every request still equals its blocking render bit for bit."""
+    with Renderer(max_paths=120000) as r:
+        a = r.upload(scenes.sphere_mesh_scene(4000, seed=2))
+        jobs = [(64, 48, 9, 31, "pcg"), (80, 64, 4, 32, "pcg"), (48, 48, 4, 2580, "minstd"), (64, 64, 16, 33, "pcg")]
+        refs = [r.render(a, W, H, spp, 8, seed=sd, rng=m)[1] for W, H, spp, sd, m in jobs]
+        ids = [r.submit(a, W, H, spp, 8, seed=sd, rng=m) for W, H, spp, sd, m in jobs]
+        for (W, H, _, _, _), rid, ref in zip(jobs, ids, refs):
+            _, acc, _ = r.wait(rid, W, H)
+            assert np.array_equal(acc, ref)
+
+
+def test_serving_image_only():
+    """CRT_RENDER_IMAGE_ONLY: the served request returns the same 8-bit image as a full request
+    and refuses to hand out an accumulation buffer it did not copy back."""
+    with Renderer(max_paths=200000) as r:
+        a = r.upload(scenes.sphere_mesh_scene(5000, seed=1))
+        full = r.submit(a, 96, 80, 9, 8, seed=5)
+        img = r.submit(a, 96, 80, 9, 8, seed=5, image_only=True)
+        img2 = r.submit(a, 96, 80, 9, 8, seed=5, image_only=True)
+        rgb_f, acc_f, _ = r.wait(full, 96, 80)
+        rgb_i, acc_i, _ = r.wait(img, 96, 80, want_accum=False)
+        assert acc_i is None and np.array_equal(rgb_f, rgb_i)
+        with pytest.raises(A.CrtError) as e:
+            r.wait(img2, 96, 80, want_accum=True)
+        assert e.value.code == A.CRT_ERR_INVALID_ARGUMENT
+
+
 @pytest.mark.parametrize("graphs", [1, 2])
 def test_serving_graph_replays_match_eager(graphs):
     """Serving iterations replayed from captured hipGraphs (graphs=1) or launched eagerly (2): many
