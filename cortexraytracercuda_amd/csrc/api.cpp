@@ -326,7 +326,7 @@ int Job::validate(const crt_render_desc& r) {
 
 uint64_t Job::bytes_needed(const crt_render_desc& r) {
     uint64_t npix = (uint64_t)r.width * r.height;
-    return npix * 16 + npix * 3 + (r.rng_mode == CRT_RNG_MINSTD ? npix * 4 : 0);
+    return npix * 16 + npix * 3 + 16 + (r.rng_mode == CRT_RNG_MINSTD ? npix * 4 : 0);
 }
 
 int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStream_t st) {
@@ -345,11 +345,12 @@ int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStrea
     req.rng_mode = r.rng_mode; req.scene = sc->slot; req.seed = r.seed;
     int rc;
     if ((rc = accum.alloc((size_t)npix * 16, ctx))) return rc;
-    if ((rc = rgb.alloc((size_t)npix * 3, ctx))) return rc;
+    if ((rc = rgb.alloc((((size_t)npix * 3 + 7) & ~(size_t)7) + 8, ctx))) return rc;
     if (r.rng_mode == CRT_RNG_MINSTD && (rc = rng.alloc((size_t)npix * 4, ctx))) return rc;
     req.accum = (float*)accum.p;
     req.rng_state = (uint32_t*)rng.p;
-    if (crt_k_init_request(st ? st : ctx->stream, (float4*)accum.p, (uint32_t*)rng.p, r.width, r.height, r.seed))
+    if (crt_k_init_request(st ? st : ctx->stream, (float4*)accum.p, (uint32_t*)rng.p, r.width, r.height, r.seed) ||
+        crt_k_fill_u32(st ? st : ctx->stream, (uint32_t*)rays_counter(), 2, 0u))
         return set_error(CRT_ERR_HIP, "init kernel launch failed");
     return CRT_OK;
 }
@@ -495,7 +496,8 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             iterations++;
         }
     }
-    if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)out_rad.p)) return set_error(CRT_ERR_HIP, "resolve launch failed");
+    if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)out_rad.p, nullptr))
+        return set_error(CRT_ERR_HIP, "resolve launch failed");
     HIP_TRY(hipEventRecord(req_event[buf], stream));
     passes++;
     return CRT_OK;

@@ -35,16 +35,19 @@ int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                  uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
                  crt_dev_counters* counters);
-// two_level: an instanced scene has paths in the launch (selects the unbounded-register variant)
+// two_level: an instanced scene has paths in the launch (selects the unbounded-register variant);
+// live_min (nullable, grid words): per block, the smallest path id still alive after the stage
 int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout, uint32_t* count_out, const float4* hit,
-                const uint32_t* hit_inst, float4* out_rad, uint32_t* done_count, int sort_octant,
+                const uint32_t* hit_inst, float4* out_rad, uint32_t* live_min, int sort_octant,
                 crt_dev_counters* counters);
 int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* req, uint32_t* gstack, crt_dev_counters* counters);
 int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_t grid, const crt_dev_scene* sc, const float4* rays, uint32_t n, int camera,
                      float4* hit, uint32_t* hit_inst, uint32_t* gstack, crt_dev_counters* counters);
-int crt_k_resolve(hipStream_t s, const crt_dev_request* reqs, int nreq, uint32_t max_pix, const float4* rad);
+// rays (nreq == 1 only, may be null): the closest-hit queries of the resolved paths are added to it
+int crt_k_resolve(hipStream_t s, const crt_dev_request* reqs, int nreq, uint32_t max_pix, const float4* rad,
+                  unsigned long long* rays);
 int crt_k_tonemap(hipStream_t s, const float4* accum, uint32_t npix, uint32_t spp, uint8_t* rgb);
 int crt_k_init_request(hipStream_t s, float4* accum, uint32_t* rng_state, uint32_t W, uint32_t H, uint64_t seed);
 int crt_k_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);

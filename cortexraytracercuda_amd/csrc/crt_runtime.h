@@ -76,6 +76,10 @@ struct Job {
     uint32_t npix = 0, sq = 0, nsamples = 0, s_begin = 0, s_end = 0, s_next = 0, p_next = 0;
     DevBuf accum, rgb, rng;
     uint64_t rays_in_flight = 0;
+    // closest-hit queries of the job's resolved paths (served jobs): 8 bytes behind the image
+    unsigned long long* rays_counter() const {
+        return (unsigned long long*)((char*)rgb.p + (((size_t)npix * 3 + 7) & ~(size_t)7));
+    }
     static int validate(const crt_render_desc& r);
     static uint64_t bytes_needed(const crt_render_desc& r);
     int setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStream_t st = nullptr);
@@ -104,9 +108,9 @@ struct ServeReq {
 // One segment (work item) of a request inside the server's open pass: a request-table entry.
 struct SrvEntry {
     ServeReq* r = nullptr;
-    uint32_t seg = 0, paths = 0, done = 0;
-    uint64_t rays = 0;
-    bool resolved = false;
+    uint32_t seg = 0, paths = 0;
+    uint32_t end = 0;                 // one past the entry's last path id
+    bool done = false, resolved = false;
 };
 
 // The serving loop's device state: its own stream, path pool and overflow stacks, and one open
@@ -116,28 +120,30 @@ struct Server {
     static constexpr uint32_t kEntries = 4096;   // request-table entries per pass (segments)
     static constexpr int kRing = 8, kLag = 4;    // snapshots in flight / host look-behind
     static constexpr int kOpenRing = 64;         // pinned (total, nreq) words for the appends
-    static constexpr int kSnapWords = CRT_CTL_WORDS + 20 + 2 * kEntries;   // ctl half, counters, per-entry stats
+    static constexpr int kMaxShadeGrid = 65536;            // shade workgroups (one live-minimum word each)
+    static constexpr int kSnapWords = CRT_CTL_WORDS + 20 + kMaxShadeGrid;   // control half, counters, live minima
     bool ready = false;
     hipStream_t s = nullptr, copy_s = nullptr;
     uint32_t slots = 0, cap = 0, egrid = 0, sgrid = 0;
-    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad, req, ctl, stats, gstack, counters;
+    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad, req, ctl, gstack, counters, live_min;
     crt_dev_request* req_host = nullptr;   // pinned mirror of the open pass's request table
     uint32_t* open_host = nullptr;         // pinned ring of (total, nreq)
     uint32_t* snap = nullptr;              // pinned ring of iteration snapshots
+    void* rays_host = nullptr;             // pinned word for a finished request's ray count
     hipEvent_t ev[kRing] = {};
     // open pass (host view)
     std::vector<SrvEntry> entries;
     uint32_t total = 0;                    // path ids appended
     uint64_t iter = 0, processed = 0;      // iterations enqueued / snapshots read
-    uint64_t done_paths = 0;               // finished paths observed
+    uint64_t done_paths = 0;               // paths of the entries known to be finished
     uint32_t gen_seen = 0;                 // generated path ids observed
+    uint32_t gen_progress = 0;             // GEN at the last iteration that generated paths
     uint32_t appends = 0;
     bool full = false;                     // no room for another segment: drain, then reset
     bool two_level = false;                // an instanced scene has paths in the open pass
     bool mixed_rng = false;                // a MINSTD request has paths in the open pass (per-path RNG mode)
     uint32_t rr = 0;                       // round-robin start among active requests
     uint64_t progress_iter = 0;            // last iteration whose snapshot showed finished paths
-    uint32_t snap_ne[kRing] = {};          // entries covered by each snapshot
     bool use_graphs = true;                // replay captured iterations
     // kernel variant of an iteration: two_level * 2 + mixed_rng
     uint32_t warm[4] = {};                 // eager iterations per variant (graphs are captured after kRing)

@@ -1191,25 +1191,20 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
                                                  const uint32_t* __restrict__ count_in, crt_paths Pin, crt_paths Pout,
                                                  uint32_t* __restrict__ count_out, const float4* __restrict__ hit,
                                                  const uint32_t* __restrict__ hit_inst, float4* __restrict__ out_rad,
-                                                 uint32_t* __restrict__ done_count, int sort_octant,
+                                                 uint32_t* __restrict__ live_min, int sort_octant,
                                                  crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_oct[8 * (BLOCK / 64)];
     __shared__ uint32_t s_base;
-    // per-block cache of the per-entry (finished, rays) counts, keyed by entry & 7: one global
-    // atomic pair per cached entry and block at the end instead of one per wave and iteration
-    // (same-address atomics serialise across the XCDs)
-    __shared__ uint32_t s_key[8], s_cnt[16];
-    if (threadIdx.x < 8) s_key[threadIdx.x] = 0xffffffffu;
-    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0u;
-    __syncthreads();
+    __shared__ uint32_t s_min[BLOCK / 64];
     const uint32_t n = *count_in;
+    uint32_t min_pid = 0xffffffffu;   // smallest path id among this thread's surviving paths
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t light_tests = 0;
     // grid-stride loop with a block-uniform trip count so every thread reaches the block scan
     const uint32_t iters = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < iters; ++it) {
         uint32_t i = it * stride + blockIdx.x * BLOCK + threadIdx.x;
-        bool keep = false, finished = false;
+        bool keep = false;
         float4 no, nd, nt, nr;
         uint32_t oct = 0;   // direction octant of the next segment
         uint2 idv = make_uint2(0u, 0u);
@@ -1231,8 +1226,8 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
             g.ms = fbits(tv.w);
             g.dyn = R.rng_mode;
             keep = shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
-            finished = !keep;
             if (keep) {
+                min_pid = min(min_pid, R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin));
                 uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8);
                 no = make_float4(o.x, o.y, o.z, ro.w);
                 nd = make_float4(d.x, d.y, d.z, bitsf(ns));
@@ -1240,36 +1235,12 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
                 nr = make_float4(L.x, L.y, L.z, 0.0f);
                 oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
             } else {
-                // finished: the radiance goes to the path's id
+                // finished: the radiance goes to the path's id, with the path's closest-hit queries
+                // (one per hit, plus the final miss) in .w for the per-request ray count
                 uint32_t pid = R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin);
-                out_rad[pid] = make_float4(L.x, L.y, L.z, 0.0f);
+                const uint32_t rays = bounce + (fbits(hv.y) == CRT_MISS ? 1u : 0u);
+                out_rad[pid] = make_float4(L.x, L.y, L.z, (float)rays);
                 if (g.minstd()) stg(R.rng_state + pix, g.ms);
-            }
-        }
-        if (done_count) {
-            // per request-table entry: (finished paths, shaded segments = rays) - the serving loop
-            // resolves an entry once all its paths are done; one atomic pair per distinct entry
-            // among the wave's lanes
-            const bool valid = i < n;
-            const uint32_t key = valid ? (idv.y >> 16) : 0xffffffffu;
-            uint64_t pending = __ballot(valid);
-            while (pending) {
-                const int leader = __ffsll((unsigned long long)pending) - 1;
-                const uint32_t lk = (uint32_t)__shfl((int)key, leader);
-                const uint64_t same = __ballot(key == lk);
-                const uint64_t fin = __ballot(key == lk && finished);
-                if ((int)(threadIdx.x & 63u) == leader) {
-                    const uint32_t c = lk & 7u;
-                    const uint32_t prev = atomicCAS(&s_key[c], 0xffffffffu, lk);
-                    if (prev == 0xffffffffu || prev == lk) {
-                        atomicAdd(&s_cnt[2 * c + 1], (uint32_t)__popcll(same));
-                        if (fin) atomicAdd(&s_cnt[2 * c], (uint32_t)__popcll(fin));
-                    } else {
-                        atomicAdd(&done_count[2 * lk + 1], (uint32_t)__popcll(same));
-                        if (fin) atomicAdd(&done_count[2 * lk], (uint32_t)__popcll(fin));
-                    }
-                }
-                pending &= ~same;
             }
         }
         // survivors go to the front of the other state buffer (grouped by direction octant when
@@ -1284,14 +1255,19 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
             Pout.ids[pos] = idv;
         }
     }
-    if (done_count) {
+    if (live_min) {
+        // per block: the smallest live path id after this stage (0xffffffff: none). Every path id
+        // below both this minimum over all blocks and the generated count has finished, which is
+        // how the serving loop knows a request-table entry is complete (no per-path counters).
+        uint32_t m = min_pid;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off));
+        if ((threadIdx.x & 63u) == 0u) s_min[threadIdx.x >> 6] = m;
         __syncthreads();
-        if (threadIdx.x < 8) {
-            const uint32_t k = s_key[threadIdx.x];
-            if (k != 0xffffffffu) {
-                if (s_cnt[2 * threadIdx.x + 1]) atomicAdd(&done_count[2 * k + 1], s_cnt[2 * threadIdx.x + 1]);
-                if (s_cnt[2 * threadIdx.x]) atomicAdd(&done_count[2 * k], s_cnt[2 * threadIdx.x]);
-            }
+        if (threadIdx.x == 0) {
+            uint32_t b = s_min[0];
+            for (int w = 1; w < BLOCK / 64; ++w) b = min(b, s_min[w]);
+            live_min[blockIdx.x] = b;
         }
     }
     wave_add(&counters->light_tests, light_tests);
@@ -1358,23 +1334,32 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
 }
 
 // Per-pixel sums in sample order: accum += rad[sample 0] + ... (same float additions as the
-// reference's pixel_color += ray_color(...) loop, core/src/gpu.cu:776-790).
+// reference's pixel_color += ray_color(...) loop, core/src/gpu.cu:776-790). With `rays` set (one
+// request), the paths' closest-hit queries (rad.w) are added to *rays: one atomic per wave.
 __global__ __launch_bounds__(256) void k_resolve(const crt_dev_request* __restrict__ reqs, int nreq,
-                                                 const float4* __restrict__ rad) {
+                                                 const float4* __restrict__ rad, unsigned long long* __restrict__ rays) {
     int r = blockIdx.y;
     if (r >= nreq) return;
     const crt_dev_request& R = reqs[r];
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < R.pix_count; q += gridDim.x * blockDim.x) {
+    uint32_t nr = 0;
+    // block-uniform trip count: every lane reaches the wave sum
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t iters = (R.pix_count + stride - 1) / stride;
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= R.pix_count) continue;
         uint32_t pix = R.pix_begin + q;
         float4* a = (float4*)R.accum + pix;
         float4 acc = ldg(a);
         for (uint32_t k = 0; k < R.sample_count; ++k) {
             float4 v = rad[R.path_base + k * R.pix_count + q];
             acc.x += v.x; acc.y += v.y; acc.z += v.z;
+            nr += (uint32_t)v.w;
         }
         acc.w += (float)R.sample_count;
         stg(a, acc);
     }
+    if (rays) wave_add(rays, nr);
 }
 
 // prepare_pixel_color + quantize (core/src/misc/color.cuh:289-309, core/src/gpu.cu:792-799).
@@ -1502,10 +1487,10 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
 extern "C" int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                            const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout,
                            uint32_t* count_out, const float4* hit, const uint32_t* hit_inst, float4* out_rad,
-                           uint32_t* done_count, int sort_octant, crt_dev_counters* counters) {
+                           uint32_t* live_min, int sort_octant, crt_dev_counters* counters) {
 #define CRT_SH(M, F)                                                                                                      \
     hipLaunchKernelGGL((k_shade<M, 256, F>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout, count_out, \
-                       hit, hit_inst, out_rad, done_count, sort_octant, counters)
+                       hit, hit_inst, out_rad, live_min, sort_octant, counters)
 #define CRT_SH_F(M) { if (two_level) CRT_SH(M, false); else CRT_SH(M, true); }
     if (rng_mode == CRT_RNG_MINSTD) CRT_SH_F(CRT_RNG_MINSTD)
     else if (rng_mode == CRT_RNG_PCG) CRT_SH_F(CRT_RNG_PCG)
@@ -1542,12 +1527,13 @@ extern "C" int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level
     return (int)hipGetLastError();
 }
 
-extern "C" int crt_k_resolve(hipStream_t s, const crt_dev_request* reqs, int nreq, uint32_t max_pix, const float4* rad) {
+extern "C" int crt_k_resolve(hipStream_t s, const crt_dev_request* reqs, int nreq, uint32_t max_pix, const float4* rad,
+                             unsigned long long* rays) {
     if (nreq == 0) return 0;
     uint32_t gx = (max_pix + 255) / 256;
     if (gx > 4096) gx = 4096;
     if (gx == 0) gx = 1;
-    hipLaunchKernelGGL(k_resolve, dim3(gx, nreq), dim3(256), 0, s, reqs, nreq, rad);
+    hipLaunchKernelGGL(k_resolve, dim3(gx, nreq), dim3(256), 0, s, reqs, nreq, rad, nreq == 1 ? rays : nullptr);
     return (int)hipGetLastError();
 }
 

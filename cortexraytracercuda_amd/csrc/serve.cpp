@@ -79,7 +79,10 @@ int Context::server_init() {
     if (S.ready) return CRT_OK;
     int rc;
     S.slots = max_paths;
-    S.cap = std::max<uint32_t>(pass_capacity, S.slots);
+    // path ids of the open pass: a pass that runs out of ids drains (a tail of part-empty launches)
+    // and starts over, so the serving loop takes a larger id space than a blocking pass by default
+    S.cap = config.pass_paths ? std::max<uint32_t>(pass_capacity, S.slots)
+                              : (uint32_t)std::max<uint64_t>(std::min<uint64_t>(1ull << 29, 32ull * S.slots), S.slots);
     HIP_TRY(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&S.copy_s, hipStreamNonBlocking));
     const size_t n = S.slots;
@@ -95,15 +98,16 @@ int Context::server_init() {
     if ((rc = S.out_rad.alloc((size_t)S.cap * 16, this))) return rc;
     if ((rc = S.req.alloc(sizeof(crt_dev_request) * Server::kEntries, this))) return rc;
     if ((rc = S.ctl.alloc(sizeof(uint32_t) * (2 * CRT_CTL_WORDS + 2), this))) return rc;
-    if ((rc = S.stats.alloc(sizeof(uint32_t) * 2 * Server::kEntries, this))) return rc;
     S.use_graphs = config.graphs != 2u;
     S.egrid = std::min(extend_grid, std::max(1u, (S.slots + 255) / 256));
-    S.sgrid = std::min(shade_grid, std::max(1u, (S.slots + 255) / 256));
+    S.sgrid = std::min(std::min(shade_grid, (uint32_t)Server::kMaxShadeGrid), std::max(1u, (S.slots + 255) / 256));
+    if ((rc = S.live_min.alloc(sizeof(uint32_t) * S.sgrid, this))) return rc;
     if ((rc = S.gstack.alloc((size_t)S.egrid * 256u * crt_k_gstack_entries() * sizeof(uint32_t), this))) return rc;
     if ((rc = S.counters.alloc(sizeof(crt_dev_counters), this))) return rc;
     HIP_TRY(hipHostMalloc((void**)&S.req_host, sizeof(crt_dev_request) * Server::kEntries, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&S.open_host, sizeof(uint32_t) * 2 * Server::kOpenRing, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&S.snap, sizeof(uint32_t) * Server::kSnapWords * Server::kRing, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&S.rays_host, sizeof(unsigned long long), hipHostMallocDefault));
     for (int k = 0; k < Server::kRing; ++k) HIP_TRY(hipEventCreateWithFlags(&S.ev[k], hipEventDisableTiming));
     if (crt_k_fill_u32(S.s, (uint32_t*)S.counters.p, sizeof(crt_dev_counters) / 4, 0u))
         return set_error(CRT_ERR_HIP, "fill launch failed");
@@ -115,8 +119,8 @@ int Context::server_init() {
 
 uint64_t Context::server_bytes() const {
     const Server& S = srv;
-    uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.stats.bytes +
-                 S.gstack.bytes + S.counters.bytes;
+    uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.gstack.bytes +
+                 S.counters.bytes + S.live_min.bytes;
     for (int k = 0; k < 2; ++k) b += S.ray_o[k].bytes + S.ray_d[k].bytes + S.thr[k].bytes + S.rad[k].bytes + S.ids[k].bytes;
     return b;
 }
@@ -128,13 +132,14 @@ void Context::server_release() {
     if (S.req_host) (void)hipHostFree(S.req_host);
     if (S.open_host) (void)hipHostFree(S.open_host);
     if (S.snap) (void)hipHostFree(S.snap);
+    if (S.rays_host) (void)hipHostFree(S.rays_host);
     for (int k = 0; k < Server::kRing; ++k) if (S.ev[k]) (void)hipEventDestroy(S.ev[k]);
     for (int t = 0; t < 4; ++t)
         for (int k = 0; k < Server::kRing; ++k)
             if (S.graph[t][k]) { (void)hipGraphExecDestroy(S.graph[t][k]); S.graph[t][k] = nullptr; }
     if (S.s) (void)hipStreamDestroy(S.s);
     if (S.copy_s) (void)hipStreamDestroy(S.copy_s);
-    S.req_host = nullptr; S.open_host = nullptr; S.snap = nullptr; S.s = nullptr; S.copy_s = nullptr;
+    S.req_host = nullptr; S.open_host = nullptr; S.snap = nullptr; S.rays_host = nullptr; S.s = nullptr; S.copy_s = nullptr;
     S.ready = false;
 }
 
@@ -147,12 +152,12 @@ void Context::server_reset_pass() {
     S.processed = 0;
     S.done_paths = 0;
     S.gen_seen = 0;
+    S.gen_progress = 0;
     S.progress_iter = 0;
     S.full = false;
     S.two_level = false;
     S.mixed_rng = false;
     (void)crt_k_pass_init(S.s, (uint32_t*)S.ctl.p, 2 * CRT_CTL_WORDS + 2);
-    (void)crt_k_fill_u32(S.s, (uint32_t*)S.stats.p, 2 * Server::kEntries, 0u);
 }
 
 static void complete(ServeReq* r, int status, std::condition_variable& cv) {
@@ -191,6 +196,7 @@ int Context::server_append(std::vector<ServeReq*>& act) {
         en.r = r;
         en.seg = r->segs++;
         en.paths = w.pix_count * w.sample_count;
+        en.end = S.total + en.paths;
         S.entries.push_back(en);
         S.total += en.paths;
         if (r->sc->host.num_instances) S.two_level = true;
@@ -241,14 +247,13 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
                      C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
     if (crt_k_shade(S.s, mode, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
-                    (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.stats.p,
+                    (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.live_min.p,
                     (int)C.ray_sort, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "shade launch failed");
     uint32_t* snap = S.snap + (size_t)(k % Server::kRing) * Server::kSnapWords;
     HIP_TRY(hipMemcpyAsync(snap, cout, sizeof(uint32_t) * CRT_CTL_WORDS, hipMemcpyDeviceToHost, S.s));
     HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS, S.counters.p, sizeof(crt_dev_counters), hipMemcpyDeviceToHost, S.s));
-    HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS + 20, S.stats.p, sizeof(uint32_t) * 2 * Server::kEntries,
-                           hipMemcpyDeviceToHost, S.s));
+    HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS + 20, S.live_min.p, sizeof(uint32_t) * S.sgrid, hipMemcpyDeviceToHost, S.s));
     return CRT_OK;
 }
 
@@ -276,7 +281,6 @@ int Context::server_iteration() {
         if ((rc = enqueue_iteration(*this, S, k))) return rc;
         S.warm[tl]++;
     }
-    S.snap_ne[slot] = (uint32_t)S.entries.size();
     HIP_TRY(hipEventRecord(S.ev[slot], S.s));
     S.iter++;
     iterations++;
@@ -296,25 +300,27 @@ int Context::server_process_snapshot(std::vector<ServeReq*>& act) {
     crt_dev_counters c;
     std::memcpy(&c, snap + CRT_CTL_WORDS, sizeof(c));
     if (c.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
-    const uint32_t* st = snap + CRT_CTL_WORDS + 20;
-    // entries appended after this iteration was enqueued are not in its snapshot
-    const size_t ns = S.snap_ne[j % Server::kRing];
+    // Completion without per-path counters: path ids are generated in order, so after iteration j
+    // every id below GEN (generated so far) that is not held by a surviving path has finished and
+    // written its radiance. The shade stage leaves the smallest surviving id per workgroup; an
+    // entry is done once GEN and that minimum have both passed its last id.
+    const uint32_t gen = snap[CRT_CTL_GEN];
+    const uint32_t* lm = snap + CRT_CTL_WORDS + 20;
+    uint32_t live = 0xffffffffu;
+    for (uint32_t b = 0; b < S.sgrid; ++b) live = std::min(live, lm[b]);
+    const uint32_t finished_below = std::min(gen, live);
     const size_t ne = S.entries.size();
     uint64_t done = 0;
     for (size_t e = 0; e < ne; ++e) {
         SrvEntry& en = S.entries[e];
-        if (e < ns) {
-            uint32_t d = std::min(st[2 * e], en.paths), ry = st[2 * e + 1];
-            if (d > en.done) en.done = d;
-            if (ry > en.rays) { en.r->timing.rays += ry - en.rays; en.rays = ry; }
-        }
-        done += en.done;
+        if (!en.done && en.end <= finished_below) en.done = true;
+        if (en.done) done += en.paths;
     }
-    if (done > S.done_paths) S.progress_iter = S.iter;
+    if (gen > S.gen_progress) { S.gen_progress = gen; S.progress_iter = S.iter; }
     S.done_paths = done;
-    // every appended path enters within a few iterations (the backlog is bounded) and finishes
-    // within max_depth iterations of entering; a pass that stops finishing paths is an error
-    if ((uint64_t)S.total > S.done_paths && S.iter - S.progress_iter > 4096)
+    // appended paths are generated within a few iterations (the backlog is bounded): a pass whose
+    // generation stops while ids are outstanding is an error
+    if (gen < S.total && S.iter - S.progress_iter > 4096)
         return set_error(CRT_ERR_INTERNAL, "serving pass did not drain");
     // resolve finished segments in order per request
     bool progress = true;
@@ -322,11 +328,11 @@ int Context::server_process_snapshot(std::vector<ServeReq*>& act) {
         progress = false;
         for (size_t e = 0; e < ne; ++e) {
             SrvEntry& en = S.entries[e];
-            if (en.resolved || en.done < en.paths || en.r->segs_resolved != en.seg) continue;
+            if (en.resolved || !en.done || en.r->segs_resolved != en.seg) continue;
             ServeReq* r = en.r;
             if (!r->cancel) {
                 if (crt_k_resolve(S.s, (const crt_dev_request*)S.req.p + e, 1, S.req_host[e].pix_count,
-                                  (const float4*)S.out_rad.p))
+                                  (const float4*)S.out_rad.p, r->job->rays_counter()))
                     return set_error(CRT_ERR_HIP, "resolve launch failed");
             }
             en.resolved = true;
@@ -440,11 +446,15 @@ void Context::server_loop() {
                 const bool want_accum = !(r->rd.flags & CRT_RENDER_IMAGE_ONLY);
                 r->rgb.resize((size_t)r->job->npix * 3);
                 r->accum.resize(want_accum ? (size_t)r->job->npix * 4 : 0);
-                if (hipMemcpyAsync(r->rgb.data(), r->job->rgb.p, r->rgb.size(), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
+                unsigned long long* rays_h = (unsigned long long*)S.rays_host;
+                if (hipMemcpyAsync(rays_h, r->job->rays_counter(), sizeof(*rays_h), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
+                    hipMemcpyAsync(r->rgb.data(), r->job->rgb.p, r->rgb.size(), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
                     (want_accum && hipMemcpyAsync(r->accum.data(), r->job->accum.p, r->accum.size() * sizeof(float),
                                                   hipMemcpyDeviceToHost, S.copy_s) != hipSuccess) ||
                     hipStreamSynchronize(S.copy_s) != hipSuccess)
                     st = set_error(CRT_ERR_HIP, "readback failed");
+                else
+                    r->timing.rays = *rays_h;
             }
             if (r->ev_done) { (void)hipEventDestroy(r->ev_done); r->ev_done = nullptr; }
             if (err) (void)hipStreamSynchronize(S.s);

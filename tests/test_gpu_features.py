@@ -127,19 +127,21 @@ def test_serving_alone_vs_mix_and_cancel():
     with Renderer(max_paths=200000) as r:
         a = r.upload(scenes.sphere_mesh_scene(5000, seed=1))
         b = r.upload(scenes.box_scene())
-        ref_a = r.render(a, 128, 128, 16, 8, seed=21)[1]
-        ref_b = r.render(b, 96, 64, 4, 8, seed=22)[1]
+        _, ref_a, st_a = r.render(a, 128, 128, 16, 8, seed=21)
+        _, ref_b, st_b = r.render(b, 96, 64, 4, 8, seed=22)
         ids = []
         for k in range(6):
-            ids.append((r.submit(a, 128, 128, 16, 8, seed=21), ref_a, 128, 128))
-            ids.append((r.submit(b, 96, 64, 4, 8, seed=22), ref_b, 96, 64))
+            ids.append((r.submit(a, 128, 128, 16, 8, seed=21), ref_a, st_a["rays"], 128, 128))
+            ids.append((r.submit(b, 96, 64, 4, 8, seed=22), ref_b, st_b["rays"], 96, 64))
         victim = r.submit(a, 256, 256, 64, 8, seed=99)
         r.cancel(victim)
-        for rid, ref, W, H in ids:
+        for rid, ref, rays, W, H in ids:
             rgb, acc, st = r.wait(rid, W, H)
             assert np.array_equal(acc, ref)
             tm = r.timing(rid)
             assert tm["status"] == 0 and tm["done_s"] >= tm["submit_s"] and tm["passes"] >= 1
+            # per-request closest-hit queries (summed at resolve) equal the blocking render's count
+            assert tm["rays"] == rays and st["rays"] == rays
         with pytest.raises(A.CrtError) as e:
             r.wait(victim, 256, 256)
         assert e.value.code == A.CRT_ERR_CANCELLED
