@@ -194,6 +194,11 @@ struct Context {
     // serving
     std::mutex srv_mutex;
     std::condition_variable srv_cv, srv_done_cv;
+    // finished requests waiting for their readback; a completion thread copies them back on the
+    // server's copy stream so the scheduler keeps the open pass fed meanwhile
+    std::deque<std::pair<ServeReq*, int>> srv_copy_q;
+    std::condition_variable srv_copy_cv;
+    bool srv_copy_stop = false;
     std::thread srv_thread;
     bool srv_running = false, srv_stop = false;
     std::deque<ServeReq*> srv_waiting;                 // submitted, not admitted
@@ -226,6 +231,8 @@ struct Context {
     // serving
     int submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_id* out);
     void server_loop();
+    void server_copy_loop();
+    void server_complete(ServeReq* r, int status);
     void stop_server();
     int server_init();
     void server_release();

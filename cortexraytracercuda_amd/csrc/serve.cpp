@@ -355,9 +355,59 @@ int Context::server_process_snapshot(std::vector<ServeReq*>& act) {
     return CRT_OK;
 }
 
+// Readback and completion of one finished request (completion thread; status != CRT_OK: no copies).
+void Context::server_complete(ServeReq* r, int st) {
+    Server& S = srv;
+    if (st == CRT_OK) {
+        // the accumulation buffer (16 B per pixel against 3 for the image) only when asked for
+        const bool want_accum = !(r->rd.flags & CRT_RENDER_IMAGE_ONLY);
+        r->rgb.resize((size_t)r->job->npix * 3);
+        r->accum.resize(want_accum ? (size_t)r->job->npix * 4 : 0);
+        unsigned long long* rays_h = (unsigned long long*)S.rays_host;
+        if (hipMemcpyAsync(rays_h, r->job->rays_counter(), sizeof(*rays_h), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
+            hipMemcpyAsync(r->rgb.data(), r->job->rgb.p, r->rgb.size(), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
+            (want_accum && hipMemcpyAsync(r->accum.data(), r->job->accum.p, r->accum.size() * sizeof(float),
+                                          hipMemcpyDeviceToHost, S.copy_s) != hipSuccess) ||
+            hipStreamSynchronize(S.copy_s) != hipSuccess)
+            st = set_error(CRT_ERR_HIP, "readback failed");
+        else
+            r->timing.rays = *rays_h;
+    }
+    if (r->ev_done) { (void)hipEventDestroy(r->ev_done); r->ev_done = nullptr; }
+    {
+        // lock order: gpu_mutex is never taken while srv_mutex is held
+        std::lock_guard<std::mutex> g(gpu_mutex);
+        r->job.reset();
+    }
+    std::lock_guard<std::mutex> lk(srv_mutex);
+    request_bytes -= Job::bytes_needed(r->rd);
+    r->sc->refs--;
+    complete(r, st, srv_done_cv);
+    srv_cv.notify_all();   // the budget it held may admit a waiting request
+}
+
+void Context::server_copy_loop() {
+    for (;;) {
+        std::pair<ServeReq*, int> job;
+        {
+            std::unique_lock<std::mutex> lk(srv_mutex);
+            srv_copy_cv.wait(lk, [&]() { return srv_copy_stop || !srv_copy_q.empty(); });
+            if (srv_copy_q.empty()) return;   // stop requested and nothing left
+            job = srv_copy_q.front();
+            srv_copy_q.pop_front();
+        }
+        server_complete(job.first, job.second);
+    }
+}
+
 void Context::server_loop() {
     Server& S = srv;
     int err = CRT_OK;
+    {
+        std::lock_guard<std::mutex> lk(srv_mutex);
+        srv_copy_stop = false;
+    }
+    std::thread copier([this]() { server_copy_loop(); });
     for (;;) {
         std::vector<ServeReq*> admitted, act;
         const bool inflight = S.processed < S.iter;
@@ -439,35 +489,16 @@ void Context::server_loop() {
             if (q == hipErrorNotReady) continue;
             finished.push_back(r);
         }
-        for (ServeReq* r : finished) {
-            int st = err ? err : (r->cancel ? CRT_ERR_CANCELLED : CRT_OK);
-            if (st == CRT_OK) {
-                // the accumulation buffer (16 B per pixel against 3 for the image) only when asked for
-                const bool want_accum = !(r->rd.flags & CRT_RENDER_IMAGE_ONLY);
-                r->rgb.resize((size_t)r->job->npix * 3);
-                r->accum.resize(want_accum ? (size_t)r->job->npix * 4 : 0);
-                unsigned long long* rays_h = (unsigned long long*)S.rays_host;
-                if (hipMemcpyAsync(rays_h, r->job->rays_counter(), sizeof(*rays_h), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
-                    hipMemcpyAsync(r->rgb.data(), r->job->rgb.p, r->rgb.size(), hipMemcpyDeviceToHost, S.copy_s) != hipSuccess ||
-                    (want_accum && hipMemcpyAsync(r->accum.data(), r->job->accum.p, r->accum.size() * sizeof(float),
-                                                  hipMemcpyDeviceToHost, S.copy_s) != hipSuccess) ||
-                    hipStreamSynchronize(S.copy_s) != hipSuccess)
-                    st = set_error(CRT_ERR_HIP, "readback failed");
-                else
-                    r->timing.rays = *rays_h;
-            }
-            if (r->ev_done) { (void)hipEventDestroy(r->ev_done); r->ev_done = nullptr; }
-            if (err) (void)hipStreamSynchronize(S.s);
-            {
-                // lock order: gpu_mutex is never taken while srv_mutex is held
-                std::lock_guard<std::mutex> g(gpu_mutex);
-                r->job.reset();
-            }
+        if (err && !finished.empty()) (void)hipStreamSynchronize(S.s);
+        if (!finished.empty()) {
+            // hand the finished requests to the completion thread (readback, release, completion)
             std::lock_guard<std::mutex> lk(srv_mutex);
-            request_bytes -= Job::bytes_needed(r->rd);
-            srv_active.erase(std::find(srv_active.begin(), srv_active.end(), r));
-            r->sc->refs--;
-            complete(r, st, srv_done_cv);
+            for (ServeReq* r : finished) {
+                const int st = err ? err : (r->cancel ? CRT_ERR_CANCELLED : CRT_OK);
+                srv_active.erase(std::find(srv_active.begin(), srv_active.end(), r));
+                srv_copy_q.emplace_back(r, st);
+            }
+            srv_copy_cv.notify_all();
         }
         if (err) {
             // a failed iteration leaves the open pass unusable: wait for the stream, start over
@@ -478,6 +509,12 @@ void Context::server_loop() {
         }
     }
     (void)hipStreamSynchronize(S.s);
+    {
+        std::lock_guard<std::mutex> lk(srv_mutex);
+        srv_copy_stop = true;
+    }
+    srv_copy_cv.notify_all();
+    copier.join();   // drains the queue first
 }
 
 }  // namespace crt
