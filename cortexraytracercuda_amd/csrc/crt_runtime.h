@@ -102,6 +102,8 @@ struct ServeReq {
     uint32_t segs = 0;             // segments appended to the open pass
     uint32_t segs_resolved = 0;    // segments whose resolve has been enqueued (in segment order)
     bool finishing = false;        // tonemap + readback enqueued, waiting for ev_done
+    int lane = 0;                  // open pass (server lane) that traces all of its segments
+    uint64_t load = 0;             // paths it adds to its lane's load until it completes
     hipEvent_t ev_done = nullptr;
 };
 
@@ -205,7 +207,12 @@ struct Context {
     std::vector<ServeReq*> srv_active;                 // admitted, on the device
     std::map<crt_request_id, std::unique_ptr<ServeReq>> srv_all;
     crt_request_id next_req_id = 1;
-    Server srv;
+    // up to kSrvLanes open passes on their own streams (each with a share of the slot pool), so one
+    // pass's drain tail overlaps the other's full launches; a request's segments stay in one lane
+    static constexpr int kSrvLanes = 2;
+    Server srv[kSrvLanes];
+    uint32_t srv_lanes = 1;
+    uint64_t lane_load[kSrvLanes] = {};
 
     int init(const crt_config* cfg);
     ~Context();
@@ -235,11 +242,14 @@ struct Context {
     void server_complete(ServeReq* r, int status);
     void stop_server();
     int server_init();
+    int server_init_lane(Server& S, uint32_t slots, uint32_t cap, bool first);
+    uint64_t server_lane_bytes(const Server& S) const;
     void server_release();
-    int server_iteration();
-    int server_append(std::vector<ServeReq*>& act);
-    int server_process_snapshot(std::vector<ServeReq*>& act);
-    void server_reset_pass();
+    void server_release_lane(Server& S);
+    int server_iteration(Server& S);
+    int server_append(Server& S, std::vector<ServeReq*>& act);
+    int server_process_snapshot(Server& S, std::vector<ServeReq*>& act);
+    void server_reset_pass(Server& S);
 };
 
 void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms);

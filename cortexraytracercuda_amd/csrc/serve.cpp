@@ -75,16 +75,28 @@ void Context::stop_server() {
 
 // Server buffers, allocated at the first submit (caller holds gpu_mutex). Counted as reserved.
 int Context::server_init() {
-    Server& S = srv;
-    if (S.ready) return CRT_OK;
+    if (srv[0].ready) return CRT_OK;
+    srv_lanes = std::max(1u, std::min<uint32_t>(sub_passes, kSrvLanes));
+    // path ids of the open passes: a pass that runs out of ids drains (a tail of part-empty
+    // launches) and starts over, so the serving loop takes a larger id space than a blocking pass
+    const uint32_t slots_all = max_paths;
+    const uint64_t cap_all = config.pass_paths ? std::max<uint32_t>(pass_capacity, slots_all)
+                                               : std::max<uint64_t>(std::min<uint64_t>(1ull << 29, 32ull * slots_all), slots_all);
+    for (uint32_t l = 0; l < srv_lanes; ++l) {
+        int rc = server_init_lane(srv[l], std::max(1u, slots_all / srv_lanes),
+                                  (uint32_t)std::max<uint64_t>(cap_all / srv_lanes, slots_all / srv_lanes), l == 0);
+        if (rc) return rc;
+    }
+    reserved = used_bytes() + server_bytes();
+    return CRT_OK;
+}
+
+int Context::server_init_lane(Server& S, uint32_t slots, uint32_t cap, bool first) {
     int rc;
-    S.slots = max_paths;
-    // path ids of the open pass: a pass that runs out of ids drains (a tail of part-empty launches)
-    // and starts over, so the serving loop takes a larger id space than a blocking pass by default
-    S.cap = config.pass_paths ? std::max<uint32_t>(pass_capacity, S.slots)
-                              : (uint32_t)std::max<uint64_t>(std::min<uint64_t>(1ull << 29, 32ull * S.slots), S.slots);
+    S.slots = std::max(1u, slots);
+    S.cap = std::max(cap, S.slots);
     HIP_TRY(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&S.copy_s, hipStreamNonBlocking));
+    if (first) HIP_TRY(hipStreamCreateWithFlags(&S.copy_s, hipStreamNonBlocking));
     const size_t n = S.slots;
     for (int k = 0; k < 2; ++k) {
         if ((rc = S.ray_o[k].alloc(n * 16, this))) return rc;
@@ -107,18 +119,22 @@ int Context::server_init() {
     HIP_TRY(hipHostMalloc((void**)&S.req_host, sizeof(crt_dev_request) * Server::kEntries, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&S.open_host, sizeof(uint32_t) * 2 * Server::kOpenRing, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&S.snap, sizeof(uint32_t) * Server::kSnapWords * Server::kRing, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc(&S.rays_host, sizeof(unsigned long long), hipHostMallocDefault));
+    if (first) HIP_TRY(hipHostMalloc(&S.rays_host, sizeof(unsigned long long), hipHostMallocDefault));
     for (int k = 0; k < Server::kRing; ++k) HIP_TRY(hipEventCreateWithFlags(&S.ev[k], hipEventDisableTiming));
     if (crt_k_fill_u32(S.s, (uint32_t*)S.counters.p, sizeof(crt_dev_counters) / 4, 0u))
         return set_error(CRT_ERR_HIP, "fill launch failed");
     S.ready = true;
-    server_reset_pass();
-    reserved = used_bytes() + server_bytes();
+    server_reset_pass(S);
     return CRT_OK;
 }
 
 uint64_t Context::server_bytes() const {
-    const Server& S = srv;
+    uint64_t b = 0;
+    for (const Server& S : srv) b += server_lane_bytes(S);
+    return b;
+}
+
+uint64_t Context::server_lane_bytes(const Server& S) const {
     uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.gstack.bytes +
                  S.counters.bytes + S.live_min.bytes;
     for (int k = 0; k < 2; ++k) b += S.ray_o[k].bytes + S.ray_d[k].bytes + S.thr[k].bytes + S.rad[k].bytes + S.ids[k].bytes;
@@ -126,7 +142,10 @@ uint64_t Context::server_bytes() const {
 }
 
 void Context::server_release() {
-    Server& S = srv;
+    for (Server& S : srv) server_release_lane(S);
+}
+
+void Context::server_release_lane(Server& S) {
     if (S.s) (void)hipStreamSynchronize(S.s);
     if (S.copy_s) (void)hipStreamSynchronize(S.copy_s);
     if (S.req_host) (void)hipHostFree(S.req_host);
@@ -144,8 +163,7 @@ void Context::server_release() {
 }
 
 // A new open pass: no entries, no path ids; control block and per-entry stats zeroed on the stream.
-void Context::server_reset_pass() {
-    Server& S = srv;
+void Context::server_reset_pass(Server& S) {
     S.entries.clear();
     S.total = 0;
     S.iter = 0;
@@ -171,8 +189,7 @@ static void complete(ServeReq* r, int status, std::condition_variable& cv) {
 // is below two slot pools (a new request starts within a few iterations). Round-robin, one segment
 // per request and call; MINSTD requests keep at most one segment in flight (each pixel's engine
 // advances sample after sample).
-int Context::server_append(std::vector<ServeReq*>& act) {
-    Server& S = srv;
+int Context::server_append(Server& S, std::vector<ServeReq*>& act) {
     if (act.empty() || S.full) return CRT_OK;
     const uint32_t seg_cap = std::max<uint32_t>(S.slots / 2, 1u);
     const uint32_t n0 = (uint32_t)S.entries.size();
@@ -257,8 +274,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
     return CRT_OK;
 }
 
-int Context::server_iteration() {
-    Server& S = srv;
+int Context::server_iteration(Server& S) {
     const uint64_t k = S.iter;
     const int slot = (int)(k % Server::kRing);
     const int tl = (S.two_level ? 2 : 0) + (S.mixed_rng ? 1 : 0);
@@ -290,8 +306,7 @@ int Context::server_iteration() {
 // Reads the oldest outstanding snapshot (waits for its iteration) and enqueues the resolves of
 // segments whose paths have all finished, in segment order per request; a request whose last
 // segment is resolved gets its tonemap and a completion event.
-int Context::server_process_snapshot(std::vector<ServeReq*>& act) {
-    Server& S = srv;
+int Context::server_process_snapshot(Server& S, std::vector<ServeReq*>& act) {
     const uint64_t j = S.processed;
     HIP_TRY(hipEventSynchronize(S.ev[j % Server::kRing]));
     const uint32_t* snap = S.snap + (size_t)(j % Server::kRing) * Server::kSnapWords;
@@ -357,7 +372,7 @@ int Context::server_process_snapshot(std::vector<ServeReq*>& act) {
 
 // Readback and completion of one finished request (completion thread; status != CRT_OK: no copies).
 void Context::server_complete(ServeReq* r, int st) {
-    Server& S = srv;
+    Server& S = srv[0];   // owns the copy stream and the pinned ray-count word
     if (st == CRT_OK) {
         // the accumulation buffer (16 B per pixel against 3 for the image) only when asked for
         const bool want_accum = !(r->rd.flags & CRT_RENDER_IMAGE_ONLY);
@@ -401,19 +416,27 @@ void Context::server_copy_loop() {
 }
 
 void Context::server_loop() {
-    Server& S = srv;
+    const uint32_t L = srv_lanes;
     int err = CRT_OK;
     {
         std::lock_guard<std::mutex> lk(srv_mutex);
         srv_copy_stop = false;
     }
     std::thread copier([this]() { server_copy_loop(); });
+    auto lanes_inflight = [&]() {
+        for (uint32_t l = 0; l < L; ++l) if (srv[l].processed < srv[l].iter) return true;
+        return false;
+    };
+    auto lanes_work = [&]() {
+        for (uint32_t l = 0; l < L; ++l) if ((uint64_t)srv[l].total > srv[l].done_paths) return true;
+        return false;
+    };
     for (;;) {
         std::vector<ServeReq*> admitted, act;
-        const bool inflight = S.processed < S.iter;
+        const bool inflight = lanes_inflight();
         {
             std::unique_lock<std::mutex> lk(srv_mutex);
-            const bool work = (uint64_t)S.total > S.done_paths;
+            const bool work = lanes_work();
             srv_cv.wait(lk, [&]() { return srv_stop || !srv_waiting.empty() || !srv_active.empty() || inflight || work; });
             // drop cancelled waiting requests
             for (auto it = srv_waiting.begin(); it != srv_waiting.end();) {
@@ -427,12 +450,19 @@ void Context::server_loop() {
                 }
             }
             if (srv_stop && srv_active.empty() && !inflight) break;
-            // first-fit admission against the budget
+            // first-fit admission against the budget; each request joins the lane with the fewest
+            // outstanding paths
             for (auto it = srv_waiting.begin(); it != srv_waiting.end();) {
                 ServeReq* r = *it;
                 uint64_t need = Job::bytes_needed(r->rd);
                 if (reserved + scene_bytes + request_bytes + need <= budget) {
                     request_bytes += need;
+                    uint32_t best = 0;
+                    for (uint32_t l = 1; l < L; ++l) if (lane_load[l] < lane_load[best]) best = l;
+                    const uint64_t sq = (uint64_t)std::sqrt((double)r->rd.spp);
+                    r->lane = (int)best;
+                    r->load = (uint64_t)r->rd.width * r->rd.height * sq * sq;
+                    lane_load[best] += r->load;
                     admitted.push_back(r);
                     it = srv_waiting.erase(it);
                 } else {
@@ -445,12 +475,13 @@ void Context::server_loop() {
             {
                 std::lock_guard<std::mutex> g(gpu_mutex);
                 r->job = std::make_unique<Job>();
-                rc = r->job->setup(this, r->sc, r->rd, S.s);
+                rc = r->job->setup(this, r->sc, r->rd, srv[r->lane].s);
             }
             r->timing.admit_s = now_s();
             std::lock_guard<std::mutex> lk(srv_mutex);
             if (rc) {
                 request_bytes -= Job::bytes_needed(r->rd);
+                lane_load[r->lane] -= r->load;
                 r->job.reset();
                 r->sc->refs--;
                 complete(r, rc, srv_done_cv);
@@ -462,25 +493,48 @@ void Context::server_loop() {
             std::lock_guard<std::mutex> lk(srv_mutex);
             act = srv_active;
         }
-        if (!err) err = server_append(act);
-        // keep up to kLag iterations ahead while paths are outstanding
-        const bool outstanding = (uint64_t)S.total > S.done_paths;
-        bool enqueued = false;
-        if (!err && outstanding && S.iter - S.processed < (uint64_t)Server::kLag) {
-            err = server_iteration();
-            enqueued = true;
+        bool busy = !admitted.empty(), any_step = false;
+        std::vector<ServeReq*> lane_act[kSrvLanes];
+        for (uint32_t l = 0; l < L && !err; ++l) {
+            Server& S = srv[l];
+            std::vector<ServeReq*>& act_l = lane_act[l];
+            for (ServeReq* r : act) if (r->lane == (int)l) act_l.push_back(r);
+            const uint64_t processed0 = S.processed;
+            err = server_append(S, act_l);
+            // keep up to kLag iterations ahead while paths are outstanding
+            const bool outstanding = (uint64_t)S.total > S.done_paths;
+            bool enqueued = false;
+            if (!err && outstanding && S.iter - S.processed < (uint64_t)Server::kLag) {
+                err = server_iteration(S);
+                enqueued = true;
+            }
+            // read a snapshot when this lane is kLag ahead, or when it has nothing to enqueue and no
+            // other lane does either (so one lane's wait does not hold back the other's launches)
+            if (!err && S.processed < S.iter && (S.iter - S.processed >= (uint64_t)Server::kLag || (!enqueued && L == 1)))
+                err = server_process_snapshot(S, act_l);
+            if (!err && S.processed < S.iter && !enqueued && L > 1) {
+                // poll: process the oldest snapshot only if its iteration has finished
+                if (hipEventQuery(S.ev[S.processed % Server::kRing]) == hipSuccess) err = server_process_snapshot(S, act_l);
+            }
+            // a full pass that has drained and resolved everything starts over
+            if (!err && S.processed == S.iter && (uint64_t)S.total == S.done_paths) {
+                bool all_resolved = true;
+                for (auto& en : S.entries) all_resolved = all_resolved && en.resolved;
+                if (all_resolved && (S.full || S.total > S.cap / 2 || S.entries.size() > Server::kEntries / 2))
+                    server_reset_pass(S);
+            }
+            busy = busy || enqueued || S.processed < S.iter;
+            any_step = any_step || enqueued || S.processed != processed0;
         }
-        if (!err && S.processed < S.iter && (!enqueued || S.iter - S.processed >= (uint64_t)Server::kLag))
-            err = server_process_snapshot(act);
-        // a full pass that has drained and resolved everything starts over
-        if (!err && S.processed == S.iter && (uint64_t)S.total == S.done_paths) {
-            bool all_resolved = true;
-            for (auto& en : S.entries) all_resolved = all_resolved && en.resolved;
-            if (all_resolved && (S.full || S.total > S.cap / 2 || S.entries.size() > Server::kEntries / 2)) server_reset_pass();
+        if (!err && L > 1 && !any_step) {
+            // nothing to launch and no finished iteration: wait for the oldest one in flight
+            int oldest = -1;
+            for (uint32_t l = 0; l < L; ++l)
+                if (srv[l].processed < srv[l].iter && (oldest < 0 || srv[l].processed < srv[oldest].processed)) oldest = (int)l;
+            if (oldest >= 0) err = server_process_snapshot(srv[oldest], lane_act[oldest]);
         }
         // completions: readback of finished requests whose stream work is done; with nothing else
         // to do this turn, wait for the oldest finishing request instead of polling
-        const bool busy = enqueued || S.processed < S.iter || !admitted.empty();
         std::vector<ServeReq*> finished;
         for (ServeReq* r : act) {
             if (err) { finished.push_back(r); continue; }
@@ -489,26 +543,30 @@ void Context::server_loop() {
             if (q == hipErrorNotReady) continue;
             finished.push_back(r);
         }
-        if (err && !finished.empty()) (void)hipStreamSynchronize(S.s);
+        if (err && !finished.empty())
+            for (uint32_t l = 0; l < L; ++l) (void)hipStreamSynchronize(srv[l].s);
         if (!finished.empty()) {
             // hand the finished requests to the completion thread (readback, release, completion)
             std::lock_guard<std::mutex> lk(srv_mutex);
             for (ServeReq* r : finished) {
                 const int st = err ? err : (r->cancel ? CRT_ERR_CANCELLED : CRT_OK);
                 srv_active.erase(std::find(srv_active.begin(), srv_active.end(), r));
+                lane_load[r->lane] -= r->load;
                 srv_copy_q.emplace_back(r, st);
             }
             srv_copy_cv.notify_all();
         }
         if (err) {
-            // a failed iteration leaves the open pass unusable: wait for the stream, start over
-            (void)hipStreamSynchronize(S.s);
-            S.iter = S.processed = 0;
-            server_reset_pass();
+            // a failed iteration leaves the open passes unusable: wait for the streams, start over
+            for (uint32_t l = 0; l < L; ++l) {
+                (void)hipStreamSynchronize(srv[l].s);
+                srv[l].iter = srv[l].processed = 0;
+                server_reset_pass(srv[l]);
+            }
             err = CRT_OK;
         }
     }
-    (void)hipStreamSynchronize(S.s);
+    for (uint32_t l = 0; l < L; ++l) (void)hipStreamSynchronize(srv[l].s);
     {
         std::lock_guard<std::mutex> lk(srv_mutex);
         srv_copy_stop = true;
