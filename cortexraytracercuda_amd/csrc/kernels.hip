@@ -153,9 +153,10 @@ __device__ __forceinline__ void stg(float4* p, float4 v) {
 }
 
 // ------------------------------------------------------------------ traversal stack
-// Short per-lane stack: RS entries in VGPRs (static shifts, no dynamic register indexing), spilling
-// to an LS-entry ring per lane in LDS, which in turn spills its oldest entries to a per-thread
-// overflow area in global memory. Lower levels are used only while the registers are full.
+// Short per-lane stack: RS entries in VGPRs (static shifts, no dynamic register indexing; one entry
+// is used: the top), spilling to an LS-entry ring per lane in LDS, which in turn spills its oldest
+// entries to a per-thread overflow area in global memory. A pop returns the register top at once
+// and reads the entry below it from LDS while the popped node is fetched.
 // Overflow area of one thread: the launch-wide base (kernel argument, scalar) and the thread's
 // offset in entries (one VGPR instead of a 64-bit per-lane pointer).
 struct GSlot { uint32_t* base; uint32_t off; };

@@ -1,18 +1,21 @@
 // serve.cpp - continuous-batching request loop (crt_submit / crt_poll / crt_wait / crt_cancel).
 //
 // The reference serves one render per process (UI/Examples/UI/main.cpp:371-395 spawns the exe)
-// with no queue, batching or residency. Here one scheduler thread owns a stream and a path pool
-// of its own and keeps ONE streaming pass open:
+// with no queue, batching or residency. Here one scheduler thread owns streams and path pools of
+// its own and keeps one streaming pass open per lane (two lanes, each with half the slots):
 //   - requests are admitted first-fit against the HBM budget (accumulation, tonemap and RNG
-//     buffers; scenes are already resident and counted at upload);
+//     buffers; scenes are already resident and counted at upload) and join the lane with the
+//     fewest outstanding paths;
 //   - an admitted request is cut into segments (pixel range x sample range); a segment becomes a
-//     request-table entry of the open pass and its path ids are appended behind the pass's last
-//     id between two iterations, so the next regeneration stage already starts its camera rays
-//     among the paths of every other request in flight;
-//   - the shade stage counts finished paths per entry; once all paths of a segment are done the
-//     host enqueues its resolve (segments of one request in order), and after the last one the
-//     tonemap and the readback: a request completes independently of the others;
-//   - when the pass has no room left (path ids or table entries) it drains and starts over.
+//     request-table entry of its lane's open pass and its path ids are appended behind the pass's
+//     last id between two iterations, so the next regeneration stage already starts its camera
+//     rays among the paths of every other request in flight;
+//   - path ids are generated in order and every live path is shaded once per iteration: once the
+//     generated count and the smallest surviving id (left by the shade stage) have passed a
+//     segment's last id, the host enqueues its resolve (segments of one request in order), and
+//     after the last one the tonemap; a completion thread reads the image back, so a request
+//     completes independently of the others;
+//   - when a pass has no room left (path ids or table entries) it drains and starts over.
 // Results do not depend on the mix: streams are keyed per (seed, pixel, sample, bounce) and each
 // pixel's samples are summed in sample order whatever the segment boundaries are.
 #include <algorithm>
@@ -162,7 +165,7 @@ void Context::server_release_lane(Server& S) {
     S.ready = false;
 }
 
-// A new open pass: no entries, no path ids; control block and per-entry stats zeroed on the stream.
+// A new open pass: no entries, no path ids; control block zeroed on the stream.
 void Context::server_reset_pass(Server& S) {
     S.entries.clear();
     S.total = 0;
@@ -234,9 +237,7 @@ int Context::server_append(Server& S, std::vector<ServeReq*>& act) {
 }
 
 // One regenerate / closest-hit / shade iteration of the open pass, then a snapshot of the control
-// block, the work counters and the per-entry (finished, rays) counts into the pinned ring.
-// One regenerate / closest-hit / shade iteration of the open pass, then a snapshot of the control
-// block, the work counters and the per-entry (finished, rays) counts into the pinned ring. The
+// block, the work counters and the shade workgroups' smallest live path ids into the pinned ring. The
 // arguments depend only on the snapshot slot (which fixes the buffer parity) and on the traversal
 // variant, so after a few eager iterations each (slot, variant) is captured once into a hipGraph and
 // replayed: one graph launch per iteration instead of three kernel launches and three copies.

@@ -248,7 +248,7 @@ typedef struct crt_config {
     uint32_t sub_passes;        /* concurrent sub-passes (one stream each) of a pass, 1-4; 0 = 2 */
     uint32_t graphs;            /* serving loop iterations replayed from hipGraphs: 0 = default (on), 1 = on,
                                    2 = off (every iteration launched eagerly) */
-    uint32_t stack_lds;         /* traversal stack: LDS ring entries per lane behind the 4 register entries,
+    uint32_t stack_lds;         /* traversal stack: LDS ring entries per lane behind the register top entry,
                                    8, 16 or 32; 0 = 16 */
     uint32_t ray_sort;          /* 1: the shade stage groups each workgroup's surviving paths by direction
                                    octant (coherent waves in the next closest-hit launch); 0 / 2: off */
