@@ -436,17 +436,15 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 k2 = h2 ? k2 : INF;
                 k3 = h3 ? k3 : INF;
                 int32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
-                int nh = (k0 < INF) + (k1 < INF) + (k2 < INF) + (k3 < INF);
-                // the sort runs whatever nh is (a wave pays for it when any lane has a hit): the child
-                // refs are then used on the common path and their load is issued with the planes,
-                // not sunk behind the box tests into the branch
+                // near-first: sort the four (distance, child) pairs (missed children sort last at
+                // +inf), push the hit ones far to near, continue with the nearest
                 cswap(k0, c0, k1, c1); cswap(k2, c2, k3, c3);
                 cswap(k0, c0, k2, c2); cswap(k1, c1, k3, c3);
                 cswap(k1, c1, k2, c2);
-                if (nh > 0) {
-                    if (nh > 3) st.push((uint32_t)c3);
-                    if (nh > 2) st.push((uint32_t)c2);
-                    if (nh > 1) st.push((uint32_t)c1);
+                if (k0 < INF) {
+                    if (k3 < INF) st.push((uint32_t)c3);
+                    if (k2 < INF) st.push((uint32_t)c2);
+                    if (k1 < INF) st.push((uint32_t)c1);
                     node = c0; next = false;
                 }
             }
