@@ -235,26 +235,61 @@ struct Emitter {
         np[3] = F4{fb(u0), fb(u1), 0.0f, 0.0f};
     }
 
-    // Four-wide node: the binary tree is collapsed by repeatedly opening the child with the
-    // largest surface area until four children are gathered (or only leaves remain).
+    // Four-wide collapse of the binary tree, chosen by SAH cost (dynamic programming over the binary
+    // nodes, as in Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs through
+    // compressed wide BVHs", HPG 2017, here for four children): plan[n][k] is the cost of spreading
+    // n's subtree over k child slots of its parent (k = 1: n becomes a four-wide node of its own).
+    // Costs are areas relative to the root's, one unit per node visit and per primitive test.
+    std::vector<float> plan_cost;      // 5 per binary node, index 1..4
+    std::vector<uint8_t> plan_split;   // k = 1: slots given to the left child; k >= 2: 0 = own node, else slots to the left
+
+    void plan(int32_t id) {
+        const BNode& n = nodes[id];
+        float* c = &plan_cost[5 * (size_t)id];
+        uint8_t* sp = &plan_split[5 * (size_t)id];
+        if (n.type >= 0) {
+            const float lc = n.box.area() / root_area * (float)n.count;
+            for (int k = 1; k <= 4; ++k) { c[k] = lc; sp[k] = 0; }
+            return;
+        }
+        plan(n.l);
+        plan(n.r);
+        const float* cl = &plan_cost[5 * (size_t)n.l];
+        const float* cr = &plan_cost[5 * (size_t)n.r];
+        float best = 3.4e38f;
+        for (int a = 1; a <= 3; ++a) {
+            const float v = cl[a] + cr[4 - a];
+            if (v < best) { best = v; sp[1] = (uint8_t)a; }
+        }
+        c[1] = n.box.area() / root_area + best;
+        for (int k = 2; k <= 4; ++k) {
+            c[k] = c[1];
+            sp[k] = 0;
+            for (int a = 1; a < k; ++a) {
+                const float v = cl[a] + cr[k - a];
+                if (v < c[k]) { c[k] = v; sp[k] = (uint8_t)a; }
+            }
+        }
+    }
+
+    // Child slots of a node: subtree `id` spread over k slots as planned.
+    void gather(int32_t id, int k, int32_t* out, int& nc) {
+        const BNode& n = nodes[id];
+        const uint8_t a = k >= 2 && n.type < 0 ? plan_split[5 * (size_t)id + k] : 0;
+        if (a == 0) { out[nc++] = id; return; }
+        gather(n.l, a, out, nc);
+        gather(n.r, k - a, out, nc);
+    }
+
     int32_t emit4(int32_t id, uint32_t depth) {
         const BNode& n = nodes[id];
         if (n.type >= 0) return leaf_ref(n);
         max_depth = std::max(max_depth, depth + 1);
-        int32_t ch[4] = {n.l, n.r, -1, -1};
-        int nc = 2;
-        while (nc < 4) {
-            int best = -1;
-            float ba = -1.0f;
-            for (int k = 0; k < nc; ++k) {
-                const BNode& c = nodes[ch[k]];
-                if (c.type < 0 && c.box.area() > ba) { ba = c.box.area(); best = k; }
-            }
-            if (best < 0) break;
-            int32_t c = ch[best];
-            ch[best] = nodes[c].l;
-            ch[nc++] = nodes[c].r;
-        }
+        int32_t ch[4] = {-1, -1, -1, -1};
+        int nc = 0;
+        const int a = plan_split[5 * (size_t)id + 1];
+        gather(n.l, a, ch, nc);
+        gather(n.r, 4 - a, ch, nc);
         uint32_t slot = (uint32_t)(S.nodes.size() / 8);
         S.nodes.resize(S.nodes.size() + 8);
         sah += (double)n.box.area() / root_area;
@@ -330,6 +365,11 @@ struct Emitter {
             return slot;
         }
         root_area = std::max(nodes[root].box.area(), 1e-30f);
+        if (width == 4) {
+            plan_cost.assign(5 * nodes.size(), 0.0f);
+            plan_split.assign(5 * nodes.size(), 0);
+            plan(root);
+        }
         return (uint32_t)emit(root, 0);
     }
 };
