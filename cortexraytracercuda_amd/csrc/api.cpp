@@ -108,7 +108,7 @@ int Context::init(const crt_config* cfg) {
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     budget = config.hbm_budget_bytes ? config.hbm_budget_bytes : (uint64_t)(0.9 * (double)freeb);
-    max_paths = config.max_paths ? config.max_paths : (1u << 25);
+    max_paths = config.max_paths ? config.max_paths : (1u << 26);
     leaf_size = config.leaf_size ? config.leaf_size : kDefaultLeafSize;
     bvh_width = config.bvh_width ? config.bvh_width : kDefaultBvhWidth;
     if (config.extend_kernel > CRT_EXTEND_WAVE_CHUNK) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown extend_kernel");

@@ -56,7 +56,7 @@ int Context::autotune(crt_scene_id scene, const crt_scene_desc* desc, const crt_
     std::memset(out, 0, sizeof(*out));
 
     // trial request: the full image, enough samples to fill the largest slot pool twice
-    const uint64_t max_slots = 1ull << 25;
+    const uint64_t max_slots = 1ull << 26;
     crt_render_desc tr = *rd;
     const uint32_t sq = (uint32_t)std::sqrt((double)rd->spp), ns = sq * sq;
     const uint64_t npix = (uint64_t)rd->width * rd->height;
@@ -153,7 +153,7 @@ int Context::autotune(crt_scene_id scene, const crt_scene_desc* desc, const crt_
         }
         return apply(bs);
     };
-    if ((rc = sweep([](Shape& s, uint32_t v) { s.slots = v; }, {1u << 22, 1u << 23, 1u << 24, 1u << 25}))) return rc;
+    if ((rc = sweep([](Shape& s, uint32_t v) { s.slots = v; }, {1u << 23, 1u << 24, 1u << 25, 1u << 26}))) return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.grid = v; }, {16u, 32u, 64u}))) return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.sgrid = v; }, {4u, 8u, 16u}))) return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.sub = v; }, {1u, 2u, 3u, 4u}))) return rc;

@@ -229,7 +229,7 @@ typedef struct crt_config {
     int32_t device;             /* HIP device ordinal */
     uint64_t hbm_budget_bytes;  /* admission budget for scenes + requests; 0 = 90% of free memory */
     uint32_t max_paths;         /* live path slots of the streaming passes (path state pools, 164 B each);
-                                   0 = 32M */
+                                   0 = 64M */
     uint32_t block_size;        /* threads per block of the stage kernels; 0 = default (256) */
     uint32_t count_work;        /* 1: use the counting traversal kernel (fills node_visits/prim_tests) */
     uint32_t leaf_size;         /* BVH max primitives per leaf (1-8); 0 = default (1) */
