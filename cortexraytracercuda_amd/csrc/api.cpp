@@ -52,9 +52,49 @@ int DevBuf::alloc(size_t n, Context* ctx) {
     bytes = n;
     return CRT_OK;
 }
+static size_t size_class(size_t n) {
+    size_t c = 4096;
+    while (c < n) c <<= 1;
+    return c;
+}
+
+int DevBuf::alloc_cached(size_t n, Context* ctx) {
+    release();
+    if (n == 0) return CRT_OK;
+    const size_t cls = size_class(n);
+    void* q = ctx->cache_take(cls);
+    if (!q) {
+        int rc = alloc(cls, ctx);
+        if (rc) return rc;
+    } else {
+        p = q;
+        bytes = cls;
+    }
+    home = ctx;
+    return CRT_OK;
+}
+
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
-    p = nullptr; bytes = 0;
+    if (p && !(home && home->cache_put(p, bytes))) (void)hipFree(p);
+    p = nullptr; bytes = 0; home = nullptr;
+}
+
+void* Context::cache_take(size_t cls) {
+    std::lock_guard<std::mutex> g(cache_mutex);
+    auto it = buf_cache.find(cls);
+    if (it == buf_cache.end()) return nullptr;
+    void* q = it->second;
+    buf_cache.erase(it);
+    cache_bytes -= cls;
+    return q;
+}
+
+bool Context::cache_put(void* q, size_t cls) {
+    std::lock_guard<std::mutex> g(cache_mutex);
+    if (cache_bytes + cls > kCacheBytes) return false;
+    buf_cache.emplace(cls, q);
+    cache_bytes += cls;
+    return true;
 }
 
 template <class T> static int upload_vec(DevBuf& b, const std::vector<T>& v, Context* ctx, hipStream_t s) {
@@ -221,6 +261,11 @@ Context::~Context() {
     for (int k = 0; k < 2; ++k) if (ev_ext[k]) (void)hipEventDestroy(ev_ext[k]);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
     if (stream) (void)hipStreamDestroy(stream);
+    // every request has completed (stop_server) and every blocking render returned: the cache
+    // holds the last per-request buffers
+    for (auto& kv : buf_cache) (void)hipFree(kv.second);
+    buf_cache.clear();
+    cache_bytes = 0;
 }
 
 // ------------------------------------------------------------------ scenes
@@ -344,9 +389,9 @@ int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStrea
     req.width = r.width; req.height = r.height; req.sqrt_spp = sq; req.spp = r.spp; req.max_depth = r.max_depth;
     req.rng_mode = r.rng_mode; req.scene = sc->slot; req.seed = r.seed;
     int rc;
-    if ((rc = accum.alloc((size_t)npix * 16, ctx))) return rc;
-    if ((rc = rgb.alloc((((size_t)npix * 3 + 7) & ~(size_t)7) + 8, ctx))) return rc;
-    if (r.rng_mode == CRT_RNG_MINSTD && (rc = rng.alloc((size_t)npix * 4, ctx))) return rc;
+    if ((rc = accum.alloc_cached((size_t)npix * 16, ctx))) return rc;
+    if ((rc = rgb.alloc_cached((((size_t)npix * 3 + 7) & ~(size_t)7) + 8, ctx))) return rc;
+    if (r.rng_mode == CRT_RNG_MINSTD && (rc = rng.alloc_cached((size_t)npix * 4, ctx))) return rc;
     req.accum = (float*)accum.p;
     req.rng_state = (uint32_t*)rng.p;
     if (crt_k_init_request(st ? st : ctx->stream, (float4*)accum.p, (uint32_t*)rng.p, r.width, r.height, r.seed) ||

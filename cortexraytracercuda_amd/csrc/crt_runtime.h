@@ -42,7 +42,11 @@ constexpr uint32_t kDefaultLeafSize = 1, kDefaultBvhWidth = 4;
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    Context* home = nullptr;   // set: the buffer came from (and returns to) the context's cache
     int alloc(size_t n, Context* ctx);
+    // per-request buffers: reuse a cached allocation of the same size class (no hipMalloc / hipFree
+    // per render; the previous user's work has completed before its buffers are released)
+    int alloc_cached(size_t n, Context* ctx);
     void release();
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
@@ -191,6 +195,13 @@ struct Context {
     hipEvent_t sub_done[kMaxSubPasses - 1] = {};
     hipEvent_t ev_pass_ready = nullptr;
     int fail_alloc_after = -1;       // test hook: fail the n-th next allocation
+    // released per-request device buffers, by size class (power of two), at most kCacheBytes
+    static constexpr size_t kCacheBytes = 4ull << 30;
+    std::mutex cache_mutex;
+    std::multimap<size_t, void*> buf_cache;
+    size_t cache_bytes = 0;
+    void* cache_take(size_t cls);
+    bool cache_put(void* p, size_t cls);
     crt_work_counters last{};        // counters of the last blocking render
 
     // serving
