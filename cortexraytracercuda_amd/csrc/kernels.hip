@@ -658,6 +658,19 @@ __global__ void k_pass_init(uint32_t* __restrict__ ctl, uint32_t nwords) {
     for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) ctl[i] = 0u;
 }
 
+// Image pixel of the s-th path of a sample when a work item covers the whole image and both sides
+// are multiples of 8: 8 x 8 tiles in row-major tile order (64 consecutive path ids, one wave, trace
+// one tile instead of a 64-pixel row strip: more coherent camera rays). Otherwise row-major. The
+// path carries the image pixel from here on (RNG key, radiance slot, accumulation), so the image
+// does not depend on this order.
+__device__ __forceinline__ uint32_t item_pixel(const crt_dev_request& R, uint32_t s) {
+    const uint32_t W = R.width, H = R.height;
+    if (R.pix_begin != 0u || R.pix_count != W * H || ((W | H) & 7u) != 0u) return R.pix_begin + s;
+    const uint32_t tile = s >> 6, l = s & 63u, tpr = W >> 3;
+    const uint32_t ty = tile / tpr, tx = tile - ty * tpr;
+    return (ty * 8u + (l >> 3)) * W + tx * 8u + (l & 7u);
+}
+
 // Regeneration: the slots behind the survivors take the next path ids of the (sub-)pass, whose
 // ids are [pid0, pid0 + total) (camera rays, in path-id order: sample-major within each request).
 template <int MODE>
@@ -683,7 +696,7 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
         const crt_dev_request& R = reqs[r];
         uint32_t local = pid - R.path_base;
         uint32_t k = local / R.pix_count;
-        uint32_t pix = R.pix_begin + (local - k * R.pix_count);
+        uint32_t pix = item_pixel(R, local - k * R.pix_count);
         uint32_t sample = R.sample_begin + k;
         Rng<MODE> g;
         g.dyn = R.rng_mode;
