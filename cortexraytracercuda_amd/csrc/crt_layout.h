@@ -89,7 +89,8 @@ struct crt_dev_request {
     uint32_t rng_mode;
     uint32_t scene;          // index into the scene table
     uint64_t seed;
-    // pass work: pixel range x sample range, laid out sample-major from path_base
+    // pass work: pixel range x sample range, laid out pixel-major from path_base (a pixel's
+    // samples consecutive)
     uint32_t path_base;      // first path slot of this request in the pass
     uint32_t pix_begin, pix_count;
     uint32_t sample_begin, sample_count;

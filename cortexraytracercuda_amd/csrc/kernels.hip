@@ -658,11 +658,11 @@ __global__ void k_pass_init(uint32_t* __restrict__ ctl, uint32_t nwords) {
     for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) ctl[i] = 0u;
 }
 
-// Image pixel of the s-th path of a sample when a work item covers the whole image and both sides
-// are multiples of 8: 8 x 8 tiles in row-major tile order (64 consecutive path ids, one wave, trace
-// one tile instead of a 64-pixel row strip: more coherent camera rays). Otherwise row-major. The
-// path carries the image pixel from here on (RNG key, radiance slot, accumulation), so the image
-// does not depend on this order.
+// Image pixel of the s-th pixel slot of a work item: when the item covers the whole image and both
+// sides are multiples of 8, 8 x 8 tiles in row-major tile order (64 consecutive slots trace one
+// tile instead of a 64-pixel row strip: more coherent camera rays); otherwise row-major. The path
+// carries the image pixel from here on (RNG key, radiance slot, accumulation), so the image does not
+// depend on this order.
 __device__ __forceinline__ uint32_t item_pixel(const crt_dev_request& R, uint32_t s) {
     const uint32_t W = R.width, H = R.height;
     if (R.pix_begin != 0u || R.pix_count != W * H || ((W | H) & 7u) != 0u) return R.pix_begin + s;
@@ -672,7 +672,8 @@ __device__ __forceinline__ uint32_t item_pixel(const crt_dev_request& R, uint32_
 }
 
 // Regeneration: the slots behind the survivors take the next path ids of the (sub-)pass, whose
-// ids are [pid0, pid0 + total) (camera rays, in path-id order: sample-major within each request).
+// ids are [pid0, pid0 + total) (camera rays, in path-id order: pixel-major within each work item,
+// the item's samples of one pixel consecutive).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t pid0,
                                                uint32_t total, uint32_t nslots, uint32_t* __restrict__ ctl_in,
@@ -695,8 +696,10 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
         int r = find_request(reqs, nreq, pid);
         const crt_dev_request& R = reqs[r];
         uint32_t local = pid - R.path_base;
-        uint32_t k = local / R.pix_count;
-        uint32_t pix = item_pixel(R, local - k * R.pix_count);
+        // pixel-major: the samples of a pixel are consecutive path ids (a wave traces 64 samples of
+        // one pixel at high spp: nearly identical camera rays)
+        const uint32_t sidx = local / R.sample_count, k = local - sidx * R.sample_count;
+        uint32_t pix = item_pixel(R, sidx);
         uint32_t sample = R.sample_begin + k;
         Rng<MODE> g;
         g.dyn = R.rng_mode;
@@ -1246,7 +1249,7 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
             g.dyn = R.rng_mode;
             keep = shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
             if (keep) {
-                min_pid = min(min_pid, R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin));
+                min_pid = min(min_pid, R.path_base + (pix - R.pix_begin) * R.sample_count + (sample - R.sample_begin));
                 uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8);
                 no = make_float4(o.x, o.y, o.z, ro.w);
                 nd = make_float4(d.x, d.y, d.z, bitsf(ns));
@@ -1256,7 +1259,7 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
             } else {
                 // finished: the radiance goes to the path's id, with the path's closest-hit queries
                 // (one per hit, plus the final miss) in .w for the per-request ray count
-                uint32_t pid = R.path_base + (sample - R.sample_begin) * R.pix_count + (pix - R.pix_begin);
+                uint32_t pid = R.path_base + (pix - R.pix_begin) * R.sample_count + (sample - R.sample_begin);
                 const uint32_t rays = bounce + (fbits(hv.y) == CRT_MISS ? 1u : 0u);
                 out_rad[pid] = make_float4(L.x, L.y, L.z, (float)rays);
                 if (g.minstd()) stg(R.rng_state + pix, g.ms);
@@ -1371,7 +1374,7 @@ __global__ __launch_bounds__(256) void k_resolve(const crt_dev_request* __restri
         float4* a = (float4*)R.accum + pix;
         float4 acc = ldg(a);
         for (uint32_t k = 0; k < R.sample_count; ++k) {
-            float4 v = rad[R.path_base + k * R.pix_count + q];
+            float4 v = rad[R.path_base + q * R.sample_count + k];
             acc.x += v.x; acc.y += v.y; acc.z += v.z;
             nr += (uint32_t)v.w;
         }
