@@ -1358,28 +1358,52 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
 // Per-pixel sums in sample order: accum += rad[sample 0] + ... (same float additions as the
 // reference's pixel_color += ray_color(...) loop, core/src/gpu.cu:776-790). With `rays` set (one
 // request), the paths' closest-hit queries (rad.w) are added to *rays: one atomic per wave.
+// A pixel's samples are consecutive in rad, so a block stages 256 pixels x 8 samples at a time
+// through LDS: the global loads are whole 128-byte lines (8 lanes per pixel) instead of one
+// float4 per lane at a 16*spp-byte stride, and each lane still adds its pixel's samples in order.
+constexpr uint32_t kResolveChunk = 8;
 __global__ __launch_bounds__(256) void k_resolve(const crt_dev_request* __restrict__ reqs, int nreq,
                                                  const float4* __restrict__ rad, unsigned long long* __restrict__ rays) {
+    __shared__ float4 stage[256 * (kResolveChunk + 1)];
     int r = blockIdx.y;
     if (r >= nreq) return;
     const crt_dev_request& R = reqs[r];
+    const uint32_t S = R.sample_count;
     uint32_t nr = 0;
-    // block-uniform trip count: every lane reaches the wave sum
-    const uint32_t stride = gridDim.x * blockDim.x;
+    // block-uniform trip counts: every lane reaches each barrier and the wave sum
+    const uint32_t stride = gridDim.x * 256u;
     const uint32_t iters = (R.pix_count + stride - 1) / stride;
     for (uint32_t it = 0; it < iters; ++it) {
-        const uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-        if (q >= R.pix_count) continue;
-        uint32_t pix = R.pix_begin + q;
-        float4* a = (float4*)R.accum + pix;
-        float4 acc = ldg(a);
-        for (uint32_t k = 0; k < R.sample_count; ++k) {
-            float4 v = rad[R.path_base + q * R.sample_count + k];
-            acc.x += v.x; acc.y += v.y; acc.z += v.z;
-            nr += (uint32_t)v.w;
+        const uint32_t q0 = it * stride + blockIdx.x * 256u;
+        const uint32_t np = R.pix_count > q0 ? min(256u, R.pix_count - q0) : 0u;
+        const uint32_t t = threadIdx.x;
+        float4* a = (float4*)R.accum + R.pix_begin + q0 + t;
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (t < np) acc = ldg(a);
+        const float4* src = rad + R.path_base + q0 * S;
+        for (uint32_t k0 = 0; k0 < S; k0 += kResolveChunk) {
+            const uint32_t n = min(kResolveChunk, S - k0);
+            const uint32_t total = np * n;
+            for (uint32_t e = t; e < total; e += 256u) {
+                const uint32_t pl = (n == kResolveChunk) ? (e >> 3) : e / n;
+                const uint32_t j = e - pl * n;
+                stage[pl * (kResolveChunk + 1) + j] = src[pl * S + k0 + j];
+            }
+            __syncthreads();
+            if (t < np) {
+                const float4* row = stage + t * (kResolveChunk + 1);
+                for (uint32_t j = 0; j < n; ++j) {
+                    const float4 v = row[j];
+                    acc.x += v.x; acc.y += v.y; acc.z += v.z;
+                    nr += (uint32_t)v.w;
+                }
+            }
+            __syncthreads();
         }
-        acc.w += (float)R.sample_count;
-        stg(a, acc);
+        if (t < np) {
+            acc.w += (float)S;
+            stg(a, acc);
+        }
     }
     if (rays) wave_add(rays, nr);
 }
