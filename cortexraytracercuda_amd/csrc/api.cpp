@@ -532,7 +532,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
                 extend_launches++;
             }
             if (crt_k_shade(u.s, mode, two_level, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, hq, hiq,
-                            (float4*)out_rad.p, nullptr, (int)ray_sort, (crt_dev_counters*)counters.p))
+                            (float4*)out_rad.p, nullptr, (int)ray_sort, nreq == 1 ? 1 : 0, (crt_dev_counters*)counters.p))
                 return set_error(CRT_ERR_HIP, "shade launch failed");
             HIP_TRY(hipMemcpyAsync(ring + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
                                    hipMemcpyDeviceToHost, u.s));

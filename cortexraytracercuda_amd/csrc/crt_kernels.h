@@ -36,10 +36,11 @@ int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32
                  uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
                  crt_dev_counters* counters);
 // two_level: an instanced scene has paths in the launch (selects the unbounded-register variant);
-// live_min (nullable, grid words): per block, the smallest path id still alive after the stage
+// live_min (nullable, grid words): per block, the smallest path id still alive after the stage;
+// one_request: every path of the launch belongs to reqs[0]
 int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout, uint32_t* count_out, const float4* hit,
-                const uint32_t* hit_inst, float4* out_rad, uint32_t* live_min, int sort_octant,
+                const uint32_t* hit_inst, float4* out_rad, uint32_t* live_min, int sort_octant, int one_request,
                 crt_dev_counters* counters);
 int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* req, uint32_t* gstack, crt_dev_counters* counters);

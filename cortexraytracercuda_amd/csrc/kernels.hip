@@ -673,8 +673,9 @@ __device__ __forceinline__ uint32_t item_pixel(const crt_dev_request& R, uint32_
 
 // Regeneration: the slots behind the survivors take the next path ids of the (sub-)pass, whose
 // ids are [pid0, pid0 + total) (camera rays, in path-id order: pixel-major within each work item,
-// the item's samples of one pixel consecutive).
-template <int MODE>
+// the item's samples of one pixel consecutive). ONE: a closed pass of one request (its record is
+// wave-uniform: scalar loads, no search).
+template <int MODE, bool ONE>
 __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict__ reqs, int nreq, uint32_t pid0,
                                                uint32_t total, uint32_t nslots, uint32_t* __restrict__ ctl_in,
                                                uint32_t* __restrict__ ctl_out, const uint32_t* __restrict__ open_pass,
@@ -693,7 +694,7 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
     }
     for (uint32_t i = tid; i < n_new; i += gridDim.x * blockDim.x) {
         const uint32_t slot = surv + i, pid = pid0 + g0 + i;
-        int r = find_request(reqs, nreq, pid);
+        const int r = ONE ? 0 : find_request(reqs, nreq, pid);
         const crt_dev_request& R = reqs[r];
         uint32_t local = pid - R.path_base;
         // pixel-major: the samples of a pixel are consecutive path ids (a wave traces 64 samples of
@@ -1207,7 +1208,9 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
 // Shade stage. FLAT (no instanced scene among the paths) holds the kernel to 6 resident
 // workgroups per CU (80 VGPRs, a few spills outside the hot paths), measured 2 % faster per frame
 // on single-level scenes; with instances (transforms, textures) the unbounded kernel is faster.
-template <int MODE, int BLOCK, bool FLAT>
+// ONE: the pass holds a single request, so the request record and its scene's pointers are
+// wave-uniform (scalar loads once per wave instead of two dependent vector loads per path).
+template <int MODE, int BLOCK, bool FLAT, bool ONE>
 __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_scene* __restrict__ scenes,
                                                  const crt_dev_request* __restrict__ reqs,
                                                  const uint32_t* __restrict__ count_in, crt_paths Pin, crt_paths Pout,
@@ -1237,7 +1240,7 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
             float4 tv = Pin.thr[i], rv = Pin.rad[i];
             idv = Pin.ids[i];
             uint32_t state = fbits(rdv.w);
-            uint32_t req = idv.y >> 16;
+            const uint32_t req = ONE ? 0u : (idv.y >> 16);
             const crt_dev_request& R = reqs[req];
             const crt_dev_scene* sc = scenes + R.scene;
             V3 o = v3(ro), d = v3(rdv);
@@ -1459,13 +1462,15 @@ extern "C" int crt_k_pass_init(hipStream_t s, uint32_t* ctl, uint32_t nwords) {
 extern "C" int crt_k_regen(hipStream_t s, int rng_mode, uint32_t grid, const crt_dev_request* reqs, int nreq,
                            uint32_t pid0, uint32_t total, uint32_t nslots, uint32_t* ctl_in, uint32_t* ctl_out,
                            const uint32_t* open_pass, crt_paths P) {
-#define CRT_RG(M)                                                                                                 \
-    hipLaunchKernelGGL(k_regen<M>, dim3(grid), dim3(256), 0, s, reqs, nreq, pid0, total, nslots, ctl_in, ctl_out, \
+#define CRT_RG1(M, O)                                                                                                \
+    hipLaunchKernelGGL((k_regen<M, O>), dim3(grid), dim3(256), 0, s, reqs, nreq, pid0, total, nslots, ctl_in, ctl_out, \
                        open_pass, P)
-    if (rng_mode == CRT_RNG_MINSTD) CRT_RG(CRT_RNG_MINSTD);
-    else if (rng_mode == CRT_RNG_PCG) CRT_RG(CRT_RNG_PCG);
-    else CRT_RG(CRT_RNG_DYNAMIC);
+#define CRT_RG(M) { if (nreq == 1 && !open_pass) CRT_RG1(M, true); else CRT_RG1(M, false); }
+    if (rng_mode == CRT_RNG_MINSTD) CRT_RG(CRT_RNG_MINSTD)
+    else if (rng_mode == CRT_RNG_PCG) CRT_RG(CRT_RNG_PCG)
+    else CRT_RG(CRT_RNG_DYNAMIC)
 #undef CRT_RG
+#undef CRT_RG1
     return (int)hipGetLastError();
 }
 
@@ -1533,15 +1538,17 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
 extern "C" int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t grid, const crt_dev_scene* scenes,
                            const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout,
                            uint32_t* count_out, const float4* hit, const uint32_t* hit_inst, float4* out_rad,
-                           uint32_t* live_min, int sort_octant, crt_dev_counters* counters) {
-#define CRT_SH(M, F)                                                                                                      \
-    hipLaunchKernelGGL((k_shade<M, 256, F>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout, count_out, \
-                       hit, hit_inst, out_rad, live_min, sort_octant, counters)
-#define CRT_SH_F(M) { if (two_level) CRT_SH(M, false); else CRT_SH(M, true); }
+                           uint32_t* live_min, int sort_octant, int one_request, crt_dev_counters* counters) {
+#define CRT_SH(M, F, O)                                                                                                   \
+    hipLaunchKernelGGL((k_shade<M, 256, F, O>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout,         \
+                       count_out, hit, hit_inst, out_rad, live_min, sort_octant, counters)
+#define CRT_SH_O(M, F) { if (one_request) CRT_SH(M, F, true); else CRT_SH(M, F, false); }
+#define CRT_SH_F(M) { if (two_level) CRT_SH_O(M, false) else CRT_SH_O(M, true) }
     if (rng_mode == CRT_RNG_MINSTD) CRT_SH_F(CRT_RNG_MINSTD)
     else if (rng_mode == CRT_RNG_PCG) CRT_SH_F(CRT_RNG_PCG)
     else CRT_SH_F(CRT_RNG_DYNAMIC)
 #undef CRT_SH_F
+#undef CRT_SH_O
 #undef CRT_SH
     return (int)hipGetLastError();
 }

@@ -266,7 +266,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
         return set_error(CRT_ERR_HIP, "extend launch failed");
     if (crt_k_shade(S.s, mode, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.live_min.p,
-                    (int)C.ray_sort, (crt_dev_counters*)S.counters.p))
+                    (int)C.ray_sort, 0, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "shade launch failed");
     uint32_t* snap = S.snap + (size_t)(k % Server::kRing) * Server::kSnapWords;
     HIP_TRY(hipMemcpyAsync(snap, cout, sizeof(uint32_t) * CRT_CTL_WORDS, hipMemcpyDeviceToHost, S.s));
