@@ -289,8 +289,17 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     if (slot == kMaxScenes) return set_error(CRT_ERR_OVER_BUDGET, "scene table full");
     auto t0 = std::chrono::steady_clock::now();
     HostScene& H = sc->host;
-    if ((rc = upload_vec(sc->nodes, H.nodes, this, stream))) return rc;
-    if ((rc = upload_vec(sc->tris, H.tris, this, stream))) return rc;
+    {
+        // nodes and triangles in one allocation, triangles behind the nodes (the traversal fetches
+        // both through 32-bit byte offsets from the node array); 256 spare bytes keep the loads of
+        // a node-less scene in bounds
+        const size_t nb = H.nodes.size() * sizeof(H.nodes[0]), tb = H.tris.size() * sizeof(H.tris[0]);
+        const size_t off = (size_t)H.tri_offset();
+        if ((rc = sc->geo.alloc(off + tb + 256, this))) return rc;
+        if (nb) HIP_TRY(hipMemcpyAsync(sc->geo.p, H.nodes.data(), nb, hipMemcpyHostToDevice, stream));
+        if (tb) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + off, H.tris.data(), tb, hipMemcpyHostToDevice, stream));
+        sc->geo_tri_off = off;
+    }
     if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
     if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
     if ((rc = upload_vec(sc->spheres, H.spheres, this, stream))) return rc;
@@ -304,7 +313,7 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     if ((rc = upload_vec(sc->light_quads, H.light_quads, this, stream))) return rc;
     crt_dev_scene& ds = sc->dev;
     std::memset(&ds, 0, sizeof(ds));
-    ds.nodes = sc->nodes.p; ds.tris = sc->tris.p; ds.tri_normals = sc->tri_normals.p;
+    ds.nodes = sc->geo.p; ds.tris = (char*)sc->geo.p + sc->geo_tri_off; ds.tri_normals = sc->tri_normals.p;
     ds.tri_uvs = sc->tri_uvs.p; ds.spheres = sc->spheres.p; ds.sphere_aux = sc->sphere_aux.p; ds.sphere_map = sc->sphere_map.p;
     ds.quads = sc->quads.p; ds.insts = sc->insts.p; ds.mats = sc->mats.p; ds.texs = sc->texs.p;
     ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;

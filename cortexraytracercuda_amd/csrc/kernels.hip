@@ -310,12 +310,14 @@ struct HitRec { float t, u, v; uint32_t prim, inst; };
 struct SceneView {
     const float4* nodes; const float4* tris; const float4* spheres; const float4* quads;
     const float4* insts; uint32_t root;
+    uint32_t tri_off;   // byte offset of the triangle array from the node array (same allocation)
 };
 __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
     SceneView v;
     v.nodes = (const float4*)sc->nodes; v.tris = (const float4*)sc->tris;
     v.spheres = (const float4*)sc->spheres; v.quads = (const float4*)sc->quads; v.insts = (const float4*)sc->insts;
     v.root = sc->root;
+    v.tri_off = (uint32_t)((const char*)sc->tris - (const char*)sc->nodes);
     return v;
 }
 
@@ -396,13 +398,124 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         node = (int32_t)S.root;
         tmin = t0;
     }
+    // Four-wide node on its near / far planes (picked by the ray's octant) and child refs: box
+    // tests, near-first order, pushes; sets `node` to the nearest hit child (next = false) or
+    // leaves next set when no child is hit. Empty child slots have boxes at +inf, which no ray hits.
+    __device__ __forceinline__ void visit4(float4 rff, float4 nx, float4 fx, float4 ny, float4 fy, float4 nz, float4 fz,
+                                           bool& next) {
+        const float tf = best.t;
+        const int4 rf = make_int4((int)fbits(rff.x), (int)fbits(rff.y), (int)fbits(rff.z), (int)fbits(rff.w));
+        bool h0, h1, h2, h3;
+        float k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, id, tmin, tf, h0);
+        float k1 = box_near_oct(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, id, tmin, tf, h1);
+        float k2 = box_near_oct(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, id, tmin, tf, h2);
+        float k3 = box_near_oct(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, id, tmin, tf, h3);
+        const float INF = __builtin_inff();
+        k0 = h0 ? k0 : INF;
+        k1 = h1 ? k1 : INF;
+        k2 = h2 ? k2 : INF;
+        k3 = h3 ? k3 : INF;
+        int32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
+        // near-first: sort the four (distance, child) pairs (missed children sort last at +inf),
+        // push the hit ones far to near, continue with the nearest
+        cswap(k0, c0, k1, c1); cswap(k2, c2, k3, c3);
+        cswap(k0, c0, k2, c2); cswap(k1, c1, k3, c3);
+        cswap(k1, c1, k2, c2);
+        if (k0 < INF) {
+            if (k3 < INF) st.push((uint32_t)c3);
+            if (k2 < INF) st.push((uint32_t)c2);
+            if (k1 < INF) st.push((uint32_t)c1);
+            node = c0; next = false;
+        }
+    }
+    // Leaf (reference written inline in the parent: no leaf-record fetch). With TRI0, the first
+    // triangle's record (v0, v1, v2) is already loaded.
+    template <bool TRI0>
+    __device__ __forceinline__ void visit_leaf(const SceneView& S, TravCount& tc, bool& next, float4 a0, float4 a1,
+                                               float4 a2) {
+        const uint32_t type = CRT_LEAF_REF_TYPE(node);
+        const uint32_t cnt = CRT_LEAF_REF_COUNT(node);
+        const uint32_t first = CRT_LEAF_REF_FIRST(node);
+        if (COUNT) tc.leaves++;
+        if (type == CRT_PRIM_TRI) {
+            if (TWO_LEVEL && !(oct & 8u)) { tp = tri_pre(d); oct |= 8u; }
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4* tp4 = S.tris + 3 * (first + k);
+                float t, bu, bv;
+                if (COUNT) tc.tris++;
+                const bool pre = TRI0 && k == 0u;
+                if (tri_hit(o, tp, pre ? a0 : ldg(tp4), pre ? a1 : ldg(tp4 + 1), pre ? a2 : ldg(tp4 + 2), tmin, best.t,
+                            t, bu, bv)) {
+                    best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
+                    best.inst = cur_inst;
+                }
+            }
+        } else if (type == CRT_PRIM_SPHERE) {
+            for (uint32_t k = 0; k < cnt; ++k) {
+                float4 s = ldg(S.spheres + first + k);
+                float t;
+                if (COUNT) tc.spheres++;
+                if (sphere_hit(o, d, s, tmin, best.t, t)) {
+                    best.t = t; best.u = 0.0f; best.v = 0.0f; best.prim = (CRT_PRIM_SPHERE << 30) | (first + k);
+                    best.inst = cur_inst;
+                }
+            }
+        } else if (type == CRT_PRIM_QUAD) {
+            for (uint32_t k = 0; k < cnt; ++k) {
+                float t, qa, qb;
+                if (COUNT) tc.quads++;
+                if (quad_hit(o, d, S.quads + 5 * (first + k), tmin, best.t, camera, t, qa, qb)) {
+                    best.t = t; best.u = qa; best.v = qb; best.prim = (CRT_PRIM_QUAD << 30) | (first + k);
+                    best.inst = cur_inst;
+                }
+            }
+        } else if (TWO_LEVEL && !in_blas) {
+            // instance leaf (count 1): move the ray to object space and descend into the BLAS
+            const float4* ip = S.insts + 4 * first;
+            float4 r0 = ldg(ip), r1 = ldg(ip + 1), r2 = ldg(ip + 2);
+            int4 meta = ldg_i4(ip + 3);
+            if (COUNT) tc.insts++;
+            (void)meta.y;
+            set_ray(v3(r0.x * ow.x + r0.y * ow.y + r0.z * ow.z + r0.w, r1.x * ow.x + r1.y * ow.y + r1.z * ow.z + r1.w,
+                       r2.x * ow.x + r2.y * ow.y + r2.z * ow.z + r2.w),
+                    v3(r0.x * dw.x + r0.y * dw.y + r0.z * dw.z, r1.x * dw.x + r1.y * dw.y + r1.z * dw.z,
+                       r2.x * dw.x + r2.y * dw.y + r2.z * dw.z));
+            cur_inst = first;   // instance record index (leaf order); the host maps it to the caller's id
+            in_blas = true;
+            blas_level = st.depth();
+            node = meta.x;
+            next = false;
+        }
+    }
     // One visit; returns true when the traversal is complete (result in `best`).
     __device__ __forceinline__ bool step(const SceneView& S, TravCount& tc) {
         bool next = true;
-        if (node >= 0) {
+        if (WIDTH == 4) {
+            // Node lanes and triangle-leaf lanes of a wave share one set of seven loads (one memory
+            // round trip per step instead of one per branch taken): node lanes fetch their near /
+            // far planes and child refs, triangle lanes their triangle record, addressed from the
+            // node array (the triangle array follows it in the same allocation). Other leaf lanes
+            // read node 0 and load their primitives in the leaf branch.
+            const bool isn = node >= 0;
+            const bool ist = !isn && CRT_LEAF_REF_TYPE(node) == CRT_PRIM_TRI;
+            const uint32_t tb = ist ? S.tri_off + CRT_LEAF_REF_FIRST(node) * 48u : 0u;
+            const uint32_t nb = (uint32_t)node * 128u;   // node byte offset (scenes stay below 4 GiB of geometry)
+            const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
+            float4 rff = ldg_at(S.nodes, isn ? nb + 96u : tb);
+            float4 nx = ldg_at(S.nodes, isn ? nb + sx : tb + 16u), fx = ldg_at(S.nodes, isn ? nb + (16u - sx) : tb + 32u),
+                   ny = ldg_at(S.nodes, isn ? nb + 32u + sy : tb), fy = ldg_at(S.nodes, isn ? nb + (48u - sy) : tb),
+                   nz = ldg_at(S.nodes, isn ? nb + 64u + sz : tb), fz = ldg_at(S.nodes, isn ? nb + (80u - sz) : tb);
+            __builtin_amdgcn_sched_group_barrier(0x020, 7, 0);   // 0x020: VMEM reads
+            if (isn) {
+                if (COUNT) tc.nodes++;
+                visit4(rff, nx, fx, ny, fy, nz, fz, next);
+            } else {
+                visit_leaf<true>(S, tc, next, rff, nx, fx);
+            }
+        } else if (node >= 0) {
             if (COUNT) tc.nodes++;
-            float tf = best.t;
             if (WIDTH == 2) {
+                const float tf = best.t;
                 const float4* np = S.nodes + 4 * (uint32_t)node;
                 float4 n0 = ldg(np), n1 = ldg(np + 1), n2 = ldg(np + 2);
                 int4 n3 = ldg_i4(np + 3);
@@ -417,96 +530,23 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 } else if (hit0) { node = n3.x; next = false; }
                 else if (hit1) { node = n3.y; next = false; }
             } else {
-                // planes lo.x hi.x lo.y hi.y lo.z hi.z (four children each) and the child refs; empty
-                // child slots have boxes at +inf, which no ray hits
+                // planes lo.x hi.x lo.y hi.y lo.z hi.z (four children each) and the child refs; the
+                // near and far plane of each axis are picked by the ray's octant (no per-axis min /
+                // max); the seven loads are kept together ahead of the box tests (one memory round
+                // trip per visit: the scheduler would otherwise place the ref load late to save
+                // registers)
                 const uint32_t nb = (uint32_t)node * 128u;   // node byte offset (scenes stay below 4 GiB of nodes)
-                // the near and far plane of each axis are picked by the ray's octant (no per-axis
-                // min / max); the seven loads are kept together ahead of the box tests (one memory
-                // round trip per visit: the scheduler would otherwise place the ref load late to
-                // save registers)
                 const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
                 float4 rff = ldg_at(S.nodes, nb + 96u);
                 float4 nx = ldg_at(S.nodes, nb + sx), fx = ldg_at(S.nodes, nb + (16u - sx)),
                        ny = ldg_at(S.nodes, nb + 32u + sy), fy = ldg_at(S.nodes, nb + (48u - sy)),
                        nz = ldg_at(S.nodes, nb + 64u + sz), fz = ldg_at(S.nodes, nb + (80u - sz));
                 __builtin_amdgcn_sched_group_barrier(0x020, 7, 0);   // 0x020: VMEM reads
-                const int4 rf = make_int4((int)fbits(rff.x), (int)fbits(rff.y), (int)fbits(rff.z), (int)fbits(rff.w));
-                bool h0, h1, h2, h3;
-                float k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, id, tmin, tf, h0);
-                float k1 = box_near_oct(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, id, tmin, tf, h1);
-                float k2 = box_near_oct(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, id, tmin, tf, h2);
-                float k3 = box_near_oct(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, id, tmin, tf, h3);
-                const float INF = __builtin_inff();
-                k0 = h0 ? k0 : INF;
-                k1 = h1 ? k1 : INF;
-                k2 = h2 ? k2 : INF;
-                k3 = h3 ? k3 : INF;
-                int32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
-                // near-first: sort the four (distance, child) pairs (missed children sort last at
-                // +inf), push the hit ones far to near, continue with the nearest
-                cswap(k0, c0, k1, c1); cswap(k2, c2, k3, c3);
-                cswap(k0, c0, k2, c2); cswap(k1, c1, k3, c3);
-                cswap(k1, c1, k2, c2);
-                if (k0 < INF) {
-                    if (k3 < INF) st.push((uint32_t)c3);
-                    if (k2 < INF) st.push((uint32_t)c2);
-                    if (k1 < INF) st.push((uint32_t)c1);
-                    node = c0; next = false;
-                }
+                visit4(rff, nx, fx, ny, fy, nz, fz, next);
             }
         } else {
-            // leaf reference written inline in the parent: no leaf-record fetch
-            const uint32_t type = CRT_LEAF_REF_TYPE(node);
-            const uint32_t cnt = CRT_LEAF_REF_COUNT(node);
-            const uint32_t first = CRT_LEAF_REF_FIRST(node);
-            if (COUNT) tc.leaves++;
-            if (type == CRT_PRIM_TRI) {
-                if (TWO_LEVEL && !(oct & 8u)) { tp = tri_pre(d); oct |= 8u; }
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    const float4* tp4 = S.tris + 3 * (first + k);
-                    float t, bu, bv;
-                    if (COUNT) tc.tris++;
-                    if (tri_hit(o, tp, ldg(tp4), ldg(tp4 + 1), ldg(tp4 + 2), tmin, best.t, t, bu, bv)) {
-                        best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
-                        best.inst = cur_inst;
-                    }
-                }
-            } else if (type == CRT_PRIM_SPHERE) {
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    float4 s = ldg(S.spheres + first + k);
-                    float t;
-                    if (COUNT) tc.spheres++;
-                    if (sphere_hit(o, d, s, tmin, best.t, t)) {
-                        best.t = t; best.u = 0.0f; best.v = 0.0f; best.prim = (CRT_PRIM_SPHERE << 30) | (first + k);
-                        best.inst = cur_inst;
-                    }
-                }
-            } else if (type == CRT_PRIM_QUAD) {
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    float t, qa, qb;
-                    if (COUNT) tc.quads++;
-                    if (quad_hit(o, d, S.quads + 5 * (first + k), tmin, best.t, camera, t, qa, qb)) {
-                        best.t = t; best.u = qa; best.v = qb; best.prim = (CRT_PRIM_QUAD << 30) | (first + k);
-                        best.inst = cur_inst;
-                    }
-                }
-            } else if (TWO_LEVEL && !in_blas) {
-                // instance leaf (count 1): move the ray to object space and descend into the BLAS
-                const float4* ip = S.insts + 4 * first;
-                float4 r0 = ldg(ip), r1 = ldg(ip + 1), r2 = ldg(ip + 2);
-                int4 meta = ldg_i4(ip + 3);
-                if (COUNT) tc.insts++;
-                (void)meta.y;
-                set_ray(v3(r0.x * ow.x + r0.y * ow.y + r0.z * ow.z + r0.w, r1.x * ow.x + r1.y * ow.y + r1.z * ow.z + r1.w,
-                           r2.x * ow.x + r2.y * ow.y + r2.z * ow.z + r2.w),
-                        v3(r0.x * dw.x + r0.y * dw.y + r0.z * dw.z, r1.x * dw.x + r1.y * dw.y + r1.z * dw.z,
-                           r2.x * dw.x + r2.y * dw.y + r2.z * dw.z));
-                cur_inst = first;   // instance record index (leaf order); the host maps it to the caller's id
-                in_blas = true;
-                blas_level = st.depth();
-                node = meta.x;
-                next = false;
-            }
+            const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            visit_leaf<false>(S, tc, next, z, z, z);
         }
         if (next) {
             if (TWO_LEVEL && in_blas && st.depth() == blas_level) {

@@ -680,6 +680,11 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
         err = "more than 2^26 primitive records of one type";
         return CRT_ERR_INVALID_SCENE;
     }
+    if (S.geometry_bytes() > kMaxGeometryBytes) {
+        err = "BVH nodes and triangles take " + std::to_string(S.geometry_bytes()) +
+              " bytes; one scene holds at most 4 GiB of them";
+        return CRT_ERR_INVALID_SCENE;
+    }
     S.stats.sah_cost = em.sah;
     S.stats.depth = em.max_depth + blas_depth;
     // per-instance material overrides are resolved in the shade stage (instance record word z)
