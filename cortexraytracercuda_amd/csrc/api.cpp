@@ -290,15 +290,17 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     auto t0 = std::chrono::steady_clock::now();
     HostScene& H = sc->host;
     {
-        // nodes and triangles in one allocation, triangles behind the nodes (the traversal fetches
-        // both through 32-bit byte offsets from the node array); 256 spare bytes keep the loads of
-        // a node-less scene in bounds
-        const size_t nb = H.nodes.size() * sizeof(H.nodes[0]), tb = H.tris.size() * sizeof(H.tris[0]);
-        const size_t off = (size_t)H.tri_offset();
-        if ((rc = sc->geo.alloc(off + tb + 256, this))) return rc;
+        // nodes, triangles and instance records in one allocation, in that order (the traversal
+        // fetches all three through 32-bit byte offsets from the node array); 256 spare bytes keep
+        // the loads of a node-less scene in bounds
+        const size_t nb = H.nodes.size() * sizeof(F4), tb = H.tris.size() * sizeof(F4), ib = H.insts.size() * sizeof(F4);
+        const size_t toff = (size_t)H.tri_offset(), ioff = (size_t)H.inst_offset();
+        if ((rc = sc->geo.alloc(ioff + ib + 256, this))) return rc;
         if (nb) HIP_TRY(hipMemcpyAsync(sc->geo.p, H.nodes.data(), nb, hipMemcpyHostToDevice, stream));
-        if (tb) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + off, H.tris.data(), tb, hipMemcpyHostToDevice, stream));
-        sc->geo_tri_off = off;
+        if (tb) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + toff, H.tris.data(), tb, hipMemcpyHostToDevice, stream));
+        if (ib) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + ioff, H.insts.data(), ib, hipMemcpyHostToDevice, stream));
+        sc->geo_tri_off = toff;
+        sc->geo_inst_off = ioff;
     }
     if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
     if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
@@ -306,7 +308,6 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     if ((rc = upload_vec(sc->sphere_aux, H.sphere_aux, this, stream))) return rc;
     if ((rc = upload_vec(sc->sphere_map, H.sphere_map, this, stream))) return rc;
     if ((rc = upload_vec(sc->quads, H.quads, this, stream))) return rc;
-    if ((rc = upload_vec(sc->insts, H.insts, this, stream))) return rc;
     if ((rc = upload_vec(sc->mats, H.mats, this, stream))) return rc;
     if ((rc = upload_vec(sc->texs, H.texs, this, stream))) return rc;
     if ((rc = upload_vec(sc->texels, H.texels, this, stream))) return rc;
@@ -315,7 +316,7 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     std::memset(&ds, 0, sizeof(ds));
     ds.nodes = sc->geo.p; ds.tris = (char*)sc->geo.p + sc->geo_tri_off; ds.tri_normals = sc->tri_normals.p;
     ds.tri_uvs = sc->tri_uvs.p; ds.spheres = sc->spheres.p; ds.sphere_aux = sc->sphere_aux.p; ds.sphere_map = sc->sphere_map.p;
-    ds.quads = sc->quads.p; ds.insts = sc->insts.p; ds.mats = sc->mats.p; ds.texs = sc->texs.p;
+    ds.quads = sc->quads.p; ds.insts = (char*)sc->geo.p + sc->geo_inst_off; ds.mats = sc->mats.p; ds.texs = sc->texs.p;
     ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;
     ds.num_lights = H.num_lights; ds.root = H.root; ds.num_quads = H.num_quads; ds.bvh_width = H.bvh_width;
     for (int k = 0; k < 3; ++k) ds.background[k] = H.background[k];

@@ -310,7 +310,8 @@ struct HitRec { float t, u, v; uint32_t prim, inst; };
 struct SceneView {
     const float4* nodes; const float4* tris; const float4* spheres; const float4* quads;
     const float4* insts; uint32_t root;
-    uint32_t tri_off;   // byte offset of the triangle array from the node array (same allocation)
+    uint32_t tri_off;   // byte offsets of the triangle and instance arrays from the node array
+    uint32_t inst_off;  // (all three in one allocation)
 };
 __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
     SceneView v;
@@ -318,6 +319,7 @@ __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
     v.spheres = (const float4*)sc->spheres; v.quads = (const float4*)sc->quads; v.insts = (const float4*)sc->insts;
     v.root = sc->root;
     v.tri_off = (uint32_t)((const char*)sc->tris - (const char*)sc->nodes);
+    v.inst_off = (uint32_t)((const char*)sc->insts - (const char*)sc->nodes);
     return v;
 }
 
@@ -428,11 +430,11 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
             node = c0; next = false;
         }
     }
-    // Leaf (reference written inline in the parent: no leaf-record fetch). With TRI0, the first
-    // triangle's record (v0, v1, v2) is already loaded.
-    template <bool TRI0>
+    // Leaf (reference written inline in the parent: no leaf-record fetch). With PRE, the first
+    // triangle's record (v0, v1, v2 in a0-a2) or the instance record (a0-a3) is already loaded.
+    template <bool PRE>
     __device__ __forceinline__ void visit_leaf(const SceneView& S, TravCount& tc, bool& next, float4 a0, float4 a1,
-                                               float4 a2) {
+                                               float4 a2, float4 a3) {
         const uint32_t type = CRT_LEAF_REF_TYPE(node);
         const uint32_t cnt = CRT_LEAF_REF_COUNT(node);
         const uint32_t first = CRT_LEAF_REF_FIRST(node);
@@ -443,7 +445,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 const float4* tp4 = S.tris + 3 * (first + k);
                 float t, bu, bv;
                 if (COUNT) tc.tris++;
-                const bool pre = TRI0 && k == 0u;
+                const bool pre = PRE && k == 0u;
                 if (tri_hit(o, tp, pre ? a0 : ldg(tp4), pre ? a1 : ldg(tp4 + 1), pre ? a2 : ldg(tp4 + 2), tmin, best.t,
                             t, bu, bv)) {
                     best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
@@ -472,8 +474,9 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         } else if (TWO_LEVEL && !in_blas) {
             // instance leaf (count 1): move the ray to object space and descend into the BLAS
             const float4* ip = S.insts + 4 * first;
-            float4 r0 = ldg(ip), r1 = ldg(ip + 1), r2 = ldg(ip + 2);
-            int4 meta = ldg_i4(ip + 3);
+            const float4 r0 = PRE ? a0 : ldg(ip), r1 = PRE ? a1 : ldg(ip + 1), r2 = PRE ? a2 : ldg(ip + 2);
+            const float4 m4 = PRE ? a3 : ldg(ip + 3);
+            const int4 meta = make_int4((int)fbits(m4.x), (int)fbits(m4.y), (int)fbits(m4.z), (int)fbits(m4.w));
             if (COUNT) tc.insts++;
             (void)meta.y;
             set_ray(v3(r0.x * ow.x + r0.y * ow.y + r0.z * ow.z + r0.w, r1.x * ow.x + r1.y * ow.y + r1.z * ow.z + r1.w,
@@ -491,26 +494,30 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     __device__ __forceinline__ bool step(const SceneView& S, TravCount& tc) {
         bool next = true;
         if (WIDTH == 4) {
-            // Node lanes and triangle-leaf lanes of a wave share one set of seven loads (one memory
-            // round trip per step instead of one per branch taken): node lanes fetch their near /
-            // far planes and child refs, triangle lanes their triangle record, addressed from the
-            // node array (the triangle array follows it in the same allocation). Other leaf lanes
-            // read node 0 and load their primitives in the leaf branch.
+            // Node lanes and leaf lanes of a wave share one set of seven loads (one memory round
+            // trip per step instead of one per branch taken): node lanes fetch their near / far
+            // planes and child refs, triangle lanes their triangle record and instance lanes their
+            // instance record, all addressed from the node array (the triangle and instance arrays
+            // follow it in the same allocation). Sphere and quad lanes read node 0 and load their
+            // primitives in the leaf branch.
             const bool isn = node >= 0;
-            const bool ist = !isn && CRT_LEAF_REF_TYPE(node) == CRT_PRIM_TRI;
-            const uint32_t tb = ist ? S.tri_off + CRT_LEAF_REF_FIRST(node) * 48u : 0u;
+            const uint32_t ltype = CRT_LEAF_REF_TYPE(node), lfirst = CRT_LEAF_REF_FIRST(node);
+            const bool ist = !isn && ltype == CRT_PRIM_TRI;
+            const bool isi = TWO_LEVEL && !isn && ltype == CRT_PRIM_INST && !in_blas;
+            const uint32_t lb = ist ? S.tri_off + lfirst * 48u : (isi ? S.inst_off + lfirst * 64u : 0u);
+            const uint32_t l3 = isi ? lb + 48u : lb;
             const uint32_t nb = (uint32_t)node * 128u;   // node byte offset (scenes stay below 4 GiB of geometry)
             const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
-            float4 rff = ldg_at(S.nodes, isn ? nb + 96u : tb);
-            float4 nx = ldg_at(S.nodes, isn ? nb + sx : tb + 16u), fx = ldg_at(S.nodes, isn ? nb + (16u - sx) : tb + 32u),
-                   ny = ldg_at(S.nodes, isn ? nb + 32u + sy : tb), fy = ldg_at(S.nodes, isn ? nb + (48u - sy) : tb),
-                   nz = ldg_at(S.nodes, isn ? nb + 64u + sz : tb), fz = ldg_at(S.nodes, isn ? nb + (80u - sz) : tb);
+            float4 rff = ldg_at(S.nodes, isn ? nb + 96u : lb);
+            float4 nx = ldg_at(S.nodes, isn ? nb + sx : lb + 16u), fx = ldg_at(S.nodes, isn ? nb + (16u - sx) : lb + 32u),
+                   ny = ldg_at(S.nodes, isn ? nb + 32u + sy : l3), fy = ldg_at(S.nodes, isn ? nb + (48u - sy) : lb),
+                   nz = ldg_at(S.nodes, isn ? nb + 64u + sz : lb), fz = ldg_at(S.nodes, isn ? nb + (80u - sz) : lb);
             __builtin_amdgcn_sched_group_barrier(0x020, 7, 0);   // 0x020: VMEM reads
             if (isn) {
                 if (COUNT) tc.nodes++;
                 visit4(rff, nx, fx, ny, fy, nz, fz, next);
             } else {
-                visit_leaf<true>(S, tc, next, rff, nx, fx);
+                visit_leaf<true>(S, tc, next, rff, nx, fx, ny);
             }
         } else if (node >= 0) {
             if (COUNT) tc.nodes++;
@@ -546,7 +553,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
             }
         } else {
             const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            visit_leaf<false>(S, tc, next, z, z, z);
+            visit_leaf<false>(S, tc, next, z, z, z, z);
         }
         if (next) {
             if (TWO_LEVEL && in_blas && st.depth() == blas_level) {

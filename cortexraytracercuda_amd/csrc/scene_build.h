@@ -49,13 +49,14 @@ struct HostScene {
     uint64_t num_world_tris = 0;
     BuildStats stats;
     uint64_t device_bytes() const;
-    // Nodes and triangles share one device allocation, triangles behind the nodes at this byte
-    // offset; the traversal addresses both with 32-bit byte offsets from the node array.
+    // Nodes, triangles and instance records share one device allocation (in that order, at these
+    // byte offsets); the traversal addresses all three with 32-bit byte offsets from the nodes.
     uint64_t tri_offset() const { return (nodes.size() * sizeof(F4) + 255u) & ~(uint64_t)255u; }
-    uint64_t geometry_bytes() const { return tri_offset() + tris.size() * sizeof(F4); }
+    uint64_t inst_offset() const { return (tri_offset() + tris.size() * sizeof(F4) + 255u) & ~(uint64_t)255u; }
+    uint64_t geometry_bytes() const { return inst_offset() + insts.size() * sizeof(F4); }
 };
 
-// Largest node + triangle arrays of one scene (32-bit byte offsets in the traversal).
+// Largest node + triangle + instance arrays of one scene (32-bit byte offsets in the traversal).
 constexpr uint64_t kMaxGeometryBytes = (1ull << 32) - 256u;
 
 // Flattens and builds. leaf_size = max primitives per leaf (instances always get their own leaf);
