@@ -290,33 +290,34 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     auto t0 = std::chrono::steady_clock::now();
     HostScene& H = sc->host;
     {
-        // nodes, triangles and instance records in one allocation, in that order (the traversal
-        // fetches all three through 32-bit byte offsets from the node array); 256 spare bytes keep
-        // the loads of a node-less scene in bounds
-        const size_t nb = H.nodes.size() * sizeof(F4), tb = H.tris.size() * sizeof(F4), ib = H.insts.size() * sizeof(F4);
-        const size_t toff = (size_t)H.tri_offset(), ioff = (size_t)H.inst_offset();
-        if ((rc = sc->geo.alloc(ioff + ib + 256, this))) return rc;
-        if (nb) HIP_TRY(hipMemcpyAsync(sc->geo.p, H.nodes.data(), nb, hipMemcpyHostToDevice, stream));
-        if (tb) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + toff, H.tris.data(), tb, hipMemcpyHostToDevice, stream));
-        if (ib) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + ioff, H.insts.data(), ib, hipMemcpyHostToDevice, stream));
-        sc->geo_tri_off = toff;
-        sc->geo_inst_off = ioff;
+        // nodes, triangle, instance, sphere and quad records in one allocation, in that order (the
+        // traversal fetches them through 32-bit byte offsets from the node array); 256 spare bytes
+        // keep the shared loads of the last record (and of a node-less scene) in bounds
+        struct Part { const void* src; size_t bytes, off; };
+        const Part parts[5] = {{H.nodes.data(), H.nodes.size() * sizeof(F4), 0},
+                               {H.tris.data(), H.tris.size() * sizeof(F4), (size_t)H.tri_offset()},
+                               {H.insts.data(), H.insts.size() * sizeof(F4), (size_t)H.inst_offset()},
+                               {H.spheres.data(), H.spheres.size() * sizeof(F4), (size_t)H.sphere_offset()},
+                               {H.quads.data(), H.quads.size() * sizeof(F4), (size_t)H.quad_offset()}};
+        if ((rc = sc->geo.alloc((size_t)H.geometry_bytes() + 256, this))) return rc;
+        for (const Part& q : parts)
+            if (q.bytes) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + q.off, q.src, q.bytes, hipMemcpyHostToDevice, stream));
     }
     if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
     if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
-    if ((rc = upload_vec(sc->spheres, H.spheres, this, stream))) return rc;
     if ((rc = upload_vec(sc->sphere_aux, H.sphere_aux, this, stream))) return rc;
     if ((rc = upload_vec(sc->sphere_map, H.sphere_map, this, stream))) return rc;
-    if ((rc = upload_vec(sc->quads, H.quads, this, stream))) return rc;
     if ((rc = upload_vec(sc->mats, H.mats, this, stream))) return rc;
     if ((rc = upload_vec(sc->texs, H.texs, this, stream))) return rc;
     if ((rc = upload_vec(sc->texels, H.texels, this, stream))) return rc;
     if ((rc = upload_vec(sc->light_quads, H.light_quads, this, stream))) return rc;
     crt_dev_scene& ds = sc->dev;
     std::memset(&ds, 0, sizeof(ds));
-    ds.nodes = sc->geo.p; ds.tris = (char*)sc->geo.p + sc->geo_tri_off; ds.tri_normals = sc->tri_normals.p;
-    ds.tri_uvs = sc->tri_uvs.p; ds.spheres = sc->spheres.p; ds.sphere_aux = sc->sphere_aux.p; ds.sphere_map = sc->sphere_map.p;
-    ds.quads = sc->quads.p; ds.insts = (char*)sc->geo.p + sc->geo_inst_off; ds.mats = sc->mats.p; ds.texs = sc->texs.p;
+    char* geo = (char*)sc->geo.p;
+    ds.nodes = geo; ds.tris = geo + H.tri_offset(); ds.tri_normals = sc->tri_normals.p;
+    ds.tri_uvs = sc->tri_uvs.p; ds.spheres = geo + H.sphere_offset(); ds.sphere_aux = sc->sphere_aux.p;
+    ds.sphere_map = sc->sphere_map.p; ds.quads = geo + H.quad_offset(); ds.insts = geo + H.inst_offset();
+    ds.mats = sc->mats.p; ds.texs = sc->texs.p;
     ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;
     ds.num_lights = H.num_lights; ds.root = H.root; ds.num_quads = H.num_quads; ds.bvh_width = H.bvh_width;
     for (int k = 0; k < 3; ++k) ds.background[k] = H.background[k];

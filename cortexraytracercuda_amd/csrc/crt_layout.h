@@ -60,7 +60,8 @@
 //       SOLID: -, CHECKER: (type, even, odd, -), IMAGE: (type, width, height, pool offset / 4 dwords)
 struct crt_dev_scene {
     const void* nodes;      // float4
-    const void* tris;       // float4; in the nodes' allocation, behind them (at most 4 GiB together)
+    const void* tris;       // float4; tris, insts, spheres and quads follow the nodes in one allocation
+                            // (at most 4 GiB together)
     const void* tri_normals;// float4 x3 per triangle (optional)
     const void* tri_uvs;    // float2 x3 per triangle (optional)
     const void* spheres;    // float4

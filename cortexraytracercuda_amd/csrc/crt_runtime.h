@@ -56,9 +56,8 @@ struct DevBuf {
 
 struct DeviceScene {
     HostScene host;  // geometry arrays are dropped after upload; stats and small tables stay
-    DevBuf geo;                // BVH nodes, then triangles at geo_tri_off, instances at geo_inst_off
-    size_t geo_tri_off = 0, geo_inst_off = 0;
-    DevBuf tri_normals, tri_uvs, spheres, sphere_aux, sphere_map, quads, mats, texs, texels,
+    DevBuf geo;   // BVH nodes, then triangle, instance, sphere and quad records (HostScene offsets)
+    DevBuf tri_normals, tri_uvs, sphere_aux, sphere_map, mats, texs, texels,
         light_quads;
     std::vector<uint32_t> tri_gid, quad_gid;   // leaf order -> global primitive id
     std::vector<uint32_t> inst_gid;            // instance record (leaf order) -> caller's instance index

@@ -307,19 +307,24 @@ __device__ __forceinline__ bool quad_hit(V3 o, V3 d, const float4* q, float tmin
 
 struct HitRec { float t, u, v; uint32_t prim, inst; };
 
+// Traversal view of a scene: BVH nodes, triangle, instance, sphere and quad records live in one
+// allocation (in that order, below 4 GiB), addressed by 32-bit byte offsets from the node array.
 struct SceneView {
-    const float4* nodes; const float4* tris; const float4* spheres; const float4* quads;
-    const float4* insts; uint32_t root;
-    uint32_t tri_off;   // byte offsets of the triangle and instance arrays from the node array
-    uint32_t inst_off;  // (all three in one allocation)
+    const float4* nodes;
+    uint32_t root, tri_off, inst_off, sph_off, quad_off;
+    __device__ __forceinline__ const float4* quad(uint32_t i) const {
+        return (const float4*)((const char*)nodes + quad_off + i * 80u);
+    }
 };
 __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
     SceneView v;
-    v.nodes = (const float4*)sc->nodes; v.tris = (const float4*)sc->tris;
-    v.spheres = (const float4*)sc->spheres; v.quads = (const float4*)sc->quads; v.insts = (const float4*)sc->insts;
+    v.nodes = (const float4*)sc->nodes;
     v.root = sc->root;
-    v.tri_off = (uint32_t)((const char*)sc->tris - (const char*)sc->nodes);
-    v.inst_off = (uint32_t)((const char*)sc->insts - (const char*)sc->nodes);
+    const char* b = (const char*)sc->nodes;
+    v.tri_off = (uint32_t)((const char*)sc->tris - b);
+    v.inst_off = (uint32_t)((const char*)sc->insts - b);
+    v.sph_off = (uint32_t)((const char*)sc->spheres - b);
+    v.quad_off = (uint32_t)((const char*)sc->quads - b);
     return v;
 }
 
@@ -431,7 +436,8 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         }
     }
     // Leaf (reference written inline in the parent: no leaf-record fetch). With PRE, the first
-    // triangle's record (v0, v1, v2 in a0-a2) or the instance record (a0-a3) is already loaded.
+    // triangle's record (v0, v1, v2 in a0-a2), the first sphere (a0) or the instance record
+    // (a0-a3) is already loaded.
     template <bool PRE>
     __device__ __forceinline__ void visit_leaf(const SceneView& S, TravCount& tc, bool& next, float4 a0, float4 a1,
                                                float4 a2, float4 a3) {
@@ -442,19 +448,19 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         if (type == CRT_PRIM_TRI) {
             if (TWO_LEVEL && !(oct & 8u)) { tp = tri_pre(d); oct |= 8u; }
             for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* tp4 = S.tris + 3 * (first + k);
+                const uint32_t to = S.tri_off + (first + k) * 48u;
                 float t, bu, bv;
                 if (COUNT) tc.tris++;
                 const bool pre = PRE && k == 0u;
-                if (tri_hit(o, tp, pre ? a0 : ldg(tp4), pre ? a1 : ldg(tp4 + 1), pre ? a2 : ldg(tp4 + 2), tmin, best.t,
-                            t, bu, bv)) {
+                if (tri_hit(o, tp, pre ? a0 : ldg_at(S.nodes, to), pre ? a1 : ldg_at(S.nodes, to + 16u),
+                            pre ? a2 : ldg_at(S.nodes, to + 32u), tmin, best.t, t, bu, bv)) {
                     best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
                     best.inst = cur_inst;
                 }
             }
         } else if (type == CRT_PRIM_SPHERE) {
             for (uint32_t k = 0; k < cnt; ++k) {
-                float4 s = ldg(S.spheres + first + k);
+                const float4 s = (PRE && k == 0u) ? a0 : ldg_at(S.nodes, S.sph_off + (first + k) * 16u);
                 float t;
                 if (COUNT) tc.spheres++;
                 if (sphere_hit(o, d, s, tmin, best.t, t)) {
@@ -466,16 +472,16 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
             for (uint32_t k = 0; k < cnt; ++k) {
                 float t, qa, qb;
                 if (COUNT) tc.quads++;
-                if (quad_hit(o, d, S.quads + 5 * (first + k), tmin, best.t, camera, t, qa, qb)) {
+                if (quad_hit(o, d, S.quad(first + k), tmin, best.t, camera, t, qa, qb)) {
                     best.t = t; best.u = qa; best.v = qb; best.prim = (CRT_PRIM_QUAD << 30) | (first + k);
                     best.inst = cur_inst;
                 }
             }
         } else if (TWO_LEVEL && !in_blas) {
             // instance leaf (count 1): move the ray to object space and descend into the BLAS
-            const float4* ip = S.insts + 4 * first;
-            const float4 r0 = PRE ? a0 : ldg(ip), r1 = PRE ? a1 : ldg(ip + 1), r2 = PRE ? a2 : ldg(ip + 2);
-            const float4 m4 = PRE ? a3 : ldg(ip + 3);
+            const uint32_t io = S.inst_off + first * 64u;
+            const float4 r0 = PRE ? a0 : ldg_at(S.nodes, io), r1 = PRE ? a1 : ldg_at(S.nodes, io + 16u);
+            const float4 r2 = PRE ? a2 : ldg_at(S.nodes, io + 32u), m4 = PRE ? a3 : ldg_at(S.nodes, io + 48u);
             const int4 meta = make_int4((int)fbits(m4.x), (int)fbits(m4.y), (int)fbits(m4.z), (int)fbits(m4.w));
             if (COUNT) tc.insts++;
             (void)meta.y;
@@ -496,15 +502,16 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         if (WIDTH == 4) {
             // Node lanes and leaf lanes of a wave share one set of seven loads (one memory round
             // trip per step instead of one per branch taken): node lanes fetch their near / far
-            // planes and child refs, triangle lanes their triangle record and instance lanes their
-            // instance record, all addressed from the node array (the triangle and instance arrays
-            // follow it in the same allocation). Sphere and quad lanes read node 0 and load their
-            // primitives in the leaf branch.
+            // planes and child refs, triangle lanes their (first) triangle record, sphere lanes
+            // their (first) sphere and instance lanes their instance record, all addressed from
+            // the node array. Quad lanes read node 0 and load their quads in the leaf branch.
             const bool isn = node >= 0;
             const uint32_t ltype = CRT_LEAF_REF_TYPE(node), lfirst = CRT_LEAF_REF_FIRST(node);
             const bool ist = !isn && ltype == CRT_PRIM_TRI;
+            const bool iss = !isn && ltype == CRT_PRIM_SPHERE;
             const bool isi = TWO_LEVEL && !isn && ltype == CRT_PRIM_INST && !in_blas;
-            const uint32_t lb = ist ? S.tri_off + lfirst * 48u : (isi ? S.inst_off + lfirst * 64u : 0u);
+            const uint32_t lb = ist ? S.tri_off + lfirst * 48u
+                                    : (iss ? S.sph_off + lfirst * 16u : (isi ? S.inst_off + lfirst * 64u : 0u));
             const uint32_t l3 = isi ? lb + 48u : lb;
             const uint32_t nb = (uint32_t)node * 128u;   // node byte offset (scenes stay below 4 GiB of geometry)
             const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
@@ -780,7 +787,7 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
         trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc, o1);
         ovf = ovf || o1;
         if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
-        const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
+        const float4* q = S.quad(CRT_HIT_INDEX(h.prim));
         uint32_t fl = fbits(ldg(q + 1).w);
         if (!((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(ldg(q + 3))) < 0.0f))) break;
         tmin = h.t + 0.001f;
@@ -946,7 +953,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
             bool again = false;
             if (!pass_flag && h.prim != CRT_MISS && CRT_HIT_TYPE(h.prim) == CRT_PRIM_QUAD) {
                 // one-sided emitter seen from behind: re-trace from t + 0.001 (camera.cuh:179-183)
-                const float4* q = S.quads + 5 * CRT_HIT_INDEX(h.prim);
+                const float4* q = S.quad(CRT_HIT_INDEX(h.prim));
                 again = (fbits(ldg(q + 1).w) & CRT_QF_ONE_SIDED) && !(dot(tr.dw, v3(ldg(q + 3))) < 0.0f);
             }
             novf += tr.st.overflow ? 1u : 0u;

@@ -681,7 +681,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
         return CRT_ERR_INVALID_SCENE;
     }
     if (S.geometry_bytes() > kMaxGeometryBytes) {
-        err = "BVH nodes, triangles and instances take " + std::to_string(S.geometry_bytes()) +
+        err = "BVH nodes and primitive records take " + std::to_string(S.geometry_bytes()) +
               " bytes; one scene holds at most 4 GiB of them";
         return CRT_ERR_INVALID_SCENE;
     }

@@ -49,14 +49,18 @@ struct HostScene {
     uint64_t num_world_tris = 0;
     BuildStats stats;
     uint64_t device_bytes() const;
-    // Nodes, triangles and instance records share one device allocation (in that order, at these
-    // byte offsets); the traversal addresses all three with 32-bit byte offsets from the nodes.
-    uint64_t tri_offset() const { return (nodes.size() * sizeof(F4) + 255u) & ~(uint64_t)255u; }
-    uint64_t inst_offset() const { return (tri_offset() + tris.size() * sizeof(F4) + 255u) & ~(uint64_t)255u; }
-    uint64_t geometry_bytes() const { return inst_offset() + insts.size() * sizeof(F4); }
+    // Nodes, triangle, instance, sphere and quad records share one device allocation (in that
+    // order, each at a 256-byte aligned offset); the traversal addresses them with 32-bit byte
+    // offsets from the nodes.
+    static uint64_t align256(uint64_t b) { return (b + 255u) & ~(uint64_t)255u; }
+    uint64_t tri_offset() const { return align256(nodes.size() * sizeof(F4)); }
+    uint64_t inst_offset() const { return align256(tri_offset() + tris.size() * sizeof(F4)); }
+    uint64_t sphere_offset() const { return align256(inst_offset() + insts.size() * sizeof(F4)); }
+    uint64_t quad_offset() const { return align256(sphere_offset() + spheres.size() * sizeof(F4)); }
+    uint64_t geometry_bytes() const { return quad_offset() + quads.size() * sizeof(F4); }
 };
 
-// Largest node + triangle + instance arrays of one scene (32-bit byte offsets in the traversal).
+// Largest geometry allocation of one scene (32-bit byte offsets in the traversal).
 constexpr uint64_t kMaxGeometryBytes = (1ull << 32) - 256u;
 
 // Flattens and builds. leaf_size = max primitives per leaf (instances always get their own leaf);
