@@ -262,7 +262,7 @@ const char* crt_last_error(void);
 int crt_device_count(int32_t* count);
 
 /* Copies and uploads a scene (host BVH build + SoA upload). Limits per scene: 2^26 records of each
- * primitive type, and at most 4 GiB of BVH nodes, triangle and instance records together (about
+ * primitive type, and at most 4 GiB of BVH nodes and triangle, instance, sphere and quad records (about
  * 40M triangles with the default BVH); beyond them CRT_ERR_INVALID_SCENE. */
 int crt_scene_upload(crt_context* ctx, const crt_scene_desc* desc, crt_scene_id* out);
 int crt_scene_release(crt_context* ctx, crt_scene_id scene);
