@@ -115,3 +115,58 @@ def test_large_first_hit(renderer):
     print("max rel t error %.3g" % rel.max())
     assert rel.max() <= 1e-5
     renderer.release(sid)
+
+
+def _edge_scenes():
+    """Scenes that leave parts of the traversal's geometry allocation empty (no triangles, no
+    spheres, a single primitive): the leaf lanes' shared loads must stay in bounds and exact."""
+    from cortexraytracercuda_amd.scene import Scene
+    out = {}
+
+    def base():
+        s = Scene().set_camera((0.3, 0.2, 3.0), (0.0, 0.0, 0.0), vfov=50.0)
+        s.background = (0.5, 0.7, 1.0)
+        return s, s.lambertian("m", (0.6, 0.5, 0.4))
+
+    s, m = base()
+    s.sphere((0.0, 0.0, 0.0), 1.0, m)
+    out["one_sphere"] = s
+    s, m = base()
+    s.mesh([[-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], [[0, 1, 2]], m)
+    out["one_triangle"] = s
+    s, m = base()
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        s.sphere(tuple(rng.uniform(-1.5, 1.5, 3)), float(rng.uniform(0.05, 0.3)), m)
+    out["spheres_only"] = s
+    s, m = base()
+    s.quad((-1.0, -1.0, 0.0), (2.0, 0.0, 0.0), (0.0, 2.0, 0.0), m)
+    out["one_quad"] = s
+    return out
+
+
+@pytest.mark.parametrize("name", ["one_sphere", "one_triangle", "spheres_only", "one_quad"])
+def test_edge_scenes_first_hit_and_widths(name):
+    from cortexraytracercuda_amd import Renderer
+    s = _edge_scenes()[name]
+    W = H = 64
+    rays = O.primary_rays(s, W, H)
+    oprim, _, ot, _ = O.trace(s, rays, camera=True, precision=64, accel=1)
+    accs = []
+    for width in (4, 2):
+        with Renderer(bvh_width=width) as r:
+            sid = r.upload(s)
+            prim, t = r.trace_primary(sid, W, H)
+            prim = prim.reshape(-1)
+            t = t.reshape(-1).astype(np.float64)
+            agree = prim == oprim
+            assert (oprim >= 0).any() and (oprim < 0).any()
+            assert np.all(_tie_ok(t[~agree], ot[~agree])), (width, prim[~agree][:10], oprim[~agree][:10])
+            hit = agree & (oprim >= 0)
+            assert (np.abs(t[hit] - ot[hit]) / ot[hit]).max() <= 1e-5
+            _, acc, _ = r.render(sid, W, H, 4, 8, seed=3)
+            accs.append(acc)
+    # the four-wide and binary trees trace the same closest hits: identical images. Without lights a
+    # Lambertian bounce that picks light sampling gets a zero direction (as in the reference) and
+    # its pixel sums become NaN (mapped to 0 at tonemap): NaN in the same pixels counts as equal.
+    assert np.array_equal(accs[0], accs[1], equal_nan=True)
