@@ -330,7 +330,11 @@ __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
 
 struct TravCount { uint32_t nodes, leaves, tris, spheres, quads, insts; };
 
-// Slab test of one child box on (lo - o) * inv_d; Ize's robust factor widens the far distance.
+// Slab test of one child box on (lo - o) * inv_d; Ize's robust factor widens the far distance. A
+// box entered at +inf is a miss: an empty child slot (+inf planes) gives near = +inf and, for a ray
+// with nothing hit yet (tmax = FLT_MAX), far * 1.0000004 overflows to +inf as well, so the
+// comparison alone would accept it and the traversal would follow the empty reference. (The
+// four-wide test below needs no such term: its callers turn a hit at +inf into a miss.)
 __device__ __forceinline__ float box_near(float lx, float hx, float ly, float hy, float lz, float hz, V3 o, V3 id,
                                           float tmin, float tmax, bool& hit) {
     float ax = (lx - o.x) * id.x, bx = (hx - o.x) * id.x;
@@ -338,7 +342,7 @@ __device__ __forceinline__ float box_near(float lx, float hx, float ly, float hy
     float az = (lz - o.z) * id.z, bz = (hz - o.z) * id.z;
     float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
     float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
-    hit = nr <= fr * 1.0000004f;
+    hit = nr <= fr * 1.0000004f && nr < __builtin_inff();
     return nr;
 }
 

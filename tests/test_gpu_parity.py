@@ -170,3 +170,19 @@ def test_edge_scenes_first_hit_and_widths(name):
     # Lambertian bounce that picks light sampling gets a zero direction (as in the reference) and
     # its pixel sums become NaN (mapped to 0 at tonemap): NaN in the same pixels counts as equal.
     assert np.array_equal(accs[0], accs[1], equal_nan=True)
+
+
+@pytest.mark.parametrize("width", [4, 2])
+def test_empty_scene_renders_background(width):
+    """A scene without primitives: the root's children are empty slots (+inf boxes), which no ray
+    may enter, not even one with nothing hit yet (tmax = FLT_MAX) in the binary tree."""
+    from cortexraytracercuda_amd import Renderer
+    from cortexraytracercuda_amd.scene import Scene
+    s = Scene().set_camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    s.background = (0.5, 0.75, 1.0)
+    with Renderer(bvh_width=width) as r:
+        sid = r.upload(s)
+        prim, _ = r.trace_primary(sid, 32, 32)
+        assert np.all(prim < 0)
+        _, acc, _ = r.render(sid, 32, 32, 4, 8, seed=1)
+    assert np.all(acc[..., :3] == np.array([2.0, 3.0, 4.0], np.float32))
