@@ -160,7 +160,7 @@ int Context::init(const crt_config* cfg) {
     ray_sort = config.ray_sort == 1u ? 1u : 0u;
     if (stack_lds != 8 && stack_lds != 16 && stack_lds != 32)
         return set_error(CRT_ERR_INVALID_ARGUMENT, "stack_lds must be 8, 16 or 32");
-    grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 16u;
+    grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 24u;
     shade_grid_per_cu = config.shade_grid_per_cu ? std::min(config.shade_grid_per_cu, 256u) : 16u;
     extend_grid = (uint32_t)num_cus * grid_per_cu;
     shade_grid = (uint32_t)num_cus * shade_grid_per_cu;

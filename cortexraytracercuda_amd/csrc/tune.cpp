@@ -154,7 +154,7 @@ int Context::autotune(crt_scene_id scene, const crt_scene_desc* desc, const crt_
         return apply(bs);
     };
     if ((rc = sweep([](Shape& s, uint32_t v) { s.slots = v; }, {1u << 23, 1u << 24, 1u << 25, 1u << 26}))) return rc;
-    if ((rc = sweep([](Shape& s, uint32_t v) { s.grid = v; }, {16u, 32u, 64u}))) return rc;
+    if ((rc = sweep([](Shape& s, uint32_t v) { s.grid = v; }, {12u, 24u, 32u}))) return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.sgrid = v; }, {4u, 8u, 16u}))) return rc;
     if ((rc = sweep([](Shape& s, uint32_t v) { s.sub = v; }, {1u, 2u, 3u, 4u}))) return rc;
     // closest-hit work distribution: grid-stride (coherent waves) or per-lane refill from the

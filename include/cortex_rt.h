@@ -243,7 +243,7 @@ typedef struct crt_config {
                                    mode 3: 64-ray groups per fetch; 0 = default */
     uint32_t pass_paths;        /* path ids per pass (16 B of radiance each); 0 = min(2^27, 16 * max_paths)
                                    for blocking renders, min(2^29, 32 * max_paths) for the serving loop */
-    uint32_t grid_per_cu;       /* workgroups per CU of the closest-hit launches (256 threads each); 0 = 16 */
+    uint32_t grid_per_cu;       /* workgroups per CU of the closest-hit launches (256 threads each); 0 = 24 */
     uint32_t shade_grid_per_cu; /* workgroups per CU of the regenerate / shade launches; 0 = 16 */
     uint32_t sub_passes;        /* concurrent sub-passes (one stream each) of a pass, 1-4; 0 = 2 */
     uint32_t graphs;            /* serving loop iterations replayed from hipGraphs: 0 = default (on), 1 = on,
