@@ -142,3 +142,37 @@ def test_no_device_is_reported_not_crashed():
     rc = lib.crt_create(None, C.byref(h))
     assert rc == A.CRT_ERR_NO_DEVICE
     assert not h.value
+
+
+def _tiny_scenes():
+    out = {"empty": Scene()}
+    s = Scene()
+    m = s.lambertian("m", (0.5, 0.5, 0.5))
+    s.sphere((0.0, 0.0, 0.0), 1.0, m)
+    out["one_sphere"] = s
+    s = Scene()
+    m = s.lambertian("m", (0.5, 0.5, 0.5))
+    s.mesh([[-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], [[0, 1, 2]], m)
+    out["one_triangle"] = s
+    s = Scene()
+    m = s.lambertian("m", (0.5, 0.5, 0.5))
+    s.quad((-1.0, -1.0, 0.0), (2.0, 0.0, 0.0), (0.0, 2.0, 0.0), m)
+    out["one_quad"] = s
+    s = Scene()
+    m = s.lambertian("m", (0.5, 0.5, 0.5))
+    rng = np.random.default_rng(5)
+    for _ in range(37):
+        s.sphere(tuple(rng.uniform(-1.5, 1.5, 3)), float(rng.uniform(0.05, 0.3)), m)
+    out["spheres_only"] = s
+    return out
+
+
+@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_bvh_invariants_tiny_scenes(width, leaf):
+    # one-primitive and empty trees (the root wraps the leaf, or holds only empty child slots)
+    for name, s in _tiny_scenes().items():
+        rc, info = _check(s, leaf=leaf, width=width)
+        assert rc == 0, (name, A.load().crt_last_error())
+        assert info["num_prims"] == s.num_world_prims, name
+        assert info["num_nodes"] >= 1, name
