@@ -532,7 +532,8 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], u.s));
             if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)width, two_level, u.egrid, dsc, dreq, uniform_scene,
                              nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, hq, hiq, gq, nullptr, extend_mode,
-                             cin + CRT_CTL_WORK, extend_param, (int)stack_lds, (crt_dev_counters*)counters.p))
+                             cin + CRT_CTL_WORK, extend_param, (int)stack_lds, cin + CRT_CTL_SURV,
+                             (crt_dev_counters*)counters.p))
                 return set_error(CRT_ERR_HIP, "extend launch failed");
             if (time_stages) {
                 HIP_TRY(hipEventRecord(ev_ext[1], u.s));

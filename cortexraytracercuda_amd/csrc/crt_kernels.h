@@ -34,7 +34,9 @@ int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32
                  const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                  uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
-                 crt_dev_counters* counters);
+                 const uint32_t* first_camera, crt_dev_counters* counters);
+// first_camera (nullable): queue index where the camera rays of the launch begin (the survivors of
+// the last shade stage come first); the lane-refill kernel keeps camera-ray waves whole
 // two_level: an instanced scene has paths in the launch (selects the unbounded-register variant);
 // live_min (nullable, grid words): per block, the smallest path id still alive after the stage;
 // one_request: every path of the launch belongs to reqs[0]

@@ -881,14 +881,16 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
     wave_add(&counters->stack_overflow, novf);
 }
 
-// Closest-hit stage with per-lane refill: every wave owns a contiguous share of the queue, and a
-// lane whose ray is finished takes the wave's next ray instead of waiting for the slowest lane of
-// the wave. A refill is made when at least `refill_min` lanes are idle (or all are): the idle lanes
-// take the next rays of the share in lane order (mbcnt rank), so refilled rays are neighbours of
-// each other and of the rays in flight. The share pointer is wave-uniform and private: no atomics.
-// A wave leaves when its share is handed out and its lanes are idle; each traversal ends (every
-// node is visited at most once per query, at most one re-trace). Pays when rays of one wave differ
-// much in length (many primitive tests per ray, little coherence); costs coherence on primary rays.
+// Closest-hit stage with per-lane refill for the survivors of the last shade stage: every wave owns
+// a contiguous share of them, and a lane whose ray is finished takes the wave's next ray instead of
+// waiting for the slowest lane of the wave. A refill is made when at least `refill_min` lanes are
+// idle (or all are): the idle lanes take the next rays of the share in lane order (mbcnt rank), so
+// refilled rays are neighbours of each other and of the rays in flight. Camera rays (behind the
+// survivors when `first_camera` is given) keep whole-wave chunks in grid-stride order, whose
+// coherence refills would break. Share and chunk pointers are wave-uniform and private: no atomics.
+// A wave leaves when its share and chunks are handed out and its lanes are idle; each traversal
+// ends (every node is visited at most once per query, at most one re-trace). Pays when rays of one
+// wave differ much in length (many primitive tests per ray, little coherence).
 // one-level: 6 (80 VGPRs, a few per-ray spills; 2.5 % faster than at 5); two-level: unbounded
 // (at 6 the loop spills and the instanced scene takes 1.6x as long)
 constexpr int extend_pt_min_waves(bool count, bool uniform, bool two_level, int width) {
@@ -904,10 +906,12 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
                                                      const uint2* __restrict__ ids, float4* __restrict__ hit,
                                                      uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
                                                      uint32_t* __restrict__ zero_counter, uint32_t refill_min,
+                                                     const uint32_t* __restrict__ first_camera,
                                                      crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
     const uint32_t n = count_ptr ? *count_ptr : count_direct;
+    const uint32_t cam0 = first_camera ? *first_camera : n;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t* lds_col = s_stack + threadIdx.x;
     const GSlot gst{gstack, (blockIdx.x * BLOCK + threadIdx.x) * GS};
@@ -918,34 +922,49 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
     Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
     bool active = false, exhausted = false;
     uint32_t slot = 0, pass_flag = 0;
-    // this wave's share [nxt, wend) of the queue
+    // Survivors of the last shade stage, queue [0, nsec): this wave's contiguous share [nxt, wend),
+    // idle lanes refilled as soon as refill_min of them are idle. Camera rays, [nsec, n): chunks of
+    // 64 consecutive rays in grid-stride order (chunk c of the wave starts at nsec + (wid + c *
+    // nwaves) * 64, so neighbouring waves trace neighbouring pixels, as in the grid-stride
+    // kernel), a chunk taken when all 64 lanes are idle.
     const uint32_t nwaves = gridDim.x * (BLOCK / 64);
     const uint32_t wid = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    const uint32_t share = (n + nwaves - 1) / nwaves;
-    uint32_t nxt = min(wid * share, n);
-    const uint32_t wend = min(nxt + share, n);
-    if (nxt >= wend) exhausted = true;
+    const uint32_t nsec = min(cam0, n);
+    const uint32_t share = (nsec + nwaves - 1) / nwaves;
+    uint32_t nxt = min(wid * share, nsec);
+    const uint32_t wend = min(nxt + share, nsec);
+    uint32_t cnext = nsec + wid * 64u;
+    const uint32_t cstride = nwaves * 64u;
+    bool camera_phase = nxt >= wend;
     for (;;) {
         if (!exhausted) {
             uint64_t idle = __ballot(!active);
             uint32_t cnt = (uint32_t)__popcll(idle);
-            if (cnt >= refill_min || cnt == 64u) {
-                const uint32_t base = nxt;
-                nxt += cnt;
-                if (nxt >= wend) exhausted = true;
-                if (!active) {
-                    uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                    if (i < wend) {
-                        slot = queue ? queue[i] : i;
-                        float4 ro = ray_o[slot], rdv = ray_d[slot];
-                        if (!UNIFORM) S = scene_view(scenes + reqs[ids[slot].y >> 16].scene);
-                        tr.init(S, v3(ro), v3(rdv), 0.00001f, 3.402823466e+38f, (fbits(rdv.w) & ST_CAMERA) != 0,
-                                lds_col, gst);
-                        pass_flag = 0u;
-                        active = true;
+            uint32_t i = 0xffffffffu, lim = 0u;
+            if (camera_phase) {
+                if (cnt == 64u) {
+                    if (cnext >= n) {
+                        exhausted = true;
+                    } else {
+                        i = cnext + lane;
+                        lim = n;
+                        cnext += cstride;
                     }
                 }
+            } else if (cnt >= refill_min || cnt == 64u) {
+                const uint32_t base = nxt;
+                nxt += cnt;
+                if (nxt >= wend) camera_phase = true;
+                i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                lim = wend;
+            }
+            if (!active && i < lim) {
+                slot = queue ? queue[i] : i;
+                float4 ro = ray_o[slot], rdv = ray_d[slot];
+                if (!UNIFORM) S = scene_view(scenes + reqs[ids[slot].y >> 16].scene);
+                tr.init(S, v3(ro), v3(rdv), 0.00001f, 3.402823466e+38f, (fbits(rdv.w) & ST_CAMERA) != 0, lds_col, gst);
+                pass_flag = 0u;
+                active = true;
             }
         }
         if (__ballot(active) == 0ull) {
@@ -1547,7 +1566,7 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                             const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                             const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                             uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
-                            crt_dev_counters* counters) {
+                            const uint32_t* first_camera, crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
     if (mode != CRT_EXTEND_WAVE_CHUNK) work_counter = nullptr;
     if (mode == CRT_EXTEND_WAVE_CHUNK && !work_counter) mode = CRT_EXTEND_GRID_STRIDE;
@@ -1567,7 +1586,8 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
     {                                                                                                                   \
         auto kfn = k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                               \
         hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr, count_direct, ray_o,  \
-                           ray_d, ids, hit, hit_inst, gstack, zero_counter, param ? param : 1u, counters);             \
+                           ray_d, ids, hit, hit_inst, gstack, zero_counter, param ? param : 1u, first_camera,       \
+                           counters);                                                                                   \
     }
     // counting variants: one stack depth; timed variants: LDS ring of 8, 16 or 32 entries per lane
 #define CRT_EXT_C(U, W, T) \

@@ -262,7 +262,8 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
         return set_error(CRT_ERR_HIP, "regen launch failed");
     if (crt_k_extend(S.s, 0, (int)C.bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
                      A.ray_d, A.ids, (float4*)S.hit.p, (uint32_t*)S.hit_inst.p, (uint32_t*)S.gstack.p, nullptr,
-                     C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, (crt_dev_counters*)S.counters.p))
+                     C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, cin + CRT_CTL_SURV,
+                     (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
     if (crt_k_shade(S.s, mode, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.live_min.p,
