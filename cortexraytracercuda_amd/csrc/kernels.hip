@@ -891,10 +891,11 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
 // A wave leaves when its share and chunks are handed out and its lanes are idle; each traversal
 // ends (every node is visited at most once per query, at most one re-trace). Pays when rays of one
 // wave differ much in length (many primitive tests per ray, little coherence).
-// one-level: 6 (80 VGPRs, a few per-ray spills; 2.5 % faster than at 5); two-level: unbounded
-// (at 6 the loop spills and the instanced scene takes 1.6x as long)
+// one-level: 6 (80 VGPRs, a few per-ray spills; 2.5 % faster than at 5); two-level: 5 (96 VGPRs,
+// 7 % faster on the instanced scene than unbounded at 4; at 6 the loop spills and it takes 1.6x
+// as long)
 constexpr int extend_pt_min_waves(bool count, bool uniform, bool two_level, int width) {
-    return (count || two_level) ? 1 : 6;
+    return count ? 1 : (two_level ? 5 : 6);
 }
 
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
