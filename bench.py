@@ -265,6 +265,37 @@ def _megakernel_bench(r, sid, cfg, ranges):
     return {"mrays_s": mk_rays / mk_wall / 1e6, "ms_per_step": mk_wall * 1e3}
 
 
+def _progressive_served(r, sid, cfg, ranges, args):
+    """The progressive passes submitted to the serving loop together (each pass a request of its
+    own sample range), so one pass's drain overlaps the next passes' full launches; wall time from
+    the first submit to the last image. The images equal the blocking passes' (checked untimed)."""
+    W, H = cfg["width"], cfg["height"]
+
+    def step():
+        rids = [r.submit(sid, W, H, cfg["spp"], cfg["max_depth"], seed=1, image_only=True, sample_begin=sb,
+                         sample_count=sc) for sb, sc in ranges]
+        rays, imgs = 0, []
+        for rid in rids:
+            rgb, _, st = r.wait(rid, W, H, want_accum=False)
+            rays += st["rays"]
+            imgs.append(rgb)
+        return rays, imgs
+
+    for _ in range(args.warmup):
+        step()
+    rays = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rays += step()[0]
+    wall = time.perf_counter() - t0
+    _, imgs = step()
+    sb, sc = ranges[-1]
+    ref, _, _ = r.render(sid, W, H, cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb, sample_count=sc,
+                         want_accum=False)
+    return {"mrays_s": rays / wall / 1e6, "ms_per_step": wall / args.steps * 1e3, "passes": len(ranges),
+            "last_pass_identical_to_blocking": bool(np.array_equal(imgs[-1], ref))}
+
+
 def single_render_bench(scene, cfg, args, workload, progressive=False, workload_key=""):
     from cortexraytracercuda_amd import Renderer
     r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
@@ -306,6 +337,9 @@ def single_render_bench(scene, cfg, args, workload, progressive=False, workload_
         log("step %d: %.1f ms GPU, %d rays" % (k, gg, rr))
     wall = time.perf_counter() - t0
     value = rays / wall / 1e6
+    served = None
+    if progressive:
+        served = _progressive_served(r, sid, cfg, ranges, args)
     mk = None
     if not args.no_megakernel:
         mk = _megakernel_bench(r, sid, cfg, ranges)
@@ -326,7 +360,7 @@ def single_render_bench(scene, cfg, args, workload, progressive=False, workload_
             "rays_per_step": rays // args.steps, "segments_per_s": value * 1e6, "gpu_ms_per_step": gpu_ms / args.steps,
             "bvh": {"nodes": info["num_nodes"], "width": info["bvh_width"], "depth": info["max_depth"],
                     "build_ms": info["build_ms"], "upload_ms": info["upload_ms"], "device_bytes": info["device_bytes"]},
-            "roofline": roof, "autotune": tune,
+            "roofline": roof, "autotune": tune, "progressive_served": served,
             "megakernel": mk and dict(mk, wavefront_speedup=value / mk["mrays_s"])}
 
 

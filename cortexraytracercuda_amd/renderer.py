@@ -184,10 +184,13 @@ class Renderer:
         return prim, t
 
     # ---------------------------------------------------------------- serving
-    def submit(self, sid, width, height, spp, max_depth, seed=1, rng="pcg", image_only=False):
+    def submit(self, sid, width, height, spp, max_depth, seed=1, rng="pcg", image_only=False, sample_begin=0,
+               sample_count=0):
         """Queues a request in the serving loop. image_only: only the RGB8 image is copied back
-        (wait(..., want_accum=False) then)."""
-        rd = render_desc(width, height, spp, max_depth, seed, rng, flags=A.CRT_RF_IMAGE_ONLY if image_only else 0)
+        (wait(..., want_accum=False) then). sample_begin / sample_count: a sample range of the
+        request, as for render() (one pass of a progressive render)."""
+        rd = render_desc(width, height, spp, max_depth, seed, rng, sample_begin=sample_begin, sample_count=sample_count,
+                         flags=A.CRT_RF_IMAGE_ONLY if image_only else 0)
         rid = C.c_uint64()
         A.check(self.lib.crt_submit(self.h, sid, C.byref(rd), C.byref(rid)))
         return rid.value

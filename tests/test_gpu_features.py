@@ -176,6 +176,24 @@ def test_serving_image_only():
         assert e.value.code == A.CRT_ERR_INVALID_ARGUMENT
 
 
+def test_serving_sample_ranges_match_blocking():
+    """Progressive passes as served requests: each sample range gives the accumulation of the
+    blocking render of the same range, bit for bit, and the ranges sum to the whole request."""
+    with Renderer(max_paths=200000) as r:
+        a = r.upload(scenes.sphere_mesh_scene(5000, seed=2))
+        ranges = [(0, 1), (1, 3), (4, 5), (9, 7)]
+        rids = [r.submit(a, 72, 56, 16, 8, seed=9, sample_begin=sb, sample_count=sc) for sb, sc in ranges]
+        total = None
+        for (sb, sc), rid in zip(ranges, rids):
+            _, acc, _ = r.wait(rid, 72, 56)
+            _, ref, _ = r.render(a, 72, 56, 16, 8, seed=9, sample_begin=sb, sample_count=sc)
+            assert np.array_equal(acc, ref), (sb, sc)
+            assert np.all(acc[..., 3] == sc)
+            total = acc[..., :3] if total is None else total + acc[..., :3]
+        _, whole, _ = r.render(a, 72, 56, 16, 8, seed=9)
+        assert np.allclose(total, whole[..., :3], rtol=1e-5, atol=1e-5)
+
+
 @pytest.mark.parametrize("graphs", [1, 2])
 def test_serving_graph_replays_match_eager(graphs):
     """Serving iterations replayed from captured hipGraphs (graphs=1) or launched eagerly (2): many
