@@ -292,8 +292,22 @@ def _progressive_served(r, sid, cfg, ranges, args):
     sb, sc = ranges[-1]
     ref, _, _ = r.render(sid, W, H, cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb, sample_count=sc,
                          want_accum=False)
-    return {"mrays_s": rays / wall / 1e6, "ms_per_step": wall / args.steps * 1e3, "passes": len(ranges),
-            "last_pass_identical_to_blocking": bool(np.array_equal(imgs[-1], ref))}
+    served = {"mrays_s": rays / wall / 1e6, "ms_per_step": wall / args.steps * 1e3, "passes": len(ranges),
+              "last_pass_identical_to_blocking": bool(np.array_equal(imgs[-1], ref))}
+    # the same passes fused into one batched launch (one request per pass): all images at the end
+    jobs = [dict(sid=sid, width=W, height=H, spp=cfg["spp"], max_depth=cfg["max_depth"], seed=1, sample_begin=sb,
+                 sample_count=sc) for sb, sc in ranges]
+    for _ in range(args.warmup):
+        r.render_batch(jobs, want_accum=False)
+    rays = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs, st = r.render_batch(jobs, want_accum=False)
+        rays += st["rays"]
+    wall = time.perf_counter() - t0
+    served["batched"] = {"mrays_s": rays / wall / 1e6, "ms_per_step": wall / args.steps * 1e3,
+                         "last_pass_identical_to_blocking": bool(np.array_equal(outs[-1][0], ref))}
+    return served
 
 
 def single_render_bench(scene, cfg, args, workload, progressive=False, workload_key=""):

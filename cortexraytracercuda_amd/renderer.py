@@ -148,8 +148,9 @@ class Renderer:
         return self.counters()["extend_launches"]
 
     def render_batch(self, jobs, want_accum=True, time_stages=False):
-        """jobs: list of dicts (sid, width, height, spp, max_depth, seed, rng). Returns
-        ([(rgb, accum)], stats) with every request rendered in fused passes."""
+        """jobs: list of dicts (sid, width, height, spp, max_depth, seed, rng; optional sample_begin,
+        sample_count for one pass of a progressive render). Returns ([(rgb, accum)], stats) with every
+        request rendered in fused passes."""
         n = len(jobs)
         items = (A.crt_batch_item * n)()
         outs = []
@@ -159,6 +160,7 @@ class Renderer:
             acc = np.zeros((H, W, 4), np.float32) if want_accum else None
             items[k].scene = j["sid"]
             items[k].rd = render_desc(W, H, j["spp"], j["max_depth"], j.get("seed", 1), j.get("rng", "pcg"),
+                                      sample_begin=j.get("sample_begin", 0), sample_count=j.get("sample_count", 0),
                                       flags=A.CRT_RF_TIME_STAGES if time_stages else 0)
             items[k].out = A.crt_image(rgb.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(acc) if acc is not None else None)
             outs.append((rgb, acc))
