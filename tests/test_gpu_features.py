@@ -177,8 +177,9 @@ def test_serving_image_only():
 
 
 def test_serving_sample_ranges_match_blocking():
-    """Progressive passes as served requests: each sample range gives the accumulation of the
-    blocking render of the same range, bit for bit, and the ranges sum to the whole request."""
+    """Progressive passes as served requests and as one batched launch: each sample range gives
+    the accumulation of the blocking render of the same range, bit for bit, and the ranges sum to
+    the whole request."""
     with Renderer(max_paths=200000) as r:
         a = r.upload(scenes.sphere_mesh_scene(5000, seed=2))
         ranges = [(0, 1), (1, 3), (4, 5), (9, 7)]
@@ -192,6 +193,12 @@ def test_serving_sample_ranges_match_blocking():
             total = acc[..., :3] if total is None else total + acc[..., :3]
         _, whole, _ = r.render(a, 72, 56, 16, 8, seed=9)
         assert np.allclose(total, whole[..., :3], rtol=1e-5, atol=1e-5)
+        # the same passes fused into one batched launch
+        outs, _ = r.render_batch([dict(sid=a, width=72, height=56, spp=16, max_depth=8, seed=9, sample_begin=sb,
+                                       sample_count=sc) for sb, sc in ranges])
+        for (sb, sc), (_, acc) in zip(ranges, outs):
+            _, ref, _ = r.render(a, 72, 56, 16, 8, seed=9, sample_begin=sb, sample_count=sc)
+            assert np.array_equal(acc, ref), (sb, sc)
 
 
 @pytest.mark.parametrize("graphs", [1, 2])
