@@ -180,7 +180,7 @@ EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "
             "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
             "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
             "crt_memory_get_info", "crt_obj_load", "crt_obj_free", "crt_scenefile_load",
-            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes", "crt_autotune"]
+            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes", "crt_autotune", "crt_debug_fail_alloc"]
 
 _P = C.POINTER
 _SIGS = {
@@ -211,6 +211,7 @@ _SIGS = {
     "crt_scenefile_close": ([C.c_void_p], C.c_int),
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
     "crt_autotune": ([C.c_void_p, C.c_uint32, _P(crt_scene_desc), _P(crt_render_desc), _P(crt_tune_result)], C.c_int),
+    "crt_debug_fail_alloc": ([C.c_void_p, C.c_int32], C.c_int),
 }
 
 _lib = None

@@ -43,9 +43,12 @@ double now_s() {
 int DevBuf::alloc(size_t n, Context* ctx) {
     release();
     if (n == 0) return CRT_OK;
-    if (ctx && ctx->fail_alloc_after >= 0) {
-        if (ctx->fail_alloc_after == 0) return set_error(CRT_ERR_OUT_OF_MEMORY, "injected allocation failure");
-        ctx->fail_alloc_after--;
+    if (ctx && ctx->fail_alloc_after.load() >= 0) {
+        // test hook: the n-th allocation from now fails (n = 0: this one); each failure re-arms it
+        // at -1 so exactly one allocation fails
+        int left = ctx->fail_alloc_after.fetch_sub(1);
+        if (left == 0) return set_error(CRT_ERR_OUT_OF_MEMORY, "injected allocation failure");
+        if (left < 0) ctx->fail_alloc_after.store(-1);
     }
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) { p = nullptr; return set_error(CRT_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e)); }
@@ -305,7 +308,6 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     }
     if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
     if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
-    if ((rc = upload_vec(sc->sphere_aux, H.sphere_aux, this, stream))) return rc;
     if ((rc = upload_vec(sc->sphere_map, H.sphere_map, this, stream))) return rc;
     if ((rc = upload_vec(sc->mats, H.mats, this, stream))) return rc;
     if ((rc = upload_vec(sc->texs, H.texs, this, stream))) return rc;
@@ -315,7 +317,7 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     std::memset(&ds, 0, sizeof(ds));
     char* geo = (char*)sc->geo.p;
     ds.nodes = geo; ds.tris = geo + H.tri_offset(); ds.tri_normals = sc->tri_normals.p;
-    ds.tri_uvs = sc->tri_uvs.p; ds.spheres = geo + H.sphere_offset(); ds.sphere_aux = sc->sphere_aux.p;
+    ds.tri_uvs = sc->tri_uvs.p; ds.spheres = geo + H.sphere_offset();
     ds.sphere_map = sc->sphere_map.p; ds.quads = geo + H.quad_offset(); ds.insts = geo + H.inst_offset();
     ds.mats = sc->mats.p; ds.texs = sc->texs.p;
     ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;
@@ -621,7 +623,9 @@ int Context::read_counters(crt_dev_counters& c) {
 void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms) {
     if (!st) return;
     st->paths = paths;
-    st->rays = c.rays;
+    // every closest-hit query is a ray, pass-through re-traces included (the reference issues a
+    // second world hit for them, core/src/cameras/camera.cuh:180-183)
+    st->rays = c.rays + c.retraces;
     st->retraces = c.retraces;
     st->light_pdf_tests = c.light_tests;
     st->node_visits = c.node_visits;
@@ -674,7 +678,7 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     if ((rc = read_counters(c))) return rc;
     if (c.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
     fill_stats(stats, c, paths, gms, ext_ms, (now_s() - h0) * 1e3);
-    last.rays = c.rays; last.retraces = c.retraces; last.light_pdf_tests = c.light_tests;
+    last.rays = c.rays + c.retraces; last.retraces = c.retraces; last.light_pdf_tests = c.light_tests;
     last.node_visits = c.node_visits; last.leaf_visits = c.leaf_visits; last.tri_tests = c.tri_tests;
     last.sphere_tests = c.sphere_tests; last.quad_tests = c.quad_tests; last.inst_visits = c.inst_visits;
     last.extend_launches = (time_stages ? extend_launches - launches0 : 0);
@@ -894,6 +898,12 @@ int crt_memory_get_info(crt_context* c, crt_memory_info* out) {
     std::lock_guard<std::mutex> lk(ctx->srv_mutex);
     out->budget = ctx->budget; out->reserved = ctx->reserved; out->scenes = ctx->scene_bytes;
     out->requests = ctx->request_bytes;
+    return CRT_OK;
+}
+
+int crt_debug_fail_alloc(crt_context* c, int32_t n) {
+    if (!c) return set_error(CRT_ERR_INVALID_ARGUMENT, "null context");
+    reinterpret_cast<Context*>(c)->fail_alloc_after.store(n < 0 ? -1 : n);
     return CRT_OK;
 }
 

@@ -12,7 +12,7 @@
 //           or BVH4, 8 x float4 (128 B) per interior node, one float4 per axis bound over the four
 //             children: lo.x hi.x lo.y hi.y lo.z hi.z, then (child0..child3) as int, then padding
 //   tris    : 3 x float4 per triangle in leaf order: (v0, gid) (v1, material) (v2, flags)
-//   spheres : float4 (centre, radius) per sphere in leaf order + uint4 aux (gid, material, flags, -)
+//   spheres : 2 x float4 per sphere in leaf order: (centre, radius) (gid, material, -, -) as bits
 //   quads   : 5 x float4 per quad/area light: (Q, D) (u, flags) (v, area) (n, gid) (w, material)
 //   insts   : 4 x float4 per instance: three rows of world->object [R|t] and
 //             (blas_root, instance id, -, -) as int
@@ -64,8 +64,7 @@ struct crt_dev_scene {
                             // (at most 4 GiB together)
     const void* tri_normals;// float4 x3 per triangle (optional)
     const void* tri_uvs;    // float2 x3 per triangle (optional)
-    const void* spheres;    // float4
-    const void* sphere_aux; // uint4
+    const void* spheres;    // float4 x2 per sphere
     const void* sphere_map; // float2 x3 per sphere (scale, offset, repeat)
     const void* quads;      // float4
     const void* insts;      // float4

@@ -57,7 +57,7 @@ struct DevBuf {
 struct DeviceScene {
     HostScene host;  // geometry arrays are dropped after upload; stats and small tables stay
     DevBuf geo;   // BVH nodes, then triangle, instance, sphere and quad records (HostScene offsets)
-    DevBuf tri_normals, tri_uvs, sphere_aux, sphere_map, mats, texs, texels,
+    DevBuf tri_normals, tri_uvs, sphere_map, mats, texs, texels,
         light_quads;
     std::vector<uint32_t> tri_gid, quad_gid;   // leaf order -> global primitive id
     std::vector<uint32_t> inst_gid;            // instance record (leaf order) -> caller's instance index
@@ -195,7 +195,7 @@ struct Context {
     hipStream_t sub_stream[kMaxSubPasses - 1] = {};   // default priority, like `stream`
     hipEvent_t sub_done[kMaxSubPasses - 1] = {};
     hipEvent_t ev_pass_ready = nullptr;
-    int fail_alloc_after = -1;       // test hook: fail the n-th next allocation
+    std::atomic<int> fail_alloc_after{-1};   // test hook (crt_debug_fail_alloc): fail the n-th next device allocation
     // released per-request device buffers, by size class (power of two), at most kCacheBytes
     static constexpr size_t kCacheBytes = 4ull << 30;
     std::mutex cache_mutex;
@@ -262,6 +262,8 @@ struct Context {
     int server_append(Server& S, std::vector<ServeReq*>& act);
     int server_process_snapshot(Server& S, std::vector<ServeReq*>& act);
     void server_reset_pass(Server& S);
+    bool server_idle();
+    void server_drop_graphs();
 };
 
 void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms);

@@ -278,10 +278,11 @@ __device__ __forceinline__ bool sphere_hit(V3 o, V3 d, float4 s, float tmin, flo
 }
 
 // Quad / area light: quad::hit (core/src/primitives/quad.cuh:100-133) and directional_light::hit
-// (core/src/lights/directional_light.cuh:76-118), inclusive interval.
+// (core/src/lights/directional_light.cuh:76-118), inclusive interval. `key` = the quad's global id.
 __device__ __forceinline__ bool quad_hit(V3 o, V3 d, const float4* q, float tmin, float tmax, bool camera,
-                                         float& t, float& qa, float& qb) {
+                                         float& t, float& qa, float& qb, uint32_t& key) {
     float4 q0 = ldg(q), q1 = ldg(q + 1), q2 = ldg(q + 2), q3 = ldg(q + 3), q4 = ldg(q + 4);
+    key = fbits(q3.w);
     V3 n = v3(q3);
     float denom = dot(n, d);
     if (fabsf(denom) < 1e-8f) return false;
@@ -305,7 +306,19 @@ __device__ __forceinline__ bool quad_hit(V3 o, V3 d, const float4* q, float tmin
     return true;
 }
 
-struct HitRec { float t, u, v; uint32_t prim, inst; };
+// Closest hit with a deterministic tie rule: hits whose distances lie within a relative 2^-20 of each
+// other tie, and the lower global primitive id (`key`) wins, whatever order the traversal finds them
+// in (two triangles sharing an edge both report a ray through that edge, at distances a few ulps
+// apart). A candidate is tested up to best.t * kTieHi (boxes too, so a tying primitive in a later
+// box is still reached) and taken when it is nearer than best.t * kTieLo or ties with a lower key.
+// With no hit yet the key is 0, which nothing undercuts: a hit must then be nearer than the ray's
+// end by the relative margin. Both factors are exact in FP32.
+#define CRT_TIE_HI 1.00000095367431640625f   // 1 + 2^-20
+#define CRT_TIE_LO 0.99999904632568359375f   // 1 - 2^-20
+struct HitRec { float t, u, v; uint32_t prim, inst, key; };
+__device__ __forceinline__ bool hit_wins(float t, uint32_t key, const HitRec& b) {
+    return t < b.t * CRT_TIE_LO || key < b.key;
+}
 
 // Traversal view of a scene: BVH nodes, triangle, instance, sphere and quad records live in one
 // allocation (in that order, below 4 GiB), addressed by 32-bit byte offsets from the node array.
@@ -402,7 +415,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         st.init(lds_col, gstk);
         ow = ro; dw = rd;
         set_ray(ro, rd);
-        best.t = t1; best.prim = CRT_MISS; best.inst = CRT_NO_INST; best.u = 0.0f; best.v = 0.0f;
+        best.t = t1; best.prim = CRT_MISS; best.inst = CRT_NO_INST; best.u = 0.0f; best.v = 0.0f; best.key = 0u;
         cur_inst = CRT_NO_INST;
         in_blas = false; camera = cam;
         blas_level = 0;
@@ -414,7 +427,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     // leaves next set when no child is hit. Empty child slots have boxes at +inf, which no ray hits.
     __device__ __forceinline__ void visit4(float4 rff, float4 nx, float4 fx, float4 ny, float4 fy, float4 nz, float4 fz,
                                            bool& next) {
-        const float tf = best.t;
+        const float tf = best.t * CRT_TIE_HI;
         const int4 rf = make_int4((int)fbits(rff.x), (int)fbits(rff.y), (int)fbits(rff.z), (int)fbits(rff.w));
         bool h0, h1, h2, h3;
         float k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, id, tmin, tf, h0);
@@ -456,29 +469,36 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 float t, bu, bv;
                 if (COUNT) tc.tris++;
                 const bool pre = PRE && k == 0u;
-                if (tri_hit(o, tp, pre ? a0 : ldg_at(S.nodes, to), pre ? a1 : ldg_at(S.nodes, to + 16u),
-                            pre ? a2 : ldg_at(S.nodes, to + 32u), tmin, best.t, t, bu, bv)) {
+                const float4 va = pre ? a0 : ldg_at(S.nodes, to);
+                if (tri_hit(o, tp, va, pre ? a1 : ldg_at(S.nodes, to + 16u), pre ? a2 : ldg_at(S.nodes, to + 32u), tmin,
+                            best.t * CRT_TIE_HI, t, bu, bv) &&
+                    hit_wins(t, fbits(va.w), best)) {
                     best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
-                    best.inst = cur_inst;
+                    best.inst = cur_inst; best.key = fbits(va.w);
                 }
             }
         } else if (type == CRT_PRIM_SPHERE) {
             for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 s = (PRE && k == 0u) ? a0 : ldg_at(S.nodes, S.sph_off + (first + k) * 16u);
+                const bool pre = PRE && k == 0u;
+                const uint32_t so = S.sph_off + (first + k) * 32u;
+                const float4 s = pre ? a0 : ldg_at(S.nodes, so);
+                const uint32_t key = fbits((pre ? a1 : ldg_at(S.nodes, so + 16u)).x);
                 float t;
                 if (COUNT) tc.spheres++;
-                if (sphere_hit(o, d, s, tmin, best.t, t)) {
+                if (sphere_hit(o, d, s, tmin, best.t * CRT_TIE_HI, t) && hit_wins(t, key, best)) {
                     best.t = t; best.u = 0.0f; best.v = 0.0f; best.prim = (CRT_PRIM_SPHERE << 30) | (first + k);
-                    best.inst = cur_inst;
+                    best.inst = cur_inst; best.key = key;
                 }
             }
         } else if (type == CRT_PRIM_QUAD) {
             for (uint32_t k = 0; k < cnt; ++k) {
                 float t, qa, qb;
+                uint32_t key;
                 if (COUNT) tc.quads++;
-                if (quad_hit(o, d, S.quad(first + k), tmin, best.t, camera, t, qa, qb)) {
+                if (quad_hit(o, d, S.quad(first + k), tmin, best.t * CRT_TIE_HI, camera, t, qa, qb, key) &&
+                    hit_wins(t, key, best)) {
                     best.t = t; best.u = qa; best.v = qb; best.prim = (CRT_PRIM_QUAD << 30) | (first + k);
-                    best.inst = cur_inst;
+                    best.inst = cur_inst; best.key = key;
                 }
             }
         } else if (TWO_LEVEL && !in_blas) {
@@ -507,7 +527,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
             // Node lanes and leaf lanes of a wave share one set of seven loads (one memory round
             // trip per step instead of one per branch taken): node lanes fetch their near / far
             // planes and child refs, triangle lanes their (first) triangle record, sphere lanes
-            // their (first) sphere and instance lanes their instance record, all addressed from
+            // their (first) sphere record (both halves) and instance lanes their instance record, all addressed from
             // the node array. Quad lanes read node 0 and load their quads in the leaf branch.
             const bool isn = node >= 0;
             const uint32_t ltype = CRT_LEAF_REF_TYPE(node), lfirst = CRT_LEAF_REF_FIRST(node);
@@ -515,7 +535,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
             const bool iss = !isn && ltype == CRT_PRIM_SPHERE;
             const bool isi = TWO_LEVEL && !isn && ltype == CRT_PRIM_INST && !in_blas;
             const uint32_t lb = ist ? S.tri_off + lfirst * 48u
-                                    : (iss ? S.sph_off + lfirst * 16u : (isi ? S.inst_off + lfirst * 64u : 0u));
+                                    : (iss ? S.sph_off + lfirst * 32u : (isi ? S.inst_off + lfirst * 64u : 0u));
             const uint32_t l3 = isi ? lb + 48u : lb;
             const uint32_t nb = (uint32_t)node * 128u;   // node byte offset (scenes stay below 4 GiB of geometry)
             const uint32_t sx = (oct & 1u) << 4, sy = (oct & 2u) << 3, sz = (oct & 4u) << 2;
@@ -533,7 +553,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         } else if (node >= 0) {
             if (COUNT) tc.nodes++;
             if (WIDTH == 2) {
-                const float tf = best.t;
+                const float tf = best.t * CRT_TIE_HI;
                 const float4* np = S.nodes + 4 * (uint32_t)node;
                 float4 n0 = ldg(np), n1 = ldg(np + 1), n2 = ldg(np + 2);
                 int4 n3 = ldg_i4(np + 3);
@@ -654,7 +674,8 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, uint32_t v) {
 // ======================================================================================== kernels
 
 // Path state (SoA over path slots):
-//   ray_o  float4 (origin, time)           ray_d float4 (dir, state: depth | bounce << 8 | camera << 16)
+//   ray_o  float4 (origin, time)           ray_d float4 (dir, state: depth | bounce << 8 | camera << 16
+//                                                                 | pass-through re-traces so far << 24)
 //   thr    float4 (throughput, minstd state) rad   float4 (radiance sum, -)
 //   ids    uint2  (storage pixel, request << 16 | sample-in-request)
 //   hit    float4 (t, prim bits, u, v)      hit_inst uint32
@@ -662,6 +683,7 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, uint32_t v) {
 #define ST_DEPTH(s) ((s)&0xffu)
 #define ST_BOUNCE(s) (((s) >> 8) & 0xffu)
 #define ST_CAMERA 0x10000u
+#define ST_RETRACES(s) ((s) >> 24)
 
 __device__ __forceinline__ int find_request(const crt_dev_request* reqs, int nreq, uint32_t p) {
     int lo = 0, hi = nreq - 1;
@@ -1125,9 +1147,8 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
             }
             outward = nrm;
         } else if (type == CRT_PRIM_SPHERE) {
-            float4 s = ldg((const float4*)sc->spheres + idx);
-            uint4 aux = ldg((const uint4*)sc->sphere_aux + idx);
-            mat_id = aux.y;
+            float4 s = ldg((const float4*)sc->spheres + 2 * idx);
+            mat_id = fbits(ldg((const float4*)sc->spheres + 2 * idx + 1).y);
             sphere_c = v3(s); sphere_r = s.w;
             outward = (p - sphere_c) / sphere_r;
             need_uv_sphere = true;
@@ -1254,9 +1275,10 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                 for (uint32_t li = 0; li < nl; ++li) {
                     const float4* q = qd + 5 * ldg(sc->light_quads + li);
                     float tl, qa, qb;
+                    uint32_t qkey;
                     light_tests++;
                     float pv = 0.0f;
-                    if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb)) {
+                    if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb, qkey)) {
                         float dist2 = tl * tl * len2(dir);
                         float cosine = fabsf(dot(dir, v3(ldg(q + 3))) / vlen(dir));
                         pv = dist2 / (cosine * ldg(q + 2).w);
@@ -1325,13 +1347,15 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
             V3 att = v3(tv), L = v3(rv);
             uint32_t pix = idv.x, sample = idv.y & 0xffffu;
             uint32_t depth = ST_DEPTH(state), bounce = ST_BOUNCE(state);
+            // closest-hit queries of this segment beyond the first: the pass-through re-trace
+            const uint32_t retraces = ST_RETRACES(state) + ((iw & CRT_PASSTHROUGH) ? 1u : 0u);
             Rng<MODE> g;
             g.ms = fbits(tv.w);
             g.dyn = R.rng_mode;
             keep = shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
             if (keep) {
                 min_pid = min(min_pid, R.path_base + (pix - R.pix_begin) * R.sample_count + (sample - R.sample_begin));
-                uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8);
+                uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8) | (retraces << 24);
                 no = make_float4(o.x, o.y, o.z, ro.w);
                 nd = make_float4(d.x, d.y, d.z, bitsf(ns));
                 nt = make_float4(att.x, att.y, att.z, bitsf(g.ms));
@@ -1339,9 +1363,10 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
                 oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
             } else {
                 // finished: the radiance goes to the path's id, with the path's closest-hit queries
-                // (one per hit, plus the final miss) in .w for the per-request ray count
+                // (one per hit, the final miss and every pass-through re-trace) in .w for the
+                // per-request ray count
                 uint32_t pid = R.path_base + (pix - R.pix_begin) * R.sample_count + (sample - R.sample_begin);
-                const uint32_t rays = bounce + (fbits(hv.y) == CRT_MISS ? 1u : 0u);
+                const uint32_t rays = bounce + (fbits(hv.y) == CRT_MISS ? 1u : 0u) + retraces;
                 out_rad[pid] = make_float4(L.x, L.y, L.z, (float)rays);
                 if (g.minstd()) stg(R.rng_state + pix, g.ms);
             }

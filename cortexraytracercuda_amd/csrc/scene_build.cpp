@@ -207,7 +207,7 @@ struct Emitter {
         uint32_t first = 0;
         uint32_t type = (uint32_t)n.type;
         if (type == CRT_PRIM_TRI) first = (uint32_t)(S.tris.size() / 3);
-        else if (type == CRT_PRIM_SPHERE) first = (uint32_t)S.spheres.size();
+        else if (type == CRT_PRIM_SPHERE) first = (uint32_t)(S.spheres.size() / 2);
         else if (type == CRT_PRIM_QUAD) first = (uint32_t)(S.quads.size() / 5);
         else first = (uint32_t)(S.insts.size() / 4);
         for (uint32_t i = 0; i < n.count; ++i) {
@@ -388,7 +388,7 @@ bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1])
 
 uint64_t HostScene::device_bytes() const {
     return nodes.size() * 16 + tris.size() * 16 + tri_normals.size() * 16 + tri_uvs.size() * 8 +
-           spheres.size() * 16 + sphere_aux.size() * 16 + sphere_map.size() * 8 + quads.size() * 16 + insts.size() * 16 +
+           spheres.size() * 16 + sphere_map.size() * 8 + quads.size() * 16 + insts.size() * 16 +
            mats.size() * 16 + texs.size() * 16 + texels.size() + light_quads.size() * 4;
 }
 
@@ -636,7 +636,14 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     };
     em.emit_sphere = [&](uint32_t i) {
         const crt_sphere& s = d->spheres[i];
+        // record: (centre, radius) and (global id, material) as bits: the traversal reads the id
+        // with the same load as the sphere (tie rule), the shade stage the material
+        const uint32_t gid_m[2] = {sphere_gid0 + i, s.material};
+        F4 aux{0.0f, 0.0f, 0.0f, 0.0f};
+        std::memcpy(&aux.x, &gid_m[0], 4);
+        std::memcpy(&aux.y, &gid_m[1], 4);
         S.spheres.push_back(F4{s.center[0], s.center[1], s.center[2], s.radius});
+        S.spheres.push_back(aux);
         S.sphere_aux.push_back(U4{sphere_gid0 + i, s.material, 0u, 0u});
         S.sphere_map.push_back(F2{s.mapping.scale_u, s.mapping.scale_v});
         S.sphere_map.push_back(F2{s.mapping.offset_u, s.mapping.offset_v});
@@ -710,7 +717,7 @@ namespace crt {
 int check_host_scene(const HostScene& S, std::string& err) {
     const size_t per = S.bvh_width == 4 ? 8 : 4;
     const size_t nn = S.nodes.size() / per;
-    std::vector<uint32_t> cover_tri(S.tris.size() / 3, 0), cover_sph(S.spheres.size(), 0), cover_quad(S.quads.size() / 5, 0),
+    std::vector<uint32_t> cover_tri(S.tris.size() / 3, 0), cover_sph(S.spheres.size() / 2, 0), cover_quad(S.quads.size() / 5, 0),
         cover_inst(S.insts.size() / 4, 0);
     auto u32 = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
     struct B { float lo[3], hi[3]; };
@@ -772,7 +779,7 @@ int check_host_scene(const HostScene& S, std::string& err) {
             } else if (type == CRT_PRIM_SPHERE) {
                 if (k >= cover_sph.size()) { err = "sphere leaf out of range"; return false; }
                 cover_sph[k]++;
-                const F4& s = S.spheres[k];
+                const F4& s = S.spheres[2 * (size_t)k];
                 float r = std::fabs(s.w);
                 pb = B{{s.x - r, s.y - r, s.z - r}, {s.x + r, s.y + r, s.z + r}};
             } else if (type == CRT_PRIM_QUAD) {

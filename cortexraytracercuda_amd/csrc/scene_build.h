@@ -31,8 +31,8 @@ struct HostScene {
     std::vector<F4> tris;         // 3 per triangle
     std::vector<F4> tri_normals;  // 3 per triangle, only when a mesh has normals
     std::vector<F2> tri_uvs;      // 3 per triangle, only when a mesh has uvs
-    std::vector<F4> spheres;
-    std::vector<U4> sphere_aux;
+    std::vector<F4> spheres;      // 2 per sphere: (centre, radius), (global id, material, -, -) as bits
+    std::vector<U4> sphere_aux;   // host only: (global id, material) per sphere in leaf order
     std::vector<F2> sphere_map;   // 3 per sphere: scale, offset, repeat
     std::vector<F4> quads;        // 5 per quad / light
     std::vector<F4> insts;        // 4 per instance

@@ -38,6 +38,10 @@ int Context::submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_i
         if ((rc = server_init())) return rc;
         sc = find_scene(scene);
         if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+        // the serving iterations launch the traversal kernel of the context's BVH width (their
+        // captured graphs bake it in): a scene built at another width cannot join the open passes
+        if (sc->host.bvh_width != bvh_width)
+            return set_error(CRT_ERR_INVALID_ARGUMENT, "scene BVH width differs from the serving loop's");
         if (reserved + scene_bytes + need > budget)
             return set_error(CRT_ERR_OVER_BUDGET, "request can never fit the HBM budget");
         sc->refs++;
@@ -179,6 +183,32 @@ void Context::server_reset_pass(Server& S) {
     S.two_level = false;
     S.mixed_rng = false;
     (void)crt_k_pass_init(S.s, (uint32_t*)S.ctl.p, 2 * CRT_CTL_WORDS + 2);
+}
+
+// True when no request is waiting or on the device and every lane has drained (caller holds
+// gpu_mutex, so no request can be submitted meanwhile).
+bool Context::server_idle() {
+    std::lock_guard<std::mutex> lk(srv_mutex);
+    if (!srv_waiting.empty() || !srv_active.empty()) return false;
+    for (uint32_t l = 0; l < (uint32_t)kSrvLanes; ++l) {
+        const Server& S = srv[l];
+        if (S.ready && ((uint64_t)S.total > S.done_paths || S.processed < S.iter)) return false;
+    }
+    return true;
+}
+
+// Drops every captured serving iteration: they bake in the traversal variant (BVH width, stack
+// depth, work distribution), which the autotuner may change. Only called on an idle serving loop
+// (server_idle) with gpu_mutex held: the scheduler thread then waits for work and touches no graph.
+void Context::server_drop_graphs() {
+    for (Server& S : srv) {
+        if (!S.ready) continue;
+        (void)hipStreamSynchronize(S.s);
+        for (int t = 0; t < 4; ++t)
+            for (int k = 0; k < Server::kRing; ++k)
+                if (S.graph[t][k]) { (void)hipGraphExecDestroy(S.graph[t][k]); S.graph[t][k] = nullptr; }
+        for (int t = 0; t < 4; ++t) S.warm[t] = 0;
+    }
 }
 
 static void complete(ServeReq* r, int status, std::condition_variable& cv) {
@@ -391,11 +421,8 @@ void Context::server_complete(ServeReq* r, int st) {
             r->timing.rays = *rays_h;
     }
     if (r->ev_done) { (void)hipEventDestroy(r->ev_done); r->ev_done = nullptr; }
-    {
-        // lock order: gpu_mutex is never taken while srv_mutex is held
-        std::lock_guard<std::mutex> g(gpu_mutex);
-        r->job.reset();
-    }
+    // the job's buffers go back to the cache (own lock); its stream work has completed
+    r->job.reset();
     std::lock_guard<std::mutex> lk(srv_mutex);
     request_bytes -= Job::bytes_needed(r->rd);
     r->sc->refs--;
@@ -461,9 +488,12 @@ void Context::server_loop() {
                     request_bytes += need;
                     uint32_t best = 0;
                     for (uint32_t l = 1; l < L; ++l) if (lane_load[l] < lane_load[best]) best = l;
-                    const uint64_t sq = (uint64_t)std::sqrt((double)r->rd.spp);
+                    // load = the paths the request adds: its clamped sample range, not all spp
+                    const uint64_t sq = (uint64_t)std::sqrt((double)r->rd.spp), ns = sq * sq;
+                    const uint64_t sb = std::min<uint64_t>(r->rd.sample_begin, ns);
+                    const uint64_t se = r->rd.sample_count ? std::min<uint64_t>(ns, sb + r->rd.sample_count) : ns;
                     r->lane = (int)best;
-                    r->load = (uint64_t)r->rd.width * r->rd.height * sq * sq;
+                    r->load = (uint64_t)r->rd.width * r->rd.height * (se - sb);
                     lane_load[best] += r->load;
                     admitted.push_back(r);
                     it = srv_waiting.erase(it);
@@ -473,12 +503,11 @@ void Context::server_loop() {
             }
         }
         for (ServeReq* r : admitted) {
-            int rc;
-            {
-                std::lock_guard<std::mutex> g(gpu_mutex);
-                r->job = std::make_unique<Job>();
-                rc = r->job->setup(this, r->sc, r->rd, srv[r->lane].s);
-            }
+            // no gpu_mutex: the setup touches the buffer cache (own lock), the request's scene
+            // (pinned by its reference) and the lane's own stream, so a blocking render holding
+            // the context lock does not stall admission
+            r->job = std::make_unique<Job>();
+            const int rc = r->job->setup(this, r->sc, r->rd, srv[r->lane].s);
             r->timing.admit_s = now_s();
             std::lock_guard<std::mutex> lk(srv_mutex);
             if (rc) {

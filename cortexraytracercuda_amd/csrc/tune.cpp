@@ -52,6 +52,13 @@ int Context::autotune(crt_scene_id scene, const crt_scene_desc* desc, const crt_
     DeviceScene* sc = find_scene(scene);
     if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
     if (sc->refs > 0) return set_error(CRT_ERR_INVALID_ARGUMENT, "scene has requests in flight");
+    // the serving loop's captured iterations bake in the settings tuned here: tune only while it
+    // is idle, and drop its graphs afterwards (they are captured again from the new settings)
+    if (!server_idle()) return set_error(CRT_ERR_INVALID_ARGUMENT, "serving requests in flight");
+    struct DropGraphs {
+        Context* c;
+        ~DropGraphs() { c->server_drop_graphs(); }
+    } drop_graphs{this};
     if (rd->flags & CRT_RENDER_MEGAKERNEL) return set_error(CRT_ERR_INVALID_ARGUMENT, "the megakernel render has no tunables");
     std::memset(out, 0, sizeof(*out));
 

@@ -210,8 +210,10 @@ typedef struct crt_image {
 
 typedef struct crt_stats {
     uint64_t paths;            /* camera paths traced */
-    uint64_t rays;             /* closest-hit queries (= path segments; the estimator has no shadow rays) */
-    uint64_t retraces;         /* pass-through re-traces behind one-sided lights (core/src/cameras/camera.cuh:180-183) */
+    uint64_t rays;             /* closest-hit queries: one per path segment plus every pass-through re-trace
+                                  (the estimator has no shadow rays) */
+    uint64_t retraces;         /* pass-through re-traces behind one-sided lights (core/src/cameras/camera.cuh:180-183),
+                                  included in rays */
     uint64_t light_pdf_tests;  /* analytic light-PDF intersection tests */
     uint64_t node_visits;      /* interior nodes fetched (counting kernels only, else 0) */
     uint64_t prim_tests;       /* primitive intersection tests (counting kernels only, else 0) */
@@ -397,6 +399,12 @@ typedef struct crt_memory_info {
     uint64_t budget, reserved, scenes, requests;
 } crt_memory_info;
 int crt_memory_get_info(crt_context* ctx, crt_memory_info* out);
+
+/* Test hook for the out-of-memory paths (no counterpart in the reference, whose allocation
+ * failures exit(99), core/src/gpu.cu:119-129): the n-th device allocation the context makes from
+ * now on (n = 0: the next one) fails with CRT_ERR_OUT_OF_MEMORY, once; n < 0 disarms it. Per-request
+ * buffers reused from the context's cache are not allocations. */
+int crt_debug_fail_alloc(crt_context* ctx, int32_t n);
 
 /* Sizes of the public structs as compiled into the library (binding self-check). */
 int crt_struct_sizes(uint64_t* sizes, uint32_t n);

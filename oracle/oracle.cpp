@@ -365,7 +365,7 @@ template <class R> static int build_scene(Scene<R>& s, const crt_scene_desc* d, 
 // degrees_to_radians = deg * M_PI / 180 (core/src/utilities/math_utils.cuh:55-58).
 template <class R> static void setup_camera(Scene<R>& s, const crt_camera& c) {
     V3<R> lookfrom = from3<R>(c.lookfrom), lookat = from3<R>(c.lookat), vup = from3<R>(c.vup);
-    R theta = R(c.vfov_deg) * R(3.14159265358979323846f) / R(180.0f);
+    R theta = R(c.vfov_deg) * R(3.14159265358979323846) / R(180.0f);
     R focus = R(c.focus_dist), ratio = R(c.aspect_ratio);
     s.time0 = R(c.time0); s.time1 = R(c.time1);
     s.lens_radius = R(c.aperture) / R(2);
@@ -448,12 +448,14 @@ template <class R> static void set_face_normal(Hit<R>& h, const Ray<R>& r, V3<R>
     h.normal = h.front ? outward : -outward;
 }
 
-// get_sphere_uv (core/src/utilities/uvmapping.cuh:204-228)
+// get_sphere_uv (core/src/utilities/uvmapping.cuh:204-228). M_PI is the float literal of
+// core/src/misc/constants.cuh:35, so the reference evaluates this in float (R = float here); the FP64
+// oracle uses pi in double.
 template <class R> static void sphere_uv(V3<R> p, const crt_uvmap& m, R& u, R& v) {
     R theta = std::acos(-p.y);
-    R phi = std::atan2(-p.z, p.x) + R(3.14159265358979323846f);
-    R s = phi / (R(2) * R(3.14159265358979323846f));
-    R t = theta / R(3.14159265358979323846f);
+    R phi = std::atan2(-p.z, p.x) + R(3.14159265358979323846);
+    R s = phi / (R(2) * R(3.14159265358979323846));
+    R t = theta / R(3.14159265358979323846);
     s = std::fmod(s * R(m.repeat_u), R(1));
     t = std::fmod(t * R(m.repeat_v), R(1));
     u = R(m.scale_u) * s + R(m.offset_u);
@@ -622,9 +624,54 @@ template <class R> static bool bvh_hit(const Scene<R>& s, int node, bool leaf, c
     return hl || hr;
 }
 
+// The build's deterministic closest hit (oracle64 only; refcpu keeps the reference's traversal
+// order): hits within a relative 2^-20 of each other in t tie and the lower global primitive id
+// wins. A candidate is tested up to bt * (1 + 2^-20) and taken when it is nearer than
+// bt * (1 - 2^-20) or ties with a lower id; with no hit yet the id to beat is 0, so a first hit
+// must lie below tmax by the same margin. Both factors are exact in FP32 and FP64.
+constexpr double kTieHi = 1.0 + 1.0 / 1048576.0, kTieLo = 1.0 - 1.0 / 1048576.0;
+template <class R> static inline bool tie_wins(R t, uint32_t key, R bt, uint32_t bkey) {
+    return t < bt * R(kTieLo) || key < bkey;
+}
+template <class R> static void bvh_hit_tie(const Scene<R>& s, int node, bool leaf, const Ray<R>& r, R tmin, bool camera,
+                                           R& bt, uint32_t& bkey, Hit<R>& rec, bool& any, Counters& cnt) {
+    if (leaf) {
+        cnt.prims++;
+        Hit<R> tmp;
+        const uint32_t key = (uint32_t)s.objs[node].gid;
+        if (obj_hit(s, node, r, tmin, bt * R(kTieHi), camera, tmp) && tie_wins(tmp.t, key, bt, bkey)) {
+            rec = tmp; bt = tmp.t; bkey = key; any = true;
+        }
+        return;
+    }
+    const auto& n = s.nodes[node];
+    cnt.nodes++;
+    if (!box_hit(n.box, r, tmin, bt * R(kTieHi))) return;
+    bvh_hit_tie(s, n.left, n.leaf_l, r, tmin, camera, bt, bkey, rec, any, cnt);
+    bvh_hit_tie(s, n.right, n.leaf_r, r, tmin, camera, bt, bkey, rec, any, cnt);
+}
+
 // hittable_list::hit over the world (core/src/primitives/hittable_list.cuh:149-166).
 template <class R> static bool world_hit(const Scene<R>& s, const Ray<R>& r, R tmin, R tmax, bool camera, Hit<R>& rec,
                                          Counters& cnt) {
+    if (!s.ref_style) {
+        R bt = tmax;
+        uint32_t bkey = 0;
+        bool any = false;
+        if (s.accel == 1 || s.root < 0) {
+            for (size_t i = 0; i < s.objs.size(); ++i) {
+                cnt.prims++;
+                Hit<R> tmp;
+                const uint32_t key = (uint32_t)s.objs[i].gid;
+                if (obj_hit(s, (int)i, r, tmin, bt * R(kTieHi), camera, tmp) && tie_wins(tmp.t, key, bt, bkey)) {
+                    rec = tmp; bt = tmp.t; bkey = key; any = true;
+                }
+            }
+        } else {
+            bvh_hit_tie(s, s.root, s.root_is_leaf, r, tmin, camera, bt, bkey, rec, any, cnt);
+        }
+        return any;
+    }
     Hit<R> tmp;  // fresh record: hit_point starts at (0,0,0)
     bool any = false;
     if (s.accel == 1 || s.root < 0) {
@@ -739,6 +786,46 @@ template <class R> static V3<R> lights_random(const Scene<R>& s, V3<R> o, Rng& g
     return p - o;
 }
 
+// ------------------------------------------------------------------ scatter directions
+// metal::scatter (core/src/materials/metal.cuh:57-65): mirror of the unit incoming direction about
+// the normal (glm::reflect) plus fuzz times a point of the unit ball; never absorbs.
+template <class R> static V3<R> metal_dir(V3<R> d, V3<R> n, R fuzz, Rng& g) {
+    V3<R> refl = reflect(unit(d), n);
+    return refl + fuzz * random_in_unit_sphere<R>(g);
+}
+
+// dielectric::scatter (core/src/materials/dielectric.cuh:32-51) with reflectance (Schlick on the
+// refraction ratio, :57-63). `draw` stands for get_real(rng): it is consumed only when refraction
+// is possible (the || short-circuits on total internal reflection).
+template <class R, class Draw> static V3<R> dielectric_dir(V3<R> d, V3<R> n, bool front, R ir, Draw draw,
+                                                         bool* reflected) {
+    R ratio = front ? (R(1.0f) / ir) : ir;
+    V3<R> ud = unit(d);
+    R dn = dot(-ud, n);
+    R cos_t = dn < R(1.0f) ? dn : R(1.0f);  // ffmin (core/src/misc/color.cuh:222-223)
+    R sin_t = std::sqrt(R(1.0f) - cos_t * cos_t);
+    bool cannot = ratio * sin_t > R(1.0);
+    bool do_reflect = cannot;
+    if (!cannot) {
+        R r0 = (R(1) - ratio) / (R(1) + ratio);
+        r0 = r0 * r0;
+        R refl = r0 + (R(1) - r0) * std::pow(R(1) - cos_t, R(5));
+        do_reflect = refl > R(draw());
+    }
+    if (reflected) *reflected = do_reflect;
+    return do_reflect ? reflect(ud, n) : refract(ud, n, ratio);
+}
+
+// Camera ray of image-plane coordinates (u, v): perspective_camera::get_ray
+// (core/src/cameras/perspective_camera.cuh:106-113): lens offset from random_in_unit_disk scaled by
+// the lens radius in the camera's (u, v) plane, direction to the focus-plane point, time draw.
+template <class R> static Ray<R> camera_ray(const Scene<R>& s, R u, R v, Rng& g) {
+    V3<R> rdisk = s.lens_radius * random_in_unit_disk<R>(g);
+    V3<R> off = s.cu * rdisk.x + s.cv * rdisk.y;
+    R tm = s.time0 + R(g.uniform()) * (s.time1 - s.time0);
+    return Ray<R>{s.origin + off, s.llc + u * s.horizontal + v * s.vertical - s.origin - off, tm};
+}
+
 // ------------------------------------------------------------------ estimator
 // camera::ray_color, iterative version (core/src/cameras/camera.cuh:149-264).
 template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max_depth, Rng& g, Counters& cnt) {
@@ -781,30 +868,12 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
         result = result + att * emitted;
         if (m.type == CRT_MAT_METAL) {
             V3<R> alb = albedo_of(s, m, rec);
-            V3<R> refl = reflect(unit(ray.d), rec.normal);
-            V3<R> dir = refl + R(m.fuzz) * random_in_unit_sphere<R>(g);
-            ray = Ray<R>{rec.p, dir, ray.time};
+            ray = Ray<R>{rec.p, metal_dir(ray.d, rec.normal, R(m.fuzz), g), ray.time};
             att = att * alb;
             continue;
         }
         if (m.type == CRT_MAT_DIELECTRIC) {
-            R ir = R(m.ior);
-            R ratio = rec.front ? (R(1.0f) / ir) : ir;
-            V3<R> ud = unit(ray.d);
-            R dn = dot(-ud, rec.normal);
-            R cos_t = dn < R(1.0f) ? dn : R(1.0f);  // ffmin (core/src/misc/color.cuh:222-223)
-            R sin_t = std::sqrt(R(1.0f) - cos_t * cos_t);
-            bool cannot = ratio * sin_t > R(1.0);
-            V3<R> dir;
-            bool do_reflect = cannot;
-            if (!cannot) {
-                // reflectance: Schlick with the ratio (dielectric.cuh:57-63); drawn only if refraction is possible
-                R r0 = (R(1) - ratio) / (R(1) + ratio);
-                r0 = r0 * r0;
-                R refl = r0 + (R(1) - r0) * std::pow(R(1) - cos_t, R(5));
-                do_reflect = refl > R(g.uniform());
-            }
-            dir = do_reflect ? reflect(ud, rec.normal) : refract(ud, rec.normal, ratio);
+            V3<R> dir = dielectric_dir(ray.d, rec.normal, rec.front, R(m.ior), [&]() { return g.uniform(); }, nullptr);
             ray = Ray<R>{rec.p, dir, ray.time};
             continue;  // attenuation 1
         }
@@ -816,11 +885,11 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
         if (g.uniform() < 0.5f) dir = lights_random(s, rec.p, g);
         else dir = uvw.local(random_cosine_direction<R>(g));
         R pl = lights_pdf_value(s, rec.p, dir, cnt);
-        R cv = dot(unit(dir), uvw.w) / R(3.14159265358979323846f);
+        R cv = dot(unit(dir), uvw.w) / R(3.14159265358979323846);
         R pc = R(0) > cv ? R(0) : cv;  // ffmax(0, x)
         R pdf = R(0.5f) * pl + (R(1.0f) - R(0.5f)) * pc;
         R ct = dot(rec.normal, unit(dir));
-        R spdf = ct < 0 ? R(0) : ct / R(3.14159265358979323846f);
+        R spdf = ct < 0 ? R(0) : ct / R(3.14159265358979323846);
         // colour / float is (1/t) * colour (core/src/misc/color.cuh:105-108)
         att = att * ((R(1) / pdf) * (alb * spdf));
         ray = Ray<R>{rec.p, dir, ray.time};
@@ -879,11 +948,7 @@ template <class R> static int render_impl(const crt_scene_desc* d, const crt_ren
                         float r = g.uniform();
                         R u = (R(i) + (R(si) + R(r)) / R(sq)) / R(W);
                         R v = (R(j) + (R(sj) + R(r)) / R(sq)) / R(H);
-                        // perspective_camera::get_ray (perspective_camera.cuh:106-113)
-                        V3<R> rdisk = s.lens_radius * random_in_unit_disk<R>(g);
-                        V3<R> off = s.cu * rdisk.x + s.cv * rdisk.y;
-                        R tm = s.time0 + R(g.uniform()) * (s.time1 - s.time0);
-                        Ray<R> ray{s.origin + off, s.llc + u * s.horizontal + v * s.vertical - s.origin - off, tm};
+                        Ray<R> ray = camera_ray(s, u, v, g);
                         V3<R> L = ray_color(s, ray, (int)rd->max_depth, g, cnt);
                         pc = pc + L;
                     }
@@ -1052,6 +1117,94 @@ int oracle_ref_bvh_info(const crt_scene_desc* d, uint64_t* nodes, uint32_t* dept
         return a > b ? a : b;
     };
     *depth = s.root >= 0 ? dep(s.root) : 0;
+    return CRT_OK;
+}
+
+/* ---- entry points of single oracle functions, for the known-answer and closed-form tests ---- */
+
+/* Camera basis of setup_camera (FP64): origin, lower-left corner, horizontal, vertical, u, v, w
+ * (3 doubles each) and the lens radius: 22 doubles. */
+int oracle_camera_basis(const crt_scene_desc* d, double* out) {
+    orc::Scene<double> s;
+    orc::setup_camera(s, d->camera);
+    const orc::V3<double> v[7] = {s.origin, s.llc, s.horizontal, s.vertical, s.cu, s.cv, s.cw};
+    for (int k = 0; k < 7; ++k) { out[3 * k] = v[k].x; out[3 * k + 1] = v[k].y; out[3 * k + 2] = v[k].z; }
+    out[21] = s.lens_radius;
+    return CRT_OK;
+}
+
+/* camera_ray at image-plane points uv[2k], uv[2k+1] (FP64), draws from the keyed PCG stream
+ * (seed, pixel k, sample 0, bounce 0): rows of 8 doubles (origin, time, direction, 0). */
+int oracle_camera_rays(const crt_scene_desc* d, uint64_t n, const double* uv, uint64_t seed, double* out) {
+    orc::Scene<double> s;
+    orc::setup_camera(s, d->camera);
+    for (uint64_t k = 0; k < n; ++k) {
+        orc::Rng g;
+        g.mode = CRT_RNG_PCG; g.seed = seed; g.pixel = (uint32_t)k; g.sample = 0;
+        g.bounce(0);
+        orc::Ray<double> r = orc::camera_ray(s, uv[2 * k], uv[2 * k + 1], g);
+        double* q = out + 8 * k;
+        q[0] = r.o.x; q[1] = r.o.y; q[2] = r.o.z; q[3] = r.time; q[4] = r.d.x; q[5] = r.d.y; q[6] = r.d.z; q[7] = 0;
+    }
+    return CRT_OK;
+}
+
+/* metal_dir (FP64) for n incoming directions / normals, draws from keyed PCG (seed, k, 0, 0). */
+int oracle_metal(uint64_t n, const double* dirs, const double* nrms, double fuzz, uint64_t seed, double* out) {
+    for (uint64_t k = 0; k < n; ++k) {
+        orc::Rng g;
+        g.mode = CRT_RNG_PCG; g.seed = seed; g.pixel = (uint32_t)k; g.sample = 0;
+        g.bounce(0);
+        orc::V3<double> dd(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]), nn(nrms[3 * k], nrms[3 * k + 1], nrms[3 * k + 2]);
+        orc::V3<double> r = orc::metal_dir(dd, nn, fuzz, g);
+        out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
+    }
+    return CRT_OK;
+}
+
+/* dielectric_dir (FP64) with the draw given per call (draws[k]); reflected[k] = 1 when the ray
+ * reflects, drawn[k] = 1 when the draw was consumed. */
+int oracle_dielectric(uint64_t n, const double* dirs, const double* nrms, const int32_t* front, double ir,
+                      const double* draws, double* out, int32_t* reflected, int32_t* drawn) {
+    for (uint64_t k = 0; k < n; ++k) {
+        orc::V3<double> dd(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]), nn(nrms[3 * k], nrms[3 * k + 1], nrms[3 * k + 2]);
+        bool refl = false;
+        int used = 0;
+        orc::V3<double> r = orc::dielectric_dir(dd, nn, front[k] != 0, ir, [&]() { used = 1; return draws[k]; }, &refl);
+        out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
+        reflected[k] = refl ? 1 : 0;
+        drawn[k] = used;
+    }
+    return CRT_OK;
+}
+
+/* get_sphere_uv at unit-sphere points p (precision 32 or 64). */
+int oracle_sphere_uv(uint64_t n, const double* p, const crt_uvmap* map, int32_t precision, double* uv) {
+    for (uint64_t k = 0; k < n; ++k) {
+        if (precision == 64) {
+            double u, v;
+            orc::sphere_uv(orc::V3<double>(p[3 * k], p[3 * k + 1], p[3 * k + 2]), *map, u, v);
+            uv[2 * k] = u; uv[2 * k + 1] = v;
+        } else {
+            float u, v;
+            orc::sphere_uv(orc::V3<float>((float)p[3 * k], (float)p[3 * k + 1], (float)p[3 * k + 2]), *map, u, v);
+            uv[2 * k] = u; uv[2 * k + 1] = v;
+        }
+    }
+    return CRT_OK;
+}
+
+/* tex_value of texture `tex` of a scene at (u, v, p): q rows of 5 doubles, rgb rows of 3. */
+int oracle_tex_value(const crt_scene_desc* d, int32_t tex, uint64_t n, const double* q, double* rgb) {
+    orc::Scene<double> s;
+    int rc = orc::build_scene(s, d, false, 1);
+    if (rc) return rc;
+    if (tex < 0 || tex >= (int32_t)s.texs.size()) return CRT_ERR_INVALID_ARGUMENT;
+    for (uint64_t k = 0; k < n; ++k) {
+        const double* a = q + 5 * k;
+        orc::V3<double> c = orc::tex_value(s, tex, a[0], a[1], orc::V3<double>(a[2], a[3], a[4]));
+        rgb[3 * k] = c.x; rgb[3 * k + 1] = c.y; rgb[3 * k + 2] = c.z;
+    }
     return CRT_OK;
 }
 

@@ -157,3 +157,84 @@ def ref_bvh_info(scene):
     f(C.cast(C.pointer(desc), C.c_void_p), C.byref(nodes), C.byref(depth))
     del keep
     return nodes.value, depth.value
+
+
+def camera_basis(scene):
+    """setup_camera in FP64: dict of origin, llc, horizontal, vertical, u, v, w, lens_radius."""
+    desc, keep = scene.to_desc()
+    out = np.zeros(22)
+    f = lib().oracle_camera_basis
+    f.argtypes = [C.c_void_p, C.c_void_p]
+    f(C.cast(C.pointer(desc), C.c_void_p), _fp(out))
+    del keep
+    names = ["origin", "llc", "horizontal", "vertical", "u", "v", "w"]
+    d = {k: out[3 * i:3 * i + 3].copy() for i, k in enumerate(names)}
+    d["lens_radius"] = float(out[21])
+    return d
+
+
+def camera_rays(scene, uv, seed=1):
+    """camera_ray at image-plane points (u, v) in FP64: (origins, times, directions)."""
+    desc, keep = scene.to_desc()
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((uv.shape[0], 8))
+    f = lib().oracle_camera_rays
+    f.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]
+    f(C.cast(C.pointer(desc), C.c_void_p), uv.shape[0], _fp(uv), seed, _fp(out))
+    del keep
+    return out[:, 0:3], out[:, 3], out[:, 4:7]
+
+
+def metal(dirs, normals, fuzz, seed=1):
+    d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    n = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(d)
+    f = lib().oracle_metal
+    f.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_double, C.c_uint64, C.c_void_p]
+    f(d.shape[0], _fp(d), _fp(n), fuzz, seed, _fp(out))
+    return out
+
+
+def dielectric(dirs, normals, front, ior, draws):
+    """Returns (directions, reflected flags, draw-consumed flags)."""
+    d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    n = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+    fr = np.ascontiguousarray(np.broadcast_to(np.asarray(front, dtype=np.int32), (d.shape[0],)))
+    dr = np.ascontiguousarray(np.broadcast_to(np.asarray(draws, dtype=np.float64), (d.shape[0],)))
+    out = np.zeros_like(d)
+    refl = np.zeros(d.shape[0], np.int32)
+    used = np.zeros(d.shape[0], np.int32)
+    f = lib().oracle_dielectric
+    f.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_void_p]
+    f(d.shape[0], _fp(d), _fp(n), _fp(fr), ior, _fp(dr), _fp(out), _fp(refl), _fp(used))
+    return out, refl.astype(bool), used.astype(bool)
+
+
+def sphere_uv(points, mapping=(1, 1, 0, 0, 1, 1), precision=64):
+    """get_sphere_uv; mapping = (scale_u, scale_v, offset_u, offset_v, repeat_u, repeat_v)."""
+    from cortexraytracercuda_amd import _abi as A
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    m = A.crt_uvmap(*[float(x) for x in mapping])
+    out = np.zeros((p.shape[0], 2))
+    f = lib().oracle_sphere_uv
+    f.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    f(p.shape[0], _fp(p), C.cast(C.pointer(m), C.c_void_p), precision, _fp(out))
+    return out
+
+
+def tex_value(scene, tex, u, v, points):
+    """tex_value of texture index `tex` at texture coordinates (u, v) and hit points."""
+    desc, keep = scene.to_desc()
+    u = np.asarray(u, dtype=np.float64).reshape(-1)
+    v = np.asarray(v, dtype=np.float64).reshape(-1)
+    p = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(np.column_stack([u, v, p]))
+    out = np.zeros((q.shape[0], 3))
+    f = lib().oracle_tex_value
+    f.argtypes = [C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]
+    rc = f(C.cast(C.pointer(desc), C.c_void_p), tex, q.shape[0], _fp(q), _fp(out))
+    del keep
+    if rc:
+        raise RuntimeError("oracle_tex_value failed: %d" % rc)
+    return out

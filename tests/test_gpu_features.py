@@ -30,11 +30,12 @@ def _first_hit_check(r, s, W, H, accel):
 
 @pytest.mark.parametrize("width", [2, 4])
 def test_bvh_widths_first_hit(width):
+    """Both tree widths give the oracle's closest hits, ties included (lower global id wins)."""
     s = scenes.large_scene(levels=5, n_spheres=300)
     with Renderer(bvh_width=width) as r:
         agree, bad, rel = _first_hit_check(r, s, 240, 135, accel=0)
     print("width %d: agreement %.6f non-tie mismatches %.2e max rel t %.2e" % (width, agree, bad, rel))
-    assert bad <= 1e-5 and rel <= 1e-5
+    assert agree >= 0.99999 and bad == 0 and rel <= 1e-5
 
 
 @pytest.mark.parametrize("width", [2, 4])
@@ -54,8 +55,10 @@ def test_instanced_first_hit_and_texture_render():
     s.set_camera((0.0, 3.0, 40.0), (0.0, 0.0, 0.0), vfov=50.0, aspect_ratio=1.0, focus_dist=10.0)
     with Renderer() as r:
         agree, bad, rel = _first_hit_check(r, s, 160, 160, accel=0)
+        # instanced triangles carry their mesh-local id, so a tie between two instances of one mesh
+        # is not decided by id: such rays may differ, at equal distance
         print("instanced: agreement %.6f non-tie mismatches %.2e max rel t %.2e" % (agree, bad, rel))
-        assert bad <= 1e-4 and rel <= 1e-5
+        assert agree >= 0.9999 and bad <= 1e-4 and rel <= 1e-5
         sid = r.upload(s)
         _, acc, st = r.render(sid, 48, 48, 4, 6, seed=5)
     oacc, _ = O.render(s, 48, 48, 4, 6, seed=5, rng="pcg", precision=64, threads=NT)

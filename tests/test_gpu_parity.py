@@ -15,22 +15,32 @@ def _tie_ok(gpu_t, ora_t):
     return np.abs(gpu_t - ora_t) <= 1e-5 * np.maximum(np.abs(ora_t), 1e-30)
 
 
+def _first_hit_parity(prim, t, oprim, ot, label, min_agree=0.99999):
+    """Primitive ids must agree on at least min_agree of the rays: both sides apply the same tie rule
+    (hits within a relative 2^-20 in t tie, the lower global id wins), so rays through shared edges
+    report the same triangle. Any remaining mismatch must still be a tie in distance (same t within
+    1e-5), and hit distances agree within 1e-5 relative."""
+    prim = prim.reshape(-1)
+    t = t.reshape(-1).astype(np.float64)
+    agree = prim == oprim
+    print("%s first-hit agreement %.7f (%d mismatches of %d rays)" % (label, agree.mean(), (~agree).sum(), agree.size))
+    assert agree.mean() >= min_agree, (prim[~agree][:10], oprim[~agree][:10], t[~agree][:10], ot[~agree][:10])
+    assert np.all(_tie_ok(t[~agree], ot[~agree])), (prim[~agree][:10], oprim[~agree][:10])
+    hit = agree & (oprim >= 0)
+    rel = np.abs(t[hit] - ot[hit]) / ot[hit]
+    print("%s max rel t error %.3g" % (label, rel.max()))
+    assert rel.max() <= 1e-5
+
+
 def test_box_first_hit(renderer, box):
+    """256 x 256 pixel-centre rays of the box scene. The diagonal pixel rays pass exactly through
+    the edge shared by the two triangles of each wall: the tie rule decides them."""
     W = H = 256
     sid = renderer.upload(box)
     prim, t = renderer.trace_primary(sid, W, H)
     rays = O.primary_rays(box, W, H)
     oprim, _, ot, _ = O.trace(box, rays, camera=True, precision=64, accel=1)
-    prim = prim.reshape(-1)
-    t = t.reshape(-1).astype(np.float64)
-    agree = prim == oprim
-    print("box first-hit agreement %.7f (%d mismatches)" % (agree.mean(), (~agree).sum()))
-    # mismatches only at ties (shared edges): same distance
-    assert np.all(_tie_ok(t[~agree], ot[~agree])), (prim[~agree][:10], oprim[~agree][:10])
-    hit = agree & (oprim >= 0)
-    rel = np.abs(t[hit] - ot[hit]) / ot[hit]
-    print("max rel t error %.3g" % rel.max())
-    assert rel.max() <= 1e-5
+    _first_hit_parity(prim, t, oprim, ot, "box")
     renderer.release(sid)
 
 
@@ -96,7 +106,9 @@ def test_box_pcg_vs_reference_render_statistically(renderer, box):
 
 
 def test_large_first_hit(renderer):
-    s = scenes.large_scene(levels=6, n_spheres=500)
+    """The large scene at its full size (1,310,720 displaced icosphere triangles + 500 spheres),
+    pixel-centre rays at 320 x 180."""
+    s = scenes.large_scene(levels=8, n_spheres=500)
     W, H = 320, 180
     sid = renderer.upload(s)
     info = renderer.info(sid)
@@ -104,16 +116,7 @@ def test_large_first_hit(renderer):
     prim, t = renderer.trace_primary(sid, W, H)
     rays = O.primary_rays(s, W, H)
     oprim, _, ot, _ = O.trace(s, rays, camera=True, precision=64, accel=0)
-    prim = prim.reshape(-1)
-    t = t.reshape(-1).astype(np.float64)
-    agree = prim == oprim
-    print("large first-hit agreement %.7f (%d mismatches)" % (agree.mean(), (~agree).sum()))
-    bad = ~agree & ~_tie_ok(t, ot)
-    assert bad.sum() == 0 or bad.mean() <= 1e-5, (prim[bad][:10], oprim[bad][:10], t[bad][:10], ot[bad][:10])
-    hit = agree & (oprim >= 0)
-    rel = np.abs(t[hit] - ot[hit]) / ot[hit]
-    print("max rel t error %.3g" % rel.max())
-    assert rel.max() <= 1e-5
+    _first_hit_parity(prim, t, oprim, ot, "large")
     renderer.release(sid)
 
 
@@ -157,13 +160,8 @@ def test_edge_scenes_first_hit_and_widths(name):
         with Renderer(bvh_width=width) as r:
             sid = r.upload(s)
             prim, t = r.trace_primary(sid, W, H)
-            prim = prim.reshape(-1)
-            t = t.reshape(-1).astype(np.float64)
-            agree = prim == oprim
             assert (oprim >= 0).any() and (oprim < 0).any()
-            assert np.all(_tie_ok(t[~agree], ot[~agree])), (width, prim[~agree][:10], oprim[~agree][:10])
-            hit = agree & (oprim >= 0)
-            assert (np.abs(t[hit] - ot[hit]) / ot[hit]).max() <= 1e-5
+            _first_hit_parity(prim, t, oprim, ot, "%s width %d" % (name, width))
             _, acc, _ = r.render(sid, W, H, 4, 8, seed=3)
             accs.append(acc)
     # the four-wide and binary trees trace the same closest hits: identical images. Without lights a

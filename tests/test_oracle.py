@@ -235,3 +235,214 @@ def test_oracles_agree_statistically(box):
     m64 = a64[..., :3].mean() / 16
     m32 = a32[..., :3].mean() / 16
     assert abs(m64 - m32) < 0.03 * m64
+
+
+# ------------------------------------------------------------------ scatter directions
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def test_metal_fuzz_zero_is_exact_mirror():
+    # metal.cuh:57-65 with fuzz 0: glm::reflect of the unit direction, r = d - 2 (d.n) n, so the
+    # angle of reflection equals the angle of incidence and the tangential part is kept
+    rng = np.random.default_rng(1)
+    d = rng.normal(size=(256, 3)) * rng.uniform(0.1, 50.0, (256, 1))   # unnormalised, as in the reference
+    n = _unit(rng.normal(size=(256, 3)))
+    out = O.metal(d, n, 0.0)
+    ud = _unit(d)
+    mirror = ud - 2.0 * np.sum(ud * n, axis=1, keepdims=True) * n
+    assert np.allclose(out, mirror, rtol=0, atol=1e-14)
+    assert np.allclose(np.sum(out * n, axis=1), -np.sum(ud * n, axis=1), atol=1e-14)
+    assert np.allclose(np.linalg.norm(out, axis=1), 1.0, atol=1e-14)
+
+
+def test_metal_fuzz_ball_around_mirror():
+    # fuzz > 0: mirror + fuzz * (point of the unit ball): every direction lies within fuzz of the
+    # mirror, the offsets fill the ball (not all equal, mean near zero, some near the surface)
+    n = np.tile([0.0, 1.0, 0.0], (4000, 1))
+    d = np.tile([1.0, -1.0, 0.0], (4000, 1))
+    fuzz = 0.3
+    out = O.metal(d, n, fuzz, seed=7)
+    mirror = _unit([1.0, 1.0, 0.0])
+    off = out - mirror
+    r = np.linalg.norm(off, axis=1)
+    assert r.max() < fuzz and r.max() > 0.95 * fuzz
+    assert np.all(np.abs(off.mean(axis=0)) < 0.02)
+    # uniform in the ball: the fraction inside half the radius is 1/8
+    assert abs((r < fuzz / 2).mean() - 0.125) < 0.03
+    assert len(np.unique(np.round(out, 12), axis=0)) == len(out)
+
+
+def test_dielectric_schlick_normal_incidence():
+    # normal incidence: cos = 1, Schlick reflectance = r0 = ((1 - eta) / (1 + eta))^2 = 0.04 for
+    # eta = 1/1.5 (entering) and for eta = 1.5 (leaving); the draw decides at exactly 0.04
+    n = np.array([[0.0, 0.0, 1.0]])
+    for front, d in ((1, [0.0, 0.0, -2.0]), (0, [0.0, 0.0, -2.0])):
+        nn = n if front else -n * -1   # the normal faces the incoming ray in both cases
+        out_r, refl_r, used_r = O.dielectric([d], nn, front, 1.5, 0.0399)
+        out_t, refl_t, used_t = O.dielectric([d], nn, front, 1.5, 0.0401)
+        assert refl_r[0] and not refl_t[0] and used_r[0] and used_t[0]
+        assert np.allclose(out_r[0], [0.0, 0.0, 1.0], atol=1e-12)    # straight back
+        assert np.allclose(out_t[0], [0.0, 0.0, -1.0], atol=1e-12)   # straight through
+
+
+def test_dielectric_total_internal_reflection_threshold():
+    # leaving glass (front = 0, ratio = 1.5): total internal reflection exactly when
+    # 1.5 * sin(theta) > 1; the draw is then not consumed (|| short-circuits, dielectric.cuh:43)
+    n = np.array([0.0, 0.0, 1.0])
+    crit = math.asin(1.0 / 1.5)
+    for theta, tir in ((crit + 1e-6, True), (crit - 1e-6, False), (0.2, False), (1.2, True)):
+        d = np.array([math.sin(theta), 0.0, -math.cos(theta)])
+        out, refl, used = O.dielectric([d], [n], 0, 1.5, 2.0)   # a draw above 1: Schlick never reflects
+        assert refl[0] == tir and used[0] == (not tir), theta
+        if tir:
+            assert np.allclose(out[0], [d[0], 0.0, -d[2]], atol=1e-12)
+
+
+def test_dielectric_refraction_obeys_snell():
+    rng = np.random.default_rng(3)
+    n = np.array([0.0, 0.0, 1.0])
+    for front, ratio in ((1, 1.0 / 1.5), (0, 1.5)):
+        for _ in range(200):
+            theta = rng.uniform(0.0, math.asin(min(1.0, 1.0 / ratio)) * 0.999)
+            phi = rng.uniform(0, 2 * math.pi)
+            d = np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), -math.cos(theta)])
+            out, refl, _ = O.dielectric([d * 3.0], [n], front, 1.5, 2.0)
+            t = out[0]
+            assert not refl[0]
+            assert abs(np.linalg.norm(t) - 1.0) < 1e-12
+            sin_t = math.hypot(t[0], t[1])
+            assert abs(sin_t - ratio * math.sin(theta)) < 1e-12          # Snell
+            assert t[2] < 0                                              # keeps going through
+            assert abs(t[0] * d[1] - t[1] * d[0]) < 1e-12                # plane of incidence
+
+
+# ------------------------------------------------------------------ camera
+def test_camera_basis_closed_form():
+    # perspective_camera.cuh:41-104: w = unit(from - at), u = unit(vup x w), v = w x u;
+    # half height = tan(vfov / 2), half width = aspect * half height; the image plane sits at
+    # focus_dist along -w and spans 2 * half_width * focus by 2 * half_height * focus
+    s = Scene().set_camera((1.0, 2.0, 5.0), (0.5, -0.3, -1.0), vup=(0.1, 1.0, 0.0), vfov=47.0, aspect_ratio=16 / 9,
+                           aperture=0.4, focus_dist=6.5)
+    b = O.camera_basis(s)
+    fr, at = np.array([1.0, 2.0, 5.0]), np.array([0.5, -0.3, -1.0])
+    w = _unit(fr - at)
+    hh = math.tan(math.radians(47.0) / 2)
+    hw = 16 / 9 * hh
+    assert np.allclose(b["w"], w, atol=1e-7)
+    assert abs(np.dot(b["u"], w)) < 1e-7 and abs(np.dot(b["v"], w)) < 1e-7 and abs(np.dot(b["u"], b["v"])) < 1e-7
+    assert np.dot(b["v"], [0.1, 1.0, 0.0]) > 0 and np.dot(np.cross(b["u"], b["v"]), w) > 0   # right-handed, up is up
+    assert abs(np.linalg.norm(b["horizontal"]) - 2 * hw * 6.5) < 1e-5
+    assert abs(np.linalg.norm(b["vertical"]) - 2 * hh * 6.5) < 1e-5
+    centre = b["llc"] + 0.5 * b["horizontal"] + 0.5 * b["vertical"]
+    # the image centre is focus_dist straight ahead: towards lookat
+    assert np.allclose(centre - b["origin"], -6.5 * w, atol=1e-5)
+    assert abs(b["lens_radius"] - 0.2) < 1e-7
+
+
+def test_camera_rays_pinhole_centre_and_corners():
+    s = Scene().set_camera((0.0, 0.0, 0.0), (0.0, 0.0, -4.0), vfov=90.0, aspect_ratio=2.0, focus_dist=3.0)
+    o, tm, d = O.camera_rays(s, [[0.5, 0.5], [0.0, 0.0], [1.0, 1.0], [1.0, 0.0]])
+    assert np.allclose(o, 0.0)                            # aperture 0: every ray leaves the eye
+    assert np.allclose(d[0], [0.0, 0.0, -3.0], atol=1e-12)  # centre: length focus_dist, towards lookat
+    # vfov 90: half height = focus * tan(45) = 3, half width = aspect * 3 = 6
+    assert np.allclose(d[1], [-6.0, -3.0, -3.0], atol=1e-12)
+    assert np.allclose(d[2], [6.0, 3.0, -3.0], atol=1e-12)
+    assert np.allclose(d[3], [6.0, -3.0, -3.0], atol=1e-12)
+    assert np.all((tm >= 0.0) & (tm < 1.0))
+
+
+def test_camera_thin_lens_converges_on_focus_plane():
+    # with an aperture the origin moves inside the lens disk (in the u, v plane, radius aperture / 2)
+    # and the direction is changed so that origin + direction is still the focus-plane point
+    s = Scene().set_camera((1.0, 1.0, 1.0), (0.0, 0.0, 0.0), vfov=40.0, aspect_ratio=1.5, aperture=0.8, focus_dist=2.0)
+    b = O.camera_basis(s)
+    uv = np.random.default_rng(2).uniform(0, 1, (500, 2))
+    o, _, d = O.camera_rays(s, uv, seed=11)
+    target = b["llc"] + uv[:, :1] * b["horizontal"] + uv[:, 1:] * b["vertical"]
+    assert np.allclose(o + d, target, atol=1e-12)
+    off = o - b["origin"]
+    assert np.all(np.linalg.norm(off, axis=1) < 0.4) and np.linalg.norm(off, axis=1).max() > 0.3
+    assert np.allclose(off @ b["w"], 0.0, atol=1e-12)
+
+
+# ------------------------------------------------------------------ textures
+def test_image_texel_index_on_lattice():
+    # image_texture.cuh:54-71 + bitmap_image::pixel_data: u, v clamped to [0, 1], v flipped,
+    # i = int(u W), j = int((1 - v) H), clamped to the last texel; row 0 is the top row
+    W, H = 5, 7
+    img = np.zeros((H, W, 3), np.uint8)
+    for j in range(H):
+        for i in range(W):
+            img[j, i] = (i * 40, j * 30, 7)
+    s = Scene()
+    t = s.image_texture("img", img)
+    us, vs, exp = [], [], []
+    for j in range(H):
+        for i in range(W):
+            us.append((i + 0.5) / W); vs.append(1.0 - (j + 0.5) / H); exp.append((i, j))
+    for u, v, e in ((1.0, 0.5, (W - 1, 3)), (0.0, 1.0, (0, 0)), (0.5, 0.0, (2, H - 1)), (-0.3, 1.7, (0, 0)),
+                    (1.4, -2.0, (W - 1, H - 1)), (0.2, 1.0 - 3.0 / 7.0, (1, 3))):
+        us.append(u); vs.append(v); exp.append(e)
+    rgb = O.tex_value(s, t, us, vs, np.zeros((len(us), 3)))
+    got = np.rint(rgb * 255.0).astype(int)
+    for g, (i, j) in zip(got, exp):
+        assert tuple(g) == (i * 40, j * 30, 7), (g, i, j)
+
+
+def test_checker_parity_on_lattice():
+    # checker_texture.cuh:33-42: even when floor(x/s) + floor(y/s) + floor(z/s) is even
+    s = Scene()
+    t = s.checker_texture("c", 0.25, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    cells = np.array([[a, b, c] for a in range(-3, 3) for b in range(-2, 2) for c in range(-2, 3)])
+    pts = (cells + 0.5) * 0.25
+    rgb = O.tex_value(s, t, np.zeros(len(pts)), np.zeros(len(pts)), pts)
+    even = (cells.sum(axis=1) % 2) == 0
+    assert np.array_equal(rgb[:, 0] == 1.0, even) and np.array_equal(rgb[:, 2] == 1.0, ~even)
+
+
+# ------------------------------------------------------------------ sphere uv
+@pytest.mark.parametrize("prec", [32, 64])
+def test_sphere_uv_axis_points(prec):
+    # uvmapping.cuh:206-210 lists six axis points; five hold as listed. The code then applies
+    # fmod(t * repeat_v, 1), so the north pole <0 1 0> gives v = 0 (t = 1 wraps), not the listed
+    # 1.00: the oracle follows the code.
+    pts = [[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]]
+    want = [[0.5, 0.5], [0.0, 0.5], [0.5, 0.0], [0.5, 0.0], [0.25, 0.5], [0.75, 0.5]]
+    uv = O.sphere_uv(pts, precision=prec)
+    assert np.allclose(uv, want, atol=1e-6), uv
+    # mapping: u = scale_u * fmod(s * repeat_u, 1) + offset_u
+    uv2 = O.sphere_uv([[0, 0, 1]], mapping=(2.0, 3.0, 0.1, 0.2, 3.0, 1.0), precision=prec)
+    assert np.allclose(uv2, [[2.0 * 0.75 + 0.1, 3.0 * 0.5 + 0.2]], atol=1e-6)
+
+
+# ------------------------------------------------------------------ instances
+def test_instanced_triangle_equals_world_placed_triangle():
+    # an instance places the mesh with its 3x4 object-to-world matrix (row-major [R | t]); the
+    # same triangles placed in world space by that matrix give the same hits in FP64 (the world
+    # copy's vertices are stored as float32, hence the 1e-6 distance tolerance)
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-1, 1, (30, 3))
+    idx = rng.integers(0, 30, (40, 3))
+    idx = idx[(idx[:, 0] != idx[:, 1]) & (idx[:, 1] != idx[:, 2]) & (idx[:, 0] != idx[:, 2])]
+    a = math.radians(33.0)
+    R = np.array([[math.cos(a), 0.0, math.sin(a)], [0.0, 1.0, 0.0], [-math.sin(a), 0.0, math.cos(a)]]) * 1.7
+    T = np.array([0.3, -0.2, 0.5])
+    xf = np.hstack([R, T[:, None]])
+    si = Scene().set_camera((0.0, 0.0, 6.0), (0.0, 0.0, 0.0), vfov=40.0)
+    m = si.lambertian("m", (0.5, 0.5, 0.5))
+    mi = si.mesh(pos, idx, m)
+    si.instance(mi, xf)
+    sw = Scene().set_camera((0.0, 0.0, 6.0), (0.0, 0.0, 0.0), vfov=40.0)
+    m2 = sw.lambertian("m", (0.5, 0.5, 0.5))
+    world = pos.astype(np.float32).astype(np.float64) @ xf.astype(np.float32).astype(np.float64)[:, :3].T + \
+        xf.astype(np.float32).astype(np.float64)[:, 3]
+    sw.mesh(world, idx, m2)
+    rays = O.primary_rays(sw, 96, 96)
+    pi, ii, ti, _ = O.trace(si, rays, camera=True, precision=64, accel=1)
+    pw, iw, tw, _ = O.trace(sw, rays, camera=True, precision=64, accel=1)
+    assert (pw >= 0).mean() > 0.1
+    assert np.array_equal(pi, pw)
+    assert np.all(ii[pi >= 0] == 0) and np.all(iw == -1)
+    assert np.allclose(ti, tw, rtol=1e-6, atol=0)
