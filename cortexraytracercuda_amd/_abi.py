@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcortex_rt.so")
 
-CRT_ABI_VERSION = 1
+CRT_ABI_VERSION = 2
 
 CRT_OK = 0
 CRT_ERR_NO_DEVICE = 1
@@ -27,6 +27,9 @@ CRT_MAT_LAMBERTIAN = 1
 CRT_MAT_METAL = 2
 CRT_MAT_DIELECTRIC = 3
 CRT_MAT_DIFFUSE_LIGHT = 8
+CRT_MAT_SPOT_LIGHT = 9
+CRT_LIGHT_OMNI = 0
+CRT_LIGHT_SPOT = 1
 CRT_MATF_ONE_SIDED = 1
 
 CRT_TEX_SOLID = 0
@@ -40,6 +43,16 @@ CRT_RF_TIME_STAGES = 1
 CRT_RF_MEGAKERNEL = 2
 CRT_RF_IMAGE_ONLY = 4
 
+CRT_BVH_HOST = 1
+CRT_BVH_DEVICE = 2
+
+CRT_XF_TRANSLATE = 1
+CRT_XF_ROTATE = 2
+CRT_XF_SCALE = 4
+
+CRT_TRACE_CAMERA = 1
+CRT_TRACE_STAGE = 2
+
 CRT_EXTEND_DEFAULT = 0
 CRT_EXTEND_GRID_STRIDE = 1
 CRT_EXTEND_PERSISTENT = 2
@@ -50,7 +63,8 @@ f3 = C.c_float * 3
 
 class crt_camera(C.Structure):
     _fields_ = [("lookfrom", f3), ("lookat", f3), ("vup", f3), ("vfov_deg", C.c_float), ("aspect_ratio", C.c_float),
-                ("aperture", C.c_float), ("focus_dist", C.c_float), ("time0", C.c_float), ("time1", C.c_float)]
+                ("aperture", C.c_float), ("focus_dist", C.c_float), ("time0", C.c_float), ("time1", C.c_float),
+                ("orthographic", C.c_uint32), ("ortho_height", C.c_float)]
 
 
 class crt_material(C.Structure):
@@ -91,6 +105,15 @@ class crt_instance(C.Structure):
     _fields_ = [("mesh", C.c_uint32), ("object_to_world", C.c_float * 12), ("material_override", C.c_int32)]
 
 
+class crt_sphere_light(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("center", f3), ("radius", C.c_float), ("rgb", f3), ("intensity", C.c_float),
+                ("invisible", C.c_uint32), ("direction", f3), ("cutoff_deg", C.c_float), ("falloff", C.c_float)]
+
+
+class crt_transform(C.Structure):
+    _fields_ = [("has", C.c_uint32), ("translate", f3), ("rotate_deg", f3), ("scale", f3)]
+
+
 class crt_scene_desc(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("camera", crt_camera), ("background", f3),
                 ("materials", C.POINTER(crt_material)), ("num_materials", C.c_uint32),
@@ -99,7 +122,8 @@ class crt_scene_desc(C.Structure):
                 ("spheres", C.POINTER(crt_sphere)), ("num_spheres", C.c_uint32),
                 ("quads", C.POINTER(crt_quad)), ("num_quads", C.c_uint32),
                 ("meshes", C.POINTER(crt_mesh)), ("num_meshes", C.c_uint32),
-                ("instances", C.POINTER(crt_instance)), ("num_instances", C.c_uint32)]
+                ("instances", C.POINTER(crt_instance)), ("num_instances", C.c_uint32),
+                ("sphere_lights", C.POINTER(crt_sphere_light)), ("num_sphere_lights", C.c_uint32)]
 
 
 class crt_render_desc(C.Structure):
@@ -124,14 +148,14 @@ class crt_config(C.Structure):
                 ("bvh_width", C.c_uint32), ("extend_kernel", C.c_uint32), ("refill_min", C.c_uint32),
                 ("pass_paths", C.c_uint32), ("grid_per_cu", C.c_uint32), ("shade_grid_per_cu", C.c_uint32),
                 ("sub_passes", C.c_uint32), ("graphs", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("ray_sort", C.c_uint32)]
+                ("ray_sort", C.c_uint32), ("bvh_builder", C.c_uint32)]
 
 
 class crt_scene_info(C.Structure):
     _fields_ = [("num_nodes", C.c_uint64), ("num_leaves", C.c_uint64), ("num_prims", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("max_depth", C.c_uint32), ("num_instances", C.c_uint32),
                 ("bvh_width", C.c_uint32), ("node_bytes", C.c_uint32), ("leaf_size", C.c_uint32),
-                ("reserved", C.c_uint32), ("build_ms", C.c_double), ("upload_ms", C.c_double), ("sah_cost", C.c_double)]
+                ("builder", C.c_uint32), ("build_ms", C.c_double), ("upload_ms", C.c_double), ("sah_cost", C.c_double)]
 
 
 class crt_batch_item(C.Structure):
@@ -146,7 +170,7 @@ class crt_request_timing(C.Structure):
 class crt_work_counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("rays", "retraces", "light_pdf_tests", "node_visits", "leaf_visits", "tri_tests",
                                           "sphere_tests", "quad_tests", "inst_visits", "extend_launches", "passes",
-                                          "iterations")]
+                                          "iterations", "exact_queries")]
 
 
 class crt_memory_info(C.Structure):
@@ -173,14 +197,22 @@ class crt_tune_result(C.Structure):
 STRUCTS_IN_ABI_ORDER = [crt_camera, crt_material, crt_texture, crt_uvmap, crt_sphere, crt_quad, crt_area_light,
                         crt_mesh, crt_instance, crt_scene_desc, crt_render_desc, crt_image, crt_stats, crt_config,
                         crt_scene_info, crt_batch_item, crt_request_timing, crt_work_counters,
-                        crt_memory_info, crt_obj_mesh, crt_scenefile_info, crt_tune_result]
+                        crt_memory_info, crt_obj_mesh, crt_scenefile_info, crt_tune_result, crt_sphere_light,
+                        crt_transform]
 
 # Every symbol declared in include/cortex_rt.h.
 EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "crt_scene_upload", "crt_scene_release",
-            "crt_scene_get_info", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
+            "crt_scene_get_info", "crt_scene_verify", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
             "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
             "crt_memory_get_info", "crt_obj_load", "crt_obj_free", "crt_scenefile_load",
-            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes", "crt_autotune", "crt_debug_fail_alloc"]
+            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes", "crt_autotune", "crt_debug_fail_alloc",
+            "crt_builder_create", "crt_builder_destroy", "crt_builder_camera",
+            "crt_builder_background", "crt_builder_solid_texture", "crt_builder_checker_texture",
+            "crt_builder_image_texture", "crt_builder_image_texture_pixels", "crt_builder_lambertian", "crt_builder_metal",
+            "crt_builder_dielectric", "crt_builder_diffuse_light", "crt_builder_has_material",
+            "crt_builder_directional_light", "crt_builder_omni_light", "crt_builder_spot_light", "crt_builder_sphere",
+            "crt_builder_quad", "crt_builder_box", "crt_builder_obj_mesh", "crt_builder_note_unsupported",
+            "crt_builder_desc"]
 
 _P = C.POINTER
 _SIGS = {
@@ -191,6 +223,7 @@ _SIGS = {
     "crt_scene_upload": ([C.c_void_p, _P(crt_scene_desc), _P(C.c_uint32)], C.c_int),
     "crt_scene_release": ([C.c_void_p, C.c_uint32], C.c_int),
     "crt_scene_get_info": ([C.c_void_p, C.c_uint32, _P(crt_scene_info)], C.c_int),
+    "crt_scene_verify": ([C.c_void_p, C.c_uint32], C.c_int),
     "crt_scene_build_check": ([_P(crt_scene_desc), C.c_uint32, C.c_uint32, _P(crt_scene_info)], C.c_int),
     "crt_render": ([C.c_void_p, C.c_uint32, _P(crt_render_desc), _P(crt_image), _P(crt_stats)], C.c_int),
     "crt_trace_rays": ([C.c_void_p, C.c_uint32, _P(C.c_float), C.c_uint64, C.c_int32, _P(C.c_int32), _P(C.c_int32),
@@ -209,6 +242,35 @@ _SIGS = {
     "crt_scenefile_load": ([C.c_char_p, _P(C.c_void_p)], C.c_int),
     "crt_scenefile_desc": ([C.c_void_p, _P(crt_scene_desc), _P(crt_scenefile_info)], C.c_int),
     "crt_scenefile_close": ([C.c_void_p], C.c_int),
+    "crt_builder_create": ([_P(C.c_void_p)], C.c_int),
+    "crt_builder_destroy": ([C.c_void_p], C.c_int),
+    "crt_builder_camera": ([C.c_void_p, _P(crt_camera)], C.c_int),
+    "crt_builder_background": ([C.c_void_p, _P(C.c_float)], C.c_int),
+    "crt_builder_solid_texture": ([C.c_void_p, C.c_char_p, _P(C.c_float)], C.c_int),
+    "crt_builder_checker_texture": ([C.c_void_p, C.c_char_p, C.c_float, C.c_char_p, C.c_char_p, _P(C.c_float),
+                                     _P(C.c_float)], C.c_int),
+    "crt_builder_image_texture": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
+    "crt_builder_image_texture_pixels": ([C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, _P(C.c_uint8)],
+                                         C.c_int),
+    "crt_builder_lambertian": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_char_p], C.c_int),
+    "crt_builder_metal": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_float, C.c_char_p], C.c_int),
+    "crt_builder_dielectric": ([C.c_void_p, C.c_char_p, C.c_float], C.c_int),
+    "crt_builder_diffuse_light": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_float, C.c_int32], C.c_int),
+    "crt_builder_has_material": ([C.c_void_p, C.c_char_p], C.c_int),
+    "crt_builder_directional_light": ([C.c_void_p, C.c_char_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float,
+                                       _P(C.c_float), C.c_int32], C.c_int),
+    "crt_builder_omni_light": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_float, C.c_float, _P(C.c_float), C.c_int32],
+                               C.c_int),
+    "crt_builder_spot_light": ([C.c_void_p, C.c_char_p, _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, C.c_float,
+                                C.c_float, _P(C.c_float), C.c_int32], C.c_int),
+    "crt_builder_sphere": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_float, C.c_char_p, _P(crt_uvmap),
+                            _P(crt_transform)], C.c_int),
+    "crt_builder_quad": ([C.c_void_p, C.c_char_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_char_p,
+                          _P(crt_transform)], C.c_int),
+    "crt_builder_box": ([C.c_void_p, C.c_char_p, _P(C.c_float), _P(C.c_float), C.c_char_p, _P(crt_transform)], C.c_int),
+    "crt_builder_obj_mesh": ([C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32], C.c_int),
+    "crt_builder_note_unsupported": ([C.c_void_p, C.c_uint32], C.c_int),
+    "crt_builder_desc": ([C.c_void_p, _P(crt_scene_desc), _P(C.c_uint32), _P(C.c_uint32)], C.c_int),
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
     "crt_autotune": ([C.c_void_p, C.c_uint32, _P(crt_scene_desc), _P(crt_render_desc), _P(crt_tune_result)], C.c_int),
     "crt_debug_fail_alloc": ([C.c_void_p, C.c_int32], C.c_int),

@@ -110,8 +110,11 @@ template <class T> static int upload_vec(DevBuf& b, const std::vector<T>& v, Con
 
 // ------------------------------------------------------------------ camera (host, FP32)
 // perspective_camera::initialize (core/src/cameras/perspective_camera.cuh:41-104) with the same
-// float expression order; degrees_to_radians = deg * M_PI / 180 (math_utils.cuh:55-58).
-void camera_setup(const crt_camera& c, crt_dev_request& r) {
+// float expression order; degrees_to_radians = deg * M_PI / 180 (math_utils.cuh:55-58). The
+// orthographic camera: orthographic_camera::initialize (core/src/cameras/orthographic_camera.cuh:
+// 57-118), whose viewport width follows the image's: width = render width, height =
+// max(1, int(width / aspect)).
+void camera_setup(const crt_camera& c, uint32_t width, crt_dev_request& r) {
     struct V { float x, y, z; };
     auto sub = [](V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; };
     auto add = [](V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; };
@@ -133,6 +136,17 @@ void camera_setup(const crt_camera& c, crt_dev_request& r) {
     put(r.origin, from); put(r.llc, llc); put(r.horizontal, hor); put(r.vertical, ver); put(r.cu, u); put(r.cv, v);
     r.lens_radius = c.aperture / 2.0f;
     r.time0 = c.time0; r.time1 = c.time1;
+    r.ortho = 0u;
+    if (c.orthographic) {
+        int ih = (int)((float)width / ratio);
+        ih = ih < 1 ? 1 : ih;
+        const float vh = 2.0f * c.ortho_height;
+        const float vw = vh * ((float)width / (float)ih);
+        V ollc = sub(sub(from, mul(0.5f * vw, u)), mul(0.5f * vh, v));
+        put(r.llc, ollc); put(r.horizontal, mul(vw, u)); put(r.vertical, mul(vh, v));
+        put(r.ortho_dir, V{-w.x, -w.y, -w.z});
+        r.ortho = 1u;
+    }
     (void)add;
 }
 
@@ -161,6 +175,8 @@ int Context::init(const crt_config* cfg) {
     stack_lds = config.stack_lds ? config.stack_lds : 16u;
     if (config.ray_sort > 2) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown ray_sort");
     ray_sort = config.ray_sort == 1u ? 1u : 0u;
+    if (config.bvh_builder > CRT_BVH_DEVICE) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown bvh_builder");
+    bvh_builder = config.bvh_builder ? config.bvh_builder : (uint32_t)CRT_BVH_HOST;
     if (stack_lds != 8 && stack_lds != 16 && stack_lds != 32)
         return set_error(CRT_ERR_INVALID_ARGUMENT, "stack_lds must be 8, 16 or 32");
     grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 24u;
@@ -202,7 +218,7 @@ int Context::init(const crt_config* cfg) {
 uint64_t Context::used_bytes() const {
     uint64_t b = scene_table.bytes + counters.bytes + ctl.bytes + gstack.bytes + req_table[0].bytes + req_table[1].bytes;
     for (int k = 0; k < 2; ++k) b += ray_o[k].bytes + ray_d[k].bytes + thr[k].bytes + rad[k].bytes + ids[k].bytes;
-    b += hit.bytes + hit_inst.bytes + out_rad.bytes;
+    b += hit.bytes + hit_inst.bytes + exact_q.bytes + out_rad.bytes;
     return b;
 }
 
@@ -218,6 +234,7 @@ int Context::ensure_paths(uint32_t n, uint32_t pass_ids) {
         }
         if ((rc = hit.alloc((size_t)n * 16, this))) return rc;
         if ((rc = hit_inst.alloc((size_t)n * 4, this))) return rc;
+        if ((rc = exact_q.alloc((size_t)n * 4, this))) return rc;
         path_capacity = n;
     }
     if (pass_ids > pass_capacity) {
@@ -236,10 +253,24 @@ int Context::ensure_gstack(uint32_t blocks) {
 }
 
 crt_paths Context::paths(int k, uint32_t slot0) {
-    crt_paths p;
-    p.ray_o = (float4*)ray_o[k].p + slot0; p.ray_d = (float4*)ray_d[k].p + slot0; p.thr = (float4*)thr[k].p + slot0;
-    p.rad = (float4*)rad[k].p + slot0; p.ids = (uint2*)ids[k].p + slot0;
-    return p;
+    return pool_bufs().paths(k, slot0);
+}
+
+// The context's own pass buffers (allocated at crt_create, grown by the autotuner).
+PassBufs Context::pool_bufs() {
+    PassBufs B;
+    for (int k = 0; k < 2; ++k) {
+        B.ray_o[k] = (float4*)ray_o[k].p; B.ray_d[k] = (float4*)ray_d[k].p; B.thr[k] = (float4*)thr[k].p;
+        B.rad[k] = (float4*)rad[k].p; B.ids[k] = (uint2*)ids[k].p; B.req[k] = (crt_dev_request*)req_table[k].p;
+    }
+    B.hit = (float4*)hit.p; B.hit_inst = (uint32_t*)hit_inst.p; B.exact_q = (uint32_t*)exact_q.p;
+    B.out_rad = (float4*)out_rad.p;
+    B.gstack = (uint32_t*)gstack.p; B.ctl = (uint32_t*)ctl.p; B.counters = (crt_dev_counters*)counters.p;
+    B.slots = std::min(max_paths, path_capacity);
+    B.pass_ids = pass_capacity;
+    B.extend_grid = extend_grid;
+    B.max_sub = sub_passes;
+    return B;
 }
 
 Context::~Context() {
@@ -277,71 +308,168 @@ int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
     return upload_scene_locked(desc, leaf_size, bvh_width, out);
 }
 
-// Builds and uploads a scene with the given BVH shape (caller holds gpu_mutex).
+// Device bytes a device build of `d` may take at most (the scene's own allocations; the build's
+// work buffers come and go within the call): one node per binary interior node, every record,
+// per-triangle normals / uvs, sphere mappings and the tables.
+static uint64_t device_build_bound(const crt_scene_desc* d, uint32_t width, uint64_t tables) {
+    uint64_t nt = 0;
+    bool nrm = false, uvs = false;
+    for (uint32_t m = 0; m < d->num_meshes; ++m) {
+        nt += d->meshes[m].num_triangles;
+        nrm = nrm || d->meshes[m].normals; uvs = uvs || d->meshes[m].uvs;
+    }
+    const uint64_t nq = (uint64_t)d->num_quads + d->num_lights, ns = (uint64_t)d->num_spheres + d->num_sphere_lights,
+                   n = nt + ns + nq;
+    return n * (width == 4 ? 128u : 64u) + nt * 48u + ns * 32u + nq * 80u + (nrm ? nt * 48u : 0u) + (uvs ? nt * 24u : 0u) +
+           ns * 24u + ((uint64_t)d->num_lights + d->num_sphere_lights) * 4u + tables + 4 * 256u;
+}
+
+// Builds and uploads a scene with the given BVH shape (caller holds gpu_mutex): on the host (binned
+// SAH, one upload), or on the device when the context asks for it and the scene is one-level with
+// two primitives or more.
 int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint32_t width, crt_scene_id* out) {
     auto sc = std::make_unique<DeviceScene>();
     std::string err;
-    unsigned hw = std::thread::hardware_concurrency();
-    int rc = build_host_scene(desc, leaf, width, (int)std::min(16u, hw ? hw : 1u), sc->host, err);
-    if (rc) return set_error(rc, err);
-    uint64_t need = sc->host.device_bytes();
-    if (scene_bytes + request_bytes + reserved + need > budget)
-        return set_error(CRT_ERR_OVER_BUDGET, "scene of " + std::to_string(need) + " bytes exceeds the HBM budget");
     uint32_t slot = kMaxScenes;
     for (uint32_t k = 0; k < kMaxScenes; ++k) if (!slot_used[k]) { slot = k; break; }
     if (slot == kMaxScenes) return set_error(CRT_ERR_OVER_BUDGET, "scene table full");
+    uint64_t nprims = 0;
+    if (desc && desc->abi_version == CRT_ABI_VERSION) {
+        nprims = (uint64_t)desc->num_lights + desc->num_sphere_lights + desc->num_spheres + desc->num_quads;
+        for (uint32_t m = 0; m < desc->num_meshes && desc->meshes; ++m) nprims += desc->meshes[m].num_triangles;
+    }
+    const bool on_device = bvh_builder == CRT_BVH_DEVICE && desc && desc->abi_version == CRT_ABI_VERSION &&
+                           desc->num_instances == 0 && nprims >= 2 && (desc->num_meshes == 0 || desc->meshes);
     auto t0 = std::chrono::steady_clock::now();
     HostScene& H = sc->host;
-    {
-        // nodes, triangle, instance, sphere and quad records in one allocation, in that order (the
-        // traversal fetches them through 32-bit byte offsets from the node array); 256 spare bytes
-        // keep the shared loads of the last record (and of a node-less scene) in bounds
-        struct Part { const void* src; size_t bytes, off; };
-        const Part parts[5] = {{H.nodes.data(), H.nodes.size() * sizeof(F4), 0},
-                               {H.tris.data(), H.tris.size() * sizeof(F4), (size_t)H.tri_offset()},
-                               {H.insts.data(), H.insts.size() * sizeof(F4), (size_t)H.inst_offset()},
-                               {H.spheres.data(), H.spheres.size() * sizeof(F4), (size_t)H.sphere_offset()},
-                               {H.quads.data(), H.quads.size() * sizeof(F4), (size_t)H.quad_offset()}};
-        if ((rc = sc->geo.alloc((size_t)H.geometry_bytes() + 256, this))) return rc;
-        for (const Part& q : parts)
-            if (q.bytes) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + q.off, q.src, q.bytes, hipMemcpyHostToDevice, stream));
+    uint64_t need = 0;
+    if (on_device) {
+        uint64_t tables = 0;
+        for (uint32_t k = 0; k < desc->num_textures && desc->textures; ++k) {
+            const crt_texture& t = desc->textures[k];
+            tables += 32u + (t.type == CRT_TEX_IMAGE ? (uint64_t)t.width * t.height * t.channels : 0u);
+        }
+        tables += ((uint64_t)desc->num_materials + desc->num_lights + 2ull * desc->num_sphere_lights) * 32u;
+        if (scene_bytes + request_bytes + reserved + device_build_bound(desc, width, tables) > budget)
+            return set_error(CRT_ERR_OVER_BUDGET, "scene exceeds the HBM budget");
+        int rc = gpu_build_scene(*this, desc, leaf, width, *sc, err);
+        if (rc) return err.empty() ? rc : set_error(rc, err);
+        need = sc->bytes;
+    } else {
+        unsigned hw = std::thread::hardware_concurrency();
+        int rc = build_host_scene(desc, leaf, width, (int)std::min(16u, hw ? hw : 1u), H, err);
+        if (rc) return set_error(rc, err);
+        need = H.device_bytes();
+        if (scene_bytes + request_bytes + reserved + need > budget)
+            return set_error(CRT_ERR_OVER_BUDGET, "scene of " + std::to_string(need) + " bytes exceeds the HBM budget");
+        t0 = std::chrono::steady_clock::now();
+        {
+            // nodes, triangle, instance, sphere and quad records in one allocation, in that order (the
+            // traversal fetches them through 32-bit byte offsets from the node array); 256 spare bytes
+            // keep the shared loads of the last record (and of a node-less scene) in bounds
+            struct Part { const void* src; size_t bytes, off; };
+            const Part parts[5] = {{H.nodes.data(), H.nodes.size() * sizeof(F4), 0},
+                                   {H.tris.data(), H.tris.size() * sizeof(F4), (size_t)H.tri_offset()},
+                                   {H.insts.data(), H.insts.size() * sizeof(F4), (size_t)H.inst_offset()},
+                                   {H.spheres.data(), H.spheres.size() * sizeof(F4), (size_t)H.sphere_offset()},
+                                   {H.quads.data(), H.quads.size() * sizeof(F4), (size_t)H.quad_offset()}};
+            if ((rc = sc->geo.alloc((size_t)H.geometry_bytes() + 256, this))) return rc;
+            for (const Part& q : parts)
+                if (q.bytes) HIP_TRY(hipMemcpyAsync((char*)sc->geo.p + q.off, q.src, q.bytes, hipMemcpyHostToDevice, stream));
+        }
+        if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
+        if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
+        if ((rc = upload_vec(sc->sphere_map, H.sphere_map, this, stream))) return rc;
+        if ((rc = upload_vec(sc->mats, H.mats, this, stream))) return rc;
+        if ((rc = upload_vec(sc->texs, H.texs, this, stream))) return rc;
+        if ((rc = upload_vec(sc->texels, H.texels, this, stream))) return rc;
+        if ((rc = upload_vec(sc->lights, H.light_refs, this, stream))) return rc;
+        crt_dev_scene& ds = sc->dev;
+        std::memset(&ds, 0, sizeof(ds));
+        char* geo = (char*)sc->geo.p;
+        ds.nodes = geo; ds.tris = geo + H.tri_offset(); ds.tri_normals = sc->tri_normals.p;
+        ds.tri_uvs = sc->tri_uvs.p; ds.spheres = geo + H.sphere_offset();
+        ds.sphere_map = sc->sphere_map.p; ds.quads = geo + H.quad_offset(); ds.insts = geo + H.inst_offset();
+        ds.mats = sc->mats.p; ds.texs = sc->texs.p;
+        ds.texels = (const uint8_t*)sc->texels.p; ds.lights = (const uint32_t*)sc->lights.p;
+        ds.num_lights = H.num_lights; ds.root = H.root; ds.num_quads = H.num_quads; ds.bvh_width = H.bvh_width;
+        for (int k = 0; k < 3; ++k) ds.background[k] = H.background[k];
+        H.num_node_records = H.nodes.size() / (H.bvh_width == 4 ? 8u : 4u);
+        H.num_tri_records = H.tris.size() / 3;
     }
-    if ((rc = upload_vec(sc->tri_normals, H.tri_normals, this, stream))) return rc;
-    if ((rc = upload_vec(sc->tri_uvs, H.tri_uvs, this, stream))) return rc;
-    if ((rc = upload_vec(sc->sphere_map, H.sphere_map, this, stream))) return rc;
-    if ((rc = upload_vec(sc->mats, H.mats, this, stream))) return rc;
-    if ((rc = upload_vec(sc->texs, H.texs, this, stream))) return rc;
-    if ((rc = upload_vec(sc->texels, H.texels, this, stream))) return rc;
-    if ((rc = upload_vec(sc->light_quads, H.light_quads, this, stream))) return rc;
-    crt_dev_scene& ds = sc->dev;
-    std::memset(&ds, 0, sizeof(ds));
-    char* geo = (char*)sc->geo.p;
-    ds.nodes = geo; ds.tris = geo + H.tri_offset(); ds.tri_normals = sc->tri_normals.p;
-    ds.tri_uvs = sc->tri_uvs.p; ds.spheres = geo + H.sphere_offset();
-    ds.sphere_map = sc->sphere_map.p; ds.quads = geo + H.quad_offset(); ds.insts = geo + H.inst_offset();
-    ds.mats = sc->mats.p; ds.texs = sc->texs.p;
-    ds.texels = (const uint8_t*)sc->texels.p; ds.light_quads = (const uint32_t*)sc->light_quads.p;
-    ds.num_lights = H.num_lights; ds.root = H.root; ds.num_quads = H.num_quads; ds.bvh_width = H.bvh_width;
-    for (int k = 0; k < 3; ++k) ds.background[k] = H.background[k];
-    HIP_TRY(hipMemcpyAsync((crt_dev_scene*)scene_table.p + slot, &ds, sizeof(ds), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync((crt_dev_scene*)scene_table.p + slot, &sc->dev, sizeof(crt_dev_scene), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     sc->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     sc->slot = slot;
     sc->bytes = need;
-    // leaf order -> global primitive id for first-hit queries, then drop the big host copies
-    sc->tri_gid.resize(H.tris.size() / 3);
-    for (size_t i = 0; i < sc->tri_gid.size(); ++i) std::memcpy(&sc->tri_gid[i], &H.tris[3 * i].w, 4);
-    sc->inst_gid.resize(H.insts.size() / 4);
-    for (size_t i = 0; i < sc->inst_gid.size(); ++i) std::memcpy(&sc->inst_gid[i], &H.insts[4 * i + 3].y, 4);
-    sc->quad_gid.resize(H.quads.size() / 5);
-    for (size_t i = 0; i < sc->quad_gid.size(); ++i) std::memcpy(&sc->quad_gid[i], &H.quads[5 * i + 3].w, 4);
-    H.nodes.clear(); H.nodes.shrink_to_fit(); H.tris.clear(); H.tris.shrink_to_fit();
-    H.texels.clear(); H.texels.shrink_to_fit();
+    if (!on_device) {
+        // leaf order -> global primitive id for first-hit queries, then drop the big host copies
+        sc->tri_gid.resize(H.tris.size() / 3);
+        for (size_t i = 0; i < sc->tri_gid.size(); ++i) std::memcpy(&sc->tri_gid[i], &H.tris[3 * i].w, 4);
+        sc->inst_gid.resize(H.insts.size() / 4);
+        for (size_t i = 0; i < sc->inst_gid.size(); ++i) std::memcpy(&sc->inst_gid[i], &H.insts[4 * i + 3].y, 4);
+        sc->quad_gid.resize(H.quads.size() / 5);
+        for (size_t i = 0; i < sc->quad_gid.size(); ++i) std::memcpy(&sc->quad_gid[i], &H.quads[5 * i + 3].w, 4);
+        H.nodes.clear(); H.nodes.shrink_to_fit(); H.tris.clear(); H.tris.shrink_to_fit();
+        H.texels.clear(); H.texels.shrink_to_fit();
+    }
     slot_used[slot] = true;
     scene_bytes += need;
     crt_scene_id id = next_scene_id++;
     scenes[id] = std::move(sc);
     *out = id;
+    return CRT_OK;
+}
+
+// Reads the scene's BVH and records back into a HostScene and checks its invariants (see
+// crt_scene_verify). Caller holds gpu_mutex.
+int Context::verify_scene(const DeviceScene& sc, std::string& err) {
+    const HostScene& H = sc.host;
+    HostScene V;
+    V.bvh_width = H.bvh_width;
+    V.root = H.root;
+    const uint32_t per = H.bvh_width == 4 ? 8u : 4u;
+    const char* geo = (const char*)sc.geo.p;
+    const uint64_t nq = H.num_quad_records;
+    V.nodes.resize(H.num_node_records * per);
+    V.tris.resize(H.num_tri_records * 3);
+    V.spheres.resize((size_t)H.num_spheres * 2);
+    V.quads.resize(nq * 5);
+    V.insts.resize(sc.inst_gid.size() * 4);
+    auto rd = [&](std::vector<F4>& v, const void* src) -> int {
+        if (!v.empty()) HIP_TRY(hipMemcpyAsync(v.data(), src, v.size() * sizeof(F4), hipMemcpyDeviceToHost, stream));
+        return CRT_OK;
+    };
+    int rc;
+    if ((rc = rd(V.nodes, geo)) || (rc = rd(V.tris, sc.dev.tris)) || (rc = rd(V.spheres, sc.dev.spheres)) ||
+        (rc = rd(V.quads, sc.dev.quads)) || (rc = rd(V.insts, sc.dev.insts)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    if ((rc = check_host_scene(V, err))) return rc;
+    // the leaf-order records are a permutation of the scene's primitives: their global ids (kept in
+    // the records and in the host maps) agree and cover the scene's id range once
+    std::vector<uint32_t> ids;
+    ids.reserve(V.tris.size() / 3 + V.spheres.size() / 2 + nq);
+    for (size_t i = 0; i < V.tris.size() / 3; ++i) {
+        uint32_t g; std::memcpy(&g, &V.tris[3 * i].w, 4);
+        if (i < sc.tri_gid.size() && g != sc.tri_gid[i] && H.num_instances == 0) { err = "triangle record id differs from the id map"; return CRT_ERR_INTERNAL; }
+        ids.push_back(g);
+    }
+    for (size_t i = 0; i < V.spheres.size() / 2; ++i) {
+        uint32_t g; std::memcpy(&g, &V.spheres[2 * i + 1].x, 4);
+        if (i < H.sphere_aux.size() && g != H.sphere_aux[i].x) { err = "sphere record id differs from the id map"; return CRT_ERR_INTERNAL; }
+        ids.push_back(g);
+    }
+    for (size_t i = 0; i < nq; ++i) {
+        uint32_t g; std::memcpy(&g, &V.quads[5 * i + 3].w, 4);
+        if (i < sc.quad_gid.size() && g != sc.quad_gid[i]) { err = "quad record id differs from the id map"; return CRT_ERR_INTERNAL; }
+        ids.push_back(g);
+    }
+    if (H.num_instances == 0) {
+        std::sort(ids.begin(), ids.end());
+        for (size_t i = 0; i < ids.size(); ++i)
+            if (ids[i] != (uint32_t)i) { err = "primitive records do not cover the scene's ids once"; return CRT_ERR_INTERNAL; }
+    }
     return CRT_OK;
 }
 
@@ -398,7 +526,7 @@ int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStrea
     s_next = s_begin;
     p_next = 0;
     std::memset(&req, 0, sizeof(req));
-    camera_setup(sc->host.camera, req);
+    camera_setup(sc->host.camera, r.width, req);
     req.width = r.width; req.height = r.height; req.sqrt_spp = sq; req.spp = r.spp; req.max_depth = r.max_depth;
     req.rng_mode = r.rng_mode; req.scene = sc->slot; req.seed = r.seed;
     int rc;
@@ -435,7 +563,7 @@ bool Job::take(uint32_t cap, WorkItem& w) {
 
 // Runs one fused pass over the given work items (all with the same rng mode). Asynchronous on
 // ctx->stream; the caller synchronizes before reading results.
-int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms) {
+int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms, const PassBufs& B) {
     if (items.empty()) return CRT_OK;
     int buf = pass_parity;
     pass_parity ^= 1;
@@ -465,22 +593,23 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
         if (w.job->scene->host.bvh_width != width)
             return set_error(CRT_ERR_INVALID_ARGUMENT, "scenes of different BVH widths in one pass");
     }
-    if (npaths > pass_capacity) return set_error(CRT_ERR_INTERNAL, "pass exceeds the pass capacity");
-    crt_dev_request* dreq = (crt_dev_request*)req_table[buf].p;
+    if (npaths > B.pass_ids) return set_error(CRT_ERR_INTERNAL, "pass exceeds the pass capacity");
+    crt_dev_request* dreq = B.req[buf];
     HIP_TRY(hipMemcpyAsync(dreq, host, sizeof(crt_dev_request) * items.size(), hipMemcpyHostToDevice, stream));
     const crt_dev_scene* dsc = (const crt_dev_scene*)scene_table.p;
     int nreq = (int)items.size();
     // The pass runs as K independent sub-passes on K streams (contiguous path-id ranges, disjoint
     // slot ranges, own control blocks and overflow stacks), so the tail of one sub-pass's launch
     // overlaps full launches of the others. Timed-stage runs use one stream.
-    const uint32_t nslots_all = std::min(std::min(max_paths, path_capacity), npaths);
-    uint32_t K = time_stages ? 1u : std::min<uint32_t>(sub_passes, std::max<uint32_t>(1u, nslots_all / 65536u));
+    const uint32_t nslots_all = std::min(B.slots, npaths);
+    uint32_t K = time_stages ? 1u : std::min<uint32_t>(std::min(sub_passes, B.max_sub), std::max<uint32_t>(1u, nslots_all / 65536u));
+    K = std::max(1u, K);
     struct Sub { uint32_t lo, hi, slot0, nslots, egrid, sgrid; hipStream_t s; uint64_t k; bool done; };
     Sub subs[kMaxSubPasses];
-    uint32_t* cb = (uint32_t*)ctl.p;
+    uint32_t* cb = B.ctl;
     if (crt_k_pass_init(stream, cb, 2u * CRT_CTL_WORDS * K)) return set_error(CRT_ERR_HIP, "pass init launch failed");
     HIP_TRY(hipEventRecord(ev_pass_ready, stream));
-    const uint32_t egrid_all = extend_grid / K;
+    const uint32_t egrid_all = B.extend_grid / K;
     for (uint32_t q = 0; q < K; ++q) {
         Sub& u = subs[q];
         u.lo = (uint32_t)((uint64_t)npaths * q / K);
@@ -525,17 +654,17 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             const uint64_t k = u.k;
             uint32_t* cin = cb + (q * 2 + (k & 1)) * CRT_CTL_WORDS;
             uint32_t* cout = cb + (q * 2 + ((k + 1) & 1)) * CRT_CTL_WORDS;
-            crt_paths A = paths((int)(k & 1), u.slot0), B = paths((int)((k + 1) & 1), u.slot0);
-            float4* hq = (float4*)hit.p + u.slot0;
-            uint32_t* hiq = (uint32_t*)hit_inst.p + u.slot0;
-            uint32_t* gq = (uint32_t*)gstack.p + (size_t)q * egrid_all * 256u * crt_k_gstack_entries();
+            crt_paths A = B.paths((int)(k & 1), u.slot0), Bn = B.paths((int)((k + 1) & 1), u.slot0);
+            float4* hq = B.hit + u.slot0;
+            uint32_t* hiq = B.hit_inst + u.slot0;
+            uint32_t* gq = B.gstack + (size_t)q * egrid_all * 256u * crt_k_gstack_entries();
             if (crt_k_regen(u.s, mode, u.sgrid, dreq, nreq, u.lo, total, u.nslots, cin, cout, nullptr, A))
                 return set_error(CRT_ERR_HIP, "regen launch failed");
             if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], u.s));
             if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)width, two_level, u.egrid, dsc, dreq, uniform_scene,
                              nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, hq, hiq, gq, nullptr, extend_mode,
-                             cin + CRT_CTL_WORK, extend_param, (int)stack_lds, cin + CRT_CTL_SURV,
-                             (crt_dev_counters*)counters.p))
+                             cin + CRT_CTL_WORK, extend_param, (int)stack_lds, cin + CRT_CTL_SURV, B.exact_q + u.slot0,
+                             cin + CRT_CTL_EXACT, B.counters))
                 return set_error(CRT_ERR_HIP, "extend launch failed");
             if (time_stages) {
                 HIP_TRY(hipEventRecord(ev_ext[1], u.s));
@@ -545,8 +674,8 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
                 if (extend_ms) *extend_ms += ms;
                 extend_launches++;
             }
-            if (crt_k_shade(u.s, mode, two_level, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV, hq, hiq,
-                            (float4*)out_rad.p, nullptr, (int)ray_sort, nreq == 1 ? 1 : 0, (crt_dev_counters*)counters.p))
+            if (crt_k_shade(u.s, mode, two_level, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, Bn, cout + CRT_CTL_SURV, hq, hiq,
+                            B.out_rad, nullptr, (int)ray_sort, nreq == 1 ? 1 : 0, B.counters))
                 return set_error(CRT_ERR_HIP, "shade launch failed");
             HIP_TRY(hipMemcpyAsync(ring + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
                                    hipMemcpyDeviceToHost, u.s));
@@ -555,7 +684,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             iterations++;
         }
     }
-    if (crt_k_resolve(stream, dreq, nreq, max_pix, (const float4*)out_rad.p, nullptr))
+    if (crt_k_resolve(stream, dreq, nreq, max_pix, B.out_rad, nullptr))
         return set_error(CRT_ERR_HIP, "resolve launch failed");
     HIP_TRY(hipEventRecord(req_event[buf], stream));
     passes++;
@@ -607,14 +736,14 @@ int Context::finish_job(Job& j, crt_image* out) {
     return CRT_OK;
 }
 
-int Context::reset_counters() {
-    if (crt_k_fill_u32(stream, (uint32_t*)counters.p, sizeof(crt_dev_counters) / 4, 0u))
+int Context::reset_counters(crt_dev_counters* c) {
+    if (crt_k_fill_u32(stream, (uint32_t*)c, sizeof(crt_dev_counters) / 4, 0u))
         return set_error(CRT_ERR_HIP, "fill launch failed");
     return CRT_OK;
 }
 
-int Context::read_counters(crt_dev_counters& c) {
-    HIP_TRY(hipMemcpyAsync(counters_host, counters.p, sizeof(crt_dev_counters), hipMemcpyDeviceToHost, stream));
+int Context::read_counters(crt_dev_counters& c, const crt_dev_counters* dev) {
+    HIP_TRY(hipMemcpyAsync(counters_host, dev, sizeof(crt_dev_counters), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     c = *(crt_dev_counters*)counters_host;
     return CRT_OK;
@@ -636,10 +765,10 @@ void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double
 }
 
 // Plain and batched rendering: all items are admitted at once, passes are packed greedily.
-int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* stats) {
+int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* stats, const PassBufs& B) {
     double h0 = now_s();
     int rc;
-    if ((rc = reset_counters())) return rc;
+    if ((rc = reset_counters(B.counters))) return rc;
     HIP_TRY(hipEventRecord(ev_start, stream));
     double ext_ms = 0.0;
     uint64_t paths = 0;
@@ -652,7 +781,7 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     for (int mode = 0; mode < 2; ++mode) {
         for (;;) {
             std::vector<WorkItem> items;
-            uint32_t cap = pass_capacity;
+            uint32_t cap = B.pass_ids;
             for (Job* j : jobs) {
                 if ((int)j->rd.rng_mode != mode || (j->rd.flags & CRT_RENDER_MEGAKERNEL)) continue;
                 // at most one work item per request and pass: a pixel's samples are resolved by one
@@ -667,7 +796,7 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
                 }
             }
             if (items.empty()) break;
-            if ((rc = run_pass(items, time_stages, &ext_ms))) return rc;
+            if ((rc = run_pass(items, time_stages, &ext_ms, B))) return rc;
         }
     }
     HIP_TRY(hipEventRecord(ev_stop, stream));
@@ -675,7 +804,7 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     float gms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&gms, ev_start, ev_stop));
     crt_dev_counters c;
-    if ((rc = read_counters(c))) return rc;
+    if ((rc = read_counters(c, B.counters))) return rc;
     if (c.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
     fill_stats(stats, c, paths, gms, ext_ms, (now_s() - h0) * 1e3);
     last.rays = c.rays + c.retraces; last.retraces = c.retraces; last.light_pdf_tests = c.light_tests;
@@ -684,6 +813,7 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     last.extend_launches = (time_stages ? extend_launches - launches0 : 0);
     last.passes = passes - passes0;
     last.iterations = iterations - iter0;
+    last.exact_queries = c.exact;
     return CRT_OK;
 }
 
@@ -742,6 +872,7 @@ int crt_scene_get_info(crt_context* c, crt_scene_id s, crt_scene_info* info) {
     info->bvh_width = sc->host.bvh_width;
     info->node_bytes = sc->host.bvh_width == 4 ? 128u : 64u;
     info->leaf_size = sc->host.leaf_size;
+    info->builder = sc->host.device_built ? (uint32_t)CRT_BVH_DEVICE : (uint32_t)CRT_BVH_HOST;
     info->build_ms = sc->host.stats.build_ms;
     info->upload_ms = sc->upload_ms;
     info->sah_cost = sc->host.stats.sah_cost;
@@ -779,7 +910,7 @@ int crt_render_batch(crt_context* c, crt_batch_item* items, uint32_t n, crt_stat
         jobs.push_back(std::move(j));
     }
     bool time_stages = (n > 0) && (items[0].rd.flags & CRT_RENDER_TIME_STAGES);
-    int rc = ctx->render_jobs(jp, time_stages, stats);
+    int rc = ctx->render_jobs(jp, time_stages, stats, ctx->pool_bufs());
     if (rc) return rc;
     for (uint32_t k = 0; k < n; ++k) {
         if ((rc = ctx->finish_job(*jobs[k], &items[k].out))) return rc;
@@ -788,7 +919,7 @@ int crt_render_batch(crt_context* c, crt_batch_item* items, uint32_t n, crt_stat
     return CRT_OK;
 }
 
-int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n, int32_t camera_rays, int32_t* prim,
+int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n, int32_t flags, int32_t* prim,
                    int32_t* inst, float* t, float* uv) {
     if (!c || (!rays && n)) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
     if (n > (1ull << 30)) return set_error(CRT_ERR_INVALID_ARGUMENT, "too many rays");
@@ -799,24 +930,55 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
     if (n == 0) return CRT_OK;
     DevBuf dr, dh, di;
     int rc;
-    if ((rc = dr.alloc(n * 32, ctx))) return rc;
+    const bool camera = (flags & CRT_TRACE_CAMERA) != 0;
     if ((rc = dh.alloc(n * 16, ctx))) return rc;
     if ((rc = di.alloc(n * 4, ctx))) return rc;
-    HIP_TRY(hipMemcpyAsync(dr.p, rays, n * 32, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = ctx->reset_counters())) return rc;
+    if ((rc = ctx->reset_counters((crt_dev_counters*)ctx->counters.p))) return rc;
     uint32_t grid = std::min(ctx->extend_grid, (uint32_t)std::max<uint64_t>(1, (n + 255) / 256));
-    if (crt_k_trace_rays(ctx->stream, (int)sc->host.bvh_width, sc->host.num_instances ? 1 : 0, grid,
-                         (const crt_dev_scene*)ctx->scene_table.p + sc->slot, (const float4*)dr.p,
-                         (uint32_t)n, camera_rays, (float4*)dh.p, (uint32_t*)di.p, (uint32_t*)ctx->gstack.p,
+    const int two_level = sc->host.num_instances ? 1 : 0;
+    if (flags & CRT_TRACE_STAGE) {
+        // the render's closest-hit stage: path-state rays (origin, time 0; direction, depth 1 and the
+        // camera bit), one uniform scene, the stage's exact re-trace queue and control words
+        DevBuf dro, drd, dids, dq, dctl;
+        if ((rc = dro.alloc(n * 16, ctx)) || (rc = drd.alloc(n * 16, ctx)) || (rc = dids.alloc(n * 8, ctx)) ||
+            (rc = dq.alloc(n * 4, ctx)) || (rc = dctl.alloc(CRT_CTL_WORDS * 4, ctx)))
+            return rc;
+        std::vector<float> po(n * 4), pd(n * 4);
+        const uint32_t state = 1u | (camera ? 0x10000u : 0u);   // depth 1 | camera bit of the path state
+        for (uint64_t k = 0; k < n; ++k) {
+            for (int a = 0; a < 3; ++a) { po[4 * k + a] = rays[8 * k + a]; pd[4 * k + a] = rays[8 * k + 4 + a]; }
+            po[4 * k + 3] = 0.0f;
+            std::memcpy(&pd[4 * k + 3], &state, 4);
+        }
+        HIP_TRY(hipMemcpyAsync(dro.p, po.data(), n * 16, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(drd.p, pd.data(), n * 16, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemsetAsync(dids.p, 0, n * 8, ctx->stream));
+        HIP_TRY(hipMemsetAsync(dctl.p, 0, CRT_CTL_WORDS * 4, ctx->stream));
+        uint32_t* ctl = (uint32_t*)dctl.p;
+        if (crt_k_extend(ctx->stream, ctx->config.count_work ? 1 : 0, (int)sc->host.bvh_width, two_level, grid,
+                         (const crt_dev_scene*)ctx->scene_table.p, nullptr, (int32_t)sc->slot, nullptr, nullptr,
+                         (uint32_t)n, (const float4*)dro.p, (const float4*)drd.p, (const uint2*)dids.p, (float4*)dh.p,
+                         (uint32_t*)di.p, (uint32_t*)ctx->gstack.p, nullptr, ctx->extend_mode, ctl + CRT_CTL_WORK,
+                         ctx->extend_param, (int)ctx->stack_lds, nullptr, (uint32_t*)dq.p, ctl + CRT_CTL_EXACT,
                          (crt_dev_counters*)ctx->counters.p))
-        return set_error(CRT_ERR_HIP, "trace launch failed");
+            return set_error(CRT_ERR_HIP, "extend launch failed");
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    } else {
+        if ((rc = dr.alloc(n * 32, ctx))) return rc;
+        HIP_TRY(hipMemcpyAsync(dr.p, rays, n * 32, hipMemcpyHostToDevice, ctx->stream));
+        if (crt_k_trace_rays(ctx->stream, (int)sc->host.bvh_width, two_level, grid,
+                             (const crt_dev_scene*)ctx->scene_table.p + sc->slot, (const float4*)dr.p,
+                             (uint32_t)n, camera ? 1 : 0, (float4*)dh.p, (uint32_t*)di.p, (uint32_t*)ctx->gstack.p,
+                             (crt_dev_counters*)ctx->counters.p))
+            return set_error(CRT_ERR_HIP, "trace launch failed");
+    }
     std::vector<float> hh(n * 4);
     std::vector<uint32_t> hi(n);
     HIP_TRY(hipMemcpyAsync(hh.data(), dh.p, n * 16, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(hi.data(), di.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     crt_dev_counters cnt;
-    if ((rc = ctx->read_counters(cnt))) return rc;
+    if ((rc = ctx->read_counters(cnt, (const crt_dev_counters*)ctx->counters.p))) return rc;
     if (cnt.stack_overflow) return set_error(CRT_ERR_INTERNAL, "traversal stack overflow");
     // map leaf-ordered hits back to global ids: copies of the id words are kept on the host
     const HostScene& H = sc->host;
@@ -831,7 +993,8 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
             else gid = (int32_t)sc->quad_gid[idx];
         }
         if (prim) prim[k] = gid;
-        if (inst) inst[k] = (p == CRT_MISS || hi[k] == CRT_NO_INST) ? -1 : (int32_t)sc->inst_gid[hi[k]];
+        const uint32_t iw = hi[k] & CRT_INST_MASK;
+        if (inst) inst[k] = (p == CRT_MISS || iw == CRT_NO_INST) ? -1 : (int32_t)sc->inst_gid[iw];
         if (t) t[k] = p == CRT_MISS ? 0.0f : hh[4 * k];
         if (uv) { uv[2 * k] = p == CRT_MISS ? 0.0f : hh[4 * k + 2]; uv[2 * k + 1] = p == CRT_MISS ? 0.0f : hh[4 * k + 3]; }
     }
@@ -846,7 +1009,7 @@ int crt_trace_primary(crt_context* c, crt_scene_id s, uint32_t W, uint32_t H, in
         std::lock_guard<std::mutex> g(ctx->gpu_mutex);
         DeviceScene* sc = ctx->find_scene(s);
         if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
-        camera_setup(sc->host.camera, r);
+        camera_setup(sc->host.camera, W, r);
     }
     std::vector<float> rays((size_t)W * H * 8);
     for (uint32_t jj = 0; jj < H; ++jj) {
@@ -855,14 +1018,30 @@ int crt_trace_primary(crt_context* c, crt_scene_id s, uint32_t W, uint32_t H, in
             float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
             float* q = &rays[8 * ((size_t)jj * W + i)];
             for (int k = 0; k < 3; ++k) {
-                q[k] = r.origin[k];
-                q[4 + k] = r.llc[k] + u * r.horizontal[k] + v * r.vertical[k] - r.origin[k];
+                if (r.ortho) {
+                    q[k] = r.llc[k] + u * r.horizontal[k] + v * r.vertical[k];
+                    q[4 + k] = r.ortho_dir[k];
+                } else {
+                    q[k] = r.origin[k];
+                    q[4 + k] = r.llc[k] + u * r.horizontal[k] + v * r.vertical[k] - r.origin[k];
+                }
             }
             q[3] = 0.00001f; q[7] = 3.402823466e+38f;
         }
     }
     std::vector<float> uv((size_t)W * H * 2);
     return crt_trace_rays(c, s, rays.data(), (uint64_t)W * H, 1, prim, nullptr, t, uv.data());
+}
+
+int crt_scene_verify(crt_context* c, crt_scene_id s) {
+    if (!c) return set_error(CRT_ERR_INVALID_ARGUMENT, "null context");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    DeviceScene* sc = ctx->find_scene(s);
+    if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+    std::string err;
+    int rc = ctx->verify_scene(*sc, err);
+    return rc && !err.empty() ? set_error(rc, err) : rc;
 }
 
 int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, crt_scene_info* info) {
@@ -878,6 +1057,7 @@ int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, uint32_t 
         info->num_nodes = H.stats.nodes; info->num_leaves = H.stats.leaves; info->num_prims = H.stats.prims;
         info->device_bytes = H.device_bytes(); info->max_depth = H.stats.depth; info->num_instances = H.num_instances;
         info->bvh_width = H.bvh_width; info->node_bytes = H.bvh_width == 4 ? 128u : 64u; info->leaf_size = H.leaf_size;
+        info->builder = CRT_BVH_HOST;
         info->build_ms = H.stats.build_ms; info->upload_ms = 0.0; info->sah_cost = H.stats.sah_cost;
     }
     return CRT_OK;
@@ -912,7 +1092,7 @@ int crt_struct_sizes(uint64_t* sizes, uint32_t n) {
                     sizeof(crt_quad), sizeof(crt_area_light), sizeof(crt_mesh), sizeof(crt_instance), sizeof(crt_scene_desc),
                     sizeof(crt_render_desc), sizeof(crt_image), sizeof(crt_stats), sizeof(crt_config), sizeof(crt_scene_info),
                     sizeof(crt_batch_item), sizeof(crt_request_timing), sizeof(crt_work_counters), sizeof(crt_memory_info), sizeof(crt_obj_mesh),
-                    sizeof(crt_scenefile_info), sizeof(crt_tune_result)};
+                    sizeof(crt_scenefile_info), sizeof(crt_tune_result), sizeof(crt_sphere_light), sizeof(crt_transform)};
     uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
     for (uint32_t k = 0; k < n && k < m; ++k) sizes[k] = v[k];
     return (int)m;

@@ -42,14 +42,22 @@
 // triangle flags (float4 #2 .w as uint)
 #define CRT_TF_SMOOTH 1u
 #define CRT_TF_UV 2u
+// sphere flags (float4 #1 .z as uint): sphere lights (omni_light.cuh, spot_light.cuh)
+#define CRT_SF_INVISIBLE 1u    // hidden from camera rays (omni_light.cuh:62-65)
+#define CRT_SF_SPOT 2u         // spot light: back faces trigger the pass-through re-trace
+// light table entry: leaf-ordered quad index, or CRT_LIGHT_SPHERE | leaf-ordered sphere index
+#define CRT_LIGHT_SPHERE 0x80000000u
 
 // hit.prim encoding: type << 30 | leaf-ordered index; CRT_MISS = no hit
 #define CRT_MISS 0xffffffffu
 #define CRT_HIT_TYPE(p) ((p) >> 30)
 #define CRT_HIT_INDEX(p) ((p)&0x3fffffffu)
-// hit.inst: instance record index (leaf order) or CRT_NO_INST; bit 31 marks a pass-through re-trace result
-#define CRT_NO_INST 0x7fffffffu
+// hit.inst: instance record index (leaf order) or CRT_NO_INST; bit 31 marks a pass-through re-trace
+// result, bit 30 one behind a visible spot light's back face (which adds white, diffuse_spot_light.cuh:41-44)
+#define CRT_NO_INST 0x3fffffffu
 #define CRT_PASSTHROUGH 0x80000000u
+#define CRT_PASSTHROUGH_WHITE 0x40000000u
+#define CRT_INST_MASK 0x3fffffffu
 
 // material record (2 x float4)
 //  m0 = (albedo.rgb, param)   param = fuzz (metal) | ior (dielectric) | intensity (light)
@@ -71,7 +79,7 @@ struct crt_dev_scene {
     const void* mats;       // float4
     const void* texs;       // float4
     const uint8_t* texels;  // byte pool (rows top-down, RGB(A)8)
-    const uint32_t* light_quads;  // leaf-ordered quad index of each area light (world-list order)
+    const uint32_t* lights;       // light table in world-list order (CRT_LIGHT_SPHERE entries for sphere lights)
     uint32_t num_lights;
     uint32_t root;          // root node index
     uint32_t num_quads;
@@ -82,8 +90,9 @@ struct crt_dev_scene {
 
 // Per-request parameters read by the generate / shade / resolve kernels.
 struct crt_dev_request {
-    // camera (core/src/cameras/perspective_camera.cuh:41-113), computed on the host in FP32
-    float origin[3], llc[3], horizontal[3], vertical[3], cu[3], cv[3];
+    // camera (core/src/cameras/perspective_camera.cuh:41-113, orthographic_camera.cuh:57-142),
+    // computed on the host in FP32; ortho_dir = -w for the orthographic camera's parallel rays
+    float origin[3], llc[3], horizontal[3], vertical[3], cu[3], cv[3], ortho_dir[3];
     float lens_radius, time0, time1;
     uint32_t width, height, sqrt_spp, spp, max_depth;
     uint32_t rng_mode;
@@ -96,11 +105,13 @@ struct crt_dev_request {
     uint32_t sample_begin, sample_count;
     float* accum;            // float4 per pixel (rgb sums, sample count), rows top-down
     uint32_t* rng_state;     // MINSTD: per-pixel engine state carried across samples
-    uint32_t pad[2];
+    uint32_t ortho;          // 1: orthographic camera
+    uint32_t pad;
 };
 
 // Traversal statistics (counting kernel only)
 struct crt_dev_counters {
     unsigned long long node_visits, leaf_visits, tri_tests, sphere_tests, quad_tests, inst_visits;
     unsigned long long rays, retraces, light_tests, stack_overflow;
+    unsigned long long exact;   // queries re-traced with the double-precision triangle test
 };

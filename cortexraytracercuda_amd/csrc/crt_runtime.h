@@ -31,7 +31,7 @@ int set_error(int code, const std::string& msg);
                              std::string(#expr) + ": " + hipGetErrorString(e_));                           \
     } while (0)
 double now_s();
-void camera_setup(const crt_camera& c, crt_dev_request& r);
+void camera_setup(const crt_camera& c, uint32_t width, crt_dev_request& r);
 
 struct Context;
 
@@ -58,7 +58,7 @@ struct DeviceScene {
     HostScene host;  // geometry arrays are dropped after upload; stats and small tables stay
     DevBuf geo;   // BVH nodes, then triangle, instance, sphere and quad records (HostScene offsets)
     DevBuf tri_normals, tri_uvs, sphere_map, mats, texs, texels,
-        light_quads;
+        lights;
     std::vector<uint32_t> tri_gid, quad_gid;   // leaf order -> global primitive id
     std::vector<uint32_t> inst_gid;            // instance record (leaf order) -> caller's instance index
     crt_dev_scene dev{};
@@ -66,6 +66,28 @@ struct DeviceScene {
     uint64_t bytes = 0;
     double upload_ms = 0.0;
     std::atomic<int> refs{0};
+};
+
+// Device buffers a streaming pass runs on (the context's pools for crt_render and crt_render_batch).
+struct PassBufs {
+    float4* ray_o[2]; float4* ray_d[2]; float4* thr[2]; float4* rad[2]; uint2* ids[2];
+    float4* hit; uint32_t* hit_inst;
+    uint32_t* exact_q;              // exact re-trace queue of the closest-hit stage, a word per live slot
+    float4* out_rad;                // 16 B per path id of a pass
+    uint32_t* gstack;               // traversal overflow stacks of extend_grid workgroups
+    uint32_t* ctl;                  // 2 * CRT_CTL_WORDS words per sub-pass
+    crt_dev_counters* counters;
+    crt_dev_request* req[2];        // device request tables (alternating between passes)
+    uint32_t slots;                 // live path slots
+    uint32_t pass_ids;              // path ids per pass
+    uint32_t extend_grid;           // closest-hit workgroups
+    uint32_t max_sub;               // concurrent sub-passes the buffers allow
+    crt_paths paths(int k, uint32_t slot0) const {
+        crt_paths p;
+        p.ray_o = ray_o[k] + slot0; p.ray_d = ray_d[k] + slot0; p.thr = thr[k] + slot0; p.rad = rad[k] + slot0;
+        p.ids = ids[k] + slot0;
+        return p;
+    }
 };
 
 struct Job;
@@ -88,6 +110,7 @@ struct Job {
     static int validate(const crt_render_desc& r);
     static uint64_t bytes_needed(const crt_render_desc& r);
     int setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStream_t st = nullptr);
+    // a job whose accumulation buffer the caller owns (device memory): no allocation
     bool take(uint32_t cap, WorkItem& w);
     bool scheduled_all() const { return s_next >= s_end; }
 };
@@ -128,11 +151,12 @@ struct Server {
     static constexpr int kRing = 8, kLag = 4;    // snapshots in flight / host look-behind
     static constexpr int kOpenRing = 64;         // pinned (total, nreq) words for the appends
     static constexpr int kMaxShadeGrid = 65536;            // shade workgroups (one live-minimum word each)
-    static constexpr int kSnapWords = CRT_CTL_WORDS + 20 + kMaxShadeGrid;   // control half, counters, live minima
+    static constexpr int kCounterWords = (int)((sizeof(crt_dev_counters) + 3) / 4);
+    static constexpr int kSnapWords = CRT_CTL_WORDS + kCounterWords + kMaxShadeGrid;   // control half, counters, live minima
     bool ready = false;
     hipStream_t s = nullptr, copy_s = nullptr;
     uint32_t slots = 0, cap = 0, egrid = 0, sgrid = 0;
-    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad, req, ctl, gstack, counters, live_min;
+    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, exact_q, out_rad, req, ctl, gstack, counters, live_min;
     crt_dev_request* req_host = nullptr;   // pinned mirror of the open pass's request table
     uint32_t* open_host = nullptr;         // pinned ring of (total, nreq)
     uint32_t* snap = nullptr;              // pinned ring of iteration snapshots
@@ -170,6 +194,7 @@ struct Context {
     uint32_t extend_param = 2;       // lane-refill threshold or 64-ray groups per wave fetch
     uint32_t stack_lds = 16;         // LDS ring entries per lane of the traversal stack (8, 16, 32)
     uint32_t ray_sort = 0;           // shade groups each block's survivors by direction octant
+    uint32_t bvh_builder = CRT_BVH_HOST;   // where crt_scene_upload builds the BVH (CRT_BVH_*)
     uint32_t grid_per_cu = 16, shade_grid_per_cu = 16;
     std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
@@ -185,7 +210,7 @@ struct Context {
     uint64_t passes = 0, extend_launches = 0, iterations = 0;
     // streaming pass: two compact path-state buffers of max_paths slots (alternating between
     // iterations), hit records, per-path-id radiance
-    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad;
+    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, exact_q, out_rad;
     uint32_t pass_capacity = 0;      // path ids per pass (out_rad entries)
     static constexpr int kCtlRing = 8, kCtlLag = 4;   // control snapshots in flight / host look-behind
     static constexpr int kMaxSubPasses = 4;           // concurrent sub-passes (streams) of one pass
@@ -238,15 +263,17 @@ struct Context {
     crt_paths paths(int k, uint32_t slot0);
     int upload_scene(const crt_scene_desc* desc, crt_scene_id* out);
     int upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint32_t width, crt_scene_id* out);
+    int verify_scene(const DeviceScene& sc, std::string& err);
     int release_scene(crt_scene_id id);
     int release_scene_locked(crt_scene_id id);
     DeviceScene* find_scene(crt_scene_id id);
-    int run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms);
+    PassBufs pool_bufs();
+    int run_pass(std::vector<WorkItem>& items, bool time_stages, double* extend_ms, const PassBufs& B);
     int run_plain(Job& j, bool time_stages, double* extend_ms);
     int finish_job(Job& j, crt_image* out);
-    int reset_counters();
-    int read_counters(crt_dev_counters& c);
-    int render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* stats);
+    int reset_counters(crt_dev_counters* c);
+    int read_counters(crt_dev_counters& c, const crt_dev_counters* dev);
+    int render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* stats, const PassBufs& B);
     // serving
     int submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_id* out);
     void server_loop();
@@ -267,5 +294,9 @@ struct Context {
 };
 
 void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms);
+
+// Device BVH build of a one-level scene (bvh_gpu.hip): fills the scene's allocations, its device
+// record, id maps and statistics. Caller holds gpu_mutex and has checked the budget.
+int gpu_build_scene(Context& C, const crt_scene_desc* d, uint32_t leaf, uint32_t width, DeviceScene& sc, std::string& err);
 
 }  // namespace crt

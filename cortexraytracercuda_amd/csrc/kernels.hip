@@ -225,29 +225,87 @@ __device__ __forceinline__ TriPre tri_pre(V3 d) {
     return p;
 }
 
+// The watertight test evaluated in double from the same float inputs (ray direction d, origin o,
+// vertices): used when a float edge function is too close to zero for its sign to be certain, so a
+// ray near an edge is put on the same side of it as an FP64 evaluation puts it.
+__device__ __forceinline__ bool tri_hit_f64(V3 o, V3 d, float4 a, float4 b, float4 c, float tmin, float tmax, float& t, float& bu,
+                                         float& bv) {
+#pragma clang fp contract(off)
+    const double dx = d.x, dy = d.y, dz = d.z;
+    const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    auto cd = [](double x, double y, double z, int k) { return k == 0 ? x : (k == 1 ? y : z); };
+    const double dkz = cd(dx, dy, dz, kz);
+    if (dkz < 0.0) { int tk = kx; kx = ky; ky = tk; }
+    const double Sx = cd(dx, dy, dz, kx) / dkz, Sy = cd(dx, dy, dz, ky) / dkz, Sz = 1.0 / dkz;
+    const double Aa[3] = {(double)a.x - (double)o.x, (double)a.y - (double)o.y, (double)a.z - (double)o.z};
+    const double Ba[3] = {(double)b.x - (double)o.x, (double)b.y - (double)o.y, (double)b.z - (double)o.z};
+    const double Ca[3] = {(double)c.x - (double)o.x, (double)c.y - (double)o.y, (double)c.z - (double)o.z};
+    const double Ax = Aa[kx] - Sx * Aa[kz], Ay = Aa[ky] - Sy * Aa[kz];
+    const double Bx = Ba[kx] - Sx * Ba[kz], By = Ba[ky] - Sy * Ba[kz];
+    const double Cx = Ca[kx] - Sx * Ca[kz], Cy = Ca[ky] - Sy * Ca[kz];
+    const double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0.0 || V < 0.0 || W < 0.0) && (U > 0.0 || V > 0.0 || W > 0.0)) return false;
+    const double det = U + V + W;
+    if (det == 0.0) return false;
+    const double T = U * (Sz * Aa[kz]) + V * (Sz * Ba[kz]) + W * (Sz * Ca[kz]);
+    const double inv = 1.0 / det;
+    const double tt = T * inv;
+    if (!(tt >= (double)tmin && tt <= (double)tmax)) return false;
+    t = (float)tt; bu = (float)(V * inv); bv = (float)(W * inv);
+    return true;
+}
+
 // Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013), two-sided. The edge
 // functions are computed without FMA contraction so a shared edge gives exactly negated values
-// in both triangles; exact zeros are re-evaluated in double as in the paper.
-__device__ __forceinline__ bool tri_hit(V3 o, const TriPre& p, float4 a, float4 b, float4 c, float tmin, float tmax,
-                                        float& t, float& bu, float& bv) {
+// in both triangles.
+//
+// The edge functions carry a common bound E on their FP32 rounding error. With u = 2^-24, a sheared
+// coordinate such as Cx = C[kx] - Sx * C[kz] is within 4u mCx of its exact value, mCx = |C[kx]| +
+// |Sx * C[kz]| (C = vertex - origin and Sx rounded once each, the product and the difference once
+// more); cancellation makes that large against Cx itself when the vertex is far along the ray. A
+// product Cx * By is then within 4u (|Cx| mBy + mCx |By|) + 16u^2 mCx mBy of the exact one, and
+// rounding the products and their difference adds 2u (|Cx By| + |Cy Bx|), so U = Cx By - Cy Bx is
+// within 12u (aX mY + mX aY) + 32u^2 mX mY, where aX, aY are the largest |x|, |y| of the three
+// sheared vertices and mX, mY the largest magnitudes m; the same holds for V and W. E takes 16u and
+// 64u^2. An edge function larger than E has the sign of the exact value, and of a double
+// evaluation of the same expression (some 2^29 times more accurate). The FP32 answer is therefore
+// the double answer whenever every sign is certain, or when one is certainly negative and another
+// certainly positive (a miss either way). Otherwise the ray passes within rounding distance of an
+// edge or a vertex: EXACT kernels then redo the test in double (tri_hit_f64); the others flag the
+// query (`amb`) for a re-trace by an EXACT kernel and go on as if the triangle were missed.
+template <bool EXACT>
+__device__ __forceinline__ bool tri_hit(V3 o, V3 d, const TriPre& p, float4 a, float4 b, float4 c, float tmin, float tmax,
+                                        float& t, float& bu, float& bv, bool& amb) {
 #pragma clang fp contract(off)
     const int kx = (int)(p.k & 3u), ky = (int)((p.k >> 2) & 3u), kz = (int)(p.k >> 4);
     V3 A = v3(a) - o, B = v3(b) - o, C = v3(c) - o;
     float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
-    float Ax = comp(A, kx) - p.Sx * Akz, Ay = comp(A, ky) - p.Sy * Akz;
-    float Bx = comp(B, kx) - p.Sx * Bkz, By = comp(B, ky) - p.Sy * Bkz;
-    float Cx = comp(C, kx) - p.Sx * Ckz, Cy = comp(C, ky) - p.Sy * Ckz;
+    float Akx = comp(A, kx), Aky = comp(A, ky), Bkx = comp(B, kx), Bky = comp(B, ky), Ckx = comp(C, kx), Cky = comp(C, ky);
+    float pAx = p.Sx * Akz, pAy = p.Sy * Akz, pBx = p.Sx * Bkz, pBy = p.Sy * Bkz, pCx = p.Sx * Ckz, pCy = p.Sy * Ckz;
+    float Ax = Akx - pAx, Ay = Aky - pAy;
+    float Bx = Bkx - pBx, By = Bky - pBy;
+    float Cx = Ckx - pCx, Cy = Cky - pCy;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    const float mX = fmaxf(fabsf(Akx) + fabsf(pAx), fmaxf(fabsf(Bkx) + fabsf(pBx), fabsf(Ckx) + fabsf(pCx)));
+    const float mY = fmaxf(fabsf(Aky) + fabsf(pAy), fmaxf(fabsf(Bky) + fabsf(pBy), fabsf(Cky) + fabsf(pCy)));
+    const float aX = fmaxf(fabsf(Ax), fmaxf(fabsf(Bx), fabsf(Cx)));
+    const float aY = fmaxf(fabsf(Ay), fmaxf(fabsf(By), fabsf(Cy)));
+    const float E = (aX * mY + mX * aY) * 0x1p-20f + (mX * mY) * 0x1p-42f;
+    // smallest and largest edge function: a certain miss when one is below -E and another above E;
+    // every sign certain when all three are beyond E on one side
+    const float lo = fminf(U, fminf(V, W)), hi = fmaxf(U, fmaxf(V, W));
+    if (lo < -E && hi > E) return false;
+    if (!(lo > E || hi < -E)) {
+        if (EXACT) return tri_hit_f64(o, d, a, b, c, tmin, tmax, t, bu, bv);
+        amb = true;
+        return false;
     }
-    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;
-    if (det == 0.0f) return false;
     float T = U * (p.Sz * Akz) + V * (p.Sz * Bkz) + W * (p.Sz * Ckz);
     float inv = 1.0f / det;
     float tt = T * inv;
@@ -328,7 +386,37 @@ struct SceneView {
     __device__ __forceinline__ const float4* quad(uint32_t i) const {
         return (const float4*)((const char*)nodes + quad_off + i * 80u);
     }
+    __device__ __forceinline__ const float4* sphere(uint32_t i) const {
+        return (const float4*)((const char*)nodes + sph_off + i * 32u);
+    }
 };
+
+// Whether the closest hit h of ray (o, d) is an emitter seen from behind that the estimator passes
+// through (core/src/cameras/camera.cuh:179-183: emitted alpha 0 starts a re-trace from t + 0.001):
+// a one-sided area light's back face (directional_light.cuh:96-104) or a spot light's back face
+// (diffuse_spot_light.cuh:41-44, which also adds white when the light is visible). Returns 0,
+// CRT_PASSTHROUGH, or CRT_PASSTHROUGH | CRT_PASSTHROUGH_WHITE. The front-face test repeats the
+// shade stage's arithmetic, so both stages see the same face.
+__device__ __forceinline__ uint32_t passes_through(const SceneView& S, const HitRec& h, V3 o, V3 d) {
+    if (h.prim == CRT_MISS) return 0u;
+    const uint32_t type = CRT_HIT_TYPE(h.prim);
+    if (type == CRT_PRIM_QUAD) {
+        const float4* q = S.quad(CRT_HIT_INDEX(h.prim));
+        const uint32_t fl = fbits(ldg(q + 1).w);
+        return ((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(ldg(q + 3))) < 0.0f)) ? CRT_PASSTHROUGH : 0u;
+    }
+    if (type == CRT_PRIM_SPHERE) {
+        const float4* sp = S.sphere(CRT_HIT_INDEX(h.prim));
+        const uint32_t fl = fbits(ldg(sp + 1).z);
+        if (!(fl & CRT_SF_SPOT)) return 0u;
+        const float4 s = ldg(sp);
+        const V3 p = o + h.t * d;
+        const V3 outward = (p - v3(s)) / s.w;
+        if (dot(d, outward) < 0.0f) return 0u;
+        return CRT_PASSTHROUGH | ((fl & CRT_SF_INVISIBLE) ? 0u : CRT_PASSTHROUGH_WHITE);
+    }
+    return 0u;
+}
 __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
     SceneView v;
     v.nodes = (const float4*)sc->nodes;
@@ -383,7 +471,7 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t
 // move the ray to object space and descend into the instance's BLAS. The traversal is a resumable
 // state machine: step() performs one node or leaf visit, so a persistent kernel can interleave
 // traversal steps with fetching new rays for finished lanes.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL> struct Traverser {
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool EXACT> struct Traverser {
     Stack<BLOCK, RS, LS, GS> st;
     V3 o, d, id, ow, dw;
     uint32_t oct;          // bit a set where inv_d[a] < 0: the hi plane of axis a is the near one;
@@ -470,21 +558,28 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 if (COUNT) tc.tris++;
                 const bool pre = PRE && k == 0u;
                 const float4 va = pre ? a0 : ldg_at(S.nodes, to);
-                if (tri_hit(o, tp, va, pre ? a1 : ldg_at(S.nodes, to + 16u), pre ? a2 : ldg_at(S.nodes, to + 32u), tmin,
-                            best.t * CRT_TIE_HI, t, bu, bv) &&
+                bool amb = false;
+                if (tri_hit<EXACT>(o, d, tp, va, pre ? a1 : ldg_at(S.nodes, to + 16u),
+                                   pre ? a2 : ldg_at(S.nodes, to + 32u), tmin, best.t * CRT_TIE_HI, t, bu, bv, amb) &&
                     hit_wins(t, fbits(va.w), best)) {
                     best.t = t; best.u = bu; best.v = bv; best.prim = (CRT_PRIM_TRI << 30) | (first + k);
                     best.inst = cur_inst; best.key = fbits(va.w);
                 }
+                // an undecided test (not EXACT) ends the query: a negative distance culls every box
+                // and primitive left, and marks the result for the exact re-trace
+                if (!EXACT && amb) best.t = -1.0f;
             }
         } else if (type == CRT_PRIM_SPHERE) {
             for (uint32_t k = 0; k < cnt; ++k) {
                 const bool pre = PRE && k == 0u;
                 const uint32_t so = S.sph_off + (first + k) * 32u;
                 const float4 s = pre ? a0 : ldg_at(S.nodes, so);
-                const uint32_t key = fbits((pre ? a1 : ldg_at(S.nodes, so + 16u)).x);
+                const float4 s1 = pre ? a1 : ldg_at(S.nodes, so + 16u);
+                const uint32_t key = fbits(s1.x);
                 float t;
                 if (COUNT) tc.spheres++;
+                // sphere lights hide from camera rays when invisible (omni_light.cuh:62-65)
+                if ((fbits(s1.z) & CRT_SF_INVISIBLE) && camera) continue;
                 if (sphere_hit(o, d, s, tmin, best.t * CRT_TIE_HI, t) && hit_wins(t, key, best)) {
                     best.t = t; best.u = 0.0f; best.v = 0.0f; best.prim = (CRT_PRIM_SPHERE << 30) | (first + k);
                     best.inst = cur_inst; best.key = key;
@@ -598,10 +693,10 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     }
 };
 
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool EXACT>
 __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
                                       uint32_t* lds_col, GSlot gstk, HitRec& best, TravCount& tc, bool& ovf) {
-    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, EXACT> tr;
     tr.init(S, ow, dw, tmin, tmax, camera, lds_col, gstk);
     while (!tr.step(S, tc)) {
     }
@@ -719,6 +814,13 @@ __device__ __forceinline__ void camera_ray(const crt_dev_request& R, uint32_t pi
     V3 cu = v3p(R.cu), cv = v3p(R.cv), org = v3p(R.origin);
     V3 off = cu * rd.x + cv * rd.y;
     tm = R.time0 + g.uniform() * (R.time1 - R.time0);
+    if (R.ortho) {
+        // orthographic_camera::get_ray (core/src/cameras/orthographic_camera.cuh:125-142): the
+        // viewport point plus the lens offset, along -w
+        o = v3p(R.llc) + u * v3p(R.horizontal) + v * v3p(R.vertical) + off;
+        dir = v3p(R.ortho_dir);
+        return;
+    }
     o = org + off;
     dir = v3p(R.llc) + u * v3p(R.horizontal) + v * v3p(R.vertical) - org - off;
 }
@@ -769,6 +871,7 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
     if (tid == 0) {
         ctl_in[CRT_CTL_LIVE] = surv + n_new;
         ctl_in[CRT_CTL_WORK] = 0u;
+        ctl_in[CRT_CTL_EXACT] = 0u;
         ctl_out[CRT_CTL_GEN] = g0 + n_new;
         ctl_out[CRT_CTL_SURV] = 0u;
     }
@@ -802,24 +905,28 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
 // Closest hit of one path segment. One-sided emitters hit from behind are re-traced from
 // t + 0.001 (core/src/cameras/camera.cuh:179-183); the result is then flagged as a pass-through
 // (returned flag CRT_PASSTHROUGH). `nrays` counts closest-hit queries, `nretrace` the re-traces.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+// A query left undecided by a non-EXACT triangle test (its result distance set to -1) returns with
+// `amb` set and counts nothing: the caller hands it to an EXACT kernel, which answers and counts it.
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool EXACT>
 __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
                                                 GSlot gst, HitRec& h, TravCount& tc, bool& ovf, uint32_t& nrays,
-                                                uint32_t& nretrace) {
+                                                uint32_t& nretrace, bool& amb) {
     float tmin = 0.00001f;
     uint32_t pass_flag = 0u;
+    amb = false;
     for (;;) {
         bool o1 = false;
-        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc, o1);
+        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, EXACT>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc,
+                                                                 o1);
         ovf = ovf || o1;
-        if (pass_flag || h.prim == CRT_MISS || CRT_HIT_TYPE(h.prim) != CRT_PRIM_QUAD) break;
-        const float4* q = S.quad(CRT_HIT_INDEX(h.prim));
-        uint32_t fl = fbits(ldg(q + 1).w);
-        if (!((fl & CRT_QF_ONE_SIDED) && !(dot(d, v3(ldg(q + 3))) < 0.0f))) break;
+        if (!EXACT && h.t < 0.0f) { amb = true; return 0u; }
+        if (pass_flag) break;
+        const uint32_t pf = passes_through(S, h, o, d);
+        if (!pf) break;
         tmin = h.t + 0.001f;
-        pass_flag = CRT_PASSTHROUGH;
-        nretrace++;
+        pass_flag = pf;
     }
+    if (pass_flag) nretrace++;
     nrays++;
     return pass_flag;
 }
@@ -845,6 +952,7 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
                                                   float4* __restrict__ hit, uint32_t* __restrict__ hit_inst,
                                                   uint32_t* __restrict__ gstack, uint32_t* __restrict__ zero_counter,
                                                   uint32_t* __restrict__ work_counter, uint32_t chunk_groups,
+                                                  uint32_t* __restrict__ exact_q, uint32_t* __restrict__ exact_cnt,
                                                   crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
@@ -867,10 +975,14 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
         V3 o = v3(ro), d = v3(rdv);
         bool camera = (state & ST_CAMERA) != 0;
         HitRec h;
-        bool ovf = false;
-        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst, h, tc,
-                                                                                   ovf, nrays, nretrace);
+        bool ovf = false, amb = false;
+        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, COUNT>(S, o, d, camera, lds_col, gst, h,
+                                                                                          tc, ovf, nrays, nretrace, amb);
         novf += ovf ? 1u : 0u;
+        if (!COUNT && amb) {
+            exact_q[atomicAdd(exact_cnt, 1u)] = slot;   // rare: a per-lane atomic
+            return;
+        }
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
     };
@@ -930,6 +1042,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
                                                      uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
                                                      uint32_t* __restrict__ zero_counter, uint32_t refill_min,
                                                      const uint32_t* __restrict__ first_camera,
+                                                     uint32_t* __restrict__ exact_q, uint32_t* __restrict__ exact_cnt,
                                                      crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
@@ -942,7 +1055,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
     uint32_t nrays = 0, nretrace = 0, novf = 0;
     SceneView S;
     if (UNIFORM) S = scene_view(scenes + uniform_scene);
-    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL> tr;
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, COUNT> tr;
     bool active = false, exhausted = false;
     uint32_t slot = 0, pass_flag = 0;
     // Survivors of the last shade stage, queue [0, nsec): this wave's contiguous share [nxt, wend),
@@ -996,19 +1109,21 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
         }
         if (active && tr.step(S, tc)) {
             const HitRec h = tr.best;
-            bool again = false;
-            if (!pass_flag && h.prim != CRT_MISS && CRT_HIT_TYPE(h.prim) == CRT_PRIM_QUAD) {
-                // one-sided emitter seen from behind: re-trace from t + 0.001 (camera.cuh:179-183)
-                const float4* q = S.quad(CRT_HIT_INDEX(h.prim));
-                again = (fbits(ldg(q + 1).w) & CRT_QF_ONE_SIDED) && !(dot(tr.dw, v3(ldg(q + 3))) < 0.0f);
-            }
             novf += tr.st.overflow ? 1u : 0u;
+            if (!COUNT && h.t < 0.0f) {
+                // undecided triangle test: the query goes to the exact re-trace (a rare per-lane atomic)
+                exact_q[atomicAdd(exact_cnt, 1u)] = slot;
+                active = false;
+                continue;
+            }
+            // an emitter seen from behind: re-trace from t + 0.001 (camera.cuh:179-183)
+            const uint32_t again = pass_flag ? 0u : passes_through(S, h, tr.ow, tr.dw);
             if (again) {
-                nretrace++;
-                pass_flag = CRT_PASSTHROUGH;
+                pass_flag = again;
                 tr.init(S, tr.ow, tr.dw, h.t + 0.001f, 3.402823466e+38f, tr.camera, lds_col, gst);
             } else {
                 nrays++;
+                if (pass_flag) nretrace++;
                 hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
                 hit_inst[slot] = h.inst | pass_flag;
                 active = false;
@@ -1028,6 +1143,45 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
     wave_add(&counters->stack_overflow, novf);
 }
 
+// Exact re-trace of the queries the closest-hit stage left undecided (a triangle test within
+// rounding distance of an edge, queued by k_extend / k_extend_pt): the same traversal with the
+// double-precision triangle test where the float one cannot decide. Runs after the stage on its
+// stream and reuses its traversal stacks; the queue is short (usually empty).
+template <int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
+__global__ __launch_bounds__(BLOCK) void k_extend_exact(const crt_dev_scene* __restrict__ scenes,
+                                                        const crt_dev_request* __restrict__ reqs, int32_t uniform_scene,
+                                                        const uint32_t* __restrict__ exact_q,
+                                                        const uint32_t* __restrict__ exact_cnt,
+                                                        const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
+                                                        const uint2* __restrict__ ids, float4* __restrict__ hit,
+                                                        uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
+                                                        crt_dev_counters* __restrict__ counters) {
+    __shared__ uint32_t s_stack[LS * BLOCK];
+    const uint32_t n = *exact_cnt;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    if (blockIdx.x * BLOCK >= n) return;   // block-uniform: most launches find the queue empty
+    TravCount tc = {0, 0, 0, 0, 0, 0};
+    uint32_t nrays = 0, nretrace = 0, novf = 0, nexact = 0;
+    const GSlot gst{gstack, gtid * GS};
+    for (uint32_t i = gtid; i < n; i += gridDim.x * BLOCK) {
+        const uint32_t slot = exact_q[i];
+        ++nexact;
+        const float4 ro = ray_o[slot], rdv = ray_d[slot];
+        const SceneView S = scene_view(scenes + (uniform_scene >= 0 ? (uint32_t)uniform_scene : reqs[ids[slot].y >> 16].scene));
+        HitRec h;
+        bool ovf = false, amb = false;
+        const uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(
+            S, v3(ro), v3(rdv), (fbits(rdv.w) & ST_CAMERA) != 0, s_stack + threadIdx.x, gst, h, tc, ovf, nrays, nretrace, amb);
+        novf += ovf ? 1u : 0u;
+        hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
+        hit_inst[slot] = h.inst | pass_flag;
+    }
+    wave_add(&counters->rays, nrays);
+    wave_add(&counters->retraces, nretrace);
+    wave_add(&counters->stack_overflow, novf);
+    wave_add(&counters->exact, nexact);
+}
+
 // Closest hit of caller rays (first-hit parity queries).
 template <int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
 __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __restrict__ sc, const float4* __restrict__ rays,
@@ -1044,8 +1198,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __res
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         HitRec h;
         bool ovf = false;
-        trace<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
-                                        GSlot{gstack, gtid * GS}, h, tc, ovf);
+        trace<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
+                                                                GSlot{gstack, gtid * GS}, h, tc, ovf);
         novf += ovf ? 1u : 0u;
         hit[i] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[i] = h.inst;
@@ -1102,8 +1256,11 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                                           uint32_t& depth, uint32_t& bounce, Rng<MODE>& g, uint32_t& light_tests) {
     uint32_t prim = fbits(hv.y);
     bool passthrough = (iw & CRT_PASSTHROUGH) != 0;
-    uint32_t inst = iw & ~CRT_PASSTHROUGH;
+    uint32_t inst = iw & CRT_INST_MASK;
     bool done = true;
+    // a visible spot light passed through from behind emits colour(1,1,1,0) (diffuse_spot_light.cuh:
+    // 41-44), added before whatever the re-trace found is shaded (camera.cuh:179-193)
+    if (iw & CRT_PASSTHROUGH_WHITE) L = L + att * v3(1.0f, 1.0f, 1.0f);
     if (prim == CRT_MISS) {
         // miss: background (camera.cuh:161-174); a missed pass-through re-trace keeps the
         // light record, which does not scatter and adds colour(0,0,0,0)
@@ -1170,6 +1327,17 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
             if (!passthrough) {
                 bool one_sided = (m.m1.z & (int)CRT_MATF_ONE_SIDED) != 0 || is_light_quad;
                 if (!one_sided || front) L = L + att * (v3(m.m0) * m.m0.w);
+            }
+        } else if (m.m1.x == CRT_MAT_SPOT_LIGHT) {
+            // diffuse_spot_light::emitted (core/src/materials/diffuse_spot_light.cuh:38-61) on the front
+            // face (back faces were passed through by the closest-hit stage): white outside the cone,
+            // colour * intensity * cos^falloff inside; the cone axis (unit) and cos(cutoff) follow the
+            // material record, then the falloff
+            if (!passthrough) {
+                const float4 sp0 = ldg(mats + 2 * mat_id + 2), sp1 = ldg(mats + 2 * mat_id + 3);
+                const float ct = dot(unit(d), v3(sp0));
+                const V3 e = ct < sp0.w ? v3(1.0f, 1.0f, 1.0f) : v3(m.m0) * (m.m0.w * powf(ct, sp1.x));
+                L = L + att * e;
             }
         } else {
             // albedo from texture (lambertian/metal use the diffuse texture)
@@ -1248,17 +1416,38 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                 V3 bu = cross(w, bv);
                 const uint32_t nl = sc->num_lights;
                 const float4* qd = (const float4*)sc->quads;
+                const float4* sd = (const float4*)sc->spheres;
                 V3 dir;
                 if (g.uniform() < 0.5f) {
-                    // hittable_list::random -> light 0 (get_int quirk) -> directional_light::random
+                    // hittable_list::random -> light 0 (get_int quirk) -> directional_light::random or,
+                    // for a sphere light, omni_light::random (omni_light.cuh:90-98): random_to_sphere
+                    // (gpu_randomizer.cuh:82-93) in the frame built on the direction to the centre
                     if (nl == 0) {
                         dir = v3(0.0f, 0.0f, 0.0f);
                     } else {
                         if (nl > 1) g.skip();
-                        const float4* q = qd + 5 * ldg(sc->light_quads);
-                        float a = g.uniform(), b = g.uniform();
-                        V3 pp = v3(ldg(q)) + (a - 0.5f) * v3(ldg(q + 1)) + (b - 0.5f) * v3(ldg(q + 2));
-                        dir = pp - p;
+                        const uint32_t l0 = ldg(sc->lights);
+                        if (l0 & CRT_LIGHT_SPHERE) {
+                            const float4 s = ldg(sd + 2 * (l0 & ~CRT_LIGHT_SPHERE));
+                            const V3 dc = v3(s) - p;
+                            const float d2 = len2(dc);
+                            const V3 lw = unit(dc);
+                            const V3 la = fabsf(lw.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+                            const V3 lv = unit(cross(lw, la));
+                            const V3 lu = cross(lw, lv);
+                            const float r1 = g.uniform();
+                            const float r2 = g.uniform();
+                            const float z = 1.0f + r2 * (sqrtf(1.0f - s.w * s.w / d2) - 1.0f);
+                            const float phi = 6.28318548202514648438f * r1;   // 2 * M_PI (float M_PI)
+                            const float x = cosf(phi) * sqrtf(1.0f - z * z);
+                            const float y = sinf(phi) * sqrtf(1.0f - z * z);
+                            dir = x * lu + y * lv + z * lw;
+                        } else {
+                            const float4* q = qd + 5 * l0;
+                            float a = g.uniform(), b = g.uniform();
+                            V3 pp = v3(ldg(q)) + (a - 0.5f) * v3(ldg(q + 1)) + (b - 0.5f) * v3(ldg(q + 2));
+                            dir = pp - p;
+                        }
                     }
                 } else {
                     // random_cosine_direction (core/src/misc/gpu_randomizer.cuh:122-133)
@@ -1269,19 +1458,31 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                     float x = cosf(phi) * rs, y = sinf(phi) * rs;
                     dir = x * bu + y * bv + z * w;
                 }
-                // light pdf (hittable_list::pdf_value -> directional_light::pdf_value)
+                // light pdf (hittable_list::pdf_value -> directional_light::pdf_value, or for a
+                // sphere light omni_light::pdf_value, omni_light.cuh:80-88: one over the cone's
+                // solid angle 2 pi (1 - cos theta_max) when the direction meets the sphere)
                 float weight = 1.0f / (float)nl;
                 float sum = 0.0f;
                 for (uint32_t li = 0; li < nl; ++li) {
-                    const float4* q = qd + 5 * ldg(sc->light_quads + li);
-                    float tl, qa, qb;
-                    uint32_t qkey;
+                    const uint32_t lr = ldg(sc->lights + li);
+                    float tl;
                     light_tests++;
                     float pv = 0.0f;
-                    if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb, qkey)) {
-                        float dist2 = tl * tl * len2(dir);
-                        float cosine = fabsf(dot(dir, v3(ldg(q + 3))) / vlen(dir));
-                        pv = dist2 / (cosine * ldg(q + 2).w);
+                    if (lr & CRT_LIGHT_SPHERE) {
+                        const float4 s = ldg(sd + 2 * (lr & ~CRT_LIGHT_SPHERE));
+                        if (sphere_hit(p, dir, s, 0.00001f, __builtin_inff(), tl)) {
+                            const float cmax = sqrtf(1.0f - s.w * s.w / len2(v3(s) - p));
+                            pv = 1.0f / (6.28318548202514648438f * (1.0f - cmax));
+                        }
+                    } else {
+                        const float4* q = qd + 5 * lr;
+                        float qa, qb;
+                        uint32_t qkey;
+                        if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb, qkey)) {
+                            float dist2 = tl * tl * len2(dir);
+                            float cosine = fabsf(dot(dir, v3(ldg(q + 3))) / vlen(dir));
+                            pv = dist2 / (cosine * ldg(q + 2).w);
+                        }
                     }
                     sum += weight * pv;
                 }
@@ -1440,9 +1641,9 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
             bool camera = true;
             for (;;) {
                 HitRec h;
-                bool ovf = false;
-                uint32_t pf = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst, h, tc,
-                                                                                    ovf, nrays, nretrace);
+                bool ovf = false, amb = false;
+                uint32_t pf = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, o, d, camera, lds_col, gst, h,
+                                                                                          tc, ovf, nrays, nretrace, amb);
                 novf += ovf ? 1u : 0u;
                 camera = false;
                 float4 hv = make_float4(h.t, bitsf(h.prim), h.u, h.v);
@@ -1592,8 +1793,10 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                             const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                             const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                             uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
-                            const uint32_t* first_camera, crt_dev_counters* counters) {
+                            const uint32_t* first_camera, uint32_t* exact_q, uint32_t* exact_cnt,
+                            crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
+    if (!count_mode && (!exact_q || !exact_cnt)) return (int)hipErrorInvalidValue;
     if (mode != CRT_EXTEND_WAVE_CHUNK) work_counter = nullptr;
     if (mode == CRT_EXTEND_WAVE_CHUNK && !work_counter) mode = CRT_EXTEND_GRID_STRIDE;
     // grid-stride / wave-chunk kernel with an LDS stack ring of LSV entries per lane
@@ -1605,7 +1808,7 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
         uint32_t g = (mode == CRT_EXTEND_WAVE_CHUNK && grid > rb) ? rb : grid;                                          \
         hipLaunchKernelGGL(kfn, dim3(g), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr, count_direct, ray_o,     \
                            ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param ? param : 1u,         \
-                           counters);                                                                                   \
+                           exact_q, exact_cnt, counters);                                                               \
     }
     // lane-refill kernel over per-wave shares of the queue (same grid as the grid-stride kernel)
 #define CRT_EXT_PT(C, U, W, T)                                                                                         \
@@ -1613,7 +1816,7 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
         auto kfn = k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                               \
         hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr, count_direct, ray_o,  \
                            ray_d, ids, hit, hit_inst, gstack, zero_counter, param ? param : 1u, first_camera,       \
-                           counters);                                                                                   \
+                           exact_q, exact_cnt, counters);                                                               \
     }
     // counting variants: one stack depth; timed variants: LDS ring of 8, 16 or 32 entries per lane
 #define CRT_EXT_C(U, W, T) \
@@ -1630,6 +1833,17 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
         if (uniform_scene >= 0) { CRT_EXT_WT(CRT_EXT_C, true) } else { CRT_EXT_WT(CRT_EXT_C, false) }
     } else {
         if (uniform_scene >= 0) { CRT_EXT_WT(CRT_EXT_N, true) } else { CRT_EXT_WT(CRT_EXT_N, false) }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+        // the exact re-trace of the undecided queries (counting variants decide them inline); its
+        // grid is within the stage's, whose traversal stacks it reuses
+        const uint32_t xg = grid < 512u ? grid : 512u;
+#define CRT_EXT_X(W, T)                                                                                              \
+    hipLaunchKernelGGL((k_extend_exact<256, STK_RS, STK_LS, STK_GS, W, T>), dim3(xg), dim3(256), 0, s, scenes, reqs, \
+                       uniform_scene, exact_q, exact_cnt, ray_o, ray_d, ids, hit, hit_inst, gstack, counters)
+        if (width == 2) { if (two_level) CRT_EXT_X(2, true); else CRT_EXT_X(2, false); }
+        else { if (two_level) CRT_EXT_X(4, true); else CRT_EXT_X(4, false); }
+#undef CRT_EXT_X
     }
 #undef CRT_EXT_WT
 #undef CRT_EXT_N

@@ -384,17 +384,131 @@ Ref make_ref(const Box& b, uint32_t type, uint32_t idx) {
 
 bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
 
+int validate_meshes(const crt_scene_desc* d, std::string& err) {
+    const uint32_t NM = d->num_materials;
+    for (uint32_t m = 0; m < d->num_meshes; ++m) {
+        const crt_mesh& M = d->meshes[m];
+        if (M.num_triangles && (!M.indices || !M.positions)) { err = "mesh without indices/positions"; return CRT_ERR_INVALID_SCENE; }
+        if (!M.tri_materials && M.material >= NM && M.num_triangles) { err = "mesh references missing material"; return CRT_ERR_INVALID_SCENE; }
+        for (uint64_t k = 0; k < (uint64_t)M.num_triangles * 3; ++k)
+            if (M.indices[k] >= M.num_vertices) { err = "mesh index out of range"; return CRT_ERR_INVALID_SCENE; }
+        if (M.tri_materials)
+            for (uint32_t t = 0; t < M.num_triangles; ++t)
+                if (M.tri_materials[t] >= NM) { err = "mesh triangle references missing material"; return CRT_ERR_INVALID_SCENE; }
+        for (uint64_t k = 0; k < (uint64_t)M.num_vertices; ++k)
+            if (!finite3(M.positions + 3 * k)) { err = "non-finite mesh vertex"; return CRT_ERR_INVALID_SCENE; }
+    }
+    return CRT_OK;
+}
+
+// Quad / area light record (5 x float4): quad ctor (core/src/primitives/quad.cuh:66-79) /
+// directional_light ctor (core/src/lights/directional_light.cuh:45-58).
+void quad_record(V q, V u, V v, uint32_t flags, uint32_t mat, uint32_t gid, F4* out) {
+    V n = vcross(u, v);
+    float nn = vdot(n, n);
+    float len = std::sqrt(nn);
+    V un{n.x / len, n.y / len, n.z / len};
+    float D = vdot(un, q);
+    V w{n.x / nn, n.y / nn, n.z / nn};
+    out[0] = F4{q.x, q.y, q.z, D};
+    out[1] = F4{u.x, u.y, u.z, fb(flags)};
+    out[2] = F4{v.x, v.y, v.z, len};
+    out[3] = F4{un.x, un.y, un.z, fb(gid)};
+    out[4] = F4{w.x, w.y, w.z, fb(mat)};
+}
+
+// Triangle record (3 x float4: (v0, id) (v1, material) (v2, flags)) with its unit vertex normals
+// (vert_normals = unit_vector(vn), core/src/primitives/triangle.cuh:97-99) and uvs when the mesh
+// has them.
+// Sphere-shaped primitives in source order: the sphere lights (omni, then spot) and the scene's
+// spheres, with their global ids and materials, and the world light order (omni, area, spot:
+// core/src/gpu.cu:504-523) as light-table sources: CRT_LIGHT_SPHERE | sphere source index, or the
+// area light's quad source index. Area light i has global id n_omni + i; the spheres start after
+// every light. A sphere light's material is the one prepare_scene_tables appends for it.
+struct SphereSrc { float c[3]; float r; uint32_t mat, gid, flags; crt_uvmap map; };
+int gather_spheres(const crt_scene_desc* d, std::vector<SphereSrc>& sph, std::vector<uint32_t>& lights, std::string& err) {
+    const uint32_t NM = d->num_materials, NL = d->num_lights, NS = d->num_sphere_lights;
+    uint32_t n_omni = 0;
+    for (uint32_t i = 0; i < NS; ++i) n_omni += d->sphere_lights[i].kind == CRT_LIGHT_OMNI ? 1u : 0u;
+    const uint32_t total_lights = NL + NS;
+    sph.clear();
+    lights.clear();
+    const crt_uvmap unit_map{1.0f, 1.0f, 0.0f, 0.0f, 1.0f, 1.0f};
+    uint32_t mat = NM + NL, omni_gid = 0, spot_gid = n_omni + NL;
+    std::vector<uint32_t> spots;
+    for (uint32_t i = 0; i < NS; ++i) {
+        const crt_sphere_light& L = d->sphere_lights[i];
+        if (L.kind != CRT_LIGHT_OMNI && L.kind != CRT_LIGHT_SPOT) { err = "unknown sphere light kind"; return CRT_ERR_INVALID_SCENE; }
+        if (!finite3(L.center) || !std::isfinite(L.radius) || !finite3(L.direction)) {
+            err = "non-finite sphere light"; return CRT_ERR_INVALID_SCENE;
+        }
+        const bool spot = L.kind == CRT_LIGHT_SPOT;
+        SphereSrc s{{L.center[0], L.center[1], L.center[2]}, L.radius, mat, spot ? spot_gid++ : omni_gid++,
+                    (L.invisible ? CRT_SF_INVISIBLE : 0u) | (spot ? CRT_SF_SPOT : 0u), unit_map};
+        mat += spot ? 2u : 1u;
+        if (spot) spots.push_back((uint32_t)sph.size());
+        else lights.push_back(CRT_LIGHT_SPHERE | (uint32_t)sph.size());
+        sph.push_back(s);
+    }
+    for (uint32_t i = 0; i < NL; ++i) lights.push_back(i);
+    for (uint32_t k : spots) lights.push_back(CRT_LIGHT_SPHERE | k);
+    for (uint32_t i = 0; i < d->num_spheres; ++i) {
+        const crt_sphere& s = d->spheres[i];
+        if (s.material >= NM) { err = "sphere " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
+        if (!finite3(s.center) || !std::isfinite(s.radius)) { err = "non-finite sphere"; return CRT_ERR_INVALID_SCENE; }
+        sph.push_back(SphereSrc{{s.center[0], s.center[1], s.center[2]}, s.radius, s.material, total_lights + i, 0u, s.mapping});
+    }
+    return CRT_OK;
+}
+
+// Sphere record (centre, radius), (global id, material, CRT_SF_* flags, 0) as bits: the traversal
+// reads the id and flags with the same load as the sphere (tie rule, visibility), the shade stage
+// the material; its uv mapping (scale, offset, repeat).
+void sphere_record(const SphereSrc& s, F4* rec, F2* map) {
+    const uint32_t w[3] = {s.gid, s.mat, s.flags};
+    rec[0] = F4{s.c[0], s.c[1], s.c[2], s.r};
+    rec[1] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+    std::memcpy(&rec[1].x, &w[0], 4);
+    std::memcpy(&rec[1].y, &w[1], 4);
+    std::memcpy(&rec[1].z, &w[2], 4);
+    map[0] = F2{s.map.scale_u, s.map.scale_v};
+    map[1] = F2{s.map.offset_u, s.map.offset_v};
+    map[2] = F2{s.map.repeat_u, s.map.repeat_v};
+}
+
+void triangle_record(const crt_mesh& M, uint32_t t, uint32_t id_word, F4* rec, F4* nrm, F2* uv) {
+    const uint32_t* ix = M.indices + 3 * (size_t)t;
+    const float* p0 = M.positions + 3 * (size_t)ix[0];
+    const float* p1 = M.positions + 3 * (size_t)ix[1];
+    const float* p2 = M.positions + 3 * (size_t)ix[2];
+    uint32_t mat = M.tri_materials ? M.tri_materials[t] : M.material;
+    uint32_t flags = (M.normals ? CRT_TF_SMOOTH : 0u) | (M.uvs ? CRT_TF_UV : 0u);
+    rec[0] = F4{p0[0], p0[1], p0[2], fb(id_word)};
+    rec[1] = F4{p1[0], p1[1], p1[2], fb(mat)};
+    rec[2] = F4{p2[0], p2[1], p2[2], fb(flags)};
+    if (nrm) {
+        for (int k = 0; k < 3; ++k) {
+            if (!M.normals) { nrm[k] = F4{0, 0, 0, 0}; continue; }
+            V n = vload(M.normals + 3 * (size_t)ix[k]);
+            float l = std::sqrt(vdot(n, n));
+            nrm[k] = F4{n.x / l, n.y / l, n.z / l, 0.0f};
+        }
+    }
+    if (uv)
+        for (int k = 0; k < 3; ++k) uv[k] = M.uvs ? F2{M.uvs[2 * (size_t)ix[k]], M.uvs[2 * (size_t)ix[k] + 1]} : F2{0, 0};
+}
+
 }  // namespace
 
 uint64_t HostScene::device_bytes() const {
     return nodes.size() * 16 + tris.size() * 16 + tri_normals.size() * 16 + tri_uvs.size() * 8 +
            spheres.size() * 16 + sphere_map.size() * 8 + quads.size() * 16 + insts.size() * 16 +
-           mats.size() * 16 + texs.size() * 16 + texels.size() + light_quads.size() * 4;
+           mats.size() * 16 + texs.size() * 16 + texels.size() + light_refs.size() * 4;
 }
 
-int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, int threads, HostScene& S,
-                     std::string& err) {
-    auto t0 = std::chrono::steady_clock::now();
+// Checks the descriptor's pointers, textures and materials and flattens the texture and material
+// tables (shared by the host and the device BVH builds).
+int prepare_scene_tables(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, HostScene& S, std::string& err) {
     if (!d) { err = "null scene descriptor"; return CRT_ERR_INVALID_ARGUMENT; }
     if (d->abi_version != CRT_ABI_VERSION) { err = "scene descriptor ABI version mismatch"; return CRT_ERR_INVALID_ARGUMENT; }
     auto need = [&](bool ok, const char* what) { if (!ok) err = what; return ok; };
@@ -405,6 +519,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     if (!need(d->num_quads == 0 || d->quads, "quads pointer is null")) return CRT_ERR_INVALID_SCENE;
     if (!need(d->num_meshes == 0 || d->meshes, "meshes pointer is null")) return CRT_ERR_INVALID_SCENE;
     if (!need(d->num_instances == 0 || d->instances, "instances pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_sphere_lights == 0 || d->sphere_lights, "sphere lights pointer is null")) return CRT_ERR_INVALID_SCENE;
     S = HostScene();
     if (bvh_width != 2 && bvh_width != 4) { err = "bvh width must be 2 or 4"; return CRT_ERR_INVALID_ARGUMENT; }
     S.bvh_width = bvh_width;
@@ -467,23 +582,53 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
         S.mats.push_back(F4{L.rgb[0], L.rgb[1], L.rgb[2], L.intensity});
         S.mats.push_back(F4{fb((uint32_t)CRT_MAT_DIFFUSE_LIGHT), fb(0xffffffffu), fb(CRT_MATF_ONE_SIDED), 0.0f});
     }
+    // + one per omni light, two per spot light, in sphere-light order
+    for (uint32_t i = 0; i < d->num_sphere_lights; ++i) {
+        const crt_sphere_light& L = d->sphere_lights[i];
+        S.mats.push_back(F4{L.rgb[0], L.rgb[1], L.rgb[2], L.intensity});
+        if (L.kind == CRT_LIGHT_OMNI) {
+            // omni_light owns diffuse_light(rgb, intensity, directional = false, invisible)
+            S.mats.push_back(F4{fb((uint32_t)CRT_MAT_DIFFUSE_LIGHT), fb(0xffffffffu), fb(0u), 0.0f});
+        } else {
+            // spot_light owns diffuse_spot_light: the unit cone axis and cos(cutoff) (the ctor's
+            // cos(degrees_to_radians(cutoff)) in float, spot_light.cuh:40-50) and the falloff
+            S.mats.push_back(F4{fb((uint32_t)CRT_MAT_SPOT_LIGHT), fb(0xffffffffu), fb(0u), 0.0f});
+            const float* dv = L.direction;
+            const float len = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            const float cut = std::cos(L.cutoff_deg * 3.14159265358979323846f / 180.0f);
+            S.mats.push_back(F4{dv[0] / len, dv[1] / len, dv[2] / len, cut});
+            S.mats.push_back(F4{L.falloff, 0.0f, 0.0f, 0.0f});
+        }
+    }
+    return CRT_OK;
+}
+
+int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, int threads, HostScene& S,
+                     std::string& err) {
+    auto t0 = std::chrono::steady_clock::now();
+    int prc = prepare_scene_tables(d, leaf_size, bvh_width, S, err);
+    if (prc) return prc;
+    leaf_size = S.leaf_size;
+    const uint32_t NM = d->num_materials;
 
     // ---------------------------------------------------------------- primitive refs (top level)
     std::vector<Ref> refs;
     // quads: lights first (world-list order), then quads
     struct QuadSrc { V q, u, v; uint32_t flags, mat, gid; };
     std::vector<QuadSrc> qsrc;
-    uint32_t gid = 0;
-    for (uint32_t i = 0; i < d->num_lights; ++i, ++gid) {
+    std::vector<SphereSrc> ssrc;
+    std::vector<uint32_t> light_src;
+    if ((prc = gather_spheres(d, ssrc, light_src, err))) return prc;
+    uint32_t n_omni = 0;
+    for (uint32_t i = 0; i < d->num_sphere_lights; ++i) n_omni += d->sphere_lights[i].kind == CRT_LIGHT_OMNI ? 1u : 0u;
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
         const crt_area_light& L = d->lights[i];
         if (!finite3(L.center) || !finite3(L.u) || !finite3(L.v)) { err = "non-finite light geometry"; return CRT_ERR_INVALID_SCENE; }
         QuadSrc s{vload(L.center), vload(L.u), vload(L.v), CRT_QF_LIGHT | CRT_QF_ONE_SIDED | (L.invisible ? CRT_QF_INVISIBLE : 0u),
-                  NM + i, gid};
+                  NM + i, n_omni + i};
         qsrc.push_back(s);
     }
-    std::vector<uint32_t> sphere_gid;
-    uint32_t sphere_gid0 = gid;
-    gid += d->num_spheres;
+    uint32_t gid = d->num_lights + d->num_sphere_lights + d->num_spheres;
     for (uint32_t i = 0; i < d->num_quads; ++i, ++gid) {
         const crt_quad& Q = d->quads[i];
         if (Q.material >= NM) { err = "quad " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
@@ -498,28 +643,15 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
         b.grow(c0); b.grow(vadd(c0, q.u)); b.grow(vadd(c0, q.v)); b.grow(vadd(vadd(c0, q.u), q.v));
         refs.push_back(make_ref(b, CRT_PRIM_QUAD, i));
     }
-    for (uint32_t i = 0; i < d->num_spheres; ++i) {
-        const crt_sphere& s = d->spheres[i];
-        if (s.material >= NM) { err = "sphere " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
-        if (!finite3(s.center) || !std::isfinite(s.radius)) { err = "non-finite sphere"; return CRT_ERR_INVALID_SCENE; }
+    for (uint32_t i = 0; i < (uint32_t)ssrc.size(); ++i) {
+        const SphereSrc& s = ssrc[i];
         Box b;
-        float r = std::fabs(s.radius);
-        for (int a = 0; a < 3; ++a) { b.lo[a] = s.center[a] - r; b.hi[a] = s.center[a] + r; }
+        float r = std::fabs(s.r);
+        for (int a = 0; a < 3; ++a) { b.lo[a] = s.c[a] - r; b.hi[a] = s.c[a] + r; }
         refs.push_back(make_ref(b, CRT_PRIM_SPHERE, i));
     }
     // meshes
-    for (uint32_t m = 0; m < d->num_meshes; ++m) {
-        const crt_mesh& M = d->meshes[m];
-        if (M.num_triangles && (!M.indices || !M.positions)) { err = "mesh without indices/positions"; return CRT_ERR_INVALID_SCENE; }
-        if (!M.tri_materials && M.material >= NM && M.num_triangles) { err = "mesh references missing material"; return CRT_ERR_INVALID_SCENE; }
-        for (uint64_t k = 0; k < (uint64_t)M.num_triangles * 3; ++k)
-            if (M.indices[k] >= M.num_vertices) { err = "mesh index out of range"; return CRT_ERR_INVALID_SCENE; }
-        if (M.tri_materials)
-            for (uint32_t t = 0; t < M.num_triangles; ++t)
-                if (M.tri_materials[t] >= NM) { err = "mesh triangle references missing material"; return CRT_ERR_INVALID_SCENE; }
-        for (uint64_t k = 0; k < (uint64_t)M.num_vertices; ++k)
-            if (!finite3(M.positions + 3 * k)) { err = "non-finite mesh vertex"; return CRT_ERR_INVALID_SCENE; }
-    }
+    if ((prc = validate_meshes(d, err))) return prc;
     // world triangles (non-instanced): a flat list of (mesh, triangle)
     struct TriSrc { uint32_t mesh, tri, gid; };
     std::vector<TriSrc> wtris;
@@ -543,30 +675,19 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     // record emitters ----------------------------------------------------------------
     auto emit_triangle = [&](const crt_mesh& M, uint32_t t, uint32_t id_word, const float* xf) {
         (void)xf;
-        const uint32_t* ix = M.indices + 3 * (size_t)t;
-        const float* p0 = M.positions + 3 * (size_t)ix[0];
-        const float* p1 = M.positions + 3 * (size_t)ix[1];
-        const float* p2 = M.positions + 3 * (size_t)ix[2];
-        uint32_t mat = M.tri_materials ? M.tri_materials[t] : M.material;
-        uint32_t flags = (M.normals ? CRT_TF_SMOOTH : 0u) | (M.uvs ? CRT_TF_UV : 0u);
+        F4 rec[3], nrm[3];
+        F2 uv[3];
         size_t ti = S.tris.size() / 3;
-        S.tris.push_back(F4{p0[0], p0[1], p0[2], fb(id_word)});
-        S.tris.push_back(F4{p1[0], p1[1], p1[2], fb(mat)});
-        S.tris.push_back(F4{p2[0], p2[1], p2[2], fb(flags)});
-        if (M.normals || !S.tri_normals.empty()) {
+        const bool want_n = M.normals || !S.tri_normals.empty(), want_uv = M.uvs || !S.tri_uvs.empty();
+        triangle_record(M, t, id_word, rec, want_n ? nrm : nullptr, want_uv ? uv : nullptr);
+        S.tris.insert(S.tris.end(), rec, rec + 3);
+        if (want_n) {
             if (S.tri_normals.size() < 3 * ti) S.tri_normals.resize(3 * ti, F4{0, 0, 0, 0});
-            for (int k = 0; k < 3; ++k) {
-                if (!M.normals) { S.tri_normals.push_back(F4{0, 0, 0, 0}); continue; }
-                V n = vload(M.normals + 3 * (size_t)ix[k]);
-                float l = std::sqrt(vdot(n, n));
-                // vert_normals = unit_vector(vn) (core/src/primitives/triangle.cuh:97-99)
-                S.tri_normals.push_back(F4{n.x / l, n.y / l, n.z / l, 0.0f});
-            }
+            S.tri_normals.insert(S.tri_normals.end(), nrm, nrm + 3);
         }
-        if (M.uvs || !S.tri_uvs.empty()) {
+        if (want_uv) {
             if (S.tri_uvs.size() < 3 * ti) S.tri_uvs.resize(3 * ti, F2{0, 0});
-            for (int k = 0; k < 3; ++k)
-                S.tri_uvs.push_back(M.uvs ? F2{M.uvs[2 * (size_t)ix[k]], M.uvs[2 * (size_t)ix[k] + 1]} : F2{0, 0});
+            S.tri_uvs.insert(S.tri_uvs.end(), uv, uv + 3);
         }
     };
 
@@ -634,37 +755,24 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
         const TriSrc& s = wtris[i];
         emit_triangle(d->meshes[s.mesh], s.tri, s.gid, nullptr);
     };
+    std::vector<uint32_t> sphere_slot(ssrc.size(), 0);
     em.emit_sphere = [&](uint32_t i) {
-        const crt_sphere& s = d->spheres[i];
-        // record: (centre, radius) and (global id, material) as bits: the traversal reads the id
-        // with the same load as the sphere (tie rule), the shade stage the material
-        const uint32_t gid_m[2] = {sphere_gid0 + i, s.material};
-        F4 aux{0.0f, 0.0f, 0.0f, 0.0f};
-        std::memcpy(&aux.x, &gid_m[0], 4);
-        std::memcpy(&aux.y, &gid_m[1], 4);
-        S.spheres.push_back(F4{s.center[0], s.center[1], s.center[2], s.radius});
-        S.spheres.push_back(aux);
-        S.sphere_aux.push_back(U4{sphere_gid0 + i, s.material, 0u, 0u});
-        S.sphere_map.push_back(F2{s.mapping.scale_u, s.mapping.scale_v});
-        S.sphere_map.push_back(F2{s.mapping.offset_u, s.mapping.offset_v});
-        S.sphere_map.push_back(F2{s.mapping.repeat_u, s.mapping.repeat_v});
+        const SphereSrc& s = ssrc[i];
+        F4 rec[2];
+        F2 map[3];
+        sphere_record(s, rec, map);
+        sphere_slot[i] = (uint32_t)(S.spheres.size() / 2);
+        S.spheres.insert(S.spheres.end(), rec, rec + 2);
+        S.sphere_aux.push_back(U4{s.gid, s.mat, s.flags, 0u});
+        S.sphere_map.insert(S.sphere_map.end(), map, map + 3);
     };
     std::vector<uint32_t> quad_slot(qsrc.size(), 0);
     em.emit_quad = [&](uint32_t i) {
         const QuadSrc& q = qsrc[i];
-        // quad ctor (core/src/primitives/quad.cuh:66-79) / directional_light ctor (:45-58)
-        V n = vcross(q.u, q.v);
-        float nn = vdot(n, n);
-        float len = std::sqrt(nn);
-        V un{n.x / len, n.y / len, n.z / len};
-        float D = vdot(un, q.q);
-        V w{n.x / nn, n.y / nn, n.z / nn};
+        F4 rec[5];
+        quad_record(q.q, q.u, q.v, q.flags, q.mat, q.gid, rec);
         quad_slot[i] = (uint32_t)(S.quads.size() / 5);
-        S.quads.push_back(F4{q.q.x, q.q.y, q.q.z, D});
-        S.quads.push_back(F4{q.u.x, q.u.y, q.u.z, fb(q.flags)});
-        S.quads.push_back(F4{q.v.x, q.v.y, q.v.z, len});
-        S.quads.push_back(F4{un.x, un.y, un.z, fb(q.gid)});
-        S.quads.push_back(F4{w.x, w.y, w.z, fb(q.mat)});
+        S.quads.insert(S.quads.end(), rec, rec + 5);
     };
     em.emit_inst = [&](uint32_t k) {
         const crt_instance& I = d->instances[k];
@@ -695,11 +803,15 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     S.stats.sah_cost = em.sah;
     S.stats.depth = em.max_depth + blas_depth;
     // per-instance material overrides are resolved in the shade stage (instance record word z)
-    S.light_quads.resize(d->num_lights);
-    for (uint32_t i = 0; i < d->num_lights; ++i) S.light_quads[i] = quad_slot[i];
-    S.num_lights = d->num_lights;
-    S.num_spheres = d->num_spheres;
+    S.light_refs.resize(light_src.size());
+    for (size_t i = 0; i < light_src.size(); ++i) {
+        const uint32_t src = light_src[i];
+        S.light_refs[i] = (src & CRT_LIGHT_SPHERE) ? (CRT_LIGHT_SPHERE | sphere_slot[src & ~CRT_LIGHT_SPHERE]) : quad_slot[src];
+    }
+    S.num_lights = (uint32_t)light_src.size();
+    S.num_spheres = (uint32_t)ssrc.size();
     S.num_quads = d->num_quads;
+    S.num_quad_records = (uint32_t)(S.quads.size() / 5);
     if (!S.tri_normals.empty() && S.tri_normals.size() < S.tris.size()) S.tri_normals.resize(S.tris.size(), F4{0, 0, 0, 0});
     if (!S.tri_uvs.empty() && S.tri_uvs.size() < S.tris.size()) S.tri_uvs.resize(S.tris.size(), F2{0, 0});
     S.stats.nodes = S.nodes.size() / (bvh_width == 4 ? 8 : 4);
@@ -711,6 +823,76 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
 }  // namespace crt
 
 namespace crt {
+
+int flatten_sources(const crt_scene_desc* d, int threads, SourceRecords& out, std::string& err) {
+    const uint32_t NM = d->num_materials;
+    if (d->num_instances) { err = "the device build takes one-level scenes"; return CRT_ERR_INVALID_ARGUMENT; }
+    int rc = validate_meshes(d, err);
+    if (rc) return rc;
+    out = SourceRecords();
+    std::vector<SphereSrc> ssrc;
+    if ((rc = gather_spheres(d, ssrc, out.light_src, err))) return rc;
+    uint32_t n_omni = 0;
+    for (uint32_t i = 0; i < d->num_sphere_lights; ++i) n_omni += d->sphere_lights[i].kind == CRT_LIGHT_OMNI ? 1u : 0u;
+    out.num_lights = (uint32_t)out.light_src.size();
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
+        const crt_area_light& L = d->lights[i];
+        if (!finite3(L.center) || !finite3(L.u) || !finite3(L.v)) { err = "non-finite light geometry"; return CRT_ERR_INVALID_SCENE; }
+        F4 rec[5];
+        quad_record(vload(L.center), vload(L.u), vload(L.v),
+                    CRT_QF_LIGHT | CRT_QF_ONE_SIDED | (L.invisible ? CRT_QF_INVISIBLE : 0u), NM + i, n_omni + i, rec);
+        out.quads.insert(out.quads.end(), rec, rec + 5);
+        out.quad_gid.push_back(n_omni + i);
+    }
+    uint32_t gid = d->num_lights + d->num_sphere_lights + d->num_spheres;
+    for (uint32_t i = 0; i < d->num_quads; ++i, ++gid) {
+        const crt_quad& Q = d->quads[i];
+        if (Q.material >= NM) { err = "quad " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
+        if (!finite3(Q.corner) || !finite3(Q.u) || !finite3(Q.v)) { err = "non-finite quad geometry"; return CRT_ERR_INVALID_SCENE; }
+        F4 rec[5];
+        quad_record(vload(Q.corner), vload(Q.u), vload(Q.v), 0u, Q.material, gid, rec);
+        out.quads.insert(out.quads.end(), rec, rec + 5);
+        out.quad_gid.push_back(gid);
+    }
+    for (const SphereSrc& s : ssrc) {
+        F4 rec[2];
+        F2 map[3];
+        sphere_record(s, rec, map);
+        out.spheres.insert(out.spheres.end(), rec, rec + 2);
+        out.sphere_aux.push_back(U4{s.gid, s.mat, s.flags, 0u});
+        out.sphere_map.insert(out.sphere_map.end(), map, map + 3);
+    }
+    // triangles of every mesh in order (global ids consecutive from tri_gid0), flattened by threads
+    out.tri_gid0 = gid;
+    uint64_t ntri = 0;
+    bool any_n = false, any_uv = false;
+    std::vector<uint64_t> base(d->num_meshes + 1, 0);
+    for (uint32_t m = 0; m < d->num_meshes; ++m) {
+        base[m] = ntri;
+        ntri += d->meshes[m].num_triangles;
+        any_n = any_n || d->meshes[m].normals;
+        any_uv = any_uv || d->meshes[m].uvs;
+    }
+    base[d->num_meshes] = ntri;
+    out.tris.resize(3 * ntri);
+    if (any_n) out.tri_normals.resize(3 * ntri);
+    if (any_uv) out.tri_uvs.resize(3 * ntri);
+    const int nt = std::max(1, std::min(threads, (int)((ntri + 65535) / 65536)));
+    auto work = [&](int w) {
+        const uint64_t lo = ntri * (uint64_t)w / (uint64_t)nt, hi = ntri * (uint64_t)(w + 1) / (uint64_t)nt;
+        uint32_t m = (uint32_t)(std::upper_bound(base.begin(), base.end(), lo) - base.begin()) - 1;
+        for (uint64_t i = lo; i < hi; ++i) {
+            while (i >= base[m + 1]) ++m;
+            triangle_record(d->meshes[m], (uint32_t)(i - base[m]), out.tri_gid0 + (uint32_t)i, &out.tris[3 * i],
+                            any_n ? &out.tri_normals[3 * i] : nullptr, any_uv ? &out.tri_uvs[3 * i] : nullptr);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nt; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto& t : th) t.join();
+    return CRT_OK;
+}
 
 // Structural invariants of a built scene: every child box contains the primitives below it, every
 // primitive record belongs to exactly one leaf, every instance's BLAS root is an interior node.

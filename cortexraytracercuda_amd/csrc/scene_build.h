@@ -39,14 +39,21 @@ struct HostScene {
     std::vector<F4> mats;         // 2 per material (scene materials, then one per area light)
     std::vector<F4> texs;         // 2 per texture
     std::vector<uint8_t> texels;
-    std::vector<uint32_t> light_quads;
+    std::vector<uint32_t> light_refs;   // per light in world order: quad slot, or CRT_LIGHT_SPHERE | sphere slot
     uint32_t root = 0;
     uint32_t bvh_width = 4;       // 2: 64-B binary nodes, 4: 128-B four-wide SoA nodes
     uint32_t leaf_size = 1;       // max primitives per leaf the tree was built with
     float background[3] = {0, 0, 0};
     crt_camera camera{};
-    uint32_t num_lights = 0, num_spheres = 0, num_quads = 0, num_instances = 0;
+    uint32_t num_lights = 0;        // omni + area + spot lights
+    uint32_t num_spheres = 0;       // sphere records (sphere lights and spheres)
+    uint32_t num_quads = 0;         // the scene's quads (not its area lights)
+    uint32_t num_quad_records = 0;  // quad records (area lights and quads)
+    uint32_t num_instances = 0;
     uint64_t num_world_tris = 0;
+    uint64_t num_node_records = 0;  // BVH nodes in the device allocation (BLAS nodes included)
+    uint64_t num_tri_records = 0;   // triangle records in the device allocation (BLAS triangles included)
+    bool device_built = false;    // BVH and records built on the device (no host copies of them)
     BuildStats stats;
     uint64_t device_bytes() const;
     // Nodes, triangle, instance, sphere and quad records share one device allocation (in that
@@ -63,10 +70,35 @@ struct HostScene {
 // Largest geometry allocation of one scene (32-bit byte offsets in the traversal).
 constexpr uint64_t kMaxGeometryBytes = (1ull << 32) - 256u;
 
+// Checks the descriptor and fills the texture and material tables, camera, background, BVH width and
+// leaf size of S (the part of the flattening shared by the host and the device builds).
+int prepare_scene_tables(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, HostScene& S, std::string& err);
+
+// Primitive records in source order (area lights then quads; sphere lights then spheres; mesh
+// triangles) for the device BVH build of a one-level scene; the same record layouts as HostScene,
+// before they are put in leaf order.
+struct SourceRecords {
+    std::vector<F4> quads;        // 5 per quad / area light, lights first
+    std::vector<uint32_t> quad_gid;
+    uint32_t num_lights = 0;      // omni + area + spot lights
+    std::vector<uint32_t> light_src;   // world light order: quad source index, or CRT_LIGHT_SPHERE | sphere source index
+    std::vector<F4> spheres;      // 2 per sphere (sphere lights first)
+    std::vector<F2> sphere_map;   // 3 per sphere
+    std::vector<U4> sphere_aux;   // (global id, material) per sphere
+    std::vector<F4> tris;         // 3 per triangle
+    std::vector<F4> tri_normals;  // 3 per triangle when a mesh has normals
+    std::vector<F2> tri_uvs;      // 3 per triangle when a mesh has uvs
+    uint32_t tri_gid0 = 0;        // global id of the first triangle (consecutive after it)
+};
+int flatten_sources(const crt_scene_desc* d, int threads, SourceRecords& out, std::string& err);
+
 // Flattens and builds. leaf_size = max primitives per leaf (instances always get their own leaf);
 // bvh_width = 2 or 4 children per node (the binary SAH tree is collapsed for 4).
 int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_width, int threads, HostScene& out,
                      std::string& err);
+
+// Image texture file (PNG or JPEG; image_io.cpp): rows top-down, C = 3 (RGB8) or 4 (RGBA8).
+bool read_image(const std::string& path, std::vector<uint8_t>& out, uint32_t& W, uint32_t& H, uint32_t& C, std::string& err);
 
 // Structural BVH invariants (containment, exact leaf coverage, reachability).
 int check_host_scene(const HostScene& S, std::string& err);

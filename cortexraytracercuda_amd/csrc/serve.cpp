@@ -114,6 +114,7 @@ int Context::server_init_lane(Server& S, uint32_t slots, uint32_t cap, bool firs
     }
     if ((rc = S.hit.alloc(n * 16, this))) return rc;
     if ((rc = S.hit_inst.alloc(n * 4, this))) return rc;
+    if ((rc = S.exact_q.alloc(n * 4, this))) return rc;
     if ((rc = S.out_rad.alloc((size_t)S.cap * 16, this))) return rc;
     if ((rc = S.req.alloc(sizeof(crt_dev_request) * Server::kEntries, this))) return rc;
     if ((rc = S.ctl.alloc(sizeof(uint32_t) * (2 * CRT_CTL_WORDS + 2), this))) return rc;
@@ -142,7 +143,7 @@ uint64_t Context::server_bytes() const {
 }
 
 uint64_t Context::server_lane_bytes(const Server& S) const {
-    uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.gstack.bytes +
+    uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.exact_q.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.gstack.bytes +
                  S.counters.bytes + S.live_min.bytes;
     for (int k = 0; k < 2; ++k) b += S.ray_o[k].bytes + S.ray_d[k].bytes + S.thr[k].bytes + S.rad[k].bytes + S.ids[k].bytes;
     return b;
@@ -293,7 +294,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
     if (crt_k_extend(S.s, 0, (int)C.bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
                      A.ray_d, A.ids, (float4*)S.hit.p, (uint32_t*)S.hit_inst.p, (uint32_t*)S.gstack.p, nullptr,
                      C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, cin + CRT_CTL_SURV,
-                     (crt_dev_counters*)S.counters.p))
+                     (uint32_t*)S.exact_q.p, cin + CRT_CTL_EXACT, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
     if (crt_k_shade(S.s, mode, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.live_min.p,
@@ -302,7 +303,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
     uint32_t* snap = S.snap + (size_t)(k % Server::kRing) * Server::kSnapWords;
     HIP_TRY(hipMemcpyAsync(snap, cout, sizeof(uint32_t) * CRT_CTL_WORDS, hipMemcpyDeviceToHost, S.s));
     HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS, S.counters.p, sizeof(crt_dev_counters), hipMemcpyDeviceToHost, S.s));
-    HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS + 20, S.live_min.p, sizeof(uint32_t) * S.sgrid, hipMemcpyDeviceToHost, S.s));
+    HIP_TRY(hipMemcpyAsync(snap + CRT_CTL_WORDS + Server::kCounterWords, S.live_min.p, sizeof(uint32_t) * S.sgrid, hipMemcpyDeviceToHost, S.s));
     return CRT_OK;
 }
 
@@ -352,7 +353,7 @@ int Context::server_process_snapshot(Server& S, std::vector<ServeReq*>& act) {
     // written its radiance. The shade stage leaves the smallest surviving id per workgroup; an
     // entry is done once GEN and that minimum have both passed its last id.
     const uint32_t gen = snap[CRT_CTL_GEN];
-    const uint32_t* lm = snap + CRT_CTL_WORDS + 20;
+    const uint32_t* lm = snap + CRT_CTL_WORDS + Server::kCounterWords;
     uint32_t live = 0xffffffffu;
     for (uint32_t b = 0; b < S.sgrid; ++b) live = std::min(live, lm[b]);
     const uint32_t finished_below = std::min(gen, live);

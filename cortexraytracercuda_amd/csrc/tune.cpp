@@ -39,7 +39,7 @@ int Context::tune_trial(crt_scene_id scene, const crt_render_desc& rd, double& m
     if (rc) return rc;
     std::vector<Job*> jobs{&j};
     crt_stats st{};
-    if ((rc = render_jobs(jobs, false, &st))) return rc;
+    if ((rc = render_jobs(jobs, false, &st, pool_bufs()))) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
     mrays_s = st.gpu_ms > 0.0 ? (double)st.rays / (st.gpu_ms * 1e-3) / 1e6 : 0.0;
     return CRT_OK;

@@ -41,7 +41,8 @@ class Renderer:
 
     def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0,
                  extend_kernel=0, refill_min=0, pass_paths=0, grid_per_cu=0, shade_grid_per_cu=0, sub_passes=0,
-                 graphs=0, stack_lds=0, ray_sort=0):
+                 graphs=0, stack_lds=0, ray_sort=0, bvh_builder="host"):
+        """bvh_builder: "host" (binned SAH on the CPU) or "device" (linear BVH built on the GPU)."""
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -59,6 +60,10 @@ class Renderer:
         cfg.graphs = int(graphs)
         cfg.stack_lds = int(stack_lds)
         cfg.ray_sort = int(ray_sort)
+        builders = {"host": A.CRT_BVH_HOST, "device": A.CRT_BVH_DEVICE}
+        if bvh_builder not in builders:
+            raise ValueError("bvh_builder must be 'host' or 'device'")
+        cfg.bvh_builder = builders[bvh_builder]
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -103,7 +108,13 @@ class Renderer:
     def info(self, sid):
         inf = A.crt_scene_info()
         A.check(self.lib.crt_scene_get_info(self.h, sid, C.byref(inf)))
-        return {k: getattr(inf, k) for k, _ in A.crt_scene_info._fields_}
+        d = {k: getattr(inf, k) for k, _ in A.crt_scene_info._fields_}
+        d["builder"] = {A.CRT_BVH_HOST: "host", A.CRT_BVH_DEVICE: "device"}.get(d["builder"], d["builder"])
+        return d
+
+    def verify(self, sid):
+        """Reads the scene's BVH back from the device and checks its invariants (raises CrtError)."""
+        A.check(self.lib.crt_scene_verify(self.h, sid))
 
     # ---------------------------------------------------------------- rendering
     def render(self, sid, width, height, spp, max_depth, seed=1, rng="pcg", sample_begin=0, sample_count=0,
@@ -168,15 +179,18 @@ class Renderer:
         A.check(self.lib.crt_render_batch(self.h, items, n, C.byref(st)))
         return outs, stats_dict(st)
 
-    def trace_rays(self, sid, rays, camera=False):
+    def trace_rays(self, sid, rays, camera=False, stage=False):
+        """Closest hits of n x 8 rays (origin, tmin, direction, tmax). stage=True answers them with the
+        render's closest-hit stage (estimator interval, pass-through of one-sided lights seen from
+        behind) instead of the query kernel."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         n = rays.shape[0]
         prim = np.zeros(n, np.int32)
         inst = np.zeros(n, np.int32)
         t = np.zeros(n, np.float32)
         uv = np.zeros((n, 2), np.float32)
-        A.check(self.lib.crt_trace_rays(self.h, sid, _fp(rays), n, 1 if camera else 0, _ip(prim), _ip(inst), _fp(t),
-                                        _fp(uv)))
+        flags = (A.CRT_TRACE_CAMERA if camera else 0) | (A.CRT_TRACE_STAGE if stage else 0)
+        A.check(self.lib.crt_trace_rays(self.h, sid, _fp(rays), n, flags, _ip(prim), _ip(inst), _fp(t), _fp(uv)))
         return prim, inst, t, uv
 
     def trace_primary(self, sid, width, height):

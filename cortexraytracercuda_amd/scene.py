@@ -27,6 +27,7 @@ class Scene:
         self.textures = []
         self.texture_names = {}
         self.lights = []
+        self.sphere_lights = []  # omni and spot lights
         self.spheres = []
         self.quads = []
         self.meshes = []         # dicts of numpy arrays
@@ -34,10 +35,13 @@ class Scene:
 
     # ---------------------------------------------------------------- camera / image
     def set_camera(self, lookfrom, lookat, vup=(0, 1, 0), vfov=70.0, aspect_ratio=1.0, aperture=0.0, focus_dist=10.0,
-                   time0=0.0, time1=1.0):
+                   time0=0.0, time1=1.0, orthographic=False, ortho_height=1.0):
+        """Perspective camera, or with orthographic=True parallel rays from a viewport of height
+        2 * ortho_height centred at lookfrom (vfov and focus_dist unused)."""
         self.camera = dict(lookfrom=tuple(lookfrom), lookat=tuple(lookat), vup=tuple(vup), vfov=float(vfov),
                            aspect_ratio=float(aspect_ratio), aperture=float(aperture), focus_dist=float(focus_dist),
-                           time0=float(time0), time1=float(time1))
+                           time0=float(time0), time1=float(time1), orthographic=bool(orthographic),
+                           ortho_height=float(ortho_height))
         return self
 
     # ---------------------------------------------------------------- textures
@@ -109,6 +113,20 @@ class Scene:
                                 invisible=bool(invisible)))
         return len(self.lights) - 1
 
+    def omni_light(self, center, radius, rgb=(1, 1, 1), intensity=1.0, invisible=True):
+        """Spherical light emitting rgb * intensity from both faces (omni_light.cuh)."""
+        self.sphere_lights.append(dict(kind=A.CRT_LIGHT_OMNI, center=tuple(center), radius=float(radius), rgb=tuple(rgb),
+                                       intensity=float(intensity), invisible=bool(invisible), direction=(0.0, 0.0, -1.0),
+                                       cutoff_deg=0.0, falloff=0.0))
+        return len(self.sphere_lights) - 1
+
+    def spot_light(self, center, direction, cutoff_deg, falloff, radius, rgb=(1, 1, 1), intensity=1.0, invisible=True):
+        """Spherical spot light (spot_light.cuh, diffuse_spot_light.cuh)."""
+        self.sphere_lights.append(dict(kind=A.CRT_LIGHT_SPOT, center=tuple(center), radius=float(radius), rgb=tuple(rgb),
+                                       intensity=float(intensity), invisible=bool(invisible), direction=tuple(direction),
+                                       cutoff_deg=float(cutoff_deg), falloff=float(falloff)))
+        return len(self.sphere_lights) - 1
+
     def sphere(self, center, radius, material, mapping=(1, 1, 0, 0, 1, 1)):
         self.spheres.append(dict(center=tuple(center), radius=float(radius), material=self._mat_index(material),
                                  mapping=tuple(mapping)))
@@ -167,7 +185,8 @@ class Scene:
 
     @property
     def num_world_prims(self):
-        return len(self.lights) + len(self.spheres) + len(self.quads) + sum(int(m["indices"].shape[0]) for m in self.meshes)
+        return (len(self.lights) + len(self.sphere_lights) + len(self.spheres) + len(self.quads) +
+                sum(int(m["indices"].shape[0]) for m in self.meshes))
 
     # ---------------------------------------------------------------- C descriptor
     def to_desc(self):
@@ -187,7 +206,8 @@ class Scene:
         d.abi_version = A.CRT_ABI_VERSION
         c = self.camera
         d.camera = A.crt_camera(_f3(c["lookfrom"]), _f3(c["lookat"]), _f3(c["vup"]), c["vfov"], c["aspect_ratio"],
-                                c["aperture"], c["focus_dist"], c["time0"], c["time1"])
+                                c["aperture"], c["focus_dist"], c["time0"], c["time1"], 1 if c.get("orthographic") else 0,
+                                c.get("ortho_height", 1.0))
         d.background = _f3(self.background)
         mats = []
         for m in self.materials:
@@ -251,5 +271,9 @@ class Scene:
             ci.material_override = i["material_override"]
             insts.append(ci)
         d.instances, d.num_instances = arr(A.crt_instance, insts)
+        d.sphere_lights, d.num_sphere_lights = arr(A.crt_sphere_light, [
+            A.crt_sphere_light(l["kind"], _f3(l["center"]), l["radius"], _f3(l["rgb"]), l["intensity"],
+                               1 if l["invisible"] else 0, _f3(l["direction"]), l["cutoff_deg"], l["falloff"])
+            for l in self.sphere_lights])
         keep.append(d)
         return d, keep

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CRT_ABI_VERSION 1u
+#define CRT_ABI_VERSION 2u
 
 /* ------------------------------------------------------------------ status codes */
 typedef enum crt_status {
@@ -50,17 +50,24 @@ typedef enum crt_status {
 
 /* ------------------------------------------------------------------ scene description */
 
-/* Perspective camera, core/src/cameras/perspective_camera.cuh:41-119 (initialize + get_ray).
- * The aspect ratio is the camera's own (reference: -ratio CLI flag), not width/height. */
+/* Perspective camera, core/src/cameras/perspective_camera.cuh:41-119 (initialize + get_ray), or
+ * orthographic camera, core/src/cameras/orthographic_camera.cuh:57-142 (cameraCfg.isOrthographic,
+ * core/src/gpu.cu:696-705). The aspect ratio is the camera's own (reference: -ratio CLI flag), not
+ * width/height. */
 typedef struct crt_camera {
     float lookfrom[3];
     float lookat[3];
     float vup[3];
-    float vfov_deg;
+    float vfov_deg;      /* perspective only */
     float aspect_ratio;
-    float aperture;      /* lens radius = aperture / 2 */
-    float focus_dist;    /* primary ray directions have length ~focus_dist (not normalized) */
+    float aperture;      /* lens radius = aperture / 2 (the orthographic camera offsets its ray origins by
+                            the lens sample too) */
+    float focus_dist;    /* perspective: primary ray directions have length ~focus_dist (not normalized) */
     float time0, time1;  /* ray time = time0 + U*(time1-time0); the reference passes 0 and 1 */
+    uint32_t orthographic;  /* 1: parallel rays along lookat - lookfrom (unit length) from a viewport of
+                               height 2 * ortho_height and width 2 * ortho_height * W / int(W / aspect) at
+                               lookfrom, W = the render width */
+    float ortho_height;
 } crt_camera;
 
 /* Material type codes follow MaterialTypeID (core/src/materials/material.cuh:8-19). */
@@ -68,7 +75,9 @@ enum {
     CRT_MAT_LAMBERTIAN = 1,      /* core/src/materials/lambertian.cuh:63-143 */
     CRT_MAT_METAL = 2,           /* core/src/materials/metal.cuh:52-65 */
     CRT_MAT_DIELECTRIC = 3,      /* core/src/materials/dielectric.cuh:32-63 */
-    CRT_MAT_DIFFUSE_LIGHT = 8    /* core/src/materials/diffuse_light.cuh:45-67 */
+    CRT_MAT_DIFFUSE_LIGHT = 8,   /* core/src/materials/diffuse_light.cuh:45-67 */
+    CRT_MAT_SPOT_LIGHT = 9       /* diffuse_spot_light, core/src/materials/diffuse_spot_light.cuh:33-61: the
+                                    material of a spot light (crt_sphere_light), not for crt_material */
 };
 /* crt_material.flags */
 #define CRT_MATF_ONE_SIDED 1u    /* diffuse_light "directional": back faces emit colour(0,0,0,0) */
@@ -130,6 +139,27 @@ typedef struct crt_area_light {
     uint32_t invisible;
 } crt_area_light;
 
+/* Spherical light (core/src/lights/omni_light.cuh:18-93, spot_light.cuh:21-117): a sphere hit like
+ * sphere::hit, hidden from camera rays when invisible, sampled by the lights' mixture pdf with
+ * random_to_sphere over its cone of directions (pdf 1 / (2 pi (1 - cos theta_max))).
+ * OMNI: diffuse_light(rgb, intensity, non-directional): rgb * intensity from both faces.
+ * SPOT: diffuse_spot_light: a ray meeting the front face along unit direction D emits
+ * rgb * intensity * cos^falloff where cos = D . unit(direction) >= cos(cutoff_deg), and white (1,1,1)
+ * where cos is below it; a ray from inside (back face) is passed through like a one-sided light's
+ * back face (camera.cuh:179-183), adding white when the light is visible and nothing when invisible. */
+enum { CRT_LIGHT_OMNI = 0, CRT_LIGHT_SPOT = 1 };
+typedef struct crt_sphere_light {
+    uint32_t kind;         /* CRT_LIGHT_OMNI or CRT_LIGHT_SPOT */
+    float center[3];
+    float radius;
+    float rgb[3];
+    float intensity;
+    uint32_t invisible;
+    float direction[3];    /* SPOT: cone axis (any length) */
+    float cutoff_deg;      /* SPOT: cone half-angle in degrees */
+    float falloff;         /* SPOT: exponent of the cosine */
+} crt_sphere_light;
+
 /* Indexed triangle mesh (Moller-Trumbore in the reference, core/src/primitives/triangle.cuh:130-185;
  * watertight here). normals/uvs are optional per-vertex arrays (NULL = flat normal / zero uv). */
 typedef struct crt_mesh {
@@ -164,11 +194,16 @@ typedef struct crt_scene_desc {
     const crt_quad* quads;         uint32_t num_quads;
     const crt_mesh* meshes;        uint32_t num_meshes;
     const crt_instance* instances; uint32_t num_instances;
+    const crt_sphere_light* sphere_lights; uint32_t num_sphere_lights;
 } crt_scene_desc;
 
 /* Global primitive numbering used by first-hit queries (mirrors the reference's world-list order,
  * lights first, then spheres, quads, mesh triangles: core/src/gpu.cu:504-640):
  *   lights [0, L), spheres [L, L+S), quads [L+S, L+S+Q), triangles [L+S+Q, ...) in mesh order.
+ * The lights are the OMNI sphere lights, then the area lights, then the SPOT sphere lights (each
+ * in array order), as load_scene adds omni, directional and spot lights (core/src/gpu.cu:504-523);
+ * the light sampled by the estimator's mixture pdf is the first of them (hittable_list::random's
+ * get_int always returns its lower bound, core/src/misc/gpu_randomizer.cuh:29-36).
  * For instanced scenes the triangle id is the mesh-local id and the instance is reported apart. */
 
 /* ------------------------------------------------------------------ rendering */
@@ -254,7 +289,16 @@ typedef struct crt_config {
                                    8, 16 or 32; 0 = 16 */
     uint32_t ray_sort;          /* 1: the shade stage groups each workgroup's surviving paths by direction
                                    octant (coherent waves in the next closest-hit launch); 0 / 2: off */
+    uint32_t bvh_builder;       /* CRT_BVH_*: where crt_scene_upload builds the BVH; 0 = host */
 } crt_config;
+
+/* crt_config.bvh_builder. HOST: binned-SAH tree on the CPU threads, then one upload. DEVICE: linear
+ * BVH on the GPU (Morton codes, radix sort, radix tree, bottom-up boxes, collapse to the BVH width)
+ * from the uploaded primitives: a build in milliseconds for scenes that arrive with a request, at a
+ * higher traversal cost than the SAH tree. One-level scenes of two primitives or more; instanced
+ * and smaller scenes are built on the host either way. Closest hits, and so images, do not depend
+ * on the tree (deterministic tie rule). */
+enum { CRT_BVH_HOST = 1, CRT_BVH_DEVICE = 2 };
 
 enum { CRT_EXTEND_DEFAULT = 0, CRT_EXTEND_GRID_STRIDE = 1, CRT_EXTEND_PERSISTENT = 2, CRT_EXTEND_WAVE_CHUNK = 3 };
 
@@ -273,10 +317,15 @@ typedef struct crt_scene_info {
     uint64_t num_nodes, num_leaves, num_prims, device_bytes;
     uint32_t max_depth, num_instances;
     uint32_t bvh_width, node_bytes;          /* children per node and bytes per node record */
-    uint32_t leaf_size, reserved;            /* max primitives per leaf the BVH was built with; - */
+    uint32_t leaf_size, builder;             /* max primitives per leaf the BVH was built with; CRT_BVH_* */
     double build_ms, upload_ms, sah_cost;
 } crt_scene_info;
 int crt_scene_get_info(crt_context* ctx, crt_scene_id scene, crt_scene_info* info);
+/* Reads an uploaded scene's BVH and primitive records back from the device and checks the same
+ * invariants as crt_scene_build_check (containment, every primitive record in exactly one leaf,
+ * every node reachable) plus that the leaf-order records map one to one onto the scene's global
+ * primitive ids. CRT_OK, or CRT_ERR_INTERNAL naming the violated invariant. Blocking. */
+int crt_scene_verify(crt_context* ctx, crt_scene_id scene);
 /* Host-only: flattens and builds a scene without a device, checks the BVH invariants
  * (containment, every primitive in exactly one leaf, reachability) and reports its statistics.
  * Same validation errors as crt_scene_upload (CRT_ERR_INVALID_SCENE for dangling indices). */
@@ -288,10 +337,15 @@ int crt_render(crt_context* ctx, crt_scene_id scene, const crt_render_desc* rd, 
 
 /* Closest-hit queries for arbitrary rays (first-hit parity). rays: n x 8 floats
  * (ox,oy,oz,tmin, dx,dy,dz,tmax). Outputs per ray: global primitive id (-1 = miss), instance
- * (-1 = none), t, and the barycentric / planar coordinates u, v. camera_rays != 0 hides invisible
- * lights as the reference does for depth == max_depth. */
+ * (-1 = none), t, and the barycentric / planar coordinates u, v. `flags` (CRT_TRACE_*):
+ * CRT_TRACE_CAMERA hides invisible lights as the reference does for depth == max_depth;
+ * CRT_TRACE_STAGE answers the queries with the render's closest-hit stage (the configured
+ * extend_kernel and its exact re-trace of undecided triangle tests) instead of the query kernel:
+ * the interval is then the estimator's [1e-5, FLT_MAX) whatever tmin / tmax say, and a one-sided
+ * light hit from behind is passed through as in the render (core/src/cameras/camera.cuh:179-183). */
+enum { CRT_TRACE_CAMERA = 1, CRT_TRACE_STAGE = 2 };
 int crt_trace_rays(crt_context* ctx, crt_scene_id scene, const float* rays, uint64_t n,
-                   int32_t camera_rays, int32_t* prim, int32_t* inst, float* t, float* uv);
+                   int32_t flags, int32_t* prim, int32_t* inst, float* t, float* uv);
 
 /* Camera rays of sample 0 at jitter r = 0.5 with no lens/time draws (pixel-centre first hits). */
 int crt_trace_primary(crt_context* ctx, crt_scene_id scene, uint32_t width, uint32_t height,
@@ -382,6 +436,81 @@ int crt_scenefile_load(const char* path, crt_scenefile** out);
 int crt_scenefile_desc(crt_scenefile* file, crt_scene_desc* desc, crt_scenefile_info* info);
 int crt_scenefile_close(crt_scenefile* file);
 
+/* ------------------------------------------------------------------ named scene builder */
+/* The reference's scene vocabulary (core/src/scenes/scene_builder.h, sceneConfig in
+ * core/src/scenes/scene_config.h): textures and materials are added under names, primitives name
+ * their material (and materials their texture), and the names are resolved as they are added, on
+ * the host (the reference resolves them on the device, one thread comparing strings while it builds
+ * the world: fetchTexture / fetchMaterial, core/src/gpu.cu:132-474). crt_scenefile_load builds
+ * through these calls, and so does a sceneConfig-to-crt shim (one call per sceneConfig entry).
+ * Names: NULL or "" adds an anonymous entry. A primitive whose material name is unknown is skipped
+ * and counted (the reference's load_scene skips it, core/src/gpu.cu:531-534); a material whose
+ * texture name is unknown is an error (CRT_ERR_INVALID_SCENE). The builder owns every array the
+ * descriptor points to until crt_builder_destroy. Defaults: the scene builder's camera
+ * (core/src/scenes/scene_builder.cpp:49: from (0, 0, 10) at the origin, y up, fov 70, focus 100,
+ * aspect 16:9), a black background. */
+typedef struct crt_builder crt_builder;
+/* Primitive transform (rt::transform, core/src/misc/transform.cuh): translate, rotate (degrees
+ * about x, y, z) and scale, each present or not (CRT_XF_* bits in `has`), applied as the reference's
+ * translate / rotate / scale wrappers see a primitive (scene_factory.cuh:710-725): the surface is the
+ * primitive under p -> S Rz(z) Ry(y) Rx(x) (p + T). A sphere under a non-uniform scale (an ellipsoid,
+ * outside the path) keeps its radius and is counted unsupported. */
+#define CRT_XF_TRANSLATE 1u
+#define CRT_XF_ROTATE 2u
+#define CRT_XF_SCALE 4u
+typedef struct crt_transform {
+    uint32_t has;
+    float translate[3];
+    float rotate_deg[3];
+    float scale[3];
+} crt_transform;
+int crt_builder_create(crt_builder** out);
+int crt_builder_destroy(crt_builder* b);
+int crt_builder_camera(crt_builder* b, const crt_camera* camera);
+int crt_builder_background(crt_builder* b, const float* rgb);
+int crt_builder_solid_texture(crt_builder* b, const char* name, const float* rgb);
+/* checker_texture (core/src/textures/checker_texture.cuh): each side a named texture, or a solid
+ * colour when its name is NULL, empty or unknown (scene_factory.cuh:125-160) */
+int crt_builder_checker_texture(crt_builder* b, const char* name, float scale, const char* even_texture,
+                                const char* odd_texture, const float* even_rgb, const float* odd_rgb);
+/* image_texture from a PNG or JPEG file; an unreadable file leaves an empty image (sampled cyan)
+ * with a warning, as load_images prints and continues (core/src/gpu.cu:1728-1770) */
+int crt_builder_image_texture(crt_builder* b, const char* name, const char* filepath);
+int crt_builder_image_texture_pixels(crt_builder* b, const char* name, uint32_t width, uint32_t height,
+                                     uint32_t channels, const uint8_t* pixels);
+int crt_builder_lambertian(crt_builder* b, const char* name, const float* rgb, const char* texture);
+int crt_builder_metal(crt_builder* b, const char* name, const float* rgb, float fuzz, const char* texture);
+int crt_builder_dielectric(crt_builder* b, const char* name, float ior);
+int crt_builder_diffuse_light(crt_builder* b, const char* name, const float* rgb, float intensity, int32_t one_sided);
+/* 1 when a material of that name exists */
+int crt_builder_has_material(crt_builder* b, const char* name);
+/* lights (core/src/scenes/scene_builder.cpp:862-906 and the directional light): positions are
+ * centres, colours linear RGB */
+int crt_builder_directional_light(crt_builder* b, const char* name, const float* position, const float* u, const float* v,
+                                  float intensity, const float* rgb, int32_t invisible);
+int crt_builder_omni_light(crt_builder* b, const char* name, const float* position, float radius, float intensity,
+                           const float* rgb, int32_t invisible);
+int crt_builder_spot_light(crt_builder* b, const char* name, const float* position, const float* direction,
+                           float cutoff_deg, float falloff, float intensity, float radius, const float* rgb,
+                           int32_t invisible);
+/* primitives; `mapping` and `xf` may be NULL */
+int crt_builder_sphere(crt_builder* b, const char* name, const float* position, float radius, const char* material,
+                       const crt_uvmap* mapping, const crt_transform* xf);
+int crt_builder_quad(crt_builder* b, const char* name, const float* position, const float* u, const float* v,
+                     const char* material, const crt_transform* xf);
+/* box (core/src/primitives/box.cuh:41-74): [position - size/2, position + size/2] as 12 triangles */
+int crt_builder_box(crt_builder* b, const char* name, const float* position, const float* size, const char* material,
+                    const crt_transform* xf);
+/* OBJ mesh (createObjMesh, core/src/scene_factory.cuh:373-385); use_smoothing = 0 drops the
+ * file's normals (flat shading) */
+int crt_builder_obj_mesh(crt_builder* b, const char* name, const char* filepath, const char* material,
+                         int32_t use_smoothing);
+/* counts n entries the caller left out as outside the render path */
+int crt_builder_note_unsupported(crt_builder* b, uint32_t n);
+/* The descriptor of everything added so far (valid until the next builder call), and the counts of
+ * primitives skipped for an unknown material and of entries left out as unsupported. */
+int crt_builder_desc(crt_builder* b, crt_scene_desc* out, uint32_t* skipped, uint32_t* unsupported);
+
 /* ------------------------------------------------------------------ introspection */
 /* Work counters of the last crt_render / crt_render_batch on this context. The traversal counts
  * (node_visits .. inst_visits) are filled only by a context created with count_work = 1. */
@@ -390,6 +519,8 @@ typedef struct crt_work_counters {
     uint64_t node_visits, leaf_visits, tri_tests, sphere_tests, quad_tests, inst_visits;
     uint64_t extend_launches, passes;
     uint64_t iterations;        /* regenerate/extend/shade rounds of the streaming passes */
+    uint64_t exact_queries;     /* closest-hit queries whose float triangle test could not decide a ray
+                                   within rounding distance of an edge, re-traced in double */
 } crt_work_counters;
 int crt_last_counters(crt_context* ctx, crt_work_counters* out);
 

@@ -146,7 +146,7 @@ struct Rng {
 };
 
 // ------------------------------------------------------------------ scene
-enum ObjType { OBJ_TRI = 0, OBJ_SPHERE = 1, OBJ_QUAD = 2, OBJ_LIGHT = 3 };
+enum ObjType { OBJ_TRI = 0, OBJ_SPHERE = 1, OBJ_QUAD = 2, OBJ_LIGHT = 3, OBJ_SPHERE_LIGHT = 4 };
 
 template <class R> struct Box {
     V3<R> lo, hi;  // interval per axis
@@ -164,6 +164,8 @@ template <class R> struct Obj {
     V3<R> c; R r; crt_uvmap map;
     // quad / light
     V3<R> q, qu, qv, qn, qw; R qd, area; bool invisible = false;
+    // sphere light (centre and radius in c, r): emission, spot cone
+    V3<R> le; R lint = 0; bool spot = false; V3<R> sdir; R scut = 0, sfall = 0;
     Box<R> box;
 };
 
@@ -180,9 +182,10 @@ template <class R> struct Scene {
     std::vector<crt_texture> texs;
     std::vector<std::vector<uint8_t>> tex_pixels;
     V3<R> background;
-    // camera (core/src/cameras/perspective_camera.cuh:41-110)
+    // camera (core/src/cameras/perspective_camera.cuh:41-110, orthographic_camera.cuh:57-142)
     V3<R> origin, llc, horizontal, vertical, cu, cv, cw;
     R lens_radius, time0, time1;
+    bool ortho = false;
     // BVH (reference-style): nodes over objs
     struct Node { Box<R> box; int left, right; bool leaf_l, leaf_r; };
     std::vector<Node> nodes;
@@ -259,7 +262,31 @@ template <class R> static int build_scene(Scene<R>& s, const crt_scene_desc* d, 
     }
     s.background = from3<R>(d->background);
     int32_t gid = 0;
-    // lights first (core/src/gpu.cu:510-516), then spheres (:527-535), quads (:548-556), meshes (:621-640)
+    // sphere lights: omni_light / spot_light ctors (core/src/lights/omni_light.cuh:22-32,
+    // spot_light.cuh:40-50): box centre -/+ radius, the spot's cutoff stored as
+    // cos(degrees_to_radians(cutoff)) in float, diffuse_spot_light normalises the direction per hit
+    auto add_sphere_light = [&](const crt_sphere_light& L) {
+        Obj<R> o{};
+        o.type = OBJ_SPHERE_LIGHT; o.gid = gid++; o.inst = -1; o.mat = 0xffffffffu;
+        o.c = from3<R>(L.center); o.r = R(L.radius);
+        o.map = crt_uvmap{1.0f, 1.0f, 0.0f, 0.0f, 1.0f, 1.0f};
+        o.invisible = L.invisible != 0;
+        o.le = from3<R>(L.rgb); o.lint = R(L.intensity);
+        o.spot = L.kind == CRT_LIGHT_SPOT;
+        if (o.spot) {
+            o.scut = R(std::cos(L.cutoff_deg * 3.14159265358979323846f / 180.0f));
+            o.sdir = unit(from3<R>(L.direction));
+            o.sfall = R(L.falloff);
+        }
+        V3<R> rv(o.r, o.r, o.r);
+        o.box = box_points(o.c - rv, o.c + rv);
+        s.objs.push_back(o);
+        s.lights.push_back((int)s.objs.size() - 1);
+    };
+    // lights first (core/src/gpu.cu:504-523: omni, directional, spot), then spheres (:527-535),
+    // quads (:548-556), meshes (:621-640)
+    for (uint32_t i = 0; i < d->num_sphere_lights; ++i)
+        if (d->sphere_lights[i].kind == CRT_LIGHT_OMNI) add_sphere_light(d->sphere_lights[i]);
     for (uint32_t i = 0; i < d->num_lights; ++i, ++gid) {
         const crt_area_light& L = d->lights[i];
         Obj<R> o{};
@@ -283,6 +310,8 @@ template <class R> static int build_scene(Scene<R>& s, const crt_scene_desc* d, 
         s.objs.push_back(o);
         s.lights.push_back((int)s.objs.size() - 1);
     }
+    for (uint32_t i = 0; i < d->num_sphere_lights; ++i)
+        if (d->sphere_lights[i].kind == CRT_LIGHT_SPOT) add_sphere_light(d->sphere_lights[i]);
     for (uint32_t i = 0; i < d->num_spheres; ++i, ++gid) {
         const crt_sphere& S = d->spheres[i];
         if (S.material >= d->num_materials) return CRT_ERR_INVALID_SCENE;
@@ -356,14 +385,17 @@ template <class R> static int build_scene(Scene<R>& s, const crt_scene_desc* d, 
         }
     }
     for (auto& o : s.objs) {
-        if (o.type != OBJ_LIGHT && o.mat >= s.mats.size()) return CRT_ERR_INVALID_SCENE;
+        if (o.type != OBJ_LIGHT && o.type != OBJ_SPHERE_LIGHT && o.mat >= s.mats.size()) return CRT_ERR_INVALID_SCENE;
     }
     return CRT_OK;
 }
 
 // Camera: perspective_camera::initialize (core/src/cameras/perspective_camera.cuh:41-104),
-// degrees_to_radians = deg * M_PI / 180 (core/src/utilities/math_utils.cuh:55-58).
-template <class R> static void setup_camera(Scene<R>& s, const crt_camera& c) {
+// degrees_to_radians = deg * M_PI / 180 (core/src/utilities/math_utils.cuh:55-58); or
+// orthographic_camera::initialize (core/src/cameras/orthographic_camera.cuh:57-118) for an image of
+// `width` columns: image height int(width / aspect) (at least 1), viewport 2 * ortho_height high and
+// as wide as the image's proportions, its lower-left corner half a viewport from lookfrom.
+template <class R> static void setup_camera(Scene<R>& s, const crt_camera& c, uint32_t width) {
     V3<R> lookfrom = from3<R>(c.lookfrom), lookat = from3<R>(c.lookat), vup = from3<R>(c.vup);
     R theta = R(c.vfov_deg) * R(3.14159265358979323846) / R(180.0f);
     R focus = R(c.focus_dist), ratio = R(c.aspect_ratio);
@@ -378,6 +410,16 @@ template <class R> static void setup_camera(Scene<R>& s, const crt_camera& c) {
     s.llc = s.origin - (half_width * focus) * s.cu - (half_height * focus) * s.cv - focus * s.cw;
     s.horizontal = (R(2) * half_width * focus) * s.cu;
     s.vertical = (R(2) * half_height * focus) * s.cv;
+    s.ortho = c.orthographic != 0;
+    if (s.ortho) {
+        int ih = (int)((float)width / c.aspect_ratio);
+        ih = ih < 1 ? 1 : ih;
+        R vh = R(2) * R(c.ortho_height);
+        R vw = vh * (R(width) / R(ih));
+        s.llc = s.origin - (R(0.5f) * vw) * s.cu - (R(0.5f) * vh) * s.cv;
+        s.horizontal = vw * s.cu;
+        s.vertical = vh * s.cv;
+    }
 }
 
 // ------------------------------------------------------------------ reference-style BVH
@@ -513,8 +555,9 @@ template <class R> static bool watertight(const V3<R>& A0, const V3<R>& B0, cons
 template <class R> static bool obj_hit(const Scene<R>& s, int idx, const Ray<R>& r, R tmin, R tmax, bool camera,
                                        Hit<R>& rec) {
     const Obj<R>& o = s.objs[idx];
-    if (o.type == OBJ_SPHERE) {
-        // sphere::hit (core/src/primitives/sphere.cuh:111-166): strict interval (surrounds)
+    if (o.type == OBJ_SPHERE || o.type == OBJ_SPHERE_LIGHT) {
+        // sphere::hit (core/src/primitives/sphere.cuh:111-166): strict interval (surrounds); the
+        // sphere lights repeat it and then hide from camera rays when invisible (omni_light.cuh:40-78)
         V3<R> oc = r.o - o.c;
         R a = len2(r.d);
         R half_b = dot(oc, r.d);
@@ -546,6 +589,7 @@ template <class R> static bool obj_hit(const Scene<R>& s, int idx, const Ray<R>&
                 if (!(tmin < root && root < tmax)) return false;
             }
         }
+        if (o.type == OBJ_SPHERE_LIGHT && o.invisible && camera) return false;
         rec.t = root;
         rec.p = r.o + root * r.d;
         V3<R> outward = (rec.p - o.c) / o.r;
@@ -758,12 +802,19 @@ template <class R> struct Onb {
 };
 
 // directional_light::pdf_value (core/src/lights/directional_light.cuh:147-158); the hit uses
-// depth 0 so the light is always visible here.
+// depth 0 so the light is always visible here. Sphere lights (omni_light.cuh:80-88,
+// spot_light.cuh:96-104): one over the solid angle of the sphere's cone seen from o.
 template <class R> static R light_pdf_value(const Scene<R>& s, int li, V3<R> o, V3<R> v, Counters& cnt) {
     Hit<R> rec;
     Ray<R> r{o, v, 0};
     cnt.light_tests++;
     if (!obj_hit(s, li, r, R(0.00001f), std::numeric_limits<R>::infinity(), false, rec)) return 0;
+    const Obj<R>& L = s.objs[li];
+    if (L.type == OBJ_SPHERE_LIGHT) {
+        R cos_max = std::sqrt(R(1) - L.r * L.r / len2(L.c - o));
+        R solid = R(2) * R(3.14159265358979323846) * (R(1) - cos_max);
+        return R(1) / solid;
+    }
     R dist2 = rec.t * rec.t * len2(v);
     R cosine = std::fabs(dot(v, rec.normal) / length(v));
     return dist2 / (cosine * s.objs[li].area);
@@ -775,12 +826,26 @@ template <class R> static R lights_pdf_value(const Scene<R>& s, V3<R> o, V3<R> v
     for (size_t i = 0; i < s.lights.size(); ++i) sum += weight * light_pdf_value(s, s.lights[i], o, v, cnt);
     return sum;
 }
-// hittable_list::random (hittable_list.cuh:191-216) -> directional_light::random (:165-172)
+// hittable_list::random (hittable_list.cuh:191-216) -> directional_light::random (:165-172), or
+// for a sphere light omni_light::random (omni_light.cuh:90-98): random_to_sphere
+// (gpu_randomizer.cuh:82-93) in the frame built on the direction to the centre.
 template <class R> static V3<R> lights_random(const Scene<R>& s, V3<R> o, Rng& g) {
     if (s.lights.empty()) return V3<R>();
     int r = 0;
     if (s.lights.size() > 1) r = g.uniform_int_min(0, (int)s.lights.size() - 1);
     const Obj<R>& L = s.objs[s.lights[r]];
+    if (L.type == OBJ_SPHERE_LIGHT) {
+        V3<R> dir = L.c - o;
+        R d2 = len2(dir);
+        Onb<R> uvw(dir);
+        R r1 = R(g.uniform());
+        R r2 = R(g.uniform());
+        R z = R(1) + r2 * (std::sqrt(R(1) - L.r * L.r / d2) - R(1));
+        R phi = R(2) * R(3.14159265358979323846) * r1;
+        R x = std::cos(phi) * std::sqrt(R(1) - z * z);
+        R y = std::sin(phi) * std::sqrt(R(1) - z * z);
+        return uvw.local(V3<R>(x, y, z));
+    }
     R a = R(g.uniform()); R b = R(g.uniform());
     V3<R> p = L.q + (a - R(0.5f)) * L.qu + (b - R(0.5f)) * L.qv;
     return p - o;
@@ -819,10 +884,13 @@ template <class R, class Draw> static V3<R> dielectric_dir(V3<R> d, V3<R> n, boo
 // Camera ray of image-plane coordinates (u, v): perspective_camera::get_ray
 // (core/src/cameras/perspective_camera.cuh:106-113): lens offset from random_in_unit_disk scaled by
 // the lens radius in the camera's (u, v) plane, direction to the focus-plane point, time draw.
+// The orthographic camera's get_ray (orthographic_camera.cuh:125-142) draws the same way and
+// starts the ray at the viewport point plus the offset, along -w.
 template <class R> static Ray<R> camera_ray(const Scene<R>& s, R u, R v, Rng& g) {
     V3<R> rdisk = s.lens_radius * random_in_unit_disk<R>(g);
     V3<R> off = s.cu * rdisk.x + s.cv * rdisk.y;
     R tm = s.time0 + R(g.uniform()) * (s.time1 - s.time0);
+    if (s.ortho) return Ray<R>{s.llc + u * s.horizontal + v * s.vertical + off, R(-1) * s.cw, tm};
     return Ray<R>{s.origin + off, s.llc + u * s.horizontal + v * s.vertical - s.origin - off, tm};
 }
 
@@ -849,6 +917,18 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
         bool alpha0 = false;
         if (o->type == OBJ_LIGHT) {
             if (rec.front) emitted = o->c * o->r; else alpha0 = true;
+        } else if (o->type == OBJ_SPHERE_LIGHT) {
+            if (!o->spot) {
+                emitted = o->le * o->lint;   // diffuse_light, non-directional
+            } else if (!rec.front) {
+                // diffuse_spot_light::emitted (core/src/materials/diffuse_spot_light.cuh:38-61): back
+                // faces give colour(0,0,0,0) when invisible, colour(1,1,1,0) when visible
+                alpha0 = true;
+                if (!o->invisible) emitted = V3<R>(1, 1, 1);
+            } else {
+                R ct = dot(unit(ray.d), o->sdir);
+                emitted = ct < o->scut ? V3<R>(1, 1, 1) : o->le * (o->lint * std::pow(ct, o->sfall));
+            }
         } else if (s.mats[o->mat].type == CRT_MAT_DIFFUSE_LIGHT) {
             const crt_material& m = s.mats[o->mat];
             bool one_sided = (m.flags & CRT_MATF_ONE_SIDED) != 0;
@@ -862,7 +942,7 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
             o = &s.objs[rec.obj];
         }
         // scatter (lambertian.cuh:63-136, metal.cuh:57-65, dielectric.cuh:32-51); lights do not
-        bool is_light = o->type == OBJ_LIGHT || s.mats[o->mat].type == CRT_MAT_DIFFUSE_LIGHT;
+        bool is_light = o->type == OBJ_LIGHT || o->type == OBJ_SPHERE_LIGHT || s.mats[o->mat].type == CRT_MAT_DIFFUSE_LIGHT;
         if (is_light) { result = result + att * emitted; break; }
         const crt_material& m = s.mats[o->mat];
         result = result + att * emitted;
@@ -911,7 +991,7 @@ template <class R> static int render_impl(const crt_scene_desc* d, const crt_ren
     Scene<R> s;
     int rc = build_scene(s, d, o.ref_style, o.accel);
     if (rc) return rc;
-    setup_camera(s, d->camera);
+    setup_camera(s, d->camera, rd->width);
     if (o.accel == 0 && !s.objs.empty()) {
         std::vector<int> ids(s.objs.size());
         for (size_t i = 0; i < ids.size(); ++i) ids[i] = (int)i;
@@ -1046,14 +1126,16 @@ int oracle_trace(const crt_scene_desc* d, const double* rays, uint64_t n, int32_
 /* Pixel-centre camera rays (sample jitter 0.5, no lens) in double: n = W*H rows of 8 doubles. */
 int oracle_primary_rays(const crt_scene_desc* d, uint32_t W, uint32_t H, double* rays) {
     orc::Scene<double> s;
-    orc::setup_camera(s, d->camera);
+    orc::setup_camera(s, d->camera, W);
     for (uint32_t jj = 0; jj < H; ++jj) {
         uint32_t j = H - 1 - jj;
         for (uint32_t i = 0; i < W; ++i) {
             double u = ((double)i + 0.5) / W, v = ((double)j + 0.5) / H;
-            orc::V3<double> dir = s.llc + u * s.horizontal + v * s.vertical - s.origin;
+            orc::V3<double> p = s.llc + u * s.horizontal + v * s.vertical;
+            orc::V3<double> o = s.ortho ? p : s.origin;
+            orc::V3<double> dir = s.ortho ? -1.0 * s.cw : p - s.origin;
             double* q = rays + 8 * ((size_t)jj * W + i);
-            q[0] = s.origin.x; q[1] = s.origin.y; q[2] = s.origin.z; q[3] = 0.00001;
+            q[0] = o.x; q[1] = o.y; q[2] = o.z; q[3] = 0.00001;
             q[4] = dir.x; q[5] = dir.y; q[6] = dir.z; q[7] = std::numeric_limits<float>::max();
         }
     }
@@ -1124,9 +1206,9 @@ int oracle_ref_bvh_info(const crt_scene_desc* d, uint64_t* nodes, uint32_t* dept
 
 /* Camera basis of setup_camera (FP64): origin, lower-left corner, horizontal, vertical, u, v, w
  * (3 doubles each) and the lens radius: 22 doubles. */
-int oracle_camera_basis(const crt_scene_desc* d, double* out) {
+int oracle_camera_basis(const crt_scene_desc* d, uint32_t width, double* out) {
     orc::Scene<double> s;
-    orc::setup_camera(s, d->camera);
+    orc::setup_camera(s, d->camera, width);
     const orc::V3<double> v[7] = {s.origin, s.llc, s.horizontal, s.vertical, s.cu, s.cv, s.cw};
     for (int k = 0; k < 7; ++k) { out[3 * k] = v[k].x; out[3 * k + 1] = v[k].y; out[3 * k + 2] = v[k].z; }
     out[21] = s.lens_radius;
@@ -1135,9 +1217,9 @@ int oracle_camera_basis(const crt_scene_desc* d, double* out) {
 
 /* camera_ray at image-plane points uv[2k], uv[2k+1] (FP64), draws from the keyed PCG stream
  * (seed, pixel k, sample 0, bounce 0): rows of 8 doubles (origin, time, direction, 0). */
-int oracle_camera_rays(const crt_scene_desc* d, uint64_t n, const double* uv, uint64_t seed, double* out) {
+int oracle_camera_rays(const crt_scene_desc* d, uint32_t width, uint64_t n, const double* uv, uint64_t seed, double* out) {
     orc::Scene<double> s;
-    orc::setup_camera(s, d->camera);
+    orc::setup_camera(s, d->camera, width);
     for (uint64_t k = 0; k < n; ++k) {
         orc::Rng g;
         g.mode = CRT_RNG_PCG; g.seed = seed; g.pixel = (uint32_t)k; g.sample = 0;
@@ -1217,6 +1299,22 @@ int oracle_light_pdf(const crt_scene_desc* d, const double* o, const double* v, 
     for (uint64_t k = 0; k < n; ++k) {
         orc::V3<double> oo(o[3 * k], o[3 * k + 1], o[3 * k + 2]), vv(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
         out[k] = orc::lights_pdf_value(s, oo, vv, c);
+    }
+    return CRT_OK;
+}
+
+/* Light sample directions (lights_random, FP64) from points o, draws from the keyed PCG stream
+ * (seed, pixel k, sample 0, bounce 0). */
+int oracle_light_sample(const crt_scene_desc* d, const double* o, uint64_t n, uint64_t seed, double* out) {
+    orc::Scene<double> s;
+    int rc = orc::build_scene(s, d, false, 1);
+    if (rc) return rc;
+    for (uint64_t k = 0; k < n; ++k) {
+        orc::Rng g;
+        g.mode = CRT_RNG_PCG; g.seed = seed; g.pixel = (uint32_t)k; g.sample = 0;
+        g.bounce(0);
+        orc::V3<double> r = orc::lights_random(s, orc::V3<double>(o[3 * k], o[3 * k + 1], o[3 * k + 2]), g);
+        out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
     }
     return CRT_OK;
 }

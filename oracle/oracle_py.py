@@ -148,6 +148,18 @@ def light_pdf(scene, origins, dirs):
     return out
 
 
+def light_sample(scene, origins, seed=1):
+    """lights_random (FP64) from each origin: the direction towards a sample of light 0."""
+    desc, keep = scene.to_desc()
+    o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(o)
+    f = lib().oracle_light_sample
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]
+    f(C.cast(C.pointer(desc), C.c_void_p), _fp(o), o.shape[0], seed, _fp(out))
+    del keep
+    return out
+
+
 def ref_bvh_info(scene):
     desc, keep = scene.to_desc()
     nodes = C.c_uint64()
@@ -159,13 +171,14 @@ def ref_bvh_info(scene):
     return nodes.value, depth.value
 
 
-def camera_basis(scene):
-    """setup_camera in FP64: dict of origin, llc, horizontal, vertical, u, v, w, lens_radius."""
+def camera_basis(scene, width=0):
+    """setup_camera in FP64 for an image `width` columns wide (the orthographic viewport follows the
+    image): dict of origin, llc, horizontal, vertical, u, v, w, lens_radius."""
     desc, keep = scene.to_desc()
     out = np.zeros(22)
     f = lib().oracle_camera_basis
-    f.argtypes = [C.c_void_p, C.c_void_p]
-    f(C.cast(C.pointer(desc), C.c_void_p), _fp(out))
+    f.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    f(C.cast(C.pointer(desc), C.c_void_p), width, _fp(out))
     del keep
     names = ["origin", "llc", "horizontal", "vertical", "u", "v", "w"]
     d = {k: out[3 * i:3 * i + 3].copy() for i, k in enumerate(names)}
@@ -173,14 +186,15 @@ def camera_basis(scene):
     return d
 
 
-def camera_rays(scene, uv, seed=1):
-    """camera_ray at image-plane points (u, v) in FP64: (origins, times, directions)."""
+def camera_rays(scene, uv, seed=1, width=0):
+    """camera_ray at image-plane points (u, v) in FP64 for an image `width` columns wide:
+    (origins, times, directions)."""
     desc, keep = scene.to_desc()
     uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
     out = np.zeros((uv.shape[0], 8))
     f = lib().oracle_camera_rays
-    f.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]
-    f(C.cast(C.pointer(desc), C.c_void_p), uv.shape[0], _fp(uv), seed, _fp(out))
+    f.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]
+    f(C.cast(C.pointer(desc), C.c_void_p), width, uv.shape[0], _fp(uv), seed, _fp(out))
     del keep
     return out[:, 0:3], out[:, 3], out[:, 4:7]
 

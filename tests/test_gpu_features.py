@@ -15,9 +15,9 @@ NT = os.cpu_count() or 4
 
 def _first_hit_check(r, s, W, H, accel):
     sid = r.upload(s)
-    rays = O.primary_rays(s, W, H)
-    prim, inst, t, _ = r.trace_rays(sid, rays.astype(np.float32), camera=True)
-    oprim, oinst, ot, _ = O.trace(s, rays, camera=True, precision=64, accel=accel)
+    rays = O.primary_rays(s, W, H).astype(np.float32)
+    prim, inst, t, _ = r.trace_rays(sid, rays, camera=True)
+    oprim, oinst, ot, _ = O.trace(s, rays.astype(np.float64), camera=True, precision=64, accel=accel)
     t = t.astype(np.float64)
     agree = (prim == oprim) & (inst == oinst)
     tie = np.abs(t - ot) <= 1e-5 * np.maximum(np.abs(ot), 1e-30)

@@ -200,6 +200,105 @@ def test_light_pdf_integrates_to_one():
     assert abs(est - 1.0) < 0.02, est
 
 
+def test_sphere_light_pdf_is_uniform_over_its_cone():
+    # omni_light.cuh:80-98: directions towards the sphere have pdf 1 / (2 pi (1 - cos theta_max)),
+    # cos theta_max = sqrt(1 - r^2 / d^2); the pdf integrates to one over the sphere of directions and
+    # the samples (random_to_sphere) are the cone's directions, uniform in solid angle
+    s = Scene()
+    c, r = np.array([0.3, 2.0, -0.4]), 0.7
+    s.omni_light(tuple(c), r, rgb=(1, 1, 1), intensity=2.0)
+    c, r = c.astype(np.float32).astype(np.float64), float(np.float32(r))   # the scene holds float32
+    o = np.array([0.1, -0.2, 0.05])
+    dc = c - o
+    dist = np.linalg.norm(dc)
+    cos_max = math.sqrt(1 - r * r / (dist * dist))
+    rng = np.random.default_rng(7)
+    n = 400000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pdf = O.light_pdf(s, np.tile(o, (n, 1)), d)
+    inside = d @ (dc / dist) > cos_max
+    assert np.allclose(pdf[inside], 1.0 / (2 * math.pi * (1 - cos_max)), rtol=1e-12)
+    assert np.all(pdf[~inside] == 0.0) or np.abs(d[~inside & (pdf > 0)] @ (dc / dist) - cos_max).max() < 1e-9
+    assert abs(pdf.mean() * 4 * math.pi - 1.0) < 0.02
+    smp = O.light_sample(s, np.tile(o, (20000, 1)), seed=5)
+    cs = (smp / np.linalg.norm(smp, axis=1, keepdims=True)) @ (dc / dist)
+    assert cs.min() >= cos_max - 1e-12 and cs.max() <= 1.0 + 1e-12
+    # uniform in solid angle over the cone: cos is uniform on [cos_max, 1] (mean and variance)
+    assert abs(cs.mean() - (1 + cos_max) / 2) < 3e-3 * (1 - cos_max) * 10
+    assert abs(cs.var() - (1 - cos_max) ** 2 / 12) < 0.05 * (1 - cos_max) ** 2 / 12
+
+
+def test_sphere_lights_come_first_and_last_in_the_world_order():
+    # load_scene adds omni, directional then spot lights (core/src/gpu.cu:504-523): the first light,
+    # the one the mixture pdf samples, is an omni light when there is one; global ids follow that order
+    s = Scene()
+    s.spot_light((0, 5, 0), (0, -1, 0), 30.0, 2.0, 0.5, invisible=False)
+    s.area_light((0, 3, 0), (1, 0, 0), (0, 0, 1), invisible=False)
+    s.omni_light((4, 0, 0), 0.5, invisible=False)
+    # rays from far away straight at each light
+    rays = np.array([[10, 0, 0, 1e-5, -1, 0, 0, 1e30],          # omni at x = 4
+                     [0.2, -10, 0.1, 1e-5, 0, 1, 0, 1e30],     # area light at y = 3 (below the spot)
+                     [0, 20, 0, 1e-5, 0, -1, 0, 1e30]], float)  # spot at y = 5 from above
+    prim, _, t, _ = O.trace(s, rays, camera=False, precision=64, accel=1)
+    assert list(prim) == [0, 1, 2]
+    assert np.allclose(t, [5.5, 13.0, 14.5])
+    # light 0 is the omni light: its samples point at the sphere around (4, 0, 0)
+    smp = O.light_sample(s, np.zeros((200, 3)), seed=2)
+    u = smp / np.linalg.norm(smp, axis=1, keepdims=True)
+    assert np.all(u @ np.array([1.0, 0.0, 0.0]) > math.sqrt(1 - 0.25 / 16) - 1e-12)
+
+
+def _spot_scene(invisible, cutoff=30.0, falloff=3.0):
+    s = Scene().set_camera((0, 0, 4), (0, 0, 0), vfov=10.0, aspect_ratio=1.0, focus_dist=4.0)
+    s.background = (0.0, 0.0, 0.0)
+    s.spot_light((0, 0, 0), (0, 0, 1), cutoff, falloff, 1.0, rgb=(2.0, 1.0, 0.5), intensity=3.0, invisible=invisible)
+    return s
+
+
+def test_spot_light_emission_front_face():
+    # diffuse_spot_light.cuh:38-61: a ray along D meeting the front face emits rgb * intensity *
+    # cos^falloff with cos = D . axis when cos >= cos(cutoff), white otherwise. A camera ray along -z
+    # meets a spot aimed at +z with cos = -1 (outside any cone): white
+    s = _spot_scene(False)
+    acc, _ = O.render(s, 1, 1, 1, 1, seed=1, rng="pcg", precision=64)
+    assert np.allclose(acc[0, 0, :3], [1.0, 1.0, 1.0])
+    # aimed at the camera's rays (-z): cos = 1 for the centre ray: rgb * intensity
+    s2 = Scene().set_camera((0, 0, 4), (0, 0, 0), vfov=1.0, aspect_ratio=1.0, focus_dist=4.0)
+    s2.spot_light((0, 0, 0), (0, 0, -1), 30.0, 3.0, 1.0, rgb=(2.0, 1.0, 0.5), intensity=3.0, invisible=False)
+    acc, _ = O.render(s2, 1, 1, 1, 1, seed=1, rng="pcg", precision=64)
+    # the pixel ray is within half a degree of the axis: cos^3 >= 0.9999
+    assert np.allclose(acc[0, 0, :3], [6.0, 3.0, 1.5], rtol=1e-4)
+
+
+def test_spot_light_back_face_passes_through():
+    # a camera inside the spot's sphere meets its back face: colour(1,1,1,0) when visible (added, then
+    # the re-trace from t + 0.001 finds nothing and keeps the light record: no background), colour
+    # (0,0,0,0) when invisible; invisible lights are hidden from camera rays anyway
+    for invisible, want in ((False, 1.0), (True, 0.0)):
+        s = Scene().set_camera((0, 0, 0.2), (0, 0, -1), vfov=10.0, aspect_ratio=1.0, focus_dist=1.0)
+        s.background = (0.25, 0.25, 0.25)
+        s.spot_light((0, 0, 0), (0, 0, 1), 30.0, 3.0, 1.0, invisible=invisible)
+        acc, st = O.render(s, 1, 1, 1, 1, seed=1, rng="pcg", precision=64)
+        want_rgb = want if not invisible else 0.25   # invisible: the camera ray misses to the background
+        assert np.allclose(acc[0, 0, :3], want_rgb), (invisible, acc[0, 0])
+        assert st["retraces"] == (0 if invisible else 1)
+
+
+def test_omni_light_emits_from_both_faces_and_hides_from_camera_rays():
+    for invisible in (False, True):
+        s = Scene().set_camera((0, 0, 4), (0, 0, 0), vfov=5.0, aspect_ratio=1.0, focus_dist=4.0)
+        s.background = (0.1, 0.2, 0.3)
+        s.omni_light((0, 0, 0), 1.0, rgb=(0.5, 1.0, 2.0), intensity=4.0, invisible=invisible)
+        acc, _ = O.render(s, 1, 1, 1, 1, seed=1, rng="pcg", precision=64)
+        assert np.allclose(acc[0, 0, :3], [0.1, 0.2, 0.3] if invisible else [2.0, 4.0, 8.0])
+        # from inside (back face): the same emission, no pass-through
+        s.set_camera((0, 0, 0.5), (0, 0, -1), vfov=5.0, aspect_ratio=1.0, focus_dist=1.0)
+        acc, st = O.render(s, 1, 1, 1, 1, seed=1, rng="pcg", precision=64)
+        assert np.allclose(acc[0, 0, :3], [0.1, 0.2, 0.3] if invisible else [2.0, 4.0, 8.0])
+        assert st["retraces"] == 0
+
+
 def test_white_furnace_floor_under_sky():
     # Lambertian floor of albedo rho under a uniform sky of radiance 1, camera looking down, one
     # tiny far light of zero emission: pixel -> rho * (1 - fraction of cosine mass on the light)
@@ -365,6 +464,47 @@ def test_camera_thin_lens_converges_on_focus_plane():
     off = o - b["origin"]
     assert np.all(np.linalg.norm(off, axis=1) < 0.4) and np.linalg.norm(off, axis=1).max() > 0.3
     assert np.allclose(off @ b["w"], 0.0, atol=1e-12)
+
+
+def test_orthographic_camera_closed_form():
+    # orthographic_camera.cuh:57-142: image height int(W / aspect) (at least 1); viewport 2 *
+    # ortho_height high, as wide as the image's proportions, centred at lookfrom in the (u, v) plane;
+    # every ray points along -w with unit length and starts on the viewport
+    s = Scene().set_camera((1.0, 2.0, 9.0), (0.0, 0.6, 0.0), vfov=18.0, aspect_ratio=16 / 9, orthographic=True,
+                           ortho_height=2.0)
+    W = 512
+    Hc = int(W / (16 / 9))   # 288
+    b = O.camera_basis(s, width=W)
+    w = _unit(np.array([1.0, 2.0, 9.0]) - np.array([0.0, np.float32(0.6), 0.0]))   # inputs are float32
+    assert np.allclose(b["w"], w, rtol=0, atol=1e-15)
+    w = b["w"]
+    assert abs(np.linalg.norm(b["vertical"]) - 4.0) < 1e-12
+    assert abs(np.linalg.norm(b["horizontal"]) - 4.0 * W / Hc) < 1e-9
+    centre = b["llc"] + 0.5 * b["horizontal"] + 0.5 * b["vertical"]
+    assert np.allclose(centre, [1.0, 2.0, 9.0], atol=1e-12)
+    uv = np.random.default_rng(4).uniform(0, 1, (200, 2))
+    o, tm, d = O.camera_rays(s, uv, width=W)
+    assert np.allclose(d, -w, rtol=0, atol=1e-15)
+    assert np.allclose(o, b["llc"] + uv[:, :1] * b["horizontal"] + uv[:, 1:] * b["vertical"], rtol=0, atol=1e-12)
+    assert np.allclose((o - b["origin"]) @ w, 0.0, rtol=0, atol=1e-12)   # the viewport plane holds lookfrom
+    assert np.all((tm >= 0.0) & (tm < 1.0))
+    # a width whose height rounds down: the viewport follows int(W / aspect), not W / aspect
+    b2 = O.camera_basis(s, width=100)
+    assert abs(np.linalg.norm(b2["horizontal"]) - 4.0 * 100 / 56) < 1e-9
+
+
+def test_orthographic_camera_lens_offsets_the_origin():
+    # the orthographic get_ray still draws the lens disk: the origin moves by up to aperture / 2 in
+    # the (u, v) plane, the direction stays -w
+    s = Scene().set_camera((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), aspect_ratio=1.0, aperture=1.0, orthographic=True,
+                           ortho_height=1.0)
+    b = O.camera_basis(s, width=64)
+    uv = np.random.default_rng(5).uniform(0, 1, (400, 2))
+    o, _, d = O.camera_rays(s, uv, width=64, seed=3)
+    p = b["llc"] + uv[:, :1] * b["horizontal"] + uv[:, 1:] * b["vertical"]
+    off = np.linalg.norm(o - p, axis=1)
+    assert np.all(off < 0.5) and off.max() > 0.4
+    assert np.allclose(d, [0.0, 0.0, -1.0], rtol=0, atol=1e-15)
 
 
 # ------------------------------------------------------------------ textures

@@ -88,3 +88,144 @@ def test_reference_scene_files_parse():
             SceneFile(path)
         except A.CrtError as e:
             assert e.code == A.CRT_ERR_IO and ("cannot load mesh" in str(e) or "cannot open" in str(e)), (path, str(e))
+
+
+def _rot(axis, deg):
+    a = np.radians(deg)
+    c, s = np.cos(a), np.sin(a)
+    i, j = (axis + 1) % 3, (axis + 2) % 3
+    R = np.eye(3)
+    R[i, i], R[i, j], R[j, i], R[j, j] = c, -s, s, c
+    return R
+
+
+def test_primitive_transforms(tmp_path):
+    # translate, rotate (degrees), scale as the reference's wrappers see them: a world ray is divided
+    # by the scale, rotated by Rx(-x) Ry(-y) Rz(-z) and offset by -T, so the surface it meets is
+    # S Rz(z) Ry(y) Rx(x) (p + T) (scale.cuh:50-56, rotate.cuh:104-119, translate.cuh:38-43)
+    text = """
+    materials: { lambertian: ( { name = "m"; color = { r = 0.5; g = 0.5; b = 0.5; }; } ); };
+    primitives: {
+      spheres: ( { name = "s"; position = { x = 1.0; y = 0.0; z = 0.0; }; radius = 0.5; material = "m";
+                   transform = { translate = { x = 0.0; y = 2.0; z = 0.0; }; rotate = { x = 0.0; y = 0.0; z = 90.0; };
+                                 scale = { x = 2.0; y = 2.0; z = 2.0; }; }; } );
+      quads: ( { name = "q"; position = { x = 0.0; y = 0.0; z = 0.0; }; u = { x = 1.0; y = 0.0; z = 0.0; };
+                 v = { x = 0.0; y = 1.0; z = 0.0; }; material = "m";
+                 transform = { translate = { x = 1.0; y = 0.0; z = 0.0; }; rotate = { x = 30.0; y = 45.0; z = 0.0; };
+                               scale = { x = 1.0; y = 2.0; z = 3.0; }; }; } );
+      boxes: ( { name = "b"; position = { x = 0.0; y = 0.0; z = 0.0; }; size = { x = 2.0; y = 2.0; z = 2.0; };
+                 material = "m"; transform = { translate = { x = 5.0; y = 0.0; z = 0.0; }; }; } );
+    };
+    """
+    p = tmp_path / "xf.scene"
+    p.write_text(text)
+    f = SceneFile(p)
+    d, _ = f.to_desc()
+    assert f.info()["unsupported"] == 0
+    s = d.spheres[0]
+    want_c = 2.0 * (_rot(2, 90.0) @ (np.array([1.0, 0.0, 0.0]) + [0.0, 2.0, 0.0]))
+    assert list(s.center) == pytest.approx(list(want_c), abs=1e-5) and s.radius == pytest.approx(1.0)
+    q = d.quads[0]
+    L = np.diag([1.0, 2.0, 3.0]) @ _rot(2, 0.0) @ _rot(1, 45.0) @ _rot(0, 30.0)
+    assert list(q.corner) == pytest.approx(list(L @ np.array([1.0, 0.0, 0.0])), abs=1e-5)
+    assert list(q.u) == pytest.approx(list(L @ [1.0, 0.0, 0.0]), abs=1e-5)
+    assert list(q.v) == pytest.approx(list(L @ [0.0, 1.0, 0.0]), abs=1e-5)
+    box = np.ctypeslib.as_array(d.meshes[0].positions, (24,)).reshape(8, 3)
+    assert box.min(0) == pytest.approx([4.0, -1.0, -1.0]) and box.max(0) == pytest.approx([6.0, 1.0, 1.0])
+    # a non-uniform scale of a sphere (an ellipsoid) is not representable: counted as unsupported
+    p.write_text(text.replace("scale = { x = 2.0; y = 2.0; z = 2.0; }", "scale = { x = 2.0; y = 1.0; z = 2.0; }"))
+    assert SceneFile(p).info()["unsupported"] == 1
+
+
+def _scene_texture(tmp_path, image_name):
+    p = tmp_path / "tex.scene"
+    p.write_text('textures: { image: ( { name = "t"; filepath = "%s"; } ); };\n'
+                 'materials: { lambertian: ( { name = "m"; texture = "t"; } ); };\n' % image_name)
+    f = SceneFile(p)
+    d, _ = f.to_desc()
+    t = d.textures[0]
+    assert t.type == A.CRT_TEX_IMAGE
+    px = np.ctypeslib.as_array(t.pixels, (t.height * t.width * t.channels,)).reshape(t.height, t.width, t.channels).copy()
+    return px
+
+
+def _synthetic_rgb(h, w, seed=0):
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    img = np.stack([(x * 255 // max(w - 1, 1)), (y * 255 // max(h - 1, 1)), ((x + y) * 3) % 256], axis=-1).astype(np.float64)
+    img += rng.normal(0, 6, img.shape)
+    img[h // 3:h // 2, w // 4:w // 2] = [250, 20, 30]     # a sharp-edged patch
+    return np.clip(img, 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("opts", [dict(subsampling=0, quality=95), dict(subsampling=2, quality=90),
+                                  dict(subsampling=1, quality=85), dict(subsampling=0, quality=92, progressive=True),
+                                  dict(subsampling=2, quality=80, progressive=True), dict(subsampling=2, quality=90, restart_marker_blocks=3),
+                                  dict(subsampling=2, quality=85, progressive=True, restart_marker_blocks=5)])
+def test_jpeg_textures_decode_like_libjpeg(tmp_path, opts):
+    # JPEGs written by Pillow (libjpeg): baseline and progressive, 4:4:4 / 4:2:2 / 4:2:0, restart
+    # intervals; the library's decoder agrees with Pillow's (libjpeg's integer IDCT, triangle-filter
+    # chroma upsampling) within the rounding of the IDCT and of the upsampling
+    PIL = pytest.importorskip("PIL.Image")
+    img = _synthetic_rgb(37, 61)
+    path = tmp_path / "t.jpg"
+    kw = dict(opts)
+    rst = kw.pop("restart_marker_blocks", None)
+    if rst:
+        kw["restart_marker_blocks"] = rst
+    PIL.fromarray(img).save(path, "JPEG", **kw)
+    ref = np.asarray(PIL.open(path).convert("RGB")).astype(int)
+    got = _scene_texture(tmp_path, "t.jpg").astype(int)
+    assert got.shape == ref.shape
+    diff = np.abs(got - ref)
+    print(opts, "mean |diff| %.3f max %d" % (diff.mean(), diff.max()))
+    if opts["subsampling"] == 0:
+        assert diff.max() <= 3 and diff.mean() < 0.1
+    else:
+        assert diff.max() <= 6 and diff.mean() < 0.6
+
+
+def test_jpeg_grayscale_texture(tmp_path):
+    PIL = pytest.importorskip("PIL.Image")
+    img = _synthetic_rgb(40, 24)[..., 0]
+    PIL.fromarray(img, "L").save(tmp_path / "g.jpg", "JPEG", quality=93)
+    ref = np.asarray(PIL.open(tmp_path / "g.jpg").convert("RGB")).astype(int)
+    got = _scene_texture(tmp_path, "g.jpg").astype(int)
+    assert got.shape == ref.shape and np.abs(got - ref).max() <= 3
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/data/textures"), reason="reference textures absent")
+def test_shipped_jpeg_textures_decode(tmp_path):
+    # every JPEG texture the shipped scenes can name (baseline and progressive) decodes close to
+    # Pillow's result
+    PIL = pytest.importorskip("PIL.Image")
+    for path in sorted(glob.glob("/root/reference/data/textures/*.jpg")):
+        link = tmp_path / os.path.basename(path)
+        if not link.exists():
+            os.symlink(path, link)
+        got = _scene_texture(tmp_path, os.path.basename(path)).astype(int)
+        ref = np.asarray(PIL.open(path).convert("RGB")).astype(int)
+        assert got.shape == ref.shape, path
+        diff = np.abs(got - ref)
+        assert diff.mean() < 1.0 and np.percentile(diff, 99.9) <= 8, (path, diff.mean())
+
+
+def test_camera_defaults_and_orthographic_camera(tmp_path):
+    # scene_builder.cpp:49 defaults (look from (0, 0, 10) at the origin, focus 100, fov 70) and the
+    # orthographic keys of scene_loader.cpp:361-366 (test_cameras.scene uses them)
+    p = tmp_path / "cam.scene"
+    p.write_text('camera: { aspectRatio = "16:9"; };')
+    d, _ = SceneFile(str(p)).to_desc()
+    c = d.camera
+    assert list(c.lookfrom) == [0, 0, 10] and list(c.lookat) == [0, 0, 0] and list(c.vup) == [0, 1, 0]
+    assert c.vfov_deg == 70 and c.focus_dist == 100 and c.aperture == 0 and c.orthographic == 0
+    assert c.aspect_ratio == pytest.approx(16 / 9)
+    p.write_text('camera: { aspectRatio = "16:9"; lookFrom = { x = 0.0; y = 2.0; z = 9.0; }; '
+                 'lookAt = { x = 0.0; y = 0.6; z = 0.0; }; orthographic = true; orthoHeight = 2.0; };')
+    f = SceneFile(str(p))
+    d, _ = f.to_desc()
+    assert d.camera.orthographic == 1 and d.camera.ortho_height == 2.0
+    assert f.info()["unsupported"] == 0
+    # every camera ray of the orthographic camera is parallel to lookat - lookfrom
+    b = O.camera_basis(f, width=512)
+    assert np.allclose(np.linalg.norm(b["vertical"]), 4.0)
