@@ -410,6 +410,29 @@ def _device_bvh_bench(scene, cfg, args, r_host, sid_host):
     return out
 
 
+def one_render_probe(scene, cfg, args, progressive):
+    """One render of the workload with the library's default shape (no timing, no counting variant):
+    the rays and kernel launches a counter run divides its per-dispatch sums by."""
+    from cortexraytracercuda_amd import Renderer
+    r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
+                 refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
+                 sub_passes=args.sub_passes, leaf_size=args.leaf_size)
+    sid = r.upload(scene)
+    info = r.info(sid)
+    ranges = [(k, 1) for k in range(cfg["spp"])] if progressive else [(0, 0)]
+    rays = launches = exact = 0
+    for sb, sc in ranges:
+        _, _, st = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb,
+                            sample_count=sc, want_accum=False)
+        c = r.counters()
+        rays += st["rays"]
+        launches += c["extend_launches"]
+        exact += c["exact_queries"]
+    r.close()
+    return {"one_render": True, "config": cfg, "passes": len(ranges), "rays": rays, "extend_launches": launches,
+            "exact_queries": exact, "bvh_width": info["bvh_width"], "leaf_size": info["leaf_size"]}
+
+
 def single_render_bench(scene, cfg, args, workload, progressive=False, workload_key=""):
     from cortexraytracercuda_amd import Renderer
     r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
@@ -664,6 +687,9 @@ def main():
                     help="run crt_autotune on the workload before timing (default: the library's default shape)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--one-render", action="store_true",
+                    help="only upload the workload's scene and render it once (every pass of a progressive "
+                         "workload), for counter runs: every closest-hit dispatch then belongs to that render")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -702,6 +728,9 @@ def main():
     if args.quick:
         cfg = dict(cfg, width=cfg["width"] // 4, height=cfg["height"] // 4, spp=min(cfg["spp"], 16))
     log("scene generated in %.1f s" % (time.perf_counter() - t_gen))
+    if args.one_render:
+        print(json.dumps(one_render_probe(scene, cfg, args, progressive)), flush=True)
+        return
     key = "" if (args.quick or args.bvh_width or args.leaf_size) else {
         "large": "large_1080p_64spp", "box": "box_256_16spp", "instanced": "instanced_4k_16spp"}[args.config]
     out = single_render_bench(scene, cfg, args, workload, progressive, key)

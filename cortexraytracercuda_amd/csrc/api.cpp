@@ -218,7 +218,7 @@ int Context::init(const crt_config* cfg) {
 uint64_t Context::used_bytes() const {
     uint64_t b = scene_table.bytes + counters.bytes + ctl.bytes + gstack.bytes + req_table[0].bytes + req_table[1].bytes;
     for (int k = 0; k < 2; ++k) b += ray_o[k].bytes + ray_d[k].bytes + thr[k].bytes + rad[k].bytes + ids[k].bytes;
-    b += hit.bytes + hit_inst.bytes + exact_q.bytes + out_rad.bytes;
+    b += hit.bytes + hit_inst.bytes + out_rad.bytes;
     return b;
 }
 
@@ -234,7 +234,6 @@ int Context::ensure_paths(uint32_t n, uint32_t pass_ids) {
         }
         if ((rc = hit.alloc((size_t)n * 16, this))) return rc;
         if ((rc = hit_inst.alloc((size_t)n * 4, this))) return rc;
-        if ((rc = exact_q.alloc((size_t)n * 4, this))) return rc;
         path_capacity = n;
     }
     if (pass_ids > pass_capacity) {
@@ -263,8 +262,7 @@ PassBufs Context::pool_bufs() {
         B.ray_o[k] = (float4*)ray_o[k].p; B.ray_d[k] = (float4*)ray_d[k].p; B.thr[k] = (float4*)thr[k].p;
         B.rad[k] = (float4*)rad[k].p; B.ids[k] = (uint2*)ids[k].p; B.req[k] = (crt_dev_request*)req_table[k].p;
     }
-    B.hit = (float4*)hit.p; B.hit_inst = (uint32_t*)hit_inst.p; B.exact_q = (uint32_t*)exact_q.p;
-    B.out_rad = (float4*)out_rad.p;
+    B.hit = (float4*)hit.p; B.hit_inst = (uint32_t*)hit_inst.p; B.out_rad = (float4*)out_rad.p;
     B.gstack = (uint32_t*)gstack.p; B.ctl = (uint32_t*)ctl.p; B.counters = (crt_dev_counters*)counters.p;
     B.slots = std::min(max_paths, path_capacity);
     B.pass_ids = pass_capacity;
@@ -663,8 +661,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], u.s));
             if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)width, two_level, u.egrid, dsc, dreq, uniform_scene,
                              nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, hq, hiq, gq, nullptr, extend_mode,
-                             cin + CRT_CTL_WORK, extend_param, (int)stack_lds, cin + CRT_CTL_SURV, B.exact_q + u.slot0,
-                             cin + CRT_CTL_EXACT, B.counters))
+                             cin + CRT_CTL_WORK, extend_param, (int)stack_lds, cin + CRT_CTL_SURV, B.counters))
                 return set_error(CRT_ERR_HIP, "extend launch failed");
             if (time_stages) {
                 HIP_TRY(hipEventRecord(ev_ext[1], u.s));
@@ -938,10 +935,10 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
     const int two_level = sc->host.num_instances ? 1 : 0;
     if (flags & CRT_TRACE_STAGE) {
         // the render's closest-hit stage: path-state rays (origin, time 0; direction, depth 1 and the
-        // camera bit), one uniform scene, the stage's exact re-trace queue and control words
-        DevBuf dro, drd, dids, dq, dctl;
+        // camera bit), one uniform scene, the stage's control words
+        DevBuf dro, drd, dids, dctl;
         if ((rc = dro.alloc(n * 16, ctx)) || (rc = drd.alloc(n * 16, ctx)) || (rc = dids.alloc(n * 8, ctx)) ||
-            (rc = dq.alloc(n * 4, ctx)) || (rc = dctl.alloc(CRT_CTL_WORDS * 4, ctx)))
+            (rc = dctl.alloc(CRT_CTL_WORDS * 4, ctx)))
             return rc;
         std::vector<float> po(n * 4), pd(n * 4);
         const uint32_t state = 1u | (camera ? 0x10000u : 0u);   // depth 1 | camera bit of the path state
@@ -959,8 +956,7 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
                          (const crt_dev_scene*)ctx->scene_table.p, nullptr, (int32_t)sc->slot, nullptr, nullptr,
                          (uint32_t)n, (const float4*)dro.p, (const float4*)drd.p, (const uint2*)dids.p, (float4*)dh.p,
                          (uint32_t*)di.p, (uint32_t*)ctx->gstack.p, nullptr, ctx->extend_mode, ctl + CRT_CTL_WORK,
-                         ctx->extend_param, (int)ctx->stack_lds, nullptr, (uint32_t*)dq.p, ctl + CRT_CTL_EXACT,
-                         (crt_dev_counters*)ctx->counters.p))
+                         ctx->extend_param, (int)ctx->stack_lds, nullptr, (crt_dev_counters*)ctx->counters.p))
             return set_error(CRT_ERR_HIP, "extend launch failed");
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     } else {

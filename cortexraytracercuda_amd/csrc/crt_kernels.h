@@ -12,7 +12,6 @@
 #define CRT_CTL_GEN 2    // next path id of the pass to generate
 #define CRT_CTL_LIVE 3   // paths in this iteration's state buffer (survivors + new)
 #define CRT_CTL_WORK 4   // ray hand-out counter of the persistent closest-hit stage
-#define CRT_CTL_EXACT 5  // queries of this iteration queued for the exact re-trace
 
 // SoA path state of one buffer (two buffers alternate between iterations)
 struct crt_paths {
@@ -35,10 +34,7 @@ int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32
                  const crt_dev_request* reqs, int32_t uniform_scene, const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                  const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                  uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
-                 const uint32_t* first_camera, uint32_t* exact_q, uint32_t* exact_cnt, crt_dev_counters* counters);
-// exact_q / exact_cnt (required unless count_mode): queue of the queries whose triangle tests the
-// float arithmetic left undecided (a slot per live path) and its length, zero at the launch; the
-// launcher re-traces them exactly behind the stage, on the same stream
+                 const uint32_t* first_camera, crt_dev_counters* counters);
 // first_camera (nullable): queue index where the camera rays of the launch begin (the survivors of
 // the last shade stage come first); the lane-refill kernel keeps camera-ray waves whole
 // two_level: an instanced scene has paths in the launch (selects the unbounded-register variant);

@@ -72,7 +72,6 @@ struct DeviceScene {
 struct PassBufs {
     float4* ray_o[2]; float4* ray_d[2]; float4* thr[2]; float4* rad[2]; uint2* ids[2];
     float4* hit; uint32_t* hit_inst;
-    uint32_t* exact_q;              // exact re-trace queue of the closest-hit stage, a word per live slot
     float4* out_rad;                // 16 B per path id of a pass
     uint32_t* gstack;               // traversal overflow stacks of extend_grid workgroups
     uint32_t* ctl;                  // 2 * CRT_CTL_WORDS words per sub-pass
@@ -156,7 +155,7 @@ struct Server {
     bool ready = false;
     hipStream_t s = nullptr, copy_s = nullptr;
     uint32_t slots = 0, cap = 0, egrid = 0, sgrid = 0;
-    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, exact_q, out_rad, req, ctl, gstack, counters, live_min;
+    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad, req, ctl, gstack, counters, live_min;
     crt_dev_request* req_host = nullptr;   // pinned mirror of the open pass's request table
     uint32_t* open_host = nullptr;         // pinned ring of (total, nreq)
     uint32_t* snap = nullptr;              // pinned ring of iteration snapshots
@@ -210,7 +209,7 @@ struct Context {
     uint64_t passes = 0, extend_launches = 0, iterations = 0;
     // streaming pass: two compact path-state buffers of max_paths slots (alternating between
     // iterations), hit records, per-path-id radiance
-    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, exact_q, out_rad;
+    DevBuf ray_o[2], ray_d[2], thr[2], rad[2], ids[2], hit, hit_inst, out_rad;
     uint32_t pass_capacity = 0;      // path ids per pass (out_rad entries)
     static constexpr int kCtlRing = 8, kCtlLag = 4;   // control snapshots in flight / host look-behind
     static constexpr int kMaxSubPasses = 4;           // concurrent sub-passes (streams) of one pass

@@ -274,8 +274,9 @@ __device__ __forceinline__ bool tri_hit_f64(V3 o, V3 d, float4 a, float4 b, floa
 // evaluation of the same expression (some 2^29 times more accurate). The FP32 answer is therefore
 // the double answer whenever every sign is certain, or when one is certainly negative and another
 // certainly positive (a miss either way). Otherwise the ray passes within rounding distance of an
-// edge or a vertex: EXACT kernels then redo the test in double (tri_hit_f64); the others flag the
-// query (`amb`) for a re-trace by an EXACT kernel and go on as if the triangle were missed.
+// edge or a vertex: EXACT traversals then redo the test in double (tri_hit_f64); the others report
+// the test undecided (`amb`), which ends the query, and the lane re-traces it with an EXACT
+// traversal outside its traversal loop.
 template <bool EXACT>
 __device__ __forceinline__ bool tri_hit(V3 o, V3 d, const TriPre& p, float4 a, float4 b, float4 c, float tmin, float tmax,
                                         float& t, float& bu, float& bv, bool& amb) {
@@ -871,7 +872,6 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
     if (tid == 0) {
         ctl_in[CRT_CTL_LIVE] = surv + n_new;
         ctl_in[CRT_CTL_WORK] = 0u;
-        ctl_in[CRT_CTL_EXACT] = 0u;
         ctl_out[CRT_CTL_GEN] = g0 + n_new;
         ctl_out[CRT_CTL_SURV] = 0u;
     }
@@ -906,7 +906,8 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
 // t + 0.001 (core/src/cameras/camera.cuh:179-183); the result is then flagged as a pass-through
 // (returned flag CRT_PASSTHROUGH). `nrays` counts closest-hit queries, `nretrace` the re-traces.
 // A query left undecided by a non-EXACT triangle test (its result distance set to -1) returns with
-// `amb` set and counts nothing: the caller hands it to an EXACT kernel, which answers and counts it.
+// `amb` set and counts nothing: the caller re-traces it with an EXACT traversal, which answers and
+// counts it.
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool EXACT>
 __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
                                                 GSlot gst, HitRec& h, TravCount& tc, bool& ovf, uint32_t& nrays,
@@ -952,7 +953,6 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
                                                   float4* __restrict__ hit, uint32_t* __restrict__ hit_inst,
                                                   uint32_t* __restrict__ gstack, uint32_t* __restrict__ zero_counter,
                                                   uint32_t* __restrict__ work_counter, uint32_t chunk_groups,
-                                                  uint32_t* __restrict__ exact_q, uint32_t* __restrict__ exact_cnt,
                                                   crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
@@ -962,7 +962,7 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
     uint32_t* lds_col = s_stack + threadIdx.x;
     const GSlot gst{gstack, gtid * GS};
     TravCount tc = {0, 0, 0, 0, 0, 0};
-    uint32_t nrays = 0, nretrace = 0, novf = 0;
+    uint32_t nrays = 0, nretrace = 0, novf = 0, nexact = 0;
     SceneView SU;
     if (UNIFORM) SU = scene_view(scenes + uniform_scene);
     auto one_ray = [&](uint32_t i) {
@@ -980,8 +980,13 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
                                                                                           tc, ovf, nrays, nretrace, amb);
         novf += ovf ? 1u : 0u;
         if (!COUNT && amb) {
-            exact_q[atomicAdd(exact_cnt, 1u)] = slot;   // rare: a per-lane atomic
-            return;
+            // the exact re-trace of an undecided query, by its own lane (a rare, divergent branch:
+            // the double-precision code stays out of the traversal loop above)
+            bool o2 = false, amb2 = false;
+            pass_flag = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, o, d, camera, lds_col, gst, h, tc,
+                                                                                    o2, nrays, nretrace, amb2);
+            novf += o2 ? 1u : 0u;
+            ++nexact;
         }
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
@@ -1013,6 +1018,7 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
         wave_add(&counters->inst_visits, tc.insts);
     }
     wave_add(&counters->stack_overflow, novf);
+    wave_add(&counters->exact, nexact);
 }
 
 // Closest-hit stage with per-lane refill for the survivors of the last shade stage: every wave owns
@@ -1042,7 +1048,6 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
                                                      uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
                                                      uint32_t* __restrict__ zero_counter, uint32_t refill_min,
                                                      const uint32_t* __restrict__ first_camera,
-                                                     uint32_t* __restrict__ exact_q, uint32_t* __restrict__ exact_cnt,
                                                      crt_dev_counters* __restrict__ counters) {
     __shared__ uint32_t s_stack[LS * BLOCK];
     if (zero_counter && blockIdx.x == 0 && threadIdx.x == 0) *zero_counter = 0u;
@@ -1052,7 +1057,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
     uint32_t* lds_col = s_stack + threadIdx.x;
     const GSlot gst{gstack, (blockIdx.x * BLOCK + threadIdx.x) * GS};
     TravCount tc = {0, 0, 0, 0, 0, 0};
-    uint32_t nrays = 0, nretrace = 0, novf = 0;
+    uint32_t nrays = 0, nretrace = 0, novf = 0, nexact = 0;
     SceneView S;
     if (UNIFORM) S = scene_view(scenes + uniform_scene);
     Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, COUNT> tr;
@@ -1111,8 +1116,15 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
             const HitRec h = tr.best;
             novf += tr.st.overflow ? 1u : 0u;
             if (!COUNT && h.t < 0.0f) {
-                // undecided triangle test: the query goes to the exact re-trace (a rare per-lane atomic)
-                exact_q[atomicAdd(exact_cnt, 1u)] = slot;
+                // undecided triangle test: the lane re-traces the query exactly (rare, divergent)
+                HitRec h2;
+                bool o2 = false, amb2 = false;
+                const uint32_t pf = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(
+                    S, tr.ow, tr.dw, tr.camera, lds_col, gst, h2, tc, o2, nrays, nretrace, amb2);
+                novf += o2 ? 1u : 0u;
+                ++nexact;
+                hit[slot] = make_float4(h2.t, bitsf(h2.prim), h2.u, h2.v);
+                hit_inst[slot] = h2.inst | pf;
                 active = false;
                 continue;
             }
@@ -1140,44 +1152,6 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
         wave_add(&counters->quad_tests, tc.quads);
         wave_add(&counters->inst_visits, tc.insts);
     }
-    wave_add(&counters->stack_overflow, novf);
-}
-
-// Exact re-trace of the queries the closest-hit stage left undecided (a triangle test within
-// rounding distance of an edge, queued by k_extend / k_extend_pt): the same traversal with the
-// double-precision triangle test where the float one cannot decide. Runs after the stage on its
-// stream and reuses its traversal stacks; the queue is short (usually empty).
-template <int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
-__global__ __launch_bounds__(BLOCK) void k_extend_exact(const crt_dev_scene* __restrict__ scenes,
-                                                        const crt_dev_request* __restrict__ reqs, int32_t uniform_scene,
-                                                        const uint32_t* __restrict__ exact_q,
-                                                        const uint32_t* __restrict__ exact_cnt,
-                                                        const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
-                                                        const uint2* __restrict__ ids, float4* __restrict__ hit,
-                                                        uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
-                                                        crt_dev_counters* __restrict__ counters) {
-    __shared__ uint32_t s_stack[LS * BLOCK];
-    const uint32_t n = *exact_cnt;
-    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
-    if (blockIdx.x * BLOCK >= n) return;   // block-uniform: most launches find the queue empty
-    TravCount tc = {0, 0, 0, 0, 0, 0};
-    uint32_t nrays = 0, nretrace = 0, novf = 0, nexact = 0;
-    const GSlot gst{gstack, gtid * GS};
-    for (uint32_t i = gtid; i < n; i += gridDim.x * BLOCK) {
-        const uint32_t slot = exact_q[i];
-        ++nexact;
-        const float4 ro = ray_o[slot], rdv = ray_d[slot];
-        const SceneView S = scene_view(scenes + (uniform_scene >= 0 ? (uint32_t)uniform_scene : reqs[ids[slot].y >> 16].scene));
-        HitRec h;
-        bool ovf = false, amb = false;
-        const uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(
-            S, v3(ro), v3(rdv), (fbits(rdv.w) & ST_CAMERA) != 0, s_stack + threadIdx.x, gst, h, tc, ovf, nrays, nretrace, amb);
-        novf += ovf ? 1u : 0u;
-        hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
-        hit_inst[slot] = h.inst | pass_flag;
-    }
-    wave_add(&counters->rays, nrays);
-    wave_add(&counters->retraces, nretrace);
     wave_add(&counters->stack_overflow, novf);
     wave_add(&counters->exact, nexact);
 }
@@ -1793,10 +1767,8 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
                             const uint32_t* queue, const uint32_t* count_ptr, uint32_t count_direct, const float4* ray_o,
                             const float4* ray_d, const uint2* ids, float4* hit, uint32_t* hit_inst, uint32_t* gstack,
                             uint32_t* zero_counter, int mode, uint32_t* work_counter, uint32_t param, int stack_lds,
-                            const uint32_t* first_camera, uint32_t* exact_q, uint32_t* exact_cnt,
-                            crt_dev_counters* counters) {
+                            const uint32_t* first_camera, crt_dev_counters* counters) {
     uint32_t us = uniform_scene >= 0 ? (uint32_t)uniform_scene : 0u;
-    if (!count_mode && (!exact_q || !exact_cnt)) return (int)hipErrorInvalidValue;
     if (mode != CRT_EXTEND_WAVE_CHUNK) work_counter = nullptr;
     if (mode == CRT_EXTEND_WAVE_CHUNK && !work_counter) mode = CRT_EXTEND_GRID_STRIDE;
     // grid-stride / wave-chunk kernel with an LDS stack ring of LSV entries per lane
@@ -1808,7 +1780,7 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
         uint32_t g = (mode == CRT_EXTEND_WAVE_CHUNK && grid > rb) ? rb : grid;                                          \
         hipLaunchKernelGGL(kfn, dim3(g), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr, count_direct, ray_o,     \
                            ray_d, ids, hit, hit_inst, gstack, zero_counter, work_counter, param ? param : 1u,         \
-                           exact_q, exact_cnt, counters);                                                               \
+                           counters);                                                                                   \
     }
     // lane-refill kernel over per-wave shares of the queue (same grid as the grid-stride kernel)
 #define CRT_EXT_PT(C, U, W, T)                                                                                         \
@@ -1816,7 +1788,7 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
         auto kfn = k_extend_pt<256, STK_RS, STK_LS, STK_GS, C, U, W, T>;                                               \
         hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), 0, s, scenes, reqs, us, queue, count_ptr, count_direct, ray_o,  \
                            ray_d, ids, hit, hit_inst, gstack, zero_counter, param ? param : 1u, first_camera,       \
-                           exact_q, exact_cnt, counters);                                                               \
+                           counters);                                                                                   \
     }
     // counting variants: one stack depth; timed variants: LDS ring of 8, 16 or 32 entries per lane
 #define CRT_EXT_C(U, W, T) \
@@ -1833,17 +1805,6 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
         if (uniform_scene >= 0) { CRT_EXT_WT(CRT_EXT_C, true) } else { CRT_EXT_WT(CRT_EXT_C, false) }
     } else {
         if (uniform_scene >= 0) { CRT_EXT_WT(CRT_EXT_N, true) } else { CRT_EXT_WT(CRT_EXT_N, false) }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return (int)e;
-        // the exact re-trace of the undecided queries (counting variants decide them inline); its
-        // grid is within the stage's, whose traversal stacks it reuses
-        const uint32_t xg = grid < 512u ? grid : 512u;
-#define CRT_EXT_X(W, T)                                                                                              \
-    hipLaunchKernelGGL((k_extend_exact<256, STK_RS, STK_LS, STK_GS, W, T>), dim3(xg), dim3(256), 0, s, scenes, reqs, \
-                       uniform_scene, exact_q, exact_cnt, ray_o, ray_d, ids, hit, hit_inst, gstack, counters)
-        if (width == 2) { if (two_level) CRT_EXT_X(2, true); else CRT_EXT_X(2, false); }
-        else { if (two_level) CRT_EXT_X(4, true); else CRT_EXT_X(4, false); }
-#undef CRT_EXT_X
     }
 #undef CRT_EXT_WT
 #undef CRT_EXT_N

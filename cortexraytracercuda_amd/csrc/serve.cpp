@@ -114,7 +114,6 @@ int Context::server_init_lane(Server& S, uint32_t slots, uint32_t cap, bool firs
     }
     if ((rc = S.hit.alloc(n * 16, this))) return rc;
     if ((rc = S.hit_inst.alloc(n * 4, this))) return rc;
-    if ((rc = S.exact_q.alloc(n * 4, this))) return rc;
     if ((rc = S.out_rad.alloc((size_t)S.cap * 16, this))) return rc;
     if ((rc = S.req.alloc(sizeof(crt_dev_request) * Server::kEntries, this))) return rc;
     if ((rc = S.ctl.alloc(sizeof(uint32_t) * (2 * CRT_CTL_WORDS + 2), this))) return rc;
@@ -143,7 +142,7 @@ uint64_t Context::server_bytes() const {
 }
 
 uint64_t Context::server_lane_bytes(const Server& S) const {
-    uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.exact_q.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.gstack.bytes +
+    uint64_t b = S.hit.bytes + S.hit_inst.bytes + S.out_rad.bytes + S.req.bytes + S.ctl.bytes + S.gstack.bytes +
                  S.counters.bytes + S.live_min.bytes;
     for (int k = 0; k < 2; ++k) b += S.ray_o[k].bytes + S.ray_d[k].bytes + S.thr[k].bytes + S.rad[k].bytes + S.ids[k].bytes;
     return b;
@@ -294,7 +293,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
     if (crt_k_extend(S.s, 0, (int)C.bvh_width, two_level, S.egrid, dsc, dreq, -1, nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o,
                      A.ray_d, A.ids, (float4*)S.hit.p, (uint32_t*)S.hit_inst.p, (uint32_t*)S.gstack.p, nullptr,
                      C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, cin + CRT_CTL_SURV,
-                     (uint32_t*)S.exact_q.p, cin + CRT_CTL_EXACT, (crt_dev_counters*)S.counters.p))
+                     (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
     if (crt_k_shade(S.s, mode, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.live_min.p,

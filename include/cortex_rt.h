@@ -340,7 +340,7 @@ int crt_render(crt_context* ctx, crt_scene_id scene, const crt_render_desc* rd, 
  * (-1 = none), t, and the barycentric / planar coordinates u, v. `flags` (CRT_TRACE_*):
  * CRT_TRACE_CAMERA hides invisible lights as the reference does for depth == max_depth;
  * CRT_TRACE_STAGE answers the queries with the render's closest-hit stage (the configured
- * extend_kernel and its exact re-trace of undecided triangle tests) instead of the query kernel:
+ * extend_kernel, with its exact re-trace of undecided triangle tests) instead of the query kernel:
  * the interval is then the estimator's [1e-5, FLT_MAX) whatever tmin / tmax say, and a one-sided
  * light hit from behind is passed through as in the render (core/src/cameras/camera.cuh:179-183). */
 enum { CRT_TRACE_CAMERA = 1, CRT_TRACE_STAGE = 2 };

@@ -3,9 +3,12 @@ reference tree exist; the tests only read the JSON files).
 
   thrust_kat.json : rocThrust minstd_rand / uniform distributions (tests/golden/thrust_kat.hip)
   pcg32_kat.json  : pcg32 from the reference's vendored libs/pcg (oracle/_ref/pcg_kat)
+  scenes/shipped_*.scene : input data: the reference's self-contained scene files (no mesh or
+                    texture files) copied byte for byte, rendered by the GPU tests
 """
 import json
 import os
+import shutil
 import subprocess
 import tempfile
 
@@ -35,6 +38,15 @@ def pcg():
         json.dump({"source": "libs/pcg/pcg_random.hpp pcg32", "cases": cases}, f)
 
 
+SHIPPED = ("all_lights_types.scene", "simple_spot_light.scene")
+
+
+def shipped_scenes():
+    for name in SHIPPED:
+        shutil.copyfile(os.path.join("/root/reference/data/scenes", name), os.path.join(HERE, "scenes", "shipped_" + name))
+
+
 if __name__ == "__main__":
     thrust()
     pcg()
+    shipped_scenes()

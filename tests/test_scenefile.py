@@ -229,3 +229,26 @@ def test_camera_defaults_and_orthographic_camera(tmp_path):
     # every camera ray of the orthographic camera is parallel to lookat - lookfrom
     b = O.camera_basis(f, width=512)
     assert np.allclose(np.linalg.norm(b["vertical"]), 4.0)
+
+
+SHIPPED = ["all_lights_types.scene", "simple_spot_light.scene"]
+
+
+@pytest.mark.parametrize("name", SHIPPED)
+def test_shipped_scene_fixtures(name):
+    # the copies the GPU tests render are the reference's files byte for byte and read in full:
+    # all_lights_types has its quad light only (spot and omni are active = false), simple_spot_light
+    # a spot light between four boxes (ground, roof, two walls) of 12 triangles each
+    path = os.path.join(HERE, "golden", "scenes", "shipped_" + name)
+    src = os.path.join("/root/reference/data/scenes", name)
+    if os.path.exists(src):
+        with open(src, "rb") as a, open(path, "rb") as b:
+            assert a.read() == b.read()
+    f = SceneFile(path)
+    assert f.info()["unsupported"] == 0 and f.info()["skipped"] == 0
+    d, _ = f.to_desc()
+    if name == "all_lights_types.scene":
+        assert (d.num_spheres, d.num_lights, d.num_sphere_lights) == (7, 1, 0)
+    else:
+        assert (d.num_spheres, d.num_lights, d.num_sphere_lights) == (2, 0, 1)
+        assert sum(d.meshes[k].num_triangles for k in range(d.num_meshes)) == 4 * 12
