@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcortex_rt.so")
-SOURCES = ["kernels.hip", "bvh_gpu.hip", "scene_build.cpp", "api.cpp", "serve.cpp", "tune.cpp",
+SOURCES = ["kernels.hip", "bvh_gpu.hip", "noalloc.hip", "scene_build.cpp", "api.cpp", "serve.cpp", "tune.cpp",
            "obj_loader.cpp", "scene_builder.cpp", "scenefile.cpp", "image_io.cpp"]
 CLI = os.path.join(HERE, "crt_render")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

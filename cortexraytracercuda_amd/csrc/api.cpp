@@ -25,6 +25,7 @@
 #include "crt_kernels.h"
 #include "crt_layout.h"
 #include "crt_runtime.h"
+#include "crt_scratch.h"
 #include "scene_build.h"
 
 namespace crt {
@@ -212,6 +213,48 @@ int Context::init(const crt_config* cfg) {
     uint64_t pp = config.pass_paths ? config.pass_paths : std::min<uint64_t>(1ull << 27, 16ull * max_paths);
     if ((rc = ensure_paths(max_paths, (uint32_t)std::max<uint64_t>(pp, max_paths)))) return rc;
     reserved = used_bytes();
+    // the no-allocation render's scratch array: its device address on this device (current here)
+    hipError_t se = (hipError_t)crt_scratch_setup(device, &scratch, &scratch_bytes);
+    if (se != hipSuccess)
+        return set_error(CRT_ERR_HIP, std::string("scratch array address: ") + hipGetErrorString(se));
+    return CRT_OK;
+}
+
+// Carves the pass buffers of crt_render_noalloc out of the device's scratch array: one sub-pass,
+// kCrtScratchSlots live paths, kCrtScratchPassIds path ids per pass, closest-hit overflow stacks for
+// kCrtScratchExtendPerCu workgroups per CU (fewer if the device's CUs would not fit).
+int Context::scratch_bufs(PassBufs& B) {
+    if (!scratch) return set_error(CRT_ERR_INTERNAL, "no scratch array on this device");
+    const uint64_t S = kCrtScratchSlots, P = kCrtScratchPassIds;
+    const uint64_t per_block_stack = 256ull * crt_k_gstack_entries() * sizeof(uint32_t);
+    uint64_t off = 0;
+    char* base = (char*)scratch;
+    auto take = [&](uint64_t bytes) -> void* {
+        void* p = base + off;
+        off = (off + bytes + 255u) & ~(uint64_t)255u;
+        return p;
+    };
+    for (int k = 0; k < 2; ++k) {
+        B.ray_o[k] = (float4*)take(S * 16); B.ray_d[k] = (float4*)take(S * 16); B.thr[k] = (float4*)take(S * 16);
+        B.rad[k] = (float4*)take(S * 16); B.ids[k] = (uint2*)take(S * 8);
+    }
+    B.hit = (float4*)take(S * 16);
+    B.hit_inst = (uint32_t*)take(S * 4);
+    B.out_rad = (float4*)take(P * 16);
+    B.ctl = (uint32_t*)take(2 * CRT_CTL_WORDS * sizeof(uint32_t));
+    B.counters = (crt_dev_counters*)take(sizeof(crt_dev_counters));
+    B.req[0] = (crt_dev_request*)take(sizeof(crt_dev_request));
+    B.req[1] = (crt_dev_request*)take(sizeof(crt_dev_request));
+    if (off >= scratch_bytes) return set_error(CRT_ERR_INTERNAL, "scratch array too small for the pass state");
+    const uint64_t fit = (scratch_bytes - off) / per_block_stack;
+    uint64_t grid = std::min<uint64_t>((uint64_t)kCrtScratchExtendPerCu * (uint64_t)num_cus, fit);
+    if (grid == 0) return set_error(CRT_ERR_INTERNAL, "scratch array too small for the traversal stacks");
+    B.gstack = (uint32_t*)take(grid * per_block_stack);
+    if (off > scratch_bytes) return set_error(CRT_ERR_INTERNAL, "scratch layout exceeds the scratch array");
+    B.slots = (uint32_t)S;
+    B.pass_ids = (uint32_t)P;
+    B.extend_grid = (uint32_t)grid;
+    B.max_sub = 1;
     return CRT_OK;
 }
 
@@ -539,6 +582,24 @@ int Job::setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStrea
     return CRT_OK;
 }
 
+void Job::setup_external(DeviceScene* sc, const crt_render_desc& r, float* accum_dev) {
+    rd = r;
+    scene = sc;
+    npix = r.width * r.height;
+    sq = (uint32_t)std::sqrt((double)r.spp);
+    nsamples = sq * sq;
+    s_begin = std::min(r.sample_begin, nsamples);
+    s_end = r.sample_count ? std::min(nsamples, s_begin + r.sample_count) : nsamples;
+    s_next = s_begin;
+    p_next = 0;
+    std::memset(&req, 0, sizeof(req));
+    camera_setup(sc->host.camera, r.width, req);
+    req.width = r.width; req.height = r.height; req.sqrt_spp = sq; req.spp = r.spp; req.max_depth = r.max_depth;
+    req.rng_mode = r.rng_mode; req.scene = sc->slot; req.seed = r.seed;
+    req.accum = accum_dev;
+    req.rng_state = nullptr;
+}
+
 // Takes up to `cap` path slots of this job's remaining work (sample-major). MINSTD jobs take at
 // most one sample per pass so each pixel's engine advances sample after sample.
 bool Job::take(uint32_t cap, WorkItem& w) {
@@ -814,6 +875,46 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
     return CRT_OK;
 }
 
+// True when `p` is device memory of `device` (hipMalloc'd or managed), as the no-allocation render
+// needs for its caller-owned outputs.
+static bool device_pointer_on(const void* p, int device) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+    if (a.isManaged) return true;
+    return a.type == hipMemoryTypeDevice && a.device == device;
+}
+
+// crt_render on the device's scratch array: no device allocation and no allocating library call.
+// The caller owns the outputs (device memory); the scene is resident. Keyed PCG streams only (the
+// reference stream would need per-pixel engine state).
+int Context::render_noalloc(crt_scene_id scene, const crt_render_desc* rd, float* accum_dev, uint8_t* rgb_dev,
+                            crt_stats* stats) {
+    std::lock_guard<std::mutex> g(gpu_mutex);
+    int rc = Job::validate(*rd);
+    if (rc) return rc;
+    if (rd->rng_mode != CRT_RNG_PCG)
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "crt_render_noalloc renders keyed PCG streams only");
+    if (rd->flags & CRT_RENDER_MEGAKERNEL)
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "crt_render_noalloc runs the wavefront passes only");
+    if (!accum_dev) return set_error(CRT_ERR_INVALID_ARGUMENT, "accumulation buffer is null");
+    if (!device_pointer_on(accum_dev, device) || (rgb_dev && !device_pointer_on(rgb_dev, device)))
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "outputs must be device memory of the context's device");
+    DeviceScene* sc = find_scene(scene);
+    if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+    PassBufs B;
+    if ((rc = scratch_bufs(B))) return rc;
+    Job j;
+    j.setup_external(sc, *rd, accum_dev);
+    if (crt_k_init_request(stream, (float4*)accum_dev, nullptr, rd->width, rd->height, rd->seed))
+        return set_error(CRT_ERR_HIP, "init kernel launch failed");
+    std::vector<Job*> jobs{&j};
+    if ((rc = render_jobs(jobs, (rd->flags & CRT_RENDER_TIME_STAGES) != 0, stats, B))) return rc;
+    if (rgb_dev && crt_k_tonemap(stream, (const float4*)accum_dev, j.npix, rd->spp, rgb_dev))
+        return set_error(CRT_ERR_HIP, "tonemap launch failed");
+    HIP_TRY(hipStreamSynchronize(stream));
+    return CRT_OK;
+}
+
 }  // namespace crt
 
 using namespace crt;
@@ -821,6 +922,12 @@ using namespace crt;
 extern "C" {
 
 const char* crt_last_error(void) { return g_last_error.c_str(); }
+
+int crt_render_noalloc(crt_context* c, crt_scene_id s, const crt_render_desc* rd, float* accum_dev, uint8_t* rgb_dev,
+                       crt_stats* stats) {
+    if (!c || !rd) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    return reinterpret_cast<Context*>(c)->render_noalloc(s, rd, accum_dev, rgb_dev, stats);
+}
 
 int crt_device_count(int32_t* count) {
     int n = 0;

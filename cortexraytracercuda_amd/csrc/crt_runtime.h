@@ -68,7 +68,8 @@ struct DeviceScene {
     std::atomic<int> refs{0};
 };
 
-// Device buffers a streaming pass runs on (the context's pools for crt_render and crt_render_batch).
+// Device buffers a streaming pass runs on: the context's pools (crt_render, crt_render_batch) or
+// the fixed scratch array of crt_render_noalloc.
 struct PassBufs {
     float4* ray_o[2]; float4* ray_d[2]; float4* thr[2]; float4* rad[2]; uint2* ids[2];
     float4* hit; uint32_t* hit_inst;
@@ -110,6 +111,7 @@ struct Job {
     static uint64_t bytes_needed(const crt_render_desc& r);
     int setup(Context* ctx, DeviceScene* sc, const crt_render_desc& r, hipStream_t st = nullptr);
     // a job whose accumulation buffer the caller owns (device memory): no allocation
+    void setup_external(DeviceScene* sc, const crt_render_desc& r, float* accum_dev);
     bool take(uint32_t cap, WorkItem& w);
     bool scheduled_all() const { return s_next >= s_end; }
 };
@@ -273,6 +275,12 @@ struct Context {
     int reset_counters(crt_dev_counters* c);
     int read_counters(crt_dev_counters& c, const crt_dev_counters* dev);
     int render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* stats, const PassBufs& B);
+    // no-allocation render (crt_render_noalloc): passes run on the device's scratch array
+    void* scratch = nullptr;         // this device's scratch array (hipGetSymbolAddress at init)
+    size_t scratch_bytes = 0;
+    int scratch_bufs(PassBufs& B);
+    int render_noalloc(crt_scene_id scene, const crt_render_desc* rd, float* accum_dev, uint8_t* rgb_dev,
+                       crt_stats* stats);
     // serving
     int submit(crt_scene_id scene, const crt_render_desc* rd, crt_request_id* out);
     void server_loop();

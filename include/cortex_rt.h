@@ -335,6 +335,20 @@ int crt_scene_build_check(const crt_scene_desc* desc, uint32_t leaf_size, uint32
 int crt_render(crt_context* ctx, crt_scene_id scene, const crt_render_desc* rd, crt_image* out,
                crt_stats* stats);
 
+/* No counterpart in the reference: crt_render for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It makes no
+ * device allocation and calls no allocating library routine: its passes run in a fixed scratch
+ * array that every GPU the library runs on holds for the life of the process (128 MiB; 256K live
+ * paths, 1M path ids per pass, so a larger render takes more passes, never more memory). The scene
+ * must be resident (crt_scene_upload). The caller owns the outputs, in device memory of the
+ * context's device: accum_dev holds width*height*4 floats (rgb sums, sample count; zeroed here) and
+ * rgb_dev (optional) width*height*3 bytes. The image equals crt_render's bit for bit. Keyed PCG
+ * streams only (CRT_ERR_INVALID_ARGUMENT for CRT_RNG_MINSTD, CRT_RENDER_MEGAKERNEL or host
+ * pointers). Blocking. Calls on one context are serialised; calls from different contexts on one
+ * GPU share its scratch array and race. */
+int crt_render_noalloc(crt_context* ctx, crt_scene_id scene, const crt_render_desc* rd, float* accum_dev,
+                       uint8_t* rgb_dev, crt_stats* stats);
+
 /* Closest-hit queries for arbitrary rays (first-hit parity). rays: n x 8 floats
  * (ox,oy,oz,tmin, dx,dy,dz,tmax). Outputs per ray: global primitive id (-1 = miss), instance
  * (-1 = none), t, and the barycentric / planar coordinates u, v. `flags` (CRT_TRACE_*):
