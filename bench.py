@@ -410,9 +410,32 @@ def _device_bvh_bench(scene, cfg, args, r_host, sid_host):
     return out
 
 
+def valu_probe(iters=100000):
+    """Wave64 VALU instructions per second of independent v_fma_f32, v_add_u32 and v_mul_f32 chains
+    on every SIMD (2, 4 and 8 waves per SIMD), and of v_fma_f32 with part of each wave active (16
+    lanes in one group or spread over all four, 32, 48 or 63 lanes): the issue rate the closest-hit
+    kernel's VALU count is priced against, and how it depends on the active lanes."""
+    import ctypes as C
+    from cortexraytracercuda_amd import Renderer
+    out = {}
+    with Renderer() as r:
+        for kind, name in ((0, "v_fma_f32"), (1, "v_add_u32"), (2, "v_mul_f32"), (3, "v_fma_f32_lanes0_15"),
+                           (4, "v_fma_f32_every4th_lane"), (5, "v_fma_f32_lanes0_31"), (6, "v_fma_f32_lanes0_62"),
+                           (7, "v_fma_f32_lanes0_47")):
+            for wps in (2, 4, 8):
+                ms, n = C.c_double(), C.c_uint64()
+                rc = r.lib.crt_debug_valu_probe(r.h, kind, wps, iters, C.byref(ms), C.byref(n))
+                if rc:
+                    raise RuntimeError("valu probe failed: %d" % rc)
+                out["%s_%dwps" % (name, wps)] = {"ms": ms.value, "wave_insts": n.value,
+                                                 "ginst_s": n.value / (ms.value * 1e-3) / 1e9}
+    return {"valu_probe": out}
+
+
 def one_render_probe(scene, cfg, args, progressive):
-    """One render of the workload with the library's default shape (no timing, no counting variant):
-    the rays and kernel launches a counter run divides its per-dispatch sums by."""
+    """One render of the workload with the library's default shape, in the form the roofline times
+    (stages timed with HIP events: one sub-pass, the launches of traversal_roofline; no counting
+    variant): the rays and kernel launches a counter run divides its per-dispatch sums by."""
     from cortexraytracercuda_amd import Renderer
     r = Renderer(bvh_width=args.bvh_width, max_paths=args.max_paths, extend_kernel=args.extend_kernel,
                  refill_min=args.refill_min, grid_per_cu=args.grid_per_cu, shade_grid_per_cu=args.shade_grid_per_cu,
@@ -423,7 +446,7 @@ def one_render_probe(scene, cfg, args, progressive):
     rays = launches = exact = 0
     for sb, sc in ranges:
         _, _, st = r.render(sid, cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], seed=1, sample_begin=sb,
-                            sample_count=sc, want_accum=False)
+                            sample_count=sc, want_accum=False, time_stages=True)
         c = r.counters()
         rays += st["rays"]
         launches += c["extend_launches"]
@@ -687,6 +710,8 @@ def main():
                     help="run crt_autotune on the workload before timing (default: the library's default shape)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the VALU issue-rate probe (calibrates the closest-hit kernel's VALU roofline)")
     ap.add_argument("--one-render", action="store_true",
                     help="only upload the workload's scene and render it once (every pass of a progressive "
                          "workload), for counter runs: every closest-hit dispatch then belongs to that render")
@@ -700,6 +725,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 3}), flush=True)
+        return
+    if args.valu_probe:
+        print(json.dumps(valu_probe()), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and this path is single-GPU "

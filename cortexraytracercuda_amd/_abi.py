@@ -148,7 +148,7 @@ class crt_config(C.Structure):
                 ("bvh_width", C.c_uint32), ("extend_kernel", C.c_uint32), ("refill_min", C.c_uint32),
                 ("pass_paths", C.c_uint32), ("grid_per_cu", C.c_uint32), ("shade_grid_per_cu", C.c_uint32),
                 ("sub_passes", C.c_uint32), ("graphs", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("ray_sort", C.c_uint32), ("bvh_builder", C.c_uint32)]
+                ("ray_sort", C.c_uint32), ("bvh_builder", C.c_uint32), ("batch_group_paths", C.c_uint32)]
 
 
 class crt_scene_info(C.Structure):
@@ -205,7 +205,7 @@ EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "
             "crt_scene_get_info", "crt_scene_verify", "crt_scene_build_check", "crt_render", "crt_trace_rays", "crt_trace_primary", "crt_render_batch",
             "crt_submit", "crt_poll", "crt_wait", "crt_cancel", "crt_request_get_timing", "crt_last_counters",
             "crt_memory_get_info", "crt_obj_load", "crt_obj_free", "crt_scenefile_load",
-            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes", "crt_autotune", "crt_debug_fail_alloc",
+            "crt_scenefile_desc", "crt_scenefile_close", "crt_struct_sizes", "crt_autotune", "crt_debug_fail_alloc", "crt_debug_valu_probe",
             "crt_render_noalloc", "crt_builder_create", "crt_builder_destroy", "crt_builder_camera",
             "crt_builder_background", "crt_builder_solid_texture", "crt_builder_checker_texture",
             "crt_builder_image_texture", "crt_builder_image_texture_pixels", "crt_builder_lambertian", "crt_builder_metal",
@@ -274,6 +274,7 @@ _SIGS = {
     "crt_struct_sizes": ([_P(C.c_uint64), C.c_uint32], C.c_int),
     "crt_autotune": ([C.c_void_p, C.c_uint32, _P(crt_scene_desc), _P(crt_render_desc), _P(crt_tune_result)], C.c_int),
     "crt_debug_fail_alloc": ([C.c_void_p, C.c_int32], C.c_int),
+    "crt_debug_valu_probe": ([C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, _P(C.c_double), _P(C.c_uint64)], C.c_int),
     "crt_render_noalloc": ([C.c_void_p, C.c_uint32, _P(crt_render_desc), C.c_void_p, C.c_void_p, _P(crt_stats)], C.c_int),
 }
 

@@ -178,6 +178,8 @@ int Context::init(const crt_config* cfg) {
     ray_sort = config.ray_sort == 1u ? 1u : 0u;
     if (config.bvh_builder > CRT_BVH_DEVICE) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown bvh_builder");
     bvh_builder = config.bvh_builder ? config.bvh_builder : (uint32_t)CRT_BVH_HOST;
+    batch_group = config.batch_group_paths == 0xffffffffu ? UINT64_MAX
+                : config.batch_group_paths ? (uint64_t)config.batch_group_paths : (1ull << 22);
     if (stack_lds != 8 && stack_lds != 16 && stack_lds != 32)
         return set_error(CRT_ERR_INVALID_ARGUMENT, "stack_lds must be 8, 16 or 32");
     grid_per_cu = config.grid_per_cu ? std::min(config.grid_per_cu, 256u) : 24u;
@@ -345,7 +347,7 @@ Context::~Context() {
 
 // ------------------------------------------------------------------ scenes
 int Context::upload_scene(const crt_scene_desc* desc, crt_scene_id* out) {
-    std::lock_guard<std::mutex> g(gpu_mutex);
+    ExecLock g(*this);
     return upload_scene_locked(desc, leaf_size, bvh_width, out);
 }
 
@@ -365,7 +367,7 @@ static uint64_t device_build_bound(const crt_scene_desc* d, uint32_t width, uint
            ns * 24u + ((uint64_t)d->num_lights + d->num_sphere_lights) * 4u + tables + 4 * 256u;
 }
 
-// Builds and uploads a scene with the given BVH shape (caller holds gpu_mutex): on the host (binned
+// Builds and uploads a scene with the given BVH shape (caller holds both context locks): on the host (binned
 // SAH, one upload), or on the device when the context asks for it and the scene is one-level with
 // two primitives or more.
 int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint32_t width, crt_scene_id* out) {
@@ -463,7 +465,7 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
 }
 
 // Reads the scene's BVH and records back into a HostScene and checks its invariants (see
-// crt_scene_verify). Caller holds gpu_mutex.
+// crt_scene_verify). Caller holds both context locks.
 int Context::verify_scene(const DeviceScene& sc, std::string& err) {
     const HostScene& H = sc.host;
     HostScene V;
@@ -515,7 +517,7 @@ int Context::verify_scene(const DeviceScene& sc, std::string& err) {
 }
 
 int Context::release_scene(crt_scene_id id) {
-    std::lock_guard<std::mutex> g(gpu_mutex);
+    ExecLock g(*this);
     return release_scene_locked(id);
 }
 
@@ -836,12 +838,22 @@ int Context::render_jobs(std::vector<Job*>& jobs, bool time_stages, crt_stats* s
         paths += (uint64_t)j->npix * (j->s_end - j->s_next);
         if ((rc = run_plain(*j, time_stages, &ext_ms))) return rc;
     }
+    // Passes of one scene run the single-scene traversal kernel and keep one tree in the caches.
+    // A request with at least batch_group path ids left leads a pass of its own scene (with every
+    // other request on that scene); the smaller remainders share passes in request order.
+    auto mine = [](const Job* j, int mode) {
+        return (int)j->rd.rng_mode == mode && !(j->rd.flags & CRT_RENDER_MEGAKERNEL) && !j->scheduled_all();
+    };
     for (int mode = 0; mode < 2; ++mode) {
         for (;;) {
             std::vector<WorkItem> items;
             uint32_t cap = B.pass_ids;
+            const DeviceScene* lead = nullptr;
+            for (Job* j : jobs)
+                if (mine(j, mode) && j->paths_left() >= batch_group) { lead = j->scene; break; }
             for (Job* j : jobs) {
-                if ((int)j->rd.rng_mode != mode || (j->rd.flags & CRT_RENDER_MEGAKERNEL)) continue;
+                if (!mine(j, mode)) continue;
+                if (lead ? j->scene != lead : j->paths_left() >= batch_group) continue;
                 // at most one work item per request and pass: a pixel's samples are resolved by one
                 // thread in sample order, so two items of one request never race on a pixel
                 if (cap > 0 && items.size() < kMaxPassRequests) {
@@ -889,7 +901,7 @@ static bool device_pointer_on(const void* p, int device) {
 // reference stream would need per-pixel engine state).
 int Context::render_noalloc(crt_scene_id scene, const crt_render_desc* rd, float* accum_dev, uint8_t* rgb_dev,
                             crt_stats* stats) {
-    std::lock_guard<std::mutex> g(gpu_mutex);
+    std::lock_guard<std::mutex> ex(exec_mutex);
     int rc = Job::validate(*rd);
     if (rc) return rc;
     if (rd->rng_mode != CRT_RNG_PCG)
@@ -899,7 +911,11 @@ int Context::render_noalloc(crt_scene_id scene, const crt_render_desc* rd, float
     if (!accum_dev) return set_error(CRT_ERR_INVALID_ARGUMENT, "accumulation buffer is null");
     if (!device_pointer_on(accum_dev, device) || (rgb_dev && !device_pointer_on(rgb_dev, device)))
         return set_error(CRT_ERR_INVALID_ARGUMENT, "outputs must be device memory of the context's device");
-    DeviceScene* sc = find_scene(scene);
+    DeviceScene* sc = nullptr;
+    {
+        std::lock_guard<std::mutex> g(gpu_mutex);
+        sc = find_scene(scene);
+    }
     if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
     PassBufs B;
     if ((rc = scratch_bufs(B))) return rc;
@@ -994,21 +1010,28 @@ int crt_render(crt_context* c, crt_scene_id s, const crt_render_desc* rd, crt_im
 int crt_render_batch(crt_context* c, crt_batch_item* items, uint32_t n, crt_stats* stats) {
     if (!c || (!items && n)) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
     Context* ctx = reinterpret_cast<Context*>(c);
-    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
-    uint64_t need = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        if (!ctx->find_scene(items[k].scene)) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
-        int rc = Job::validate(items[k].rd);
-        if (rc) return rc;
-        need += Job::bytes_needed(items[k].rd);
+    // the whole render holds exec_mutex (the context stream and pass buffers); the scene table and
+    // the budget are read under gpu_mutex, so the serving loop keeps admitting meanwhile. Scenes
+    // stay resident: only release_scene frees one, and it takes exec_mutex.
+    std::lock_guard<std::mutex> ex(ctx->exec_mutex);
+    std::vector<DeviceScene*> scs(n);
+    {
+        std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+        uint64_t need = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            if (!(scs[k] = ctx->find_scene(items[k].scene))) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
+            int rc = Job::validate(items[k].rd);
+            if (rc) return rc;
+            need += Job::bytes_needed(items[k].rd);
+        }
+        if (ctx->scene_bytes + ctx->request_bytes + ctx->reserved + need > ctx->budget)
+            return set_error(CRT_ERR_OVER_BUDGET, "requests exceed the HBM budget");
     }
-    if (ctx->scene_bytes + ctx->request_bytes + ctx->reserved + need > ctx->budget)
-        return set_error(CRT_ERR_OVER_BUDGET, "requests exceed the HBM budget");
     std::vector<std::unique_ptr<Job>> jobs;
     std::vector<Job*> jp;
     for (uint32_t k = 0; k < n; ++k) {
         auto j = std::make_unique<Job>();
-        int rc = j->setup(ctx, ctx->find_scene(items[k].scene), items[k].rd);
+        int rc = j->setup(ctx, scs[k], items[k].rd);
         if (rc) return rc;
         jp.push_back(j.get());
         jobs.push_back(std::move(j));
@@ -1028,7 +1051,7 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
     if (!c || (!rays && n)) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
     if (n > (1ull << 30)) return set_error(CRT_ERR_INVALID_ARGUMENT, "too many rays");
     Context* ctx = reinterpret_cast<Context*>(c);
-    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    ExecLock g(*ctx);
     DeviceScene* sc = ctx->find_scene(s);
     if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
     if (n == 0) return CRT_OK;
@@ -1139,7 +1162,7 @@ int crt_trace_primary(crt_context* c, crt_scene_id s, uint32_t W, uint32_t H, in
 int crt_scene_verify(crt_context* c, crt_scene_id s) {
     if (!c) return set_error(CRT_ERR_INVALID_ARGUMENT, "null context");
     Context* ctx = reinterpret_cast<Context*>(c);
-    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    ExecLock g(*ctx);
     DeviceScene* sc = ctx->find_scene(s);
     if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
     std::string err;
@@ -1169,7 +1192,7 @@ int crt_scene_build_check(const crt_scene_desc* d, uint32_t leaf_size, uint32_t 
 int crt_last_counters(crt_context* c, crt_work_counters* out) {
     if (!c || !out) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
     Context* ctx = reinterpret_cast<Context*>(c);
-    std::lock_guard<std::mutex> g(ctx->gpu_mutex);
+    std::lock_guard<std::mutex> g(ctx->exec_mutex);
     *out = ctx->last;
     return CRT_OK;
 }
@@ -1187,6 +1210,37 @@ int crt_memory_get_info(crt_context* c, crt_memory_info* out) {
 int crt_debug_fail_alloc(crt_context* c, int32_t n) {
     if (!c) return set_error(CRT_ERR_INVALID_ARGUMENT, "null context");
     reinterpret_cast<Context*>(c)->fail_alloc_after.store(n < 0 ? -1 : n);
+    return CRT_OK;
+}
+
+int crt_debug_valu_probe(crt_context* c, int32_t kind, uint32_t waves_per_simd, uint32_t iters, double* ms,
+                         uint64_t* wave_insts) {
+    if (!c || !ms || !wave_insts) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
+    if (kind < 0 || kind > 7 || waves_per_simd < 1 || waves_per_simd > 8 || iters < 1 || iters > (1u << 24))
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "probe kind 0-7, 1-8 waves per SIMD, 1-2^24 rounds");
+    Context* ctx = reinterpret_cast<Context*>(c);
+    ExecLock g(*ctx);
+    // four SIMDs per CU, four waves per 256-lane workgroup: waves_per_simd workgroups per CU
+    const uint32_t blocks = ctx->num_cus * waves_per_simd;
+    DevBuf out;
+    int rc;
+    if ((rc = out.alloc((size_t)blocks * 256 * sizeof(float), ctx))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) { (void)hipEventDestroy(e0); HIP_TRY(e); }
+    rc = crt_k_valu_probe(ctx->stream, kind, blocks, iters, (float*)out.p);
+    if (!rc) { (void)hipEventRecord(e0, ctx->stream); rc = crt_k_valu_probe(ctx->stream, kind, blocks, iters, (float*)out.p); }
+    (void)hipEventRecord(e1, ctx->stream);
+    float t = 0.0f;
+    hipError_t se = hipEventSynchronize(e1);
+    if (!rc && se == hipSuccess) se = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return set_error(CRT_ERR_HIP, "probe launch failed");
+    HIP_TRY(se);
+    *ms = t;
+    *wave_insts = (uint64_t)blocks * 4u * iters * 16u;
     return CRT_OK;
 }
 

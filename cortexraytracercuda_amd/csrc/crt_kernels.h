@@ -54,6 +54,10 @@ int crt_k_resolve(hipStream_t s, const crt_dev_request* reqs, int nreq, uint32_t
 int crt_k_tonemap(hipStream_t s, const float4* accum, uint32_t npix, uint32_t spp, uint8_t* rgb);
 int crt_k_init_request(hipStream_t s, float4* accum, uint32_t* rng_state, uint32_t W, uint32_t H, uint64_t seed);
 int crt_k_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
+// VALU issue-rate calibration (kind 0: v_fma_f32, 1: v_add_u32, 2: v_mul_f32; 3-7: v_fma_f32 on
+// lanes 0-15, every fourth lane, lanes 0-31, 0-62, 0-47 of each wave): blocks x 256 lanes, iters
+// rounds of 16 wave instructions; one float per lane written to out
+int crt_k_valu_probe(hipStream_t s, int kind, uint32_t blocks, uint32_t iters, float* out);
 uint32_t crt_k_stack_capacity(void);
 uint32_t crt_k_gstack_entries(void);
 }

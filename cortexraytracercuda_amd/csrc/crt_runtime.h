@@ -114,6 +114,8 @@ struct Job {
     void setup_external(DeviceScene* sc, const crt_render_desc& r, float* accum_dev);
     bool take(uint32_t cap, WorkItem& w);
     bool scheduled_all() const { return s_next >= s_end; }
+    // path ids not yet handed to a pass
+    uint64_t paths_left() const { return s_next >= s_end ? 0 : (uint64_t)(s_end - s_next) * npix - p_next; }
 };
 
 // Serving request (continuous batching)
@@ -196,8 +198,14 @@ struct Context {
     uint32_t stack_lds = 16;         // LDS ring entries per lane of the traversal stack (8, 16, 32)
     uint32_t ray_sort = 0;           // shade groups each block's survivors by direction octant
     uint32_t bvh_builder = CRT_BVH_HOST;   // where crt_scene_upload builds the BVH (CRT_BVH_*)
+    uint64_t batch_group = 1ull << 22;     // path ids left for a batched request to get passes of its own scene
     uint32_t grid_per_cu = 16, shade_grid_per_cu = 16;
-    std::mutex gpu_mutex;            // serialises every use of the stream and the pass buffers
+    // Lock order: exec_mutex before gpu_mutex. exec_mutex serialises the blocking calls (their
+    // stream, pass buffers and `last` counters) and is held for a whole render; gpu_mutex guards
+    // the scene table, the budget figures and the serving loop's set-up and is held briefly, so a
+    // request can be submitted while a blocking render runs.
+    std::mutex exec_mutex;
+    std::mutex gpu_mutex;
     std::map<crt_scene_id, std::unique_ptr<DeviceScene>> scenes;
     bool slot_used[kMaxScenes] = {};
     crt_scene_id next_scene_id = 1;
@@ -300,10 +308,16 @@ struct Context {
     void server_drop_graphs();
 };
 
+// Both context locks in their order (blocking calls that also change or walk the scene table).
+struct ExecLock {
+    std::lock_guard<std::mutex> e, g;
+    explicit ExecLock(Context& c) : e(c.exec_mutex), g(c.gpu_mutex) {}
+};
+
 void fill_stats(crt_stats* st, const crt_dev_counters& c, uint64_t paths, double gpu_ms, double ext_ms, double host_ms);
 
 // Device BVH build of a one-level scene (bvh_gpu.hip): fills the scene's allocations, its device
-// record, id maps and statistics. Caller holds gpu_mutex and has checked the budget.
+// record, id maps and statistics. Caller holds both context locks and has checked the budget.
 int gpu_build_scene(Context& C, const crt_scene_desc* d, uint32_t leaf, uint32_t width, DeviceScene& sc, std::string& err);
 
 }  // namespace crt

@@ -432,11 +432,15 @@ __device__ __forceinline__ SceneView scene_view(const crt_dev_scene* sc) {
 
 struct TravCount { uint32_t nodes, leaves, tris, spheres, quads, insts; };
 
-// Slab test of one child box on (lo - o) * inv_d; Ize's robust factor widens the far distance. A
-// box entered at +inf is a miss: an empty child slot (+inf planes) gives near = +inf and, for a ray
-// with nothing hit yet (tmax = FLT_MAX), far * 1.0000004 overflows to +inf as well, so the
-// comparison alone would accept it and the traversal would follow the empty reference. (The
-// four-wide test below needs no such term: its callers turn a hit at +inf into a miss.)
+// Slab test of one child box on (lo - o) * inv_d, Ize's robust traversal: each slab distance is
+// within a relative 4u of the exact one (u = 2^-24: the subtraction, the hardware reciprocal of at
+// most 1 ulp, the product), so near <= far * (1 + 8u) plus the rounding of that product holds for
+// every box the ray meets; CRT_BOX_FAR = 1 + 5 * 2^-23 covers it. A box entered at +inf is a miss:
+// an empty child slot (+inf planes) gives near = +inf and, for a ray with nothing hit yet (tmax =
+// FLT_MAX), far * CRT_BOX_FAR overflows to +inf as well, so the comparison alone would accept it
+// and the traversal would follow the empty reference. (The four-wide test below needs no such
+// term: its callers turn a hit at +inf into a miss.)
+#define CRT_BOX_FAR 1.0000006f
 __device__ __forceinline__ float box_near(float lx, float hx, float ly, float hy, float lz, float hz, V3 o, V3 id,
                                           float tmin, float tmax, bool& hit) {
     float ax = (lx - o.x) * id.x, bx = (hx - o.x) * id.x;
@@ -444,7 +448,7 @@ __device__ __forceinline__ float box_near(float lx, float hx, float ly, float hy
     float az = (lz - o.z) * id.z, bz = (hz - o.z) * id.z;
     float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
     float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
-    hit = nr <= fr * 1.0000004f && nr < __builtin_inff();
+    hit = nr <= fr * CRT_BOX_FAR && nr < __builtin_inff();
     return nr;
 }
 
@@ -455,7 +459,7 @@ __device__ __forceinline__ float box_near_oct(float nx, float fx, float ny, floa
                                               float tmin, float tmax, bool& hit) {
     float nr = fmaxf(fmaxf((nx - o.x) * id.x, (ny - o.y) * id.y), fmaxf((nz - o.z) * id.z, tmin));
     float fr = fminf(fminf((fx - o.x) * id.x, (fy - o.y) * id.y), fminf((fz - o.z) * id.z, tmax));
-    hit = nr <= fr * 1.0000004f;
+    hit = nr <= fr * CRT_BOX_FAR;
     return nr;
 }
 
@@ -485,8 +489,10 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     int32_t node;
     float tmin;
 
+    // hardware reciprocal (1 ulp; the box test's far factor covers it): one instruction instead of
+    // the ten of a correctly rounded division, paid again at every instance entry and exit
     static __device__ __forceinline__ float inv_of(float x) {
-        return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x));
+        return __builtin_amdgcn_rcpf(fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x));
     }
     __device__ __forceinline__ void set_ray(V3 ro, V3 rd) {
         o = ro; d = rd;
@@ -932,16 +938,57 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
     return pass_flag;
 }
 
+// The exact re-trace of a query the float triangle test left undecided, by the lane that met it
+// after its float traversal (rare and divergent; the double-precision code stays out of the
+// traversal loop). Out of line it would be allocated without the kernel's register bound: the
+// whole kernel would then take its count (180-210 VGPRs, two waves per SIMD).
+struct ExactHit { HitRec h; uint32_t pass_flag, ovf; };
+template <int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
+__device__ __forceinline__ ExactHit exact_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col, GSlot gst) {
+    ExactHit r;
+    TravCount tc = {0, 0, 0, 0, 0, 0};
+    bool ovf = false, amb = false;
+    uint32_t nrays = 0, nretrace = 0;
+    r.pass_flag = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, o, d, camera, lds_col, gst, r.h, tc, ovf,
+                                                                               nrays, nretrace, amb);
+    r.ovf = ovf ? 1u : 0u;
+    return r;
+}
+
+// Closest hit with the float triangle test and, for the rare query it leaves undecided, the exact
+// re-trace (counted in nexact); answers and counts as closest_hit<EXACT = true> would.
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+__device__ __forceinline__ uint32_t closest_hit_checked(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
+                                                        GSlot gst, HitRec& h, TravCount& tc, uint32_t& novf,
+                                                        uint32_t& nrays, uint32_t& nretrace, uint32_t& nexact) {
+    bool ovf = false, amb = false;
+    uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, COUNT>(S, o, d, camera, lds_col, gst, h, tc,
+                                                                                      ovf, nrays, nretrace, amb);
+    novf += ovf ? 1u : 0u;
+    if (!COUNT && amb) {
+        const ExactHit e = exact_hit<BLOCK, RS, LS, GS, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst);
+        h = e.h;
+        pass_flag = e.pass_flag;
+        novf += e.ovf;
+        nrays++;
+        if (pass_flag) nretrace++;
+        ++nexact;
+    }
+    return pass_flag;
+}
+
 // Closest-hit stage over the live path queue.
 // UNIFORM: every ray of the launch belongs to scene `uniform_scene` (scene pointers stay scalar).
 // Minimum resident blocks per CU the register allocation must allow. Blocks are 256 threads, one
 // wave on each of a CU's 4 SIMDs, so this is also waves per SIMD. The traversal is latency-bound and
-// occupancy pays more than the spills it costs (measured): one-level kernels are held to 6 (80
-// VGPRs; the single-scene one needs no spill, the multi-scene one spills its per-ray values: 8.6 %
-// faster on a 16-scene batch than at its natural 5), two-level kernels to 5 (96 VGPRs: 6 % faster
-// than at 4; at 6 they spill inside the loop and lose 17 %). Counting variants are not bounded.
+// occupancy pays more than spills outside the traversal loop, not more than spills inside it
+// (measured): the single-scene one-level kernel is held to 6 (80 VGPRs, no spill in the loop); the
+// multi-scene one-level kernel, whose scene offsets are per-lane values, to 5 (96 VGPRs: at 6 the
+// loop spills, and a 64-scene mixed batch and the saturated serving mix run 9-10 % slower);
+// two-level kernels to 5 (96 VGPRs: 6 % faster than at 4; at 6 they spill inside the loop and lose
+// 17 %). Counting variants are not bounded.
 constexpr int extend_min_waves(bool count, bool uniform, bool two_level) {
-    return count ? 1 : (two_level ? 5 : 6);
+    return count ? 1 : (two_level ? 5 : (uniform ? 6 : 5));
 }
 
 template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
@@ -975,19 +1022,8 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
         V3 o = v3(ro), d = v3(rdv);
         bool camera = (state & ST_CAMERA) != 0;
         HitRec h;
-        bool ovf = false, amb = false;
-        uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, COUNT>(S, o, d, camera, lds_col, gst, h,
-                                                                                          tc, ovf, nrays, nretrace, amb);
-        novf += ovf ? 1u : 0u;
-        if (!COUNT && amb) {
-            // the exact re-trace of an undecided query, by its own lane (a rare, divergent branch:
-            // the double-precision code stays out of the traversal loop above)
-            bool o2 = false, amb2 = false;
-            pass_flag = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, o, d, camera, lds_col, gst, h, tc,
-                                                                                    o2, nrays, nretrace, amb2);
-            novf += o2 ? 1u : 0u;
-            ++nexact;
-        }
+        const uint32_t pass_flag = closest_hit_checked<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(
+            S, o, d, camera, lds_col, gst, h, tc, novf, nrays, nretrace, nexact);
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
     };
@@ -1117,14 +1153,13 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
             novf += tr.st.overflow ? 1u : 0u;
             if (!COUNT && h.t < 0.0f) {
                 // undecided triangle test: the lane re-traces the query exactly (rare, divergent)
-                HitRec h2;
-                bool o2 = false, amb2 = false;
-                const uint32_t pf = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(
-                    S, tr.ow, tr.dw, tr.camera, lds_col, gst, h2, tc, o2, nrays, nretrace, amb2);
-                novf += o2 ? 1u : 0u;
+                const ExactHit e = exact_hit<BLOCK, RS, LS, GS, WIDTH, TWO_LEVEL>(S, tr.ow, tr.dw, tr.camera, lds_col, gst);
+                novf += e.ovf;
+                nrays++;
+                if (e.pass_flag) nretrace++;
                 ++nexact;
-                hit[slot] = make_float4(h2.t, bitsf(h2.prim), h2.u, h2.v);
-                hit_inst[slot] = h2.inst | pf;
+                hit[slot] = make_float4(e.h.t, bitsf(e.h.prim), e.h.u, e.h.v);
+                hit_inst[slot] = e.h.inst | e.pass_flag;
                 active = false;
                 continue;
             }
@@ -1594,7 +1629,7 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
     uint32_t* lds_col = s_stack + threadIdx.x;
     const GSlot gst{gstack, gtid * GS};
     TravCount tc = {0, 0, 0, 0, 0, 0};
-    uint32_t nrays = 0, nretrace = 0, novf = 0, light_tests = 0;
+    uint32_t nrays = 0, nretrace = 0, novf = 0, light_tests = 0, nexact = 0;
     float4* accum = (float4*)R.accum;
     for (uint32_t q = gtid; q < R.pix_count; q += stride) {
         uint32_t pix = R.pix_begin + q;
@@ -1615,10 +1650,8 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
             bool camera = true;
             for (;;) {
                 HitRec h;
-                bool ovf = false, amb = false;
-                uint32_t pf = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, o, d, camera, lds_col, gst, h,
-                                                                                          tc, ovf, nrays, nretrace, amb);
-                novf += ovf ? 1u : 0u;
+                const uint32_t pf = closest_hit_checked<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL>(
+                    S, o, d, camera, lds_col, gst, h, tc, novf, nrays, nretrace, nexact);
                 camera = false;
                 float4 hv = make_float4(h.t, bitsf(h.prim), h.u, h.v);
                 if (!shade_hit<MODE>(sc, R, pix, sample, hv, h.inst | pf, o, d, att, L, depth, bounce, g, light_tests))
@@ -1634,6 +1667,7 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
     wave_add(&counters->retraces, nretrace);
     wave_add(&counters->light_tests, light_tests);
     wave_add(&counters->stack_overflow, novf);
+    wave_add(&counters->exact, nexact);
 }
 
 // Per-pixel sums in sample order: accum += rad[sample 0] + ... (same float additions as the
@@ -1725,6 +1759,49 @@ __global__ __launch_bounds__(256) void k_init_request(float4* __restrict__ accum
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// VALU issue-rate calibration: each lane runs `iters` rounds of 16 independent dependency chains
+// of one instruction (KIND 0: v_fma_f32, 1: v_add_u32, 2: v_mul_f32), 16 wave instructions per
+// round and wave, and writes its chains' sum so that none is removed. b and c come from the
+// arguments, so nothing folds. KIND 3-7 run the v_fma_f32 rounds on part of each wave: lanes 0-15
+// (one 16-lane group), every fourth lane (16 lanes over all four groups), lanes 0-31, lanes 0-62
+// and lanes 0-47, which tells how a wave instruction's issue time depends on its active lanes.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, float c, float* __restrict__ out) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    if ((KIND == 3 && lane >= 16u) || (KIND == 4 && (lane & 3u) != 0u) || (KIND == 5 && lane >= 32u) ||
+        (KIND == 6 && lane == 63u) || (KIND == 7 && lane >= 48u)) {
+        out[tid] = 0.0f;
+        return;
+    }
+    if (KIND == 1) {
+        uint32_t a[16];
+        const uint32_t ib = __float_as_uint(b) | 1u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = tid + (uint32_t)k;
+        for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = a[k] + ib;
+        }
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s ^= a[k];
+        out[tid] = (float)s;
+    } else {
+        float a[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = (float)(tid + (uint32_t)k) * 1e-7f;
+        for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = KIND == 2 ? a[k] * b : __builtin_fmaf(a[k], b, c);
+        }
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s += a[k];
+        out[tid] = s;
+    }
 }
 
 // ======================================================================================== launchers
@@ -1885,6 +1962,18 @@ extern "C" int crt_k_init_request(hipStream_t s, float4* accum, uint32_t* rng_st
 extern "C" int crt_k_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return (int)hipGetLastError();
+}
+
+extern "C" int crt_k_valu_probe(hipStream_t s, int kind, uint32_t blocks, uint32_t iters, float* out) {
+    if (kind == 0) hipLaunchKernelGGL(k_valu_probe<0>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 1) hipLaunchKernelGGL(k_valu_probe<1>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 2) hipLaunchKernelGGL(k_valu_probe<2>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 3) hipLaunchKernelGGL(k_valu_probe<3>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 4) hipLaunchKernelGGL(k_valu_probe<4>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 5) hipLaunchKernelGGL(k_valu_probe<5>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 6) hipLaunchKernelGGL(k_valu_probe<6>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else hipLaunchKernelGGL(k_valu_probe<7>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     return (int)hipGetLastError();
 }
 

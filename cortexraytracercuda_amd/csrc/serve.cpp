@@ -186,7 +186,7 @@ void Context::server_reset_pass(Server& S) {
 }
 
 // True when no request is waiting or on the device and every lane has drained (caller holds
-// gpu_mutex, so no request can be submitted meanwhile).
+// both context locks, so no request can be submitted meanwhile).
 bool Context::server_idle() {
     std::lock_guard<std::mutex> lk(srv_mutex);
     if (!srv_waiting.empty() || !srv_active.empty()) return false;
@@ -199,7 +199,7 @@ bool Context::server_idle() {
 
 // Drops every captured serving iteration: they bake in the traversal variant (BVH width, stack
 // depth, work distribution), which the autotuner may change. Only called on an idle serving loop
-// (server_idle) with gpu_mutex held: the scheduler thread then waits for work and touches no graph.
+// (server_idle) with both context locks held: the scheduler thread then waits for work and touches no graph.
 void Context::server_drop_graphs() {
     for (Server& S : srv) {
         if (!S.ready) continue;
@@ -331,7 +331,6 @@ int Context::server_iteration(Server& S) {
     }
     HIP_TRY(hipEventRecord(S.ev[slot], S.s));
     S.iter++;
-    iterations++;
     return CRT_OK;
 }
 
@@ -505,7 +504,7 @@ void Context::server_loop() {
         for (ServeReq* r : admitted) {
             // no gpu_mutex: the setup touches the buffer cache (own lock), the request's scene
             // (pinned by its reference) and the lane's own stream, so a blocking render holding
-            // the context lock does not stall admission
+            // exec_mutex does not stall admission
             r->job = std::make_unique<Job>();
             const int rc = r->job->setup(this, r->sc, r->rd, srv[r->lane].s);
             r->timing.admit_s = now_s();

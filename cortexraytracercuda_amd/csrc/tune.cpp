@@ -30,7 +30,7 @@ Shape current_shape(const Context& c) {
 }  // namespace
 
 // One timed trial render of `rd` on `scene` with the context's current shape (caller holds
-// gpu_mutex). Throughput = closest-hit queries / GPU time of the passes.
+// both context locks). Throughput = closest-hit queries / GPU time of the passes.
 int Context::tune_trial(crt_scene_id scene, const crt_render_desc& rd, double& mrays_s) {
     DeviceScene* sc = find_scene(scene);
     if (!sc) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown scene id");
@@ -46,7 +46,7 @@ int Context::tune_trial(crt_scene_id scene, const crt_render_desc& rd, double& m
 }
 
 int Context::autotune(crt_scene_id scene, const crt_scene_desc* desc, const crt_render_desc* rd, crt_tune_result* out) {
-    std::lock_guard<std::mutex> g(gpu_mutex);
+    ExecLock g(*this);
     int rc = Job::validate(*rd);
     if (rc) return rc;
     DeviceScene* sc = find_scene(scene);

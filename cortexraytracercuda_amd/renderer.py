@@ -41,8 +41,10 @@ class Renderer:
 
     def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0,
                  extend_kernel=0, refill_min=0, pass_paths=0, grid_per_cu=0, shade_grid_per_cu=0, sub_passes=0,
-                 graphs=0, stack_lds=0, ray_sort=0, bvh_builder="host"):
-        """bvh_builder: "host" (binned SAH on the CPU) or "device" (linear BVH built on the GPU)."""
+                 graphs=0, stack_lds=0, ray_sort=0, bvh_builder="host", batch_group_paths=0):
+        """bvh_builder: "host" (binned SAH on the CPU) or "device" (linear BVH built on the GPU).
+        batch_group_paths: render_batch gives a request with at least this many path ids left passes
+        of its own scene (0: the library default, 0xffffffff: every pass mixes requests in order)."""
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -64,6 +66,7 @@ class Renderer:
         if bvh_builder not in builders:
             raise ValueError("bvh_builder must be 'host' or 'device'")
         cfg.bvh_builder = builders[bvh_builder]
+        cfg.batch_group_paths = int(batch_group_paths)
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -290,6 +290,9 @@ typedef struct crt_config {
     uint32_t ray_sort;          /* 1: the shade stage groups each workgroup's surviving paths by direction
                                    octant (coherent waves in the next closest-hit launch); 0 / 2: off */
     uint32_t bvh_builder;       /* CRT_BVH_*: where crt_scene_upload builds the BVH; 0 = host */
+    uint32_t batch_group_paths; /* crt_render_batch: a request with at least this many path ids left gets
+                                   passes of its own scene (single-scene kernel, one tree in the caches);
+                                   smaller remainders share passes; 0 = 2^22, 0xffffffff = always share */
 } crt_config;
 
 /* crt_config.bvh_builder. HOST: binned-SAH tree on the CPU threads, then one upload. DEVICE: linear
@@ -550,6 +553,15 @@ int crt_memory_get_info(crt_context* ctx, crt_memory_info* out);
  * now on (n = 0: the next one) fails with CRT_ERR_OUT_OF_MEMORY, once; n < 0 disarms it. Per-request
  * buffers reused from the context's cache are not allocations. */
 int crt_debug_fail_alloc(crt_context* ctx, int32_t n);
+
+/* VALU issue-rate calibration for the closest-hit kernel's roofline (no counterpart in the
+ * reference): one launch of `waves_per_simd` waves on every SIMD of the context's GPU, each lane
+ * running `iters` rounds of 16 independent chains of one instruction (kind 0: v_fma_f32, 1:
+ * v_add_u32, 2: v_mul_f32; 3-7: v_fma_f32 with part of each wave active: lanes 0-15, every fourth
+ * lane, lanes 0-31, lanes 0-62, lanes 0-47), timed with HIP events after one untimed launch. Returns the GPU time and the
+ * wave instructions the rounds issue (waves x iters x 16). */
+int crt_debug_valu_probe(crt_context* ctx, int32_t kind, uint32_t waves_per_simd, uint32_t iters, double* ms,
+                         uint64_t* wave_insts);
 
 /* Sizes of the public structs as compiled into the library (binding self-check). */
 int crt_struct_sizes(uint64_t* sizes, uint32_t n);

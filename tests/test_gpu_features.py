@@ -163,6 +163,40 @@ def test_serving_rng_modes_mixed_into_one_pass():
             assert np.array_equal(acc, ref)
 
 
+def test_served_request_completes_during_a_blocking_render():
+    """A request submitted while another thread's blocking render runs is admitted, rendered and
+    read back before that render ends: the blocking call holds the context's execution lock for
+    the whole render and the scene table only while it sets up. Its image equals its blocking
+    render alone."""
+    import threading
+    import time
+    with Renderer() as r:
+        big = r.upload(scenes.sphere_mesh_scene(200000, seed=7))
+        small = r.upload(scenes.box_scene())
+        _, ref, _ = r.render(small, 64, 64, 4, 8, seed=3)
+        r.wait(r.submit(small, 8, 8, 1, 1), 8, 8)   # the serving loop is set up before the timed part
+        t = {}
+
+        def blocking():
+            t["start"] = time.perf_counter()
+            _, _, st = r.render(big, 2048, 2048, 256, 8, seed=1, want_accum=False)
+            t["end"] = time.perf_counter()
+            t["rays"] = st["rays"]
+
+        th = threading.Thread(target=blocking)
+        th.start()
+        time.sleep(0.3)
+        t0 = time.perf_counter()
+        rid = r.submit(small, 64, 64, 4, 8, seed=3)
+        _, acc, _ = r.wait(rid, 64, 64)
+        t1 = time.perf_counter()
+        th.join()
+        print("served request: %.1f ms, submitted %.0f ms into a %.0f ms blocking render of %d rays"
+              % ((t1 - t0) * 1e3, (t0 - t["start"]) * 1e3, (t["end"] - t["start"]) * 1e3, t["rays"]), flush=True)
+        assert np.array_equal(acc, ref)
+        assert t1 < t["end"]
+
+
 def test_serving_image_only():
     """CRT_RENDER_IMAGE_ONLY: the served request returns the same 8-bit image as a full request
     and refuses to hand out an accumulation buffer it did not copy back."""

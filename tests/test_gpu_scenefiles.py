@@ -12,7 +12,7 @@ import pytest
 import oracle_py as O
 from cortexraytracercuda_amd import SceneFile
 
-from test_gpu_parity import _first_hit_parity, _same_rays
+from test_gpu_parity import _first_hit_parity, _same_rays, _sphere_t_bound
 
 pytestmark = pytest.mark.gpu
 NT = os.cpu_count() or 4
@@ -31,20 +31,40 @@ def test_shipped_scene_first_hits_and_image_against_fp64(renderer, name):
     sid = renderer.upload(f)
     prim, _, t, _ = renderer.trace_rays(sid, r32, camera=True)
     oprim, _, ot, _ = O.trace(f, r64, camera=True, precision=64, accel=1)
-    _first_hit_parity(prim, t, oprim, ot, name, min_agree=1.0)
+    d, _ = f.to_desc()
+    spheres = [tuple(d.spheres[k].center) + (d.spheres[k].radius,) for k in range(d.num_spheres)]
+    spheres += [tuple(d.sphere_lights[k].center) + (d.sphere_lights[k].radius,) for k in range(d.num_sphere_lights)]
+    _first_hit_parity(prim, t, oprim, ot, name, min_agree=1.0, t_abs=_sphere_t_bound(spheres, r64, ot))
     assert (oprim >= 0).mean() > 0.3
     spp, depth = 16, info["max_depth"]
     rgb, acc, st = renderer.render(sid, W, H, spp, depth, seed=7)
     oacc, ost = O.render(f, W, H, spp, depth, seed=7, rng="pcg", precision=64, threads=NT)
+    acc32, ost32 = O.render(f, W, H, spp, depth, seed=7, rng="pcg", precision=32, threads=NT)
     renderer.release(sid)
-    g, o = acc[..., :3] / spp, oacc[..., :3] / spp
+    # a sample whose FP32 arithmetic gives NaN (on the spot-light scene the FP32 oracle has 3 such
+    # pixels on the same streams, the FP64 oracle none) leaves its pixel channel NaN, which the
+    # image output turns into 0 as the reference's prepare_pixel_color does
+    # (core/src/misc/color.cuh:291-293)
+    nan_px = int((~np.isfinite(acc[..., :3])).any(axis=-1).sum())
+    assert nan_px <= 0.001 * W * H
+    g, o = np.nan_to_num(acc[..., :3], nan=0.0) / spp, np.nan_to_num(oacc[..., :3], nan=0.0) / spp
+    o32 = np.nan_to_num(acc32[..., :3], nan=0.0) / spp
     close = np.all(np.abs(g - o) <= 1e-3 * np.maximum(o, 1.0), axis=-1)
-    rmse = float(np.sqrt(np.mean((g - o) ** 2)))
-    scale = max(1.0, float(np.abs(o).mean()))
-    print("%s %dx%d %d spp depth %d: pixels within 1e-3 %.5f rmse %.5f (mean %.3f) rays %d vs %d"
-          % (name, W, H, spp, depth, close.mean(), rmse, np.abs(o).mean(), st["rays"], ost["rays"]), flush=True)
+    close32 = np.all(np.abs(o32 - o) <= 1e-3 * np.maximum(o, 1.0), axis=-1)
+    rmse, rmse32 = float(np.sqrt(np.mean((g - o) ** 2))), float(np.sqrt(np.mean((o32 - o) ** 2)))
+    print("%s %dx%d %d spp depth %d: pixels within 1e-3 %.5f (fp32 oracle %.5f) rmse %.5f (fp32 oracle %.5f) "
+          "rays %d vs %d (fp32 oracle) %d (fp64), %d NaN pixels"
+          % (name, W, H, spp, depth, close.mean(), close32.mean(), rmse, rmse32, st["rays"], ost32["rays"], ost["rays"],
+             nan_px), flush=True)
     assert np.all(acc[..., 3] == spp)
     assert float(o.max()) > 0.1       # the lights reach the image
-    assert close.mean() >= 0.97
-    assert rmse <= 0.1 * scale
-    assert abs(st["rays"] - ost["rays"]) <= 0.01 * ost["rays"]
+    # FP32 arithmetic takes paths the FP64 oracle does not: on all_lights_types a ray leaving the
+    # radius-1000 ground sphere from an FP32 hit point a few ulps inside it meets the sphere again
+    # just past the 1e-5 parametric floor (the FP32 oracle traces 7 % more rays than the FP64 one,
+    # the reference's FP32 renderer does the same). The GPU must be as close to the FP64 image as
+    # the FP32 oracle is, and trace the FP32 oracle's rays.
+    # (FP32 oracle against FP64: 86.6 % of pixels within 1e-3, RMSE 0.021 on all_lights_types; 98.9 %,
+    # RMSE 0.0022 on simple_spot_light)
+    assert close.mean() >= close32.mean() - 0.01
+    assert rmse <= 1.5 * rmse32 + 1e-3
+    assert abs(st["rays"] - ost32["rays"]) <= 0.002 * ost32["rays"]
