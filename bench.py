@@ -8,8 +8,10 @@ Printed JSON line (rank 0):
   value        Mrays/s over the timed steps (closest-hit queries / wall time of the renders)
   roofline     traversal kernel (k_extend) against its binding resource, VALU issue: VALU wave
                instructions per ray (rocprofv3 SQ counters, profiles/) x rays per launch / HIP-event
-               launch time, against 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction; with the
-               measured DRAM traffic (FETCH_SIZE / WRITE_SIZE) and the algorithmic bytes beside it
+               launch time, against 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction (dense
+               FMA-class issue; the measured probe rates of profiles/valu_issue_probe_mi355x.json
+               beside it); with the measured DRAM traffic (FETCH_SIZE / WRITE_SIZE) and the
+               algorithmic bytes
   cpu_baseline reference-faithful CPU renderer (oracle/, FP32 minstd) on the host cores, 4 spp,
                plus a one-thread figure and the host CPU model
   megakernel   the plain path on the same workload: one kernel launch renders the image
@@ -297,6 +299,8 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False,
     traffic = None if tr is None else tr["hbm_bytes_per_ray"] * rays_per_launch
     dram = None if traffic is None else traffic / launch_s / 1e9
     sq = _profile_json("sq", workload_key, inf)
+    probe_path = os.path.join(ROOT, "profiles", "valu_issue_probe_mi355x.json")
+    probe = json.load(open(probe_path))["rates_ginst_s"] if os.path.exists(probe_path) else {}
     roof = {"kernel": "k_extend", "launches": launches, "avg_launch_ms": ext_ms / launches,
             "rays_per_launch": rays_per_launch,
             "traffic": traffic, "traffic_unit": "bytes per launch",
@@ -311,15 +315,18 @@ def traversal_roofline(scene, cfg, args, timed_renderer, sid, progressive=False,
         valu = sq["valu_insts_per_ray"] * rays_per_launch / launch_s / 1e9
         roof.update({"bound": "valu", "achieved": valu, "peak": VALU_PEAK_GINST, "unit": "G wave64 VALU instructions/s",
                      "frac": valu / VALU_PEAK_GINST, "valu_insts_per_ray": sq["valu_insts_per_ray"],
-                     "lanes_active_per_valu": sq.get("lanes_active_per_valu")})
+                     "lanes_active_per_valu": sq.get("lanes_active_per_valu"),
+                     "probe_dense_fma_ginst": probe.get("v_fma_f32_8wps"), "probe_mov_ginst": probe.get("v_mov_b32_8wps")})
     elif dram is not None:
         roof.update({"bound": "hbm", "achieved": dram, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dram / HBM_PEAK_GBS})
     else:
         roof.update({"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINST, "unit": "G wave64 VALU instructions/s",
                      "frac": None, "note": "no counter profile committed for this workload and BVH shape"})
     # a fraction of the binding resource cannot exceed it: a larger value means the committed
-    # counter profile is stale for this build
-    assert roof["frac"] is None or roof["frac"] <= 1.0, roof
+    # counter profile is stale for this build. The VALU peak prices every wave64 instruction at 4
+    # cycles; moves issue in 2 (the probe reaches 1030 G/s with v_mov_b32), so a stream rich in
+    # them can pass the dense peak, never twice it
+    assert roof["frac"] is None or roof["frac"] <= (2.0 if roof["bound"] == "valu" else 1.0), roof
     return roof
 
 
@@ -421,7 +428,8 @@ def valu_probe(iters=100000):
     with Renderer() as r:
         for kind, name in ((0, "v_fma_f32"), (1, "v_add_u32"), (2, "v_mul_f32"), (3, "v_fma_f32_lanes0_15"),
                            (4, "v_fma_f32_every4th_lane"), (5, "v_fma_f32_lanes0_31"), (6, "v_fma_f32_lanes0_62"),
-                           (7, "v_fma_f32_lanes0_47")):
+                           (7, "v_fma_f32_lanes0_47"), (8, "v_fma_f32_no_lane"), (9, "v_cndmask_b32"),
+                           (10, "v_max_f32"), (11, "v_cmp_lt_f32"), (12, "v_mov_b32")):
             for wps in (2, 4, 8):
                 ms, n = C.c_double(), C.c_uint64()
                 rc = r.lib.crt_debug_valu_probe(r.h, kind, wps, iters, C.byref(ms), C.byref(n))

@@ -45,6 +45,8 @@ CRT_RF_IMAGE_ONLY = 4
 
 CRT_BVH_HOST = 1
 CRT_BVH_DEVICE = 2
+CRT_SERVE_SMALL_THEN_OLDEST = 1
+CRT_SERVE_ROUND_ROBIN = 2
 
 CRT_XF_TRANSLATE = 1
 CRT_XF_ROTATE = 2
@@ -148,7 +150,8 @@ class crt_config(C.Structure):
                 ("bvh_width", C.c_uint32), ("extend_kernel", C.c_uint32), ("refill_min", C.c_uint32),
                 ("pass_paths", C.c_uint32), ("grid_per_cu", C.c_uint32), ("shade_grid_per_cu", C.c_uint32),
                 ("sub_passes", C.c_uint32), ("graphs", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("ray_sort", C.c_uint32), ("bvh_builder", C.c_uint32), ("batch_group_paths", C.c_uint32)]
+                ("ray_sort", C.c_uint32), ("bvh_builder", C.c_uint32), ("batch_group_paths", C.c_uint32),
+                ("serve_order", C.c_uint32), ("serve_small_paths", C.c_uint32)]
 
 
 class crt_scene_info(C.Structure):

@@ -178,6 +178,9 @@ int Context::init(const crt_config* cfg) {
     ray_sort = config.ray_sort == 1u ? 1u : 0u;
     if (config.bvh_builder > CRT_BVH_DEVICE) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown bvh_builder");
     bvh_builder = config.bvh_builder ? config.bvh_builder : (uint32_t)CRT_BVH_HOST;
+    if (config.serve_order > CRT_SERVE_ROUND_ROBIN) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown serve_order");
+    serve_order = config.serve_order ? config.serve_order : (uint32_t)CRT_SERVE_SMALL_THEN_OLDEST;
+    serve_small = config.serve_small_paths ? (uint64_t)config.serve_small_paths : (1ull << 22);
     batch_group = config.batch_group_paths == 0xffffffffu ? UINT64_MAX
                 : config.batch_group_paths ? (uint64_t)config.batch_group_paths : (1ull << 22);
     if (stack_lds != 8 && stack_lds != 16 && stack_lds != 32)
@@ -1216,8 +1219,8 @@ int crt_debug_fail_alloc(crt_context* c, int32_t n) {
 int crt_debug_valu_probe(crt_context* c, int32_t kind, uint32_t waves_per_simd, uint32_t iters, double* ms,
                          uint64_t* wave_insts) {
     if (!c || !ms || !wave_insts) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
-    if (kind < 0 || kind > 7 || waves_per_simd < 1 || waves_per_simd > 8 || iters < 1 || iters > (1u << 24))
-        return set_error(CRT_ERR_INVALID_ARGUMENT, "probe kind 0-7, 1-8 waves per SIMD, 1-2^24 rounds");
+    if (kind < 0 || kind > 12 || waves_per_simd < 1 || waves_per_simd > 8 || iters < 1 || iters > (1u << 24))
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "probe kind 0-12, 1-8 waves per SIMD, 1-2^24 rounds");
     Context* ctx = reinterpret_cast<Context*>(c);
     ExecLock g(*ctx);
     // four SIMDs per CU, four waves per 256-lane workgroup: waves_per_simd workgroups per CU

@@ -199,6 +199,8 @@ struct Context {
     uint32_t ray_sort = 0;           // shade groups each block's survivors by direction octant
     uint32_t bvh_builder = CRT_BVH_HOST;   // where crt_scene_upload builds the BVH (CRT_BVH_*)
     uint64_t batch_group = 1ull << 22;     // path ids left for a batched request to get passes of its own scene
+    uint32_t serve_order = CRT_SERVE_SMALL_THEN_OLDEST;   // serving loop feed order (CRT_SERVE_*)
+    uint64_t serve_small = 1ull << 22;     // path ids left below which a served request goes first
     uint32_t grid_per_cu = 16, shade_grid_per_cu = 16;
     // Lock order: exec_mutex before gpu_mutex. exec_mutex serialises the blocking calls (their
     // stream, pass buffers and `last` counters) and is held for a whole render; gpu_mutex guards
@@ -302,6 +304,7 @@ struct Context {
     void server_release_lane(Server& S);
     int server_iteration(Server& S);
     int server_append(Server& S, std::vector<ServeReq*>& act);
+    bool append_segment(Server& S, ServeReq* r, uint32_t seg_cap);
     int server_process_snapshot(Server& S, std::vector<ServeReq*>& act);
     void server_reset_pass(Server& S);
     bool server_idle();

@@ -452,13 +452,17 @@ __device__ __forceinline__ float box_near(float lx, float hx, float ly, float hy
     return nr;
 }
 
-// The same slab test with the near and far plane of each axis chosen by the sign of the inverse
-// direction (per ray): (near - o) * inv_d <= (far - o) * inv_d holds after rounding, so the
-// per-axis min / max of the two-plane form are not needed and the result is bit-identical.
-__device__ __forceinline__ float box_near_oct(float nx, float fx, float ny, float fy, float nz, float fz, V3 o, V3 id,
-                                              float tmin, float tmax, bool& hit) {
-    float nr = fmaxf(fmaxf((nx - o.x) * id.x, (ny - o.y) * id.y), fmaxf((nz - o.z) * id.z, tmin));
-    float fr = fminf(fminf((fx - o.x) * id.x, (fy - o.y) * id.y), fminf((fz - o.z) * id.z, tmax));
+// The four-wide slab test, one fused multiply-add per plane: t = p * inv_d - o * inv_d. The
+// product o * inv_d is rounded once, which moves every plane distance by up to 2^-24 |o * inv_d| on
+// top of the relative error the far factor covers; near planes subtract on = o * inv_d + e and far
+// planes of = o * inv_d - e (e = 2^-22 |o * inv_d| per axis), so the test stays conservative. The
+// near and far plane of each axis are chosen by the sign of the inverse direction (per ray): the
+// fused form is monotone in the plane coordinate, so near <= far holds after rounding and the
+// per-axis min / max of the two-plane form are not needed.
+__device__ __forceinline__ float box_near_oct(float nx, float fx, float ny, float fy, float nz, float fz, V3 on, V3 of,
+                                              V3 id, float tmin, float tmax, bool& hit) {
+    float nr = fmaxf(fmaxf(fmaf(nx, id.x, -on.x), fmaf(ny, id.y, -on.y)), fmaxf(fmaf(nz, id.z, -on.z), tmin));
+    float fr = fminf(fminf(fmaf(fx, id.x, -of.x), fmaf(fy, id.y, -of.y)), fminf(fmaf(fz, id.z, -of.z), tmax));
     hit = nr <= fr * CRT_BOX_FAR;
     return nr;
 }
@@ -525,10 +529,18 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
         const float tf = best.t * CRT_TIE_HI;
         const int4 rf = make_int4((int)fbits(rff.x), (int)fbits(rff.y), (int)fbits(rff.z), (int)fbits(rff.w));
         bool h0, h1, h2, h3;
-        float k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, id, tmin, tf, h0);
-        float k1 = box_near_oct(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, id, tmin, tf, h1);
-        float k2 = box_near_oct(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, id, tmin, tf, h2);
-        float k3 = box_near_oct(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, id, tmin, tf, h3);
+        // the plane offsets are formed at every visit: kept for the whole ray (hoisted out of the
+        // loop) they take six registers, and the one-level kernel then spills inside its loop (5 %
+        // slower than the two-operation planes); the empty asm keeps the compiler from hoisting
+        float ox = o.x, oy = o.y, oz = o.z;
+        asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
+        const V3 oid = v3(ox * id.x, oy * id.y, oz * id.z);
+        const V3 e = v3(fabsf(oid.x) * 2.384185791e-7f, fabsf(oid.y) * 2.384185791e-7f, fabsf(oid.z) * 2.384185791e-7f);
+        const V3 on = oid + e, of = oid - e;
+        float k0 = box_near_oct(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, on, of, id, tmin, tf, h0);
+        float k1 = box_near_oct(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, on, of, id, tmin, tf, h1);
+        float k2 = box_near_oct(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, on, of, id, tmin, tf, h2);
+        float k3 = box_near_oct(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, on, of, id, tmin, tf, h3);
         const float INF = __builtin_inff();
         k0 = h0 ? k0 : INF;
         k1 = h1 ? k1 : INF;
@@ -1767,6 +1779,11 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 // arguments, so nothing folds. KIND 3-7 run the v_fma_f32 rounds on part of each wave: lanes 0-15
 // (one 16-lane group), every fourth lane (16 lanes over all four groups), lanes 0-31, lanes 0-62
 // and lanes 0-47, which tells how a wave instruction's issue time depends on its active lanes.
+// KIND 8 issues the 16 v_fma_f32 of a round with EXEC cleared (saved and restored inside one asm
+// block, which touches no memory): what an instruction of a branch that no lane of the wave takes
+// costs when the compiler does not jump around it. KIND 9-12 issue other instructions the traversal
+// is made of, 16 per round: v_cndmask_b32 (selects of the child sort), v_max_f32 (slab tests),
+// v_cmp_lt_f32 (compares into a lane mask) and v_mov_b32.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, float c, float* __restrict__ out) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1774,6 +1791,78 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, flo
     if ((KIND == 3 && lane >= 16u) || (KIND == 4 && (lane & 3u) != 0u) || (KIND == 5 && lane >= 32u) ||
         (KIND == 6 && lane == 63u) || (KIND == 7 && lane >= 48u)) {
         out[tid] = 0.0f;
+        return;
+    }
+    if (KIND >= 9) {
+        float a0 = (float)tid * 1e-7f, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f;
+        for (uint32_t i = 0; i < iters; ++i) {
+            uint64_t m0, m1;
+            if (KIND == 9)
+                asm volatile(
+                    "v_cmp_lt_f32 vcc, %[a0], %[a1]\n\t"
+                    "v_cndmask_b32 %[a0], %[a0], %[a1], vcc\n\tv_cndmask_b32 %[a1], %[a1], %[a2], vcc\n\t"
+                    "v_cndmask_b32 %[a2], %[a2], %[a3], vcc\n\tv_cndmask_b32 %[a3], %[a3], %[a0], vcc\n\t"
+                    "v_cndmask_b32 %[a0], %[a0], %[a1], vcc\n\tv_cndmask_b32 %[a1], %[a1], %[a2], vcc\n\t"
+                    "v_cndmask_b32 %[a2], %[a2], %[a3], vcc\n\tv_cndmask_b32 %[a3], %[a3], %[a0], vcc\n\t"
+                    "v_cndmask_b32 %[a0], %[a0], %[a1], vcc\n\tv_cndmask_b32 %[a1], %[a1], %[a2], vcc\n\t"
+                    "v_cndmask_b32 %[a2], %[a2], %[a3], vcc\n\tv_cndmask_b32 %[a3], %[a3], %[a0], vcc\n\t"
+                    "v_cndmask_b32 %[a0], %[a0], %[a1], vcc\n\tv_cndmask_b32 %[a1], %[a1], %[a2], vcc\n\t"
+                    "v_cndmask_b32 %[a2], %[a2], %[a3], vcc"
+                    : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3) : : "vcc");
+            else if (KIND == 10)
+                asm volatile(
+                    "v_max_f32 %[a0], %[a0], %[a1]\n\tv_max_f32 %[a1], %[a1], %[a2]\n\t"
+                    "v_max_f32 %[a2], %[a2], %[a3]\n\tv_max_f32 %[a3], %[a3], %[a0]\n\t"
+                    "v_max_f32 %[a0], %[a0], %[a1]\n\tv_max_f32 %[a1], %[a1], %[a2]\n\t"
+                    "v_max_f32 %[a2], %[a2], %[a3]\n\tv_max_f32 %[a3], %[a3], %[a0]\n\t"
+                    "v_max_f32 %[a0], %[a0], %[a1]\n\tv_max_f32 %[a1], %[a1], %[a2]\n\t"
+                    "v_max_f32 %[a2], %[a2], %[a3]\n\tv_max_f32 %[a3], %[a3], %[a0]\n\t"
+                    "v_max_f32 %[a0], %[a0], %[a1]\n\tv_max_f32 %[a1], %[a1], %[a2]\n\t"
+                    "v_max_f32 %[a2], %[a2], %[a3]\n\tv_max_f32 %[a3], %[a3], %[a0]"
+                    : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3));
+            else if (KIND == 11)
+                asm volatile(
+                    "v_cmp_lt_f32 %[m0], %[a0], %[a1]\n\tv_cmp_lt_f32 %[m1], %[a1], %[a2]\n\t"
+                    "v_cmp_lt_f32 %[m0], %[a2], %[a3]\n\tv_cmp_lt_f32 %[m1], %[a3], %[a0]\n\t"
+                    "v_cmp_lt_f32 %[m0], %[a0], %[a2]\n\tv_cmp_lt_f32 %[m1], %[a1], %[a3]\n\t"
+                    "v_cmp_lt_f32 %[m0], %[a2], %[a0]\n\tv_cmp_lt_f32 %[m1], %[a3], %[a1]\n\t"
+                    "v_cmp_lt_f32 %[m0], %[a0], %[a1]\n\tv_cmp_lt_f32 %[m1], %[a1], %[a2]\n\t"
+                    "v_cmp_lt_f32 %[m0], %[a2], %[a3]\n\tv_cmp_lt_f32 %[m1], %[a3], %[a0]\n\t"
+                    "v_cmp_lt_f32 %[m0], %[a0], %[a2]\n\tv_cmp_lt_f32 %[m1], %[a1], %[a3]\n\t"
+                    "v_cmp_lt_f32 %[m0], %[a2], %[a0]\n\tv_cmp_lt_f32 %[m1], %[a3], %[a1]"
+                    : [m0] "=&s"(m0), [m1] "=&s"(m1) : [a0] "v"(a0), [a1] "v"(a1), [a2] "v"(a2), [a3] "v"(a3));
+            else
+                asm volatile(
+                    "v_mov_b32 %[a0], %[a1]\n\tv_mov_b32 %[a1], %[a2]\n\tv_mov_b32 %[a2], %[a3]\n\tv_mov_b32 %[a3], %[a0]\n\t"
+                    "v_mov_b32 %[a0], %[a1]\n\tv_mov_b32 %[a1], %[a2]\n\tv_mov_b32 %[a2], %[a3]\n\tv_mov_b32 %[a3], %[a0]\n\t"
+                    "v_mov_b32 %[a0], %[a1]\n\tv_mov_b32 %[a1], %[a2]\n\tv_mov_b32 %[a2], %[a3]\n\tv_mov_b32 %[a3], %[a0]\n\t"
+                    "v_mov_b32 %[a0], %[a1]\n\tv_mov_b32 %[a1], %[a2]\n\tv_mov_b32 %[a2], %[a3]\n\tv_mov_b32 %[a3], %[a0]"
+                    : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3));
+            if (KIND == 11) a0 += (float)(uint32_t)(m0 ^ m1) * 0.0f;
+        }
+        out[tid] = a0 + a1 + a2 + a3;
+        return;
+    }
+    if (KIND == 8) {
+        float a0 = (float)tid * 1e-7f, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f;
+        for (uint32_t i = 0; i < iters; ++i) {
+            uint64_t sv;
+            asm volatile(
+                "s_mov_b64 %[sv], exec\n\t"
+                "s_mov_b64 exec, 0\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "s_mov_b64 exec, %[sv]"
+                : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3), [sv] "=&s"(sv)
+                : [b] "s"(b), [c] "v"(c));
+        }
+        out[tid] = a0 + a1 + a2 + a3;
         return;
     }
     if (KIND == 1) {
@@ -1973,7 +2062,12 @@ extern "C" int crt_k_valu_probe(hipStream_t s, int kind, uint32_t blocks, uint32
     else if (kind == 4) hipLaunchKernelGGL(k_valu_probe<4>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     else if (kind == 5) hipLaunchKernelGGL(k_valu_probe<5>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     else if (kind == 6) hipLaunchKernelGGL(k_valu_probe<6>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
-    else hipLaunchKernelGGL(k_valu_probe<7>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 7) hipLaunchKernelGGL(k_valu_probe<7>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 8) hipLaunchKernelGGL(k_valu_probe<8>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 9) hipLaunchKernelGGL(k_valu_probe<9>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 10) hipLaunchKernelGGL(k_valu_probe<10>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 11) hipLaunchKernelGGL(k_valu_probe<11>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else hipLaunchKernelGGL(k_valu_probe<12>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     return (int)hipGetLastError();
 }
 

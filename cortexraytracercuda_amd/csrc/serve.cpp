@@ -218,43 +218,69 @@ static void complete(ServeReq* r, int status, std::condition_variable& cv) {
     cv.notify_all();
 }
 
+// One segment (at most seg_cap path ids, within the pass's id room) of request r becomes a
+// request-table entry behind the pass's last path id. False when r has nothing left to take.
+bool Context::append_segment(Server& S, ServeReq* r, uint32_t seg_cap) {
+    const uint32_t room = S.cap - S.total;
+    WorkItem w;
+    if (!r->job->take(std::min(seg_cap, room), w)) return false;
+    crt_dev_request q = r->job->req;
+    q.path_base = S.total;
+    q.pix_begin = w.pix_begin; q.pix_count = w.pix_count;
+    q.sample_begin = w.sample_begin; q.sample_count = w.sample_count;
+    const uint32_t e = (uint32_t)S.entries.size();
+    S.req_host[e] = q;
+    SrvEntry en;
+    en.r = r;
+    en.seg = r->segs++;
+    en.paths = w.pix_count * w.sample_count;
+    en.end = S.total + en.paths;
+    S.entries.push_back(en);
+    S.total += en.paths;
+    if (r->sc->host.num_instances) S.two_level = true;
+    if (r->rd.rng_mode == CRT_RNG_MINSTD) S.mixed_rng = true;
+    r->timing.passes++;
+    if (r->timing.first_pass_s == 0.0) r->timing.first_pass_s = now_s();
+    return true;
+}
+
 // Appends segments of active requests to the open pass while the backlog of ungenerated path ids
-// is below two slot pools (a new request starts within a few iterations). Round-robin, one segment
-// per request and call; MINSTD requests keep at most one segment in flight (each pixel's engine
-// advances sample after sample).
+// is below two slot pools (a new request starts within a few iterations). MINSTD requests keep at
+// most one segment in flight (each pixel's engine advances sample after sample).
+// Order (CRT_SERVE_SMALL_THEN_OLDEST, the default): requests with at most serve_small path ids left,
+// in admission order, then the others oldest first, each appending segments while the backlog
+// allows. Draining the large requests in arrival order bounds the largest latency of a loop that
+// is offered more work than it renders (the last request to finish is then one that arrived late);
+// small requests pass them, finish within a few iterations and keep the median low.
+// CRT_SERVE_ROUND_ROBIN: one segment per request and call in rotating order (every request
+// progresses; the large ones finish last whenever they arrived).
 int Context::server_append(Server& S, std::vector<ServeReq*>& act) {
     if (act.empty() || S.full) return CRT_OK;
     const uint32_t seg_cap = std::max<uint32_t>(S.slots / 2, 1u);
     const uint32_t n0 = (uint32_t)S.entries.size();
     const size_t na = act.size();
-    for (size_t t = 0; t < na; ++t) {
-        ServeReq* r = act[(S.rr + t) % na];
-        if (r->cancel || r->finishing || r->job->scheduled_all()) continue;
-        if (r->rd.rng_mode == CRT_RNG_MINSTD && r->segs_resolved < r->segs) continue;
-        if ((uint64_t)S.total - S.gen_seen >= 2ull * S.slots) break;
-        if (S.entries.size() >= Server::kEntries || S.total >= S.cap) { S.full = true; break; }
-        uint32_t room = S.cap - S.total;
-        WorkItem w;
-        if (!r->job->take(std::min(seg_cap, room), w)) continue;
-        crt_dev_request q = r->job->req;
-        q.path_base = S.total;
-        q.pix_begin = w.pix_begin; q.pix_count = w.pix_count;
-        q.sample_begin = w.sample_begin; q.sample_count = w.sample_count;
-        const uint32_t e = (uint32_t)S.entries.size();
-        S.req_host[e] = q;
-        SrvEntry en;
-        en.r = r;
-        en.seg = r->segs++;
-        en.paths = w.pix_count * w.sample_count;
-        en.end = S.total + en.paths;
-        S.entries.push_back(en);
-        S.total += en.paths;
-        if (r->sc->host.num_instances) S.two_level = true;
-        if (r->rd.rng_mode == CRT_RNG_MINSTD) S.mixed_rng = true;
-        r->timing.passes++;
-        if (r->timing.first_pass_s == 0.0) r->timing.first_pass_s = now_s();
+    std::vector<ServeReq*> order;
+    order.reserve(na);
+    const bool rr = serve_order == CRT_SERVE_ROUND_ROBIN;
+    if (rr) {
+        for (size_t t = 0; t < na; ++t) order.push_back(act[(S.rr + t) % na]);
+        S.rr++;
+    } else {
+        for (ServeReq* r : act) if (r->job->paths_left() <= serve_small) order.push_back(r);
+        for (ServeReq* r : act) if (r->job->paths_left() > serve_small) order.push_back(r);
     }
-    S.rr++;
+    bool stop = false;
+    for (ServeReq* r : order) {
+        if (stop) break;
+        for (;;) {
+            if (r->cancel || r->finishing || r->job->scheduled_all()) break;
+            if (r->rd.rng_mode == CRT_RNG_MINSTD && r->segs_resolved < r->segs) break;
+            if ((uint64_t)S.total - S.gen_seen >= 2ull * S.slots) { stop = true; break; }
+            if (S.entries.size() >= Server::kEntries || S.total >= S.cap) { S.full = true; stop = true; break; }
+            if (!append_segment(S, r, seg_cap)) break;
+            if (rr) break;
+        }
+    }
     const uint32_t n1 = (uint32_t)S.entries.size();
     if (n1 == n0) return CRT_OK;
     HIP_TRY(hipMemcpyAsync((crt_dev_request*)S.req.p + n0, S.req_host + n0, sizeof(crt_dev_request) * (n1 - n0),

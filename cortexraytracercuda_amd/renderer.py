@@ -41,10 +41,13 @@ class Renderer:
 
     def __init__(self, device=0, hbm_budget=0, max_paths=0, count_work=False, leaf_size=0, bvh_width=0,
                  extend_kernel=0, refill_min=0, pass_paths=0, grid_per_cu=0, shade_grid_per_cu=0, sub_passes=0,
-                 graphs=0, stack_lds=0, ray_sort=0, bvh_builder="host", batch_group_paths=0):
+                 graphs=0, stack_lds=0, ray_sort=0, bvh_builder="host", batch_group_paths=0, serve_order="small_then_oldest",
+                 serve_small_paths=0):
         """bvh_builder: "host" (binned SAH on the CPU) or "device" (linear BVH built on the GPU).
         batch_group_paths: render_batch gives a request with at least this many path ids left passes
-        of its own scene (0: the library default, 0xffffffff: every pass mixes requests in order)."""
+        of its own scene (0: the library default, 0xffffffff: every pass mixes requests in order).
+        serve_order: "small_then_oldest" (requests with at most serve_small_paths path ids left first,
+        then the others oldest first) or "round_robin" (one segment per request in turn)."""
         self.lib = A.load()
         cfg = A.crt_config()
         cfg.device = device
@@ -67,6 +70,11 @@ class Renderer:
             raise ValueError("bvh_builder must be 'host' or 'device'")
         cfg.bvh_builder = builders[bvh_builder]
         cfg.batch_group_paths = int(batch_group_paths)
+        orders = {"small_then_oldest": A.CRT_SERVE_SMALL_THEN_OLDEST, "round_robin": A.CRT_SERVE_ROUND_ROBIN}
+        if serve_order not in orders:
+            raise ValueError("serve_order must be 'small_then_oldest' or 'round_robin'")
+        cfg.serve_order = orders[serve_order]
+        cfg.serve_small_paths = int(serve_small_paths)
         h = C.c_void_p()
         A.check(self.lib.crt_create(C.byref(cfg), C.byref(h)))
         self.h = h

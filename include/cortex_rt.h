@@ -293,7 +293,19 @@ typedef struct crt_config {
     uint32_t batch_group_paths; /* crt_render_batch: a request with at least this many path ids left gets
                                    passes of its own scene (single-scene kernel, one tree in the caches);
                                    smaller remainders share passes; 0 = 2^22, 0xffffffff = always share */
+    uint32_t serve_order;       /* CRT_SERVE_*: the order in which the serving loop feeds admitted requests
+                                   into its passes; 0 = CRT_SERVE_SMALL_THEN_OLDEST */
+    uint32_t serve_small_paths; /* CRT_SERVE_SMALL_THEN_OLDEST: requests with at most this many path ids left
+                                   go first; 0 = 2^22 */
 } crt_config;
+
+/* crt_config.serve_order. SMALL_THEN_OLDEST: requests with at most serve_small_paths path ids left
+ * first (in admission order), then the others oldest first, each taking as much of a pass as its
+ * backlog bound allows: an overloaded loop then drains its large requests in arrival order, which
+ * bounds the largest latency, while small requests pass them. ROUND_ROBIN: one segment per request
+ * in turn (every request progresses; large requests finish last, whenever they arrived). Images do
+ * not depend on the order. */
+enum { CRT_SERVE_SMALL_THEN_OLDEST = 1, CRT_SERVE_ROUND_ROBIN = 2 };
 
 /* crt_config.bvh_builder. HOST: binned-SAH tree on the CPU threads, then one upload. DEVICE: linear
  * BVH on the GPU (Morton codes, radix sort, radix tree, bottom-up boxes, collapse to the BVH width)
@@ -558,7 +570,8 @@ int crt_debug_fail_alloc(crt_context* ctx, int32_t n);
  * reference): one launch of `waves_per_simd` waves on every SIMD of the context's GPU, each lane
  * running `iters` rounds of 16 independent chains of one instruction (kind 0: v_fma_f32, 1:
  * v_add_u32, 2: v_mul_f32; 3-7: v_fma_f32 with part of each wave active: lanes 0-15, every fourth
- * lane, lanes 0-31, lanes 0-62, lanes 0-47), timed with HIP events after one untimed launch. Returns the GPU time and the
+ * lane, lanes 0-31, lanes 0-62, lanes 0-47; 8: v_fma_f32 with no lane active; 9: v_cndmask_b32,
+ * 10: v_max_f32, 11: v_cmp_lt_f32, 12: v_mov_b32), timed with HIP events after one untimed launch. Returns the GPU time and the
  * wave instructions the rounds issue (waves x iters x 16). */
 int crt_debug_valu_probe(crt_context* ctx, int32_t kind, uint32_t waves_per_simd, uint32_t iters, double* ms,
                          uint64_t* wave_insts);
