@@ -1783,7 +1783,8 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 // block, which touches no memory): what an instruction of a branch that no lane of the wave takes
 // costs when the compiler does not jump around it. KIND 9-12 issue other instructions the traversal
 // is made of, 16 per round: v_cndmask_b32 (selects of the child sort), v_max_f32 (slab tests),
-// v_cmp_lt_f32 (compares into a lane mask) and v_mov_b32.
+// v_cmp_lt_f32 (compares into a lane mask) and v_mov_b32; KIND 13 the select in its three-operand
+// form with the mask in an SGPR pair instead of VCC.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, float c, float* __restrict__ out) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1820,6 +1821,18 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, flo
                     "v_max_f32 %[a0], %[a0], %[a1]\n\tv_max_f32 %[a1], %[a1], %[a2]\n\t"
                     "v_max_f32 %[a2], %[a2], %[a3]\n\tv_max_f32 %[a3], %[a3], %[a0]"
                     : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3));
+            else if (KIND == 13)
+                asm volatile(
+                    "v_cmp_lt_f32 %[m0], %[a0], %[a1]\n\t"
+                    "v_cndmask_b32_e64 %[a0], %[a0], %[a1], %[m0]\n\tv_cndmask_b32_e64 %[a1], %[a1], %[a2], %[m0]\n\t"
+                    "v_cndmask_b32_e64 %[a2], %[a2], %[a3], %[m0]\n\tv_cndmask_b32_e64 %[a3], %[a3], %[a0], %[m0]\n\t"
+                    "v_cndmask_b32_e64 %[a0], %[a0], %[a1], %[m0]\n\tv_cndmask_b32_e64 %[a1], %[a1], %[a2], %[m0]\n\t"
+                    "v_cndmask_b32_e64 %[a2], %[a2], %[a3], %[m0]\n\tv_cndmask_b32_e64 %[a3], %[a3], %[a0], %[m0]\n\t"
+                    "v_cndmask_b32_e64 %[a0], %[a0], %[a1], %[m0]\n\tv_cndmask_b32_e64 %[a1], %[a1], %[a2], %[m0]\n\t"
+                    "v_cndmask_b32_e64 %[a2], %[a2], %[a3], %[m0]\n\tv_cndmask_b32_e64 %[a3], %[a3], %[a0], %[m0]\n\t"
+                    "v_cndmask_b32_e64 %[a0], %[a0], %[a1], %[m0]\n\tv_cndmask_b32_e64 %[a1], %[a1], %[a2], %[m0]\n\t"
+                    "v_cndmask_b32_e64 %[a2], %[a2], %[a3], %[m0]"
+                    : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3), [m0] "=&s"(m0));
             else if (KIND == 11)
                 asm volatile(
                     "v_cmp_lt_f32 %[m0], %[a0], %[a1]\n\tv_cmp_lt_f32 %[m1], %[a1], %[a2]\n\t"
@@ -1839,6 +1852,7 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, flo
                     "v_mov_b32 %[a0], %[a1]\n\tv_mov_b32 %[a1], %[a2]\n\tv_mov_b32 %[a2], %[a3]\n\tv_mov_b32 %[a3], %[a0]"
                     : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3));
             if (KIND == 11) a0 += (float)(uint32_t)(m0 ^ m1) * 0.0f;
+            (void)m1;
         }
         out[tid] = a0 + a1 + a2 + a3;
         return;
@@ -2067,7 +2081,8 @@ extern "C" int crt_k_valu_probe(hipStream_t s, int kind, uint32_t blocks, uint32
     else if (kind == 9) hipLaunchKernelGGL(k_valu_probe<9>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     else if (kind == 10) hipLaunchKernelGGL(k_valu_probe<10>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     else if (kind == 11) hipLaunchKernelGGL(k_valu_probe<11>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
-    else hipLaunchKernelGGL(k_valu_probe<12>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 12) hipLaunchKernelGGL(k_valu_probe<12>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else hipLaunchKernelGGL(k_valu_probe<13>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     return (int)hipGetLastError();
 }
 

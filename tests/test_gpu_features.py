@@ -197,6 +197,22 @@ def test_served_request_completes_during_a_blocking_render():
         assert t1 < t["end"]
 
 
+@pytest.mark.parametrize("order", ["small_then_oldest", "round_robin"])
+def test_serving_orders_give_the_blocking_images(order):
+    """Both feed orders of the serving loop render every request as its blocking render does: a
+    large request and small ones on two scenes, submitted together (the large one first)."""
+    with Renderer(serve_order=order, serve_small_paths=1 << 16) as r:
+        a = r.upload(scenes.sphere_mesh_scene(5000, seed=1))
+        b = r.upload(scenes.box_scene())
+        jobs = [(a, 512, 512, 16, 31), (b, 64, 64, 4, 32), (a, 96, 96, 1, 33), (b, 256, 256, 9, 34), (a, 48, 48, 4, 35)]
+        refs = [r.render(sid, W, H, spp, 8, seed=sd)[1] for sid, W, H, spp, sd in jobs]
+        ids = [r.submit(sid, W, H, spp, 8, seed=sd) for sid, W, H, spp, sd in jobs]
+        for rid, (sid, W, H, spp, sd), ref in zip(ids, jobs, refs):
+            _, acc, _ = r.wait(rid, W, H)
+            assert np.array_equal(acc, ref), (order, W, H, spp)
+        assert all(r.timing(rid)["status"] == 0 for rid in ids)
+
+
 def test_serving_image_only():
     """CRT_RENDER_IMAGE_ONLY: the served request returns the same 8-bit image as a full request
     and refuses to hand out an accumulation buffer it did not copy back."""
