@@ -429,7 +429,8 @@ def valu_probe(iters=100000):
         for kind, name in ((0, "v_fma_f32"), (1, "v_add_u32"), (2, "v_mul_f32"), (3, "v_fma_f32_lanes0_15"),
                            (4, "v_fma_f32_every4th_lane"), (5, "v_fma_f32_lanes0_31"), (6, "v_fma_f32_lanes0_62"),
                            (7, "v_fma_f32_lanes0_47"), (8, "v_fma_f32_no_lane"), (9, "v_cndmask_b32"),
-                           (10, "v_max_f32"), (11, "v_cmp_lt_f32"), (12, "v_mov_b32"), (13, "v_cndmask_b32_e64_sgpr")):
+                           (10, "v_max_f32"), (11, "v_cmp_lt_f32"), (12, "v_mov_b32"), (13, "v_cndmask_b32_e64_sgpr"),
+                           (14, "v_fma_f32_exec_lanes0_15"), (15, "v_fma_f32_exec_lanes0_31")):
             for wps in (2, 4, 8):
                 ms, n = C.c_double(), C.c_uint64()
                 rc = r.lib.crt_debug_valu_probe(r.h, kind, wps, iters, C.byref(ms), C.byref(n))

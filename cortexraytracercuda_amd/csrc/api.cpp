@@ -1219,8 +1219,8 @@ int crt_debug_fail_alloc(crt_context* c, int32_t n) {
 int crt_debug_valu_probe(crt_context* c, int32_t kind, uint32_t waves_per_simd, uint32_t iters, double* ms,
                          uint64_t* wave_insts) {
     if (!c || !ms || !wave_insts) return set_error(CRT_ERR_INVALID_ARGUMENT, "null argument");
-    if (kind < 0 || kind > 13 || waves_per_simd < 1 || waves_per_simd > 8 || iters < 1 || iters > (1u << 24))
-        return set_error(CRT_ERR_INVALID_ARGUMENT, "probe kind 0-13, 1-8 waves per SIMD, 1-2^24 rounds");
+    if (kind < 0 || kind > 15 || waves_per_simd < 1 || waves_per_simd > 8 || iters < 1 || iters > (1u << 24))
+        return set_error(CRT_ERR_INVALID_ARGUMENT, "probe kind 0-15, 1-8 waves per SIMD, 1-2^24 rounds");
     Context* ctx = reinterpret_cast<Context*>(c);
     ExecLock g(*ctx);
     // four SIMDs per CU, four waves per 256-lane workgroup: waves_per_simd workgroups per CU

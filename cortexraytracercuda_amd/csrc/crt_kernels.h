@@ -57,7 +57,8 @@ int crt_k_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 // VALU issue-rate calibration (kind 0: v_fma_f32, 1: v_add_u32, 2: v_mul_f32; 3-7: v_fma_f32 on
 // lanes 0-15, every fourth lane, lanes 0-31, 0-62, 0-47 of each wave; 8: v_fma_f32 with EXEC
 // cleared; 9: v_cndmask_b32 (one v_cmp per 15), 10: v_max_f32, 11: v_cmp_lt_f32, 12: v_mov_b32,
-// 13: v_cndmask_b32_e64 with the mask in an SGPR pair):
+// 13: v_cndmask_b32_e64 with the mask in an SGPR pair; 14 / 15: v_fma_f32 with EXEC set to lanes
+// 0-15 / 0-31 inside the asm):
 // blocks x 256 lanes, iters rounds of 16 wave instructions; one float per lane to out
 int crt_k_valu_probe(hipStream_t s, int kind, uint32_t blocks, uint32_t iters, float* out);
 uint32_t crt_k_stack_capacity(void);

@@ -1784,7 +1784,8 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 // costs when the compiler does not jump around it. KIND 9-12 issue other instructions the traversal
 // is made of, 16 per round: v_cndmask_b32 (selects of the child sort), v_max_f32 (slab tests),
 // v_cmp_lt_f32 (compares into a lane mask) and v_mov_b32; KIND 13 the select in its three-operand
-// form with the mask in an SGPR pair instead of VCC.
+// form with the mask in an SGPR pair instead of VCC. KIND 14 and 15 issue the KIND 8 block with EXEC
+// set to lanes 0-15 and to lanes 0-31 inside the asm block (every lane of the wave alive).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, float c, float* __restrict__ out) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1853,6 +1854,29 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float b, flo
                     : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3));
             if (KIND == 11) a0 += (float)(uint32_t)(m0 ^ m1) * 0.0f;
             (void)m1;
+        }
+        out[tid] = a0 + a1 + a2 + a3;
+        return;
+    }
+    if (KIND == 14 || KIND == 15) {
+        float a0 = (float)tid * 1e-7f, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f;
+        const uint64_t em = KIND == 14 ? 0xffffull : 0xffffffffull;
+        for (uint32_t i = 0; i < iters; ++i) {
+            uint64_t sv;
+            asm volatile(
+                "s_mov_b64 %[sv], exec\n\t"
+                "s_mov_b64 exec, %[em]\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "v_fma_f32 %[a0], %[b], %[a0], %[c]\n\tv_fma_f32 %[a1], %[b], %[a1], %[c]\n\t"
+                "v_fma_f32 %[a2], %[b], %[a2], %[c]\n\tv_fma_f32 %[a3], %[b], %[a3], %[c]\n\t"
+                "s_mov_b64 exec, %[sv]"
+                : [a0] "+v"(a0), [a1] "+v"(a1), [a2] "+v"(a2), [a3] "+v"(a3), [sv] "=&s"(sv)
+                : [b] "s"(b), [c] "v"(c), [em] "s"(em));
         }
         out[tid] = a0 + a1 + a2 + a3;
         return;
@@ -2082,7 +2106,9 @@ extern "C" int crt_k_valu_probe(hipStream_t s, int kind, uint32_t blocks, uint32
     else if (kind == 10) hipLaunchKernelGGL(k_valu_probe<10>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     else if (kind == 11) hipLaunchKernelGGL(k_valu_probe<11>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     else if (kind == 12) hipLaunchKernelGGL(k_valu_probe<12>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
-    else hipLaunchKernelGGL(k_valu_probe<13>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 13) hipLaunchKernelGGL(k_valu_probe<13>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else if (kind == 14) hipLaunchKernelGGL(k_valu_probe<14>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
+    else hipLaunchKernelGGL(k_valu_probe<15>, dim3(blocks), dim3(256), 0, s, iters, 0.999f, 1e-6f, out);
     return (int)hipGetLastError();
 }
 

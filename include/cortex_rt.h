@@ -571,8 +571,9 @@ int crt_debug_fail_alloc(crt_context* ctx, int32_t n);
  * running `iters` rounds of 16 independent chains of one instruction (kind 0: v_fma_f32, 1:
  * v_add_u32, 2: v_mul_f32; 3-7: v_fma_f32 with part of each wave active: lanes 0-15, every fourth
  * lane, lanes 0-31, lanes 0-62, lanes 0-47; 8: v_fma_f32 with no lane active; 9: v_cndmask_b32,
- * 10: v_max_f32, 11: v_cmp_lt_f32, 12: v_mov_b32, 13: v_cndmask_b32 with an SGPR-pair mask), timed
- * with HIP events after one untimed launch. Returns the GPU time and the
+ * 10: v_max_f32, 11: v_cmp_lt_f32, 12: v_mov_b32, 13: v_cndmask_b32 with an SGPR-pair mask; 14 and
+ * 15: v_fma_f32 with EXEC set to lanes 0-15 and 0-31, every lane alive), timed with HIP events after
+ * one untimed launch. Returns the GPU time and the
  * wave instructions the rounds issue (waves x iters x 16). */
 int crt_debug_valu_probe(crt_context* ctx, int32_t kind, uint32_t waves_per_simd, uint32_t iters, double* ms,
                          uint64_t* wave_insts);
