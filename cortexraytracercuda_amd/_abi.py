@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcortex_rt.so")
 
-CRT_ABI_VERSION = 2
+CRT_ABI_VERSION = 3
 
 CRT_OK = 0
 CRT_ERR_NO_DEVICE = 1
@@ -97,6 +97,18 @@ class crt_area_light(C.Structure):
     _fields_ = [("center", f3), ("u", f3), ("v", f3), ("rgb", f3), ("intensity", C.c_float), ("invisible", C.c_uint32)]
 
 
+class crt_transform(C.Structure):
+    _fields_ = [("has", C.c_uint32), ("translate", f3), ("rotate_deg", f3), ("scale", f3)]
+
+
+CRT_SHAPE_DISK, CRT_SHAPE_CYLINDER, CRT_SHAPE_CONE = 0, 1, 2
+
+
+class crt_shape(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("center", f3), ("radius", C.c_float), ("height", C.c_float),
+                ("material", C.c_uint32), ("mapping", crt_uvmap), ("transform", crt_transform)]
+
+
 class crt_mesh(C.Structure):
     _fields_ = [("positions", C.POINTER(C.c_float)), ("num_vertices", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
                 ("num_triangles", C.c_uint32), ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
@@ -112,8 +124,6 @@ class crt_sphere_light(C.Structure):
                 ("invisible", C.c_uint32), ("direction", f3), ("cutoff_deg", C.c_float), ("falloff", C.c_float)]
 
 
-class crt_transform(C.Structure):
-    _fields_ = [("has", C.c_uint32), ("translate", f3), ("rotate_deg", f3), ("scale", f3)]
 
 
 class crt_scene_desc(C.Structure):
@@ -125,7 +135,8 @@ class crt_scene_desc(C.Structure):
                 ("quads", C.POINTER(crt_quad)), ("num_quads", C.c_uint32),
                 ("meshes", C.POINTER(crt_mesh)), ("num_meshes", C.c_uint32),
                 ("instances", C.POINTER(crt_instance)), ("num_instances", C.c_uint32),
-                ("sphere_lights", C.POINTER(crt_sphere_light)), ("num_sphere_lights", C.c_uint32)]
+                ("sphere_lights", C.POINTER(crt_sphere_light)), ("num_sphere_lights", C.c_uint32),
+                ("shapes", C.POINTER(crt_shape)), ("num_shapes", C.c_uint32)]
 
 
 class crt_render_desc(C.Structure):
@@ -214,7 +225,7 @@ EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "
             "crt_builder_image_texture", "crt_builder_image_texture_pixels", "crt_builder_lambertian", "crt_builder_metal",
             "crt_builder_dielectric", "crt_builder_diffuse_light", "crt_builder_has_material",
             "crt_builder_directional_light", "crt_builder_omni_light", "crt_builder_spot_light", "crt_builder_sphere",
-            "crt_builder_quad", "crt_builder_box", "crt_builder_obj_mesh", "crt_builder_note_unsupported",
+            "crt_builder_quad", "crt_builder_box", "crt_builder_shape", "crt_builder_obj_mesh", "crt_builder_note_unsupported",
             "crt_builder_desc"]
 
 _P = C.POINTER
@@ -271,6 +282,8 @@ _SIGS = {
     "crt_builder_quad": ([C.c_void_p, C.c_char_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_char_p,
                           _P(crt_transform)], C.c_int),
     "crt_builder_box": ([C.c_void_p, C.c_char_p, _P(C.c_float), _P(C.c_float), C.c_char_p, _P(crt_transform)], C.c_int),
+    "crt_builder_shape": ([C.c_void_p, C.c_uint32, C.c_char_p, _P(C.c_float), C.c_float, C.c_float, C.c_char_p,
+                           _P(crt_uvmap), _P(crt_transform)], C.c_int),
     "crt_builder_obj_mesh": ([C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32], C.c_int),
     "crt_builder_note_unsupported": ([C.c_void_p, C.c_uint32], C.c_int),
     "crt_builder_desc": ([C.c_void_p, _P(crt_scene_desc), _P(C.c_uint32), _P(C.c_uint32)], C.c_int),

@@ -364,7 +364,7 @@ static uint64_t device_build_bound(const crt_scene_desc* d, uint32_t width, uint
         nt += d->meshes[m].num_triangles;
         nrm = nrm || d->meshes[m].normals; uvs = uvs || d->meshes[m].uvs;
     }
-    const uint64_t nq = (uint64_t)d->num_quads + d->num_lights, ns = (uint64_t)d->num_spheres + d->num_sphere_lights,
+    const uint64_t nq = (uint64_t)d->num_quads + d->num_lights + d->num_shapes, ns = (uint64_t)d->num_spheres + d->num_sphere_lights,
                    n = nt + ns + nq;
     return n * (width == 4 ? 128u : 64u) + nt * 48u + ns * 32u + nq * 80u + (nrm ? nt * 48u : 0u) + (uvs ? nt * 24u : 0u) +
            ns * 24u + ((uint64_t)d->num_lights + d->num_sphere_lights) * 4u + tables + 4 * 256u;
@@ -381,7 +381,7 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
     if (slot == kMaxScenes) return set_error(CRT_ERR_OVER_BUDGET, "scene table full");
     uint64_t nprims = 0;
     if (desc && desc->abi_version == CRT_ABI_VERSION) {
-        nprims = (uint64_t)desc->num_lights + desc->num_sphere_lights + desc->num_spheres + desc->num_quads;
+        nprims = (uint64_t)desc->num_lights + desc->num_sphere_lights + desc->num_spheres + desc->num_quads + desc->num_shapes;
         for (uint32_t m = 0; m < desc->num_meshes && desc->meshes; ++m) nprims += desc->meshes[m].num_triangles;
     }
     const bool on_device = bvh_builder == CRT_BVH_DEVICE && desc && desc->abi_version == CRT_ABI_VERSION &&
@@ -395,7 +395,7 @@ int Context::upload_scene_locked(const crt_scene_desc* desc, uint32_t leaf, uint
             const crt_texture& t = desc->textures[k];
             tables += 32u + (t.type == CRT_TEX_IMAGE ? (uint64_t)t.width * t.height * t.channels : 0u);
         }
-        tables += ((uint64_t)desc->num_materials + desc->num_lights + 2ull * desc->num_sphere_lights) * 32u;
+        tables += ((uint64_t)desc->num_materials + desc->num_lights + 2ull * desc->num_sphere_lights + desc->num_shapes) * 32u;
         if (scene_bytes + request_bytes + reserved + device_build_bound(desc, width, tables) > budget)
             return set_error(CRT_ERR_OVER_BUDGET, "scene exceeds the HBM budget");
         int rc = gpu_build_scene(*this, desc, leaf, width, *sc, err);
@@ -649,11 +649,13 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
     }
     uint32_t npaths = base;
     int32_t uniform_scene = (int32_t)items[0].job->req.scene;
-    int two_level = 0;
+    int two_level = 0, full_shade = 0, any_shapes = 0;
     const uint32_t width = items[0].job->scene->host.bvh_width;   // the traversal kernel follows the scenes' BVH
     for (auto& w : items) {
         if ((int32_t)w.job->req.scene != uniform_scene) uniform_scene = -1;
         if (w.job->scene->host.num_instances) two_level = 1;
+        if (w.job->scene->host.num_instances || w.job->scene->host.num_shapes) full_shade = 1;
+        if (w.job->scene->host.num_shapes) any_shapes = 1;
         if (w.job->scene->host.bvh_width != width)
             return set_error(CRT_ERR_INVALID_ARGUMENT, "scenes of different BVH widths in one pass");
     }
@@ -725,7 +727,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
             if (crt_k_regen(u.s, mode, u.sgrid, dreq, nreq, u.lo, total, u.nslots, cin, cout, nullptr, A))
                 return set_error(CRT_ERR_HIP, "regen launch failed");
             if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], u.s));
-            if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)width, two_level, u.egrid, dsc, dreq, uniform_scene,
+            if (crt_k_extend(u.s, config.count_work ? 1 : 0, (int)width, two_level ? 1 + any_shapes : 0, u.egrid, dsc, dreq, uniform_scene,
                              nullptr, cin + CRT_CTL_LIVE, 0, A.ray_o, A.ray_d, A.ids, hq, hiq, gq, nullptr, extend_mode,
                              cin + CRT_CTL_WORK, extend_param, (int)stack_lds, cin + CRT_CTL_SURV, B.counters))
                 return set_error(CRT_ERR_HIP, "extend launch failed");
@@ -737,7 +739,7 @@ int Context::run_pass(std::vector<WorkItem>& items, bool time_stages, double* ex
                 if (extend_ms) *extend_ms += ms;
                 extend_launches++;
             }
-            if (crt_k_shade(u.s, mode, two_level, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, Bn, cout + CRT_CTL_SURV, hq, hiq,
+            if (crt_k_shade(u.s, mode, full_shade, u.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, Bn, cout + CRT_CTL_SURV, hq, hiq,
                             B.out_rad, nullptr, (int)ray_sort, nreq == 1 ? 1 : 0, B.counters))
                 return set_error(CRT_ERR_HIP, "shade launch failed");
             HIP_TRY(hipMemcpyAsync(ring + (k % kCtlRing) * CRT_CTL_WORDS, cout, sizeof(uint32_t) * CRT_CTL_WORDS,
@@ -775,7 +777,7 @@ int Context::run_plain(Job& j, bool time_stages, double* extend_ms) {
     int rc = ensure_gstack(grid);
     if (rc) return rc;
     if (time_stages) HIP_TRY(hipEventRecord(ev_ext[0], stream));
-    if (crt_k_plain(stream, (int)j.rd.rng_mode, (int)j.scene->host.bvh_width, j.scene->host.num_instances ? 1 : 0, grid,
+    if (crt_k_plain(stream, (int)j.rd.rng_mode, (int)j.scene->host.bvh_width, j.scene->host.num_instances ? (j.scene->host.num_shapes ? 2 : 1) : 0, grid,
                     (const crt_dev_scene*)scene_table.p, dreq, (uint32_t*)gstack.p, (crt_dev_counters*)counters.p))
         return set_error(CRT_ERR_HIP, "megakernel launch failed");
     if (time_stages) {
@@ -1065,7 +1067,7 @@ int crt_trace_rays(crt_context* c, crt_scene_id s, const float* rays, uint64_t n
     if ((rc = di.alloc(n * 4, ctx))) return rc;
     if ((rc = ctx->reset_counters((crt_dev_counters*)ctx->counters.p))) return rc;
     uint32_t grid = std::min(ctx->extend_grid, (uint32_t)std::max<uint64_t>(1, (n + 255) / 256));
-    const int two_level = sc->host.num_instances ? 1 : 0;
+    const int two_level = sc->host.num_instances ? (sc->host.num_shapes ? 2 : 1) : 0;
     if (flags & CRT_TRACE_STAGE) {
         // the render's closest-hit stage: path-state rays (origin, time 0; direction, depth 1 and the
         // camera bit), one uniform scene, the stage's control words

@@ -49,6 +49,7 @@ struct TypeCountAdd {
 // [Q + S, N).
 struct Prims {
     const float4* quads;     // 5 per quad
+    const float4* qboxes;    // 2 per quad record: lo, hi (computed on the host, shapes included)
     const float4* spheres;   // 2 per sphere
     const float4* tris;      // 3 per triangle
     uint32_t nq, ns, nt;
@@ -65,23 +66,14 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-// Box of one primitive, as the host build computes it (scene_build.cpp).
+// Box of one primitive, as the host build computes it (scene_build.cpp; quad and shape boxes come
+// from the host).
 __device__ __forceinline__ void prim_box(const Prims& P, uint32_t p, float3& lo, float3& hi) {
 #pragma clang fp contract(off)
     if (p < P.nq) {
-        const float4* q = P.quads + 5 * (size_t)p;
-        const float4 q0 = q[0], q1 = q[1], q2 = q[2];
-        float3 c0 = make_float3(q0.x, q0.y, q0.z);
-        const float3 u = make_float3(q1.x, q1.y, q1.z), v = make_float3(q2.x, q2.y, q2.z);
-        if (__float_as_uint(q1.w) & CRT_QF_LIGHT) {
-            c0 = make_float3(c0.x - (0.5f * u.x + 0.5f * v.x), c0.y - (0.5f * u.y + 0.5f * v.y), c0.z - (0.5f * u.z + 0.5f * v.z));
-        }
-        const float3 c1 = make_float3(c0.x + u.x, c0.y + u.y, c0.z + u.z), c2 = make_float3(c0.x + v.x, c0.y + v.y, c0.z + v.z);
-        const float3 c3 = make_float3(c1.x + v.x, c1.y + v.y, c1.z + v.z);
-        lo = make_float3(fminf(fminf(c0.x, c1.x), fminf(c2.x, c3.x)), fminf(fminf(c0.y, c1.y), fminf(c2.y, c3.y)),
-                         fminf(fminf(c0.z, c1.z), fminf(c2.z, c3.z)));
-        hi = make_float3(fmaxf(fmaxf(c0.x, c1.x), fmaxf(c2.x, c3.x)), fmaxf(fmaxf(c0.y, c1.y), fmaxf(c2.y, c3.y)),
-                         fmaxf(fmaxf(c0.z, c1.z), fmaxf(c2.z, c3.z)));
+        const float4 a = P.qboxes[2 * (size_t)p], b = P.qboxes[2 * (size_t)p + 1];
+        lo = make_float3(a.x, a.y, a.z);
+        hi = make_float3(b.x, b.y, b.z);
     } else if (p < P.nq + P.ns) {
         const float4 s = P.spheres[2 * (size_t)(p - P.nq)];
         const float r = fabsf(s.w);
@@ -462,7 +454,7 @@ int gpu_build_scene(Context& C, const crt_scene_desc* d, uint32_t leaf, uint32_t
     if (rc) return rc;
     SourceRecords src;
     unsigned hw = std::thread::hardware_concurrency();
-    if ((rc = flatten_sources(d, (int)std::min(16u, hw ? hw : 1u), src, err))) return rc;
+    if ((rc = flatten_sources(d, H.shape_map_base, (int)std::min(16u, hw ? hw : 1u), src, err))) return rc;
     const uint32_t nq = (uint32_t)(src.quads.size() / 5), ns = (uint32_t)(src.spheres.size() / 2),
                    nt = (uint32_t)(src.tris.size() / 3);
     const uint64_t n64 = (uint64_t)nq + ns + nt;
@@ -472,7 +464,7 @@ int gpu_build_scene(Context& C, const crt_scene_desc* d, uint32_t leaf, uint32_t
     hipStream_t s = C.stream;
     Context* ctx = &C;
     // source records on the device
-    DevBuf squads, ssph, stris, snrm, suv, smap;
+    DevBuf squads, sqbox, ssph, stris, snrm, suv, smap;
     auto up = [&](DevBuf& b, const void* h, size_t bytes) -> int {
         if (!bytes) return CRT_OK;
         int r = b.alloc(bytes, ctx);
@@ -481,12 +473,13 @@ int gpu_build_scene(Context& C, const crt_scene_desc* d, uint32_t leaf, uint32_t
         return CRT_OK;
     };
     if ((rc = up(squads, src.quads.data(), src.quads.size() * sizeof(F4)))) return rc;
+    if ((rc = up(sqbox, src.quad_boxes.data(), src.quad_boxes.size() * sizeof(F4)))) return rc;
     if ((rc = up(ssph, src.spheres.data(), src.spheres.size() * sizeof(F4)))) return rc;
     if ((rc = up(smap, src.sphere_map.data(), src.sphere_map.size() * sizeof(F2)))) return rc;
     if ((rc = up(stris, src.tris.data(), src.tris.size() * sizeof(F4)))) return rc;
     if ((rc = up(snrm, src.tri_normals.data(), src.tri_normals.size() * sizeof(F4)))) return rc;
     if ((rc = up(suv, src.tri_uvs.data(), src.tri_uvs.size() * sizeof(F2)))) return rc;
-    Prims P{dptr<float4>(squads), dptr<float4>(ssph), dptr<float4>(stris), nq, ns, nt};
+    Prims P{dptr<float4>(squads), dptr<float4>(sqbox), dptr<float4>(ssph), dptr<float4>(stris), nq, ns, nt};
 
     // work buffers
     DevBuf plo, phi, keys[2], vals[2], cb, cl, cr, parent, lparent, nlo, nhi, first, count, tmask, visits, tflags, ranks,
@@ -659,6 +652,7 @@ int gpu_build_scene(Context& C, const crt_scene_desc* d, uint32_t leaf, uint32_t
     ds.num_lights = src.num_lights; ds.root = 0; ds.num_quads = d->num_quads; ds.bvh_width = width;
     for (int k = 0; k < 3; ++k) ds.background[k] = H.background[k];
     H.num_lights = src.num_lights; H.num_spheres = ns; H.num_quads = d->num_quads; H.num_instances = 0;
+    H.num_shapes = d->num_shapes;
     H.num_quad_records = nq;
     H.num_world_tris = nt;
     H.root = 0;

@@ -37,10 +37,13 @@ int crt_k_extend(hipStream_t s, int count_mode, int width, int two_level, uint32
                  const uint32_t* first_camera, crt_dev_counters* counters);
 // first_camera (nullable): queue index where the camera rays of the launch begin (the survivors of
 // the last shade stage come first); the lane-refill kernel keeps camera-ray waves whole
-// two_level: an instanced scene has paths in the launch (selects the unbounded-register variant);
+// two_level: 0 one-level scenes only, 1 an instanced scene has paths in the launch, 2 and a scene
+// with analytic shapes too (the two-level kernels with the shape test; crt_k_extend, crt_k_plain and
+// crt_k_trace_rays read it the same way);
 // live_min (nullable, grid words): per block, the smallest path id still alive after the stage;
-// one_request: every path of the launch belongs to reqs[0]
-int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t grid, const crt_dev_scene* scenes,
+// one_request: every path of the launch belongs to reqs[0]; full: a scene of the launch is instanced
+// or has analytic shapes (the unbounded kernel with the shapes' shading)
+int crt_k_shade(hipStream_t s, int rng_mode, int full, uint32_t grid, const crt_dev_scene* scenes,
                 const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout, uint32_t* count_out, const float4* hit,
                 const uint32_t* hit_inst, float4* out_rad, uint32_t* live_min, int sort_octant, int one_request,
                 crt_dev_counters* counters);

@@ -14,6 +14,10 @@
 //   tris    : 3 x float4 per triangle in leaf order: (v0, gid) (v1, material) (v2, flags)
 //   spheres : 2 x float4 per sphere in leaf order: (centre, radius) (gid, material, -, -) as bits
 //   quads   : 5 x float4 per quad/area light: (Q, D) (u, flags) (v, area) (n, gid) (w, material)
+//             or per analytic shape (disk, cylinder, cone; flags CRT_QF_DISK.. below), whose hit runs
+//             in its own frame: local = A x + b for a world point x, with
+//             (A row 0, b.x) (radius, height, -, flags) (A row 1, b.y) (A row 2, gid)
+//             (b.z, uv-mapping slot in the material table as uint, -, material)
 //   insts   : 4 x float4 per instance: three rows of world->object [R|t] and
 //             (blas_root, instance id, -, -) as int
 // Materials (2 x float4), textures (2 x float4) and one texel byte pool complete the scene.
@@ -39,6 +43,11 @@
 #define CRT_QF_LIGHT 1u        // centred parametrisation, interior in [-0.5, 0.5]^2, uv shifted by 0.5
 #define CRT_QF_INVISIBLE 2u    // hidden from camera rays (directional_light.cuh:106-109)
 #define CRT_QF_ONE_SIDED 4u    // back faces trigger the pass-through re-trace
+#define CRT_QF_DISK 8u         // analytic shapes (records in the quad array)
+#define CRT_QF_CYLINDER 16u
+#define CRT_QF_CONE 32u
+#define CRT_QF_SHAPE (CRT_QF_DISK | CRT_QF_CYLINDER | CRT_QF_CONE)
+#define CRT_QF_UNIT_NORMAL 64u // shape under a scale: the world normal is renormalised (scale.cuh:50-56)
 // triangle flags (float4 #2 .w as uint)
 #define CRT_TF_SMOOTH 1u
 #define CRT_TF_UV 2u

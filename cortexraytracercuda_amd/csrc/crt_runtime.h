@@ -175,13 +175,15 @@ struct Server {
     uint32_t appends = 0;
     bool full = false;                     // no room for another segment: drain, then reset
     bool two_level = false;                // an instanced scene has paths in the open pass
+    bool shapes = false;                   // a scene with analytic shapes has paths in the open pass
     bool mixed_rng = false;                // a MINSTD request has paths in the open pass (per-path RNG mode)
     uint32_t rr = 0;                       // round-robin start among active requests
     uint64_t progress_iter = 0;            // last iteration whose snapshot showed finished paths
     bool use_graphs = true;                // replay captured iterations
-    // kernel variant of an iteration: two_level * 2 + mixed_rng
-    uint32_t warm[4] = {};                 // eager iterations per variant (graphs are captured after kRing)
-    hipGraphExec_t graph[4][kRing] = {};   // [variant][snapshot slot]
+    // kernel variant of an iteration: two_level * 4 + shapes * 2 + mixed_rng
+    static constexpr int kVariants = 8;
+    uint32_t warm[kVariants] = {};                 // eager iterations per variant (graphs are captured after kRing)
+    hipGraphExec_t graph[kVariants][kRing] = {};   // [variant][snapshot slot]
 };
 
 struct Context {

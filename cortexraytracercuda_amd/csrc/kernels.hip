@@ -338,10 +338,82 @@ __device__ __forceinline__ bool sphere_hit(V3 o, V3 d, float4 s, float tmin, flo
 
 // Quad / area light: quad::hit (core/src/primitives/quad.cuh:100-133) and directional_light::hit
 // (core/src/lights/directional_light.cuh:76-118), inclusive interval. `key` = the quad's global id.
+// Analytic shapes, whose records share the quad array (flags CRT_QF_DISK / CYLINDER / CONE). The ray
+// enters the shape's frame as under the reference's scale / rotate / translate wrappers
+// (local = A x + b, composed on the host; untransformed, local origin = origin - centre exactly).
+__device__ __forceinline__ void shape_local_ray(const float4& q0, const float4& q2, const float4& q3, const float4& q4,
+                                                V3 o, V3 d, V3& lo, V3& ld) {
+    lo = v3(q0.x * o.x + q0.y * o.y + q0.z * o.z + q0.w, q2.x * o.x + q2.y * o.y + q2.z * o.z + q2.w,
+            q3.x * o.x + q3.y * o.y + q3.z * o.z + q4.x);
+    ld = v3(q0.x * d.x + q0.y * d.y + q0.z * d.z, q2.x * d.x + q2.y * d.y + q2.z * d.z, q3.x * d.x + q3.y * d.y + q3.z * d.z);
+}
+
+// disk::hit (core/src/primitives/disk.cuh:53-82): the plane y = 0 of the frame, radius r;
+// cylinder::hit (cylinder.cuh:63-103): the open tube x^2 + z^2 = r^2, 0 <= y <= h, the near root
+// taken when it lies in the interval, else the far one, and then rejected when outside the height
+// (the far root is not tried again); cone::hit (cone.cuh:58-128): x^2 + z^2 = (r/h)^2 (h - y)^2,
+// 0 <= y <= h, rejected when |discriminant| < 0.001. The interval is closed. Comparisons are
+// written so that a NaN or infinite distance (a ray parallel to the disk or the axis) is no hit,
+// where the reference's would accept some of them.
+__device__ __forceinline__ bool shape_hit(V3 o, V3 d, const float4* q, uint32_t flags, float tmin, float tmax, float& t) {
+    // the frame's rows are read again here, one at a time (they are in the cache), so the shape path
+    // holds no more registers than the quad path it shares the traversal loop with
+    V3 lo, ld;
+    float4 row = ldg(q);
+    lo.x = row.x * o.x + row.y * o.y + row.z * o.z + row.w;
+    ld.x = row.x * d.x + row.y * d.y + row.z * d.z;
+    row = ldg(q + 2);
+    lo.y = row.x * o.x + row.y * o.y + row.z * o.z + row.w;
+    ld.y = row.x * d.x + row.y * d.y + row.z * d.z;
+    row = ldg(q + 3);
+    const float bz = ldg(q + 4).x;
+    lo.z = row.x * o.x + row.y * o.y + row.z * o.z + bz;
+    ld.z = row.x * d.x + row.y * d.y + row.z * d.z;
+    const float4 q1 = ldg(q + 1);
+    const float r = q1.x, h = q1.y;
+    if (flags & CRT_QF_DISK) {
+        const float tt = (-lo.y) / ld.y;
+        if (!(tmin <= tt && tt <= tmax)) return false;
+        const float hx = lo.x + tt * ld.x, hz = lo.z + tt * ld.z;
+        if (!(hx * hx + hz * hz <= r * r)) return false;
+        t = tt;
+        return true;
+    }
+    // one quadric for both: a = dx^2 + dz^2 - k dy^2, b = 2 x dx + 2 z dz + 2 k D dy,
+    // c = x^2 + z^2 - (k D^2 + m r^2) with D = h - y, k = 0 and m = 1 for the cylinder, k = (r/h)^2
+    // and m = 0 for the cone: each term is the reference's own expression (scaling by 2 and 4 and
+    // adding zero are exact), the cone's threshold is 0 for the cylinder
+    const bool cone = (flags & CRT_QF_CONE) != 0;
+    const float D = h - lo.y;
+    const float k = cone ? (r / h) * (r / h) : 0.0f;
+    const float m = cone ? 0.0f : r * r;
+    const float a = ld.x * ld.x + ld.z * ld.z - k * (ld.y * ld.y);
+    const float b = 2.0f * lo.x * ld.x + 2.0f * lo.z * ld.z + 2.0f * k * D * ld.y;
+    const float c = lo.x * lo.x + lo.z * lo.z - (k * (D * D) + m);
+    const float disc = b * b - 4.0f * (a * c);
+    if (fabsf(disc) < (cone ? 0.001f : 0.0f) || !(disc >= 0.0f)) return false;
+    const float sq = sqrtf(disc);
+    float root = (-b - sq) / (2.0f * a);
+    if (!(tmin <= root && root <= tmax)) {
+        root = (-b + sq) / (2.0f * a);
+        if (!(tmin <= root && root <= tmax)) return false;
+    }
+    const float y = lo.y + root * ld.y;
+    if (!(y >= 0.0f && y <= h)) return false;
+    t = root;
+    return true;
+}
+
+// SHAPES = false for records known to be quads or area lights (the light table).
+template <bool SHAPES = true>
 __device__ __forceinline__ bool quad_hit(V3 o, V3 d, const float4* q, float tmin, float tmax, bool camera,
                                          float& t, float& qa, float& qb, uint32_t& key) {
     float4 q0 = ldg(q), q1 = ldg(q + 1), q2 = ldg(q + 2), q3 = ldg(q + 3), q4 = ldg(q + 4);
     key = fbits(q3.w);
+    if (SHAPES && (fbits(q1.w) & CRT_QF_SHAPE)) {
+        qa = 0.0f; qb = 0.0f;
+        return shape_hit(o, d, q, fbits(q1.w), tmin, tmax, t);
+    }
     V3 n = v3(q3);
     float denom = dot(n, d);
     if (fabsf(denom) < 1e-8f) return false;
@@ -480,7 +552,12 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t
 // move the ray to object space and descend into the instance's BLAS. The traversal is a resumable
 // state machine: step() performs one node or leaf visit, so a persistent kernel can interleave
 // traversal steps with fetching new rays for finished lanes.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool EXACT> struct Traverser {
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, int TL, bool EXACT> struct Traverser {
+    // TL: 0 one level (analytic shapes among the quads), 1 two levels without shapes, 2 two levels
+    // with shapes (the shape test costs the two-level loop registers, so it is compiled in only
+    // where a scene of the launch has shapes)
+    static constexpr bool TWO_LEVEL = TL != 0;
+    static constexpr bool SHAPES = TL != 1;
     Stack<BLOCK, RS, LS, GS> st;
     V3 o, d, id, ow, dw;
     uint32_t oct;          // bit a set where inv_d[a] < 0: the hi plane of axis a is the near one;
@@ -609,7 +686,7 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
                 float t, qa, qb;
                 uint32_t key;
                 if (COUNT) tc.quads++;
-                if (quad_hit(o, d, S.quad(first + k), tmin, best.t * CRT_TIE_HI, camera, t, qa, qb, key) &&
+                if (quad_hit<SHAPES>(o, d, S.quad(first + k), tmin, best.t * CRT_TIE_HI, camera, t, qa, qb, key) &&
                     hit_wins(t, key, best)) {
                     best.t = t; best.u = qa; best.v = qb; best.prim = (CRT_PRIM_QUAD << 30) | (first + k);
                     best.inst = cur_inst; best.key = key;
@@ -712,10 +789,10 @@ template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEV
     }
 };
 
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool EXACT>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, int TL, bool EXACT>
 __device__ __forceinline__ void trace(const SceneView& S, V3 ow, V3 dw, float tmin, float tmax, bool camera,
                                       uint32_t* lds_col, GSlot gstk, HitRec& best, TravCount& tc, bool& ovf) {
-    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, EXACT> tr;
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TL, EXACT> tr;
     tr.init(S, ow, dw, tmin, tmax, camera, lds_col, gstk);
     while (!tr.step(S, tc)) {
     }
@@ -926,7 +1003,7 @@ __global__ __launch_bounds__(256) void k_regen(const crt_dev_request* __restrict
 // A query left undecided by a non-EXACT triangle test (its result distance set to -1) returns with
 // `amb` set and counts nothing: the caller re-traces it with an EXACT traversal, which answers and
 // counts it.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL, bool EXACT>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, int TL, bool EXACT>
 __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
                                                 GSlot gst, HitRec& h, TravCount& tc, bool& ovf, uint32_t& nrays,
                                                 uint32_t& nretrace, bool& amb) {
@@ -935,7 +1012,7 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
     amb = false;
     for (;;) {
         bool o1 = false;
-        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, EXACT>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc,
+        trace<BLOCK, RS, LS, GS, COUNT, WIDTH, TL, EXACT>(S, o, d, tmin, 3.402823466e+38f, camera, lds_col, gst, h, tc,
                                                                  o1);
         ovf = ovf || o1;
         if (!EXACT && h.t < 0.0f) { amb = true; return 0u; }
@@ -955,13 +1032,13 @@ __device__ __forceinline__ uint32_t closest_hit(const SceneView& S, V3 o, V3 d, 
 // traversal loop). Out of line it would be allocated without the kernel's register bound: the
 // whole kernel would then take its count (180-210 VGPRs, two waves per SIMD).
 struct ExactHit { HitRec h; uint32_t pass_flag, ovf; };
-template <int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
+template <int BLOCK, int RS, int LS, int GS, int WIDTH, int TL>
 __device__ __forceinline__ ExactHit exact_hit(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col, GSlot gst) {
     ExactHit r;
     TravCount tc = {0, 0, 0, 0, 0, 0};
     bool ovf = false, amb = false;
     uint32_t nrays = 0, nretrace = 0;
-    r.pass_flag = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, o, d, camera, lds_col, gst, r.h, tc, ovf,
+    r.pass_flag = closest_hit<BLOCK, RS, LS, GS, false, WIDTH, TL, true>(S, o, d, camera, lds_col, gst, r.h, tc, ovf,
                                                                                nrays, nretrace, amb);
     r.ovf = ovf ? 1u : 0u;
     return r;
@@ -969,16 +1046,16 @@ __device__ __forceinline__ ExactHit exact_hit(const SceneView& S, V3 o, V3 d, bo
 
 // Closest hit with the float triangle test and, for the rare query it leaves undecided, the exact
 // re-trace (counted in nexact); answers and counts as closest_hit<EXACT = true> would.
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, bool TWO_LEVEL>
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, int WIDTH, int TL>
 __device__ __forceinline__ uint32_t closest_hit_checked(const SceneView& S, V3 o, V3 d, bool camera, uint32_t* lds_col,
                                                         GSlot gst, HitRec& h, TravCount& tc, uint32_t& novf,
                                                         uint32_t& nrays, uint32_t& nretrace, uint32_t& nexact) {
     bool ovf = false, amb = false;
-    uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, COUNT>(S, o, d, camera, lds_col, gst, h, tc,
+    uint32_t pass_flag = closest_hit<BLOCK, RS, LS, GS, COUNT, WIDTH, TL, COUNT>(S, o, d, camera, lds_col, gst, h, tc,
                                                                                       ovf, nrays, nretrace, amb);
     novf += ovf ? 1u : 0u;
     if (!COUNT && amb) {
-        const ExactHit e = exact_hit<BLOCK, RS, LS, GS, WIDTH, TWO_LEVEL>(S, o, d, camera, lds_col, gst);
+        const ExactHit e = exact_hit<BLOCK, RS, LS, GS, WIDTH, TL>(S, o, d, camera, lds_col, gst);
         h = e.h;
         pass_flag = e.pass_flag;
         novf += e.ovf;
@@ -1003,8 +1080,8 @@ constexpr int extend_min_waves(bool count, bool uniform, bool two_level) {
     return count ? 1 : (two_level ? 5 : (uniform ? 6 : 5));
 }
 
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
-__global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL)) void k_extend(const crt_dev_scene* __restrict__ scenes,
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, int TL>
+__global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TL != 0)) void k_extend(const crt_dev_scene* __restrict__ scenes,
                                                   const crt_dev_request* __restrict__ reqs, uint32_t uniform_scene,
                                                   const uint32_t* __restrict__ queue, const uint32_t* __restrict__ count_ptr,
                                                   uint32_t count_direct, const float4* __restrict__ ray_o,
@@ -1034,7 +1111,7 @@ __global__ __launch_bounds__(BLOCK, extend_min_waves(COUNT, UNIFORM, TWO_LEVEL))
         V3 o = v3(ro), d = v3(rdv);
         bool camera = (state & ST_CAMERA) != 0;
         HitRec h;
-        const uint32_t pass_flag = closest_hit_checked<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL>(
+        const uint32_t pass_flag = closest_hit_checked<BLOCK, RS, LS, GS, COUNT, WIDTH, TL>(
             S, o, d, camera, lds_col, gst, h, tc, novf, nrays, nretrace, nexact);
         hit[slot] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
         hit_inst[slot] = h.inst | pass_flag;
@@ -1086,8 +1163,8 @@ constexpr int extend_pt_min_waves(bool count, bool uniform, bool two_level, int 
     return count ? 1 : (two_level ? 5 : 6);
 }
 
-template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, bool TWO_LEVEL>
-__global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVEL, WIDTH)) void k_extend_pt(const crt_dev_scene* __restrict__ scenes,
+template <int BLOCK, int RS, int LS, int GS, bool COUNT, bool UNIFORM, int WIDTH, int TL>
+__global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TL != 0, WIDTH)) void k_extend_pt(const crt_dev_scene* __restrict__ scenes,
                                                      const crt_dev_request* __restrict__ reqs, uint32_t uniform_scene,
                                                      const uint32_t* __restrict__ queue,
                                                      const uint32_t* __restrict__ count_ptr, uint32_t count_direct,
@@ -1108,7 +1185,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
     uint32_t nrays = 0, nretrace = 0, novf = 0, nexact = 0;
     SceneView S;
     if (UNIFORM) S = scene_view(scenes + uniform_scene);
-    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TWO_LEVEL, COUNT> tr;
+    Traverser<BLOCK, RS, LS, GS, COUNT, WIDTH, TL, COUNT> tr;
     bool active = false, exhausted = false;
     uint32_t slot = 0, pass_flag = 0;
     // Survivors of the last shade stage, queue [0, nsec): this wave's contiguous share [nxt, wend),
@@ -1165,7 +1242,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
             novf += tr.st.overflow ? 1u : 0u;
             if (!COUNT && h.t < 0.0f) {
                 // undecided triangle test: the lane re-traces the query exactly (rare, divergent)
-                const ExactHit e = exact_hit<BLOCK, RS, LS, GS, WIDTH, TWO_LEVEL>(S, tr.ow, tr.dw, tr.camera, lds_col, gst);
+                const ExactHit e = exact_hit<BLOCK, RS, LS, GS, WIDTH, TL>(S, tr.ow, tr.dw, tr.camera, lds_col, gst);
                 novf += e.ovf;
                 nrays++;
                 if (e.pass_flag) nretrace++;
@@ -1204,7 +1281,7 @@ __global__ __launch_bounds__(BLOCK, extend_pt_min_waves(COUNT, UNIFORM, TWO_LEVE
 }
 
 // Closest hit of caller rays (first-hit parity queries).
-template <int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
+template <int BLOCK, int RS, int LS, int GS, int WIDTH, int TL>
 __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __restrict__ sc, const float4* __restrict__ rays,
                                                       uint32_t n, int camera, float4* __restrict__ hit,
                                                       uint32_t* __restrict__ hit_inst, uint32_t* __restrict__ gstack,
@@ -1219,7 +1296,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(const crt_dev_scene* __res
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         HitRec h;
         bool ovf = false;
-        trace<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL, true>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
+        trace<BLOCK, RS, LS, GS, false, WIDTH, TL, true>(S, v3(a), v3(b), a.w, b.w, camera != 0, s_stack + threadIdx.x,
                                                                 GSlot{gstack, gtid * GS}, h, tc, ovf);
         novf += ovf ? 1u : 0u;
         hit[i] = make_float4(h.t, bitsf(h.prim), h.u, h.v);
@@ -1271,7 +1348,8 @@ __device__ __forceinline__ float4 tex_value(const float4* texs, const uint8_t* t
 // Shading of one hit (core/src/cameras/camera.cuh:150-223): emission, background on a miss,
 // material scatter with the light/cosine mixture pdf for Lambertian surfaces. Updates the path in
 // place; returns true while the path continues (o, d then hold the next segment).
-template <int MODE>
+// SHAPES = false leaves out the analytic shapes' shading (the caller knows no scene of the pass has any).
+template <int MODE, bool SHAPES>
 __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, const crt_dev_request& R, uint32_t pix,
                                           uint32_t sample, float4 hv, uint32_t iw, V3& o, V3& d, V3& att, V3& L,
                                           uint32_t& depth, uint32_t& bounce, Rng<MODE>& g, uint32_t& light_tests) {
@@ -1297,6 +1375,7 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
         float tu = hv.z, tvv = hv.w;  // texture coordinates (filled lazily below)
         bool need_uv_sphere = false;
         bool is_light_quad = false;
+        bool shape = false, shape_front = false;
         V3 sphere_c; float sphere_r = 1.0f;
         uint32_t tri_flags = 0;
         float4 tq0, tq1, tq2;
@@ -1334,10 +1413,62 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
             const float4* q = (const float4*)sc->quads + 5 * idx;
             float4 q1 = ldg(q + 1), q3 = ldg(q + 3), q4 = ldg(q + 4);
             mat_id = fbits(q4.w);
+            const uint32_t qf = fbits(q1.w);
             outward = v3(q3);
-            is_light_quad = (fbits(q1.w) & CRT_QF_LIGHT) != 0;
+            is_light_quad = (qf & CRT_QF_LIGHT) != 0;
+            if (SHAPES && (qf & CRT_QF_SHAPE)) {
+                shape = true;
+                const float4 q0 = ldg(q), q2 = ldg(q + 2);
+                V3 lo, ld;
+                shape_local_ray(q0, q2, q3, q4, o, d, lo, ld);
+                const V3 hp = lo + t * ld;   // the hit relative to the centre, in the shape's frame
+                const float r = q1.x, h = q1.y;
+                const float4* mp = (const float4*)sc->mats + fbits(q4.y);
+                const float4 m0 = ldg(mp), m1 = ldg(mp + 1);   // (scale, offset), (repeat, -)
+                V3 out_l;
+                float su, sv;
+                const float kPi = 3.14159265358979323846f;
+                if (qf & CRT_QF_DISK) {
+                    // normal +y (disk.cuh:72-73); get_disk_uv on hit - centre (uvmapping.cuh:306-343)
+                    out_l = v3(0.0f, 1.0f, 0.0f);
+                    su = fmodf((hp.x / (2.0f * r) + 0.5f) * m1.x, 1.0f);
+                    sv = fmodf((hp.z / (2.0f * r) + 0.5f) * m1.y, 1.0f);
+                } else if (qf & CRT_QF_CYLINDER) {
+                    // (hit - centre) / radius with its y part, not renormalised (cylinder.cuh:93-95);
+                    // get_cylinder_uv on it (uvmapping.cuh:285-304; M_PI is a float literal,
+                    // core/src/misc/constants.cuh:35)
+                    out_l = hp / r;
+                    const float th = atan2f(out_l.x, out_l.z);
+                    su = 1.0f - (th + kPi) / (2.0f * kPi);
+                    sv = (out_l.y + h / 2.0f) / h;
+                    su = fmodf(su * m1.x, 1.0f);
+                    sv = fmodf(sv * m1.y, 1.0f);
+                } else {
+                    // cone.cuh:108-121: caps' normals at the end planes, else (x, |xz| r / h, z), unit;
+                    // get_cone_uv on the vector before normalising (uvmapping.cuh:346-368, which
+                    // applies the u repeat twice)
+                    V3 ov;
+                    if (hp.y <= 0.0f) ov = v3(0.0f, -1.0f, 0.0f);
+                    else if (hp.y >= h) ov = v3(0.0f, 1.0f, 0.0f);
+                    else ov = v3(hp.x, sqrtf(hp.x * hp.x + hp.z * hp.z) * (r / h), hp.z);
+                    out_l = unit(ov);
+                    const float th = atan2f(ov.x, ov.z);
+                    su = (th + kPi) / (2.0f * kPi);
+                    sv = sqrtf(ov.x * ov.x + ov.z * ov.z) / r;
+                    su = fmodf(su * m1.x, 1.0f);
+                    su = fmodf(su * m1.x, 1.0f);
+                    sv = fmodf(sv * m1.y, 1.0f);
+                }
+                tu = m0.x * su + m0.z;
+                tvv = m0.y * sv + m0.w;
+                shape_front = dot(ld, out_l) < 0.0f;   // decided in the shape's frame (set_face_normal)
+                // back to the world: A^T (rotation back, then divided by the scale), unit under a scale
+                outward = v3(q0.x * out_l.x + q2.x * out_l.y + q3.x * out_l.z, q0.y * out_l.x + q2.y * out_l.y + q3.y * out_l.z,
+                             q0.z * out_l.x + q2.z * out_l.y + q3.z * out_l.z);
+                if (qf & CRT_QF_UNIT_NORMAL) outward = unit(outward);
+            }
         }
-        bool front = dot(d, outward) < 0.0f;
+        bool front = shape ? shape_front : dot(d, outward) < 0.0f;
         V3 nrm = front ? outward : -outward;
         const float4* mats = (const float4*)sc->mats;
         Mat m;
@@ -1499,7 +1630,7 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                         const float4* q = qd + 5 * lr;
                         float qa, qb;
                         uint32_t qkey;
-                        if (quad_hit(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb, qkey)) {
+                        if (quad_hit<false>(p, dir, q, 0.00001f, __builtin_inff(), false, tl, qa, qb, qkey)) {
                             float dist2 = tl * tl * len2(dir);
                             float cosine = fabsf(dot(dir, v3(ldg(q + 3))) / vlen(dir));
                             pv = dist2 / (cosine * ldg(q + 2).w);
@@ -1527,9 +1658,10 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
     return !done;
 }
 
-// Shade stage. FLAT (no instanced scene among the paths) holds the kernel to 6 resident
-// workgroups per CU (80 VGPRs, a few spills outside the hot paths), measured 2 % faster per frame
-// on single-level scenes; with instances (transforms, textures) the unbounded kernel is faster.
+// Shade stage. FLAT (no instanced scene and no scene with analytic shapes among the paths) holds the
+// kernel to 6 resident workgroups per CU (80 VGPRs, a few spills outside the hot paths), measured
+// 2 % faster per frame on single-level scenes, and leaves the shapes' shading out; with instances
+// (transforms, textures) the unbounded kernel is faster.
 // ONE: the pass holds a single request, so the request record and its scene's pointers are
 // wave-uniform (scalar loads once per wave instead of two dependent vector loads per path).
 template <int MODE, int BLOCK, bool FLAT, bool ONE>
@@ -1574,7 +1706,7 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
             Rng<MODE> g;
             g.ms = fbits(tv.w);
             g.dyn = R.rng_mode;
-            keep = shade_hit<MODE>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
+            keep = shade_hit<MODE, !FLAT>(sc, R, pix, sample, hv, iw, o, d, att, L, depth, bounce, g, light_tests);
             if (keep) {
                 min_pid = min(min_pid, R.path_base + (pix - R.pix_begin) * R.sample_count + (sample - R.sample_begin));
                 uint32_t ns = (depth & 0xffu) | ((bounce & 0xffu) << 8) | (retraces << 24);
@@ -1628,7 +1760,7 @@ __global__ __launch_bounds__(BLOCK, FLAT ? 6 : 1) void k_shade(const crt_dev_sce
 // and shading in one kernel. Same estimator, streams and per-pixel float sums as the wavefront
 // pipeline, so the images agree; it serves as the plain baseline the batched passes are measured
 // against. Grid-stride over the request's pixel range.
-template <int MODE, int BLOCK, int RS, int LS, int GS, int WIDTH, bool TWO_LEVEL>
+template <int MODE, int BLOCK, int RS, int LS, int GS, int WIDTH, int TL>
 __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict__ scenes,
                                                  const crt_dev_request* __restrict__ req, uint32_t* __restrict__ gstack,
                                                  crt_dev_counters* __restrict__ counters) {
@@ -1662,11 +1794,11 @@ __global__ __launch_bounds__(BLOCK) void k_plain(const crt_dev_scene* __restrict
             bool camera = true;
             for (;;) {
                 HitRec h;
-                const uint32_t pf = closest_hit_checked<BLOCK, RS, LS, GS, false, WIDTH, TWO_LEVEL>(
+                const uint32_t pf = closest_hit_checked<BLOCK, RS, LS, GS, false, WIDTH, TL>(
                     S, o, d, camera, lds_col, gst, h, tc, novf, nrays, nretrace, nexact);
                 camera = false;
                 float4 hv = make_float4(h.t, bitsf(h.prim), h.u, h.v);
-                if (!shade_hit<MODE>(sc, R, pix, sample, hv, h.inst | pf, o, d, att, L, depth, bounce, g, light_tests))
+                if (!shade_hit<MODE, true>(sc, R, pix, sample, hv, h.inst | pf, o, d, att, L, depth, bounce, g, light_tests))
                     break;
             }
             acc.x += L.x; acc.y += L.y; acc.z += L.z;
@@ -2003,8 +2135,8 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
     else if (stack_lds == 32) CRT_EXT_GS(32, false, U, W, T)                   \
     else CRT_EXT_GS(STK_LS, false, U, W, T)
 #define CRT_EXT_WT(X, U)                                                                     \
-    if (width == 2) { if (two_level) { X(U, 2, true) } else { X(U, 2, false) } }             \
-    else { if (two_level) { X(U, 4, true) } else { X(U, 4, false) } }
+    if (width == 2) { if (two_level == 2) { X(U, 2, 2) } else if (two_level) { X(U, 2, 1) } else { X(U, 2, 0) } } \
+    else { if (two_level == 2) { X(U, 4, 2) } else if (two_level) { X(U, 4, 1) } else { X(U, 4, 0) } }
     if (count_mode) {
         if (uniform_scene >= 0) { CRT_EXT_WT(CRT_EXT_C, true) } else { CRT_EXT_WT(CRT_EXT_C, false) }
     } else {
@@ -2018,7 +2150,7 @@ extern "C" int crt_k_extend(hipStream_t s, int count_mode, int width, int two_le
     return (int)hipGetLastError();
 }
 
-extern "C" int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t grid, const crt_dev_scene* scenes,
+extern "C" int crt_k_shade(hipStream_t s, int rng_mode, int full, uint32_t grid, const crt_dev_scene* scenes,
                            const crt_dev_request* reqs, const uint32_t* count_in, crt_paths Pin, crt_paths Pout,
                            uint32_t* count_out, const float4* hit, const uint32_t* hit_inst, float4* out_rad,
                            uint32_t* live_min, int sort_octant, int one_request, crt_dev_counters* counters) {
@@ -2026,7 +2158,7 @@ extern "C" int crt_k_shade(hipStream_t s, int rng_mode, int two_level, uint32_t 
     hipLaunchKernelGGL((k_shade<M, 256, F, O>), dim3(grid), dim3(256), 0, s, scenes, reqs, count_in, Pin, Pout,         \
                        count_out, hit, hit_inst, out_rad, live_min, sort_octant, counters)
 #define CRT_SH_O(M, F) { if (one_request) CRT_SH(M, F, true); else CRT_SH(M, F, false); }
-#define CRT_SH_F(M) { if (two_level) CRT_SH_O(M, false) else CRT_SH_O(M, true) }
+#define CRT_SH_F(M) { if (full) CRT_SH_O(M, false) else CRT_SH_O(M, true) }
     if (rng_mode == CRT_RNG_MINSTD) CRT_SH_F(CRT_RNG_MINSTD)
     else if (rng_mode == CRT_RNG_PCG) CRT_SH_F(CRT_RNG_PCG)
     else CRT_SH_F(CRT_RNG_DYNAMIC)
@@ -2042,8 +2174,8 @@ extern "C" int crt_k_trace_rays(hipStream_t s, int width, int two_level, uint32_
 #define CRT_TR(W, T)                                                                                              \
     hipLaunchKernelGGL((k_trace_rays<256, STK_RS, STK_LS, STK_GS, W, T>), dim3(grid), dim3(256), 0, s, sc, rays, n, \
                        camera, hit, hit_inst, gstack, counters)
-    if (width == 2) { if (two_level) CRT_TR(2, true); else CRT_TR(2, false); }
-    else { if (two_level) CRT_TR(4, true); else CRT_TR(4, false); }
+    if (width == 2) { if (two_level == 2) CRT_TR(2, 2); else if (two_level) CRT_TR(2, 1); else CRT_TR(2, 0); }
+    else { if (two_level == 2) CRT_TR(4, 2); else if (two_level) CRT_TR(4, 1); else CRT_TR(4, 0); }
 #undef CRT_TR
     return (int)hipGetLastError();
 }
@@ -2055,8 +2187,8 @@ extern "C" int crt_k_plain(hipStream_t s, int rng_mode, int width, int two_level
     hipLaunchKernelGGL((k_plain<M, 256, STK_RS, STK_LS, STK_GS, W, T>), dim3(grid), dim3(256), 0, s, scenes, req, \
                        gstack, counters)
 #define CRT_PL_WT(M)                                                              \
-    if (width == 2) { if (two_level) CRT_PL(M, 2, true); else CRT_PL(M, 2, false); } \
-    else { if (two_level) CRT_PL(M, 4, true); else CRT_PL(M, 4, false); }
+    if (width == 2) { if (two_level == 2) CRT_PL(M, 2, 2); else if (two_level) CRT_PL(M, 2, 1); else CRT_PL(M, 2, 0); } \
+    else { if (two_level == 2) CRT_PL(M, 4, 2); else if (two_level) CRT_PL(M, 4, 1); else CRT_PL(M, 4, 0); }
     if (rng_mode == CRT_RNG_MINSTD) { CRT_PL_WT(CRT_RNG_MINSTD) } else { CRT_PL_WT(CRT_RNG_PCG) }
 #undef CRT_PL_WT
 #undef CRT_PL

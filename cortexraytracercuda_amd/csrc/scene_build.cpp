@@ -417,6 +417,87 @@ void quad_record(V q, V u, V v, uint32_t flags, uint32_t mat, uint32_t gid, F4* 
     out[4] = F4{w.x, w.y, w.z, fb(mat)};
 }
 
+// Box of a quad or area-light record's four corners (a light's corner is its centre less half of
+// each edge).
+Box quad_box(V q, V u, V v, uint32_t flags) {
+    Box b;
+    b.empty();
+    V c0 = q;
+    if (flags & CRT_QF_LIGHT) c0 = vsub(q, vadd(vmul(0.5f, u), vmul(0.5f, v)));
+    b.grow(c0); b.grow(vadd(c0, u)); b.grow(vadd(c0, v)); b.grow(vadd(vadd(c0, u), v));
+    return b;
+}
+
+// Analytic shape record (5 x float4, in the quad array) and its world box. Under the reference's
+// wrappers (scene_factory.cuh:710-725) a world ray reaches the shape divided by the scale
+// (scale.cuh:50-56), rotated by Rx(-x) Ry(-y) Rz(-z) (rotate.cuh:104-119) and offset by -T
+// (translate.cuh:38-43); the shape's own hit then works relative to its centre c. Composed:
+// local = A x + b with A = Rx(-x) Ry(-y) Rz(-z) diag(1/s) and b = -(T + c). Untransformed, A = I
+// and b = -c, so the local origin is exactly the reference's origin - centre. The world normal is
+// A^T applied to the local one (rotation back, then divided by the scale), renormalised when the
+// scale wrapper is present.
+int shape_record(const crt_shape& sh, uint32_t gid, uint32_t map_slot, F4* rec, Box& box, std::string& err) {
+    if (sh.kind > CRT_SHAPE_CONE) { err = "unknown shape kind"; return CRT_ERR_INVALID_SCENE; }
+    if (!finite3(sh.center) || !std::isfinite(sh.radius) || !std::isfinite(sh.height)) {
+        err = "non-finite shape"; return CRT_ERR_INVALID_SCENE;
+    }
+    const crt_transform& x = sh.transform;
+    double T[3] = {0, 0, 0}, r[3] = {0, 0, 0}, sc[3] = {1, 1, 1};
+    for (int a = 0; a < 3; ++a) {
+        if (x.has & CRT_XF_TRANSLATE) T[a] = x.translate[a];
+        if (x.has & CRT_XF_ROTATE) r[a] = x.rotate_deg[a];
+        if (x.has & CRT_XF_SCALE) sc[a] = x.scale[a];
+        if (!std::isfinite(T[a]) || !std::isfinite(r[a]) || !std::isfinite(sc[a]) || sc[a] == 0.0) {
+            err = "invalid shape transform"; return CRT_ERR_INVALID_SCENE;
+        }
+    }
+    // Rinv = Rx(-x) Ry(-y) Rz(-z): rotations about the axes, applied right to left
+    double Ri[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int axis = 0; axis < 3; ++axis) {
+        const double ang = -r[axis] * 3.14159265358979323846 / 180.0, c = std::cos(ang), sn = std::sin(ang);
+        double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        const int i = (axis + 1) % 3, j = (axis + 2) % 3;
+        R[3 * i + i] = c; R[3 * i + j] = -sn; R[3 * j + i] = sn; R[3 * j + j] = c;
+        double P[9];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) P[3 * p + q] = Ri[3 * p] * R[q] + Ri[3 * p + 1] * R[3 + q] + Ri[3 * p + 2] * R[6 + q];
+        std::memcpy(Ri, P, sizeof(P));
+    }
+    float A[9], b[3];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) A[3 * i + j] = (float)(Ri[3 * i + j] / sc[j]);
+        b[i] = -(float)(T[i] + (double)sh.center[i]);
+    }
+    const uint32_t kind_flag = sh.kind == CRT_SHAPE_DISK ? CRT_QF_DISK : (sh.kind == CRT_SHAPE_CYLINDER ? CRT_QF_CYLINDER : CRT_QF_CONE);
+    const uint32_t flags = kind_flag | ((x.has & CRT_XF_SCALE) ? CRT_QF_UNIT_NORMAL : 0u);
+    rec[0] = F4{A[0], A[1], A[2], b[0]};
+    rec[1] = F4{sh.radius, sh.height, 0.0f, fb(flags)};
+    rec[2] = F4{A[3], A[4], A[5], b[1]};
+    rec[3] = F4{A[6], A[7], A[8], fb(gid)};
+    rec[4] = F4{b[2], fb(map_slot), 0.0f, fb(sh.material)};
+    // world box: the corners of the shape's own box (disk.cuh:49, cylinder.cuh:58-61, cone.cuh:52-55)
+    // mapped by x -> S R (p + T), R = Rinv^T, then widened so FP32 hits on the surface stay inside
+    const double rad = std::fabs((double)sh.radius), h = sh.height;
+    const double ylo = sh.kind == CRT_SHAPE_DISK ? -h / 2 : std::min(0.0, h), yhi = sh.kind == CRT_SHAPE_DISK ? h / 2 : std::max(0.0, h);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int k = 0; k < 8; ++k) {
+        const double p[3] = {((k & 1) ? rad : -rad) + sh.center[0] + T[0], ((k & 2) ? yhi : ylo) + sh.center[1] + T[1],
+                             ((k & 4) ? rad : -rad) + sh.center[2] + T[2]};
+        for (int i = 0; i < 3; ++i) {
+            const double w = sc[i] * (Ri[i] * p[0] + Ri[3 + i] * p[1] + Ri[6 + i] * p[2]);
+            lo[i] = std::min(lo[i], w); hi[i] = std::max(hi[i], w);
+        }
+    }
+    double ext = 0.0;
+    for (int i = 0; i < 3; ++i) ext = std::max(ext, std::max(std::fabs(lo[i]), std::fabs(hi[i])));
+    const double pad = 1e-5 * ext + 1e-6;
+    for (int i = 0; i < 3; ++i) {
+        box.lo[i] = (float)(lo[i] - pad);
+        box.hi[i] = (float)(hi[i] + pad);
+    }
+    return CRT_OK;
+}
+
 // Triangle record (3 x float4: (v0, id) (v1, material) (v2, flags)) with its unit vertex normals
 // (vert_normals = unit_vector(vn), core/src/primitives/triangle.cuh:97-99) and uvs when the mesh
 // has them.
@@ -520,6 +601,7 @@ int prepare_scene_tables(const crt_scene_desc* d, uint32_t leaf_size, uint32_t b
     if (!need(d->num_meshes == 0 || d->meshes, "meshes pointer is null")) return CRT_ERR_INVALID_SCENE;
     if (!need(d->num_instances == 0 || d->instances, "instances pointer is null")) return CRT_ERR_INVALID_SCENE;
     if (!need(d->num_sphere_lights == 0 || d->sphere_lights, "sphere lights pointer is null")) return CRT_ERR_INVALID_SCENE;
+    if (!need(d->num_shapes == 0 || d->shapes, "shapes pointer is null")) return CRT_ERR_INVALID_SCENE;
     S = HostScene();
     if (bvh_width != 2 && bvh_width != 4) { err = "bvh width must be 2 or 4"; return CRT_ERR_INVALID_ARGUMENT; }
     S.bvh_width = bvh_width;
@@ -600,6 +682,15 @@ int prepare_scene_tables(const crt_scene_desc* d, uint32_t leaf_size, uint32_t b
             S.mats.push_back(F4{L.falloff, 0.0f, 0.0f, 0.0f});
         }
     }
+    // + the shapes' uv mappings: (scale_u, scale_v, offset_u, offset_v), (repeat_u, repeat_v, -, -)
+    S.shape_map_base = (uint32_t)S.mats.size();
+    for (uint32_t i = 0; i < d->num_shapes; ++i) {
+        const crt_shape& h = d->shapes[i];
+        if (h.material >= NM) { err = "shape " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
+        const crt_uvmap& m = h.mapping;
+        S.mats.push_back(F4{m.scale_u, m.scale_v, m.offset_u, m.offset_v});
+        S.mats.push_back(F4{m.repeat_u, m.repeat_v, 0.0f, 0.0f});
+    }
     return CRT_OK;
 }
 
@@ -614,7 +705,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     // ---------------------------------------------------------------- primitive refs (top level)
     std::vector<Ref> refs;
     // quads: lights first (world-list order), then quads
-    struct QuadSrc { V q, u, v; uint32_t flags, mat, gid; };
+    struct QuadSrc { F4 rec[5]; Box box; };
     std::vector<QuadSrc> qsrc;
     std::vector<SphereSrc> ssrc;
     std::vector<uint32_t> light_src;
@@ -624,8 +715,10 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     for (uint32_t i = 0; i < d->num_lights; ++i) {
         const crt_area_light& L = d->lights[i];
         if (!finite3(L.center) || !finite3(L.u) || !finite3(L.v)) { err = "non-finite light geometry"; return CRT_ERR_INVALID_SCENE; }
-        QuadSrc s{vload(L.center), vload(L.u), vload(L.v), CRT_QF_LIGHT | CRT_QF_ONE_SIDED | (L.invisible ? CRT_QF_INVISIBLE : 0u),
-                  NM + i, n_omni + i};
+        const uint32_t fl = CRT_QF_LIGHT | CRT_QF_ONE_SIDED | (L.invisible ? CRT_QF_INVISIBLE : 0u);
+        QuadSrc s;
+        quad_record(vload(L.center), vload(L.u), vload(L.v), fl, NM + i, n_omni + i, s.rec);
+        s.box = quad_box(vload(L.center), vload(L.u), vload(L.v), fl);
         qsrc.push_back(s);
     }
     uint32_t gid = d->num_lights + d->num_sphere_lights + d->num_spheres;
@@ -633,16 +726,17 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
         const crt_quad& Q = d->quads[i];
         if (Q.material >= NM) { err = "quad " + std::to_string(i) + " references missing material"; return CRT_ERR_INVALID_SCENE; }
         if (!finite3(Q.corner) || !finite3(Q.u) || !finite3(Q.v)) { err = "non-finite quad geometry"; return CRT_ERR_INVALID_SCENE; }
-        qsrc.push_back(QuadSrc{vload(Q.corner), vload(Q.u), vload(Q.v), 0u, Q.material, gid});
+        QuadSrc s;
+        quad_record(vload(Q.corner), vload(Q.u), vload(Q.v), 0u, Q.material, gid, s.rec);
+        s.box = quad_box(vload(Q.corner), vload(Q.u), vload(Q.v), 0u);
+        qsrc.push_back(s);
     }
-    for (uint32_t i = 0; i < (uint32_t)qsrc.size(); ++i) {
-        const QuadSrc& q = qsrc[i];
-        Box b; b.empty();
-        V c0 = q.q;
-        if (q.flags & CRT_QF_LIGHT) c0 = vsub(q.q, vadd(vmul(0.5f, q.u), vmul(0.5f, q.v)));
-        b.grow(c0); b.grow(vadd(c0, q.u)); b.grow(vadd(c0, q.v)); b.grow(vadd(vadd(c0, q.u), q.v));
-        refs.push_back(make_ref(b, CRT_PRIM_QUAD, i));
+    for (uint32_t i = 0; i < d->num_shapes; ++i, ++gid) {
+        QuadSrc s;
+        if ((prc = shape_record(d->shapes[i], gid, S.shape_map_base + 2u * i, s.rec, s.box, err))) return prc;
+        qsrc.push_back(s);
     }
+    for (uint32_t i = 0; i < (uint32_t)qsrc.size(); ++i) refs.push_back(make_ref(qsrc[i].box, CRT_PRIM_QUAD, i));
     for (uint32_t i = 0; i < (uint32_t)ssrc.size(); ++i) {
         const SphereSrc& s = ssrc[i];
         Box b;
@@ -768,11 +862,8 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     };
     std::vector<uint32_t> quad_slot(qsrc.size(), 0);
     em.emit_quad = [&](uint32_t i) {
-        const QuadSrc& q = qsrc[i];
-        F4 rec[5];
-        quad_record(q.q, q.u, q.v, q.flags, q.mat, q.gid, rec);
         quad_slot[i] = (uint32_t)(S.quads.size() / 5);
-        S.quads.insert(S.quads.end(), rec, rec + 5);
+        S.quads.insert(S.quads.end(), qsrc[i].rec, qsrc[i].rec + 5);
     };
     em.emit_inst = [&](uint32_t k) {
         const crt_instance& I = d->instances[k];
@@ -811,6 +902,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
     S.num_lights = (uint32_t)light_src.size();
     S.num_spheres = (uint32_t)ssrc.size();
     S.num_quads = d->num_quads;
+    S.num_shapes = d->num_shapes;
     S.num_quad_records = (uint32_t)(S.quads.size() / 5);
     if (!S.tri_normals.empty() && S.tri_normals.size() < S.tris.size()) S.tri_normals.resize(S.tris.size(), F4{0, 0, 0, 0});
     if (!S.tri_uvs.empty() && S.tri_uvs.size() < S.tris.size()) S.tri_uvs.resize(S.tris.size(), F2{0, 0});
@@ -824,7 +916,7 @@ int build_host_scene(const crt_scene_desc* d, uint32_t leaf_size, uint32_t bvh_w
 
 namespace crt {
 
-int flatten_sources(const crt_scene_desc* d, int threads, SourceRecords& out, std::string& err) {
+int flatten_sources(const crt_scene_desc* d, uint32_t shape_map_base, int threads, SourceRecords& out, std::string& err) {
     const uint32_t NM = d->num_materials;
     if (d->num_instances) { err = "the device build takes one-level scenes"; return CRT_ERR_INVALID_ARGUMENT; }
     int rc = validate_meshes(d, err);
@@ -835,13 +927,18 @@ int flatten_sources(const crt_scene_desc* d, int threads, SourceRecords& out, st
     uint32_t n_omni = 0;
     for (uint32_t i = 0; i < d->num_sphere_lights; ++i) n_omni += d->sphere_lights[i].kind == CRT_LIGHT_OMNI ? 1u : 0u;
     out.num_lights = (uint32_t)out.light_src.size();
+    auto push_box = [&](const Box& b) {
+        out.quad_boxes.push_back(F4{b.lo[0], b.lo[1], b.lo[2], 0.0f});
+        out.quad_boxes.push_back(F4{b.hi[0], b.hi[1], b.hi[2], 0.0f});
+    };
     for (uint32_t i = 0; i < d->num_lights; ++i) {
         const crt_area_light& L = d->lights[i];
         if (!finite3(L.center) || !finite3(L.u) || !finite3(L.v)) { err = "non-finite light geometry"; return CRT_ERR_INVALID_SCENE; }
         F4 rec[5];
-        quad_record(vload(L.center), vload(L.u), vload(L.v),
-                    CRT_QF_LIGHT | CRT_QF_ONE_SIDED | (L.invisible ? CRT_QF_INVISIBLE : 0u), NM + i, n_omni + i, rec);
+        const uint32_t fl = CRT_QF_LIGHT | CRT_QF_ONE_SIDED | (L.invisible ? CRT_QF_INVISIBLE : 0u);
+        quad_record(vload(L.center), vload(L.u), vload(L.v), fl, NM + i, n_omni + i, rec);
         out.quads.insert(out.quads.end(), rec, rec + 5);
+        push_box(quad_box(vload(L.center), vload(L.u), vload(L.v), fl));
         out.quad_gid.push_back(n_omni + i);
     }
     uint32_t gid = d->num_lights + d->num_sphere_lights + d->num_spheres;
@@ -852,6 +949,15 @@ int flatten_sources(const crt_scene_desc* d, int threads, SourceRecords& out, st
         F4 rec[5];
         quad_record(vload(Q.corner), vload(Q.u), vload(Q.v), 0u, Q.material, gid, rec);
         out.quads.insert(out.quads.end(), rec, rec + 5);
+        push_box(quad_box(vload(Q.corner), vload(Q.u), vload(Q.v), 0u));
+        out.quad_gid.push_back(gid);
+    }
+    for (uint32_t i = 0; i < d->num_shapes; ++i, ++gid) {
+        F4 rec[5];
+        Box b;
+        if ((rc = shape_record(d->shapes[i], gid, shape_map_base + 2u * i, rec, b, err))) return rc;
+        out.quads.insert(out.quads.end(), rec, rec + 5);
+        push_box(b);
         out.quad_gid.push_back(gid);
     }
     for (const SphereSrc& s : ssrc) {

@@ -36,7 +36,9 @@ struct HostScene {
     std::vector<F2> sphere_map;   // 3 per sphere: scale, offset, repeat
     std::vector<F4> quads;        // 5 per quad / light
     std::vector<F4> insts;        // 4 per instance
-    std::vector<F4> mats;         // 2 per material (scene materials, then one per area light)
+    std::vector<F4> mats;         // 2 per material (scene materials, then one per area light), then the
+                                  // shapes' uv mappings (2 per shape from shape_map_base)
+    uint32_t shape_map_base = 0;  // float4 index of the first shape's uv mapping in mats
     std::vector<F4> texs;         // 2 per texture
     std::vector<uint8_t> texels;
     std::vector<uint32_t> light_refs;   // per light in world order: quad slot, or CRT_LIGHT_SPHERE | sphere slot
@@ -50,6 +52,7 @@ struct HostScene {
     uint32_t num_quads = 0;         // the scene's quads (not its area lights)
     uint32_t num_quad_records = 0;  // quad records (area lights and quads)
     uint32_t num_instances = 0;
+    uint32_t num_shapes = 0;        // analytic shapes (records in the quad array)
     uint64_t num_world_tris = 0;
     uint64_t num_node_records = 0;  // BVH nodes in the device allocation (BLAS nodes included)
     uint64_t num_tri_records = 0;   // triangle records in the device allocation (BLAS triangles included)
@@ -78,7 +81,8 @@ int prepare_scene_tables(const crt_scene_desc* d, uint32_t leaf_size, uint32_t b
 // triangles) for the device BVH build of a one-level scene; the same record layouts as HostScene,
 // before they are put in leaf order.
 struct SourceRecords {
-    std::vector<F4> quads;        // 5 per quad / area light, lights first
+    std::vector<F4> quads;        // 5 per quad / area light / shape, lights first
+    std::vector<F4> quad_boxes;   // 2 per quad record: (lo, -), (hi, -)
     std::vector<uint32_t> quad_gid;
     uint32_t num_lights = 0;      // omni + area + spot lights
     std::vector<uint32_t> light_src;   // world light order: quad source index, or CRT_LIGHT_SPHERE | sphere source index
@@ -90,7 +94,7 @@ struct SourceRecords {
     std::vector<F2> tri_uvs;      // 3 per triangle when a mesh has uvs
     uint32_t tri_gid0 = 0;        // global id of the first triangle (consecutive after it)
 };
-int flatten_sources(const crt_scene_desc* d, int threads, SourceRecords& out, std::string& err);
+int flatten_sources(const crt_scene_desc* d, uint32_t shape_map_base, int threads, SourceRecords& out, std::string& err);
 
 // Flattens and builds. leaf_size = max primitives per leaf (instances always get their own leaf);
 // bvh_width = 2 or 4 children per node (the binary SAH tree is collapsed for 4).

@@ -33,6 +33,7 @@ struct Builder {
     std::vector<crt_sphere_light> sphere_lights;
     std::vector<crt_sphere> spheres;
     std::vector<crt_quad> quads;
+    std::vector<crt_shape> shapes;
     std::vector<crt_mesh> meshes;
     std::vector<std::unique_ptr<crt_obj_mesh, void (*)(crt_obj_mesh*)>> objs;
     std::vector<std::vector<float>> mesh_pos;
@@ -421,6 +422,26 @@ int crt_builder_box(crt_builder* b, const char* name, const float* position, con
     return CRT_OK;
 }
 
+// disk / cylinder / cone (core/src/primitives/disk.cuh, cylinder.cuh, cone.cuh): the transform stays
+// with the shape, whose hit runs in its own frame as under the reference's wrappers
+int crt_builder_shape(crt_builder* b, uint32_t kind, const char* name, const float* position, float radius, float height,
+                      const char* material, const crt_uvmap* mapping, const crt_transform* xf) {
+    CRT_B_CHECK(b);
+    CRT_B(b);
+    (void)name;
+    if (kind > CRT_SHAPE_CONE) return set_error(CRT_ERR_INVALID_ARGUMENT, "unknown shape kind");
+    crt_shape s{};
+    if (!material_of(B, material, s.material)) return CRT_OK;
+    s.kind = kind;
+    copy3(s.center, position);
+    s.radius = radius;
+    s.height = height;
+    s.mapping = mapping ? *mapping : crt_uvmap{1, 1, 0, 0, 1, 1};
+    if (xf) s.transform = *xf;
+    B.shapes.push_back(s);
+    return CRT_OK;
+}
+
 // OBJ mesh (createObjMesh, core/src/scene_factory.cuh:373-385): position and transform are not
 // applied, as in the reference; use_smoothing = 0 keeps flat normals
 int crt_builder_obj_mesh(crt_builder* b, const char* name, const char* filepath, const char* material,
@@ -466,6 +487,7 @@ int crt_builder_desc(crt_builder* b, crt_scene_desc* out, uint32_t* skipped, uin
     d.sphere_lights = B.sphere_lights.data(); d.num_sphere_lights = (uint32_t)B.sphere_lights.size();
     d.spheres = B.spheres.data(); d.num_spheres = (uint32_t)B.spheres.size();
     d.quads = B.quads.data(); d.num_quads = (uint32_t)B.quads.size();
+    d.shapes = B.shapes.data(); d.num_shapes = (uint32_t)B.shapes.size();
     d.meshes = B.meshes.data(); d.num_meshes = (uint32_t)B.meshes.size();
     *out = d;
     if (skipped) *skipped = B.skipped;

@@ -13,13 +13,14 @@
 //   materials  lambertian (name color | texture), metal (name color fuzziness), glass
 //              (name refraction) and dielectric (same keys)
 //   primitives spheres (name position radius material uvmapping), quads (name position u v
-//              material), boxes (name position size material)
+//              material), boxes (name position size material), cones / cylinders / disks (name
+//              position radius height material uvmapping)
 //   meshes     obj (name filepath material use_smoothing active)
 // Semantics kept from the reference loader: colours written as integers are 0-255
 // (getColor, scene_loader.cpp:1622-1641), metal reads `fuzziness` only (scene_loader.cpp:1128),
 // box `position` is the centre (core/src/primitives/box.cuh:41-46), quad `position` is a corner,
 // directional-light `position` is the centre, primitive transforms (translate, rotate, scale) are
-// applied to spheres, quads and boxes and ignored for OBJ meshes (scene_factory.cuh:373-385). Fixed
+// applied to spheres, quads, boxes and shapes and ignored for OBJ meshes (scene_factory.cuh:373-385). Fixed
 // on purpose: counts are not truncated to 8 bits and `glass` materials are loaded (the reference
 // sizes them from the `dielectric` key, scene_loader.cpp:191-192 vs 1092-1108, and drops them
 // otherwise). Groups and the features outside the render path (omni/spot lights, other materials,
@@ -493,7 +494,33 @@ static int load_scenefile(SceneFile& sf, std::string& err) {
                     return rc;
             }
         }
-        for (const char* k : {"planes", "cones", "cylinders", "disks", "toruses", "volumes"})
+        // cones, cylinders, disks in load_scene's order (core/src/gpu.cu:565-595), read as
+        // addCone/addCylinderPrimitives/addDiskPrimitives read them (scene_loader.cpp:1311-1440):
+        // position, radius and height default to 0, uvmapping to the unit mapping
+        const struct { const char* key; uint32_t kind; } shapes[] = {
+            {"cones", CRT_SHAPE_CONE}, {"cylinders", CRT_SHAPE_CYLINDER}, {"disks", CRT_SHAPE_DISK}};
+        for (const auto& sk : shapes) {
+            if (const cfg::Value* l = list_of(P, sk.key)) {
+                for (auto& it : l->items) {
+                    if (!bool_of(it->get("active"), true)) continue;
+                    float c[3];
+                    vec_of(it->get("position"), c);
+                    crt_uvmap map{1, 1, 0, 0, 1, 1};
+                    if (const cfg::Value* uv = it->get("uvmapping")) {
+                        map.scale_u = num_of(uv->get("scale_u"), 1.0f); map.scale_v = num_of(uv->get("scale_v"), 1.0f);
+                        map.offset_u = num_of(uv->get("offset_u"), 0.0f); map.offset_v = num_of(uv->get("offset_v"), 0.0f);
+                        map.repeat_u = num_of(uv->get("repeat_u"), 1.0f); map.repeat_v = num_of(uv->get("repeat_v"), 1.0f);
+                    }
+                    if ((rc = crt_builder_shape(B, sk.kind, str_of(it->get("name")).c_str(), c, num_of(it->get("radius"), 0.0f),
+                                                num_of(it->get("height"), 0.0f), str_of(it->get("material")).c_str(), &map,
+                                                xform_of(it.get(), xf))))
+                        return rc;
+                }
+            }
+        }
+        // tori are left out: torus::hit reports a hit without a distance (core/src/primitives/
+        // torus.cuh:61-110, its intersection code commented out), which no renderer can reproduce
+        for (const char* k : {"planes", "toruses", "volumes"})
             if (const cfg::Value* l = list_of(P, k)) unsupported += (uint32_t)l->items.size();
     }
     // meshes (OBJ)

@@ -160,7 +160,7 @@ void Context::server_release_lane(Server& S) {
     if (S.snap) (void)hipHostFree(S.snap);
     if (S.rays_host) (void)hipHostFree(S.rays_host);
     for (int k = 0; k < Server::kRing; ++k) if (S.ev[k]) (void)hipEventDestroy(S.ev[k]);
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < Server::kVariants; ++t)
         for (int k = 0; k < Server::kRing; ++k)
             if (S.graph[t][k]) { (void)hipGraphExecDestroy(S.graph[t][k]); S.graph[t][k] = nullptr; }
     if (S.s) (void)hipStreamDestroy(S.s);
@@ -181,6 +181,7 @@ void Context::server_reset_pass(Server& S) {
     S.progress_iter = 0;
     S.full = false;
     S.two_level = false;
+    S.shapes = false;
     S.mixed_rng = false;
     (void)crt_k_pass_init(S.s, (uint32_t*)S.ctl.p, 2 * CRT_CTL_WORDS + 2);
 }
@@ -204,10 +205,10 @@ void Context::server_drop_graphs() {
     for (Server& S : srv) {
         if (!S.ready) continue;
         (void)hipStreamSynchronize(S.s);
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < Server::kVariants; ++t)
             for (int k = 0; k < Server::kRing; ++k)
                 if (S.graph[t][k]) { (void)hipGraphExecDestroy(S.graph[t][k]); S.graph[t][k] = nullptr; }
-        for (int t = 0; t < 4; ++t) S.warm[t] = 0;
+        for (int t = 0; t < Server::kVariants; ++t) S.warm[t] = 0;
     }
 }
 
@@ -238,6 +239,7 @@ bool Context::append_segment(Server& S, ServeReq* r, uint32_t seg_cap) {
     S.entries.push_back(en);
     S.total += en.paths;
     if (r->sc->host.num_instances) S.two_level = true;
+    if (r->sc->host.num_shapes) S.shapes = true;
     if (r->rd.rng_mode == CRT_RNG_MINSTD) S.mixed_rng = true;
     r->timing.passes++;
     if (r->timing.first_pass_s == 0.0) r->timing.first_pass_s = now_s();
@@ -311,7 +313,10 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
     const crt_paths& B = P[(k + 1) & 1];
     const crt_dev_request* dreq = (const crt_dev_request*)S.req.p;
     const crt_dev_scene* dsc = (const crt_dev_scene*)C.scene_table.p;
-    const int two_level = S.two_level ? 1 : 0;
+    // traversal: one level, or two levels with or without the shape test; shade: the full kernel
+    // once an instanced scene or one with shapes is in the pass
+    const int two_level = S.two_level ? (S.shapes ? 2 : 1) : 0;
+    const int full_shade = (S.two_level || S.shapes) ? 1 : 0;
     // the per-path RNG mode only once a MINSTD request is in the pass (keyed PCG otherwise)
     const int mode = S.mixed_rng ? CRT_RNG_DYNAMIC : CRT_RNG_PCG;
     if (crt_k_regen(S.s, mode, S.sgrid, dreq, 0, 0, 0, S.slots, cin, cout, open, A))
@@ -321,7 +326,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
                      C.extend_mode, cin + CRT_CTL_WORK, C.extend_param, (int)C.stack_lds, cin + CRT_CTL_SURV,
                      (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "extend launch failed");
-    if (crt_k_shade(S.s, mode, two_level, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
+    if (crt_k_shade(S.s, mode, full_shade, S.sgrid, dsc, dreq, cin + CRT_CTL_LIVE, A, B, cout + CRT_CTL_SURV,
                     (const float4*)S.hit.p, (const uint32_t*)S.hit_inst.p, (float4*)S.out_rad.p, (uint32_t*)S.live_min.p,
                     (int)C.ray_sort, 0, (crt_dev_counters*)S.counters.p))
         return set_error(CRT_ERR_HIP, "shade launch failed");
@@ -335,7 +340,7 @@ static int enqueue_iteration(Context& C, Server& S, uint64_t k) {
 int Context::server_iteration(Server& S) {
     const uint64_t k = S.iter;
     const int slot = (int)(k % Server::kRing);
-    const int tl = (S.two_level ? 2 : 0) + (S.mixed_rng ? 1 : 0);
+    const int tl = (S.two_level ? 4 : 0) + (S.shapes ? 2 : 0) + (S.mixed_rng ? 1 : 0);
     int rc;
     if (S.use_graphs && S.warm[tl] >= (uint32_t)Server::kRing) {
         hipGraphExec_t& g = S.graph[tl][slot];

@@ -30,6 +30,7 @@ class Scene:
         self.sphere_lights = []  # omni and spot lights
         self.spheres = []
         self.quads = []
+        self.shapes = []         # disks, cylinders, cones
         self.meshes = []         # dicts of numpy arrays
         self.instances = []
 
@@ -136,6 +137,31 @@ class Scene:
         self.quads.append(dict(corner=tuple(corner), u=tuple(u), v=tuple(v), material=self._mat_index(material)))
         return len(self.quads) - 1
 
+    def _shape(self, kind, center, radius, height, material, mapping, translate, rotate_deg, scale):
+        has = ((A.CRT_XF_TRANSLATE if translate is not None else 0) | (A.CRT_XF_ROTATE if rotate_deg is not None else 0) |
+               (A.CRT_XF_SCALE if scale is not None else 0))
+        self.shapes.append(dict(kind=kind, center=tuple(center), radius=float(radius), height=float(height),
+                                material=self._mat_index(material), mapping=tuple(mapping), has=has,
+                                translate=tuple(translate or (0, 0, 0)), rotate_deg=tuple(rotate_deg or (0, 0, 0)),
+                                scale=tuple(scale or (1, 1, 1))))
+        return len(self.shapes) - 1
+
+    def disk(self, center, radius, height, material, mapping=(1, 1, 0, 0, 1, 1), translate=None, rotate_deg=None,
+             scale=None):
+        """Disk of `radius` in the plane y = center.y, facing +y (disk.cuh); `height` sizes its box.
+        translate / rotate_deg / scale place it as the reference's wrappers do."""
+        return self._shape(A.CRT_SHAPE_DISK, center, radius, height, material, mapping, translate, rotate_deg, scale)
+
+    def cylinder(self, center, radius, height, material, mapping=(1, 1, 0, 0, 1, 1), translate=None, rotate_deg=None,
+                 scale=None):
+        """Open vertical tube from the base centre up `height` (cylinder.cuh)."""
+        return self._shape(A.CRT_SHAPE_CYLINDER, center, radius, height, material, mapping, translate, rotate_deg, scale)
+
+    def cone(self, center, radius, height, material, mapping=(1, 1, 0, 0, 1, 1), translate=None, rotate_deg=None,
+             scale=None):
+        """Cone side, base circle at center.y, apex `height` above (cone.cuh)."""
+        return self._shape(A.CRT_SHAPE_CONE, center, radius, height, material, mapping, translate, rotate_deg, scale)
+
     def mesh(self, positions, indices, material, normals=None, uvs=None, tri_materials=None):
         m = dict(positions=np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3),
                  indices=np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3),
@@ -185,7 +211,7 @@ class Scene:
 
     @property
     def num_world_prims(self):
-        return (len(self.lights) + len(self.sphere_lights) + len(self.spheres) + len(self.quads) +
+        return (len(self.lights) + len(self.sphere_lights) + len(self.spheres) + len(self.quads) + len(self.shapes) +
                 sum(int(m["indices"].shape[0]) for m in self.meshes))
 
     # ---------------------------------------------------------------- C descriptor
@@ -241,6 +267,11 @@ class Scene:
             for s in self.spheres])
         d.quads, d.num_quads = arr(A.crt_quad, [
             A.crt_quad(_f3(q["corner"]), _f3(q["u"]), _f3(q["v"]), q["material"]) for q in self.quads])
+        d.shapes, d.num_shapes = arr(A.crt_shape, [
+            A.crt_shape(h["kind"], _f3(h["center"]), h["radius"], h["height"], h["material"],
+                        A.crt_uvmap(*[float(x) for x in h["mapping"]]),
+                        A.crt_transform(h["has"], _f3(h["translate"]), _f3(h["rotate_deg"]), _f3(h["scale"])))
+            for h in self.shapes])
         meshes = []
         fp = C.POINTER(C.c_float)
         up = C.POINTER(C.c_uint32)

@@ -199,6 +199,28 @@ def box_scene(coloured_walls=True):
     return s
 
 
+def shapes_scene():
+    """The box scene's walls and light with analytic shapes in place of the spheres: a checkered
+    disk on the floor, an image-textured cylinder, a metal cone, a glass tube lying on its side
+    (rotated about z, translated) and an elliptical disk on the back wall (rotated, translated and
+    scaled non-uniformly)."""
+    s = box_scene()
+    s.spheres = []
+    s.checker_texture("floor_check", 0.05, (0.9, 0.9, 0.9), (0.1, 0.2, 0.6))
+    s.image_texture("noise", checker_noise_texture(64, seed=7, cells=8))
+    s.lambertian("disk_mat", texture="floor_check")
+    s.lambertian("tube_mat", texture="noise")
+    s.metal("cone_mat", (0.85, 0.8, 0.7), fuzz=0.05)
+    s.dielectric("glass_tube", 1.5)
+    s.disk((-0.5, -0.99, -0.4), 0.35, 0.02, "disk_mat", mapping=(1, 1, 0, 0, 2, 2))
+    s.cylinder((0.0, -1.0, -0.5), 0.25, 0.6, "tube_mat", mapping=(1, 1, 0, 0, 2, 1))
+    s.cone((0.55, -1.0, -0.3), 0.3, 0.7, "cone_mat")
+    s.cylinder((0.0, 0.0, 0.0), 0.12, 0.7, "glass_tube", rotate_deg=(0, 0, 90), translate=(0.3, -0.2, -0.1))
+    s.disk((0.0, 0.0, 0.0), 0.2, 0.01, "red", rotate_deg=(90, 0, 0), translate=(0.2, -0.98, -0.3),
+           scale=(1.5, 1.0, 1.0))
+    return s
+
+
 def large_scene(seed=1, levels=8, n_spheres=500, aspect=16.0 / 9.0):
     """Icosphere of radius 3 subdivided `levels` times (1,310,720 triangles at 8) displaced by
     3-octave value noise of amplitude 0.05, plus `n_spheres` spheres with centres U([-10,10]^3),

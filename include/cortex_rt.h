@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CRT_ABI_VERSION 2u
+#define CRT_ABI_VERSION 3u
 
 /* ------------------------------------------------------------------ status codes */
 typedef enum crt_status {
@@ -160,6 +160,41 @@ typedef struct crt_sphere_light {
     float falloff;         /* SPOT: exponent of the cosine */
 } crt_sphere_light;
 
+/* Primitive transform (rt::transform, core/src/misc/transform.cuh): translate, rotate (degrees
+ * about x, y, z) and scale, each present or not (CRT_XF_* bits in `has`), applied as the reference's
+ * translate / rotate / scale wrappers see a primitive (scene_factory.cuh:710-725): the surface is the
+ * primitive under p -> S Rz(z) Ry(y) Rx(x) (p + T). A sphere under a non-uniform scale (an ellipsoid,
+ * outside the path) keeps its radius and is counted unsupported. */
+#define CRT_XF_TRANSLATE 1u
+#define CRT_XF_ROTATE 2u
+#define CRT_XF_SCALE 4u
+typedef struct crt_transform {
+    uint32_t has;
+    float translate[3];
+    float rotate_deg[3];
+    float scale[3];
+} crt_transform;
+
+/* Analytic shapes (core/src/primitives/disk.cuh:44-82, cylinder.cuh:50-103, cone.cuh:44-128),
+ * built as load_scene builds them (core/src/gpu.cu:565-595) and placed by `transform`:
+ * DISK     the disk |(x, z) - (cx, cz)| <= radius in the plane y = cy, normal +y (both faces);
+ *          `height` only sizes its box
+ * CYLINDER the open tube of `radius` around the vertical axis through `center`, cy <= y <= cy + height
+ *          (no caps); its normal is (p - center) / radius, not renormalised, as the reference has it
+ * CONE     the side of the cone with its base circle of `radius` at y = cy and its apex at
+ *          y = cy + height (no base cap)
+ * Texture coordinates follow get_disk_uv / get_cylinder_uv / get_cone_uv
+ * (core/src/utilities/uvmapping.cuh:285-368) under `mapping`. Shapes are never lights. */
+enum { CRT_SHAPE_DISK = 0, CRT_SHAPE_CYLINDER = 1, CRT_SHAPE_CONE = 2 };
+typedef struct crt_shape {
+    uint32_t kind;
+    float center[3];
+    float radius, height;
+    uint32_t material;
+    crt_uvmap mapping;
+    crt_transform transform;
+} crt_shape;
+
 /* Indexed triangle mesh (Moller-Trumbore in the reference, core/src/primitives/triangle.cuh:130-185;
  * watertight here). normals/uvs are optional per-vertex arrays (NULL = flat normal / zero uv). */
 typedef struct crt_mesh {
@@ -195,11 +230,13 @@ typedef struct crt_scene_desc {
     const crt_mesh* meshes;        uint32_t num_meshes;
     const crt_instance* instances; uint32_t num_instances;
     const crt_sphere_light* sphere_lights; uint32_t num_sphere_lights;
+    const crt_shape* shapes;       uint32_t num_shapes;
 } crt_scene_desc;
 
 /* Global primitive numbering used by first-hit queries (mirrors the reference's world-list order,
- * lights first, then spheres, quads, mesh triangles: core/src/gpu.cu:504-640):
- *   lights [0, L), spheres [L, L+S), quads [L+S, L+S+Q), triangles [L+S+Q, ...) in mesh order.
+ * lights first, then spheres, quads, shapes, mesh triangles: core/src/gpu.cu:504-640):
+ *   lights [0, L), spheres [L, L+S), quads [L+S, L+S+Q), shapes [L+S+Q, L+S+Q+H),
+ *   triangles [L+S+Q+H, ...) in mesh order.
  * The lights are the OMNI sphere lights, then the area lights, then the SPOT sphere lights (each
  * in array order), as load_scene adds omni, directional and spot lights (core/src/gpu.cu:504-523);
  * the light sampled by the estimator's mixture pdf is the first of them (hittable_list::random's
@@ -479,20 +516,7 @@ int crt_scenefile_close(crt_scenefile* file);
  * (core/src/scenes/scene_builder.cpp:49: from (0, 0, 10) at the origin, y up, fov 70, focus 100,
  * aspect 16:9), a black background. */
 typedef struct crt_builder crt_builder;
-/* Primitive transform (rt::transform, core/src/misc/transform.cuh): translate, rotate (degrees
- * about x, y, z) and scale, each present or not (CRT_XF_* bits in `has`), applied as the reference's
- * translate / rotate / scale wrappers see a primitive (scene_factory.cuh:710-725): the surface is the
- * primitive under p -> S Rz(z) Ry(y) Rx(x) (p + T). A sphere under a non-uniform scale (an ellipsoid,
- * outside the path) keeps its radius and is counted unsupported. */
-#define CRT_XF_TRANSLATE 1u
-#define CRT_XF_ROTATE 2u
-#define CRT_XF_SCALE 4u
-typedef struct crt_transform {
-    uint32_t has;
-    float translate[3];
-    float rotate_deg[3];
-    float scale[3];
-} crt_transform;
+
 int crt_builder_create(crt_builder** out);
 int crt_builder_destroy(crt_builder* b);
 int crt_builder_camera(crt_builder* b, const crt_camera* camera);
@@ -530,6 +554,11 @@ int crt_builder_quad(crt_builder* b, const char* name, const float* position, co
 /* box (core/src/primitives/box.cuh:41-74): [position - size/2, position + size/2] as 12 triangles */
 int crt_builder_box(crt_builder* b, const char* name, const float* position, const float* size, const char* material,
                     const crt_transform* xf);
+/* disk, cylinder or cone (CRT_SHAPE_*; createDisk / createCylinder / createCone,
+ * core/src/scene_factory.cuh): `position` is the disk's centre or the base centre; the transform is
+ * kept with the shape and applied at hit time as the reference's wrappers do */
+int crt_builder_shape(crt_builder* b, uint32_t kind, const char* name, const float* position, float radius, float height,
+                      const char* material, const crt_uvmap* mapping, const crt_transform* xf);
 /* OBJ mesh (createObjMesh, core/src/scene_factory.cuh:373-385); use_smoothing = 0 drops the
  * file's normals (flat shading) */
 int crt_builder_obj_mesh(crt_builder* b, const char* name, const char* filepath, const char* material,

@@ -146,7 +146,59 @@ struct Rng {
 };
 
 // ------------------------------------------------------------------ scene
-enum ObjType { OBJ_TRI = 0, OBJ_SPHERE = 1, OBJ_QUAD = 2, OBJ_LIGHT = 3, OBJ_SPHERE_LIGHT = 4 };
+enum ObjType { OBJ_TRI = 0, OBJ_SPHERE = 1, OBJ_QUAD = 2, OBJ_LIGHT = 3, OBJ_SPHERE_LIGHT = 4, OBJ_SHAPE = 5 };
+
+// The reference's translate / rotate / scale wrappers around one primitive
+// (core/src/scene_factory.cuh:710-725: translate innermost, scale outermost). A world ray reaches the
+// primitive divided by the scale (scale.cuh:50-56), rotated by Rx(-x) Ry(-y) Rz(-z) (glm::rotate
+// post-multiplies, rotate.cuh:104-119) and offset by -T (translate.cuh:38-43). The normal goes back
+// through the rotation's inverse Rz(z) Ry(y) Rx(x) and then the scale, where it is divided by the
+// scale and made unit (scale.cuh:58-60). (The reference rotates the normal back by Rx(x) Ry(y) Rz(z),
+// which is the same matrix for a rotation about one axis; this restatement takes the inverse for
+// two or more axes, as the builder maps the geometry of the other primitives.)
+template <class R> struct Xf {
+    bool t = false, rot = false, sc = false;
+    V3<R> T, S;
+    R Ri[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};   // Rx(-x) Ry(-y) Rz(-z)
+    void set(const crt_transform& x) {
+        t = (x.has & CRT_XF_TRANSLATE) != 0; rot = (x.has & CRT_XF_ROTATE) != 0; sc = (x.has & CRT_XF_SCALE) != 0;
+        T = V3<R>(R(x.translate[0]), R(x.translate[1]), R(x.translate[2]));
+        S = V3<R>(R(x.scale[0]), R(x.scale[1]), R(x.scale[2]));
+        if (!rot) return;
+        for (int axis = 0; axis < 3; ++axis) {
+            R a = -R(x.rotate_deg[axis]) * R(3.14159265358979323846) / R(180), c = std::cos(a), sn = std::sin(a);
+            R M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            int i = (axis + 1) % 3, j = (axis + 2) % 3;
+            M[3 * i + i] = c; M[3 * i + j] = -sn; M[3 * j + i] = sn; M[3 * j + j] = c;
+            R P[9];
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) P[3 * p + q] = Ri[3 * p] * M[q] + Ri[3 * p + 1] * M[3 + q] + Ri[3 * p + 2] * M[6 + q];
+            for (int k = 0; k < 9; ++k) Ri[k] = P[k];
+        }
+    }
+    V3<R> inv_rot(V3<R> v) const {
+        return {Ri[0] * v.x + Ri[1] * v.y + Ri[2] * v.z, Ri[3] * v.x + Ri[4] * v.y + Ri[5] * v.z, Ri[6] * v.x + Ri[7] * v.y + Ri[8] * v.z};
+    }
+    V3<R> fwd_rot(V3<R> v) const {
+        return {Ri[0] * v.x + Ri[3] * v.y + Ri[6] * v.z, Ri[1] * v.x + Ri[4] * v.y + Ri[7] * v.z, Ri[2] * v.x + Ri[5] * v.y + Ri[8] * v.z};
+    }
+    void to_object(V3<R>& o, V3<R>& d) const {
+        if (sc) { o = V3<R>(o.x / S.x, o.y / S.y, o.z / S.z); d = V3<R>(d.x / S.x, d.y / S.y, d.z / S.z); }
+        if (rot) { o = inv_rot(o); d = inv_rot(d); }
+        if (t) o = o - T;
+    }
+    V3<R> normal_to_world(V3<R> n) const {
+        if (rot) n = fwd_rot(n);
+        if (sc) n = unit(V3<R>(n.x / S.x, n.y / S.y, n.z / S.z));
+        return n;
+    }
+    V3<R> point_to_world(V3<R> p) const {
+        if (t) p = p + T;
+        if (rot) p = fwd_rot(p);
+        if (sc) p = V3<R>(p.x * S.x, p.y * S.y, p.z * S.z);
+        return p;
+    }
+};
 
 template <class R> struct Box {
     V3<R> lo, hi;  // interval per axis
@@ -164,6 +216,8 @@ template <class R> struct Obj {
     V3<R> c; R r; crt_uvmap map;
     // quad / light
     V3<R> q, qu, qv, qn, qw; R qd, area; bool invisible = false;
+    // analytic shape (CRT_SHAPE_*; centre and radius in c, r, mapping in map) under its wrappers
+    uint32_t kind = 0; R h = 0; Xf<R> xf;
     // sphere light (centre and radius in c, r): emission, spot cone
     V3<R> le; R lint = 0; bool spot = false; V3<R> sdir; R scut = 0, sfall = 0;
     Box<R> box;
@@ -340,6 +394,31 @@ template <class R> static int build_scene(Scene<R>& s, const crt_scene_desc* d, 
         }
         s.objs.push_back(o);
     }
+    // disks, cylinders and cones (created after the quads and boxes, core/src/gpu.cu:565-595); the
+    // box holds the corners of the shape's own box (disk.cuh:49, cylinder.cuh:58-61, cone.cuh:52-55)
+    // mapped to the world, padded
+    if (d->num_shapes && !d->shapes) return CRT_ERR_INVALID_SCENE;
+    for (uint32_t i = 0; i < d->num_shapes; ++i, ++gid) {
+        const crt_shape& H = d->shapes[i];
+        if (H.material >= d->num_materials || H.kind > CRT_SHAPE_CONE) return CRT_ERR_INVALID_SCENE;
+        Obj<R> o{};
+        o.type = OBJ_SHAPE; o.gid = gid; o.inst = -1; o.mat = H.material;
+        o.kind = H.kind; o.c = from3<R>(H.center); o.r = R(H.radius); o.h = R(H.height); o.map = H.mapping;
+        o.xf.set(H.transform);
+        const R rad = std::fabs(o.r);
+        const R ylo = H.kind == CRT_SHAPE_DISK ? -o.h / R(2) : std::min(R(0), o.h);
+        const R yhi = H.kind == CRT_SHAPE_DISK ? o.h / R(2) : std::max(R(0), o.h);
+        Box<R> b{o.xf.point_to_world(o.c + V3<R>(-rad, ylo, -rad)), o.xf.point_to_world(o.c + V3<R>(-rad, ylo, -rad))};
+        for (int k = 1; k < 8; ++k) {
+            V3<R> w = o.xf.point_to_world(o.c + V3<R>((k & 1) ? rad : -rad, (k & 2) ? yhi : ylo, (k & 4) ? rad : -rad));
+            b = merge(b, box_points(w, w));
+        }
+        R ext = 0;
+        for (int a = 0; a < 3; ++a) ext = std::max(ext, std::max(std::fabs(b.lo[a]), std::fabs(b.hi[a])));
+        const V3<R> pd(R(1e-5) * ext + R(1e-6), R(1e-5) * ext + R(1e-6), R(1e-5) * ext + R(1e-6));
+        o.box = pad(Box<R>{b.lo - pd, b.hi + pd});
+        s.objs.push_back(o);
+    }
     auto add_mesh = [&](const crt_mesh& M, const float* xf, int32_t inst, int32_t mat_override, int32_t gid0) -> int {
         for (uint32_t t = 0; t < M.num_triangles; ++t) {
             uint32_t i0 = M.indices[3 * t], i1 = M.indices[3 * t + 1], i2 = M.indices[3 * t + 2];
@@ -504,6 +583,35 @@ template <class R> static void sphere_uv(V3<R> p, const crt_uvmap& m, R& u, R& v
     v = R(m.scale_v) * t + R(m.offset_v);
 }
 
+// get_cylinder_uv / get_disk_uv / get_cone_uv (core/src/utilities/uvmapping.cuh:285-368; M_PI is the
+// float literal of core/src/misc/constants.cuh:35). The cylinder and the cone are given their
+// outward vector, the disk its hit point relative to the centre; the cone applies the u repeat twice.
+template <class R> static void finish_uv(R s_, R t_, const crt_uvmap& m, R& u, R& v) {
+    u = R(m.scale_u) * s_ + R(m.offset_u);
+    v = R(m.scale_v) * t_ + R(m.offset_v);
+}
+template <class R> static void cylinder_uv(V3<R> p, R h, const crt_uvmap& m, R& u, R& v) {
+    const R pi = R(3.14159265358979323846);
+    R theta = std::atan2(p.x, p.z);
+    R s_ = R(1) - (theta + pi) / (R(2) * pi);
+    R t_ = (p.y + h / R(2)) / h;
+    finish_uv(std::fmod(s_ * R(m.repeat_u), R(1)), std::fmod(t_ * R(m.repeat_v), R(1)), m, u, v);
+}
+template <class R> static void disk_uv(V3<R> p, R r, const crt_uvmap& m, R& u, R& v) {
+    R s_ = p.x / (R(2) * r) + R(0.5f);
+    R t_ = p.z / (R(2) * r) + R(0.5f);
+    finish_uv(std::fmod(s_ * R(m.repeat_u), R(1)), std::fmod(t_ * R(m.repeat_v), R(1)), m, u, v);
+}
+template <class R> static void cone_uv(V3<R> p, R r, const crt_uvmap& m, R& u, R& v) {
+    const R pi = R(3.14159265358979323846);
+    R theta = std::atan2(p.x, p.z);
+    R s_ = (theta + pi) / (R(2) * pi);
+    R t_ = std::sqrt(p.x * p.x + p.z * p.z) / r;
+    s_ = std::fmod(s_ * R(m.repeat_u), R(1));
+    s_ = std::fmod(s_ * R(m.repeat_u), R(1));
+    finish_uv(s_, std::fmod(t_ * R(m.repeat_v), R(1)), m, u, v);
+}
+
 // get_triangle_uv (core/src/utilities/uvmapping.cuh:415-447, get_barycenter math_utils.cuh:73-86)
 template <class R> static void triangle_uv_at(const Obj<R>& o, V3<R> p, R& u, R& v) {
     V3<R> v0 = o.v[1] - o.v[0], v1 = o.v[2] - o.v[0], v2 = p - o.v[0];
@@ -595,6 +703,74 @@ template <class R> static bool obj_hit(const Scene<R>& s, int idx, const Ray<R>&
         V3<R> outward = (rec.p - o.c) / o.r;
         set_face_normal(rec, r, outward);
         sphere_uv(outward, o.map, rec.u, rec.v);
+        rec.obj = idx;
+        return true;
+    }
+    if (o.type == OBJ_SHAPE) {
+        // the ray in the shape's space (its wrappers), then disk::hit (core/src/primitives/disk.cuh:
+        // 53-82), cylinder::hit (cylinder.cuh:63-103) or cone::hit (cone.cuh:58-128) on the closed
+        // interval. Written so that a NaN distance (a ray parallel to the disk, or along the
+        // cylinder's axis) is no hit; the reference's comparisons let some of those through.
+        V3<R> ro = r.o, rd = r.d;
+        o.xf.to_object(ro, rd);
+        R t;
+        V3<R> outward, uvp;
+        if (o.kind == CRT_SHAPE_DISK) {
+            t = (o.c.y - ro.y) / rd.y;
+            if (!(tmin <= t && t <= tmax)) return false;
+            V3<R> hp = ro + t * rd;
+            R dist2 = (hp.x - o.c.x) * (hp.x - o.c.x) + (hp.z - o.c.z) * (hp.z - o.c.z);
+            if (!(dist2 <= o.r * o.r)) return false;
+            outward = V3<R>(0, 1, 0);
+            uvp = hp - o.c;
+        } else {
+            R a, b, c, disc;
+            if (o.kind == CRT_SHAPE_CYLINDER) {
+                V3<R> oc = ro - o.c;
+                a = rd.x * rd.x + rd.z * rd.z;
+                b = R(2) * (oc.x * rd.x + oc.z * rd.z);
+                c = oc.x * oc.x + oc.z * oc.z - o.r * o.r;
+                disc = b * b - R(4) * a * c;
+                if (!(disc >= 0)) return false;
+            } else {
+                R A = ro.x - o.c.x, B = ro.z - o.c.z, D = o.h - ro.y + o.c.y;
+                R tn = (o.r / o.h) * (o.r / o.h);
+                a = rd.x * rd.x + rd.z * rd.z - tn * (rd.y * rd.y);
+                b = R(2) * A * rd.x + R(2) * B * rd.z + R(2) * tn * D * rd.y;
+                c = A * A + B * B - tn * (D * D);
+                disc = b * b - R(4) * (a * c);
+                if (std::fabs(disc) < R(0.001f) || !(disc >= 0)) return false;
+            }
+            R sq = std::sqrt(disc);
+            t = (-b - sq) / (R(2) * a);
+            if (!(tmin <= t && t <= tmax)) {
+                t = (-b + sq) / (R(2) * a);
+                if (!(tmin <= t && t <= tmax)) return false;
+            }
+            R y = ro.y + t * rd.y;
+            if (!(y >= o.c.y && y <= o.c.y + o.h)) return false;
+            V3<R> hp = ro + t * rd;
+            if (o.kind == CRT_SHAPE_CYLINDER) {
+                outward = (hp - o.c) / o.r;   // with its y part, not made unit (cylinder.cuh:93-95)
+                uvp = outward;
+            } else {
+                if (hp.y <= o.c.y) uvp = V3<R>(0, -1, 0);
+                else if (hp.y >= o.c.y + o.h) uvp = V3<R>(0, 1, 0);
+                else {
+                    R rs = std::sqrt((hp.x - o.c.x) * (hp.x - o.c.x) + (hp.z - o.c.z) * (hp.z - o.c.z));
+                    uvp = V3<R>(hp.x - o.c.x, rs * (o.r / o.h), hp.z - o.c.z);
+                }
+                outward = unit(uvp);
+            }
+        }
+        // set_face_normal in the shape's space, then the normal back through the wrappers
+        rec.front = dot(rd, outward) < R(0);
+        rec.normal = o.xf.normal_to_world(rec.front ? outward : -outward);
+        if (o.kind == CRT_SHAPE_DISK) disk_uv(uvp, o.r, o.map, rec.u, rec.v);
+        else if (o.kind == CRT_SHAPE_CYLINDER) cylinder_uv(uvp, o.h, o.map, rec.u, rec.v);
+        else cone_uv(uvp, o.r, o.map, rec.u, rec.v);
+        rec.t = t;
+        rec.p = r.o + t * r.d;
         rec.obj = idx;
         return true;
     }
@@ -1092,7 +1268,8 @@ int oracle_render(const crt_scene_desc* d, const crt_render_desc* rd, const orac
 
 /* Closest hit of arbitrary rays: rays n x 8 doubles (o, tmin, d, tmax). */
 int oracle_trace(const crt_scene_desc* d, const double* rays, uint64_t n, int32_t camera_rays, int32_t precision,
-                 int32_t accel, int32_t ref_style, int32_t* prim, int32_t* inst, double* t, double* uv) {
+                 int32_t accel, int32_t ref_style, int32_t* prim, int32_t* inst, double* t, double* uv,
+                 double* normal) {
     auto run = [&](auto tag) -> int {
         using R = decltype(tag);
         orc::Scene<R> s;
@@ -1114,8 +1291,10 @@ int oracle_trace(const crt_scene_desc* d, const double* rays, uint64_t n, int32_
                 prim[k] = o.gid; inst[k] = o.inst; t[k] = (double)h.t;
                 if (o.type == orc::OBJ_TRI) { uv[2 * k] = (double)h.bu; uv[2 * k + 1] = (double)h.bv; }
                 else { uv[2 * k] = (double)h.u; uv[2 * k + 1] = (double)h.v; }
+                if (normal) for (int a = 0; a < 3; ++a) normal[3 * k + a] = (double)h.normal[a];
             } else {
                 prim[k] = -1; inst[k] = -1; t[k] = 0; uv[2 * k] = uv[2 * k + 1] = 0;
+                if (normal) normal[3 * k] = normal[3 * k + 1] = normal[3 * k + 2] = 0;
             }
         }
         return CRT_OK;

@@ -65,7 +65,9 @@ def render(scene, width, height, spp, max_depth, seed=1, rng="pcg", precision=64
     return acc, {k: getattr(st, k) for k, _ in oracle_stats._fields_}
 
 
-def trace(scene, rays, camera=False, precision=64, accel=1, ref_style=False):
+def trace(scene, rays, camera=False, precision=64, accel=1, ref_style=False, normals=False):
+    """Closest hits: (prim, inst, t, uv), and with normals=True also the hit record's normal (the
+    face the ray meets, as set_face_normal leaves it)."""
     desc, keep = scene.to_desc()
     rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 8)
     n = rays.shape[0]
@@ -73,14 +75,17 @@ def trace(scene, rays, camera=False, precision=64, accel=1, ref_style=False):
     inst = np.zeros(n, np.int32)
     t = np.zeros(n, np.float64)
     uv = np.zeros((n, 2), np.float64)
+    nrm = np.zeros((n, 3), np.float64)
     f = lib().oracle_trace
     f.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
-                  C.c_void_p, C.c_void_p]
+                  C.c_void_p, C.c_void_p, C.c_void_p]
     rc = f(C.cast(C.pointer(desc), C.c_void_p), _fp(rays), n, 1 if camera else 0, precision, accel,
-           1 if ref_style else 0, _fp(prim), _fp(inst), _fp(t), _fp(uv))
+           1 if ref_style else 0, _fp(prim), _fp(inst), _fp(t), _fp(uv), _fp(nrm) if normals else None)
     if rc:
         raise RuntimeError("oracle_trace failed: %d" % rc)
     del keep
+    if normals:
+        return prim, inst, t, uv, nrm
     return prim, inst, t, uv
 
 

@@ -3,8 +3,9 @@ reference tree exist; the tests only read the JSON files).
 
   thrust_kat.json : rocThrust minstd_rand / uniform distributions (tests/golden/thrust_kat.hip)
   pcg32_kat.json  : pcg32 from the reference's vendored libs/pcg (oracle/_ref/pcg_kat)
-  scenes/shipped_*.scene : input data: the reference's self-contained scene files (no mesh or
-                    texture files) copied byte for byte, rendered by the GPU tests
+  scenes/shipped_*.scene : input data: the reference's scene files that name no mesh file, copied
+                    byte for byte, rendered by the GPU tests (extended_primitives names a JPEG
+                    texture that is not kept: its material samples an empty image)
 """
 import json
 import os
@@ -38,7 +39,7 @@ def pcg():
         json.dump({"source": "libs/pcg/pcg_random.hpp pcg32", "cases": cases}, f)
 
 
-SHIPPED = ("all_lights_types.scene", "simple_spot_light.scene")
+SHIPPED = ("all_lights_types.scene", "simple_spot_light.scene", "extended_primitives.scene")
 
 
 def shipped_scenes():

@@ -24,7 +24,7 @@ def test_box_scene_file_mapping():
     assert (info["width"], info["height"], info["spp"], info["max_depth"]) == (256, 256, 16, 8)
     assert abs(info["aspect_ratio"] - 1.0) < 1e-7
     assert info["skipped"] == 1            # the sphere whose material does not exist
-    assert info["unsupported"] == 1        # the cone
+    assert info["unsupported"] == 1        # the torus
     assert d.num_lights == 1 and d.num_quads == 6 and d.num_spheres == 3 and d.num_meshes == 2
     L = d.lights[0]
     assert list(L.center) == pytest.approx([0, 0.99, 0]) and L.invisible == 1 and list(L.rgb) == [15, 15, 15]
@@ -231,23 +231,33 @@ def test_camera_defaults_and_orthographic_camera(tmp_path):
     assert np.allclose(np.linalg.norm(b["vertical"]), 4.0)
 
 
-SHIPPED = ["all_lights_types.scene", "simple_spot_light.scene"]
+SHIPPED = ["all_lights_types.scene", "simple_spot_light.scene", "extended_primitives.scene"]
 
 
 @pytest.mark.parametrize("name", SHIPPED)
 def test_shipped_scene_fixtures(name):
     # the copies the GPU tests render are the reference's files byte for byte and read in full:
     # all_lights_types has its quad light only (spot and omni are active = false), simple_spot_light
-    # a spot light between four boxes (ground, roof, two walls) of 12 triangles each
+    # a spot light between four boxes (ground, roof, two walls) of 12 triangles each,
+    # extended_primitives a cone, a cylinder and a disk with two boxes, a sphere and an omni light
+    # (its torus is counted unsupported)
     path = os.path.join(HERE, "golden", "scenes", "shipped_" + name)
     src = os.path.join("/root/reference/data/scenes", name)
     if os.path.exists(src):
         with open(src, "rb") as a, open(path, "rb") as b:
             assert a.read() == b.read()
     f = SceneFile(path)
-    assert f.info()["unsupported"] == 0 and f.info()["skipped"] == 0
+    extended = name == "extended_primitives.scene"
+    assert f.info()["unsupported"] == (1 if extended else 0) and f.info()["skipped"] == 0
     d, _ = f.to_desc()
-    if name == "all_lights_types.scene":
+    if extended:
+        assert (d.num_spheres, d.num_lights, d.num_sphere_lights, d.num_shapes, d.num_meshes) == (1, 0, 1, 3, 2)
+        kinds = [d.shapes[k].kind for k in range(3)]
+        assert kinds == [A.CRT_SHAPE_CONE, A.CRT_SHAPE_CYLINDER, A.CRT_SHAPE_DISK]   # load_scene's order
+        cyl = d.shapes[1]
+        assert list(cyl.center) == pytest.approx([-2.0, 0.0, 1.0]) and (cyl.radius, cyl.height) == pytest.approx((0.4, 0.6))
+        assert (cyl.mapping.repeat_u, cyl.mapping.repeat_v) == pytest.approx((1.8, 0.2)) and cyl.transform.has == 0
+    elif name == "all_lights_types.scene":
         assert (d.num_spheres, d.num_lights, d.num_sphere_lights) == (7, 1, 0)
     else:
         assert (d.num_spheres, d.num_lights, d.num_sphere_lights) == (2, 0, 1)
