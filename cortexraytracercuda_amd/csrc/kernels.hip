@@ -1559,8 +1559,11 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                     d = kk >= 0.0f ? (ratio * ud - (ratio * dd + sqrtf(kk)) * nrm) : v3(0.0f, 0.0f, 0.0f);
                 }
                 done = false;
-            } else if (m.m1.x == CRT_MAT_LAMBERTIAN) {
-                // mixture of light sampling and cosine sampling (camera.cuh:203-216)
+            } else if (m.m1.x == CRT_MAT_LAMBERTIAN || m.m1.x == CRT_MAT_ISOTROPIC) {
+                // mixture of light sampling and cosine sampling (camera.cuh:203-216); isotropic
+                // (isotropic.cuh) mixes the lights with sphere_pdf instead: uniform directions
+                // (get_unit_vector, gpu_randomizer.cuh:53-60) and pdf and scattering pdf 1 / (4 pi)
+                const bool iso = m.m1.x == CRT_MAT_ISOTROPIC;
                 // onb::build_from_w (core/src/misc/onb.cuh:122-128)
                 V3 w = unit(nrm);
                 V3 aa = fabsf(w.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
@@ -1601,6 +1604,11 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                             dir = pp - p;
                         }
                     }
+                } else if (iso) {
+                    const float a = g.uniform() * 6.28318530718f;   // get_real(0, M_DOUBLE_PI)
+                    const float z = g.uniform() * 2.0f - 1.0f;      // get_real(-1, 1)
+                    const float r = sqrtf(1.0f - z * z);
+                    dir = v3(r * cosf(a), r * sinf(a), z);
                 } else {
                     // random_cosine_direction (core/src/misc/gpu_randomizer.cuh:122-133)
                     float r1 = g.uniform(), r2 = g.uniform();
@@ -1640,10 +1648,11 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                 }
                 V3 ud = unit(dir);
                 float cv = dot(ud, w) / 3.14159265358979323846f;
-                float pc = 0.0f > cv ? 0.0f : cv;
+                const float inv4pi = 1.0f / (4.0f * 3.14159265358979323846f);
+                float pc = iso ? inv4pi : (0.0f > cv ? 0.0f : cv);
                 float pdf = 0.5f * sum + (1.0f - 0.5f) * pc;
                 float ct = dot(nrm, ud);
-                float spdf = ct < 0.0f ? 0.0f : ct / 3.14159265358979323846f;
+                float spdf = iso ? inv4pi : (ct < 0.0f ? 0.0f : ct / 3.14159265358979323846f);
                 att = att * ((1.0f / pdf) * (alb * spdf));
                 d = dir;
                 done = false;

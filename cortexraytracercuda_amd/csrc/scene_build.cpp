@@ -651,7 +651,9 @@ int prepare_scene_tables(const crt_scene_desc* d, uint32_t leaf_size, uint32_t b
         if (m.type == CRT_MAT_METAL) param = m.fuzz < 1.0f ? m.fuzz : 1.0f;  // metal ctor clamp (metal.cuh:52-55)
         else if (m.type == CRT_MAT_DIELECTRIC) param = m.ior;
         else if (m.type == CRT_MAT_DIFFUSE_LIGHT) param = m.intensity;
-        else if (m.type != CRT_MAT_LAMBERTIAN) { err = "unsupported material type " + std::to_string(m.type); return CRT_ERR_INVALID_SCENE; }
+        else if (m.type != CRT_MAT_LAMBERTIAN && m.type != CRT_MAT_ISOTROPIC) {
+            err = "unsupported material type " + std::to_string(m.type); return CRT_ERR_INVALID_SCENE;
+        }
         if (m.texture >= 0 && (uint32_t)m.texture >= d->num_textures) {
             err = "material " + std::to_string(i) + " references missing texture"; return CRT_ERR_INVALID_SCENE;
         }

@@ -256,6 +256,20 @@ int crt_builder_lambertian(crt_builder* b, const char* name, const float* rgb, c
     return CRT_OK;
 }
 
+// isotropic (core/src/materials/isotropic.cuh): the albedo of its texture, or of a solid rgb when no
+// texture is named (createIsotropicMaterial, core/src/gpu.cu:407-421)
+int crt_builder_isotropic(crt_builder* b, const char* name, const float* rgb, const char* texture) {
+    CRT_B_CHECK(b);
+    CRT_B(b);
+    crt_material m{};
+    m.type = CRT_MAT_ISOTROPIC;
+    copy3(m.albedo, rgb);
+    int rc = texture_of(B, texture, m.texture);
+    if (rc) return rc;
+    add_material(B, name, m);
+    return CRT_OK;
+}
+
 int crt_builder_metal(crt_builder* b, const char* name, const float* rgb, float fuzz, const char* texture) {
     CRT_B_CHECK(b);
     CRT_B(b);

@@ -11,7 +11,7 @@
 //   textures   solidColor (name color), checker (name scale oddColor/evenColor or
 //              oddTextureName/evenTextureName), image (name filepath: PNG or JPEG)
 //   materials  lambertian (name color | texture), metal (name color fuzziness), glass
-//              (name refraction) and dielectric (same keys)
+//              (name refraction) and dielectric (same keys), isotropic (name color texture)
 //   primitives spheres (name position radius material uvmapping), quads (name position u v
 //              material), boxes (name position size material), cones / cylinders / disks (name
 //              position radius height material uvmapping)
@@ -437,8 +437,19 @@ static int load_scenefile(SceneFile& sf, std::string& err) {
                 }
             }
         }
+        // isotropic: added only when it names a texture, as addIsotropicMaterial does
+        // (scene_loader.cpp: `if (!textureName.empty())`); a primitive naming one without is skipped
+        if (const cfg::Value* l = list_of(M, "isotropic")) {
+            for (auto& it : l->items) {
+                const std::string tex = str_of(it->get("texture"));
+                if (tex.empty()) continue;
+                float rgb[3] = {0.0f, 0.0f, 0.0f};
+                if (it->get("color")) color_of(it->get("color"), rgb);
+                if ((rc = crt_builder_isotropic(B, str_of(it->get("name")).c_str(), rgb, tex.c_str()))) return rc;
+            }
+        }
         // materials outside the render path become Lambertian with their colour (mid grey if none)
-        for (const char* k : {"isotropic", "anisotropic", "orennayar", "phong"}) {
+        for (const char* k : {"anisotropic", "orennayar", "phong"}) {
             if (const cfg::Value* l = list_of(M, k)) {
                 for (auto& it : l->items) {
                     float rgb[3] = {0.5f, 0.5f, 0.5f};

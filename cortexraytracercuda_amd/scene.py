@@ -96,6 +96,10 @@ class Scene:
         return self._add_mat(name, dict(type=A.CRT_MAT_METAL, albedo=tuple(rgb), fuzz=float(fuzz),
                                         texture=self._tex_index(texture)))
 
+    def isotropic(self, name, rgb=(0.5, 0.5, 0.5), texture=None):
+        """isotropic.cuh: albedo from the texture (or rgb), scattered over the whole sphere."""
+        return self._add_mat(name, dict(type=A.CRT_MAT_ISOTROPIC, albedo=tuple(rgb), texture=self._tex_index(texture)))
+
     def dielectric(self, name, ior):
         return self._add_mat(name, dict(type=A.CRT_MAT_DIELECTRIC, albedo=(1.0, 1.0, 1.0), ior=float(ior)))
 

@@ -75,6 +75,9 @@ enum {
     CRT_MAT_LAMBERTIAN = 1,      /* core/src/materials/lambertian.cuh:63-143 */
     CRT_MAT_METAL = 2,           /* core/src/materials/metal.cuh:52-65 */
     CRT_MAT_DIELECTRIC = 3,      /* core/src/materials/dielectric.cuh:32-63 */
+    CRT_MAT_ISOTROPIC = 4,       /* core/src/materials/isotropic.cuh: albedo from its texture, directions from
+                                    the mixture of the lights and the uniform sphere (sphere_pdf), scattering
+                                    pdf 1 / (4 pi) */
     CRT_MAT_DIFFUSE_LIGHT = 8,   /* core/src/materials/diffuse_light.cuh:45-67 */
     CRT_MAT_SPOT_LIGHT = 9       /* diffuse_spot_light, core/src/materials/diffuse_spot_light.cuh:33-61: the
                                     material of a spot light (crt_sphere_light), not for crt_material */
@@ -534,6 +537,7 @@ int crt_builder_image_texture_pixels(crt_builder* b, const char* name, uint32_t 
 int crt_builder_lambertian(crt_builder* b, const char* name, const float* rgb, const char* texture);
 int crt_builder_metal(crt_builder* b, const char* name, const float* rgb, float fuzz, const char* texture);
 int crt_builder_dielectric(crt_builder* b, const char* name, float ior);
+int crt_builder_isotropic(crt_builder* b, const char* name, const float* rgb, const char* texture);
 int crt_builder_diffuse_light(crt_builder* b, const char* name, const float* rgb, float intensity, int32_t one_sided);
 /* 1 when a material of that name exists */
 int crt_builder_has_material(crt_builder* b, const char* name);

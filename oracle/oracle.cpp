@@ -1133,19 +1133,29 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
             ray = Ray<R>{rec.p, dir, ray.time};
             continue;  // attenuation 1
         }
-        if (m.type != CRT_MAT_LAMBERTIAN) break;  // unsupported material: no scatter
+        const bool iso = m.type == CRT_MAT_ISOTROPIC;
+        if (m.type != CRT_MAT_LAMBERTIAN && !iso) break;  // unsupported material: no scatter
         V3<R> alb = albedo_of(s, m, rec);
-        // mixture_pdf(hittable_pdf(lights, p), cosine_pdf(n)) (camera.cuh:203-216, mixture_pdf.cuh:28-48)
+        // mixture_pdf(hittable_pdf(lights, p), cosine_pdf(n)) (camera.cuh:203-216, mixture_pdf.cuh:28-48);
+        // isotropic (isotropic.cuh) puts sphere_pdf in the cosine's place (sphere_pdf.cuh:20-28): a
+        // uniform direction from get_unit_vector (gpu_randomizer.cuh:53-60: angle u * 2pi, then
+        // z = 2u - 1, get_real(lo, hi) = u (hi - lo) + lo), pdf and scattering pdf 1 / (4 pi)
         Onb<R> uvw(rec.normal);
         V3<R> dir;
         if (g.uniform() < 0.5f) dir = lights_random(s, rec.p, g);
-        else dir = uvw.local(random_cosine_direction<R>(g));
+        else if (iso) {
+            R a = R(g.uniform()) * R(6.28318530718f);
+            R z = R(g.uniform()) * R(2) - R(1);
+            R r = std::sqrt(R(1) - z * z);
+            dir = V3<R>(r * std::cos(a), r * std::sin(a), z);
+        } else dir = uvw.local(random_cosine_direction<R>(g));
         R pl = lights_pdf_value(s, rec.p, dir, cnt);
+        const R inv4pi = R(1) / (R(4) * R(3.14159265358979323846));
         R cv = dot(unit(dir), uvw.w) / R(3.14159265358979323846);
-        R pc = R(0) > cv ? R(0) : cv;  // ffmax(0, x)
+        R pc = iso ? inv4pi : (R(0) > cv ? R(0) : cv);  // ffmax(0, x)
         R pdf = R(0.5f) * pl + (R(1.0f) - R(0.5f)) * pc;
         R ct = dot(rec.normal, unit(dir));
-        R spdf = ct < 0 ? R(0) : ct / R(3.14159265358979323846);
+        R spdf = iso ? inv4pi : (ct < 0 ? R(0) : ct / R(3.14159265358979323846));
         // colour / float is (1/t) * colour (core/src/misc/color.cuh:105-108)
         att = att * ((R(1) / pdf) * (alb * spdf));
         ray = Ray<R>{rec.p, dir, ray.time};

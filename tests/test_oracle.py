@@ -586,3 +586,27 @@ def test_instanced_triangle_equals_world_placed_triangle():
     assert np.array_equal(pi, pw)
     assert np.all(ii[pi >= 0] == 0) and np.all(iw == -1)
     assert np.allclose(ti, tw, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("occluded", [False, True])
+def test_isotropic_furnace(occluded):
+    # isotropic floor of albedo rho under a uniform sky of radiance 1 (isotropic.cuh, sphere_pdf.cuh):
+    # half of the draws sample the zero-emission light, the other half a uniform direction weighted
+    # (1/4pi) / (0.5 * 1/4pi) = 2, so the pixel is rho when every direction sees the sky; with a black
+    # floor just below, the lower half of the sphere sees black and the pixel is rho / 2 (a cosine or
+    # hemisphere sampler would give rho)
+    rho = 0.6
+    s = Scene()
+    s.set_camera((0, 1, 0), (0, 0, 0), (0, 0, -1), vfov=30.0, aspect_ratio=1.0, focus_dist=1.0)
+    s.background = (1.0, 1.0, 1.0)
+    m = s.isotropic("floor", (rho, rho, rho))
+    s.quad((-50, 0, -50), (100, 0, 0), (0, 0, 100), m)
+    if occluded:
+        black = s.lambertian("black", (0.0, 0.0, 0.0))
+        s.quad((-50, -0.001, -50), (100, 0, 0), (0, 0, 100), black)
+    s.area_light((0, 100, 0), (0.01, 0, 0), (0, 0, 0.01), rgb=(0, 0, 0), invisible=True)
+    want = rho / 2 if occluded else rho
+    for prec, rng, seed in ((64, "pcg", 3), (32, "minstd", 2580)):
+        acc, _ = O.render(s, 16, 16, 64, 2, seed=seed, rng=rng, precision=prec, threads=os.cpu_count() or 4)
+        val = acc[..., :3].sum() / (16 * 16 * 64 * 3)
+        assert abs(val - want) < 0.015, (prec, val, want)
