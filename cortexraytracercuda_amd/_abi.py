@@ -27,6 +27,7 @@ CRT_MAT_LAMBERTIAN = 1
 CRT_MAT_METAL = 2
 CRT_MAT_DIELECTRIC = 3
 CRT_MAT_ISOTROPIC = 4
+CRT_MAT_OREN_NAYAR = 5
 CRT_MAT_DIFFUSE_LIGHT = 8
 CRT_MAT_SPOT_LIGHT = 9
 CRT_LIGHT_OMNI = 0
@@ -226,7 +227,7 @@ EXPORTED = ["crt_create", "crt_destroy", "crt_last_error", "crt_device_count", "
             "crt_builder_image_texture", "crt_builder_image_texture_pixels", "crt_builder_lambertian", "crt_builder_metal",
             "crt_builder_dielectric", "crt_builder_diffuse_light", "crt_builder_has_material",
             "crt_builder_directional_light", "crt_builder_omni_light", "crt_builder_spot_light", "crt_builder_sphere",
-            "crt_builder_quad", "crt_builder_box", "crt_builder_shape", "crt_builder_isotropic", "crt_builder_obj_mesh", "crt_builder_note_unsupported",
+            "crt_builder_quad", "crt_builder_box", "crt_builder_shape", "crt_builder_isotropic", "crt_builder_oren_nayar", "crt_builder_obj_mesh", "crt_builder_note_unsupported",
             "crt_builder_desc"]
 
 _P = C.POINTER
@@ -271,6 +272,7 @@ _SIGS = {
     "crt_builder_metal": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_float, C.c_char_p], C.c_int),
     "crt_builder_dielectric": ([C.c_void_p, C.c_char_p, C.c_float], C.c_int),
     "crt_builder_isotropic": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_char_p], C.c_int),
+    "crt_builder_oren_nayar": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_char_p, C.c_float, C.c_float], C.c_int),
     "crt_builder_diffuse_light": ([C.c_void_p, C.c_char_p, _P(C.c_float), C.c_float, C.c_int32], C.c_int),
     "crt_builder_has_material": ([C.c_void_p, C.c_char_p], C.c_int),
     "crt_builder_directional_light": ([C.c_void_p, C.c_char_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float,

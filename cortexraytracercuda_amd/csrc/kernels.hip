@@ -1559,11 +1559,18 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                     d = kk >= 0.0f ? (ratio * ud - (ratio * dd + sqrtf(kk)) * nrm) : v3(0.0f, 0.0f, 0.0f);
                 }
                 done = false;
-            } else if (m.m1.x == CRT_MAT_LAMBERTIAN || m.m1.x == CRT_MAT_ISOTROPIC) {
+            } else if ((m.m1.x == CRT_MAT_LAMBERTIAN || m.m1.x == CRT_MAT_ISOTROPIC || m.m1.x == CRT_MAT_OREN_NAYAR) &&
+                       (m.m1.x != CRT_MAT_OREN_NAYAR || sc->num_lights > 0)) {
                 // mixture of light sampling and cosine sampling (camera.cuh:203-216); isotropic
                 // (isotropic.cuh) mixes the lights with sphere_pdf instead: uniform directions
                 // (get_unit_vector, gpu_randomizer.cuh:53-60) and pdf and scattering pdf 1 / (4 pi)
                 const bool iso = m.m1.x == CRT_MAT_ISOTROPIC;
+                // oren_nayar::scatter (oren_nayar.cuh): a random_on_hemisphere direction it never uses
+                // (two draws), sphere_pdf in the mixture, the cosine scattering pdf, and the
+                // attenuation of light 0 (world light order) below
+                const bool on = m.m1.x == CRT_MAT_OREN_NAYAR;
+                if (on) { g.skip(); g.skip(); }
+                const bool sph = iso || on;
                 // onb::build_from_w (core/src/misc/onb.cuh:122-128)
                 V3 w = unit(nrm);
                 V3 aa = fabsf(w.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
@@ -1604,7 +1611,7 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                             dir = pp - p;
                         }
                     }
-                } else if (iso) {
+                } else if (sph) {
                     const float a = g.uniform() * 6.28318530718f;   // get_real(0, M_DOUBLE_PI)
                     const float z = g.uniform() * 2.0f - 1.0f;      // get_real(-1, 1)
                     const float r = sqrtf(1.0f - z * z);
@@ -1649,10 +1656,24 @@ __device__ __forceinline__ bool shade_hit(const crt_dev_scene* __restrict__ sc, 
                 V3 ud = unit(dir);
                 float cv = dot(ud, w) / 3.14159265358979323846f;
                 const float inv4pi = 1.0f / (4.0f * 3.14159265358979323846f);
-                float pc = iso ? inv4pi : (0.0f > cv ? 0.0f : cv);
+                float pc = sph ? inv4pi : (0.0f > cv ? 0.0f : cv);
                 float pdf = 0.5f * sum + (1.0f - 0.5f) * pc;
                 float ct = dot(nrm, ud);
                 float spdf = iso ? inv4pi : (ct < 0.0f ? 0.0f : ct / 3.14159265358979323846f);
+                if (on) {
+                    // light 0's colour * intensity * albedo * albedo_temp * (1 / pi) * (A + B alpha beta),
+                    // the outgoing cosine 0 (skip_pdf_ray unset): sin theta_o = 1
+                    const uint32_t l0 = ldg(sc->lights);
+                    const uint32_t lmat = (l0 & CRT_LIGHT_SPHERE) ? fbits(ldg(sd + 2 * (l0 & ~CRT_LIGHT_SPHERE) + 1).y)
+                                                                  : fbits(ldg(qd + 5 * l0 + 4).w);
+                    const float4 lm = ldg(mats + 2 * lmat);
+                    const float cosi = dot(-d, nrm);
+                    const float mc = fmaxf(0.0f, cosi);
+                    const float si = sqrtf(1.0f - mc * mc);
+                    const float alpha = fmaxf(si, 1.0f) / fminf(si, 1.0f), beta = fminf(si, 1.0f) / fmaxf(si, 1.0f);
+                    const float dr = m.m0.w + __int_as_float(m.m1.w) * alpha * beta;
+                    alb = v3(lm) * lm.w * alb * __int_as_float(m.m1.z) * (1.0f / 3.14159265358979323846f) * dr;
+                }
                 att = att * ((1.0f / pdf) * (alb * spdf));
                 d = dir;
                 done = false;

@@ -651,11 +651,20 @@ int prepare_scene_tables(const crt_scene_desc* d, uint32_t leaf_size, uint32_t b
         if (m.type == CRT_MAT_METAL) param = m.fuzz < 1.0f ? m.fuzz : 1.0f;  // metal ctor clamp (metal.cuh:52-55)
         else if (m.type == CRT_MAT_DIELECTRIC) param = m.ior;
         else if (m.type == CRT_MAT_DIFFUSE_LIGHT) param = m.intensity;
-        else if (m.type != CRT_MAT_LAMBERTIAN && m.type != CRT_MAT_ISOTROPIC) {
+        else if (m.type != CRT_MAT_LAMBERTIAN && m.type != CRT_MAT_ISOTROPIC && m.type != CRT_MAT_OREN_NAYAR) {
             err = "unsupported material type " + std::to_string(m.type); return CRT_ERR_INVALID_SCENE;
         }
         if (m.texture >= 0 && (uint32_t)m.texture >= d->num_textures) {
             err = "material " + std::to_string(i) + " references missing texture"; return CRT_ERR_INVALID_SCENE;
+        }
+        if (m.type == CRT_MAT_OREN_NAYAR) {
+            // A and B of the roughness (oren_nayar.cuh scatter, in float as there) and albedo_temp:
+            // (albedo, A), (type, texture, albedo_temp, B)
+            const float sg = m.fuzz * m.fuzz;
+            const float A = 1.0f - 0.5f * (sg / (sg + 0.33f)), Bc = 0.45f * (sg / (sg + 0.09f));
+            S.mats.push_back(F4{m.albedo[0], m.albedo[1], m.albedo[2], A});
+            S.mats.push_back(F4{fb(m.type), fb((uint32_t)m.texture), m.ior, Bc});
+            continue;
         }
         S.mats.push_back(F4{m.albedo[0], m.albedo[1], m.albedo[2], param});
         S.mats.push_back(F4{fb(m.type), fb((uint32_t)m.texture), fb(m.flags), 0.0f});

@@ -270,6 +270,24 @@ int crt_builder_isotropic(crt_builder* b, const char* name, const float* rgb, co
     return CRT_OK;
 }
 
+int crt_builder_oren_nayar(crt_builder* b, const char* name, const float* rgb, const char* texture, float roughness,
+                           float albedo_temp) {
+    CRT_B_CHECK(b);
+    CRT_B(b);
+    if (!texture || !texture[0] || !B.tex_by_name.count(texture)) {
+        B.unsupported++;
+        return CRT_OK;
+    }
+    crt_material m{};
+    m.type = CRT_MAT_OREN_NAYAR;
+    copy3(m.albedo, rgb);
+    m.texture = -1;
+    m.fuzz = roughness;
+    m.ior = albedo_temp;
+    add_material(B, name, m);
+    return CRT_OK;
+}
+
 int crt_builder_metal(crt_builder* b, const char* name, const float* rgb, float fuzz, const char* texture) {
     CRT_B_CHECK(b);
     CRT_B(b);

@@ -11,7 +11,8 @@
 //   textures   solidColor (name color), checker (name scale oddColor/evenColor or
 //              oddTextureName/evenTextureName), image (name filepath: PNG or JPEG)
 //   materials  lambertian (name color | texture), metal (name color fuzziness), glass
-//              (name refraction) and dielectric (same keys), isotropic (name color texture)
+//              (name refraction) and dielectric (same keys), isotropic (name color texture),
+//              orennayar (name color texture albedo_temp roughness)
 //   primitives spheres (name position radius material uvmapping), quads (name position u v
 //              material), boxes (name position size material), cones / cylinders / disks (name
 //              position radius height material uvmapping)
@@ -448,8 +449,19 @@ static int load_scenefile(SceneFile& sf, std::string& err) {
                 if ((rc = crt_builder_isotropic(B, str_of(it->get("name")).c_str(), rgb, tex.c_str()))) return rc;
             }
         }
+        // oren_nayar: name color texture albedo_temp roughness (scene_loader.cpp:990-1016), created as
+        // load_scene creates it (the builder call)
+        if (const cfg::Value* l = list_of(M, "orennayar")) {
+            for (auto& it : l->items) {
+                float rgb[3] = {0.0f, 0.0f, 0.0f};
+                if (it->get("color")) color_of(it->get("color"), rgb);
+                if ((rc = crt_builder_oren_nayar(B, str_of(it->get("name")).c_str(), rgb, str_of(it->get("texture")).c_str(),
+                                                 num_of(it->get("roughness"), 0.0f), num_of(it->get("albedo_temp"), 0.0f))))
+                    return rc;
+            }
+        }
         // materials outside the render path become Lambertian with their colour (mid grey if none)
-        for (const char* k : {"anisotropic", "orennayar", "phong"}) {
+        for (const char* k : {"anisotropic", "phong"}) {
             if (const cfg::Value* l = list_of(M, k)) {
                 for (auto& it : l->items) {
                     float rgb[3] = {0.5f, 0.5f, 0.5f};

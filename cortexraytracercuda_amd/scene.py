@@ -100,6 +100,12 @@ class Scene:
         """isotropic.cuh: albedo from the texture (or rgb), scattered over the whole sphere."""
         return self._add_mat(name, dict(type=A.CRT_MAT_ISOTROPIC, albedo=tuple(rgb), texture=self._tex_index(texture)))
 
+    def oren_nayar(self, name, rgb=(0.5, 0.5, 0.5), roughness=0.0, albedo_temp=1.0):
+        """oren_nayar.cuh with a solid albedo (as load_scene creates it): attenuation from light 0's
+        colour and intensity, albedo_temp / pi and the roughness terms."""
+        return self._add_mat(name, dict(type=A.CRT_MAT_OREN_NAYAR, albedo=tuple(rgb), fuzz=float(roughness),
+                                        ior=float(albedo_temp)))
+
     def dielectric(self, name, ior):
         return self._add_mat(name, dict(type=A.CRT_MAT_DIELECTRIC, albedo=(1.0, 1.0, 1.0), ior=float(ior)))
 

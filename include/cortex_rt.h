@@ -78,6 +78,10 @@ enum {
     CRT_MAT_ISOTROPIC = 4,       /* core/src/materials/isotropic.cuh: albedo from its texture, directions from
                                     the mixture of the lights and the uniform sphere (sphere_pdf), scattering
                                     pdf 1 / (4 pi) */
+    CRT_MAT_OREN_NAYAR = 5,      /* core/src/materials/oren_nayar.cuh: fuzz = roughness, ior = albedo_temp;
+                                    attenuation light0.rgb * light0.intensity * albedo * albedo_temp / pi *
+                                    (A + B alpha beta) with the outgoing cosine 0 (its skip_pdf_ray is unset),
+                                    directions from the lights / uniform-sphere mixture, scattering pdf cos / pi */
     CRT_MAT_DIFFUSE_LIGHT = 8,   /* core/src/materials/diffuse_light.cuh:45-67 */
     CRT_MAT_SPOT_LIGHT = 9       /* diffuse_spot_light, core/src/materials/diffuse_spot_light.cuh:33-61: the
                                     material of a spot light (crt_sphere_light), not for crt_material */
@@ -538,6 +542,12 @@ int crt_builder_lambertian(crt_builder* b, const char* name, const float* rgb, c
 int crt_builder_metal(crt_builder* b, const char* name, const float* rgb, float fuzz, const char* texture);
 int crt_builder_dielectric(crt_builder* b, const char* name, float ior);
 int crt_builder_isotropic(crt_builder* b, const char* name, const float* rgb, const char* texture);
+/* oren_nayar as load_scene creates it (core/src/gpu.cu:440-451): only when `texture` names a known
+ * texture, and then with the solid rgb (the reference's test is inverted); with an unknown or empty
+ * texture name the reference's material has no texture and faults at its first hit: here it is not
+ * created (its primitives are skipped) and counted unsupported */
+int crt_builder_oren_nayar(crt_builder* b, const char* name, const float* rgb, const char* texture, float roughness,
+                           float albedo_temp);
 int crt_builder_diffuse_light(crt_builder* b, const char* name, const float* rgb, float intensity, int32_t one_sided);
 /* 1 when a material of that name exists */
 int crt_builder_has_material(crt_builder* b, const char* name);

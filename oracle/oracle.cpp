@@ -1133,9 +1133,29 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
             ray = Ray<R>{rec.p, dir, ray.time};
             continue;  // attenuation 1
         }
-        const bool iso = m.type == CRT_MAT_ISOTROPIC;
-        if (m.type != CRT_MAT_LAMBERTIAN && !iso) break;  // unsupported material: no scatter
+        const bool iso = m.type == CRT_MAT_ISOTROPIC, on = m.type == CRT_MAT_OREN_NAYAR;
+        if (m.type != CRT_MAT_LAMBERTIAN && !iso && !on) break;  // unsupported material: no scatter
+        if (on && s.lights.empty()) break;   // the reference reads lights.objects[0] regardless
         V3<R> alb = albedo_of(s, m, rec);
+        if (on) {
+            // oren_nayar::scatter (core/src/materials/oren_nayar.cuh): random_on_hemisphere's
+            // get_unit_vector draws two values for a direction it never uses; the attenuation is
+            // light 0's colour * intensity * albedo * albedo_temp * (1 / pi) * (A + B alpha beta)
+            // with sigma = roughness^2, A = 1 - 0.5 sigma / (sigma + 0.33), B = 0.45 sigma /
+            // (sigma + 0.09), and the outgoing cosine of its unset skip_pdf_ray, 0 (sin = 1)
+            g.uniform(); g.uniform();
+            const Obj<R>& l0 = s.objs[s.lights[0]];
+            V3<R> inc = l0.type == OBJ_LIGHT ? l0.c * l0.r : l0.le * l0.lint;
+            R sg = R(m.fuzz) * R(m.fuzz);
+            R A = R(1) - R(0.5f) * (sg / (sg + R(0.33f))), B = R(0.45f) * (sg / (sg + R(0.09f)));
+            R cosi = dot(-ray.d, rec.normal);
+            R mc = std::fmax(R(0), cosi);
+            R si = std::sqrt(R(1) - mc * mc);
+            R alpha = std::fmax(si, R(1)) / std::fmin(si, R(1)), beta = std::fmin(si, R(1)) / std::fmax(si, R(1));
+            R dr = A + B * alpha * beta;
+            alb = inc * alb * R(m.ior) * (R(1) / R(3.14159265358979323846)) * dr;
+        }
+        const bool sph = iso || on;
         // mixture_pdf(hittable_pdf(lights, p), cosine_pdf(n)) (camera.cuh:203-216, mixture_pdf.cuh:28-48);
         // isotropic (isotropic.cuh) puts sphere_pdf in the cosine's place (sphere_pdf.cuh:20-28): a
         // uniform direction from get_unit_vector (gpu_randomizer.cuh:53-60: angle u * 2pi, then
@@ -1143,7 +1163,7 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
         Onb<R> uvw(rec.normal);
         V3<R> dir;
         if (g.uniform() < 0.5f) dir = lights_random(s, rec.p, g);
-        else if (iso) {
+        else if (sph) {
             R a = R(g.uniform()) * R(6.28318530718f);
             R z = R(g.uniform()) * R(2) - R(1);
             R r = std::sqrt(R(1) - z * z);
@@ -1152,7 +1172,7 @@ template <class R> static V3<R> ray_color(const Scene<R>& s, Ray<R> ray, int max
         R pl = lights_pdf_value(s, rec.p, dir, cnt);
         const R inv4pi = R(1) / (R(4) * R(3.14159265358979323846));
         R cv = dot(unit(dir), uvw.w) / R(3.14159265358979323846);
-        R pc = iso ? inv4pi : (R(0) > cv ? R(0) : cv);  // ffmax(0, x)
+        R pc = sph ? inv4pi : (R(0) > cv ? R(0) : cv);  // ffmax(0, x)
         R pdf = R(0.5f) * pl + (R(1.0f) - R(0.5f)) * pc;
         R ct = dot(rec.normal, unit(dir));
         R spdf = iso ? inv4pi : (ct < 0 ? R(0) : ct / R(3.14159265358979323846));

@@ -1,6 +1,7 @@
 """GPU parity of the isotropic material (core/src/materials/isotropic.cuh with sphere_pdf,
-core/src/pdf/sphere_pdf.cuh) against the FP64 oracle: the box scene with a solid and a textured
-isotropic ball, on the keyed streams and on the reference's minstd stream."""
+core/src/pdf/sphere_pdf.cuh) and the Oren-Nayar material (core/src/materials/oren_nayar.cuh)
+against the FP64 oracle: the box scene with a solid and a textured isotropic ball and an Oren-Nayar
+ball, on the keyed streams and on the reference's minstd stream."""
 import os
 
 import numpy as np
@@ -18,8 +19,10 @@ def _iso_box():
     s.checker_texture("iso_check", 0.1, (0.9, 0.6, 0.2), (0.2, 0.3, 0.8))
     s.isotropic("iso_solid", (0.7, 0.7, 0.7))
     s.isotropic("iso_tex", texture="iso_check")
+    s.oren_nayar("rough", (0.8, 0.5, 0.3), roughness=0.6, albedo_temp=0.5)
     s.spheres[0]["material"] = s.material_names["iso_solid"]
     s.spheres[1]["material"] = s.material_names["iso_tex"]
+    s.sphere((0.45, -0.75, 0.2), 0.25, "rough")
     return s
 
 
@@ -49,5 +52,15 @@ def test_isotropic_render_against_fp64(renderer, rng):
     # to the RMSE bound and the ray count below.
     if rng == "pcg":
         assert close.mean() >= 0.97 and close.mean() >= close32.mean() - 0.01
-    assert rmse <= 1.5 * rmse32 + 1e-3
-    assert abs(st["rays"] - ost32["rays"]) <= 0.005 * ost32["rays"]
+        assert rmse <= 1.5 * rmse32 + 1e-3
+        assert abs(st["rays"] - ost32["rays"]) <= 0.002 * ost32["rays"]
+    else:
+        # with the minstd stream the FP32 oracle is the reference-faithful renderer (its sphere
+        # quadratic with cancellation decides the self-hits differently from the GPU's robust test,
+        # measured 0.4 % more rays), so the yardstick is oracle64 on another seed: the GPU image is
+        # no farther from oracle64 than that, and its mean radiance agrees
+        ob, _ = O.render(s, W, H, spp, depth, seed=13, rng=rng, precision=64, threads=NT)
+        rmse_seed = float(np.sqrt(np.mean((ob[..., :3] / spp - o) ** 2)))
+        print("minstd: rmse %.6f, oracle64 seed-to-seed %.6f" % (rmse, rmse_seed), flush=True)
+        assert rmse <= rmse_seed
+        assert abs(float(g.mean()) - float(o.mean())) <= 0.03 * float(o.mean())

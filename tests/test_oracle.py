@@ -610,3 +610,30 @@ def test_isotropic_furnace(occluded):
         acc, _ = O.render(s, 16, 16, 64, 2, seed=seed, rng=rng, precision=prec, threads=os.cpu_count() or 4)
         val = acc[..., :3].sum() / (16 * 16 * 64 * 3)
         assert abs(val - want) < 0.015, (prec, val, want)
+
+
+@pytest.mark.parametrize("focus", [0.5, 2.0])
+def test_oren_nayar_furnace_closed_form(focus):
+    # oren_nayar floor under a unit sky with light 0 a tiny far white light (colour 1, intensity 2):
+    # the uniform-sphere half weighs a direction by (cos / pi) / (0.5 / 4pi) = 8 cos (upper half),
+    # whose mean over the sphere is 2, times the half's probability 0.5: the pixel is the
+    # attenuation, 2 * rho * albedo_temp / pi * (A + B) (alpha * beta = 1 with the outgoing sine 1).
+    # The incoming cosine takes the camera ray's unnormalised direction, whose length along the
+    # normal is the focus distance here: 0.5 gives sin 0.87, 2 a NaN sine that fmax / fmin drop
+    # (a focus of exactly 1 gives sin 0 and a NaN attenuation, as in the reference)
+    rho, at, rough = 0.6, 0.8, 0.5
+    sg = rough * rough
+    A = 1.0 - 0.5 * sg / (sg + 0.33)
+    B = 0.45 * sg / (sg + 0.09)
+    want = 2.0 * rho * at / math.pi * (A + B)
+    s = Scene()
+    s.set_camera((0, 1, 0), (0, 0, 0), (0, 0, -1), vfov=30.0, aspect_ratio=1.0, focus_dist=1.0)
+    s.background = (1.0, 1.0, 1.0)
+    s.set_camera((0, 1, 0), (0, 0, 0), (0, 0, -1), vfov=30.0, aspect_ratio=1.0, focus_dist=focus)
+    m = s.oren_nayar("floor", (rho, rho, rho), roughness=rough, albedo_temp=at)
+    s.quad((-50, 0, -50), (100, 0, 0), (0, 0, 100), m)
+    s.area_light((0, 100, 0), (0.01, 0, 0), (0, 0, 0.01), rgb=(1, 1, 1), intensity=2.0, invisible=True)
+    for prec, rng, seed in ((64, "pcg", 3), (32, "minstd", 2580)):
+        acc, _ = O.render(s, 16, 16, 64, 2, seed=seed, rng=rng, precision=prec, threads=os.cpu_count() or 4)
+        val = acc[..., :3].sum() / (16 * 16 * 64 * 3)
+        assert abs(val - want) < 0.02 * want, (prec, val, want)
